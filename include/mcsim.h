@@ -1,0 +1,186 @@
+/*
+ * mcsim.h — C ABI of the MI355X milling-physics library (libmcsim.so).
+ *
+ * Two hot paths of the browser milling simulator are served here, each behind the
+ * reference's own JavaScript interface (the N-API addon in machinig-sim_amd/csrc binds
+ * these entry points; the JS modules in machinig-sim_amd/js keep the reference's
+ * exported signatures):
+ *
+ *   Path A  time-domain regenerative-chatter stability search
+ *           (src/stabtd.js, src/cutting.js, src/frf.js, src/sld.js of the reference)
+ *   Path B  heightmap stock removal (src/stock.js, driven by src/sim.js:351-449)
+ *
+ * Conventions: lengths in mm, forces in N, modal stiffness k in N/mm (the engine converts
+ * to SI like src/cutting.js:55), angles in rad, speeds in rpm.  Every entry point takes
+ * plain host pointers, returns an mc_status and never throws; mc_last_error() gives the
+ * message of the last failure on the calling thread.  The library allocates its own
+ * device memory; nothing allocated by it crosses the ABI except opaque handles.
+ * All entry points are synchronous.  Calls that overlap on one device are serialised
+ * by an internal lock.
+ */
+#ifndef MCSIM_H
+#define MCSIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- limits (checked at the boundary; larger inputs return MC_ERR_LIMIT) ---- */
+#define MC_BINS        180   /* angular bins of the engagement map (src/cutting.js:15) */
+#define MC_MAX_MODES     4   /* modes per axis: fitModes keeps at most 4 (src/frf.js:81) */
+#define MC_MAX_FLUTES    8
+#define MC_MAX_SLICES   64   /* ceil(fluteLen/dz) <= 49 by src/cutting.js:63-64 */
+#define MC_CHATTER_IDX 0.45  /* verdict threshold of chatterAt (src/stabtd.js:10) */
+
+typedef enum {
+  MC_OK = 0,
+  MC_ERR_ARG = 1,     /* invalid argument (null pointer, negative count, NaN input) */
+  MC_ERR_LIMIT = 2,   /* input exceeds one of the MC_MAX_* limits */
+  MC_ERR_NODEV = 3,   /* no usable gfx950 device */
+  MC_ERR_HIP = 4,     /* a HIP runtime call failed */
+  MC_ERR_NOMEM = 5    /* device allocation failed */
+} mc_status;
+
+enum { MC_TOOL_FLAT = 0, MC_TOOL_BALL = 1 };
+
+/* ---- Path A data ---- */
+
+/* tool: {N, R, helixDeg, type, fluteLen, runout, runoutPhase} (src/cutting.js:37-44) */
+typedef struct {
+  int32_t N;            /* flutes */
+  int32_t type;         /* MC_TOOL_FLAT | MC_TOOL_BALL */
+  double R;             /* radius [mm] */
+  double helixDeg;      /* helix angle [deg] */
+  double fluteLen;      /* flute length [mm] */
+  double runout;        /* radial runout [mm]; chatterAt forces 0 (src/stabtd.js:16) */
+  double runoutPhase;   /* [rad] */
+} mc_tool;
+
+/* mode {k [N/mm], fn [Hz], zeta} (src/frf.js:1-3) */
+typedef struct { double k, fn, zeta; } mc_mode;
+
+/* cutting coefficients (src/materials.js:24-33); absent mc / Ci are 0 */
+typedef struct {
+  double Ktc, Krc, Kac;   /* shearing [N/mm^2] */
+  double Kte, Kre, Kae;   /* edge [N/mm] */
+  double mc;              /* size-effect exponent */
+  double Ci;              /* process-damping coefficient [N/m] */
+} mc_material;
+
+/* One stability cell: the arguments of chatterAt / timeDomainLimit except the depth
+ * (src/stabtd.js:14,24).  modeSet semantics (src/frf.js:6-11) are resolved by the
+ * caller: mx/my hold the x and y mode lists (y = x when the reference has no modesY). */
+typedef struct {
+  mc_tool tool;
+  mc_material mat;
+  int32_t nmx, nmy;                 /* 1..MC_MAX_MODES */
+  mc_mode mx[MC_MAX_MODES];
+  mc_mode my[MC_MAX_MODES];
+  double S;                         /* spindle speed [rpm] */
+  double phiSt, phiEx;              /* engagement arc [rad]: Hb = 0 inside, -1e3 outside */
+  double fz;                        /* feed per tooth [mm] */
+  int32_t revs;                     /* revolutions per trial (opts.revs) */
+  int32_t nonlinear;                /* opts.nonlinear: size effect + process damping */
+} mc_cell;
+
+/* result of one chatterAt trial (src/stabtd.js:14-22, src/cutting.js:299-318) */
+typedef struct {
+  double index;      /* chatter index sOnce/(sAll+0.05) */
+  double amp;        /* sAll [um] */
+  double once;       /* sOnce [um] */
+  int32_t chatter;   /* index > MC_CHATTER_IDX */
+  int32_t nz;        /* axial slices engaged */
+  int64_t steps;     /* dynamics steps run */
+} mc_trial;
+
+/* timeDomainLimit options (src/stabtd.js:24) */
+typedef struct {
+  double aMin, aMax;   /* depth range [mm] */
+  int32_t iters;       /* bisection iterations */
+  int32_t reserved;
+} mc_bisect_opts;
+
+/* timeDomainLimit result: {alim, bracket:[lo, hi]}; open = stable at aMax, then
+ * alim = lo = aMax and hi = +inf (src/stabtd.js:25) */
+typedef struct {
+  double alim, lo, hi;
+  int32_t open;
+  int32_t trials;     /* chatterAt trials run for this cell */
+  int64_t steps;      /* dynamics steps summed over the trials */
+  int64_t visits;     /* (tooth, slice) visits summed over the trials */
+} mc_limit;
+
+/* ---- Path A entry points ---- */
+
+/* Initialise the library on a HIP device (0 for the first).  Called implicitly with
+ * device 0 by the first entry point that needs the GPU. */
+int mc_init(int device);
+const char* mc_last_error(void);
+/* Writes "<device name> <gcnArch> CUs=<n>" into buf. */
+int mc_device_info(char* buf, int len);
+
+/* chatterAt (src/stabtd.js:14-22) for n (cell, depth) pairs: cells[i] at depth ap[i]. */
+int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* out);
+
+/* timeDomainLimit (src/stabtd.js:24-32) for n cells, one persistent launch.
+ * Results are identical to the serial reference: same trials, same verdicts. */
+int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_limit* out);
+
+/* Bracketed search: like mc_bisect_batch, but each cell starts from the bracket
+ * [lo_hint[i], hi_hint[i]] (e.g. a surrogate's prediction).  Both ends are checked
+ * first; if lo_hint chatters or hi_hint is stable the cell falls back to the full
+ * timeDomainLimit search.  Bisection stops when hi - lo <= tol_mm (or after iters
+ * halvings inside the hint).  fell_back[i] = 1 when the hint failed (may be NULL). */
+int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts,
+                            const double* lo_hint, const double* hi_hint, double tol_mm,
+                            mc_limit* out, int32_t* fell_back);
+
+/* Per-step trace of one chatterAt trial: Fx, Fy [N] and x, y [m] after each of the
+ * first nsteps steps (CuttingEngine.run, src/cutting.js:168-293).  Any pointer may be
+ * NULL.  nsteps may exceed the trial length; the trace stops at the trial's end and
+ * *steps_out receives the number of steps written. */
+int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps,
+                    double* Fx, double* Fy, double* x, double* y, int64_t* steps_out);
+
+/* ---- Path B: heightmap stock (src/stock.js) ---- */
+
+typedef struct mc_stock mc_stock;   /* device-resident heightfield + side maps */
+
+/* new Stock(w, l, h, cell) (src/stock.js:9-40): nx = round(w/cell)+1, hx = w/(nx-1),
+ * H = zTop = 0, zBottom = -h, wallD = +inf. */
+int mc_stock_create(double w, double l, double h, double cell, mc_stock** out);
+int mc_stock_destroy(mc_stock* s);
+int mc_stock_dims(const mc_stock* s, int32_t* nx, int32_t* ny, double* hx, double* hy);
+/* wallBand (src/stock.js:123); 0 by default */
+int mc_stock_set_wall_band(mc_stock* s, double wallBand);
+/* copy the whole H (f32, row-major k = j*nx + i) between host and device */
+int mc_stock_read_h(const mc_stock* s, float* H);
+int mc_stock_write_h(mc_stock* s, const float* H);
+/* side maps: err [um] (f32), wallD [mm] (f32), lineCut, lineWall (i32) */
+int mc_stock_read_side(const mc_stock* s, float* err, float* wallD, int32_t* lineCut,
+                       int32_t* lineWall);
+
+/* one Stock.stamp call (src/stock.js:118): flat/ball, no LUT, no feed marks */
+typedef struct {
+  double cx, cy, tipZ, R;
+  int32_t type;        /* MC_TOOL_FLAT | MC_TOOL_BALL */
+  int32_t line;        /* curLine recorded in lineCut / lineWall */
+  double errVal;       /* value written to err where material is removed */
+} mc_stamp;
+
+/* Apply n stamps in order with the reference's sequential semantics: final H, err,
+ * wallD, lineCut and lineWall equal those of n successive Stock.stamp calls.
+ * vol (may be NULL) receives each stamp's returned volume [mm^3] (reduction order
+ * differs from the reference: agrees to ~1e-12 relative).  Stamps with
+ * tipZ >= zTop are skipped (vol 0) like src/stock.js:119. */
+int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol);
+
+/* Stock.sample (src/stock.js:85-92): bilinear height at n points, -inf outside. */
+int mc_stock_sample(const mc_stock* s, const double* xy, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCSIM_H */

@@ -1,0 +1,96 @@
+"""ctypes mirror of include/mcsim.h (the C ABI of libmcsim.so).
+
+Only plain data layouts live here; loading the library is done by ``mcsim.lib``.
+"""
+import ctypes as C
+
+MC_BINS = 180
+MC_MAX_MODES = 4
+MC_MAX_FLUTES = 8
+MC_MAX_SLICES = 64
+MC_CHATTER_IDX = 0.45
+
+MC_OK, MC_ERR_ARG, MC_ERR_LIMIT, MC_ERR_NODEV, MC_ERR_HIP, MC_ERR_NOMEM = range(6)
+MC_TOOL_FLAT, MC_TOOL_BALL = 0, 1
+
+
+class Tool(C.Structure):
+    _fields_ = [("N", C.c_int32), ("type", C.c_int32), ("R", C.c_double),
+                ("helixDeg", C.c_double), ("fluteLen", C.c_double),
+                ("runout", C.c_double), ("runoutPhase", C.c_double)]
+
+
+class Mode(C.Structure):
+    _fields_ = [("k", C.c_double), ("fn", C.c_double), ("zeta", C.c_double)]
+
+
+class Material(C.Structure):
+    _fields_ = [("Ktc", C.c_double), ("Krc", C.c_double), ("Kac", C.c_double),
+                ("Kte", C.c_double), ("Kre", C.c_double), ("Kae", C.c_double),
+                ("mc", C.c_double), ("Ci", C.c_double)]
+
+
+class Cell(C.Structure):
+    _fields_ = [("tool", Tool), ("mat", Material), ("nmx", C.c_int32), ("nmy", C.c_int32),
+                ("mx", Mode * MC_MAX_MODES), ("my", Mode * MC_MAX_MODES),
+                ("S", C.c_double), ("phiSt", C.c_double), ("phiEx", C.c_double),
+                ("fz", C.c_double), ("revs", C.c_int32), ("nonlinear", C.c_int32)]
+
+
+class Trial(C.Structure):
+    _fields_ = [("index", C.c_double), ("amp", C.c_double), ("once", C.c_double),
+                ("chatter", C.c_int32), ("nz", C.c_int32), ("steps", C.c_int64)]
+
+
+class BisectOpts(C.Structure):
+    _fields_ = [("aMin", C.c_double), ("aMax", C.c_double), ("iters", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class Limit(C.Structure):
+    _fields_ = [("alim", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
+                ("open", C.c_int32), ("trials", C.c_int32), ("steps", C.c_int64),
+                ("visits", C.c_int64)]
+
+
+class Stamp(C.Structure):
+    _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("tipZ", C.c_double),
+                ("R", C.c_double), ("type", C.c_int32), ("line", C.c_int32),
+                ("errVal", C.c_double)]
+
+
+def make_cell(tool, modes_x, modes_y, mat, S, arc, fz, revs, nonlinear=True):
+    """Build a Cell from plain values.
+
+    tool: dict(N, R, helixDeg, type ('flat'|'ball' or int), fluteLen, runout=0, runoutPhase=0)
+    modes_x / modes_y: lists of (k [N/mm], fn [Hz], zeta); modes_y None -> modes_x
+    mat: dict with Ktc Krc Kac Kte Kre Kae mc Ci (missing -> 0)
+    """
+    c = Cell()
+    t = tool.get("type", 0)
+    c.tool = Tool(int(tool["N"]), MC_TOOL_BALL if t in ("ball", 1) else MC_TOOL_FLAT,
+                  float(tool["R"]), float(tool["helixDeg"]), float(tool["fluteLen"]),
+                  float(tool.get("runout", 0.0)), float(tool.get("runoutPhase", 0.0)))
+    c.mat = Material(*[float(mat.get(k, 0.0)) for k in
+                       ("Ktc", "Krc", "Kac", "Kte", "Kre", "Kae", "mc", "Ci")])
+    modes_y = modes_x if not modes_y else modes_y
+    if not (1 <= len(modes_x) <= MC_MAX_MODES and 1 <= len(modes_y) <= MC_MAX_MODES):
+        raise ValueError("1..%d modes per axis" % MC_MAX_MODES)
+    c.nmx, c.nmy = len(modes_x), len(modes_y)
+    for i, m in enumerate(modes_x):
+        c.mx[i] = Mode(*map(float, m))
+    for i, m in enumerate(modes_y):
+        c.my[i] = Mode(*map(float, m))
+    c.S = float(S)
+    c.phiSt, c.phiEx = float(arc[0]), float(arc[1])
+    c.fz = float(fz)
+    c.revs = int(revs)
+    c.nonlinear = 1 if nonlinear else 0
+    return c
+
+
+def cell_array(cells):
+    arr = (Cell * len(cells))()
+    for i, c in enumerate(cells):
+        arr[i] = c
+    return arr

@@ -1,0 +1,148 @@
+"""Workload definitions for the two hot paths, with their exact sizes.
+
+* ``sld_40rpm()``     40-speed stability lobe diagram, one tool, one mode per axis
+  (3-flute 12 mm end mill, 30 deg helix, fn 1000 Hz, zeta 0.03, k 2e4 N/mm,
+  half-immersion down-milling, 40 revolutions per trial).  FP64, CPU-runnable.
+* ``sld_grid()``      256 speeds x 64 seeded tool/mode variants (16,384 cells),
+  2-4 flutes, 1-4 modes per axis, immersion 10-100 %, process damping on,
+  20-80 revolutions per trial.
+* ``zigzag_path()``   512 x 512 heightmap at 0.1 mm, 5,000 seeded zigzag G1 segments
+  alternating a flat 10 mm and a ball 6 mm tool, and the stamp schedule of
+  src/sim.js:355-377 along it.
+
+Random draws use the Park-Miller generator of the reference's tests
+(seed <- seed * 16807 mod 2147483647, test/identify.test.mjs:20).
+Units follow the reference: modal k in N/mm (src/cutting.js:55), lengths in mm.
+"""
+import math
+
+from ._abi import make_cell, MC_TOOL_BALL, MC_TOOL_FLAT
+
+JS_PI = 3.141592653589793
+
+
+class ParkMiller:
+    def __init__(self, seed=1):
+        self.seed = int(seed)
+
+    def u(self):
+        """uniform in (0, 1)"""
+        self.seed = (self.seed * 16807) % 2147483647
+        return self.seed / 2147483647
+
+    def uniform(self, a, b):
+        return a + (b - a) * self.u()
+
+    def log_uniform(self, a, b):
+        return math.exp(math.log(a) + (math.log(b) - math.log(a)) * self.u())
+
+    def integer(self, a, b):
+        """uniform integer in [a, b]"""
+        return a + min(b - a, int(self.u() * (b - a + 1)))
+
+
+def arc_of(ae, D):
+    """engagement arc of down milling at radial depth ae (src/conditions.js:11-15)"""
+    if not (ae < D):
+        return (0.0, JS_PI)
+    return (JS_PI - math.acos(1 - 2 * ae / D), JS_PI)
+
+
+# cutting coefficients: the 40-speed diagram's own set, and the A5052 row of
+# src/materials.js:25 (with its size effect mc 0.10 and process damping Ci 2e4)
+MAT_SLD40 = dict(Ktc=700.0, Krc=210.0, Kac=0.0, Kte=20.0, Kre=15.0, Kae=0.0, mc=0.0, Ci=0.0)
+MAT_A5052 = dict(Ktc=700.0, Krc=210.0, Kac=120.0, Kte=20.0, Kre=25.0, Kae=5.0, mc=0.10, Ci=2e4)
+
+BISECT_DEFAULT = dict(aMin=0.05, aMax=10.0, iters=10)
+
+
+def sld_40rpm():
+    """40 cells: S_i = 3000 + 12000 i / 39, i = 0..39"""
+    tool = dict(N=3, R=6.0, helixDeg=30.0, type="flat", fluteLen=24.0)
+    modes = [(2e4, 1000.0, 0.03)]
+    arc = (JS_PI / 2, JS_PI)
+    cells = []
+    for i in range(40):
+        S = 3000 + 12000 * i / 39
+        cells.append(make_cell(tool, modes, None, MAT_SLD40, S, arc, 0.05, 40, True))
+    return cells, dict(BISECT_DEFAULT)
+
+
+def grid_tools(n_tools=64, seed=1):
+    """64 seeded tool / mode-set / immersion variants"""
+    rng = ParkMiller(seed)
+    tools = []
+    for _ in range(n_tools):
+        N = rng.integer(2, 4)
+        helix = rng.uniform(15, 45)
+        D = rng.uniform(6, 20)
+        runout = rng.uniform(0, 0.010)          # mm; chatterAt forces 0 (src/stabtd.js:16)
+        nmx = rng.integer(1, 4)
+        nmy = rng.integer(1, 4)
+        mx = [(rng.log_uniform(5e3, 5e4), rng.log_uniform(500, 4000), rng.uniform(0.01, 0.05))
+              for _ in range(nmx)]
+        my = [(rng.log_uniform(5e3, 5e4), rng.log_uniform(500, 4000), rng.uniform(0.01, 0.05))
+              for _ in range(nmy)]
+        imm = rng.uniform(0.1, 1.0)
+        revs = rng.integer(20, 80)
+        tools.append(dict(tool=dict(N=N, R=D / 2, helixDeg=helix, type="flat", fluteLen=24.0,
+                                    runout=runout),
+                          mx=mx, my=my, imm=imm, D=D, revs=revs))
+    return tools
+
+
+def grid_speeds(n=256):
+    return [2000 + 28000 * i / (n - 1) for i in range(n)]
+
+
+def sld_grid(n_speeds=256, n_tools=64, seed=1):
+    """n_tools x n_speeds cells (tool-major).  Defaults: 16,384 cells."""
+    tools = grid_tools(n_tools, seed)
+    speeds = grid_speeds(n_speeds)
+    cells = []
+    for t in tools:
+        arc = arc_of(t["imm"] * t["D"], t["D"])
+        for S in speeds:
+            cells.append(make_cell(t["tool"], t["mx"], t["my"], MAT_A5052, S, arc, 0.05,
+                                   t["revs"], True))
+    return cells, dict(BISECT_DEFAULT)
+
+
+# ---------------------------------------------------------------- heightmap path
+
+
+def zigzag_segments(n_segments=5000, width=51.1, seed=1):
+    """Seeded zigzag G1 program.  Layers alternate flat R5 / ball R3; per layer a depth
+    d ~ U(0.5, 3) (tip at z = -d) and a stepover ~ U(2, 5); raster lines along x across
+    the stock, stepover moves along y, a diagonal link to the next layer's start.
+    Returns a list of (x0, y0, z0, x1, y1, z1, R, type, line)."""
+    rng = ParkMiller(seed)
+    segs = []
+    layer = 0
+    pos = (0.0, 0.0, 0.0)
+    while len(segs) < n_segments:
+        flat = layer % 2 == 0
+        R, typ = (5.0, MC_TOOL_FLAT) if flat else (3.0, MC_TOOL_BALL)
+        d = rng.uniform(0.5, 3.0)
+        step = rng.uniform(2.0, 5.0)
+        z = -d
+        y = 0.0
+        x_left = 0.0
+        start = (x_left, y, z)
+        if len(segs) < n_segments:
+            segs.append(pos + start + (R, typ, len(segs) + 1))
+        pos = start
+        direction = 1
+        while len(segs) < n_segments:
+            x_end = width if direction > 0 else 0.0
+            nxt = (x_end, pos[1], z)
+            segs.append(pos + nxt + (R, typ, len(segs) + 1))
+            pos = nxt
+            if pos[1] + step > width or len(segs) >= n_segments:
+                break
+            nxt = (pos[0], pos[1] + step, z)
+            segs.append(pos + nxt + (R, typ, len(segs) + 1))
+            pos = nxt
+            direction = -direction
+        layer += 1
+    return segs[:n_segments]

@@ -1,0 +1,87 @@
+/*
+ * oracle.h — TEST INFRASTRUCTURE ONLY.  CPU restatement of the reference's hot paths,
+ * used by tests/, the smoke check and bench.py's CPU-baseline leg as the checker.
+ * The product (libmcsim.so) never includes, links or calls anything declared here.
+ */
+#ifndef MC_ORACLE_H
+#define MC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mcsim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fdlibm.c: V8 Math.* restated */
+double or_sin(double x);
+double or_cos(double x);
+double or_tan(double x);
+double or_pow(double x, double y);
+double or_hypot(double a, double b);
+double or_js_round(double x);
+double or_atan(double x);
+int or_rem_pio2(double x, double *y);
+double or_kernel_sin(double x, double y, int iy);
+double or_kernel_cos(double x, double y);
+double or_kernel_tan(double x, double y, int iy);
+/* fn: 0 sin, 1 cos, 2 tan, 3 pow(x, y), 4 hypot(x, y), 5 atan */
+void or_math_batch(int fn, const double *x, const double *y, double *out, int64_t n);
+
+/* engine.c: CuttingEngine / chatterAt / timeDomainLimit restated */
+typedef struct {
+  /* accumulated over a trial, for the FP64 roofline count and the per-step stats */
+  int64_t steps;     /* dynamics steps */
+  int64_t visits;    /* (tooth, slice) visits: N*nz per step */
+  int64_t engaged;   /* visits that reach the force law (h > 0) */
+  double maxF;       /* max |(Fx, Fy)| (st.maxF, src/cutting.js:281) */
+  double sumF;       /* sum |(Fx, Fy)| */
+} or_counts;
+
+int or_chatter_at(const mc_cell *c, double ap, mc_trial *out, or_counts *cnt);
+int or_time_domain_limit(const mc_cell *c, const mc_bisect_opts *o, mc_limit *out);
+int or_bracket_limit(const mc_cell *c, const mc_bisect_opts *o, double lo_hint,
+                     double hi_hint, double tol, mc_limit *out, int32_t *fell_back);
+int64_t or_engine_trace(const mc_cell *c, double ap, int64_t nsteps, double *Fx, double *Fy,
+                        double *x, double *y);
+/* n cells on nthreads host threads (one cell per thread at a time) */
+int or_bisect_batch_mt(const mc_cell *cells, int n, const mc_bisect_opts *o, int nthreads,
+                       mc_limit *out);
+int or_chatter_batch_mt(const mc_cell *cells, const double *ap, int n, int nthreads,
+                        mc_trial *out);
+
+/* sld.c: zero-order lobes (src/sld.js) */
+void or_directional_coeffs(double Kr, double phiSt, double phiEx, double out[4]);
+/* writes up to max_pts (rpm, alim) pairs, lobe j's points contiguous; lobe_len[j] =
+ * points of lobe j (0..lobes-1); returns total points */
+int or_stability_lobes(int N, double Ktc, double Krc, double phiSt, double phiEx, int lobes,
+                       double rpmMin, double rpmMax, const mc_mode *mx, int nmx,
+                       const mc_mode *my, int nmy, double *rpm, double *alim, int max_pts,
+                       int *lobe_len);
+double or_alim_at(const double *rpm, const double *alim, const int *lobe_len, int nlobes,
+                  double S);
+
+/* stock.c: Stock (src/stock.js) */
+typedef struct or_stock or_stock;
+or_stock *or_stock_new(double w, double l, double h, double cell);
+void or_stock_free(or_stock *s);
+void or_stock_dims(const or_stock *s, int32_t *nx, int32_t *ny, double *hx, double *hy);
+void or_stock_set_wall_band(or_stock *s, double band);
+float *or_stock_h(or_stock *s);
+float *or_stock_err(or_stock *s);
+float *or_stock_walld(or_stock *s);
+int32_t *or_stock_linecut(or_stock *s);
+int32_t *or_stock_linewall(or_stock *s);
+double or_stock_stamp(or_stock *s, double cx, double cy, double tipZ, double R, int type,
+                      double errVal, int line);
+void or_stamp_batch(or_stock *s, const mc_stamp *st, int n, double *vol);
+double or_stock_sample(const or_stock *s, double x, double y);
+
+/* toolModel (src/tool.js:16-32) for a uniform carbide beam: k [N/mm], fn [Hz] */
+void or_tool_model(double D, double stickout, double kSpindle, double mHolder, double *k,
+                   double *fn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+// Runs the REFERENCE implementation (read from its source tree at run time, down-levelled
+// in memory by downlevel_loader.mjs) on the cases in a JSON file and writes its answers.
+// Used only in the build container by make_golden.py; the fixtures it writes are data.
+//
+// usage: node --experimental-loader ./downlevel_loader.mjs run_reference.mjs cases.json out.json
+import fs from 'fs';
+
+const REF = process.env.MC_REFERENCE_SRC || '/root/reference/src/';
+const [casesPath, outPath] = process.argv.slice(2);
+const cases = JSON.parse(fs.readFileSync(casesPath, 'utf8'));
+
+let CE, NB;
+async function main() {
+const { CuttingEngine, BINS } = await import(REF + 'cutting.js');
+const { chatterAt, timeDomainLimit } = await import(REF + 'stabtd.js');
+const { stabilityLobes, alimAt, directionalCoeffs } = await import(REF + 'sld.js');
+const { Stock } = await import(REF + 'stock.js');
+const { toolModel } = await import(REF + 'tool.js');
+CE = CuttingEngine; NB = BINS;
+
+const b64f32 = a => Buffer.from(new Float32Array(a).buffer).toString('base64');
+const b64i32 = a => Buffer.from(new Int32Array(a).buffer).toString('base64');
+const b64f64 = a => Buffer.from(new Float64Array(a).buffer).toString('base64');
+
+// the engine chatterAt builds (src/stabtd.js:15-20), kept open to read its chatter index
+function chatterEngine(c, ap) {
+  const e = new CE();
+  const BINS = NB;
+  e.configure({ N: c.tool.N, R: c.tool.R, helixDeg: c.tool.helixDeg, type: c.tool.type,
+    fluteLen: Math.max(c.tool.fluteLen, ap + 1), runout: 0, nonlinear: c.nonlinear }, c.modal, c.mat, c.S);
+  for (let b = 0; b < BINS; b++) { const phi = (b + 0.5) / BINS * 2 * Math.PI; e.Hb[b] = phi >= c.arc[0] && phi <= c.arc[1] ? 0 : -1e3; }
+  e.setEngagement([1, 0], c.fz, -ap, ap);
+  e.vx = 0.01;
+  return e;
+}
+
+const out = {};
+const t0 = Date.now();
+
+if (cases.chatter) {
+  out.chatter = cases.chatter.map(({ cell: c, ap }) => {
+    const e = chatterEngine(c, ap);
+    e.run(e.stepsPerTooth * c.tool.N * c.revs);
+    const ci = e.chatterIndex();
+    const verdict = chatterAt(c.tool, c.modal, c.mat, c.S, c.arc, c.fz, ap, { revs: c.revs, nonlinear: c.nonlinear });
+    return { index: ci.index, amp: ci.amp, once: ci.once === undefined ? 0 : ci.once, chatter: verdict ? 1 : 0,
+      nz: e.nz, stepsPerTooth: e.stepsPerTooth, decim: e.decim };
+  });
+  console.error('chatter', out.chatter.length, (Date.now() - t0) / 1e3, 's');
+}
+
+if (cases.tdl) {
+  out.tdl = cases.tdl.map(({ cell: c, opts }) => {
+    const r = timeDomainLimit(c.tool, c.modal, c.mat, c.S, c.arc, c.fz, { ...opts, revs: c.revs, nonlinear: c.nonlinear });
+    return { alim: r.alim, lo: r.bracket[0], hi: r.bracket[1] === Infinity ? 'inf' : r.bracket[1] };
+  });
+  console.error('tdl', out.tdl.length, (Date.now() - t0) / 1e3, 's');
+}
+
+if (cases.trace) {
+  out.trace = cases.trace.map(({ cell: c, ap, steps }) => {
+    const e = chatterEngine(c, ap);
+    const Fx = [], Fy = [], x = [], y = [];
+    e.takeStats();
+    for (let s = 0; s < steps; s++) {
+      e.run(1);
+      const st = e.takeStats();
+      Fx.push(st.sumFx); Fy.push(st.sumFy); x.push(e.x); y.push(e.y);
+    }
+    return { Fx: b64f64(Fx), Fy: b64f64(Fy), x: b64f64(x), y: b64f64(y) };
+  });
+  console.error('trace', (Date.now() - t0) / 1e3, 's');
+}
+
+if (cases.stock) {
+  out.stock = cases.stock.map(({ w, l, h, cell, wallBand, stamps, samples }) => {
+    const s = new Stock(w, l, h, cell);
+    if (wallBand) s.wallBand = wallBand;
+    const vol = [];
+    for (const st of stamps) {
+      s.curLine = st.line;
+      vol.push(s.stamp(st.cx, st.cy, st.tipZ, st.R, st.type, st.errVal, null, null));
+    }
+    const smp = samples.map(([x, y]) => { const v = s.sample(x, y); return v === -Infinity ? '-inf' : v; });
+    return { nx: s.nx, ny: s.ny, hx: s.hx, hy: s.hy, H: b64f32(s.H), err: b64f32(s.err), wallD: b64f32(s.wallD),
+      lineCut: b64i32(s.lineCut), lineWall: b64i32(s.lineWall), vol, removed: s.removed, sample: smp };
+  });
+  console.error('stock', (Date.now() - t0) / 1e3, 's');
+}
+
+if (cases.lobes) {
+  out.lobes = cases.lobes.map(({ params, speeds }) => {
+    const lob = stabilityLobes(params);
+    const alim = speeds.map(S => { const a = alimAt(lob, S); return a === Infinity ? 'inf' : a; });
+    const dc = directionalCoeffs(params.Krc / params.Ktc, params.phiSt, params.phiEx);
+    return { lobes: lob.map(L => ({ rpm: b64f64(L.map(p => p.rpm)), alim: b64f64(L.map(p => p.alim)) })), alimAt: alim,
+      dc: [dc.xx, dc.xy, dc.yx, dc.yy] };
+  });
+}
+
+if (cases.toolModel) out.toolModel = cases.toolModel.map(t => { const m = toolModel(t); return { k: m.k, fn: m.fn }; });
+
+fs.writeFileSync(outPath, JSON.stringify(out));
+console.error('done', (Date.now() - t0) / 1e3, 's');
+}
+main().catch(e => { console.error(e); process.exit(1); });
