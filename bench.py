@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the two hot paths on one MI355X.
+
+A step is one pass of the time-domain stability search (timeDomainLimit,
+{aMin 0.05, aMax 10, iters 10}) over every cell of the 256-speed x 64-tool lobe grid
+(16,384 cells, 2-4 flutes, 1-4 modes per axis, 20-80 revolutions per trial, process
+damping and size effect on), with the cell table already on the device side of the call.
+value = SLD cells resolved per second.  The heightmap path (512 x 512 at 0.1 mm, 5,000
+seeded zigzag segments = 2.41 M stamps) is reported beside it in GB/s.
+
+Prints ONE JSON line.  usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "machinig-sim_amd"))
+
+import numpy as np  # noqa: E402
+
+from mcsim import lib, workloads as W, path  # noqa: E402
+from mcsim._abi import cell_array, MC_TOOL_BALL  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, vendor figure
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, vendor figure
+
+
+def flop_alg(cells, res):
+    """FP64 flops of the dynamic (non-revolution-periodic) part of the search:
+    6 per visit (edge position, chip thickness), 18 per engaged visit (forces and their
+    accumulation), +5 with process damping, +1 with the size effect, 11 per mode and 2
+    per step (modal update)."""
+    tot = 0.0
+    for c, r in zip(cells, res):
+        per_eng = 18 + (5 if c.nonlinear and c.mat.Ci != 0 else 0) + (1 if c.nonlinear and c.mat.mc != 0 else 0)
+        tot += 6.0 * r.visits + per_eng * r.engaged + (11.0 * (c.nmx + c.nmy) + 2) * r.steps
+    return tot
+
+
+def cpu_baseline(cells, opts, target_s):
+    """The oracle (FP64 CPU restatement) on a bounded sample of the same cells, all host
+    threads, one cell per thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 64))
+    n = len(cells)
+    # calibrate on a strided probe, then size the sample for ~target_s seconds
+    probe = list(range(0, n, max(1, n // (4 * threads))))[: 4 * threads]
+    arr = cell_array([cells[i] for i in probe])
+    t0 = time.perf_counter()
+    oracle.bisect_batch(arr, len(probe), threads=threads, **opts)
+    dt = time.perf_counter() - t0
+    per_cell = dt / len(probe)
+    m = int(max(len(probe), min(n, target_s / max(per_cell, 1e-9))))
+    stride = max(1, n // m)
+    idx = list(range(0, n, stride))[:m]
+    arr = cell_array([cells[i] for i in idx])
+    t0 = time.perf_counter()
+    oracle.bisect_batch(arr, len(idx), threads=threads, **opts)
+    dt = time.perf_counter() - t0
+    return dict(value=len(idx) / dt, unit="SLD cells/s", cores=threads, kind="port",
+                sample="%d of %d cells (every %d-th), %.1f s" % (len(idx), n, stride, dt))
+
+
+def heightmap(steps):
+    segs = W.zigzag_segments(5000)
+    rec, _ = path.schedule(segs, 0.1)
+    arr = path.as_stamp_array(rec)
+    n = len(rec)
+    # algorithmic bytes: each stamp reads and writes the f32 height of every node of its
+    # bounding box once (src/stock.js:132-155)
+    hx = 0.1
+    Rw = rec["R"] + 2 * hx
+    i0 = np.maximum(0, np.floor((rec["cx"] - Rw) / hx)); i1 = np.minimum(511, np.ceil((rec["cx"] + Rw) / hx))
+    j0 = np.maximum(0, np.floor((rec["cy"] - Rw) / hx)); j1 = np.minimum(511, np.ceil((rec["cy"] + Rw) / hx))
+    nodes = np.where((i1 >= i0) & (j1 >= j0), (i1 - i0 + 1) * (j1 - j0 + 1), 0).sum()
+    alg_bytes = 8.0 * nodes
+    times, kms = [], []
+    for _ in range(steps + 1):
+        s = lib.Stock(51.1, 51.1, 20, 0.1)
+        t0 = time.perf_counter()
+        s.stamps(arr, want_vol=True)
+        times.append(time.perf_counter() - t0)
+        kms.append(lib.last_kernel_ms())
+        s.close()
+    wall = float(np.mean(times[1:]))
+    k = float(np.mean(kms[1:]))
+    return dict(workload="zigzag_512x512_5000seg", stamps=n, ms_per_batch=wall * 1e3,
+                kernel_ms=k, stamps_per_s=n / wall, GBps_algorithmic=alg_bytes / (k * 1e-3) / 1e9,
+                GBps_frac_of_hbm=alg_bytes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                note="algorithmic bytes = 8 B x bbox nodes per stamp; H stays on chip between stamps")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tools", type=int, default=64)
+    ap.add_argument("--speeds", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-heightmap", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    cells, opts = W.sld_grid(n_speeds=a.speeds, n_tools=a.tools)
+    arr = cell_array(cells)
+    print("device:", lib.device_info(), file=sys.stderr, flush=True)
+    for w in range(a.warmup):
+        lib.bisect_batch(arr, **opts)
+        print("warmup %d done" % w, file=sys.stderr, flush=True)
+    times, kms = [], []
+    res = None
+    for s in range(a.steps):
+        t0 = time.perf_counter()
+        res = lib.bisect_batch(arr, **opts)
+        times.append(time.perf_counter() - t0)
+        kms.append(lib.last_kernel_ms())
+        print("step %d: %.3f s (kernel %.3f s)" % (s, times[-1], kms[-1] / 1e3), file=sys.stderr, flush=True)
+    ms = 1e3 * float(np.mean(times))
+    kms_mean = float(np.mean(kms))
+    n = len(cells)
+    flops = flop_alg(cells, res)
+    achieved = flops / (kms_mean * 1e-3) / 1e12
+    trials = sum(r.trials for r in res)
+    visits = sum(r.visits for r in res)
+    out = {
+        "metric": "SLD cells/sec at <=0.05 mm b_lim error (plain time-domain bisection)",
+        "value": n / (ms / 1e3),
+        "unit": "SLD cells/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Park-Miller tool/mode grid)",
+        "config": {"workload": "sld_grid_%dspeeds_x_%dtools" % (a.speeds, a.tools), "cells": n,
+                   "bisection": opts, "trials": trials, "slice_visits": visits,
+                   "slice_visits_per_s": visits / (ms / 1e3)},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "chatter_search", "kernel_ms": kms_mean, "flop_alg": flops},
+    }
+    if not a.no_heightmap:
+        out["heightmap"] = heightmap(2)
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cells, opts, a.cpu_seconds)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

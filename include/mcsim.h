@@ -110,6 +110,7 @@ typedef struct {
   int32_t trials;     /* chatterAt trials run for this cell */
   int64_t steps;      /* dynamics steps summed over the trials */
   int64_t visits;     /* (tooth, slice) visits summed over the trials */
+  int64_t engaged;    /* visits that reached the force law (chip thickness > 0) */
 } mc_limit;
 
 /* ---- Path A entry points ---- */
@@ -120,6 +121,10 @@ int mc_init(int device);
 const char* mc_last_error(void);
 /* Writes "<device name> <gcnArch> CUs=<n>" into buf. */
 int mc_device_info(char* buf, int len);
+/* Device time [ms] of the main kernel of the last batch entry point called on this
+ * device (the chatter search, or the tile pass of mc_stamp_batch), measured with HIP
+ * events on the library's stream around that kernel alone. */
+int mc_last_kernel_time(double* ms);
 
 /* chatterAt (src/stabtd.js:14-22) for n (cell, depth) pairs: cells[i] at depth ap[i]. */
 int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* out);

@@ -1,0 +1,568 @@
+// api.cpp — the C ABI of libmcsim.so (include/mcsim.h): argument checks, the per-cell
+// constants of CuttingEngine.configure (src/cutting.js:39-82), device buffers, launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mcsim.h"
+#include "mc_dev.h"
+#include "mc_math.h"
+
+// chatter.hip
+int mcd_group_lds_bytes(int nv_cap, int nz_cap);
+int mcd_launch_chatter(const mcd::LaunchDev& P, int G, int blocks, hipStream_t stream);
+// stock.hip
+size_t mcs_stampdev_bytes();
+size_t mcs_chunkbox_bytes();
+int mcs_chunk();
+int mcs_launch_stamps(const mc_stamp* d_in, int n, void* d_stampdev, void* d_box, int nx, int ny,
+                      double hx, double hy, double zTop, double zB, double band, float* H,
+                      float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, double* vol,
+                      hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
+int mcs_launch_sample(const float* H, int nx, int ny, double hx, double hy, const double* d_xy,
+                      int n, double* d_out, hipStream_t stream);
+int mcs_launch_fill(float* p, size_t n, float v, hipStream_t stream);
+
+namespace {
+
+constexpr double JS_PI = 3.141592653589793;
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return fail(MC_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+double js_max(double a, double b) { return a > b ? a : b; }
+
+// ---- per-device state: stream and grow-only work buffers
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= n) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MC_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    n = bytes;
+    return 0;
+  }
+};
+
+struct Ctx {
+  int dev = -1;
+  int cus = 0;
+  std::string name, arch;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  DevBuf cells, tasks, out, queue, ring, hist, trace;
+  std::mutex mu;
+};
+
+Ctx g_ctx;
+std::mutex g_init_mu;
+
+int ensure_init(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_ctx.dev >= 0) {
+    if (device >= 0 && device != g_ctx.dev) return fail(MC_ERR_ARG, "already initialised on device %d", g_ctx.dev);
+    return MC_OK;
+  }
+  if (device < 0) device = 0;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device)
+    return fail(MC_ERR_NODEV, "no HIP device %d (found %d)", device, count);
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(MC_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&g_ctx.ev0));
+  HIPCHK(hipEventCreate(&g_ctx.ev1));
+  g_ctx.cus = prop.multiProcessorCount;
+  g_ctx.name = prop.name;
+  g_ctx.arch = prop.gcnArchName;
+  g_ctx.dev = device;
+  return MC_OK;
+}
+
+int env_int(const char* k, int dflt) {
+  const char* v = getenv(k);
+  return v && *v ? atoi(v) : dflt;
+}
+
+// ---- CuttingEngine.configure for chatterAt's engine (runout forced to 0, src/stabtd.js:16)
+int check_cell(const mc_cell& c, int i) {
+  if (c.tool.N < 1 || c.tool.N > MC_MAX_FLUTES) return fail(MC_ERR_LIMIT, "cell %d: N=%d outside 1..%d", i, c.tool.N, MC_MAX_FLUTES);
+  if (c.nmx < 1 || c.nmx > MC_MAX_MODES || c.nmy < 1 || c.nmy > MC_MAX_MODES)
+    return fail(MC_ERR_LIMIT, "cell %d: modes per axis must be 1..%d", i, MC_MAX_MODES);
+  if (!(c.tool.R > 0) || !std::isfinite(c.S) || !std::isfinite(c.fz) || c.revs < 0 ||
+      !std::isfinite(c.tool.fluteLen) || !std::isfinite(c.tool.helixDeg))
+    return fail(MC_ERR_ARG, "cell %d: invalid tool / speed / feed / revs", i);
+  for (int j = 0; j < c.nmx; j++)
+    if (!(c.mx[j].k > 0) || !(c.mx[j].fn > 0)) return fail(MC_ERR_ARG, "cell %d: mode x%d needs k > 0, fn > 0", i, j);
+  for (int j = 0; j < c.nmy; j++)
+    if (!(c.my[j].k > 0) || !(c.my[j].fn > 0)) return fail(MC_ERR_ARG, "cell %d: mode y%d needs k > 0, fn > 0", i, j);
+  return MC_OK;
+}
+
+void make_celldev(const mc_cell& c, mcd::CellDev& d) {
+  std::memset(&d, 0, sizeof d);
+  const int N = c.tool.N;
+  const double S = js_max(100, c.S);
+  d.omega = S * 2 * JS_PI / 60;
+  const double Ttooth = 60 / (S * N);
+  double fTop = -INFINITY;
+  for (int j = 0; j < c.nmx; j++) fTop = js_max(fTop, c.mx[j].fn);
+  for (int j = 0; j < c.nmy; j++) fTop = js_max(fTop, c.my[j].fn);
+  d.spt = (int)js_max(48, std::ceil(Ttooth * js_max(fTop, 0) * 24));
+  d.dt = Ttooth / d.spt;
+  d.spr = N * d.spt;
+  d.dphi = d.omega * d.dt;
+  d.pitch = 2 * JS_PI / N;
+  d.binScale = MC_BINS / (2 * JS_PI);
+  d.fz = c.fz;
+  d.hMax = js_max(0.1, 3 * c.fz);
+  d.R = c.tool.R;
+  d.tanB = mcm::tan_js(c.tool.helixDeg * JS_PI / 180);
+  d.toolFluteLen = c.tool.fluteLen;
+  d.ball = c.tool.type == MC_TOOL_BALL;
+  d.N = N;
+  d.revs = c.revs;
+  d.decim = (int)js_max(1, mcm::round_js(d.spt / 40.0));
+  d.nmx = c.nmx;
+  d.nmy = c.nmy;
+  const bool nl = c.nonlinear != 0;
+  d.Ktc = c.mat.Ktc; d.Krc = c.mat.Krc; d.Kte = c.mat.Kte; d.Kre = c.mat.Kre;
+  d.mc = nl ? c.mat.mc : 0;
+  d.Ci = nl ? c.mat.Ci : 0;
+  for (int j = 0; j < c.nmx; j++) {
+    const double w = 2 * JS_PI * c.mx[j].fn;
+    d.kx[j] = c.mx[j].k * 1e3;
+    d.mx[j] = d.kx[j] / (w * w);
+    d.cx[j] = 2 * c.mx[j].zeta * std::sqrt(d.kx[j] * d.mx[j]);
+  }
+  for (int j = 0; j < c.nmy; j++) {
+    const double w = 2 * JS_PI * c.my[j].fn;
+    d.ky[j] = c.my[j].k * 1e3;
+    d.my[j] = d.ky[j] / (w * w);
+    d.cy[j] = 2 * c.my[j].zeta * std::sqrt(d.ky[j] * d.my[j]);
+  }
+  for (int i = 0; i < N; i++) d.dR[i] = 0.0 * mcm::cos_js(2 * JS_PI * i / N - 0.0);
+  for (int b = 0; b < MC_BINS; b++) {
+    const double phi = (b + 0.5) / MC_BINS * 2 * JS_PI;
+    d.Hb[b] = (phi >= c.phiSt && phi <= c.phiEx) ? 0.0f : -1e3f;
+  }
+}
+
+// upper bound of the slice count of any trial depth <= apMax (src/cutting.js:63-64,101)
+int nz_bound(const mc_cell& c, double apMax) {
+  // dz >= max(0.25, tool.fluteLen/48) for every depth, and maxSlices <= 49
+  const double dzMin = js_max(0.25, c.tool.fluteLen / 48);
+  const double b = std::ceil(apMax / dzMin);
+  if (!(b >= 0)) return 0;
+  return (int)std::min(b, 49.0);
+}
+
+struct TaskSet {
+  std::vector<mcd::CellDev> cells;
+  std::vector<mcd::TaskDev> tasks;
+  std::vector<double> cost;
+  std::vector<int> nzcap;
+};
+
+int run_tasks(TaskSet& ts, std::vector<mcd::ResultDev>& res, double* h_trace, int64_t trace_steps) {
+  Ctx& X = g_ctx;
+  const int n = (int)ts.tasks.size();
+  res.assign(n, mcd::ResultDev{});
+  if (n == 0) return MC_OK;
+  // largest-first order balances uneven cells across the persistent groups
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts.cost[a] > ts.cost[b]; });
+  std::vector<mcd::TaskDev> sorted(n);
+  int nz_cap = 1, nv_cap = 1;
+  int64_t ring_cap = 1;
+  for (int q = 0; q < n; q++) {
+    sorted[q] = ts.tasks[order[q]];
+    const mcd::CellDev& c = ts.cells[sorted[q].cell];
+    const int nzc = std::max(1, ts.nzcap[order[q]]);
+    nz_cap = std::max(nz_cap, nzc);
+    nv_cap = std::max(nv_cap, nzc * c.N);
+    ring_cap = std::max<int64_t>(ring_cap, (int64_t)nzc * c.spr);
+  }
+  if (nz_cap > MC_MAX_SLICES) return fail(MC_ERR_LIMIT, "a trial needs %d slices (max %d)", nz_cap, MC_MAX_SLICES);
+  int G = env_int("MC_GROUP_LANES", 0);
+  if (G == 0) G = nv_cap <= 8 * 16 ? 16 : nv_cap <= 8 * 32 ? 32 : 64;
+  if (G != 8 && G != 16 && G != 32 && G != 64) return fail(MC_ERR_ARG, "MC_GROUP_LANES must be 8, 16, 32 or 64");
+  if (nv_cap > 8 * G) return fail(MC_ERR_LIMIT, "%d visits per step exceed 8 passes of %d lanes", nv_cap, G);
+  const int gpw = 64 / G;
+  const int wpc = std::max(1, env_int("MC_WAVES_PER_CU", 8));
+  int64_t groups = std::min<int64_t>(n, (int64_t)X.cus * wpc * gpw);
+  if (trace_steps > 0) groups = 1;
+  const int blocks = (int)((groups + gpw - 1) / gpw);
+  groups = (int64_t)blocks * gpw;
+  const int grp_bytes = mcd_group_lds_bytes(nv_cap, nz_cap);
+  if ((size_t)gpw * grp_bytes > 64 * 1024) return fail(MC_ERR_LIMIT, "LDS per workgroup %d B too large", gpw * grp_bytes);
+
+  std::lock_guard<std::mutex> lk(X.mu);
+  int st;
+  if ((st = X.cells.ensure(ts.cells.size() * sizeof(mcd::CellDev)))) return st;
+  if ((st = X.tasks.ensure(n * sizeof(mcd::TaskDev)))) return st;
+  if ((st = X.out.ensure(n * sizeof(mcd::ResultDev)))) return st;
+  if ((st = X.queue.ensure(256))) return st;
+  if ((st = X.ring.ensure((size_t)groups * ring_cap * sizeof(float)))) return st;
+  if ((st = X.hist.ensure((size_t)groups * mcd::HIST * sizeof(float2)))) return st;
+  if (trace_steps > 0 && (st = X.trace.ensure((size_t)trace_steps * 4 * sizeof(double)))) return st;
+  HIPCHK(hipMemcpyAsync(X.cells.p, ts.cells.data(), ts.cells.size() * sizeof(mcd::CellDev), hipMemcpyHostToDevice, X.stream));
+  HIPCHK(hipMemcpyAsync(X.tasks.p, sorted.data(), n * sizeof(mcd::TaskDev), hipMemcpyHostToDevice, X.stream));
+  HIPCHK(hipMemsetAsync(X.queue.p, 0, sizeof(int32_t), X.stream));
+  mcd::LaunchDev P;
+  P.cells = (const mcd::CellDev*)X.cells.p;
+  P.tasks = (const mcd::TaskDev*)X.tasks.p;
+  P.out = (mcd::ResultDev*)X.out.p;
+  P.queue = (int32_t*)X.queue.p;
+  P.ring = (float*)X.ring.p;
+  P.hist = (float2*)X.hist.p;
+  P.trace = trace_steps > 0 ? (double*)X.trace.p : nullptr;
+  P.trace_steps = trace_steps;
+  P.ring_cap = ring_cap;
+  P.ntasks = n;
+  P.nz_cap = nz_cap;
+  P.nv_cap = nv_cap;
+  P.grp_bytes = grp_bytes;
+  HIPCHK(hipEventRecord(X.ev0, X.stream));
+  if (mcd_launch_chatter(P, G, blocks, X.stream)) return fail(MC_ERR_HIP, "chatter kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+  HIPCHK(hipEventRecord(X.ev1, X.stream));
+  std::vector<mcd::ResultDev> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), X.out.p, n * sizeof(mcd::ResultDev), hipMemcpyDeviceToHost, X.stream));
+  if (trace_steps > 0) HIPCHK(hipMemcpyAsync(h_trace, X.trace.p, (size_t)trace_steps * 4 * sizeof(double), hipMemcpyDeviceToHost, X.stream));
+  HIPCHK(hipStreamSynchronize(X.stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
+  X.last_ms = ms;
+  for (int q = 0; q < n; q++) res[order[q]] = tmp[q];
+  return MC_OK;
+}
+
+double cell_cost(const mcd::CellDev& c) { return (double)c.spt * c.N * c.revs * c.N; }
+
+int prepare(const mc_cell* cells, int n, TaskSet& ts) {
+  if (n < 0 || (n > 0 && !cells)) return fail(MC_ERR_ARG, "null cells or negative count");
+  ts.cells.resize(n);
+  for (int i = 0; i < n; i++) {
+    int st = check_cell(cells[i], i);
+    if (st) return st;
+    make_celldev(cells[i], ts.cells[i]);
+  }
+  return MC_OK;
+}
+
+int check_opts(const mc_bisect_opts* o) {
+  if (!o) return fail(MC_ERR_ARG, "null options");
+  if (!(o->aMin > 0) || !(o->aMax >= o->aMin) || !std::isfinite(o->aMax) || o->iters < 0 || o->iters > 64)
+    return fail(MC_ERR_ARG, "need 0 < aMin <= aMax, 0 <= iters <= 64");
+  return MC_OK;
+}
+
+void to_limit(const mcd::ResultDev& r, mc_limit& l) {
+  l.alim = r.alim; l.lo = r.lo; l.hi = r.hi; l.open = r.open; l.trials = r.trials;
+  l.steps = r.steps; l.visits = r.visits; l.engaged = r.engaged;
+}
+
+// ---- stock
+struct StockImpl {
+  int nx = 0, ny = 0;
+  double w = 0, l = 0, hx = 0, hy = 0, zTop = 0, zB = 0, band = 0;
+  float *H = nullptr, *err = nullptr, *wallD = nullptr;
+  int32_t *lineCut = nullptr, *lineWall = nullptr;
+  DevBuf stamps, sdev, box, vol, xy, smp;
+};
+
+}  // namespace
+
+struct mc_stock : StockImpl {};
+
+extern "C" {
+
+int mc_init(int device) { return ensure_init(device); }
+
+const char* mc_last_error(void) { return g_err.c_str(); }
+
+int mc_last_kernel_time(double* ms) {
+  if (!ms) return fail(MC_ERR_ARG, "null pointer");
+  *ms = g_ctx.last_ms;
+  return MC_OK;
+}
+
+int mc_device_info(char* buf, int len) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if (!buf || len <= 0) return fail(MC_ERR_ARG, "null buffer");
+  snprintf(buf, len, "%s %s CUs=%d", g_ctx.name.c_str(), g_ctx.arch.c_str(), g_ctx.cus);
+  return MC_OK;
+}
+
+int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* out) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if (n > 0 && (!ap || !out)) return fail(MC_ERR_ARG, "null depth or output array");
+  TaskSet ts;
+  if ((st = prepare(cells, n, ts))) return st;
+  ts.tasks.resize(n);
+  ts.cost.resize(n);
+  ts.nzcap.resize(n);
+  for (int i = 0; i < n; i++) {
+    if (!std::isfinite(ap[i])) return fail(MC_ERR_ARG, "depth %d not finite", i);
+    mcd::TaskDev& t = ts.tasks[i];
+    std::memset(&t, 0, sizeof t);
+    t.cell = i; t.mode = mcd::TASK_SINGLE; t.ap = ap[i];
+    ts.nzcap[i] = nz_bound(cells[i], ap[i]);
+    ts.cost[i] = cell_cost(ts.cells[i]) * std::max(1, ts.nzcap[i]);
+  }
+  std::vector<mcd::ResultDev> res;
+  if ((st = run_tasks(ts, res, nullptr, 0))) return st;
+  for (int i = 0; i < n; i++) {
+    out[i].index = res[i].index; out[i].amp = res[i].amp; out[i].once = res[i].once;
+    out[i].chatter = res[i].chatter; out[i].nz = res[i].nz; out[i].steps = res[i].steps;
+  }
+  return MC_OK;
+}
+
+int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_limit* out) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if ((st = check_opts(opts))) return st;
+  if (n > 0 && !out) return fail(MC_ERR_ARG, "null output array");
+  TaskSet ts;
+  if ((st = prepare(cells, n, ts))) return st;
+  ts.tasks.resize(n);
+  ts.cost.resize(n);
+  ts.nzcap.resize(n);
+  for (int i = 0; i < n; i++) {
+    mcd::TaskDev& t = ts.tasks[i];
+    std::memset(&t, 0, sizeof t);
+    t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = opts->iters; t.aMin = opts->aMin; t.aMax = opts->aMax;
+    ts.nzcap[i] = nz_bound(cells[i], opts->aMax);
+    ts.cost[i] = cell_cost(ts.cells[i]) * (ts.nzcap[i] + opts->iters * 0.25 * ts.nzcap[i]);
+  }
+  std::vector<mcd::ResultDev> res;
+  if ((st = run_tasks(ts, res, nullptr, 0))) return st;
+  for (int i = 0; i < n; i++) to_limit(res[i], out[i]);
+  return MC_OK;
+}
+
+int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts,
+                            const double* lo_hint, const double* hi_hint, double tol_mm,
+                            mc_limit* out, int32_t* fell_back) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if ((st = check_opts(opts))) return st;
+  if (n > 0 && (!out || !lo_hint || !hi_hint)) return fail(MC_ERR_ARG, "null hint or output array");
+  TaskSet ts;
+  if ((st = prepare(cells, n, ts))) return st;
+  ts.tasks.resize(n);
+  ts.cost.resize(n);
+  ts.nzcap.resize(n);
+  for (int i = 0; i < n; i++) {
+    mcd::TaskDev& t = ts.tasks[i];
+    std::memset(&t, 0, sizeof t);
+    t.cell = i; t.iters = opts->iters; t.aMin = opts->aMin; t.aMax = opts->aMax;
+    t.lo = lo_hint[i]; t.hi = hi_hint[i]; t.tol = tol_mm;
+    const bool ok = lo_hint[i] > 0 && hi_hint[i] > lo_hint[i] && hi_hint[i] <= opts->aMax &&
+                    lo_hint[i] >= opts->aMin;
+    t.mode = ok ? mcd::TASK_BRACKET : mcd::TASK_BISECT;
+    ts.nzcap[i] = nz_bound(cells[i], opts->aMax);
+    ts.cost[i] = cell_cost(ts.cells[i]) * ts.nzcap[i];
+  }
+  std::vector<mcd::ResultDev> res;
+  if ((st = run_tasks(ts, res, nullptr, 0))) return st;
+  for (int i = 0; i < n; i++) {
+    to_limit(res[i], out[i]);
+    if (fell_back) fell_back[i] = ts.tasks[i].mode == mcd::TASK_BISECT ? 1 : res[i].fell_back;
+  }
+  return MC_OK;
+}
+
+int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps, double* Fx, double* Fy,
+                    double* x, double* y, int64_t* steps_out) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if (!cell || nsteps < 0 || !std::isfinite(ap)) return fail(MC_ERR_ARG, "bad trace arguments");
+  TaskSet ts;
+  if ((st = prepare(cell, 1, ts))) return st;
+  const mcd::CellDev& c = ts.cells[0];
+  const int64_t total = (int64_t)c.spt * c.N * c.revs;
+  const int64_t ns = std::min(nsteps, total);
+  ts.tasks.resize(1);
+  std::memset(&ts.tasks[0], 0, sizeof(mcd::TaskDev));
+  ts.tasks[0].cell = 0; ts.tasks[0].mode = mcd::TASK_SINGLE; ts.tasks[0].ap = ap;
+  ts.cost = {1.0};
+  ts.nzcap = {nz_bound(*cell, ap)};
+  std::vector<double> buf((size_t)std::max<int64_t>(ns, 1) * 4);
+  std::vector<mcd::ResultDev> res;
+  if ((st = run_tasks(ts, res, buf.data(), std::max<int64_t>(ns, 1)))) return st;
+  for (int64_t s = 0; s < ns; s++) {
+    if (Fx) Fx[s] = buf[4 * s];
+    if (Fy) Fy[s] = buf[4 * s + 1];
+    if (x) x[s] = buf[4 * s + 2];
+    if (y) y[s] = buf[4 * s + 3];
+  }
+  if (steps_out) *steps_out = ns;
+  return MC_OK;
+}
+
+// ---------------------------------------------------------------- stock
+
+int mc_stock_create(double w, double l, double h, double cell, mc_stock** out) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if (!out || !(w > 0) || !(l > 0) || !(h > 0) || !(cell > 0)) return fail(MC_ERR_ARG, "bad stock size");
+  const double nxd = mcm::round_js(w / cell) + 1, nyd = mcm::round_js(l / cell) + 1;
+  if (nxd * nyd > 1.0e9) return fail(MC_ERR_LIMIT, "stock of %.0f x %.0f nodes too large", nxd, nyd);
+  mc_stock* s = new mc_stock();
+  s->nx = (int)nxd; s->ny = (int)nyd;
+  s->w = w; s->l = l; s->zTop = 0; s->zB = -h;
+  s->hx = w / (s->nx - 1); s->hy = l / (s->ny - 1);
+  const size_t n = (size_t)s->nx * s->ny;
+  if (hipMalloc(&s->H, n * 4) != hipSuccess || hipMalloc(&s->err, n * 4) != hipSuccess ||
+      hipMalloc(&s->wallD, n * 4) != hipSuccess || hipMalloc(&s->lineCut, n * 4) != hipSuccess ||
+      hipMalloc(&s->lineWall, n * 4) != hipSuccess) {
+    mc_stock_destroy(s);
+    return fail(MC_ERR_NOMEM, "stock allocation failed");
+  }
+  hipStream_t q = g_ctx.stream;
+  if (mcs_launch_fill(s->H, n, (float)s->zTop, q) || mcs_launch_fill(s->err, n, 0.0f, q) ||
+      mcs_launch_fill(s->wallD, n, INFINITY, q) || mcs_launch_fill((float*)s->lineCut, n, 0.0f, q) ||
+      mcs_launch_fill((float*)s->lineWall, n, 0.0f, q) || hipStreamSynchronize(q) != hipSuccess) {
+    mc_stock_destroy(s);
+    return fail(MC_ERR_HIP, "stock initialisation failed");
+  }
+  *out = s;
+  return MC_OK;
+}
+
+int mc_stock_destroy(mc_stock* s) {
+  if (!s) return MC_OK;
+  for (void* p : {(void*)s->H, (void*)s->err, (void*)s->wallD, (void*)s->lineCut, (void*)s->lineWall})
+    if (p) (void)hipFree(p);
+  for (DevBuf* b : {&s->stamps, &s->sdev, &s->box, &s->vol, &s->xy, &s->smp})
+    if (b->p) (void)hipFree(b->p);
+  delete s;
+  return MC_OK;
+}
+
+int mc_stock_dims(const mc_stock* s, int32_t* nx, int32_t* ny, double* hx, double* hy) {
+  if (!s) return fail(MC_ERR_ARG, "null stock");
+  if (nx) *nx = s->nx;
+  if (ny) *ny = s->ny;
+  if (hx) *hx = s->hx;
+  if (hy) *hy = s->hy;
+  return MC_OK;
+}
+
+int mc_stock_set_wall_band(mc_stock* s, double wallBand) {
+  if (!s || !std::isfinite(wallBand)) return fail(MC_ERR_ARG, "bad wallBand");
+  s->band = wallBand;
+  return MC_OK;
+}
+
+int mc_stock_read_h(const mc_stock* s, float* H) {
+  if (!s || !H) return fail(MC_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpy(H, s->H, (size_t)s->nx * s->ny * 4, hipMemcpyDeviceToHost));
+  return MC_OK;
+}
+
+int mc_stock_write_h(mc_stock* s, const float* H) {
+  if (!s || !H) return fail(MC_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpy(s->H, H, (size_t)s->nx * s->ny * 4, hipMemcpyHostToDevice));
+  return MC_OK;
+}
+
+int mc_stock_read_side(const mc_stock* s, float* err, float* wallD, int32_t* lineCut, int32_t* lineWall) {
+  if (!s) return fail(MC_ERR_ARG, "null stock");
+  const size_t b = (size_t)s->nx * s->ny * 4;
+  if (err) HIPCHK(hipMemcpy(err, s->err, b, hipMemcpyDeviceToHost));
+  if (wallD) HIPCHK(hipMemcpy(wallD, s->wallD, b, hipMemcpyDeviceToHost));
+  if (lineCut) HIPCHK(hipMemcpy(lineCut, s->lineCut, b, hipMemcpyDeviceToHost));
+  if (lineWall) HIPCHK(hipMemcpy(lineWall, s->lineWall, b, hipMemcpyDeviceToHost));
+  return MC_OK;
+}
+
+int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol) {
+  if (!s || n < 0 || (n > 0 && !stamps)) return fail(MC_ERR_ARG, "bad stamp batch");
+  for (int i = 0; i < n; i++) {
+    const mc_stamp& p = stamps[i];
+    if (!std::isfinite(p.cx) || !std::isfinite(p.cy) || !std::isfinite(p.tipZ) || !(p.R > 0) ||
+        (p.type != MC_TOOL_FLAT && p.type != MC_TOOL_BALL))
+      return fail(MC_ERR_ARG, "stamp %d: bad position, radius or type", i);
+  }
+  if (n == 0) return MC_OK;
+  int st;
+  if ((st = s->stamps.ensure((size_t)n * sizeof(mc_stamp)))) return st;
+  const int chunks = (n + mcs_chunk() - 1) / mcs_chunk();
+  if ((st = s->sdev.ensure((size_t)n * mcs_stampdev_bytes()))) return st;
+  if ((st = s->box.ensure((size_t)chunks * mcs_chunkbox_bytes()))) return st;
+  if (vol && (st = s->vol.ensure((size_t)n * sizeof(double)))) return st;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  hipStream_t q = g_ctx.stream;
+  HIPCHK(hipMemcpyAsync(s->stamps.p, stamps, (size_t)n * sizeof(mc_stamp), hipMemcpyHostToDevice, q));
+  if (vol && mcs_launch_fill((float*)s->vol.p, (size_t)n * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
+  if (mcs_launch_stamps((const mc_stamp*)s->stamps.p, n, s->sdev.p, s->box.p, s->nx, s->ny, s->hx, s->hy,
+                        s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut, s->lineWall,
+                        vol ? (double*)s->vol.p : nullptr, q, g_ctx.ev0, g_ctx.ev1))
+    return fail(MC_ERR_HIP, "stamp kernels failed: %s", hipGetErrorString(hipGetLastError()));
+  if (vol) HIPCHK(hipMemcpyAsync(vol, s->vol.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, q));
+  HIPCHK(hipStreamSynchronize(q));
+  {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1));
+    g_ctx.last_ms = ms;
+  }
+  if (vol) {
+    const double area = s->hx * s->hy;
+    for (int i = 0; i < n; i++) vol[i] *= area;
+  }
+  return MC_OK;
+}
+
+int mc_stock_sample(const mc_stock* s, const double* xy, int n, double* out) {
+  if (!s || n < 0 || (n > 0 && (!xy || !out))) return fail(MC_ERR_ARG, "bad sample arguments");
+  if (n == 0) return MC_OK;
+  mc_stock* m = const_cast<mc_stock*>(s);
+  int st;
+  if ((st = m->xy.ensure((size_t)n * 16))) return st;
+  if ((st = m->smp.ensure((size_t)n * 8))) return st;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  hipStream_t q = g_ctx.stream;
+  HIPCHK(hipMemcpyAsync(m->xy.p, xy, (size_t)n * 16, hipMemcpyHostToDevice, q));
+  if (mcs_launch_sample(s->H, s->nx, s->ny, s->hx, s->hy, (const double*)m->xy.p, n, (double*)m->smp.p, q))
+    return fail(MC_ERR_HIP, "sample kernel failed");
+  HIPCHK(hipMemcpyAsync(out, m->smp.p, (size_t)n * 8, hipMemcpyDeviceToHost, q));
+  HIPCHK(hipStreamSynchronize(q));
+  return MC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// mc_dev.h — device-side data layouts of the chatter search (shared by the host glue in
+// api.cpp and the kernels in chatter.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mcsim.h"
+
+namespace mcd {
+
+constexpr int HIST = 2048;   // chatter-index history kept per group (the index reads <= 1416)
+constexpr int TS = 64;       // once-per-tooth sample ring (src/cutting.js:29)
+
+// Per-cell constants: CuttingEngine.configure (src/cutting.js:39-82) evaluated on the
+// host for the cell's speed and modes.  Everything that depends on the trial depth
+// (fluteLen = max(tool.fluteLen, ap + 1), dz, slices) is derived per trial on the device.
+struct CellDev {
+  double R, tanB, toolFluteLen, omega, dt, dphi, pitch, binScale, fz, hMax;
+  double Ktc, Krc, Kte, Kre, mc, Ci;  // mc and Ci already 0 when nonlinear is off
+  double kx[MC_MAX_MODES], mx[MC_MAX_MODES], cx[MC_MAX_MODES];
+  double ky[MC_MAX_MODES], my[MC_MAX_MODES], cy[MC_MAX_MODES];
+  double dR[MC_MAX_FLUTES];
+  int32_t N, ball, spt, spr, decim, nmx, nmy, revs;
+  float Hb[MC_BINS];
+};
+
+enum TaskMode : int32_t {
+  TASK_SINGLE = 0,   // one chatterAt trial at depth ap
+  TASK_BISECT = 1,   // timeDomainLimit (src/stabtd.js:24-32)
+  TASK_BRACKET = 2,  // bracketed search from [lo, hi], falls back to TASK_BISECT
+};
+
+struct TaskDev {
+  int32_t cell, mode, iters, pad;
+  double ap;          // TASK_SINGLE depth
+  double aMin, aMax;  // bisection range
+  double lo, hi, tol; // TASK_BRACKET hint and stopping width
+};
+
+struct ResultDev {
+  double alim, lo, hi;
+  double index, amp, once;  // last trial's chatter index
+  int32_t open, trials, chatter, nz, fell_back, pad;
+  int64_t steps, visits, engaged;
+};
+
+struct LaunchDev {
+  const CellDev* cells;
+  const TaskDev* tasks;
+  ResultDev* out;
+  int32_t* queue;        // next task (atomic counter)
+  float* ring;           // delay lines, ring_cap floats per group
+  float2* hist;          // chatter-index history, HIST per group
+  double* trace;         // TASK_SINGLE trace: 4 doubles per step (Fx, Fy, x, y) or null
+  int64_t trace_steps;
+  int64_t ring_cap;
+  int32_t ntasks;
+  int32_t nz_cap;        // slice capacity of the per-group LDS tables
+  int32_t nv_cap;        // (tooth, slice) visit capacity of the per-group LDS tables
+  int32_t grp_bytes;     // LDS bytes per group
+};
+
+}  // namespace mcd

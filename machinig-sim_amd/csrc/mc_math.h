@@ -1,0 +1,368 @@
+// mc_math.h — JavaScript-faithful FP64 math for host and gfx950 device code.
+//
+// The reference evaluates its physics with V8's Math.sin / Math.cos / Math.tan /
+// Math.pow / Math.hypot, which are ports of Sun's fdlibm 5.3 (V8 src/base/ieee754.cc).
+// Per-cell chatter verdicts near the 0.45 threshold (src/stabtd.js:10) are only robust
+// when the GPU reproduces those doubles bit for bit, so this header restates the
+// published fdlibm method: argument reduction by pi/2 (e_rem_pio2.c, medium range),
+// the minimax kernels k_sin.c / k_cos.c / k_tan.c and e_pow.c, plus V8's hypot
+// (scale by the largest magnitude, compensated sum, sqrt, rescale) and JavaScript's
+// double % (an exact remainder, here by exact power-of-two multiples of the divisor).
+//
+// Everything is written so that `hipcc -ffp-contract=off` produces the same doubles on
+// the host and on the device: no fused multiply-add, IEEE division and sqrt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MC_HD __host__ __device__ __forceinline__
+
+namespace mcm {
+
+constexpr double JS_PI = 3.141592653589793;
+constexpr double TWO_PI = 2 * JS_PI;
+
+MC_HD int32_t hiw(double x) { return (int32_t)(__builtin_bit_cast(uint64_t, x) >> 32); }
+MC_HD uint32_t low(double x) { return (uint32_t)__builtin_bit_cast(uint64_t, x); }
+MC_HD double words(uint32_t h, uint32_t l) {
+  return __builtin_bit_cast(double, ((uint64_t)h << 32) | (uint64_t)l);
+}
+MC_HD double clear_lo(double x) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) & 0xffffffff00000000ull);
+}
+MC_HD double fabs_(double x) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) & 0x7fffffffffffffffull);
+}
+
+// ---- reduction by pi/2 for |x| <= 2^19 * pi/2 --------------------------------------
+__host__ __device__ inline int rem_pio2(double x, double& y0, double& y1) {
+  constexpr double invpio2 = 6.36619772367581382433e-01;  // 0x3FE45F30 6DC9C883
+  constexpr double pio2_1 = 1.57079632673412561417e+00;   // 0x3FF921FB 54400000
+  constexpr double pio2_1t = 6.07710050650619224932e-11;  // 0x3DD0B461 1A626331
+  constexpr double pio2_2 = 6.07710050630396597660e-11;   // 0x3DD0B461 1A600000
+  constexpr double pio2_2t = 2.02226624879595063154e-21;  // 0x3BA3198A 2E037073
+  constexpr double pio2_3 = 2.02226624871116645580e-21;   // 0x3BA3198A 2E000000
+  constexpr double pio2_3t = 8.47842766036889956997e-32;  // 0x397B839A 252049C1
+  // high words of n*pi/2, n = 1..32: the cancellation check runs only near them
+  constexpr int32_t npio2_hw[32] = {
+      0x3FF921FB, 0x400921FB, 0x4012D97C, 0x401921FB, 0x401F6A7A, 0x4022D97C, 0x4025FDBB,
+      0x402921FB, 0x402C463A, 0x402F6A7A, 0x4031475C, 0x4032D97C, 0x40346B9C, 0x4035FDBB,
+      0x40378FDB, 0x403921FB, 0x403AB41B, 0x403C463A, 0x403DD85A, 0x403F6A7A, 0x40407E4C,
+      0x4041475C, 0x4042106C, 0x4042D97C, 0x4043A28C, 0x40446B9C, 0x404534AC, 0x4045FDBB,
+      0x4046C6CB, 0x40478FDB, 0x404858EB, 0x404921FB};
+  const int32_t hx = hiw(x), ix = hx & 0x7fffffff;
+  if (ix <= 0x3fe921fb) { y0 = x; y1 = 0; return 0; }
+  if (ix < 0x4002d97c) {  // |x| < 3pi/4: n = +-1
+    double z;
+    if (hx > 0) {
+      z = x - pio2_1;
+      if (ix != 0x3ff921fb) { y0 = z - pio2_1t; y1 = (z - y0) - pio2_1t; }
+      else { z -= pio2_2; y0 = z - pio2_2t; y1 = (z - y0) - pio2_2t; }
+      return 1;
+    }
+    z = x + pio2_1;
+    if (ix != 0x3ff921fb) { y0 = z + pio2_1t; y1 = (z - y0) + pio2_1t; }
+    else { z += pio2_2; y0 = z + pio2_2t; y1 = (z - y0) + pio2_2t; }
+    return -1;
+  }
+  // medium size (the only other range the engine produces)
+  double t = fabs_(x);
+  const int n = (int)(t * invpio2 + 0.5);
+  const double fn = (double)n;
+  double r = t - fn * pio2_1;
+  double w = fn * pio2_1t;
+  if (n < 32 && ix != npio2_hw[n - 1]) {
+    y0 = r - w;
+  } else {
+    const int j = ix >> 20;
+    y0 = r - w;
+    int i = j - ((hiw(y0) >> 20) & 0x7ff);
+    if (i > 16) {
+      t = r; w = fn * pio2_2; r = t - w;
+      w = fn * pio2_2t - ((t - r) - w);
+      y0 = r - w;
+      i = j - ((hiw(y0) >> 20) & 0x7ff);
+      if (i > 49) {
+        t = r; w = fn * pio2_3; r = t - w;
+        w = fn * pio2_3t - ((t - r) - w);
+        y0 = r - w;
+      }
+    }
+  }
+  y1 = (r - y0) - w;
+  if (hx < 0) { y0 = -y0; y1 = -y1; return -n; }
+  return n;
+}
+
+MC_HD double k_sin(double x, double y, int iy) {
+  constexpr double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                   S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                   S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+  const int32_t ix = hiw(x) & 0x7fffffff;
+  if (ix < 0x3e400000 && (int)x == 0) return x;
+  const double z = x * x, v = z * x;
+  const double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+  if (iy == 0) return x + v * (S1 + z * r);
+  return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+
+MC_HD double k_cos(double x, double y) {
+  constexpr double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                   C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                   C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+  const int32_t ix = hiw(x) & 0x7fffffff;
+  if (ix < 0x3e400000 && (int)x == 0) return 1.0;
+  const double z = x * x;
+  const double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+  if (ix < 0x3FD33333) return 1.0 - (0.5 * z - (z * r - x * y));
+  const double qx = ix > 0x3fe90000 ? 0.28125 : words((uint32_t)(ix - 0x00200000), 0);
+  const double hz = 0.5 * z - qx, a = 1.0 - qx;
+  return a - (hz - (z * r - x * y));
+}
+
+// sin and cos of one argument, sharing the reduction (each equals Math.sin / Math.cos)
+MC_HD void sincos_js(double x, double& s, double& c) {
+  const int32_t ix = hiw(x) & 0x7fffffff;
+  if (ix <= 0x3fe921fb) { s = k_sin(x, 0.0, 0); c = k_cos(x, 0.0); return; }
+  if (ix >= 0x7ff00000) { s = c = x - x; return; }
+  double y0, y1;
+  const int n = rem_pio2(x, y0, y1);
+  const double ks = k_sin(y0, y1, 1), kc = k_cos(y0, y1);
+  switch (n & 3) {
+    case 0: s = ks; c = kc; break;
+    case 1: s = kc; c = -ks; break;
+    case 2: s = -ks; c = -kc; break;
+    default: s = -kc; c = ks; break;
+  }
+}
+
+MC_HD double cos_js(double x) { double s, c; sincos_js(x, s, c); return c; }
+MC_HD double sin_js(double x) { double s, c; sincos_js(x, s, c); return s; }
+
+__host__ __device__ inline double k_tan(double x, double y, int iy) {
+  constexpr double T[13] = {
+      3.33333333333334091986e-01, 1.33333333333201242699e-01, 5.39682539762260521377e-02,
+      2.18694882948595424599e-02, 8.86323982359930005737e-03, 3.59207910759131235356e-03,
+      1.45620945432529025516e-03, 5.88041240820264096874e-04, 2.46463134818469906812e-04,
+      7.81794442939557092300e-05, 7.14072491382608190305e-05, -1.85586374855275456654e-05,
+      2.59073051863633712884e-05};
+  constexpr double pio4 = 7.85398163397448278999e-01, pio4lo = 3.06161699786838301793e-17;
+  double z, r, v, w, s;
+  const int32_t hx = hiw(x), ix = hx & 0x7fffffff;
+  if (ix < 0x3e300000 && (int)x == 0) {
+    if (((ix | (int32_t)low(x)) | (iy + 1)) == 0) return 1.0 / fabs_(x);
+    if (iy == 1) return x;
+    z = w = x + y;
+    z = clear_lo(z);
+    v = y - (z - x);
+    double t, a;
+    t = a = -1.0 / w;
+    t = clear_lo(t);
+    s = 1.0 + t * z;
+    return t + a * (s + t * v);
+  }
+  if (ix >= 0x3FE59428) {
+    if (hx < 0) { x = -x; y = -y; }
+    z = pio4 - x;
+    w = pio4lo - y;
+    x = z + w;
+    y = 0.0;
+  }
+  z = x * x;
+  w = z * z;
+  r = T[1] + w * (T[3] + w * (T[5] + w * (T[7] + w * (T[9] + w * T[11]))));
+  v = z * (T[2] + w * (T[4] + w * (T[6] + w * (T[8] + w * (T[10] + w * T[12])))));
+  s = z * x;
+  r = y + z * (s * (r + v) + y);
+  r += T[0] * s;
+  w = x + r;
+  if (ix >= 0x3FE59428) {
+    v = (double)iy;
+    return (double)(1 - ((hx >> 30) & 2)) * (v - 2.0 * (x - (w * w / (w + v) - r)));
+  }
+  if (iy == 1) return w;
+  z = clear_lo(w);
+  v = r - (z - x);
+  double t, a;
+  t = a = -1.0 / w;
+  t = clear_lo(t);
+  s = 1.0 + t * z;
+  return t + a * (s + t * v);
+}
+
+__host__ __device__ inline double tan_js(double x) {
+  const int32_t ix = hiw(x) & 0x7fffffff;
+  if (ix <= 0x3fe921fb) return k_tan(x, 0.0, 1);
+  if (ix >= 0x7ff00000) return x - x;
+  double y0, y1;
+  const int n = rem_pio2(x, y0, y1);
+  return k_tan(y0, y1, 1 - ((n & 1) << 1));
+}
+
+// pow(x, y) for the size-effect law (h0/h)^mc: x > 0 finite, y finite (e_pow.c; the
+// special-value prologue keeps only the branches reachable with such arguments)
+__host__ __device__ inline double pow_js(double x, double y) {
+  constexpr double bp[2] = {1.0, 1.5};
+  constexpr double dp_h[2] = {0.0, 5.84962487220764160156e-01};
+  constexpr double dp_l[2] = {0.0, 1.35003920212974897128e-08};
+  constexpr double two53 = 9007199254740992.0;
+  constexpr double L1 = 5.99999999999994648725e-01, L2 = 4.28571428578550184252e-01,
+                   L3 = 3.33333329818377432918e-01, L4 = 2.72728123808534006489e-01,
+                   L5 = 2.30660745775561754067e-01, L6 = 2.06975017800338417784e-01;
+  constexpr double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03,
+                   P3 = 6.61375632143793436117e-05, P4 = -1.65339022054652515390e-06,
+                   P5 = 4.13813679705723846039e-08;
+  constexpr double lg2 = 6.93147180559945286227e-01, lg2_h = 6.93147182464599609375e-01,
+                   lg2_l = -1.90465429995776804525e-09;
+  constexpr double ovt = 8.0085662595372944372e-17;
+  constexpr double cp = 9.61796693925975554329e-01, cp_h = 9.61796700954437255859e-01,
+                   cp_l = -7.02846165095275826516e-09;
+  constexpr double ivln2 = 1.44269504088896338700e+00, ivln2_h = 1.44269502162933349609e+00,
+                   ivln2_l = 1.92596299112661746887e-08;
+  constexpr double huge = 1.0e300, tiny = 1.0e-300;
+  double z, ax, z_h, z_l, p_h, p_l, y1, t1, t2, r, t, u, v, w;
+  int32_t i, j, k, n;
+  const int32_t hx = hiw(x), hy = hiw(y);
+  const uint32_t lx = low(x), ly = low(y);
+  int32_t ix = hx & 0x7fffffff;
+  const int32_t iy = hy & 0x7fffffff;
+  if ((iy | (int32_t)ly) == 0) return 1.0;
+  if (ix > 0x7ff00000 || (ix == 0x7ff00000 && lx != 0) || iy > 0x7ff00000 ||
+      (iy == 0x7ff00000 && ly != 0))
+    return x + y;
+  if (ly == 0) {
+    if (iy == 0x3ff00000) return hy < 0 ? 1.0 / x : x;
+    if (hy == 0x40000000) return x * x;
+    if (hy == 0x3fe00000 && hx >= 0) return __builtin_sqrt(x);
+  }
+  ax = fabs_(x);
+  if (lx == 0 && (ix == 0x7ff00000 || ix == 0 || ix == 0x3ff00000)) {
+    z = ax;
+    if (hy < 0) z = 1.0 / z;
+    return z;  // x > 0 here
+  }
+  if (iy > 0x41e00000) {  // |y| > 2^31
+    if (iy > 0x43f00000) {
+      if (ix <= 0x3fefffff) return hy < 0 ? huge * huge : tiny * tiny;
+      if (ix >= 0x3ff00000) return hy > 0 ? huge * huge : tiny * tiny;
+    }
+    if (ix < 0x3fefffff) return hy < 0 ? huge * huge : tiny * tiny;
+    if (ix > 0x3ff00000) return hy > 0 ? huge * huge : tiny * tiny;
+    t = ax - 1.0;
+    w = (t * t) * (0.5 - t * (0.3333333333333333333333 - t * 0.25));
+    u = ivln2_h * t;
+    v = t * ivln2_l - w * ivln2;
+    t1 = clear_lo(u + v);
+    t2 = v - (t1 - u);
+  } else {
+    double ss, s2, s_h, s_l, t_h, t_l;
+    n = 0;
+    if (ix < 0x00100000) { ax *= two53; n -= 53; ix = hiw(ax); }
+    n += (ix >> 20) - 0x3ff;
+    j = ix & 0x000fffff;
+    ix = j | 0x3ff00000;
+    if (j <= 0x3988E) k = 0;
+    else if (j < 0xBB67A) k = 1;
+    else { k = 0; n += 1; ix -= 0x00100000; }
+    ax = words((uint32_t)ix, low(ax));
+    u = ax - bp[k];
+    v = 1.0 / (ax + bp[k]);
+    ss = u * v;
+    s_h = clear_lo(ss);
+    t_h = words((uint32_t)(((ix >> 1) | 0x20000000) + 0x00080000 + (k << 18)), 0);
+    t_l = ax - (t_h - bp[k]);
+    s_l = v * ((u - s_h * t_h) - s_h * t_l);
+    s2 = ss * ss;
+    r = s2 * s2 * (L1 + s2 * (L2 + s2 * (L3 + s2 * (L4 + s2 * (L5 + s2 * L6)))));
+    r += s_l * (s_h + ss);
+    s2 = s_h * s_h;
+    t_h = clear_lo(3.0 + s2 + r);
+    t_l = r - ((t_h - 3.0) - s2);
+    u = s_h * t_h;
+    v = s_l * t_h + t_l * ss;
+    p_h = clear_lo(u + v);
+    p_l = v - (p_h - u);
+    z_h = cp_h * p_h;
+    z_l = cp_l * p_h + p_l * cp + dp_l[k];
+    t = (double)n;
+    t1 = clear_lo(((z_h + z_l) + dp_h[k]) + t);
+    t2 = z_l - (((t1 - t) - dp_h[k]) - z_h);
+  }
+  y1 = clear_lo(y);
+  p_l = (y - y1) * t1 + y * t2;
+  p_h = y1 * t1;
+  z = p_l + p_h;
+  j = hiw(z);
+  i = (int32_t)low(z);
+  if (j >= 0x40900000) {
+    if (((j - 0x40900000) | i) != 0) return huge * huge;
+    if (p_l + ovt > z - p_h) return huge * huge;
+  } else if ((j & 0x7fffffff) >= 0x4090cc00) {
+    if (((j - (int32_t)0xc090cc00) | i) != 0) return tiny * tiny;
+    if (p_l <= z - p_h) return tiny * tiny;
+  }
+  i = j & 0x7fffffff;
+  k = (i >> 20) - 0x3ff;
+  n = 0;
+  if (i > 0x3fe00000) {
+    n = j + (0x00100000 >> (k + 1));
+    k = ((n & 0x7fffffff) >> 20) - 0x3ff;
+    t = words((uint32_t)(n & ~(0x000fffff >> k)), 0);
+    n = ((n & 0x000fffff) | 0x00100000) >> (20 - k);
+    if (j < 0) n = -n;
+    p_h -= t;
+  }
+  t = clear_lo(p_l + p_h);
+  u = t * lg2_h;
+  v = (p_l - (t - p_h)) * lg2 + t * lg2_l;
+  z = u + v;
+  w = v - (z - u);
+  t = z * z;
+  t1 = z - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+  r = (z * t1) / (t1 - 2.0) - (w + z * w);
+  z = 1.0 - (r - z);
+  j = hiw(z) + (n << 20);
+  if ((j >> 20) <= 0) return __builtin_ldexp(z, n);  // subnormal result
+  return words((uint32_t)j, low(z));
+}
+
+// Math.hypot(a, b) as V8's builtin computes it
+MC_HD double hypot_js(double a, double b) {
+  const double fa = fabs_(a), fb = fabs_(b);
+  if (fa == __builtin_inf() || fb == __builtin_inf()) return __builtin_inf();
+  if (a != a || b != b) return __builtin_nan("");
+  const double mx = fa > fb ? fa : fb;
+  if (mx == 0) return 0;
+  double sum = 0, comp = 0, nrm, summand, prelim;
+  nrm = fa / mx; summand = nrm * nrm - comp; prelim = sum + summand;
+  comp = (prelim - sum) - summand; sum = prelim;
+  nrm = fb / mx; summand = nrm * nrm - comp; prelim = sum + summand;
+  sum = prelim;
+  return __builtin_sqrt(sum) * mx;
+}
+
+// JavaScript x % (2*pi) for the engine's angles: an exact remainder, obtained by
+// subtracting exact power-of-two multiples of 2*pi (each subtraction is exact by
+// Sterbenz's lemma because the running value lies in [m, 2m) for the multiple m)
+MC_HD double fmod_2pi(double x) {
+  double r = fabs_(x);
+  if (r >= TWO_PI) {
+    double m = TWO_PI;
+    while (m * 2 <= r) m *= 2;
+    while (m >= TWO_PI) {
+      if (r >= m) r -= m;
+      m *= 0.5;
+    }
+  }
+  return __builtin_copysign(r, x);
+}
+
+// Math.round for the engine's positive arguments: halves round up
+MC_HD double round_js(double x) {
+  double r = __builtin_floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  return r;
+}
+
+}  // namespace mcm

@@ -50,7 +50,7 @@ class BisectOpts(C.Structure):
 class Limit(C.Structure):
     _fields_ = [("alim", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
                 ("open", C.c_int32), ("trials", C.c_int32), ("steps", C.c_int64),
-                ("visits", C.c_int64)]
+                ("visits", C.c_int64), ("engaged", C.c_int64)]
 
 
 class Stamp(C.Structure):

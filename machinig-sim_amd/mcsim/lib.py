@@ -1,0 +1,180 @@
+"""ctypes bindings of libmcsim.so (include/mcsim.h).
+
+The library must be built in-tree (``python machinig-sim_amd/build.py``); loading fails
+loudly when it is missing — there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, cell_array, MC_OK)
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libmcsim.so")
+
+EXPORTS = [
+    "mc_init", "mc_last_error", "mc_device_info", "mc_last_kernel_time", "mc_chatter_batch", "mc_bisect_batch",
+    "mc_bisect_bracket_batch", "mc_engine_trace", "mc_stock_create", "mc_stock_destroy",
+    "mc_stock_dims", "mc_stock_set_wall_band", "mc_stock_read_h", "mc_stock_write_h",
+    "mc_stock_read_side", "mc_stamp_batch", "mc_stock_sample",
+]
+
+_LIB = None
+
+
+class McError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libmcsim.so and declare its signatures (no GPU call is made)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise McError("libmcsim.so not built: run python machinig-sim_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.mc_init.argtypes = [C.c_int]
+    L.mc_last_error.restype = C.c_char_p
+    L.mc_device_info.argtypes = [C.c_char_p, C.c_int]
+    L.mc_last_kernel_time.argtypes = [C.POINTER(C.c_double)]
+    L.mc_chatter_batch.argtypes = [C.POINTER(Cell), P, C.c_int, C.POINTER(Trial)]
+    L.mc_bisect_batch.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), C.POINTER(Limit)]
+    L.mc_bisect_bracket_batch.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), P, P,
+                                          C.c_double, C.POINTER(Limit), P]
+    L.mc_engine_trace.argtypes = [C.POINTER(Cell), C.c_double, C.c_int64, P, P, P, P,
+                                  C.POINTER(C.c_int64)]
+    L.mc_stock_create.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(P)]
+    L.mc_stock_destroy.argtypes = [P]
+    L.mc_stock_dims.argtypes = [P, P, P, P, P]
+    L.mc_stock_set_wall_band.argtypes = [P, C.c_double]
+    L.mc_stock_read_h.argtypes = [P, P]
+    L.mc_stock_write_h.argtypes = [P, P]
+    L.mc_stock_read_side.argtypes = [P, P, P, P, P]
+    L.mc_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
+    L.mc_stock_sample.argtypes = [P, P, C.c_int, P]
+    _LIB = L
+    return L
+
+
+def _check(st):
+    if st != MC_OK:
+        raise McError("libmcsim error %d: %s" % (st, load().mc_last_error().decode()))
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    _check(load().mc_device_info(buf, 256))
+    return buf.value.decode()
+
+
+def last_kernel_ms():
+    ms = C.c_double()
+    _check(load().mc_last_kernel_time(C.byref(ms)))
+    return ms.value
+
+
+def _cells(cells):
+    if isinstance(cells, C.Array):
+        return cells, len(cells)
+    arr = cell_array(cells)
+    return arr, len(cells)
+
+
+def chatter_batch(cells, ap):
+    """chatterAt for (cells[i], ap[i]); returns the Trial array"""
+    arr, n = _cells(cells)
+    ap = np.ascontiguousarray(ap, dtype=np.float64)
+    assert len(ap) == n
+    out = (Trial * n)()
+    _check(load().mc_chatter_batch(arr, ap.ctypes.data, n, out))
+    return out
+
+
+def bisect_batch(cells, aMin=0.05, aMax=10.0, iters=10):
+    """timeDomainLimit for every cell; returns the Limit array"""
+    arr, n = _cells(cells)
+    o = BisectOpts(aMin, aMax, iters, 0)
+    out = (Limit * n)()
+    _check(load().mc_bisect_batch(arr, n, C.byref(o), out))
+    return out
+
+
+def bracket_batch(cells, lo, hi, tol, aMin=0.05, aMax=10.0, iters=10):
+    arr, n = _cells(cells)
+    o = BisectOpts(aMin, aMax, iters, 0)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    out = (Limit * n)()
+    fb = np.zeros(n, dtype=np.int32)
+    _check(load().mc_bisect_bracket_batch(arr, n, C.byref(o), lo.ctypes.data, hi.ctypes.data,
+                                          float(tol), out, fb.ctypes.data))
+    return out, fb
+
+
+def engine_trace(cell, ap, nsteps):
+    a = [np.zeros(max(1, nsteps)) for _ in range(4)]
+    n = C.c_int64(0)
+    _check(load().mc_engine_trace(C.byref(cell), float(ap), int(nsteps), *[x.ctypes.data for x in a],
+                                  C.byref(n)))
+    return [x[:n.value] for x in a]
+
+
+class Stock:
+    """Device-resident heightfield (new Stock(w, l, h, cell), src/stock.js:9)."""
+
+    def __init__(self, w, l, h, cell=0.3):
+        self.L = load()
+        self.p = C.c_void_p()
+        _check(self.L.mc_stock_create(float(w), float(l), float(h), float(cell), C.byref(self.p)))
+        nx, ny = C.c_int32(), C.c_int32()
+        hx, hy = C.c_double(), C.c_double()
+        _check(self.L.mc_stock_dims(self.p, C.byref(nx), C.byref(ny), C.byref(hx), C.byref(hy)))
+        self.nx, self.ny, self.hx, self.hy = nx.value, ny.value, hx.value, hy.value
+
+    def close(self):
+        if self.p:
+            self.L.mc_stock_destroy(self.p)
+            self.p = None
+
+    __del__ = close
+
+    def set_wall_band(self, b):
+        _check(self.L.mc_stock_set_wall_band(self.p, float(b)))
+
+    def H(self):
+        out = np.empty(self.nx * self.ny, dtype=np.float32)
+        _check(self.L.mc_stock_read_h(self.p, out.ctypes.data))
+        return out
+
+    def write_H(self, H):
+        H = np.ascontiguousarray(H, dtype=np.float32)
+        assert H.size == self.nx * self.ny
+        _check(self.L.mc_stock_write_h(self.p, H.ctypes.data))
+
+    def side(self):
+        n = self.nx * self.ny
+        e, w = np.empty(n, np.float32), np.empty(n, np.float32)
+        lc, lw = np.empty(n, np.int32), np.empty(n, np.int32)
+        _check(self.L.mc_stock_read_side(self.p, e.ctypes.data, w.ctypes.data, lc.ctypes.data,
+                                         lw.ctypes.data))
+        return e, w, lc, lw
+
+    def stamps(self, stamps, n=None, want_vol=True):
+        if not isinstance(stamps, C.Array):
+            arr = (Stamp * len(stamps))()
+            for i, s in enumerate(stamps):
+                arr[i] = s
+            stamps = arr
+        n = len(stamps) if n is None else n
+        vol = np.zeros(n) if want_vol else None
+        _check(self.L.mc_stamp_batch(self.p, stamps, n, vol.ctypes.data if want_vol else None))
+        return vol
+
+    def sample(self, xy):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        out = np.empty(len(xy))
+        _check(self.L.mc_stock_sample(self.p, xy.ctypes.data, len(xy), out.ctypes.data))
+        return out

@@ -1,0 +1,71 @@
+"""Stamp schedule along a G1 path: the sub-step positions Simulator.step visits
+(src/sim.js:351-377, with an unlimited frame budget), in the reference's arithmetic.
+
+segLen is Math.hypot of the three deltas (V8: scale by the largest magnitude, Kahan-
+compensated sum, sqrt, rescale); the running arc length segS accumulates ds in order;
+p = a + (b - a) * (segS / segLen).
+"""
+import math
+
+import numpy as np
+
+from ._abi import Stamp
+
+
+def hypot_js(*v):
+    a = [abs(float(x)) for x in v]
+    if any(math.isinf(x) for x in a):
+        return math.inf
+    if any(math.isnan(x) for x in a):
+        return math.nan
+    mx = max(a) if a else 0.0
+    if mx == 0:
+        return 0.0
+    s = 0.0
+    comp = 0.0
+    for x in a:
+        n = x / mx
+        summand = n * n - comp
+        pre = s + summand
+        comp = (pre - s) - summand
+        s = pre
+    return math.sqrt(s) * mx
+
+
+def stamp_step(hx):
+    """stampStep = min(hx, 0.4) * 0.6 (src/sim.js:355)"""
+    return min(hx, 0.4) * 0.6
+
+
+def schedule(segments, hx, err_val=0.0):
+    """segments: (x0, y0, z0, x1, y1, z1, R, type, line) -> numpy record arrays of stamps
+    (cx, cy, tipZ, R, type, line) and the index of each stamp's segment."""
+    step = stamp_step(hx)
+    cx, cy, cz, rr, tt, ll, seg = [], [], [], [], [], [], []
+    for si, (x0, y0, z0, x1, y1, z1, R, typ, line) in enumerate(segments):
+        seg_len = hypot_js(x1 - x0, y1 - y0, z1 - z0)
+        s = 0.0
+        while not (seg_len - s <= 1e-9):
+            ds = min(step, seg_len - s)
+            s += ds
+            f = s / seg_len if seg_len > 0 else 1.0
+            cx.append(x0 + (x1 - x0) * f)
+            cy.append(y0 + (y1 - y0) * f)
+            cz.append(z0 + (z1 - z0) * f)
+            rr.append(R)
+            tt.append(typ)
+            ll.append(line)
+            seg.append(si)
+    out = np.zeros(len(cx), dtype=[("cx", "<f8"), ("cy", "<f8"), ("tipZ", "<f8"), ("R", "<f8"),
+                                   ("type", "<i4"), ("line", "<i4"), ("errVal", "<f8")])
+    out["cx"], out["cy"], out["tipZ"], out["R"] = cx, cy, cz, rr
+    out["type"], out["line"], out["errVal"] = tt, ll, err_val
+    return out, np.asarray(seg, dtype=np.int64)
+
+
+def as_stamp_array(rec):
+    """numpy stamp records -> ctypes Stamp array sharing the same bytes layout"""
+    assert rec.dtype.itemsize == np.dtype([("a", "<f8", 4), ("b", "<i4", 2), ("c", "<f8")]).itemsize
+    rec = np.ascontiguousarray(rec)
+    arr = (Stamp * len(rec)).from_buffer_copy(rec.tobytes())
+    return arr

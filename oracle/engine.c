@@ -317,6 +317,7 @@ static int chatter(const mc_cell *c, double ap, mc_limit *acc, int *st) {
   acc->trials++;
   acc->steps += cnt.steps;
   acc->visits += cnt.visits;
+  acc->engaged += cnt.engaged;
   return t.chatter;
 }
 
@@ -362,6 +363,7 @@ fallback:
   if (fell_back) *fell_back = 1;
   st = or_time_domain_limit(c, o, out);
   out->trials += spent.trials; out->steps += spent.steps; out->visits += spent.visits;
+  out->engaged += spent.engaged;
   return st;
 }
 

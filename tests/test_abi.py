@@ -1,0 +1,48 @@
+"""The C ABI: libmcsim.so loads and exports every entry point include/mcsim.h declares,
+and the ctypes mirror matches the header's struct layouts (no GPU call is made)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+from mcsim import _abi, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mcsim.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mc_\w+)\(", txt, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    L = lib.load()
+    names = declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(lib.EXPORTS) == names
+
+
+def test_struct_layouts_match_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "mcsim.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(mc_tool), sizeof(mc_mode), sizeof(mc_material),
+         sizeof(mc_cell), sizeof(mc_trial), sizeof(mc_bisect_opts), sizeof(mc_limit), sizeof(mc_stamp));
+  printf("%zu %zu %zu\n", offsetof(mc_cell, S), offsetof(mc_cell, revs), offsetof(mc_limit, steps));
+  return 0;
+}'''
+    import tempfile
+    d = tempfile.mkdtemp()
+    cpath, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+    open(cpath, "w").write(src)
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), cpath, "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    want = [C.sizeof(t) for t in (_abi.Tool, _abi.Mode, _abi.Material, _abi.Cell, _abi.Trial,
+                                  _abi.BisectOpts, _abi.Limit, _abi.Stamp)]
+    want += [_abi.Cell.S.offset, _abi.Cell.revs.offset, _abi.Limit.steps.offset]
+    assert [int(x) for x in out] == want
