@@ -35,7 +35,26 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     for o in objs:
         os.remove(o)
+    build_addon(verbose)
     return LIB
+
+
+NODE_INCLUDE = "/usr/include/node"
+ADDON = os.path.join(HERE, "mcsim.node")
+
+
+def build_addon(verbose=False):
+    """N-API addon binding libmcsim.so for the JS host modules (skipped without Node headers)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        print("node headers not found: N-API addon not built", file=sys.stderr)
+        return None
+    cmd = ["g++", "-shared", "-fPIC", "-O2", "-std=c++17", "-DNODE_GYP_MODULE_NAME=mcsim",
+           "-I", NODE_INCLUDE, os.path.join(CSRC, "napi_addon.cc"), "-o", ADDON,
+           "-L", HERE, "-lmcsim", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    return ADDON
 
 
 if __name__ == "__main__":

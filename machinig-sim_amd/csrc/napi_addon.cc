@@ -1,0 +1,415 @@
+// napi_addon.cc — the thin N-API binding of libmcsim.so for the JavaScript host modules
+// in machinig-sim_amd/js (which keep the reference's exported signatures).
+//
+// Data crosses as ArrayBuffers laid out exactly like the C structs of include/mcsim.h
+// (the JS side packs mc_cell / mc_stamp records with a DataView), so the addon only
+// forwards pointers and maps a non-zero mc_status to a thrown Error carrying
+// mc_last_error().  bisectBatchAsync runs the search on a libuv worker thread and
+// returns a Promise, for the stability-sweep worker protocol.
+#define NAPI_VERSION 6
+#include <node_api.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mcsim.h"
+
+namespace {
+
+#define NAPI_OK(call)                                              \
+  do {                                                             \
+    if ((call) != napi_ok) {                                       \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call); \
+      return nullptr;                                              \
+    }                                                              \
+  } while (0)
+
+napi_value throw_mc(napi_env env, int st) {
+  std::string m = "libmcsim error " + std::to_string(st) + ": " + mc_last_error();
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+bool get_args(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok || argc < want) {
+    napi_throw_type_error(env, nullptr, "missing arguments");
+    return false;
+  }
+  return true;
+}
+
+// bytes of an ArrayBuffer or a TypedArray
+bool buffer_of(napi_env env, napi_value v, void** data, size_t* len) {
+  bool is_ab = false, is_ta = false;
+  napi_is_arraybuffer(env, v, &is_ab);
+  if (is_ab) return napi_get_arraybuffer_info(env, v, data, len) == napi_ok;
+  napi_is_typedarray(env, v, &is_ta);
+  if (is_ta) {
+    napi_typedarray_type t;
+    size_t n, off;
+    napi_value ab;
+    if (napi_get_typedarray_info(env, v, &t, &n, data, &ab, &off) != napi_ok) return false;
+    size_t es = (t == napi_float64_array || t == napi_bigint64_array || t == napi_biguint64_array) ? 8
+                : (t == napi_float32_array || t == napi_int32_array || t == napi_uint32_array) ? 4
+                : (t == napi_int16_array || t == napi_uint16_array) ? 2 : 1;
+    *len = n * es;
+    return true;
+  }
+  napi_throw_type_error(env, nullptr, "expected an ArrayBuffer or a typed array");
+  return false;
+}
+
+double num(napi_env env, napi_value v) {
+  double d = 0;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+napi_value new_ab(napi_env env, size_t bytes, void** data) {
+  napi_value ab;
+  if (napi_create_arraybuffer(env, bytes, data, &ab) != napi_ok) return nullptr;
+  return ab;
+}
+
+napi_value Init(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  int dev = 0;
+  size_t argc = 1;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  if (argc >= 1) dev = (int)num(env, argv[0]);
+  int st = mc_init(dev);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+napi_value DeviceInfo(napi_env env, napi_callback_info) {
+  char buf[256];
+  int st = mc_device_info(buf, sizeof buf);
+  if (st) return throw_mc(env, st);
+  napi_value s;
+  NAPI_OK(napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+napi_value LastKernelMs(napi_env env, napi_callback_info) {
+  double ms = 0;
+  int st = mc_last_kernel_time(&ms);
+  if (st) return throw_mc(env, st);
+  napi_value v;
+  NAPI_OK(napi_create_double(env, ms, &v));
+  return v;
+}
+
+// chatterBatch(cells: ArrayBuffer of n mc_cell, ap: Float64Array(n)) -> ArrayBuffer of mc_trial
+napi_value ChatterBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *cells, *ap;
+  size_t cb, ab;
+  if (!buffer_of(env, argv[0], &cells, &cb) || !buffer_of(env, argv[1], &ap, &ab)) return nullptr;
+  const int n = (int)(cb / sizeof(mc_cell));
+  if (ab / 8 < (size_t)n) { napi_throw_range_error(env, nullptr, "ap shorter than cells"); return nullptr; }
+  void* out;
+  napi_value res = new_ab(env, (size_t)n * sizeof(mc_trial) + 1, &out);
+  int st = mc_chatter_batch((const mc_cell*)cells, (const double*)ap, n, (mc_trial*)out);
+  if (st) return throw_mc(env, st);
+  return res;
+}
+
+// bisectBatch(cells, aMin, aMax, iters) -> ArrayBuffer of mc_limit
+napi_value BisectBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  void* cells;
+  size_t cb;
+  if (!buffer_of(env, argv[0], &cells, &cb)) return nullptr;
+  const int n = (int)(cb / sizeof(mc_cell));
+  mc_bisect_opts o{num(env, argv[1]), num(env, argv[2]), (int32_t)num(env, argv[3]), 0};
+  void* out;
+  napi_value res = new_ab(env, (size_t)n * sizeof(mc_limit) + 1, &out);
+  int st = mc_bisect_batch((const mc_cell*)cells, n, &o, (mc_limit*)out);
+  if (st) return throw_mc(env, st);
+  return res;
+}
+
+struct AsyncBisect {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  std::vector<mc_cell> cells;
+  std::vector<mc_limit> out;
+  mc_bisect_opts o{};
+  int st = 0;
+  std::string err;
+};
+
+void bisect_exec(napi_env, void* data) {
+  AsyncBisect* a = (AsyncBisect*)data;
+  a->st = mc_bisect_batch(a->cells.data(), (int)a->cells.size(), &a->o, a->out.data());
+  if (a->st) a->err = mc_last_error();
+}
+
+void bisect_done(napi_env env, napi_status, void* data) {
+  AsyncBisect* a = (AsyncBisect*)data;
+  if (a->st) {
+    napi_value msg, e;
+    std::string m = "libmcsim error " + std::to_string(a->st) + ": " + a->err;
+    napi_create_string_utf8(env, m.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &e);
+    napi_reject_deferred(env, a->deferred, e);
+  } else {
+    void* p;
+    napi_value ab = new_ab(env, a->out.size() * sizeof(mc_limit) + 1, &p);
+    std::memcpy(p, a->out.data(), a->out.size() * sizeof(mc_limit));
+    napi_resolve_deferred(env, a->deferred, ab);
+  }
+  napi_delete_async_work(env, a->work);
+  delete a;
+}
+
+// bisectBatchAsync(cells, aMin, aMax, iters) -> Promise<ArrayBuffer of mc_limit>
+napi_value BisectBatchAsync(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  void* cells;
+  size_t cb;
+  if (!buffer_of(env, argv[0], &cells, &cb)) return nullptr;
+  AsyncBisect* a = new AsyncBisect();
+  const size_t n = cb / sizeof(mc_cell);
+  a->cells.assign((const mc_cell*)cells, (const mc_cell*)cells + n);
+  a->out.resize(n);
+  a->o = mc_bisect_opts{num(env, argv[1]), num(env, argv[2]), (int32_t)num(env, argv[3]), 0};
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &a->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "mcsim.bisect", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, bisect_exec, bisect_done, a, &a->work));
+  NAPI_OK(napi_queue_async_work(env, a->work));
+  return promise;
+}
+
+// bracketBatch(cells, aMin, aMax, iters, lo: Float64Array, hi: Float64Array, tol)
+//   -> ArrayBuffer: n mc_limit followed by n int32 fell_back flags
+napi_value BracketBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return nullptr;
+  void *cells, *lo, *hi;
+  size_t cb, lb, hb;
+  if (!buffer_of(env, argv[0], &cells, &cb) || !buffer_of(env, argv[4], &lo, &lb) ||
+      !buffer_of(env, argv[5], &hi, &hb))
+    return nullptr;
+  const int n = (int)(cb / sizeof(mc_cell));
+  if (lb / 8 < (size_t)n || hb / 8 < (size_t)n) { napi_throw_range_error(env, nullptr, "hint arrays too short"); return nullptr; }
+  mc_bisect_opts o{num(env, argv[1]), num(env, argv[2]), (int32_t)num(env, argv[3]), 0};
+  void* out;
+  napi_value res = new_ab(env, (size_t)n * (sizeof(mc_limit) + 4) + 1, &out);
+  int st = mc_bisect_bracket_batch((const mc_cell*)cells, n, &o, (const double*)lo, (const double*)hi,
+                                   num(env, argv[6]), (mc_limit*)out,
+                                   (int32_t*)((char*)out + (size_t)n * sizeof(mc_limit)));
+  if (st) return throw_mc(env, st);
+  return res;
+}
+
+// engineTrace(cell: ArrayBuffer(1 mc_cell), ap, nsteps) -> Float64Array(4*steps): Fx, Fy, x, y blocks
+napi_value EngineTrace(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  void* cell;
+  size_t cb;
+  if (!buffer_of(env, argv[0], &cell, &cb)) return nullptr;
+  if (cb < sizeof(mc_cell)) { napi_throw_range_error(env, nullptr, "cell buffer too small"); return nullptr; }
+  const int64_t ns = (int64_t)num(env, argv[2]);
+  if (ns < 0) { napi_throw_range_error(env, nullptr, "negative step count"); return nullptr; }
+  std::vector<double> buf((size_t)(ns > 0 ? ns : 1) * 4);
+  int64_t got = 0;
+  int st = mc_engine_trace((const mc_cell*)cell, num(env, argv[1]), ns, buf.data(), buf.data() + ns,
+                           buf.data() + 2 * ns, buf.data() + 3 * ns, &got);
+  if (st) return throw_mc(env, st);
+  void* p;
+  napi_value ab = new_ab(env, (size_t)got * 4 * 8 + 1, &p);
+  for (int k = 0; k < 4; k++) std::memcpy((double*)p + k * got, buf.data() + k * ns, (size_t)got * 8);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)got * 4, ab, 0, &ta));
+  return ta;
+}
+
+// ---- stock: an External wrapping mc_stock*, freed by stockDestroy or the GC
+void stock_finalize(napi_env, void* data, void*) {
+  mc_stock** h = (mc_stock**)data;
+  if (*h) mc_stock_destroy(*h);
+  delete h;
+}
+
+mc_stock* stock_of(napi_env env, napi_value v) {
+  mc_stock** h = nullptr;
+  if (napi_get_value_external(env, v, (void**)&h) != napi_ok || !h || !*h) {
+    napi_throw_type_error(env, nullptr, "not a live stock handle");
+    return nullptr;
+  }
+  return *h;
+}
+
+napi_value StockCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  mc_stock* s = nullptr;
+  int st = mc_stock_create(num(env, argv[0]), num(env, argv[1]), num(env, argv[2]), num(env, argv[3]), &s);
+  if (st) return throw_mc(env, st);
+  mc_stock** h = new mc_stock*(s);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, h, stock_finalize, nullptr, &ext));
+  return ext;
+}
+
+napi_value StockDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_stock** h = nullptr;
+  if (napi_get_value_external(env, argv[0], (void**)&h) == napi_ok && h && *h) {
+    mc_stock_destroy(*h);
+    *h = nullptr;
+  }
+  return nullptr;
+}
+
+napi_value StockDims(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  int32_t nx, ny;
+  double hx, hy;
+  int st = mc_stock_dims(s, &nx, &ny, &hx, &hy);
+  if (st) return throw_mc(env, st);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  napi_create_int32(env, nx, &v); napi_set_named_property(env, o, "nx", v);
+  napi_create_int32(env, ny, &v); napi_set_named_property(env, o, "ny", v);
+  napi_create_double(env, hx, &v); napi_set_named_property(env, o, "hx", v);
+  napi_create_double(env, hy, &v); napi_set_named_property(env, o, "hy", v);
+  return o;
+}
+
+napi_value StockSetWallBand(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  int st = mc_stock_set_wall_band(s, num(env, argv[1]));
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// stockRead(handle, H, err, wallD, lineCut, lineWall): fills the given typed arrays (any may be null)
+napi_value StockRead(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int32_t nx, ny;
+  mc_stock_dims(s, &nx, &ny, nullptr, nullptr);
+  for (int k = 0; k < 5; k++) {
+    napi_valuetype t;
+    napi_typeof(env, argv[1 + k], &t);
+    if (t == napi_null || t == napi_undefined) continue;
+    size_t len;
+    if (!buffer_of(env, argv[1 + k], &p[k], &len)) return nullptr;
+    if (len < (size_t)nx * ny * 4) { napi_throw_range_error(env, nullptr, "array smaller than the grid"); return nullptr; }
+  }
+  int st = MC_OK;
+  if (p[0]) st = mc_stock_read_h(s, (float*)p[0]);
+  if (!st && (p[1] || p[2] || p[3] || p[4]))
+    st = mc_stock_read_side(s, (float*)p[1], (float*)p[2], (int32_t*)p[3], (int32_t*)p[4]);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+napi_value StockWriteH(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* p;
+  size_t len;
+  if (!buffer_of(env, argv[1], &p, &len)) return nullptr;
+  int32_t nx, ny;
+  mc_stock_dims(s, &nx, &ny, nullptr, nullptr);
+  if (len < (size_t)nx * ny * 4) { napi_throw_range_error(env, nullptr, "array smaller than the grid"); return nullptr; }
+  int st = mc_stock_write_h(s, (const float*)p);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// stampBatch(handle, stamps: ArrayBuffer of n mc_stamp) -> Float64Array(n) volumes
+napi_value StampBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* p;
+  size_t len;
+  if (!buffer_of(env, argv[1], &p, &len)) return nullptr;
+  const int n = (int)(len / sizeof(mc_stamp));
+  void* vol;
+  napi_value ab = new_ab(env, (size_t)n * 8 + 1, &vol);
+  int st = mc_stamp_batch(s, (const mc_stamp*)p, n, (double*)vol);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta));
+  return ta;
+}
+
+// stockSample(handle, xy: Float64Array(2n)) -> Float64Array(n)
+napi_value StockSample(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* p;
+  size_t len;
+  if (!buffer_of(env, argv[1], &p, &len)) return nullptr;
+  const int n = (int)(len / 16);
+  void* out;
+  napi_value ab = new_ab(env, (size_t)n * 8 + 1, &out);
+  int st = mc_stock_sample(s, (const double*)p, n, (double*)out);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta));
+  return ta;
+}
+
+napi_value Sizes(napi_env env, napi_callback_info) {
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  const struct { const char* k; size_t s; } t[] = {
+      {"mc_cell", sizeof(mc_cell)}, {"mc_trial", sizeof(mc_trial)}, {"mc_limit", sizeof(mc_limit)},
+      {"mc_stamp", sizeof(mc_stamp)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
+  for (auto& e : t) {
+    napi_create_uint32(env, (uint32_t)e.s, &v);
+    napi_set_named_property(env, o, e.k, v);
+  }
+  return o;
+}
+
+napi_value ModuleInit(napi_env env, napi_value exports) {
+  const struct { const char* name; napi_callback cb; } fns[] = {
+      {"init", Init}, {"deviceInfo", DeviceInfo}, {"lastKernelMs", LastKernelMs},
+      {"chatterBatch", ChatterBatch}, {"bisectBatch", BisectBatch},
+      {"bisectBatchAsync", BisectBatchAsync}, {"bracketBatch", BracketBatch},
+      {"engineTrace", EngineTrace}, {"stockCreate", StockCreate}, {"stockDestroy", StockDestroy},
+      {"stockDims", StockDims}, {"stockSetWallBand", StockSetWallBand}, {"stockRead", StockRead},
+      {"stockWriteH", StockWriteH}, {"stampBatch", StampBatch}, {"stockSample", StockSample},
+      {"sizes", Sizes}};
+  for (auto& f : fns) {
+    napi_value fn;
+    if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||
+        napi_set_named_property(env, exports, f.name, fn) != napi_ok)
+      return nullptr;
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, ModuleInit)

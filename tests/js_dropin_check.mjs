@@ -1,0 +1,87 @@
+// Drives the JavaScript drop-in modules (machinig-sim_amd/js) on the golden cases written
+// from the reference and checks they return the reference's answers.
+// usage: node js_dropin_check.mjs [lobes|gpu]
+import fs from 'fs';
+import path from 'path';
+import { fileURLToPath } from 'url';
+
+const here = path.dirname(fileURLToPath(import.meta.url));
+const mod = path.join(here, '..', 'machinig-sim_amd', 'js', 'index.js');
+const load = n => JSON.parse(fs.readFileSync(path.join(here, 'golden', 'golden_' + n + '.json'), 'utf8'));
+const f64 = s => { const b = Buffer.from(s, 'base64'); return new Float64Array(b.buffer, b.byteOffset, b.length / 8); };
+const f32 = s => { const b = Buffer.from(s, 'base64'); return new Float32Array(b.buffer.slice(b.byteOffset, b.byteOffset + b.length)); };
+const i32 = s => { const b = Buffer.from(s, 'base64'); return new Int32Array(b.buffer.slice(b.byteOffset, b.byteOffset + b.length)); };
+const num = v => (v === 'inf' ? Infinity : v === '-inf' ? -Infinity : v);
+let fails = 0;
+const check = (ok, what) => { if (!ok) { fails++; console.log('FAIL', what); } };
+const sameBits = (a, b) => a.length === b.length && a.every((v, i) => Object.is(v, b[i]));
+
+async function main() {
+  const m = await import(mod);
+  const what = process.argv[2] || 'gpu';
+  if (what === 'lobes') {
+    const g = load('lobes');
+    g.cases.forEach((c, k) => {
+      const got = m.stabilityLobes(c.params), ref = g.results[k];
+      check(got.length === ref.lobes.length, 'lobe count ' + k);
+      got.forEach((L, j) => {
+        check(sameBits(Float64Array.from(L.map(p => p.rpm)), f64(ref.lobes[j].rpm)), 'rpm ' + k + '/' + j);
+        check(sameBits(Float64Array.from(L.map(p => p.alim)), f64(ref.lobes[j].alim)), 'alim ' + k + '/' + j);
+      });
+      c.speeds.forEach((S, i) => check(Object.is(m.alimAt(got, S), num(ref.alimAt[i])), 'alimAt ' + k + '/' + i));
+      const d = m.directionalCoeffs(c.params.Krc / c.params.Ktc, c.params.phiSt, c.params.phiEx);
+      check(sameBits([d.xx, d.xy, d.yx, d.yy], ref.dc), 'dc ' + k);
+    });
+  } else {
+    console.log('device', m.native().deviceInfo());
+    const ch = load('chatter');
+    ch.cases.forEach((c, k) => {
+      const x = c.cell;
+      const v = m.chatterAt(x.tool, x.modal, x.mat, x.S, x.arc, x.fz, c.ap, { revs: x.revs, nonlinear: x.nonlinear });
+      check(v === (ch.results[k].chatter === 1), 'chatterAt ' + k);
+    });
+    const td = load('tdl');
+    const byOpts = td.cases.map(c => ({ tool: c.cell.tool, modal: c.cell.modal, mat: c.cell.mat, S: c.cell.S, arc: c.cell.arc, fz: c.cell.fz }));
+    // one launch per (revs, nonlinear) group, as timeDomainLimits takes shared options
+    const groups = {};
+    td.cases.forEach((c, k) => { const key = c.cell.revs + '|' + c.cell.nonlinear; (groups[key] = groups[key] || []).push(k); });
+    for (const key of Object.keys(groups)) {
+      const ks = groups[key], c0 = td.cases[ks[0]];
+      const res = m.timeDomainLimits(ks.map(k => byOpts[k]), Object.assign({}, c0.opts, { revs: c0.cell.revs, nonlinear: c0.cell.nonlinear }));
+      res.forEach((r, i) => {
+        const ref = td.results[ks[i]];
+        check(r.alim === ref.alim && r.bracket[0] === ref.lo && r.bracket[1] === num(ref.hi), 'timeDomainLimit ' + ks[i]);
+      });
+    }
+    const one = td.cases[0];
+    const r1 = m.timeDomainLimit(one.cell.tool, one.cell.modal, one.cell.mat, one.cell.S, one.cell.arc, one.cell.fz,
+      Object.assign({}, one.opts, { revs: one.cell.revs, nonlinear: one.cell.nonlinear }));
+    check(r1.alim === td.results[0].alim, 'timeDomainLimit single');
+    const ra = await m.timeDomainLimitsAsync([byOpts[0]], Object.assign({}, one.opts, { revs: one.cell.revs, nonlinear: one.cell.nonlinear }));
+    check(ra[0].alim === td.results[0].alim, 'timeDomainLimitsAsync');
+    const st = load('stock');
+    st.cases.forEach((c, k) => {
+      const s = new m.Stock(c.w, c.l, c.h, c.cell), ref = st.results[k];
+      if (c.wallBand) s.wallBand = c.wallBand;
+      const half = c.stamps.length >> 1;
+      const vol = [];
+      for (const p of c.stamps.slice(0, half)) { s.curLine = p.line; vol.push(s.stamp(p.cx, p.cy, p.tipZ, p.R, p.type, p.errVal)); }
+      vol.push(...s.stampBatch(c.stamps.slice(half)));
+      check(sameBits(s.H, f32(ref.H)), 'H ' + k);
+      check(sameBits(s.err, f32(ref.err)) && sameBits(s.wallD, f32(ref.wallD)), 'err/wallD ' + k);
+      check(sameBits(s.lineCut, i32(ref.lineCut)) && sameBits(s.lineWall, i32(ref.lineWall)), 'lines ' + k);
+      check(vol.every((v, i) => Math.abs(v - ref.vol[i]) <= 1e-12 * Math.max(1, Math.abs(ref.vol[i]))), 'vol ' + k);
+      check(Math.abs(s.removed - ref.removed) <= 1e-9 * Math.max(1, ref.removed), 'removed ' + k);
+      c.samples.forEach(([x, y], i) => check(Object.is(s.sample(x, y), num(ref.sample[i])), 'sample ' + k + '/' + i));
+      s.dispose();
+    });
+    const { handle } = await import(path.join(here, '..', 'machinig-sim_amd', 'js', 'stabworker.js'));
+    const msgs = [];
+    const c0 = td.cases[0].cell;
+    await handle({ key: 7, points: [{ S: c0.S, modal: c0.modal }], tool: c0.tool, mat: c0.mat, arc: c0.arc, fz: c0.fz, nonlinear: c0.nonlinear }, r => msgs.push(r));
+    check(msgs.length === 2 && msgs[1].done === true && msgs[0].key === 7, 'stabworker protocol');
+  }
+  console.log(fails ? fails + ' failures' : 'all ' + what + ' checks passed');
+  process.exit(fails ? 1 : 0);
+}
+main().catch(e => { console.error(e); process.exit(1); });
