@@ -19,6 +19,10 @@
 // chatter.hip
 int mcd_group_lds_bytes(int nv_cap, int nz_cap);
 int mcd_launch_chatter(const mcd::LaunchDev& P, int G, int blocks, hipStream_t stream);
+// search.hip
+int mcd_search_lds_bytes(int nv_cap, int nz_cap);
+int mcd_launch_search(const mcd::LaunchDev& P, const mcd::TaskDev* d_tasks, int blocks, hipStream_t stream);
+constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;
 // stock.hip
 size_t mcs_stampdev_bytes();
 size_t mcs_chunkbox_bytes();
@@ -267,6 +271,122 @@ int run_tasks(TaskSet& ts, std::vector<mcd::ResultDev>& res, double* h_trace, in
   return MC_OK;
 }
 
+
+// The speculative wave-per-cell search (search.hip): tasks are aMax checks
+// (TASK_SINGLE at aMax) and bisection trees (TASK_BISECT / TASK_BRACKET).
+int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
+  Ctx& X = g_ctx;
+  const int n = (int)ts.tasks.size();
+  res.assign(n, mcd::ResultDev{});
+  if (n == 0) return MC_OK;
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts.cost[a] > ts.cost[b]; });
+  std::vector<mcd::TaskDev> sorted(n);
+  int nz_cap = 1, nv_cap = 1;
+  int64_t ring_cap = 1;
+  for (int q = 0; q < n; q++) {
+    sorted[q] = ts.tasks[order[q]];
+    const mcd::CellDev& c = ts.cells[sorted[q].cell];
+    const int nzc = std::max(1, ts.nzcap[order[q]]);
+    nz_cap = std::max(nz_cap, nzc);
+    nv_cap = std::max(nv_cap, nzc * c.N);
+    ring_cap = std::max<int64_t>(ring_cap, (int64_t)nzc * c.spr);
+  }
+  const int wpc = std::max(1, env_int("MC_WAVES_PER_CU", 8));
+  const int waves = (int)std::min<int64_t>(n, (int64_t)X.cus * wpc);
+  const int lds = mcd_search_lds_bytes(nv_cap, nz_cap);
+  if (lds > 64 * 1024) return fail(MC_ERR_LIMIT, "search LDS %d B too large", lds);
+  std::lock_guard<std::mutex> lk(X.mu);
+  int st;
+  if ((st = X.cells.ensure(ts.cells.size() * sizeof(mcd::CellDev)))) return st;
+  if ((st = X.tasks.ensure(n * sizeof(mcd::TaskDev)))) return st;
+  if ((st = X.out.ensure(n * sizeof(mcd::ResultDev)))) return st;
+  if ((st = X.queue.ensure(256))) return st;
+  if ((st = X.ring.ensure((size_t)waves * SEARCH_MAXSLOT * ring_cap * sizeof(float)))) return st;
+  if ((st = X.hist.ensure((size_t)waves * SEARCH_MAXSLOT * mcd::HIST * sizeof(float2)))) return st;
+  HIPCHK(hipMemcpyAsync(X.cells.p, ts.cells.data(), ts.cells.size() * sizeof(mcd::CellDev), hipMemcpyHostToDevice, X.stream));
+  HIPCHK(hipMemcpyAsync(X.tasks.p, sorted.data(), n * sizeof(mcd::TaskDev), hipMemcpyHostToDevice, X.stream));
+  HIPCHK(hipMemsetAsync(X.queue.p, 0, sizeof(int32_t), X.stream));
+  mcd::LaunchDev P;
+  std::memset(&P, 0, sizeof P);
+  P.cells = (const mcd::CellDev*)X.cells.p;
+  P.tasks = (const mcd::TaskDev*)X.tasks.p;
+  P.out = (mcd::ResultDev*)X.out.p;
+  P.queue = (int32_t*)X.queue.p;
+  P.ring = (float*)X.ring.p;
+  P.hist = (float2*)X.hist.p;
+  P.ring_cap = ring_cap;
+  P.ntasks = n;
+  P.nz_cap = nz_cap;
+  P.nv_cap = nv_cap;
+  HIPCHK(hipEventRecord(X.ev0, X.stream));
+  if (mcd_launch_search(P, (const mcd::TaskDev*)X.tasks.p, waves, X.stream))
+    return fail(MC_ERR_HIP, "search kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+  HIPCHK(hipEventRecord(X.ev1, X.stream));
+  std::vector<mcd::ResultDev> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), X.out.p, n * sizeof(mcd::ResultDev), hipMemcpyDeviceToHost, X.stream));
+  HIPCHK(hipStreamSynchronize(X.stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
+  X.last_ms = ms;
+  for (int q = 0; q < n; q++) res[order[q]] = tmp[q];
+  return MC_OK;
+}
+
+bool use_speculative(const mc_cell* cells, int n, double aMax) {
+  const char* v = getenv("MC_SEARCH");
+  if (v && std::strcmp(v, "serial") == 0) return false;
+  for (int i = 0; i < n; i++)
+    if (cells[i].tool.N * nz_bound(cells[i], aMax) > SEARCH_MAXP * 64) return false;
+  return true;
+}
+
+// timeDomainLimit for the cells listed in idx: an aMax-check task and a bisection-tree
+// task per cell, combined into the reference's result
+int plain_search(const mc_cell* cells, const TaskSet& base, const std::vector<int>& idx,
+                 const mc_bisect_opts* o, std::vector<mc_limit>& out) {
+  TaskSet ts;
+  ts.cells = base.cells;
+  const int m = (int)idx.size();
+  for (int q = 0; q < m; q++) {
+    const int i = idx[q];
+    const mcd::CellDev& c = ts.cells[i];
+    const double steps = (double)c.spt * c.N * c.revs;
+    mcd::TaskDev a;
+    std::memset(&a, 0, sizeof a);
+    a.cell = i; a.mode = mcd::TASK_SINGLE; a.ap = o->aMax;
+    ts.tasks.push_back(a);
+    ts.nzcap.push_back(nz_bound(cells[i], o->aMax));
+    ts.cost.push_back(steps * 1.0);
+    mcd::TaskDev t;
+    std::memset(&t, 0, sizeof t);
+    t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = o->iters; t.lo = o->aMin; t.hi = o->aMax; t.tol = -1;
+    t.aMin = o->aMin; t.aMax = o->aMax;
+    ts.tasks.push_back(t);
+    ts.nzcap.push_back(nz_bound(cells[i], o->aMax));
+    ts.cost.push_back(steps * 1.5);
+  }
+  std::vector<mcd::ResultDev> res;
+  int st = run_search(ts, res);
+  if (st) return st;
+  out.resize(m);
+  for (int q = 0; q < m; q++) {
+    const mcd::ResultDev& A = res[2 * q];
+    const mcd::ResultDev& B = res[2 * q + 1];
+    mc_limit& l = out[q];
+    std::memset(&l, 0, sizeof l);
+    if (!A.chatter) {
+      l.alim = o->aMax; l.lo = o->aMax; l.hi = INFINITY; l.open = 1; l.trials = 1;
+      l.steps = A.steps; l.visits = A.visits; l.engaged = A.engaged;
+    } else {
+      l.alim = B.alim; l.lo = B.lo; l.hi = B.hi; l.open = 0; l.trials = 1 + B.trials;
+      l.steps = A.steps + B.steps; l.visits = A.visits + B.visits; l.engaged = A.engaged + B.engaged;
+    }
+  }
+  return MC_OK;
+}
+
 double cell_cost(const mcd::CellDev& c) { return (double)c.spt * c.N * c.revs * c.N; }
 
 int prepare(const mc_cell* cells, int n, TaskSet& ts) {
@@ -358,6 +478,14 @@ int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_
   if (n > 0 && !out) return fail(MC_ERR_ARG, "null output array");
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
+  if (use_speculative(cells, n, opts->aMax)) {
+    std::vector<int> all(n);
+    std::iota(all.begin(), all.end(), 0);
+    std::vector<mc_limit> lim;
+    if ((st = plain_search(cells, ts, all, opts, lim))) return st;
+    for (int i = 0; i < n; i++) out[i] = lim[i];
+    return MC_OK;
+  }
   ts.tasks.resize(n);
   ts.cost.resize(n);
   ts.nzcap.resize(n);
@@ -383,6 +511,49 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
   if (n > 0 && (!out || !lo_hint || !hi_hint)) return fail(MC_ERR_ARG, "null hint or output array");
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
+  if (use_speculative(cells, n, opts->aMax)) {
+    // hinted trees; cells whose hint fails (or is invalid) rerun the plain search
+    TaskSet hs;
+    hs.cells = ts.cells;
+    std::vector<int> hinted, plain;
+    for (int i = 0; i < n; i++) {
+      const bool ok = lo_hint[i] > 0 && hi_hint[i] > lo_hint[i] && hi_hint[i] <= opts->aMax &&
+                      lo_hint[i] >= opts->aMin;
+      if (!ok) { plain.push_back(i); continue; }
+      mcd::TaskDev t;
+      std::memset(&t, 0, sizeof t);
+      t.cell = i; t.mode = mcd::TASK_BRACKET; t.iters = opts->iters; t.lo = lo_hint[i]; t.hi = hi_hint[i];
+      t.tol = tol_mm; t.aMin = opts->aMin; t.aMax = opts->aMax;
+      hs.tasks.push_back(t);
+      hs.nzcap.push_back(nz_bound(cells[i], hi_hint[i]));
+      hs.cost.push_back((double)ts.cells[i].spt * ts.cells[i].N * ts.cells[i].revs);
+      hinted.push_back(i);
+    }
+    std::vector<mcd::ResultDev> hr;
+    if ((st = run_search(hs, hr))) return st;
+    std::vector<mcd::ResultDev> spent(n);
+    for (size_t q = 0; q < hinted.size(); q++) {
+      const int i = hinted[q];
+      if (hr[q].fell_back) { plain.push_back(i); spent[i] = hr[q]; continue; }
+      mc_limit& l = out[i];
+      std::memset(&l, 0, sizeof l);
+      l.alim = hr[q].alim; l.lo = hr[q].lo; l.hi = hr[q].hi; l.open = 0; l.trials = hr[q].trials;
+      l.steps = hr[q].steps; l.visits = hr[q].visits; l.engaged = hr[q].engaged;
+      if (fell_back) fell_back[i] = 0;
+    }
+    std::sort(plain.begin(), plain.end());
+    std::vector<mc_limit> pl;
+    if (!plain.empty() && (st = plain_search(cells, ts, plain, opts, pl))) return st;
+    for (size_t q = 0; q < plain.size(); q++) {
+      const int i = plain[q];
+      mc_limit l = pl[q];
+      l.trials += spent[i].trials; l.steps += spent[i].steps; l.visits += spent[i].visits;
+      l.engaged += spent[i].engaged;
+      out[i] = l;
+      if (fell_back) fell_back[i] = 1;
+    }
+    return MC_OK;
+  }
   ts.tasks.resize(n);
   ts.cost.resize(n);
   ts.nzcap.resize(n);

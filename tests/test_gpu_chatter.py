@@ -85,3 +85,17 @@ def test_bracket_search_matches_oracle(mc, orc):
         r, f = orc.bracket_limit(c, lo[i], hi[i], 0.05, **opts)
         assert (out[i].alim, out[i].lo, out[i].hi, out[i].trials, int(fb[i])) == \
                (r.alim, r.lo, r.hi, r.trials, f), i
+
+
+def test_speculative_search_equals_serial_search(mc, orc, monkeypatch):
+    """the speculative wave-per-cell search and the one-trial-at-a-time search agree
+    exactly (depths, brackets, serial-path trial/step/visit counts)"""
+    from mcsim import workloads as W
+    cells, opts = W.sld_grid(n_speeds=12, n_tools=10, seed=7)
+    spec = mc.bisect_batch(cells, **opts)
+    monkeypatch.setenv("MC_SEARCH", "serial")
+    ser = mc.bisect_batch(cells, **opts)
+    for i in range(len(cells)):
+        a, b = spec[i], ser[i]
+        assert (a.alim, a.lo, a.hi, a.open, a.trials, a.steps, a.visits, a.engaged) == \
+               (b.alim, b.lo, b.hi, b.open, b.trials, b.steps, b.visits, b.engaged), i
