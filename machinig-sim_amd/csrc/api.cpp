@@ -22,6 +22,7 @@ int mcd_launch_chatter(const mcd::LaunchDev& P, int G, int blocks, hipStream_t s
 // search.hip
 int mcd_search_lds_bytes(int nv_cap, int nz_cap);
 int mcd_launch_search(const mcd::LaunchDev& P, const mcd::TaskDev* d_tasks, int blocks, hipStream_t stream);
+int mcd_search_maxspec();
 constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;
 // stock.hip
 size_t mcs_stampdev_bytes();
@@ -80,7 +81,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
-  DevBuf cells, tasks, out, queue, ring, hist, trace;
+  DevBuf cells, tasks, out, queue, ring, hist, trace, geo, gbin;
   std::mutex mu;
 };
 
@@ -284,7 +285,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts.cost[a] > ts.cost[b]; });
   std::vector<mcd::TaskDev> sorted(n);
   int nz_cap = 1, nv_cap = 1;
-  int64_t ring_cap = 1;
+  int64_t ring_cap = 1, pe_cap = 1;
   for (int q = 0; q < n; q++) {
     sorted[q] = ts.tasks[order[q]];
     const mcd::CellDev& c = ts.cells[sorted[q].cell];
@@ -292,7 +293,9 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
     nz_cap = std::max(nz_cap, nzc);
     nv_cap = std::max(nv_cap, nzc * c.N);
     ring_cap = std::max<int64_t>(ring_cap, (int64_t)nzc * c.spr);
+    pe_cap = std::max<int64_t>(pe_cap, (int64_t)c.spr * c.N);
   }
+  const int64_t geo_cap = pe_cap * (nz_cap + mcd_search_maxspec());
   const int wpc = std::max(1, env_int("MC_WAVES_PER_CU", 8));
   const int waves = (int)std::min<int64_t>(n, (int64_t)X.cus * wpc);
   const int lds = mcd_search_lds_bytes(nv_cap, nz_cap);
@@ -305,11 +308,16 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   if ((st = X.queue.ensure(256))) return st;
   if ((st = X.ring.ensure((size_t)waves * SEARCH_MAXSLOT * ring_cap * sizeof(float)))) return st;
   if ((st = X.hist.ensure((size_t)waves * SEARCH_MAXSLOT * mcd::HIST * sizeof(float2)))) return st;
+  if ((st = X.geo.ensure((size_t)waves * geo_cap * sizeof(double2)))) return st;
+  if ((st = X.gbin.ensure((size_t)waves * geo_cap))) return st;
   HIPCHK(hipMemcpyAsync(X.cells.p, ts.cells.data(), ts.cells.size() * sizeof(mcd::CellDev), hipMemcpyHostToDevice, X.stream));
   HIPCHK(hipMemcpyAsync(X.tasks.p, sorted.data(), n * sizeof(mcd::TaskDev), hipMemcpyHostToDevice, X.stream));
   HIPCHK(hipMemsetAsync(X.queue.p, 0, sizeof(int32_t), X.stream));
   mcd::LaunchDev P;
   std::memset(&P, 0, sizeof P);
+  P.geo = (double2*)X.geo.p;
+  P.gbin = (unsigned char*)X.gbin.p;
+  P.geo_cap = geo_cap;
   P.cells = (const mcd::CellDev*)X.cells.p;
   P.tasks = (const mcd::TaskDev*)X.tasks.p;
   P.out = (mcd::ResultDev*)X.out.p;
@@ -331,6 +339,15 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
   X.last_ms = ms;
   for (int q = 0; q < n; q++) res[order[q]] = tmp[q];
+  if (getenv("MC_TASK_TIMES")) {
+    FILE* f = fopen(getenv("MC_TASK_TIMES"), "w");
+    if (f) {
+      for (int q = 0; q < n; q++)
+        fprintf(f, "%d %d %lld %lld %d %d\n", ts.tasks[order[q]].cell, ts.tasks[order[q]].mode,
+                (long long)tmp[q].t_start, (long long)tmp[q].t_end, tmp[q].trials, tmp[q].pad);
+      fclose(f);
+    }
+  }
   return MC_OK;
 }
 
@@ -340,6 +357,13 @@ bool use_speculative(const mc_cell* cells, int n, double aMax) {
   for (int i = 0; i < n; i++)
     if (cells[i].tool.N * nz_bound(cells[i], aMax) > SEARCH_MAXP * 64) return false;
   return true;
+}
+
+double cell_cost(const mcd::CellDev& c) { return (double)c.spt * c.N * c.revs * c.N; }
+
+void to_limit(const mcd::ResultDev& r, mc_limit& l) {
+  l.alim = r.alim; l.lo = r.lo; l.hi = r.hi; l.open = r.open; l.trials = r.trials;
+  l.steps = r.steps; l.visits = r.visits; l.engaged = r.engaged;
 }
 
 // timeDomainLimit for the cells listed in idx: an aMax-check task and a bisection-tree
@@ -371,7 +395,31 @@ int plain_search(const mc_cell* cells, const TaskSet& base, const std::vector<in
   int st = run_search(ts, res);
   if (st) return st;
   out.resize(m);
+  // cells whose rounds needed more geometry-table columns than the search kernel holds
+  // (fell_back == 2) run the whole search on the one-trial-per-group kernel
+  TaskSet rs;
+  rs.cells = base.cells;
+  std::vector<int> redo;
   for (int q = 0; q < m; q++) {
+    if (res[2 * q].fell_back != 2 && res[2 * q + 1].fell_back != 2) continue;
+    const int i = idx[q];
+    mcd::TaskDev t;
+    std::memset(&t, 0, sizeof t);
+    t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = o->iters; t.aMin = o->aMin; t.aMax = o->aMax;
+    rs.tasks.push_back(t);
+    rs.nzcap.push_back(nz_bound(cells[i], o->aMax));
+    rs.cost.push_back(cell_cost(rs.cells[i]));
+    redo.push_back(q);
+  }
+  std::vector<mcd::ResultDev> rr;
+  if (!redo.empty() && (st = run_tasks(rs, rr, nullptr, 0))) return st;
+  std::vector<char> done(m, 0);
+  for (size_t r = 0; r < redo.size(); r++) {
+    to_limit(rr[r], out[redo[r]]);
+    done[redo[r]] = 1;
+  }
+  for (int q = 0; q < m; q++) {
+    if (done[q]) continue;
     const mcd::ResultDev& A = res[2 * q];
     const mcd::ResultDev& B = res[2 * q + 1];
     mc_limit& l = out[q];
@@ -386,8 +434,6 @@ int plain_search(const mc_cell* cells, const TaskSet& base, const std::vector<in
   }
   return MC_OK;
 }
-
-double cell_cost(const mcd::CellDev& c) { return (double)c.spt * c.N * c.revs * c.N; }
 
 int prepare(const mc_cell* cells, int n, TaskSet& ts) {
   if (n < 0 || (n > 0 && !cells)) return fail(MC_ERR_ARG, "null cells or negative count");
@@ -405,11 +451,6 @@ int check_opts(const mc_bisect_opts* o) {
   if (!(o->aMin > 0) || !(o->aMax >= o->aMin) || !std::isfinite(o->aMax) || o->iters < 0 || o->iters > 64)
     return fail(MC_ERR_ARG, "need 0 < aMin <= aMax, 0 <= iters <= 64");
   return MC_OK;
-}
-
-void to_limit(const mcd::ResultDev& r, mc_limit& l) {
-  l.alim = r.alim; l.lo = r.lo; l.hi = r.hi; l.open = r.open; l.trials = r.trials;
-  l.steps = r.steps; l.visits = r.visits; l.engaged = r.engaged;
 }
 
 // ---- stock

@@ -43,6 +43,7 @@ struct ResultDev {
   double index, amp, once;  // last trial's chatter index
   int32_t open, trials, chatter, nz, fell_back, pad;
   int64_t steps, visits, engaged;
+  int64_t t_start, t_end;   // device wall clock (100 MHz) around the task
 };
 
 struct LaunchDev {
@@ -59,6 +60,9 @@ struct LaunchDev {
   int32_t nz_cap;        // slice capacity of the per-group LDS tables
   int32_t nv_cap;        // (tooth, slice) visit capacity of the per-group LDS tables
   int32_t grp_bytes;     // LDS bytes per group
+  double2* geo;          // edge geometry table (sin, cos), geo_cap entries per wave
+  unsigned char* gbin;   // engagement-map bin of each table entry
+  int64_t geo_cap;
 };
 
 }  // namespace mcd

@@ -44,13 +44,6 @@ __host__ __device__ inline int rem_pio2(double x, double& y0, double& y1) {
   constexpr double pio2_2t = 2.02226624879595063154e-21;  // 0x3BA3198A 2E037073
   constexpr double pio2_3 = 2.02226624871116645580e-21;   // 0x3BA3198A 2E000000
   constexpr double pio2_3t = 8.47842766036889956997e-32;  // 0x397B839A 252049C1
-  // high words of n*pi/2, n = 1..32: the cancellation check runs only near them
-  constexpr int32_t npio2_hw[32] = {
-      0x3FF921FB, 0x400921FB, 0x4012D97C, 0x401921FB, 0x401F6A7A, 0x4022D97C, 0x4025FDBB,
-      0x402921FB, 0x402C463A, 0x402F6A7A, 0x4031475C, 0x4032D97C, 0x40346B9C, 0x4035FDBB,
-      0x40378FDB, 0x403921FB, 0x403AB41B, 0x403C463A, 0x403DD85A, 0x403F6A7A, 0x40407E4C,
-      0x4041475C, 0x4042106C, 0x4042D97C, 0x4043A28C, 0x40446B9C, 0x404534AC, 0x4045FDBB,
-      0x4046C6CB, 0x40478FDB, 0x404858EB, 0x404921FB};
   const int32_t hx = hiw(x), ix = hx & 0x7fffffff;
   if (ix <= 0x3fe921fb) { y0 = x; y1 = 0; return 0; }
   if (ix < 0x4002d97c) {  // |x| < 3pi/4: n = +-1
@@ -72,7 +65,10 @@ __host__ __device__ inline int rem_pio2(double x, double& y0, double& y1) {
   const double fn = (double)n;
   double r = t - fn * pio2_1;
   double w = fn * pio2_1t;
-  if (n < 32 && ix != npio2_hw[n - 1]) {
+  // fdlibm's table npio2_hw[n-1] holds the high words of n*pi/2 (n = 1..32), which
+  // equal hiw(n * fl(pi/2)) for every n; computing them keeps the device code free of
+  // memory loads on this path.  The cancellation check runs only near those multiples.
+  if (n < 32 && ix != hiw((double)n * 1.5707963267948966)) {
     y0 = r - w;
   } else {
     const int j = ix >> 20;
@@ -203,9 +199,9 @@ __host__ __device__ inline double tan_js(double x) {
 // pow(x, y) for the size-effect law (h0/h)^mc: x > 0 finite, y finite (e_pow.c; the
 // special-value prologue keeps only the branches reachable with such arguments)
 __host__ __device__ inline double pow_js(double x, double y) {
-  constexpr double bp[2] = {1.0, 1.5};
-  constexpr double dp_h[2] = {0.0, 5.84962487220764160156e-01};
-  constexpr double dp_l[2] = {0.0, 1.35003920212974897128e-08};
+  // bp = {1, 1.5}, dp_h = {0, log2(1.5) high}, dp_l = {0, its low part}; selected by
+  // value rather than indexed, so no table load reaches the device code
+  constexpr double dp_h1 = 5.84962487220764160156e-01, dp_l1 = 1.35003920212974897128e-08;
   constexpr double two53 = 9007199254740992.0;
   constexpr double L1 = 5.99999999999994648725e-01, L2 = 4.28571428578550184252e-01,
                    L3 = 3.33333329818377432918e-01, L4 = 2.72728123808534006489e-01,
@@ -266,12 +262,13 @@ __host__ __device__ inline double pow_js(double x, double y) {
     else if (j < 0xBB67A) k = 1;
     else { k = 0; n += 1; ix -= 0x00100000; }
     ax = words((uint32_t)ix, low(ax));
-    u = ax - bp[k];
-    v = 1.0 / (ax + bp[k]);
+    const double bpk = k ? 1.5 : 1.0, dphk = k ? dp_h1 : 0.0, dplk = k ? dp_l1 : 0.0;
+    u = ax - bpk;
+    v = 1.0 / (ax + bpk);
     ss = u * v;
     s_h = clear_lo(ss);
     t_h = words((uint32_t)(((ix >> 1) | 0x20000000) + 0x00080000 + (k << 18)), 0);
-    t_l = ax - (t_h - bp[k]);
+    t_l = ax - (t_h - bpk);
     s_l = v * ((u - s_h * t_h) - s_h * t_l);
     s2 = ss * ss;
     r = s2 * s2 * (L1 + s2 * (L2 + s2 * (L3 + s2 * (L4 + s2 * (L5 + s2 * L6)))));
@@ -284,10 +281,10 @@ __host__ __device__ inline double pow_js(double x, double y) {
     p_h = clear_lo(u + v);
     p_l = v - (p_h - u);
     z_h = cp_h * p_h;
-    z_l = cp_l * p_h + p_l * cp + dp_l[k];
+    z_l = cp_l * p_h + p_l * cp + dplk;
     t = (double)n;
-    t1 = clear_lo(((z_h + z_l) + dp_h[k]) + t);
-    t2 = z_l - (((t1 - t) - dp_h[k]) - z_h);
+    t1 = clear_lo(((z_h + z_l) + dphk) + t);
+    t2 = z_l - (((t1 - t) - dphk) - z_h);
   }
   y1 = clear_lo(y);
   p_l = (y - y1) * t1 + y * t2;

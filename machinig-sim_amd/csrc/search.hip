@@ -4,21 +4,27 @@
 // geometrically: every trial is a full chatterAt simulation (src/cutting.js:146-318) of
 // spt*N*revs steps whose next depth depends on the previous verdict.  Here one wavefront
 // owns one cell and, per ROUND, evaluates the next L levels of the bisection tree at once
-// (1, 3 or 7 trials, one per lane SLOT of 64, 16 or 8 lanes): it then walks the tree with
+// (1, 3 or 7 trials, one per lane SLOT of 64, 16 or 8 lanes); it then walks the tree with
 // the verdicts, which reproduces the serial bisection exactly (same depths, same
-// brackets), in ceil(levels / L) rounds instead of one trial per level.  Every trial of a
-// round has the same length (same cell), so the slots advance in lockstep, and all cell
-// constants are wave-uniform.  The aMax check is a separate task of its own wave, run
-// concurrently; the host combines the two (stable at aMax -> open result).
+// brackets) in ceil(levels / L) rounds.  All trials of a round have the same length (same
+// cell), so the slots advance in lockstep, and all cell constants are wave-uniform.  The
+// aMax check is a separate task of its own wave, run concurrently; the host combines the
+// two (stable at aMax -> open result).
 //
-// The slot size is chosen per round from the trial depths so that the N*nz (tooth,
-// slice) visits of every slot fit in at most MAXP passes of its lanes.  Per step the
-// lanes of a slot split the visits (engagement, chip thickness, forces, delay line),
-// leave their force terms in LDS, then every lane sums its slot's terms serially in the
-// reference's (tooth, slice) order (bit-identical FP64) and advances that slot's modal
-// state.  The delay-line values a step reads were written a tooth period earlier and are
-// prefetched a step ahead; the edge angles, sines and cosines of step s+1 (independent of
-// the vibration) are computed while step s's forces are evaluated.
+// Per step the lanes of a slot split the N*nz (tooth, slice) visits (at most MAXP passes;
+// the slot size is chosen per round from the trial depths), leave their force terms in
+// LDS, then every lane sums its slot's terms serially in the reference's (tooth, slice)
+// order (bit-identical FP64).  The modes of a slot are integrated in parallel, one lane
+// per mode (one division for all modes), and summed in mode order.
+//
+// Edge geometry table: the edge angle of tooth i on slice k at step pr,
+// phi = (pr*dphi + i*pitch - lag_k) mod 2pi, depends only on (pr, i, lag_k) and repeats
+// every revolution, so its sine, cosine and engagement-map bin are tabulated once per cell
+// (full slices) and once per round (the partial top slice of each trial) in a per-wave
+// global table; the step loop reads it (prefetched a step ahead, like the delay line)
+// instead of evaluating fdlibm sin/cos per visit.  A round needing more distinct lags than
+// the table holds (possible only when the depth changes the slice pitch) aborts the task
+// with fell_back = 2, and the host runs that cell on the one-trial-per-group kernel.
 #include <hip/hip_runtime.h>
 
 #include "mc_dev.h"
@@ -29,20 +35,19 @@ using namespace mcd;
 namespace {
 
 constexpr int MAXP = 2;        // passes of a slot's lanes per step
-constexpr int MAXSLOT = 8;     // slots of a wave (8 lanes each at most 8 slots)
+constexpr int MAXSLOT = 8;     // slots of a wave
+constexpr int MAXSPEC = 16;    // per-round table columns for partial / non-standard slices
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+__device__ __forceinline__ void global_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double unid(double v) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
 
 __device__ __forceinline__ double js_min(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double js_max(double a, double b) { return a > b ? a : b; }
@@ -55,51 +60,73 @@ __device__ __forceinline__ int slices_at(double toolFL, double ap, double& dz) {
   return ap <= 0 ? 0 : (int)js_min((double)maxSlices, __builtin_ceil(ap / dz));
 }
 
+// the cell constants the step loop reads, staged in LDS
+struct CellCst {
+  double fz, dphi, pitch, binScale, hMax, dt, Ktc, Krc, Kte, Kre, mc;
+  double dR[MC_MAX_FLUTES];
+};
+
+struct WaveLds {
+  CellCst K;
+  double mq[MAXSLOT][2 * MC_MAX_MODES], mv[MAXSLOT][2 * MC_MAX_MODES];
+  double node_ap[MAXSLOT];
+  double idx[MAXSLOT], amp[MAXSLOT], once[MAXSLOT];
+  double spec_lag[MAXSPEC];
+  long long eng[MAXSLOT];
+  int nz[MAXSLOT];
+  float Hb[MC_BINS];
+  int task, nspec, overflow;
+};
+
 struct SlotLds {
-  double2* term;   // [nv_cap]
-  double* sLag;    // [nz_cap]
+  double2* term;   // SG*MAXP force terms
+  double* sLag;    // [nz_cap] per slice
   double* sZ;
   double* sB;
   double* sKap;
   double* sR;
+  int* lagid;      // table column of each slice, -1: evaluate directly
   float* tsx;      // [TS]
   float* tsy;
 };
 
-struct WaveLds {
-  double2 geo[MAXP][64];            // next step's (sin, cos) of each lane's visits
-  int gbin[MAXP][64];               // next step's engagement-map bin
-  double mq[MAXSLOT][2 * MC_MAX_MODES], mv[MAXSLOT][2 * MC_MAX_MODES];  // modal q, v per slot
-  double node_ap[MAXSLOT], node_lo[MAXSLOT], node_hi[MAXSLOT];
-  double idx[MAXSLOT], amp[MAXSLOT], once[MAXSLOT];
-  long long eng[MAXSLOT];
-  int nz[MAXSLOT];
-  float Hb[MC_BINS];
-  int task;
-};
+__device__ __forceinline__ size_t wave_bytes() { return (sizeof(WaveLds) + 15) & ~(size_t)15; }
+__device__ __host__ __forceinline__ size_t slot_bytes_of(int nz_cap) {
+  return (8 * 5 * (size_t)nz_cap + 4 * (size_t)nz_cap + 4 * 2 * TS + 15) & ~(size_t)15;
+}
+
+// one table entry: sin, cos and bin of the edge angle at (pr, tooth i, lag)
+__device__ __forceinline__ void edge_geometry(double theta, double ipitch, double lag, double binScale,
+                                              double& sn, double& cs, int& bin) {
+  double phi = mcm::fmod_2pi((theta + ipitch) - lag);
+  if (phi < 0) phi += mcm::TWO_PI;
+  mcm::sincos_js(phi, sn, cs);
+  bin = (int)(phi * binScale);
+}
 
 }  // namespace
 
-// LDS of a wave: the per-wave block, 64*MAXP force terms (a slot of SG lanes owns
-// SG*MAXP of them), then per slot nz_cap slice records and its tooth-sample ring
+// LDS of a wave: the per-wave block, 64*MAXP force terms (a slot of SG lanes owns SG*MAXP
+// of them), then per slot its slice records, table columns and tooth-sample ring
 int mcd_search_lds_bytes(int nv_cap, int nz_cap) {
   (void)nv_cap;
   const size_t wave = (sizeof(WaveLds) + 15) & ~(size_t)15;
-  const size_t slot = (8 * 5 * (size_t)nz_cap + 4 * 2 * TS + 15) & ~(size_t)15;
-  return (int)(wave + 16 * 64 * MAXP + MAXSLOT * slot);
+  return (int)(wave + 16 * 64 * MAXP + MAXSLOT * slot_bytes_of(nz_cap));
 }
+int mcd_search_maxspec() { return MAXSPEC; }
 
 __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __restrict__ tasks) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   WaveLds& W = *reinterpret_cast<WaveLds*>(lds);
-  const size_t wave_bytes = (sizeof(WaveLds) + 15) & ~(size_t)15;
-  const size_t slot_bytes = (8 * 5 * (size_t)P.nz_cap + 4 * 2 * TS + 15) & ~(size_t)15;
-  double2* const terms = reinterpret_cast<double2*>(lds + wave_bytes);
+  const size_t slot_bytes = slot_bytes_of(P.nz_cap);
+  double2* const terms = reinterpret_cast<double2*>(lds + wave_bytes());
   const int lane = threadIdx.x;
   const int64_t wid = blockIdx.x;
   float* ring_base = P.ring + wid * (int64_t)MAXSLOT * P.ring_cap;
   float2* hist_base = P.hist + wid * (int64_t)MAXSLOT * HIST;
-  const double TWO_PI = mcm::TWO_PI;
+  double2* const gsc = P.geo + wid * P.geo_cap;
+  unsigned char* const gbn = P.gbin + wid * P.geo_cap;
+  const int NL = P.nz_cap + MAXSPEC;   // table columns: full slices, then per-round ones
   const double ux = 1.0, uy = 0.0, vx = -0.0, vy = 1.0;  // chatterAt's feed direction
 
   while (true) {
@@ -107,34 +134,45 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
     wave_sync();
     const int task = uni(W.task);
     if (task >= P.ntasks) break;
+    const long long t_start = wall_clock64();
     const TaskDev T = tasks[task];
     const int cell = uni(T.cell);
     const CellDev& C = P.cells[cell];
-    const int N = C.N, spt = C.spt, spr = C.spr, decim = C.decim, nmx = C.nmx, nmy = C.nmy;
-    const double fz = C.fz, dphi = C.dphi, pitch = C.pitch, binScale = C.binScale, hMax = C.hMax;
-    const double dt = C.dt, Ktc = C.Ktc, Krc = C.Krc, Kte = C.Kte, Kre = C.Kre, mcx = C.mc;
+    const int N = C.N, spt = C.spt, spr = C.spr, decim = C.decim;
     const long long steps = (long long)spt * N * C.revs;
     for (int j = lane; j < MC_BINS; j += 64) W.Hb[j] = C.Hb[j];
+    if (lane == 0) {
+      CellCst& K = W.K;
+      K.fz = C.fz; K.dphi = C.dphi; K.pitch = C.pitch; K.binScale = C.binScale; K.hMax = C.hMax;
+      K.dt = C.dt; K.Ktc = C.Ktc; K.Krc = C.Krc; K.Kte = C.Kte; K.Kre = C.Kre; K.mc = C.mc;
+      for (int j = 0; j < MC_MAX_FLUTES; j++) K.dR[j] = C.dR[j];
+    }
+    wave_sync();
+    const CellCst& K = W.K;
+    // standard slice pitch of this cell (depths with ap + 1 <= flute length)
+    const double dz0 = js_max(0.25, C.toolFluteLen / 48);
+    int kf_built = 0;  // full-slice table columns built for this task
 
     // ---- bisection state (wave-uniform)
-    const bool amax_task = T.mode == TASK_SINGLE;   // a single trial at T.ap (the aMax check)
+    const bool amax_task = T.mode == TASK_SINGLE;  // a single trial at T.ap (the aMax check)
     const bool check_ends = T.mode == TASK_BRACKET;
     const int iters = T.iters;
     const double tol = T.tol;
     double lo = T.lo, hi = T.hi;
     int it = 0;
     bool pend_lo = check_ends, pend_hi = check_ends, failed = false, last_chat = false;
+    bool tab_overflow = false;  // more distinct slice lags than table columns: host reruns the cell
     long long p_trials = 0, p_steps = 0, p_visits = 0, p_eng = 0;  // along the serial path
     long long x_trials = 0;                                        // executed
     double last_index = 0, last_amp = 0, last_once = 0;
     int last_nz = 0;
 
     while (true) {
-      // ================= choose this round's trials and the slot size
+      // ================= this round's trials and slot size (computed identically by all lanes)
       int nnode = 0, L = 0, SG = 64;
       bool ev_lo = false, ev_hi = false;
       if (amax_task) {
-        W.node_ap[0] = T.ap;  // uniform write, identical in every lane
+        W.node_ap[0] = T.ap;
         nnode = 1;
       } else {
         if (!pend_lo && !pend_hi && !(it < iters && hi - lo > tol)) break;
@@ -145,11 +183,10 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         for (int mode = 0; mode < 3 && nnode == 0; mode++) {
           const bool use_lo = pend_lo, use_hi = pend_hi && (mode < 2 || !pend_lo);
           if (mode == 2 && !(pend_lo && pend_hi)) break;
-          const bool lo_only = mode == 2;  // the two ends one round each
+          const bool lo_only = mode == 2;
           const int ne = lo_only ? 1 : (int)use_lo + (int)use_hi;
-          const bool tree_ok = !lo_only && (use_lo == pend_lo) && (use_hi == pend_hi);
-          for (int cand = (tree_ok ? 3 : 0); cand >= 0; cand--) {
-            if (mode == 1 && cand > 0) continue;  // second try: ends alone
+          for (int cand = (lo_only ? 0 : 3); cand >= 0; cand--) {
+            if (mode == 1 && cand > 0) continue;
             const int tree = (1 << cand) - 1;
             const int need = ne + tree;
             if (need == 0) continue;
@@ -157,8 +194,8 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
             while (slots < need) slots <<= 1;
             if (slots > MAXSLOT) continue;
             const int sg = 64 / slots;
-            // node intervals in breadth-first order: children of j are 2j+1 (chatter at
-            // the midpoint: bracket [lo, m]) and 2j+2 (stable: [m, hi])
+            // node intervals in breadth-first order: children of j are 2j+1 (chatter at the
+            // midpoint: bracket [lo, m]) and 2j+2 (stable: [m, hi])
             double nl[MAXSLOT], nh[MAXSLOT], na[MAXSLOT];
             nl[0] = lo; nh[0] = hi;
             for (int j = 0; j < tree; j++) {
@@ -173,7 +210,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
             int q = 0;
             if (use_lo) { W.node_ap[q++] = lo; ev_lo = true; }
             if (use_hi && !lo_only) { W.node_ap[q++] = hi; ev_hi = true; }
-            for (int j = 0; j < tree; j++) { W.node_ap[q + j] = na[j]; W.node_lo[q + j] = nl[j]; W.node_hi[q + j] = nh[j]; }
+            for (int j = 0; j < tree; j++) W.node_ap[q + j] = na[j];
             break;
           }
         }
@@ -183,22 +220,22 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       const int nslot = nnode;
       const int slot = lane / SG, g = lane % SG;
       const bool in_slot = slot < nslot;
+      const int sl = in_slot ? slot : 0;
       SlotLds S;
       {
-        const int sl = in_slot ? slot : 0;
         S.term = terms + sl * SG * MAXP;
-        unsigned char* sb = lds + wave_bytes + 16 * 64 * MAXP + (size_t)sl * slot_bytes;
-        double* d = reinterpret_cast<double*>(sb);
+        double* d = reinterpret_cast<double*>(lds + wave_bytes() + 16 * 64 * MAXP + (size_t)sl * slot_bytes);
         S.sLag = d; d += P.nz_cap; S.sZ = d; d += P.nz_cap; S.sB = d; d += P.nz_cap;
         S.sKap = d; d += P.nz_cap; S.sR = d; d += P.nz_cap;
-        S.tsx = reinterpret_cast<float*>(d); S.tsy = S.tsx + TS;
+        S.lagid = reinterpret_cast<int*>(d);
+        S.tsx = reinterpret_cast<float*>(S.lagid + P.nz_cap); S.tsy = S.tsx + TS;
       }
-      float* ring = ring_base + (int64_t)(in_slot ? slot : 0) * P.ring_cap;
-      float2* hist = hist_base + (in_slot ? slot : 0) * HIST;
+      float* ring = ring_base + (int64_t)sl * P.ring_cap;
+      float2* hist = hist_base + sl * HIST;
 
       // ---- fresh engine per slot at its depth (configure + setEngagement)
       const double ap = in_slot ? W.node_ap[slot] : 0.0;
-      double dz = 0.25;
+      double dz = dz0;
       const int nz = in_slot ? slices_at(C.toolFluteLen, ap, dz) : 0;
       const int NV = N * nz;
       const double db = dz;
@@ -217,11 +254,77 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       }
       for (int j = g; j < nz * spr; j += SG) ring[j] = 0.0f;
       if (g == 0 && in_slot) W.eng[slot] = 0;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      __builtin_amdgcn_wave_barrier();
+      wave_sync();
 
-      // per-pass visit of this lane: tooth i, slice k
-      int vi[MAXP], vk[MAXP];
+      // ---- table columns: full slices k carry lag(k*dz0 + dz0/2); the other slices get
+      // per-round columns (deduplicated bit for bit) or, past MAXSPEC, direct evaluation
+      int kneed = 0;
+      for (int s2 = 0; s2 < nslot; s2++) {
+        double dd = dz0;
+        kneed = max(kneed, slices_at(C.toolFluteLen, W.node_ap[s2], dd));
+      }
+      kneed = min(kneed, P.nz_cap);
+      if (lane == 0) {
+        int ns = 0;
+        for (int s2 = 0; s2 < nslot; s2++) {
+          double* d = reinterpret_cast<double*>(lds + wave_bytes() + 16 * 64 * MAXP + (size_t)s2 * slot_bytes);
+          const double* lag2 = d;
+          int* id2 = reinterpret_cast<int*>(d + 5 * P.nz_cap);
+          double dd = dz0;
+          const int nz2 = slices_at(C.toolFluteLen, W.node_ap[s2], dd);
+          for (int k = 0; k < nz2; k++) {
+            const double full = (k * dz0 + 0.5 * dz0) * C.tanB / C.R;
+            int id = -1;
+            if (__builtin_bit_cast(uint64_t, lag2[k]) == __builtin_bit_cast(uint64_t, full)) {
+              id = k;
+            } else {
+              for (int q = 0; q < ns; q++)
+                if (__builtin_bit_cast(uint64_t, W.spec_lag[q]) == __builtin_bit_cast(uint64_t, lag2[k])) id = P.nz_cap + q;
+              if (id < 0 && ns < MAXSPEC) { W.spec_lag[ns] = lag2[k]; id = P.nz_cap + ns; ns++; }
+            }
+            id2[k] = id;
+          }
+        }
+        W.nspec = ns;
+        W.overflow = 0;
+        for (int s2 = 0; s2 < nslot; s2++) {
+          double* d = reinterpret_cast<double*>(lds + wave_bytes() + 16 * 64 * MAXP + (size_t)s2 * slot_bytes);
+          const int* id2 = reinterpret_cast<int*>(d + 5 * P.nz_cap);
+          double dd = dz0;
+          const int nz2 = slices_at(C.toolFluteLen, W.node_ap[s2], dd);
+          for (int k = 0; k < nz2; k++) if (id2[k] < 0) W.overflow = 1;
+        }
+      }
+      wave_sync();
+      const int nspec = uni(W.nspec);
+      if (uni(W.overflow)) { failed = true; tab_overflow = true; break; }
+      {
+        // extend the full-slice columns, then (re)build this round's columns
+        const int kf_new = max(kf_built, kneed);
+        const long long pe = (long long)spr * N;
+        for (int col = 0; col < 2; col++) {
+          const int c0 = col == 0 ? kf_built : P.nz_cap, c1 = col == 0 ? kf_new : P.nz_cap + nspec;
+          const int nc = c1 - c0;
+          if (nc <= 0) continue;
+          for (long long e = lane; e < pe * nc; e += 64) {
+            const int c = c0 + (int)(e % nc);
+            const long long pi_ = e / nc;
+            const int pr = (int)(pi_ / N), i = (int)(pi_ % N);
+            const double lag = col == 0 ? (c * dz0 + 0.5 * dz0) * C.tanB / C.R : W.spec_lag[c - P.nz_cap];
+            double sn, cs;
+            int bin;
+            edge_geometry((double)pr * K.dphi, (double)i * K.pitch, lag, K.binScale, sn, cs, bin);
+            const long long at = pi_ * NL + c;
+            gsc[at] = make_double2(sn, cs);
+            gbn[at] = (unsigned char)(bin < 255 ? bin : 255);
+          }
+        }
+        kf_built = kf_new;
+      }
+      global_sync();
+
+      // per-pass visit of this lane: tooth i, slice k, table column
+      int vi[MAXP], vk[MAXP], vc[MAXP];
       bool vok[MAXP];
 #pragma unroll
       for (int p = 0; p < MAXP; p++) {
@@ -229,60 +332,77 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         vok[p] = in_slot && v < NV;
         vi[p] = vok[p] ? v / nz : 0;
         vk[p] = vok[p] ? v - vi[p] * nz : 0;
+        vc[p] = vok[p] ? S.lagid[vk[p]] : 0;
       }
 
-      // modal state of each slot in LDS; the kick vx = 0.01 enters the first mode
-      if (g == 0 && in_slot) {
-#pragma unroll
-        for (int j = 0; j < 2 * MC_MAX_MODES; j++) { W.mq[slot][j] = 0; W.mv[slot][j] = 0; }
-        W.mv[slot][0] = 0.0 + (0.01 - 0.0);
-      }
+      // modes: lane g < MC_MAX_MODES integrates x-mode g, lane MC_MAX_MODES + j y-mode j;
+      // the kick vx = 0.01 enters the first x mode (src/stabtd.js:19, cutting.js:164)
+      const bool xmode = in_slot && g < C.nmx;
+      const bool ymode = in_slot && g >= MC_MAX_MODES && g - MC_MAX_MODES < C.nmy;
+      const int mj = g < MC_MAX_MODES ? g : g - MC_MAX_MODES;
+      double mk = 1, mm = 1, mcd_ = 0, mq = 0, mv = 0;
+      if (xmode) { mk = C.kx[mj]; mm = C.mx[mj]; mcd_ = C.cx[mj]; }
+      if (ymode) { mk = C.ky[mj]; mm = C.my[mj]; mcd_ = C.cy[mj]; }
+      if (xmode && mj == 0) mv = 0.0 + (0.01 - 0.0);
+      const int nmx = C.nmx, nmy = C.nmy;
       double x = 0, y = 0, vxs = 0.01, vys = 0;
       int pr = 0, ph = 0, dc = 0, ti = 0, tcount = 0, hpos = 0, hcount = 0;
       long long my_eng = 0;
       float cur[MAXP];
+      double2 csc[MAXP];
+      int cbin[MAXP];
 #pragma unroll
       for (int p = 0; p < MAXP; p++) {
         cur[p] = 0.0f;
+        csc[p] = make_double2(0.0, 1.0);
+        cbin[p] = 0;
         if (vok[p]) {
-          double phi = mcm::fmod_2pi((0.0 * dphi + (double)vi[p] * pitch) - S.sLag[vk[p]]);
-          if (phi < 0) phi += TWO_PI;
-          double sn, cs;
-          mcm::sincos_js(phi, sn, cs);
-          W.geo[p][lane] = make_double2(sn, cs);
-          W.gbin[p][lane] = (int)(phi * binScale);
+          const int i = vi[p];
+          if (vc[p] >= 0) {
+            const long long at = (long long)i * NL + vc[p];
+            csc[p] = gsc[at];
+            cbin[p] = gbn[at];
+          }
         }
       }
-      wave_sync();
 
       for (long long s = 0; s < steps; s++) {
         const double xm = x * 1e3, ym = y * 1e3;  // rigid workpiece
         const double rvx = vxs, rvy = vys;
         const int prn = pr + 1 == spr ? 0 : pr + 1;
-        const double thn = (double)prn * dphi;
+        // prefetch the next step's delay-line values and edge geometry
         float nxt[MAXP];
+        double2 nsc[MAXP];
+        int nbin[MAXP];
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
           nxt[p] = 0.0f;
+          nsc[p] = make_double2(0.0, 1.0);
+          nbin[p] = 0;
           if (vok[p]) {
-            int pi = prn + vi[p] * spt;
+            const int i = vi[p];
+            int pi = prn + i * spt;
             if (pi >= spr) pi -= spr;
             nxt[p] = ring[pi * nz + vk[p]];
+            if (vc[p] >= 0) {
+              const long long at = ((long long)prn * N + i) * NL + vc[p];
+              nsc[p] = gsc[at];
+              nbin[p] = gbn[at];
+            }
           }
         }
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
           if (vok[p]) {
             const int i = vi[p], k = vk[p];
-            const double2 gg = W.geo[p][lane];
-            const double sn = gg.x, cs = gg.y;
+            const double sn = csc[p].x, cs = csc[p].y;
             const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-            const double rn = xm * nxg + ym * nyg + C.dR[i];
+            const double rn = xm * nxg + ym * nyg + K.dR[i];
             int pi = pr + i * spt;
             if (pi >= spr) pi -= spr;
             const double prev = (double)cur[p];
-            const double feedH = fz * sn * S.sKap[k];
-            const int bin = W.gbin[p][lane];
+            const double feedH = K.fz * sn * S.sKap[k];
+            const int bin = cbin[p];
             const bool below = bin < MC_BINS ? ((double)W.Hb[bin] <= S.sZ[k]) : false;
             double tX = 0, tY = 0;
             float store;
@@ -295,11 +415,12 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
               } else {
                 store = (float)rn;
                 my_eng++;
-                if (h > hMax) h = hMax;
+                if (h > K.hMax) h = K.hMax;
+                const double mcx = K.mc;
                 const double se = (mcx != 0 && mcx == mcx) ? mcm::pow_js(0.05 / (h > 0.002 ? h : 0.002), mcx) : 1.0;
                 const double bk = S.sB[k];
-                const double Ft = Ktc * se * h * bk + Kte * bk;
-                double Fr = Krc * se * h * bk + Kre * bk;
+                const double Ft = K.Ktc * se * h * bk + K.Kte * bk;
+                double Fr = K.Krc * se * h * bk + K.Kre * bk;
                 if (pdK != 0 && pdK == pdK) Fr += pdK * (bk / db) / S.sR[k] * (rvx * nxg + rvy * nyg);
                 const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
                 tX = -Ft * txg - Fr * nxg;
@@ -308,13 +429,6 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
             }
             ring[pi * nz + k] = store;
             S.term[g + p * SG] = make_double2(tX, tY);
-            // geometry of the next step (independent of the vibration)
-            double phi = mcm::fmod_2pi((thn + (double)i * pitch) - S.sLag[k]);
-            if (phi < 0) phi += TWO_PI;
-            double nsn, ncs;
-            mcm::sincos_js(phi, nsn, ncs);
-            W.geo[p][lane] = make_double2(nsn, ncs);
-            W.gbin[p][lane] = (int)(phi * binScale);
           }
         }
         wave_sync();
@@ -324,33 +438,20 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           Fx += t.x;
           Fy += t.y;
         }
+        // modal integration (semi-implicit Euler, src/cutting.js:247-255): each mode lane
+        // advances its mode, then every lane sums the slot's modes in order
+        if (xmode || ymode) {
+          const double F = xmode ? Fx : Fy;
+          mv += (F - mcd_ * mv - mk * mq) / mm * K.dt;
+          mq += mv * K.dt;
+          W.mq[sl][g] = mq;
+          W.mv[sl][g] = mv;
+        }
         wave_sync();
         x = 0; y = 0; vxs = 0; vys = 0;
         if (in_slot) {
-          double* mq = W.mq[slot];
-          double* mv = W.mv[slot];
-#pragma unroll
-          for (int j = 0; j < MC_MAX_MODES; j++) {
-            if (j < nmx) {
-              double v = mv[j], q = mq[j];
-              v += (Fx - C.cx[j] * v - C.kx[j] * q) / C.mx[j] * dt;
-              q += v * dt;
-              x += q;
-              vxs += v;
-              if (g == 0) { mv[j] = v; mq[j] = q; }
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < MC_MAX_MODES; j++) {
-            if (j < nmy) {
-              double v = mv[MC_MAX_MODES + j], q = mq[MC_MAX_MODES + j];
-              v += (Fy - C.cy[j] * v - C.ky[j] * q) / C.my[j] * dt;
-              q += v * dt;
-              y += q;
-              vys += v;
-              if (g == 0) { mv[MC_MAX_MODES + j] = v; mq[MC_MAX_MODES + j] = q; }
-            }
-          }
+          for (int j = 0; j < nmx; j++) { x += W.mq[sl][j]; vxs += W.mv[sl][j]; }
+          for (int j = 0; j < nmy; j++) { y += W.mq[sl][MC_MAX_MODES + j]; vys += W.mv[sl][MC_MAX_MODES + j]; }
         }
         pr = prn;
         ph = ph + 1 == spt ? 0 : ph + 1;
@@ -366,7 +467,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           hcount++;
         }
 #pragma unroll
-        for (int p = 0; p < MAXP; p++) cur[p] = nxt[p];
+        for (int p = 0; p < MAXP; p++) { cur[p] = nxt[p]; csc[p] = nsc[p]; cbin[p] = nbin[p]; }
         wave_sync();
       }
 
@@ -389,14 +490,14 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           const int m = (double)hcount < r ? hcount : (int)r;
           double ax = 0, ay = 0, w = 0;
           for (int k = 0; k < m; k++) { const float2 h = hist[(hpos - 1 - k) & (HIST - 1)]; ax += h.x; ay += h.y; }
-          const double mm = m ? (double)m : 1.0;
-          ax /= mm; ay /= mm;
+          const double mm2 = m ? (double)m : 1.0;
+          ax /= mm2; ay /= mm2;
           for (int k = 0; k < m; k++) {
             const float2 h = hist[(hpos - 1 - k) & (HIST - 1)];
             const double dx = h.x - ax, dy = h.y - ay;
             w += dx * dx + dy * dy;
           }
-          amp = __builtin_sqrt(w / mm);
+          amp = __builtin_sqrt(w / mm2);
           index = once / (amp + 0.05);
         }
         W.idx[slot] = index; W.amp[slot] = amp; W.once[slot] = once; W.nz[slot] = nz;
@@ -446,8 +547,9 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       R.lo = lo; R.hi = hi;
       R.index = last_index; R.amp = last_amp; R.once = last_once;
       R.open = 0; R.trials = (int)p_trials; R.chatter = last_chat; R.nz = last_nz;
-      R.fell_back = failed; R.pad = (int)x_trials;
+      R.fell_back = tab_overflow ? 2 : (int)failed; R.pad = (int)x_trials;
       R.steps = p_steps; R.visits = p_visits; R.engaged = p_eng;
+      R.t_start = t_start; R.t_end = wall_clock64();
     }
     wave_sync();
   }
