@@ -323,9 +323,13 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       }
       global_sync();
 
-      // per-pass visit of this lane: tooth i, slice k, table column
+      // per-pass visit of this lane: tooth i, slice k, table column.  Lanes without a
+      // visit keep in-bounds indices (tooth 0, slice 0, column 0) so that every lane issues
+      // the same loads each step: the delay-line and table prefetch then stays in flight
+      // across the step instead of being waited for at once.
       int vi[MAXP], vk[MAXP], vc[MAXP];
       bool vok[MAXP];
+      double vKap[MAXP], vZ[MAXP], vB[MAXP], vPd[MAXP], vdR[MAXP];
 #pragma unroll
       for (int p = 0; p < MAXP; p++) {
         const int v = g + p * SG;
@@ -333,7 +337,20 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         vi[p] = vok[p] ? v / nz : 0;
         vk[p] = vok[p] ? v - vi[p] * nz : 0;
         vc[p] = vok[p] ? S.lagid[vk[p]] : 0;
+        if (vc[p] < 0) vc[p] = 0;
+        vKap[p] = vok[p] ? S.sKap[vk[p]] : 1.0;
+        vZ[p] = vok[p] ? S.sZ[vk[p]] : 0.0;
+        vB[p] = vok[p] ? S.sB[vk[p]] : 0.0;
+        // process-damping factor of the slice: pdK * (b / db) / r
+        vPd[p] = vok[p] ? pdK * (vB[p] / db) / S.sR[vk[p]] : 0.0;
+        vdR[p] = K.dR[vi[p]];
       }
+      const double fz = K.fz, hMax = K.hMax, Ktc = K.Ktc, Krc = K.Krc, Kte = K.Kte, Kre = K.Kre;
+      const double mcx = K.mc, dt = K.dt;
+      const bool use_se = mcx != 0 && mcx == mcx;
+      const bool use_pd = pdK != 0 && pdK == pdK;
+      // lanes of this lane's slot in a wave-wide ballot
+      const unsigned long long smask = SG == 64 ? ~0ull : (((1ull << SG) - 1) << (sl * SG));
 
       // modes: lane g < MC_MAX_MODES integrates x-mode g, lane MC_MAX_MODES + j y-mode j;
       // the kick vx = 0.01 enters the first x mode (src/stabtd.js:19, cutting.js:164)
@@ -351,60 +368,48 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       float cur[MAXP];
       double2 csc[MAXP];
       int cbin[MAXP];
+      unsigned rowN = (unsigned)N * (unsigned)NL;  // table entries per step row
 #pragma unroll
       for (int p = 0; p < MAXP; p++) {
         cur[p] = 0.0f;
-        csc[p] = make_double2(0.0, 1.0);
-        cbin[p] = 0;
-        if (vok[p]) {
-          const int i = vi[p];
-          if (vc[p] >= 0) {
-            const long long at = (long long)i * NL + vc[p];
-            csc[p] = gsc[at];
-            cbin[p] = gbn[at];
-          }
-        }
+        const unsigned at = (unsigned)vi[p] * (unsigned)NL + (unsigned)vc[p];
+        csc[p] = gsc[at];
+        cbin[p] = gbn[at];
       }
 
       for (long long s = 0; s < steps; s++) {
         const double xm = x * 1e3, ym = y * 1e3;  // rigid workpiece
         const double rvx = vxs, rvy = vys;
         const int prn = pr + 1 == spr ? 0 : pr + 1;
-        // prefetch the next step's delay-line values and edge geometry
+        // prefetch the next step's delay-line values and edge geometry (all lanes)
         float nxt[MAXP];
         double2 nsc[MAXP];
         int nbin[MAXP];
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
-          nxt[p] = 0.0f;
-          nsc[p] = make_double2(0.0, 1.0);
-          nbin[p] = 0;
-          if (vok[p]) {
-            const int i = vi[p];
-            int pi = prn + i * spt;
-            if (pi >= spr) pi -= spr;
-            nxt[p] = ring[pi * nz + vk[p]];
-            if (vc[p] >= 0) {
-              const long long at = ((long long)prn * N + i) * NL + vc[p];
-              nsc[p] = gsc[at];
-              nbin[p] = gbn[at];
-            }
-          }
+          int pi = prn + vi[p] * spt;
+          if (pi >= spr) pi -= spr;
+          nxt[p] = ring[pi * nz + vk[p]];
+          const unsigned at = (unsigned)prn * rowN + (unsigned)vi[p] * (unsigned)NL + (unsigned)vc[p];
+          nsc[p] = gsc[at];
+          nbin[p] = gbn[at];
         }
+        int ne = 0;  // engaged visits of this slot so far, in (tooth, slice) order
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
+          bool eng = false;
+          double tX = 0, tY = 0;
           if (vok[p]) {
             const int i = vi[p], k = vk[p];
             const double sn = csc[p].x, cs = csc[p].y;
             const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-            const double rn = xm * nxg + ym * nyg + K.dR[i];
+            const double rn = xm * nxg + ym * nyg + vdR[p];
             int pi = pr + i * spt;
             if (pi >= spr) pi -= spr;
             const double prev = (double)cur[p];
-            const double feedH = K.fz * sn * S.sKap[k];
+            const double feedH = fz * sn * vKap[p];
             const int bin = cbin[p];
-            const bool below = bin < MC_BINS ? ((double)W.Hb[bin] <= S.sZ[k]) : false;
-            double tX = 0, tY = 0;
+            const bool below = bin < MC_BINS ? ((double)W.Hb[bin] <= vZ[p]) : false;
             float store;
             if (below || sn <= 0) {
               store = (float)rn;
@@ -414,36 +419,46 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
                 store = (float)(prev - feedH);
               } else {
                 store = (float)rn;
-                my_eng++;
-                if (h > K.hMax) h = K.hMax;
-                const double mcx = K.mc;
-                const double se = (mcx != 0 && mcx == mcx) ? mcm::pow_js(0.05 / (h > 0.002 ? h : 0.002), mcx) : 1.0;
-                const double bk = S.sB[k];
-                const double Ft = K.Ktc * se * h * bk + K.Kte * bk;
-                double Fr = K.Krc * se * h * bk + K.Kre * bk;
-                if (pdK != 0 && pdK == pdK) Fr += pdK * (bk / db) / S.sR[k] * (rvx * nxg + rvy * nyg);
+                eng = true;
+                if (h > hMax) h = hMax;
+                const double se = use_se ? mcm::pow_js(0.05 / (h > 0.002 ? h : 0.002), mcx) : 1.0;
+                const double bk = vB[p];
+                const double Ft = Ktc * se * h * bk + Kte * bk;
+                double Fr = Krc * se * h * bk + Kre * bk;
+                if (use_pd) Fr += vPd[p] * (rvx * nxg + rvy * nyg);
                 const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
                 tX = -Ft * txg - Fr * nxg;
                 tY = -Ft * tyg - Fr * nyg;
               }
             }
             ring[pi * nz + k] = store;
-            S.term[g + p * SG] = make_double2(tX, tY);
           }
+          // the reference sums the engaged visits only: compact their terms in order
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(eng) & smask;
+          const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+          if (eng) S.term[ne + before] = make_double2(tX, tY);
+          ne += __builtin_popcountll(m);
         }
+        my_eng += (g == 0) ? ne : 0;
+        // pad to a multiple of 4 with +0 terms: the running sum starts at +0 and can never
+        // be -0 under round-to-nearest, so adding +0 leaves it unchanged
+        const int ne4 = (ne + 3) & ~3;
+        if (in_slot && g < ne4 - ne) S.term[ne + g] = make_double2(0.0, 0.0);
         wave_sync();
         double Fx = 0, Fy = 0;
-        for (int v = 0; v < NV; v++) {
-          const double2 t = S.term[v];
-          Fx += t.x;
-          Fy += t.y;
+        for (int v = 0; v < ne4; v += 4) {
+          const double2 t0 = S.term[v], t1 = S.term[v + 1], t2 = S.term[v + 2], t3 = S.term[v + 3];
+          Fx += t0.x; Fy += t0.y;
+          Fx += t1.x; Fy += t1.y;
+          Fx += t2.x; Fy += t2.y;
+          Fx += t3.x; Fy += t3.y;
         }
         // modal integration (semi-implicit Euler, src/cutting.js:247-255): each mode lane
         // advances its mode, then every lane sums the slot's modes in order
         if (xmode || ymode) {
           const double F = xmode ? Fx : Fy;
-          mv += (F - mcd_ * mv - mk * mq) / mm * K.dt;
-          mq += mv * K.dt;
+          mv += (F - mcd_ * mv - mk * mq) / mm * dt;
+          mq += mv * dt;
           W.mq[sl][g] = mq;
           W.mv[sl][g] = mv;
         }
@@ -472,7 +487,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       }
 
       // ---- chatter index per slot (src/cutting.js:299-318)
-      if (in_slot) atomicAdd(reinterpret_cast<unsigned long long*>(&W.eng[slot]), (unsigned long long)my_eng);
+      if (in_slot && g == 0) W.eng[slot] = my_eng;
       if (g == 0 && in_slot) {
         double index = 0, amp = 0, once = 0;
         const int n = tcount < 24 ? tcount : 24;
