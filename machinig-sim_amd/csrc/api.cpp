@@ -130,6 +130,16 @@ int check_cell(const mc_cell& c, int i) {
     if (!(c.mx[j].k > 0) || !(c.mx[j].fn > 0)) return fail(MC_ERR_ARG, "cell %d: mode x%d needs k > 0, fn > 0", i, j);
   for (int j = 0; j < c.nmy; j++)
     if (!(c.my[j].k > 0) || !(c.my[j].fn > 0)) return fail(MC_ERR_ARG, "cell %d: mode y%d needs k > 0, fn > 0", i, j);
+  // steps of one trial, spt * N * revs, as make_celldev derives them: the kernels count
+  // them in 32-bit integers
+  const double S = js_max(100, c.S);
+  double fTop = 0;
+  for (int j = 0; j < c.nmx; j++) fTop = js_max(fTop, c.mx[j].fn);
+  for (int j = 0; j < c.nmy; j++) fTop = js_max(fTop, c.my[j].fn);
+  const double spt = js_max(48, std::ceil(60 / (S * c.tool.N) * fTop * 24));
+  if (!(spt * c.tool.N * (double)c.revs <= 2147483647.0) || !(spt * c.tool.N <= 16777216.0))
+    return fail(MC_ERR_LIMIT, "cell %d: %.0f steps per trial (%.0f per revolution) too many", i,
+                spt * c.tool.N * (double)c.revs, spt * c.tool.N);
   return MC_OK;
 }
 
@@ -267,7 +277,7 @@ int run_tasks(TaskSet& ts, std::vector<mcd::ResultDev>& res, double* h_trace, in
   HIPCHK(hipStreamSynchronize(X.stream));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
-  X.last_ms = ms;
+  X.last_ms += ms;
   for (int q = 0; q < n; q++) res[order[q]] = tmp[q];
   return MC_OK;
 }
@@ -296,6 +306,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
     pe_cap = std::max<int64_t>(pe_cap, (int64_t)c.spr * c.N);
   }
   const int64_t geo_cap = pe_cap * (nz_cap + mcd_search_maxspec());
+  if (geo_cap >= ((int64_t)1 << 32)) return fail(MC_ERR_LIMIT, "edge-geometry table of %lld entries too large", (long long)geo_cap);
   const int wpc = std::max(1, env_int("MC_WAVES_PER_CU", 8));
   const int waves = (int)std::min<int64_t>(n, (int64_t)X.cus * wpc);
   const int lds = mcd_search_lds_bytes(nv_cap, nz_cap);
@@ -337,7 +348,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   HIPCHK(hipStreamSynchronize(X.stream));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
-  X.last_ms = ms;
+  X.last_ms += ms;
   for (int q = 0; q < n; q++) res[order[q]] = tmp[q];
   if (getenv("MC_TASK_TIMES")) {
     FILE* f = fopen(getenv("MC_TASK_TIMES"), "w");
@@ -351,12 +362,20 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   return MC_OK;
 }
 
-bool use_speculative(const mc_cell* cells, int n, double aMax) {
+// Cells the speculative search kernel takes: at most SEARCH_MAXP visits per lane at the
+// deepest trial, and a size-effect exponent inside pow_cell's domain.  The others (and all
+// cells when MC_SEARCH=serial) run on the one-trial-per-group kernel.
+bool spec_cell(const mc_cell& c, const mcd::CellDev& d, double aMax) {
   const char* v = getenv("MC_SEARCH");
   if (v && std::strcmp(v, "serial") == 0) return false;
-  for (int i = 0; i < n; i++)
-    if (cells[i].tool.N * nz_bound(cells[i], aMax) > SEARCH_MAXP * 64) return false;
-  return true;
+  if (c.tool.N * nz_bound(c, aMax) > SEARCH_MAXP * 64) return false;
+  const bool use_se = d.mc != 0 && d.mc == d.mc;
+  return !use_se || (mcm::pow_cell_ok(d.mc) && d.hMax <= 1e27);
+}
+
+void split_cells(const mc_cell* cells, const TaskSet& ts, int n, double aMax, std::vector<int>& spec,
+                 std::vector<int>& serial) {
+  for (int i = 0; i < n; i++) (spec_cell(cells[i], ts.cells[i], aMax) ? spec : serial).push_back(i);
 }
 
 double cell_cost(const mcd::CellDev& c) { return (double)c.spt * c.N * c.revs * c.N; }
@@ -492,6 +511,7 @@ int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* ou
   if (n > 0 && (!ap || !out)) return fail(MC_ERR_ARG, "null depth or output array");
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
+  g_ctx.last_ms = 0;
   ts.tasks.resize(n);
   ts.cost.resize(n);
   ts.nzcap.resize(n);
@@ -519,27 +539,28 @@ int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_
   if (n > 0 && !out) return fail(MC_ERR_ARG, "null output array");
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
-  if (use_speculative(cells, n, opts->aMax)) {
-    std::vector<int> all(n);
-    std::iota(all.begin(), all.end(), 0);
+  g_ctx.last_ms = 0;
+  std::vector<int> spec, serial;
+  split_cells(cells, ts, n, opts->aMax, spec, serial);
+  if (!spec.empty()) {
     std::vector<mc_limit> lim;
-    if ((st = plain_search(cells, ts, all, opts, lim))) return st;
-    for (int i = 0; i < n; i++) out[i] = lim[i];
-    return MC_OK;
+    if ((st = plain_search(cells, ts, spec, opts, lim))) return st;
+    for (size_t q = 0; q < spec.size(); q++) out[spec[q]] = lim[q];
   }
-  ts.tasks.resize(n);
-  ts.cost.resize(n);
-  ts.nzcap.resize(n);
-  for (int i = 0; i < n; i++) {
-    mcd::TaskDev& t = ts.tasks[i];
+  if (serial.empty()) return MC_OK;
+  TaskSet ss;
+  ss.cells = ts.cells;
+  for (int i : serial) {
+    mcd::TaskDev t;
     std::memset(&t, 0, sizeof t);
     t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = opts->iters; t.aMin = opts->aMin; t.aMax = opts->aMax;
-    ts.nzcap[i] = nz_bound(cells[i], opts->aMax);
-    ts.cost[i] = cell_cost(ts.cells[i]) * (ts.nzcap[i] + opts->iters * 0.25 * ts.nzcap[i]);
+    ss.tasks.push_back(t);
+    ss.nzcap.push_back(nz_bound(cells[i], opts->aMax));
+    ss.cost.push_back(cell_cost(ts.cells[i]) * (ss.nzcap.back() + opts->iters * 0.25 * ss.nzcap.back()));
   }
   std::vector<mcd::ResultDev> res;
-  if ((st = run_tasks(ts, res, nullptr, 0))) return st;
-  for (int i = 0; i < n; i++) to_limit(res[i], out[i]);
+  if ((st = run_tasks(ss, res, nullptr, 0))) return st;
+  for (size_t q = 0; q < serial.size(); q++) to_limit(res[q], out[serial[q]]);
   return MC_OK;
 }
 
@@ -552,12 +573,15 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
   if (n > 0 && (!out || !lo_hint || !hi_hint)) return fail(MC_ERR_ARG, "null hint or output array");
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
-  if (use_speculative(cells, n, opts->aMax)) {
+  g_ctx.last_ms = 0;
+  std::vector<int> spec, serial;
+  split_cells(cells, ts, n, opts->aMax, spec, serial);
+  if (!spec.empty()) {
     // hinted trees; cells whose hint fails (or is invalid) rerun the plain search
     TaskSet hs;
     hs.cells = ts.cells;
     std::vector<int> hinted, plain;
-    for (int i = 0; i < n; i++) {
+    for (int i : spec) {
       const bool ok = lo_hint[i] > 0 && hi_hint[i] > lo_hint[i] && hi_hint[i] <= opts->aMax &&
                       lo_hint[i] >= opts->aMin;
       if (!ok) { plain.push_back(i); continue; }
@@ -593,27 +617,28 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
       out[i] = l;
       if (fell_back) fell_back[i] = 1;
     }
-    return MC_OK;
   }
-  ts.tasks.resize(n);
-  ts.cost.resize(n);
-  ts.nzcap.resize(n);
-  for (int i = 0; i < n; i++) {
-    mcd::TaskDev& t = ts.tasks[i];
+  if (serial.empty()) return MC_OK;
+  TaskSet ss;
+  ss.cells = ts.cells;
+  for (int i : serial) {
+    mcd::TaskDev t;
     std::memset(&t, 0, sizeof t);
     t.cell = i; t.iters = opts->iters; t.aMin = opts->aMin; t.aMax = opts->aMax;
     t.lo = lo_hint[i]; t.hi = hi_hint[i]; t.tol = tol_mm;
     const bool ok = lo_hint[i] > 0 && hi_hint[i] > lo_hint[i] && hi_hint[i] <= opts->aMax &&
                     lo_hint[i] >= opts->aMin;
     t.mode = ok ? mcd::TASK_BRACKET : mcd::TASK_BISECT;
-    ts.nzcap[i] = nz_bound(cells[i], opts->aMax);
-    ts.cost[i] = cell_cost(ts.cells[i]) * ts.nzcap[i];
+    ss.tasks.push_back(t);
+    ss.nzcap.push_back(nz_bound(cells[i], opts->aMax));
+    ss.cost.push_back(cell_cost(ts.cells[i]) * ss.nzcap.back());
   }
   std::vector<mcd::ResultDev> res;
-  if ((st = run_tasks(ts, res, nullptr, 0))) return st;
-  for (int i = 0; i < n; i++) {
-    to_limit(res[i], out[i]);
-    if (fell_back) fell_back[i] = ts.tasks[i].mode == mcd::TASK_BISECT ? 1 : res[i].fell_back;
+  if ((st = run_tasks(ss, res, nullptr, 0))) return st;
+  for (size_t q = 0; q < serial.size(); q++) {
+    const int i = serial[q];
+    to_limit(res[q], out[i]);
+    if (fell_back) fell_back[i] = ss.tasks[q].mode == mcd::TASK_BISECT ? 1 : res[q].fell_back;
   }
   return MC_OK;
 }
@@ -625,6 +650,7 @@ int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps, double* Fx, 
   if (!cell || nsteps < 0 || !std::isfinite(ap)) return fail(MC_ERR_ARG, "bad trace arguments");
   TaskSet ts;
   if ((st = prepare(cell, 1, ts))) return st;
+  g_ctx.last_ms = 0;
   const mcd::CellDev& c = ts.cells[0];
   const int64_t total = (int64_t)c.spt * c.N * c.revs;
   const int64_t ns = std::min(nsteps, total);

@@ -324,6 +324,90 @@ __host__ __device__ inline double pow_js(double x, double y) {
   return words((uint32_t)j, low(z));
 }
 
+// pow_js for the operands of the chip-thickness size effect, (H_REF / h)^mc: x a positive
+// normal number in [2^-96, 2^96] and y with 0 < |y| <= 8 that is none of -1, 1, 2, 1/2
+// (pow_cell_ok).  There |y * log2 x| < 1024, so the result is a normal number and none of
+// pow_js's special cases, overflow or underflow branches is taken; the general path is
+// kept operation for operation with its remaining branches turned into selects, so the
+// result equals pow_js(x, y) bit for bit.  y1 is clear_lo(y) and yr is y - y1.
+MC_HD bool pow_cell_ok(double y) {
+  if (!(y == y) || y == 0 || fabs_(y) > 8) return false;
+  return !(y == 1 || y == -1 || y == 2 || y == 0.5);
+}
+MC_HD double pow_cell(double x, double y, double y1, double yr) {
+  constexpr double dp_h1 = 5.84962487220764160156e-01, dp_l1 = 1.35003920212974897128e-08;
+  constexpr double L1 = 5.99999999999994648725e-01, L2 = 4.28571428578550184252e-01,
+                   L3 = 3.33333329818377432918e-01, L4 = 2.72728123808534006489e-01,
+                   L5 = 2.30660745775561754067e-01, L6 = 2.06975017800338417784e-01;
+  constexpr double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03,
+                   P3 = 6.61375632143793436117e-05, P4 = -1.65339022054652515390e-06,
+                   P5 = 4.13813679705723846039e-08;
+  constexpr double lg2 = 6.93147180559945286227e-01, lg2_h = 6.93147182464599609375e-01,
+                   lg2_l = -1.90465429995776804525e-09;
+  constexpr double cp = 9.61796693925975554329e-01, cp_h = 9.61796700954437255859e-01,
+                   cp_l = -7.02846165095275826516e-09;
+  // log2(x) = t1 + t2 (x normal, so no subnormal scaling)
+  int32_t ix = hiw(x);
+  int32_t n = (ix >> 20) - 0x3ff;
+  const int32_t jm = ix & 0x000fffff;
+  const bool wrap = jm >= 0xBB67A;            // mantissa above sqrt(3): use 2^(n+1)
+  const bool k1 = jm > 0x3988E && !wrap;      // mantissa in (sqrt(3/2), sqrt(3)): base 1.5
+  n += wrap ? 1 : 0;
+  ix = (jm | 0x3ff00000) - (wrap ? 0x00100000 : 0);
+  const double ax = words((uint32_t)ix, low(x));
+  const double bpk = k1 ? 1.5 : 1.0, dphk = k1 ? dp_h1 : 0.0, dplk = k1 ? dp_l1 : 0.0;
+  double u = ax - bpk;
+  double v = 1.0 / (ax + bpk);
+  const double ss = u * v;
+  const double s_h = clear_lo(ss);
+  double t_h = words((uint32_t)(((ix >> 1) | 0x20000000) + 0x00080000 + (k1 ? (1 << 18) : 0)), 0);
+  double t_l = ax - (t_h - bpk);
+  const double s_l = v * ((u - s_h * t_h) - s_h * t_l);
+  double s2 = ss * ss;
+  double r = s2 * s2 * (L1 + s2 * (L2 + s2 * (L3 + s2 * (L4 + s2 * (L5 + s2 * L6)))));
+  r += s_l * (s_h + ss);
+  s2 = s_h * s_h;
+  t_h = clear_lo(3.0 + s2 + r);
+  t_l = r - ((t_h - 3.0) - s2);
+  u = s_h * t_h;
+  v = s_l * t_h + t_l * ss;
+  double p_h = clear_lo(u + v);
+  double p_l = v - (p_h - u);
+  const double z_h = cp_h * p_h;
+  const double z_l = cp_l * p_h + p_l * cp + dplk;
+  double t = (double)n;
+  const double t1 = clear_lo(((z_h + z_l) + dphk) + t);
+  const double t2 = z_l - (((t1 - t) - dphk) - z_h);
+  // y * log2(x) = p_h + p_l
+  p_l = yr * t1 + y * t2;
+  p_h = y1 * t1;
+  double z = p_l + p_h;
+  const int32_t j = hiw(z);
+  const int32_t i = j & 0x7fffffff;
+  // |z| > 1/2: split off the integer part n (|z| < 1024, so every shift is in range)
+  const bool big = i > 0x3fe00000;
+  const int32_t kz = big ? (i >> 20) - 0x3ff : -1;
+  int32_t nn = j + (0x00100000 >> (kz + 1));
+  const int32_t kn = big ? ((nn & 0x7fffffff) >> 20) - 0x3ff : 0;
+  t = words((uint32_t)(nn & ~(0x000fffff >> kn)), 0);
+  nn = ((nn & 0x000fffff) | 0x00100000) >> (20 - kn);
+  if (j < 0) nn = -nn;
+  n = big ? nn : 0;
+  p_h = big ? p_h - t : p_h;
+  // 2^z
+  t = clear_lo(p_l + p_h);
+  u = t * lg2_h;
+  v = (p_l - (t - p_h)) * lg2 + t * lg2_l;
+  z = u + v;
+  const double w = v - (z - u);
+  t = z * z;
+  const double e1 = z - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+  r = (z * e1) / (e1 - 2.0) - (w + z * w);
+  z = 1.0 - (r - z);
+  const double res = words((uint32_t)(hiw(z) + (n << 20)), low(z));
+  return x == 1.0 ? 1.0 : res;
+}
+
 // Math.hypot(a, b) as V8's builtin computes it
 MC_HD double hypot_js(double a, double b) {
   const double fa = fabs_(a), fb = fabs_(b);

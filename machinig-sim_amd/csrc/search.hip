@@ -75,25 +75,29 @@ struct WaveLds {
   long long eng[MAXSLOT];
   int nz[MAXSLOT];
   float Hb[MC_BINS];
+  float tsx[MAXSLOT][TS], tsy[MAXSLOT][TS];  // tooth-passing samples of each slot
   int task, nspec, overflow;
 };
 
+// Behind WaveLds the wave's LDS holds, at different times of a round, either the slice
+// records of every slot (set-up: lag, z, width, kappa, radius, table column) or the step
+// loop's buffers: the engaged visits of each slot (h, sin, cos, width, damping factor),
+// then their force terms in (tooth, slice) order.
 struct SlotLds {
-  double2* term;   // SG*MAXP force terms
   double* sLag;    // [nz_cap] per slice
   double* sZ;
   double* sB;
   double* sKap;
   double* sR;
-  int* lagid;      // table column of each slice, -1: evaluate directly
-  float* tsx;      // [TS]
-  float* tsy;
+  int* lagid;      // table column of each slice
 };
+constexpr int NREC = 64 * MAXP;  // engaged-visit records of a wave (a slot of SG lanes owns SG*MAXP)
 
 __device__ __forceinline__ size_t wave_bytes() { return (sizeof(WaveLds) + 15) & ~(size_t)15; }
 __device__ __host__ __forceinline__ size_t slot_bytes_of(int nz_cap) {
-  return (8 * 5 * (size_t)nz_cap + 4 * (size_t)nz_cap + 4 * 2 * TS + 15) & ~(size_t)15;
+  return (8 * 5 * (size_t)nz_cap + 4 * (size_t)nz_cap + 15) & ~(size_t)15;
 }
+__device__ __host__ __forceinline__ size_t step_bytes() { return (size_t)NREC * (16 + 5 * 8); }
 
 // one table entry: sin, cos and bin of the edge angle at (pr, tooth i, lag)
 __device__ __forceinline__ void edge_geometry(double theta, double ipitch, double lag, double binScale,
@@ -106,12 +110,13 @@ __device__ __forceinline__ void edge_geometry(double theta, double ipitch, doubl
 
 }  // namespace
 
-// LDS of a wave: the per-wave block, 64*MAXP force terms (a slot of SG lanes owns SG*MAXP
-// of them), then per slot its slice records, table columns and tooth-sample ring
+// LDS of a wave: the per-wave block, then the larger of the set-up slice records of all
+// slots and the step loop's visit records and force terms
 int mcd_search_lds_bytes(int nv_cap, int nz_cap) {
   (void)nv_cap;
   const size_t wave = (sizeof(WaveLds) + 15) & ~(size_t)15;
-  return (int)(wave + 16 * 64 * MAXP + MAXSLOT * slot_bytes_of(nz_cap));
+  const size_t setup = MAXSLOT * slot_bytes_of(nz_cap);
+  return (int)(wave + (setup > step_bytes() ? setup : step_bytes()));
 }
 int mcd_search_maxspec() { return MAXSPEC; }
 
@@ -119,7 +124,13 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   WaveLds& W = *reinterpret_cast<WaveLds*>(lds);
   const size_t slot_bytes = slot_bytes_of(P.nz_cap);
-  double2* const terms = reinterpret_cast<double2*>(lds + wave_bytes());
+  unsigned char* const area = lds + wave_bytes();
+  double2* const terms = reinterpret_cast<double2*>(area);
+  double* const rec_h = reinterpret_cast<double*>(area + 16 * NREC);
+  double* const rec_sn = rec_h + NREC;
+  double* const rec_cs = rec_sn + NREC;
+  double* const rec_b = rec_cs + NREC;
+  double* const rec_pd = rec_b + NREC;
   const int lane = threadIdx.x;
   const int64_t wid = blockIdx.x;
   float* ring_base = P.ring + wid * (int64_t)MAXSLOT * P.ring_cap;
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
     const int cell = uni(T.cell);
     const CellDev& C = P.cells[cell];
     const int N = C.N, spt = C.spt, spr = C.spr, decim = C.decim;
-    const long long steps = (long long)spt * N * C.revs;
+    const int steps = spt * N * C.revs;  // below 2^31: the host bounds spt, N and revs
     for (int j = lane; j < MC_BINS; j += 64) W.Hb[j] = C.Hb[j];
     if (lane == 0) {
       CellCst& K = W.K;
@@ -223,13 +234,15 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       const int sl = in_slot ? slot : 0;
       SlotLds S;
       {
-        S.term = terms + sl * SG * MAXP;
-        double* d = reinterpret_cast<double*>(lds + wave_bytes() + 16 * 64 * MAXP + (size_t)sl * slot_bytes);
+        double* d = reinterpret_cast<double*>(area + (size_t)sl * slot_bytes);
         S.sLag = d; d += P.nz_cap; S.sZ = d; d += P.nz_cap; S.sB = d; d += P.nz_cap;
         S.sKap = d; d += P.nz_cap; S.sR = d; d += P.nz_cap;
         S.lagid = reinterpret_cast<int*>(d);
-        S.tsx = reinterpret_cast<float*>(S.lagid + P.nz_cap); S.tsy = S.tsx + TS;
       }
+      const int rbase = sl * SG * MAXP;  // this slot's visit records and force terms
+      double2* const term = terms + rbase;
+      float* const tsx = W.tsx[sl];
+      float* const tsy = W.tsy[sl];
       float* ring = ring_base + (int64_t)sl * P.ring_cap;
       float2* hist = hist_base + sl * HIST;
 
@@ -267,7 +280,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       if (lane == 0) {
         int ns = 0;
         for (int s2 = 0; s2 < nslot; s2++) {
-          double* d = reinterpret_cast<double*>(lds + wave_bytes() + 16 * 64 * MAXP + (size_t)s2 * slot_bytes);
+          double* d = reinterpret_cast<double*>(area + (size_t)s2 * slot_bytes);
           const double* lag2 = d;
           int* id2 = reinterpret_cast<int*>(d + 5 * P.nz_cap);
           double dd = dz0;
@@ -288,7 +301,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         W.nspec = ns;
         W.overflow = 0;
         for (int s2 = 0; s2 < nslot; s2++) {
-          double* d = reinterpret_cast<double*>(lds + wave_bytes() + 16 * 64 * MAXP + (size_t)s2 * slot_bytes);
+          double* d = reinterpret_cast<double*>(area + (size_t)s2 * slot_bytes);
           const int* id2 = reinterpret_cast<int*>(d + 5 * P.nz_cap);
           double dd = dz0;
           const int nz2 = slices_at(C.toolFluteLen, W.node_ap[s2], dd);
@@ -348,6 +361,9 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       const double fz = K.fz, hMax = K.hMax, Ktc = K.Ktc, Krc = K.Krc, Kte = K.Kte, Kre = K.Kre;
       const double mcx = K.mc, dt = K.dt;
       const bool use_se = mcx != 0 && mcx == mcx;
+      // x = 0.05 / max(h, 0.002) lies in [0.05 / hMax, 25]; the host sends this kernel only
+      // cells with hMax <= 1e27 and pow_cell_ok(mc), where pow_cell equals pow_js
+      const double my1 = mcm::clear_lo(mcx), myr = mcx - my1;
       const bool use_pd = pdK != 0 && pdK == pdK;
       // lanes of this lane's slot in a wave-wide ballot
       const unsigned long long smask = SG == 64 ? ~0ull : (((1ull << SG) - 1) << (sl * SG));
@@ -377,7 +393,8 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         cbin[p] = gbn[at];
       }
 
-      for (long long s = 0; s < steps; s++) {
+      wave_sync();  // the slice records are read; the step loop reuses their LDS
+      for (int s = 0; s < steps; s++) {
         const double xm = x * 1e3, ym = y * 1e3;  // rigid workpiece
         const double rvx = vxs, rvy = vys;
         const int prn = pr + 1 == spr ? 0 : pr + 1;
@@ -394,67 +411,74 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           nsc[p] = gsc[at];
           nbin[p] = gbn[at];
         }
-        int ne = 0;  // engaged visits of this slot so far, in (tooth, slice) order
+        // engagement-map heights at both passes' bins, read together
+        float hb[MAXP];
+#pragma unroll
+        for (int p = 0; p < MAXP; p++) hb[p] = W.Hb[cbin[p] < MC_BINS ? cbin[p] : 0];
+        // 1) chip thickness and delay-line update of every visit; the engaged ones are
+        //    recorded in (tooth, slice) order, the order in which the reference sums them
+        int ne = 0;
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
-          bool eng = false;
-          double tX = 0, tY = 0;
+          // src/cutting.js:176-195, with its branches as selects
+          const double sn = csc[p].x, cs = csc[p].y;
+          const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
+          const double rn = xm * nxg + ym * nyg + vdR[p];
+          const double prev = (double)cur[p];
+          const double feedH = fz * sn * vKap[p];
+          const bool below = cbin[p] < MC_BINS && (double)hb[p] <= vZ[p];
+          const bool cutting = !(below || sn <= 0);
+          const double h = feedH + rn - prev;
+          const bool thin = h <= 0;  // the edge passes above the surface it left
+          const float store = (cutting && thin) ? (float)(prev - feedH) : (float)rn;
+          const bool eng = vok[p] && cutting && !thin;
           if (vok[p]) {
-            const int i = vi[p], k = vk[p];
-            const double sn = csc[p].x, cs = csc[p].y;
-            const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-            const double rn = xm * nxg + ym * nyg + vdR[p];
-            int pi = pr + i * spt;
+            int pi = pr + vi[p] * spt;
             if (pi >= spr) pi -= spr;
-            const double prev = (double)cur[p];
-            const double feedH = fz * sn * vKap[p];
-            const int bin = cbin[p];
-            const bool below = bin < MC_BINS ? ((double)W.Hb[bin] <= vZ[p]) : false;
-            float store;
-            if (below || sn <= 0) {
-              store = (float)rn;
-            } else {
-              double h = feedH + rn - prev;
-              if (h <= 0) {
-                store = (float)(prev - feedH);
-              } else {
-                store = (float)rn;
-                eng = true;
-                if (h > hMax) h = hMax;
-                const double se = use_se ? mcm::pow_js(0.05 / (h > 0.002 ? h : 0.002), mcx) : 1.0;
-                const double bk = vB[p];
-                const double Ft = Ktc * se * h * bk + Kte * bk;
-                double Fr = Krc * se * h * bk + Kre * bk;
-                if (use_pd) Fr += vPd[p] * (rvx * nxg + rvy * nyg);
-                const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-                tX = -Ft * txg - Fr * nxg;
-                tY = -Ft * tyg - Fr * nyg;
-              }
-            }
-            ring[pi * nz + k] = store;
+            ring[pi * nz + vk[p]] = store;
           }
-          // the reference sums the engaged visits only: compact their terms in order
           const unsigned long long m = __builtin_amdgcn_ballot_w64(eng) & smask;
           const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-          if (eng) S.term[ne + before] = make_double2(tX, tY);
+          if (eng) {
+            const int r = rbase + ne + before;
+            rec_h[r] = h; rec_sn[r] = sn; rec_cs[r] = cs; rec_b[r] = vB[p]; rec_pd[r] = vPd[p];
+          }
           ne += __builtin_popcountll(m);
         }
         my_eng += (g == 0) ? ne : 0;
+        wave_sync();
+        // 2) cutting forces of the engaged visits, one visit per lane (src/cutting.js:196-206)
+        if (in_slot) {
+          for (int r = g; r < ne; r += SG) {
+            double h = rec_h[rbase + r];
+            const double sn = rec_sn[rbase + r], cs = rec_cs[rbase + r];
+            const double bk = rec_b[rbase + r];
+            if (h > hMax) h = hMax;
+            const double se = use_se ? mcm::pow_cell(0.05 / (h > 0.002 ? h : 0.002), mcx, my1, myr) : 1.0;
+            const double Ft = Ktc * se * h * bk + Kte * bk;
+            double Fr = Krc * se * h * bk + Kre * bk;
+            const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
+            if (use_pd) Fr += rec_pd[rbase + r] * (rvx * nxg + rvy * nyg);
+            const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
+            term[r] = make_double2(-Ft * txg - Fr * nxg, -Ft * tyg - Fr * nyg);
+          }
+        }
         // pad to a multiple of 4 with +0 terms: the running sum starts at +0 and can never
         // be -0 under round-to-nearest, so adding +0 leaves it unchanged
         const int ne4 = (ne + 3) & ~3;
-        if (in_slot && g < ne4 - ne) S.term[ne + g] = make_double2(0.0, 0.0);
+        if (in_slot && g < ne4 - ne) term[ne + g] = make_double2(0.0, 0.0);
         wave_sync();
+        // 3) the serial sum, in order
         double Fx = 0, Fy = 0;
         for (int v = 0; v < ne4; v += 4) {
-          const double2 t0 = S.term[v], t1 = S.term[v + 1], t2 = S.term[v + 2], t3 = S.term[v + 3];
+          const double2 t0 = term[v], t1 = term[v + 1], t2 = term[v + 2], t3 = term[v + 3];
           Fx += t0.x; Fy += t0.y;
           Fx += t1.x; Fy += t1.y;
           Fx += t2.x; Fy += t2.y;
           Fx += t3.x; Fy += t3.y;
         }
-        // modal integration (semi-implicit Euler, src/cutting.js:247-255): each mode lane
-        // advances its mode, then every lane sums the slot's modes in order
+        // 4) modal integration (semi-implicit Euler, src/cutting.js:247-255): each mode lane
+        //    advances its mode, then every lane sums the slot's modes in order
         if (xmode || ymode) {
           const double F = xmode ? Fx : Fy;
           mv += (F - mcd_ * mv - mk * mq) / mm * dt;
@@ -465,13 +489,20 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         wave_sync();
         x = 0; y = 0; vxs = 0; vys = 0;
         if (in_slot) {
-          for (int j = 0; j < nmx; j++) { x += W.mq[sl][j]; vxs += W.mv[sl][j]; }
-          for (int j = 0; j < nmy; j++) { y += W.mq[sl][MC_MAX_MODES + j]; vys += W.mv[sl][MC_MAX_MODES + j]; }
+          double aq[2 * MC_MAX_MODES], av[2 * MC_MAX_MODES];
+#pragma unroll
+          for (int j = 0; j < 2 * MC_MAX_MODES; j++) { aq[j] = W.mq[sl][j]; av[j] = W.mv[sl][j]; }
+#pragma unroll
+          for (int j = 0; j < MC_MAX_MODES; j++)
+            if (j < nmx) { x += aq[j]; vxs += av[j]; }
+#pragma unroll
+          for (int j = 0; j < MC_MAX_MODES; j++)
+            if (j < nmy) { y += aq[MC_MAX_MODES + j]; vys += av[MC_MAX_MODES + j]; }
         }
         pr = prn;
         ph = ph + 1 == spt ? 0 : ph + 1;
         if (ph == 0) {
-          if (g == 0 && in_slot) { S.tsx[ti] = (float)x; S.tsy[ti] = (float)y; }
+          if (g == 0 && in_slot) { tsx[ti] = (float)x; tsy[ti] = (float)y; }
           ti = (ti + 1) & (TS - 1);
           tcount++;
         }
@@ -493,11 +524,11 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         const int n = tcount < 24 ? tcount : 24;
         if (n >= 8) {
           double mx = 0, my = 0, v = 0;
-          for (int k = 0; k < n; k++) { const int j = (ti - 1 - k) & (TS - 1); mx += S.tsx[j]; my += S.tsy[j]; }
+          for (int k = 0; k < n; k++) { const int j = (ti - 1 - k) & (TS - 1); mx += tsx[j]; my += tsy[j]; }
           mx /= n; my /= n;
           for (int k = 0; k < n; k++) {
             const int j = (ti - 1 - k) & (TS - 1);
-            const double dx = S.tsx[j] - mx, dy = S.tsy[j] - my;
+            const double dx = tsx[j] - mx, dy = tsy[j] - my;
             v += dx * dx + dy * dy;
           }
           once = __builtin_sqrt(v / n) * 1e6;

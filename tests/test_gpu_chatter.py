@@ -99,3 +99,30 @@ def test_speculative_search_equals_serial_search(mc, orc, monkeypatch):
         a, b = spec[i], ser[i]
         assert (a.alim, a.lo, a.hi, a.open, a.trials, a.steps, a.visits, a.engaged) == \
                (b.alim, b.lo, b.hi, b.open, b.trials, b.steps, b.visits, b.engaged), i
+
+
+def test_mixed_batch_routes_every_cell_exactly(mc, orc):
+    """one batch with cells the search kernel takes and cells it leaves to the
+    one-trial-per-group kernel: size-effect exponents 0.5, 1, 2 and 9 (outside the search
+    kernel's power function), a 4-flute tool needing 160 visits per step at aMax (above its
+    128), and a 12 mm flute whose deep trials change the slice pitch (more slice lags than
+    its geometry table holds: the cell is rerun there).  Every result equals the oracle's."""
+    from mcsim import workloads as W
+    from mcsim._abi import make_cell
+    t3 = dict(N=3, R=6.0, helixDeg=30.0, type="flat", fluteLen=24.0)
+    t4 = dict(N=4, R=6.0, helixDeg=30.0, type="flat", fluteLen=24.0)
+    t2 = dict(N=2, R=5.0, helixDeg=35.0, type="ball", fluteLen=12.0)
+    modes = [(2e4, 1000.0, 0.03)]
+    arc = (W.JS_PI / 2, W.JS_PI)
+    cells = []
+    for j, mcx in enumerate((0.1, 0.5, 1.0, 2.0, 9.0, 0.25, 0.0)):
+        mat = dict(W.MAT_A5052, mc=mcx)
+        cells.append(make_cell(t3, modes, None, mat, 4000 + 900 * j, arc, 0.05, 20, True))
+    cells.append(make_cell(t4, modes, None, W.MAT_A5052, 5200, arc, 0.05, 12, True))
+    cells.append(make_cell(t2, modes, None, W.MAT_A5052, 6100, arc, 0.05, 12, True))
+    opts = dict(aMin=0.05, aMax=20.0, iters=8)
+    out = mc.bisect_batch(cells, **opts)
+    for i, c in enumerate(cells):
+        o = orc.time_domain_limit(c, **opts)
+        assert (out[i].alim, out[i].lo, out[i].hi, out[i].open, out[i].trials, out[i].steps,
+                out[i].visits) == (o.alim, o.lo, o.hi, o.open, o.trials, o.steps, o.visits), i
