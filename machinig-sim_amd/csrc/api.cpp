@@ -306,7 +306,8 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
     pe_cap = std::max<int64_t>(pe_cap, (int64_t)c.spr * c.N);
   }
   const int64_t geo_cap = pe_cap * (nz_cap + mcd_search_maxspec());
-  if (geo_cap >= ((int64_t)1 << 32)) return fail(MC_ERR_LIMIT, "edge-geometry table of %lld entries too large", (long long)geo_cap);
+  if (geo_cap * 16 >= ((int64_t)1 << 31) || (int64_t)SEARCH_MAXSLOT * ring_cap * 4 >= ((int64_t)1 << 31))
+    return fail(MC_ERR_LIMIT, "per-wave delay lines or geometry table exceed 2 GiB");
   const int wpc = std::max(1, env_int("MC_WAVES_PER_CU", 8));
   const int waves = (int)std::min<int64_t>(n, (int64_t)X.cus * wpc);
   const int lds = mcd_search_lds_bytes(nv_cap, nz_cap);
@@ -368,7 +369,11 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
 bool spec_cell(const mc_cell& c, const mcd::CellDev& d, double aMax) {
   const char* v = getenv("MC_SEARCH");
   if (v && std::strcmp(v, "serial") == 0) return false;
-  if (c.tool.N * nz_bound(c, aMax) > SEARCH_MAXP * 64) return false;
+  const int nzb = nz_bound(c, aMax);
+  if (c.tool.N * nzb > SEARCH_MAXP * 64) return false;
+  // the kernel addresses a wave's delay lines and geometry table with 32-bit byte offsets
+  if ((double)SEARCH_MAXSLOT * std::max(nzb, 1) * d.spr * 4 >= 2147483648.0) return false;
+  if ((double)d.spr * d.N * (std::max(nzb, 1) + mcd_search_maxspec()) * 16 >= 2147483648.0) return false;
   const bool use_se = d.mc != 0 && d.mc == d.mc;
   return !use_se || (mcm::pow_cell_ok(d.mc) && d.hMax <= 1e27);
 }

@@ -136,9 +136,10 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
   float* ring_base = P.ring + wid * (int64_t)MAXSLOT * P.ring_cap;
   float2* hist_base = P.hist + wid * (int64_t)MAXSLOT * HIST;
   double2* const gsc = P.geo + wid * P.geo_cap;
+  const char* const gsc_b = reinterpret_cast<const char*>(gsc);
+  char* const ring_b = reinterpret_cast<char*>(ring_base);
   unsigned char* const gbn = P.gbin + wid * P.geo_cap;
   const int NL = P.nz_cap + MAXSPEC;   // table columns: full slices, then per-round ones
-  const double ux = 1.0, uy = 0.0, vx = -0.0, vy = 1.0;  // chatterAt's feed direction
 
   while (true) {
     if (lane == 0) W.task = atomicAdd(P.queue, 1);
@@ -384,13 +385,18 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       float cur[MAXP];
       double2 csc[MAXP];
       int cbin[MAXP];
-      unsigned rowN = (unsigned)N * (unsigned)NL;  // table entries per step row
+      const unsigned rowN = (unsigned)N * (unsigned)NL;  // table entries per step row
+      // per pass: delay-line row base of the slot and slice, the ring position of the
+      // visit's tooth at the current step ((pr + i*spt) mod spr), and its table column
+      unsigned rs[MAXP], rp[MAXP], tcol[MAXP];
 #pragma unroll
       for (int p = 0; p < MAXP; p++) {
         cur[p] = 0.0f;
-        const unsigned at = (unsigned)vi[p] * (unsigned)NL + (unsigned)vc[p];
-        csc[p] = gsc[at];
-        cbin[p] = gbn[at];
+        rs[p] = (unsigned)sl * (unsigned)P.ring_cap + (unsigned)vk[p];
+        rp[p] = (unsigned)(vi[p] * spt);  // pr = 0, and i*spt < spr
+        tcol[p] = (unsigned)vi[p] * (unsigned)NL + (unsigned)vc[p];
+        csc[p] = *reinterpret_cast<const double2*>(gsc_b + 16u * tcol[p]);
+        cbin[p] = gbn[tcol[p]];
       }
 
       wave_sync();  // the slice records are read; the step loop reuses their LDS
@@ -398,17 +404,19 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         const double xm = x * 1e3, ym = y * 1e3;  // rigid workpiece
         const double rvx = vxs, rvy = vys;
         const int prn = pr + 1 == spr ? 0 : pr + 1;
-        // prefetch the next step's delay-line values and edge geometry (all lanes)
+        // prefetch the next step's delay-line values and edge geometry (all lanes; 32-bit
+        // offsets from the wave's base pointers)
         float nxt[MAXP];
         double2 nsc[MAXP];
         int nbin[MAXP];
+        unsigned rpn[MAXP];
+        const unsigned rowat = (unsigned)prn * rowN;
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
-          int pi = prn + vi[p] * spt;
-          if (pi >= spr) pi -= spr;
-          nxt[p] = ring[pi * nz + vk[p]];
-          const unsigned at = (unsigned)prn * rowN + (unsigned)vi[p] * (unsigned)NL + (unsigned)vc[p];
-          nsc[p] = gsc[at];
+          rpn[p] = rp[p] + 1 == (unsigned)spr ? 0u : rp[p] + 1;
+          nxt[p] = *reinterpret_cast<const float*>(ring_b + 4u * (rs[p] + rpn[p] * (unsigned)nz));
+          const unsigned at = rowat + tcol[p];
+          nsc[p] = *reinterpret_cast<const double2*>(gsc_b + 16u * at);
           nbin[p] = gbn[at];
         }
         // engagement-map heights at both passes' bins, read together
@@ -420,10 +428,12 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         int ne = 0;
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
-          // src/cutting.js:176-195, with its branches as selects
+          // src/cutting.js:176-195, with its branches as selects.  chatterAt's feed frame is
+          // u = (1, 0), v = (-0, 1) (src/stabtd.js:18), so the edge normal
+          // (sn*ux + cs*vx, sn*uy + cs*vy) is (sn, cs) bit for bit: sn is never -0 (the edge
+          // angle is never -0) and cs is never 0 (no double is a zero of cosine)
           const double sn = csc[p].x, cs = csc[p].y;
-          const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-          const double rn = xm * nxg + ym * nyg + vdR[p];
+          const double rn = xm * sn + ym * cs + vdR[p];
           const double prev = (double)cur[p];
           const double feedH = fz * sn * vKap[p];
           const bool below = cbin[p] < MC_BINS && (double)hb[p] <= vZ[p];
@@ -432,11 +442,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           const bool thin = h <= 0;  // the edge passes above the surface it left
           const float store = (cutting && thin) ? (float)(prev - feedH) : (float)rn;
           const bool eng = vok[p] && cutting && !thin;
-          if (vok[p]) {
-            int pi = pr + vi[p] * spt;
-            if (pi >= spr) pi -= spr;
-            ring[pi * nz + vk[p]] = store;
-          }
+          if (vok[p]) *reinterpret_cast<float*>(ring_b + 4u * (rs[p] + rp[p] * (unsigned)nz)) = store;
           const unsigned long long m = __builtin_amdgcn_ballot_w64(eng) & smask;
           const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
           if (eng) {
@@ -457,10 +463,10 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
             const double se = use_se ? mcm::pow_cell(0.05 / (h > 0.002 ? h : 0.002), mcx, my1, myr) : 1.0;
             const double Ft = Ktc * se * h * bk + Kte * bk;
             double Fr = Krc * se * h * bk + Kre * bk;
-            const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-            if (use_pd) Fr += rec_pd[rbase + r] * (rvx * nxg + rvy * nyg);
-            const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-            term[r] = make_double2(-Ft * txg - Fr * nxg, -Ft * tyg - Fr * nyg);
+            // edge normal (sn, cs) and tangent (cs, -sn) in chatterAt's feed frame; engaged
+            // visits have sn > 0, where cs*uy - sn*vy is -sn bit for bit
+            if (use_pd) Fr += rec_pd[rbase + r] * (rvx * sn + rvy * cs);
+            term[r] = make_double2(-Ft * cs - Fr * sn, Ft * sn - Fr * cs);
           }
         }
         // pad to a multiple of 4 with +0 terms: the running sum starts at +0 and can never
@@ -513,7 +519,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           hcount++;
         }
 #pragma unroll
-        for (int p = 0; p < MAXP; p++) { cur[p] = nxt[p]; csc[p] = nsc[p]; cbin[p] = nbin[p]; }
+        for (int p = 0; p < MAXP; p++) { cur[p] = nxt[p]; csc[p] = nsc[p]; cbin[p] = nbin[p]; rp[p] = rpn[p]; }
         wave_sync();
       }
 
