@@ -185,6 +185,46 @@ int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol);
 /* Stock.sample (src/stock.js:85-92): bilinear height at n points, -inf outside. */
 int mc_stock_sample(const mc_stock* s, const double* xy, int n, double* out);
 
+/* ---- Learned bracket surrogate (no counterpart in the reference) ----
+ * An MLP 24 -> 256 -> 256 -> 256 -> 1 with GELU (tanh form) that predicts log(alim [mm])
+ * of a cell from mc_surrogate_features(); its prediction p gives the bracket
+ * [0.85 e^p, 1.15 e^p] for mc_bisect_bracket_batch, which verifies the bracket and
+ * otherwise runs the plain search.  Trained on the device: BF16 MFMA products with FP32
+ * accumulation, FP32 master weights, Adam; the training step is a replayed hipGraph.
+ * Parameters, in this order (FP32): W1[24][256] b1[256] W2[256][256] b2[256]
+ * W3[256][256] b3[256] W4[256] b4 (W[i][o]: input i, output o). */
+#define MC_SURR_IN      24
+#define MC_SURR_HIDDEN 256
+
+typedef struct mc_mlp mc_mlp;
+
+typedef struct {
+  int32_t epochs;
+  int32_t batch;          /* rows per step: a multiple of 64, <= n */
+  float lr, lr_final;     /* Adam step size, cosine-decayed from lr to lr_final */
+  float beta1, beta2, eps;
+  int32_t fit_scaling;    /* 1: set the input / target scaling from the training rows */
+  uint64_t seed;          /* shuffling */
+} mc_train_opts;
+
+/* the 24 raw inputs of n cells (row-major n x 24; layout in mlp.hip) */
+int mc_surrogate_features(const mc_cell* cells, int n, float* out);
+int mc_mlp_create(uint64_t seed, mc_mlp** out);
+int mc_mlp_destroy(mc_mlp* m);
+int mc_mlp_nparams(void);
+/* read (write = 0) or replace (write = 1) the FP32 parameters, in the order above */
+int mc_mlp_params(mc_mlp* m, float* params, int write);
+/* inputs are used as (x - mean) / std, targets as (t - y_mean) / y_std */
+int mc_mlp_set_scaling(mc_mlp* m, const float* x_mean, const float* x_std, float y_mean, float y_std);
+int mc_mlp_get_scaling(const mc_mlp* m, float* x_mean, float* x_std, float* y_mean, float* y_std);
+/* gradient of loss = sum((y - t_scaled)^2) / (2n) over n rows, in parameter order */
+int mc_mlp_grads(mc_mlp* m, const float* X, const float* t, int n, float* grads, float* loss);
+/* Adam training on n rows (X n x 24, t = log alim); epoch_loss (may be NULL) gets the mean
+ * scaled loss of each epoch.  mc_last_kernel_time() then gives the training time. */
+int mc_mlp_train(mc_mlp* m, const float* X, const float* t, int n, const mc_train_opts* o, float* epoch_loss);
+/* predicted log(alim) of n rows */
+int mc_mlp_predict(mc_mlp* m, const float* X, int n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@
 
 #include "../../include/mcsim.h"
 #include "mc_dev.h"
+#include "mc_host.h"
 #include "mc_math.h"
 
 // chatter.hip
@@ -487,6 +488,23 @@ struct StockImpl {
 };
 
 }  // namespace
+
+namespace mch {
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+int init() { return ensure_init(-1); }
+hipStream_t stream() { return g_ctx.stream; }
+std::mutex& lock() { return g_ctx.mu; }
+int cus() { return g_ctx.cus; }
+void set_last_ms(double ms) { g_ctx.last_ms = ms; }
+}  // namespace mch
 
 struct mc_stock : StockImpl {};
 

@@ -1,0 +1,21 @@
+// mc_host.h — library-internal host services shared by the source files of libmcsim.so:
+// error reporting, the per-device context (stream, events, call lock) and timing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+namespace mch {
+int fail(int code, const char* fmt, ...);           // sets mc_last_error(), returns code
+int init();                                         // the implicit mc_init(0)
+hipStream_t stream();                               // the library's stream (default priority)
+std::mutex& lock();                                 // serialises calls on the device
+int cus();                                          // compute units of the device
+void set_last_ms(double ms);                        // mc_last_kernel_time()
+}  // namespace mch
+
+#define MCH_HIPCHK(call)                                                                    \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return mch::fail(MC_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)

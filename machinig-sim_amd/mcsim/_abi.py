@@ -94,3 +94,13 @@ def cell_array(cells):
     for i, c in enumerate(cells):
         arr[i] = c
     return arr
+
+
+MC_SURR_IN = 24
+MC_SURR_HIDDEN = 256
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("lr_final", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("fit_scaling", C.c_int32), ("seed", C.c_uint64)]

@@ -8,7 +8,8 @@ import os
 
 import numpy as np
 
-from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, cell_array, MC_OK)
+from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, TrainOpts, cell_array, MC_OK,
+                   MC_SURR_IN)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -17,7 +18,9 @@ EXPORTS = [
     "mc_init", "mc_last_error", "mc_device_info", "mc_last_kernel_time", "mc_chatter_batch", "mc_bisect_batch",
     "mc_bisect_bracket_batch", "mc_engine_trace", "mc_stock_create", "mc_stock_destroy",
     "mc_stock_dims", "mc_stock_set_wall_band", "mc_stock_read_h", "mc_stock_write_h",
-    "mc_stock_read_side", "mc_stamp_batch", "mc_stock_sample",
+    "mc_stock_read_side", "mc_stamp_batch", "mc_stock_sample", "mc_surrogate_features",
+    "mc_mlp_create", "mc_mlp_destroy", "mc_mlp_nparams", "mc_mlp_params", "mc_mlp_set_scaling",
+    "mc_mlp_get_scaling", "mc_mlp_grads", "mc_mlp_train", "mc_mlp_predict",
 ]
 
 _LIB = None
@@ -55,6 +58,16 @@ def load():
     L.mc_stock_read_side.argtypes = [P, P, P, P, P]
     L.mc_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
     L.mc_stock_sample.argtypes = [P, P, C.c_int, P]
+    L.mc_surrogate_features.argtypes = [C.POINTER(Cell), C.c_int, P]
+    L.mc_mlp_create.argtypes = [C.c_uint64, C.POINTER(P)]
+    L.mc_mlp_destroy.argtypes = [P]
+    L.mc_mlp_nparams.argtypes = []
+    L.mc_mlp_params.argtypes = [P, P, C.c_int]
+    L.mc_mlp_set_scaling.argtypes = [P, P, P, C.c_float, C.c_float]
+    L.mc_mlp_get_scaling.argtypes = [P, P, P, P, P]
+    L.mc_mlp_grads.argtypes = [P, P, P, C.c_int, P, P]
+    L.mc_mlp_train.argtypes = [P, P, P, C.c_int, C.POINTER(TrainOpts), P]
+    L.mc_mlp_predict.argtypes = [P, P, C.c_int, P]
     _LIB = L
     return L
 
@@ -177,4 +190,73 @@ class Stock:
         xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
         out = np.empty(len(xy))
         _check(self.L.mc_stock_sample(self.p, xy.ctypes.data, len(xy), out.ctypes.data))
+        return out
+
+
+def surrogate_features(cells):
+    """the 24 raw surrogate inputs of each cell (n x 24 float32)"""
+    arr, n = _cells(cells)
+    out = np.zeros((n, MC_SURR_IN), dtype=np.float32)
+    _check(load().mc_surrogate_features(arr, n, out.ctypes.data))
+    return out
+
+
+class Mlp:
+    """The bracket surrogate (24 -> 256 -> 256 -> 256 -> 1), trained on the device."""
+
+    def __init__(self, seed=1):
+        self.L = load()
+        self.p = C.c_void_p()
+        _check(self.L.mc_mlp_create(int(seed), C.byref(self.p)))
+        self.nparams = self.L.mc_mlp_nparams()
+
+    def close(self):
+        if self.p:
+            self.L.mc_mlp_destroy(self.p)
+            self.p = None
+
+    __del__ = close
+
+    def params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(self.L.mc_mlp_params(self.p, out.ctypes.data, 0))
+        return out
+
+    def set_params(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        assert v.size == self.nparams
+        _check(self.L.mc_mlp_params(self.p, v.ctypes.data, 1))
+
+    def set_scaling(self, xm, xs, ym, ys):
+        xm = np.ascontiguousarray(xm, dtype=np.float32)
+        xs = np.ascontiguousarray(xs, dtype=np.float32)
+        _check(self.L.mc_mlp_set_scaling(self.p, xm.ctypes.data, xs.ctypes.data, float(ym), float(ys)))
+
+    def scaling(self):
+        xm, xs = np.empty(MC_SURR_IN, np.float32), np.empty(MC_SURR_IN, np.float32)
+        ym, ys = C.c_float(), C.c_float()
+        _check(self.L.mc_mlp_get_scaling(self.p, xm.ctypes.data, xs.ctypes.data, C.byref(ym), C.byref(ys)))
+        return xm, xs, ym.value, ys.value
+
+    def grads(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_float()
+        _check(self.L.mc_mlp_grads(self.p, X.ctypes.data, t.ctypes.data, len(t), g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def train(self, X, t, epochs=100, batch=4096, lr=2e-3, lr_final=5e-5, beta1=0.9, beta2=0.999,
+              eps=1e-8, fit_scaling=True, seed=1):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        o = TrainOpts(int(epochs), int(batch), lr, lr_final, beta1, beta2, eps, int(fit_scaling), int(seed))
+        loss = np.zeros(epochs, dtype=np.float32)
+        _check(self.L.mc_mlp_train(self.p, X.ctypes.data, t.ctypes.data, len(t), C.byref(o), loss.ctypes.data))
+        return loss
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty(len(X), dtype=np.float32)
+        _check(self.L.mc_mlp_predict(self.p, X.ctypes.data, len(X), out.ctypes.data))
         return out
