@@ -133,6 +133,15 @@ int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* ou
  * Results are identical to the serial reference: same trials, same verdicts. */
 int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_limit* out);
 
+/* timeDomainLimit guided by a prior: identical results to mc_bisect_batch (same trials on
+ * the serial path, same verdicts, same bracket), but the speculative search evaluates the
+ * bisection-tree nodes the prior ln(alim) ~ N(mu[i], sig[i]^2) makes most likely to be on
+ * the path, so a good prior (the surrogate's prediction and its residual spread) resolves
+ * more levels per round.  sig[i] <= 0 means no prior for cell i.  No reference counterpart:
+ * it replaces timeDomainLimit (src/stabtd.js:24-32) where a prior is available. */
+int mc_bisect_prior_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, const double* mu,
+                          const double* sig, mc_limit* out);
+
 /* Bracketed search: like mc_bisect_batch, but each cell starts from the bracket
  * [lo_hint[i], hi_hint[i]] (e.g. a surrogate's prediction).  Both ends are checked
  * first; if lo_hint chatters or hi_hint is stable the cell falls back to the full

@@ -21,10 +21,10 @@
 int mcd_group_lds_bytes(int nv_cap, int nz_cap);
 int mcd_launch_chatter(const mcd::LaunchDev& P, int G, int blocks, hipStream_t stream);
 // search.hip
-int mcd_search_lds_bytes(int nv_cap, int nz_cap);
-int mcd_launch_search(const mcd::LaunchDev& P, const mcd::TaskDev* d_tasks, int blocks, hipStream_t stream);
+int mcd_search_lds_bytes(int nz_cap);
+int mcd_launch_search(const mcd::LaunchDev& P, int blocks, hipStream_t stream);
 int mcd_search_maxspec();
-constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;
+constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;   // slots per wave (a wide task: 4 waves x 2)
 // stock.hip
 size_t mcs_stampdev_bytes();
 size_t mcs_chunkbox_bytes();
@@ -79,7 +79,7 @@ struct Ctx {
   int dev = -1;
   int cus = 0;
   std::string name, arch;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // default priority
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   DevBuf cells, tasks, out, queue, ring, hist, trace, geo, gbin;
@@ -285,7 +285,10 @@ int run_tasks(TaskSet& ts, std::vector<mcd::ResultDev>& res, double* h_trace, in
 
 
 // The speculative wave-per-cell search (search.hip): tasks are aMax checks
-// (TASK_SINGLE at aMax) and bisection trees (TASK_BISECT / TASK_BRACKET).
+// (TASK_SINGLE at aMax) and bisection trees (TASK_BISECT / TASK_BRACKET).  The trees of
+// the cells with the longest trials (steps >= MC_WIDE_FRAC x the longest, default 0.5; at
+// most MC_WIDE_MAX of them, default 256) run as wide tasks, four waves each; the
+// persistent workgroups take them first, then one task per wave.
 int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   Ctx& X = g_ctx;
   const int n = (int)ts.tasks.size();
@@ -294,13 +297,32 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts.cost[a] > ts.cost[b]; });
+  // wide tasks: the longest trees
+  const char* wf = getenv("MC_WIDE_FRAC");
+  const double wide_frac = wf && *wf ? atof(wf) : 0.5;
+  const int wide_max = std::max(0, env_int("MC_WIDE_MAX", 256));
+  double smax = 0;
+  for (int q = 0; q < n; q++) {
+    const mcd::CellDev& c = ts.cells[ts.tasks[q].cell];
+    if (ts.tasks[q].mode != mcd::TASK_SINGLE) smax = std::max(smax, (double)c.spt * c.N * c.revs);
+  }
+  std::vector<int> narrow, wide;
+  for (int q = 0; q < n; q++) {
+    const int i = order[q];
+    const mcd::CellDev& c = ts.cells[ts.tasks[i].cell];
+    const bool w = ts.tasks[i].mode != mcd::TASK_SINGLE && (double)c.spt * c.N * c.revs >= wide_frac * smax &&
+                   (int)wide.size() < wide_max;
+    (w ? wide : narrow).push_back(i);
+  }
+  const int nn = (int)narrow.size(), nw = (int)wide.size();
   std::vector<mcd::TaskDev> sorted(n);
   int nz_cap = 1, nv_cap = 1;
   int64_t ring_cap = 1, pe_cap = 1;
   for (int q = 0; q < n; q++) {
-    sorted[q] = ts.tasks[order[q]];
+    const int i = q < nn ? narrow[q] : wide[q - nn];
+    sorted[q] = ts.tasks[i];
     const mcd::CellDev& c = ts.cells[sorted[q].cell];
-    const int nzc = std::max(1, ts.nzcap[order[q]]);
+    const int nzc = std::max(1, ts.nzcap[i]);
     nz_cap = std::max(nz_cap, nzc);
     nv_cap = std::max(nv_cap, nzc * c.N);
     ring_cap = std::max<int64_t>(ring_cap, (int64_t)nzc * c.spr);
@@ -309,10 +331,12 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   const int64_t geo_cap = pe_cap * (nz_cap + mcd_search_maxspec());
   if (geo_cap * 16 >= ((int64_t)1 << 31) || (int64_t)SEARCH_MAXSLOT * ring_cap * 4 >= ((int64_t)1 << 31))
     return fail(MC_ERR_LIMIT, "per-wave delay lines or geometry table exceed 2 GiB");
-  const int wpc = std::max(1, env_int("MC_WAVES_PER_CU", 8));
-  const int waves = (int)std::min<int64_t>(n, (int64_t)X.cus * wpc);
-  const int lds = mcd_search_lds_bytes(nv_cap, nz_cap);
+  const int wpc = std::max(4, env_int("MC_WAVES_PER_CU", 8));
+  const int64_t want = ((int64_t)nn + 4 * (int64_t)nw + 3) / 4;      // workgroups of four waves
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)X.cus * wpc / 4));
+  const int lds = mcd_search_lds_bytes(nz_cap);
   if (lds > 64 * 1024) return fail(MC_ERR_LIMIT, "search LDS %d B too large", lds);
+  const int64_t waves = (int64_t)blocks * 4;
   std::lock_guard<std::mutex> lk(X.mu);
   int st;
   if ((st = X.cells.ensure(ts.cells.size() * sizeof(mcd::CellDev)))) return st;
@@ -325,7 +349,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   if ((st = X.gbin.ensure((size_t)waves * geo_cap))) return st;
   HIPCHK(hipMemcpyAsync(X.cells.p, ts.cells.data(), ts.cells.size() * sizeof(mcd::CellDev), hipMemcpyHostToDevice, X.stream));
   HIPCHK(hipMemcpyAsync(X.tasks.p, sorted.data(), n * sizeof(mcd::TaskDev), hipMemcpyHostToDevice, X.stream));
-  HIPCHK(hipMemsetAsync(X.queue.p, 0, sizeof(int32_t), X.stream));
+  HIPCHK(hipMemsetAsync(X.queue.p, 0, 256, X.stream));
   mcd::LaunchDev P;
   std::memset(&P, 0, sizeof P);
   P.geo = (double2*)X.geo.p;
@@ -335,14 +359,18 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   P.tasks = (const mcd::TaskDev*)X.tasks.p;
   P.out = (mcd::ResultDev*)X.out.p;
   P.queue = (int32_t*)X.queue.p;
+  P.ntasks = nn;
+  P.wtasks = P.tasks + nn;
+  P.wout = P.out + nn;
+  P.wqueue = P.queue + 16;
+  P.nwide = nw;
   P.ring = (float*)X.ring.p;
   P.hist = (float2*)X.hist.p;
   P.ring_cap = ring_cap;
-  P.ntasks = n;
   P.nz_cap = nz_cap;
   P.nv_cap = nv_cap;
   HIPCHK(hipEventRecord(X.ev0, X.stream));
-  if (mcd_launch_search(P, (const mcd::TaskDev*)X.tasks.p, waves, X.stream))
+  if (mcd_launch_search(P, blocks, X.stream))
     return fail(MC_ERR_HIP, "search kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
   HIPCHK(hipEventRecord(X.ev1, X.stream));
   std::vector<mcd::ResultDev> tmp(n);
@@ -351,13 +379,15 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
   X.last_ms += ms;
-  for (int q = 0; q < n; q++) res[order[q]] = tmp[q];
+  for (int q = 0; q < n; q++) res[q < nn ? narrow[q] : wide[q - nn]] = tmp[q];
   if (getenv("MC_TASK_TIMES")) {
     FILE* f = fopen(getenv("MC_TASK_TIMES"), "w");
     if (f) {
-      for (int q = 0; q < n; q++)
-        fprintf(f, "%d %d %lld %lld %d %d\n", ts.tasks[order[q]].cell, ts.tasks[order[q]].mode,
-                (long long)tmp[q].t_start, (long long)tmp[q].t_end, tmp[q].trials, tmp[q].pad);
+      for (int q = 0; q < n; q++) {
+        const int i = q < nn ? narrow[q] : wide[q - nn];
+        fprintf(f, "%d %d %lld %lld %d %d %d\n", ts.tasks[i].cell, ts.tasks[i].mode,
+                (long long)tmp[q].t_start, (long long)tmp[q].t_end, tmp[q].trials, tmp[q].pad, q >= nn);
+      }
       fclose(f);
     }
   }
@@ -394,7 +424,8 @@ void to_limit(const mcd::ResultDev& r, mc_limit& l) {
 // timeDomainLimit for the cells listed in idx: an aMax-check task and a bisection-tree
 // task per cell, combined into the reference's result
 int plain_search(const mc_cell* cells, const TaskSet& base, const std::vector<int>& idx,
-                 const mc_bisect_opts* o, std::vector<mc_limit>& out) {
+                 const mc_bisect_opts* o, std::vector<mc_limit>& out, const double* mu = nullptr,
+                 const double* sig = nullptr) {
   TaskSet ts;
   ts.cells = base.cells;
   const int m = (int)idx.size();
@@ -412,6 +443,7 @@ int plain_search(const mc_cell* cells, const TaskSet& base, const std::vector<in
     std::memset(&t, 0, sizeof t);
     t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = o->iters; t.lo = o->aMin; t.hi = o->aMax; t.tol = -1;
     t.aMin = o->aMin; t.aMax = o->aMax;
+    if (mu && sig && std::isfinite(mu[i]) && sig[i] > 0 && std::isfinite(sig[i])) { t.mu = mu[i]; t.sig = sig[i]; }
     ts.tasks.push_back(t);
     ts.nzcap.push_back(nz_bound(cells[i], o->aMax));
     ts.cost.push_back(steps * 1.5);
@@ -571,6 +603,40 @@ int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_
     for (size_t q = 0; q < spec.size(); q++) out[spec[q]] = lim[q];
   }
   if (serial.empty()) return MC_OK;
+  TaskSet ss;
+  ss.cells = ts.cells;
+  for (int i : serial) {
+    mcd::TaskDev t;
+    std::memset(&t, 0, sizeof t);
+    t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = opts->iters; t.aMin = opts->aMin; t.aMax = opts->aMax;
+    ss.tasks.push_back(t);
+    ss.nzcap.push_back(nz_bound(cells[i], opts->aMax));
+    ss.cost.push_back(cell_cost(ts.cells[i]) * (ss.nzcap.back() + opts->iters * 0.25 * ss.nzcap.back()));
+  }
+  std::vector<mcd::ResultDev> res;
+  if ((st = run_tasks(ss, res, nullptr, 0))) return st;
+  for (size_t q = 0; q < serial.size(); q++) to_limit(res[q], out[serial[q]]);
+  return MC_OK;
+}
+
+int mc_bisect_prior_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, const double* mu,
+                          const double* sig, mc_limit* out) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if ((st = check_opts(opts))) return st;
+  if (n > 0 && (!out || !mu || !sig)) return fail(MC_ERR_ARG, "null prior or output array");
+  TaskSet ts;
+  if ((st = prepare(cells, n, ts))) return st;
+  g_ctx.last_ms = 0;
+  std::vector<int> spec, serial;
+  split_cells(cells, ts, n, opts->aMax, spec, serial);
+  if (!spec.empty()) {
+    std::vector<mc_limit> lim;
+    if ((st = plain_search(cells, ts, spec, opts, lim, mu, sig))) return st;
+    for (size_t q = 0; q < spec.size(); q++) out[spec[q]] = lim[q];
+  }
+  if (serial.empty()) return MC_OK;
+  // the one-trial-per-group kernel runs the serial search; a prior has nothing to guide
   TaskSet ss;
   ss.cells = ts.cells;
   for (int i : serial) {

@@ -36,6 +36,7 @@ struct TaskDev {
   double ap;          // TASK_SINGLE depth
   double aMin, aMax;  // bisection range
   double lo, hi, tol; // TASK_BRACKET hint and stopping width
+  double mu, sig;     // prior on ln(alim) ~ N(mu, sig^2) guiding the speculation; sig <= 0: none
 };
 
 struct ResultDev {
@@ -60,6 +61,10 @@ struct LaunchDev {
   int32_t nz_cap;        // slice capacity of the per-group LDS tables
   int32_t nv_cap;        // (tooth, slice) visit capacity of the per-group LDS tables
   int32_t grp_bytes;     // LDS bytes per group
+  const TaskDev* wtasks; // search kernel: tasks run by a whole workgroup (first), their
+  ResultDev* wout;       //   results, queue and count
+  int32_t* wqueue;
+  int32_t nwide;
   double2* geo;          // edge geometry table (sin, cos), geo_cap entries per wave
   unsigned char* gbin;   // engagement-map bin of each table entry
   int64_t geo_cap;

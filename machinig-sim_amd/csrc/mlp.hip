@@ -668,7 +668,9 @@ int mc_mlp_predict(mc_mlp* m, const float* X, int n, float* out) {
 //   12-15 Ktc, Krc, Kte, Kre
 //   16    helix angle [deg]   17 flutes   18 diameter [mm]
 //   19    immersion: engagement arc / pi   20 log10 rpm   21 revolutions per trial
-//   22-23 modes per axis (x, y)
+//   22-23 lobe phase of the most flexible mode (lowest k over both axes): sin and cos of
+//         2 pi fn / f_tooth, f_tooth = rpm * flutes / 60 (where the stability lobes of
+//         that mode repeat)
 int mc_surrogate_features(const mc_cell* cells, int n, float* out) {
   if (n < 0 || (n > 0 && (!cells || !out))) return mch::fail(MC_ERR_ARG, "bad arguments");
   for (int i = 0; i < n; i++) {
@@ -692,8 +694,13 @@ int mc_surrogate_features(const mc_cell* cells, int n, float* out) {
     f[19] = (float)((c.phiEx - c.phiSt) / 3.141592653589793);
     f[20] = (float)std::log10(c.S > 100 ? c.S : 100.0);
     f[21] = (float)c.revs;
-    f[22] = (float)c.nmx;
-    f[23] = (float)c.nmy;
+    double kmin = INFINITY, fsoft = 0;
+    for (int j = 0; j < c.nmx && j < MC_MAX_MODES; j++) if (c.mx[j].k < kmin) { kmin = c.mx[j].k; fsoft = c.mx[j].fn; }
+    for (int j = 0; j < c.nmy && j < MC_MAX_MODES; j++) if (c.my[j].k < kmin) { kmin = c.my[j].k; fsoft = c.my[j].fn; }
+    const double ftooth = (c.S > 100 ? c.S : 100.0) * c.tool.N / 60.0;
+    const double ph = 2 * 3.141592653589793 * (fsoft / ftooth);
+    f[22] = (float)std::sin(ph);
+    f[23] = (float)std::cos(ph);
     for (int j = 0; j < IN; j++)
       if (!std::isfinite(f[j])) return mch::fail(MC_ERR_ARG, "cell %d: feature %d is not finite", i, j);
   }

@@ -25,6 +25,13 @@
 // instead of evaluating fdlibm sin/cos per visit.  A round needing more distinct lags than
 // the table holds (possible only when the depth changes the slice pitch) aborts the task
 // with fell_back = 2, and the host runs that cell on the one-trial-per-group kernel.
+//
+// Wide tasks: a workgroup is four waves.  It first takes the wide tasks (the trees of the
+// cells with the longest trials) one at a time with all four waves, up to 2 slots per wave:
+// the round plan and the verdicts live in wave 0's task block, every wave simulates its
+// own slots, and the waves meet at a barrier to walk the tree, so deep trials, which leave
+// room for one or two slots in a single wave, get 8 slots of 32 lanes.  Once no wide task
+// is left, each wave takes ordinary tasks on its own (up to 8 slots per wave).
 #include <hip/hip_runtime.h>
 
 #include "mc_dev.h"
@@ -35,7 +42,7 @@ using namespace mcd;
 namespace {
 
 constexpr int MAXP = 2;        // passes of a slot's lanes per step
-constexpr int MAXSLOT = 8;     // slots of a wave
+constexpr int MAXNODE = 8;     // trials of a task per round (over all its waves)
 constexpr int MAXSPEC = 16;    // per-round table columns for partial / non-standard slices
 
 __device__ __forceinline__ void wave_sync() {
@@ -66,17 +73,31 @@ struct CellCst {
   double dR[MC_MAX_FLUTES];
 };
 
+// per wave: cell constants, the modal state, samples and engagement map of its slots
+template <int SPW>
 struct WaveLds {
   CellCst K;
-  double mq[MAXSLOT][2 * MC_MAX_MODES], mv[MAXSLOT][2 * MC_MAX_MODES];
-  double node_ap[MAXSLOT];
-  double idx[MAXSLOT], amp[MAXSLOT], once[MAXSLOT];
+  double mq[SPW][2 * MC_MAX_MODES], mv[SPW][2 * MC_MAX_MODES];
   double spec_lag[MAXSPEC];
-  long long eng[MAXSLOT];
-  int nz[MAXSLOT];
   float Hb[MC_BINS];
-  float tsx[MAXSLOT][TS], tsy[MAXSLOT][TS];  // tooth-passing samples of each slot
-  int task, nspec, overflow;
+  float tsx[SPW][TS], tsy[SPW][TS];  // tooth-passing samples of each slot
+  int nspec, overflow;
+};
+
+// per task (workgroup): the task index, the round plan (bisection-tree nodes in best-first
+// order, their intervals and the indices of their evaluated children, -1 if none), the
+// best-first frontier (thread 0's working storage) and the verdicts of the round's trials
+struct TaskLds {
+  int task;
+  double node_ap[MAXNODE], node_lo[MAXNODE], node_hi[MAXNODE];
+  int node_kid[MAXNODE][2];
+  int plan_n, plan_sg, plan_spw, plan_first, plan_ntree, plan_evlo, plan_evhi;
+  double fr_lo[2 * MAXNODE + 2], fr_hi[2 * MAXNODE + 2], fr_p[2 * MAXNODE + 2];
+  int fr_par[2 * MAXNODE + 2], fr_side[2 * MAXNODE + 2], fr_d[2 * MAXNODE + 2];
+  double idx[MAXNODE], amp[MAXNODE], once[MAXNODE];
+  long long eng[MAXNODE];
+  int nz[MAXNODE];
+  int overflow[8];
 };
 
 // Behind WaveLds the wave's LDS holds, at different times of a round, either the slice
@@ -93,11 +114,26 @@ struct SlotLds {
 };
 constexpr int NREC = 64 * MAXP;  // engaged-visit records of a wave (a slot of SG lanes owns SG*MAXP)
 
-__device__ __forceinline__ size_t wave_bytes() { return (sizeof(WaveLds) + 15) & ~(size_t)15; }
-__device__ __host__ __forceinline__ size_t slot_bytes_of(int nz_cap) {
-  return (8 * 5 * (size_t)nz_cap + 4 * (size_t)nz_cap + 15) & ~(size_t)15;
-}
+__device__ __host__ __forceinline__ size_t task_bytes() { return (sizeof(TaskLds) + 15) & ~(size_t)15; }
+template <int SPW>
+__device__ __host__ __forceinline__ size_t wave_bytes() { return (sizeof(WaveLds<SPW>) + 15) & ~(size_t)15; }
+// a slot's set-up records: up to MAXP*SG slices (a trial's N*nz visits fit its MAXP*SG
+// lane passes), 5 doubles and 1 int each; the slots of a wave together stay within the
+// step buffers (MAXP*64 slices)
+__device__ __forceinline__ size_t slot_stride(int SG) { return (size_t)MAXP * SG * 44; }
 __device__ __host__ __forceinline__ size_t step_bytes() { return (size_t)NREC * (16 + 5 * 8); }
+// a wave's LDS: its task block, its wave block, then its step buffers (which the set-up
+// records of its slots share: 64*MAXP*44 bytes < step_bytes())
+__device__ __host__ __forceinline__ size_t per_wave_bytes() {
+  return task_bytes() + wave_bytes<8>() + step_bytes();
+}
+static_assert(64 * MAXP * 44 <= NREC * (16 + 5 * 8), "set-up records fit the step buffers");
+
+template <int WW>
+__device__ __forceinline__ void task_sync() {
+  if (WW > 1) __syncthreads();
+  else wave_sync();
+}
 
 // one table entry: sin, cos and bin of the edge angle at (pr, tooth i, lag)
 __device__ __forceinline__ void edge_geometry(double theta, double ipitch, double lag, double binScale,
@@ -108,44 +144,78 @@ __device__ __forceinline__ void edge_geometry(double theta, double ipitch, doubl
   bin = (int)(phi * binScale);
 }
 
+// P(ln a < ln alim < ln b) under N(mu, sig^2), with tails taken from erfc for precision
+__device__ __forceinline__ double prior_mass(double a, double b, double mu, double sig) {
+  const double xa = (::log(a) - mu) / (sig * 1.4142135623730951);
+  const double xb = (::log(b) - mu) / (sig * 1.4142135623730951);
+  if (xa > 0) return 0.5 * (erfc(xa) - erfc(xb));
+  if (xb < 0) return 0.5 * (erfc(-xb) - erfc(-xa));
+  return 1.0 - 0.5 * erfc(xb) - 0.5 * erfc(-xa);
+}
+
+// Up to kmax nodes of the bisection tree below [lo, hi], best first by the probability
+// that the serial bisection visits them (the prior mass of the node's interval; without a
+// prior, 2^-depth, i.e. breadth first).  Only nodes the serial search could still evaluate
+// (iteration budget and stopping width) are taken.  Any prefix of the order is a subtree
+// containing the root.  Runs in one lane; returns the node count.
+__device__ int plan_tree(TaskLds& W, double lo, double hi, int it, int iters, double tol, double mu, double sig,
+                         int kmax) {
+  int nf = 0, n = 0;
+  W.fr_lo[0] = lo; W.fr_hi[0] = hi; W.fr_p[0] = 1.0; W.fr_par[0] = -1; W.fr_side[0] = 0; W.fr_d[0] = 0;
+  nf = 1;
+  while (n < kmax && nf > 0) {
+    int b = 0;
+    for (int f = 1; f < nf; f++)
+      if (W.fr_p[f] > W.fr_p[b] || (W.fr_p[f] == W.fr_p[b] && W.fr_d[f] < W.fr_d[b])) b = f;
+    const double a = W.fr_lo[b], c = W.fr_hi[b], m = __builtin_sqrt(a * c);
+    const int d = W.fr_d[b], par = W.fr_par[b], side = W.fr_side[b];
+    W.node_lo[n] = a; W.node_hi[n] = c; W.node_ap[n] = m;
+    W.node_kid[n][0] = W.node_kid[n][1] = -1;
+    if (par >= 0) W.node_kid[par][side] = n;
+    nf--;
+    W.fr_lo[b] = W.fr_lo[nf]; W.fr_hi[b] = W.fr_hi[nf]; W.fr_p[b] = W.fr_p[nf];
+    W.fr_par[b] = W.fr_par[nf]; W.fr_side[b] = W.fr_side[nf]; W.fr_d[b] = W.fr_d[nf];
+    // children: chatter at m -> [a, m], stable -> [m, c]
+    for (int sd = 0; sd < 2; sd++) {
+      const double ca = sd ? m : a, cc = sd ? c : m;
+      if (!(it + d + 1 < iters && cc - ca > tol)) continue;
+      W.fr_lo[nf] = ca; W.fr_hi[nf] = cc;
+      W.fr_p[nf] = sig > 0 ? prior_mass(ca, cc, mu, sig) : __builtin_ldexp(1.0, -(d + 1));
+      W.fr_par[nf] = n; W.fr_side[nf] = sd; W.fr_d[nf] = d + 1;
+      nf++;
+    }
+    n++;
+  }
+  return n;
+}
+
 }  // namespace
 
-// LDS of a wave: the per-wave block, then the larger of the set-up slice records of all
-// slots and the step loop's visit records and force terms
-int mcd_search_lds_bytes(int nv_cap, int nz_cap) {
-  (void)nv_cap;
-  const size_t wave = (sizeof(WaveLds) + 15) & ~(size_t)15;
-  const size_t setup = MAXSLOT * slot_bytes_of(nz_cap);
-  return (int)(wave + (setup > step_bytes() ? setup : step_bytes()));
+// LDS of a workgroup: per wave a task block, a wave block and the step buffers
+int mcd_search_lds_bytes(int nz_cap) {
+  (void)nz_cap;
+  return (int)(4 * per_wave_bytes());
 }
 int mcd_search_maxspec() { return MAXSPEC; }
 
-__global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __restrict__ tasks) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  WaveLds& W = *reinterpret_cast<WaveLds*>(lds);
-  const size_t slot_bytes = slot_bytes_of(P.nz_cap);
-  unsigned char* const area = lds + wave_bytes();
+// one task: WW waves (wv = this wave's index among them), up to SPW slots per wave
+template <int WW, int SPW>
+__device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __restrict__ tasks,
+                                         ResultDev* __restrict__ outp, int task, TaskLds& TL,
+                                         WaveLds<8>& W, unsigned char* const area, const int wv,
+                                         const int lane, float* const ring_base, float2* const hist_base,
+                                         double2* const gsc, unsigned char* const gbn) {
   double2* const terms = reinterpret_cast<double2*>(area);
   double* const rec_h = reinterpret_cast<double*>(area + 16 * NREC);
   double* const rec_sn = rec_h + NREC;
   double* const rec_cs = rec_sn + NREC;
   double* const rec_b = rec_cs + NREC;
   double* const rec_pd = rec_b + NREC;
-  const int lane = threadIdx.x;
-  const int64_t wid = blockIdx.x;
-  float* ring_base = P.ring + wid * (int64_t)MAXSLOT * P.ring_cap;
-  float2* hist_base = P.hist + wid * (int64_t)MAXSLOT * HIST;
-  double2* const gsc = P.geo + wid * P.geo_cap;
   const char* const gsc_b = reinterpret_cast<const char*>(gsc);
   char* const ring_b = reinterpret_cast<char*>(ring_base);
-  unsigned char* const gbn = P.gbin + wid * P.geo_cap;
   const int NL = P.nz_cap + MAXSPEC;   // table columns: full slices, then per-round ones
-
-  while (true) {
-    if (lane == 0) W.task = atomicAdd(P.queue, 1);
-    wave_sync();
-    const int task = uni(W.task);
-    if (task >= P.ntasks) break;
+  const bool lead = lane == 0 && wv == 0;  // plans the rounds and writes the result
+  {
     const long long t_start = wall_clock64();
     const TaskDev T = tasks[task];
     const int cell = uni(T.cell);
@@ -180,64 +250,79 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
     int last_nz = 0;
 
     while (true) {
-      // ================= this round's trials and slot size (computed identically by all lanes)
-      int nnode = 0, L = 0, SG = 64;
-      bool ev_lo = false, ev_hi = false;
-      if (amax_task) {
-        W.node_ap[0] = T.ap;
-        nnode = 1;
-      } else {
-        if (!pend_lo && !pend_hi && !(it < iters && hi - lo > tol)) break;
+      // ================= this round's trials and slot size (planned by thread 0)
+      if (!amax_task && !pend_lo && !pend_hi && !(it < iters && hi - lo > tol)) break;
+      if (lead) {
+        // slot layouts: S slots over the task's waves, max(1, S / WW) per wave
+        int nn = 0, sgp = 64, spwp = 1, first = 0, ntree = 0, elo = 0, ehi = 0;
         double dzz;
-        const int nv_lo = pend_lo ? N * slices_at(C.toolFluteLen, lo, dzz) : 0;
-        const int nv_hi = pend_hi ? N * slices_at(C.toolFluteLen, hi, dzz) : 0;
-        // hint ends first; the tree is speculated only once no end stays pending
-        for (int mode = 0; mode < 3 && nnode == 0; mode++) {
-          const bool use_lo = pend_lo, use_hi = pend_hi && (mode < 2 || !pend_lo);
-          if (mode == 2 && !(pend_lo && pend_hi)) break;
-          const bool lo_only = mode == 2;
-          const int ne = lo_only ? 1 : (int)use_lo + (int)use_hi;
-          for (int cand = (lo_only ? 0 : 3); cand >= 0; cand--) {
-            if (mode == 1 && cand > 0) continue;
-            const int tree = (1 << cand) - 1;
-            const int need = ne + tree;
-            if (need == 0) continue;
-            int slots = 1;
-            while (slots < need) slots <<= 1;
-            if (slots > MAXSLOT) continue;
-            const int sg = 64 / slots;
-            // node intervals in breadth-first order: children of j are 2j+1 (chatter at the
-            // midpoint: bracket [lo, m]) and 2j+2 (stable: [m, hi])
-            double nl[MAXSLOT], nh[MAXSLOT], na[MAXSLOT];
-            nl[0] = lo; nh[0] = hi;
-            for (int j = 0; j < tree; j++) {
-              const double m = __builtin_sqrt(nl[j] * nh[j]);
-              na[j] = m;
-              if (2 * j + 2 < tree) { nl[2 * j + 1] = nl[j]; nh[2 * j + 1] = m; nl[2 * j + 2] = m; nh[2 * j + 2] = nh[j]; }
+        if (amax_task) {
+          TL.node_ap[0] = T.ap;
+          nn = 1;
+        } else if (pend_lo || pend_hi) {
+          // hint ends first (both, else the high one alone, else the low one alone), with
+          // the first tree levels below the hint speculated where lanes allow
+          const int nv_lo = pend_lo ? N * slices_at(C.toolFluteLen, lo, dzz) : 0;
+          const int nv_hi = pend_hi ? N * slices_at(C.toolFluteLen, hi, dzz) : 0;
+          for (int mode = 0; mode < 3 && nn == 0; mode++) {
+            const bool use_lo = pend_lo && mode != 1, use_hi = pend_hi && mode != 2;
+            if (mode > 0 && !(pend_lo && pend_hi)) break;
+            const int ne = (int)use_lo + (int)use_hi;
+            for (int ki = (mode == 0 ? 0 : 2); ki < 3; ki++) {
+              const int k = ki == 0 ? 6 : ki == 1 ? 2 : 0;   // tree nodes beside the ends
+              const int need = ne + k;
+              int slots = 1;
+              while (slots < need) slots <<= 1;
+              if (slots > WW * SPW) continue;
+              const int spw = slots > WW ? slots / WW : 1, sg = 64 / spw;
+              const int nt = k ? plan_tree(TL, lo, hi, it, iters, tol, T.mu, T.sig, k) : 0;
+              int nvmax = max(use_lo ? nv_lo : 0, use_hi ? nv_hi : 0);
+              for (int j = 0; j < nt; j++) nvmax = max(nvmax, N * slices_at(C.toolFluteLen, TL.node_ap[j], dzz));
+              if (nvmax <= MAXP * sg && nvmax <= P.nv_cap) {
+                // tree nodes move behind the ends
+                for (int j = nt - 1; j >= 0; j--) {
+                  TL.node_ap[ne + j] = TL.node_ap[j]; TL.node_lo[ne + j] = TL.node_lo[j]; TL.node_hi[ne + j] = TL.node_hi[j];
+                  TL.node_kid[ne + j][0] = TL.node_kid[j][0]; TL.node_kid[ne + j][1] = TL.node_kid[j][1];
+                }
+                int q = 0;
+                if (use_lo) { TL.node_ap[q++] = lo; elo = 1; }
+                if (use_hi) { TL.node_ap[q++] = hi; ehi = 1; }
+                nn = need; sgp = sg; spwp = spw; first = ne; ntree = nt;
+                break;
+              }
             }
-            int nvmax = lo_only ? nv_lo : max(use_lo ? nv_lo : 0, use_hi ? nv_hi : 0);
-            for (int j = 0; j < tree; j++) nvmax = max(nvmax, N * slices_at(C.toolFluteLen, na[j], dzz));
-            if (nvmax > MAXP * sg || nvmax > P.nv_cap) continue;
-            L = cand; SG = sg; nnode = need;
-            int q = 0;
-            if (use_lo) { W.node_ap[q++] = lo; ev_lo = true; }
-            if (use_hi && !lo_only) { W.node_ap[q++] = hi; ev_hi = true; }
-            for (int j = 0; j < tree; j++) W.node_ap[q + j] = na[j];
-            break;
+          }
+        } else {
+          // the largest best-first prefix of the tree whose trials fit the lanes
+          const int nt = plan_tree(TL, lo, hi, it, iters, tol, T.mu, T.sig, WW * SPW);
+          for (int slots = WW * SPW; slots >= 1 && nn == 0; slots >>= 1) {
+            const int k = min(slots, nt);
+            const int spw = slots > WW ? slots / WW : 1, sg = 64 / spw;
+            int nvmax = 0;
+            for (int j = 0; j < k; j++) nvmax = max(nvmax, N * slices_at(C.toolFluteLen, TL.node_ap[j], dzz));
+            if (k >= 1 && nvmax <= MAXP * sg && nvmax <= P.nv_cap) { nn = k; sgp = sg; spwp = spw; ntree = k; }
           }
         }
-        if (nnode == 0) { failed = true; break; }  // the host sizes the trees so this never happens
+        TL.plan_n = nn; TL.plan_sg = sgp; TL.plan_spw = spwp; TL.plan_first = first; TL.plan_ntree = ntree;
+        TL.plan_evlo = elo; TL.plan_evhi = ehi;
       }
-      wave_sync();
-      const int nslot = nnode;
-      const int slot = lane / SG, g = lane % SG;
-      const bool in_slot = slot < nslot;
-      const int sl = in_slot ? slot : 0;
+      task_sync<WW>();
+      const int nnode = uni(TL.plan_n), SG = uni(TL.plan_sg), spw = uni(TL.plan_spw);
+      const int tfirst = uni(TL.plan_first), ntree = uni(TL.plan_ntree);
+      const bool ev_lo = uni(TL.plan_evlo) != 0, ev_hi = uni(TL.plan_evhi) != 0;
+      if (nnode == 0) { failed = true; break; }  // the host sizes the trees so this never happens
+      // this wave runs the task's slots w*spw .. w*spw + spw - 1 (lslot is the local index)
+      const int lslot = lane / SG, g = lane % SG;
+      const int gslot = wv * spw + lslot;
+      const bool in_slot = lslot < spw && gslot < nnode;
+      const int sl = in_slot ? lslot : 0;
+      const int nloc = max(0, min(spw, nnode - wv * spw));   // this wave's slots
       SlotLds S;
       {
-        double* d = reinterpret_cast<double*>(area + (size_t)sl * slot_bytes);
-        S.sLag = d; d += P.nz_cap; S.sZ = d; d += P.nz_cap; S.sB = d; d += P.nz_cap;
-        S.sKap = d; d += P.nz_cap; S.sR = d; d += P.nz_cap;
+        double* d = reinterpret_cast<double*>(area + (size_t)sl * slot_stride(SG));
+        const int cap = MAXP * SG;
+        S.sLag = d; d += cap; S.sZ = d; d += cap; S.sB = d; d += cap;
+        S.sKap = d; d += cap; S.sR = d; d += cap;
         S.lagid = reinterpret_cast<int*>(d);
       }
       const int rbase = sl * SG * MAXP;  // this slot's visit records and force terms
@@ -248,7 +333,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       float2* hist = hist_base + sl * HIST;
 
       // ---- fresh engine per slot at its depth (configure + setEngagement)
-      const double ap = in_slot ? W.node_ap[slot] : 0.0;
+      const double ap = in_slot ? TL.node_ap[gslot] : 0.0;
       double dz = dz0;
       const int nz = in_slot ? slices_at(C.toolFluteLen, ap, dz) : 0;
       const int NV = N * nz;
@@ -267,25 +352,24 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         S.sLag[k] = zr * C.tanB / C.R;
       }
       for (int j = g; j < nz * spr; j += SG) ring[j] = 0.0f;
-      if (g == 0 && in_slot) W.eng[slot] = 0;
       wave_sync();
 
       // ---- table columns: full slices k carry lag(k*dz0 + dz0/2); the other slices get
       // per-round columns (deduplicated bit for bit) or, past MAXSPEC, direct evaluation
       int kneed = 0;
-      for (int s2 = 0; s2 < nslot; s2++) {
+      for (int s2 = 0; s2 < nloc; s2++) {
         double dd = dz0;
-        kneed = max(kneed, slices_at(C.toolFluteLen, W.node_ap[s2], dd));
+        kneed = max(kneed, slices_at(C.toolFluteLen, TL.node_ap[wv * spw + s2], dd));
       }
       kneed = min(kneed, P.nz_cap);
       if (lane == 0) {
         int ns = 0;
-        for (int s2 = 0; s2 < nslot; s2++) {
-          double* d = reinterpret_cast<double*>(area + (size_t)s2 * slot_bytes);
+        for (int s2 = 0; s2 < nloc; s2++) {
+          double* d = reinterpret_cast<double*>(area + (size_t)s2 * slot_stride(SG));
           const double* lag2 = d;
-          int* id2 = reinterpret_cast<int*>(d + 5 * P.nz_cap);
+          int* id2 = reinterpret_cast<int*>(d + 5 * MAXP * SG);
           double dd = dz0;
-          const int nz2 = slices_at(C.toolFluteLen, W.node_ap[s2], dd);
+          const int nz2 = slices_at(C.toolFluteLen, TL.node_ap[wv * spw + s2], dd);
           for (int k = 0; k < nz2; k++) {
             const double full = (k * dz0 + 0.5 * dz0) * C.tanB / C.R;
             int id = -1;
@@ -300,18 +384,21 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           }
         }
         W.nspec = ns;
-        W.overflow = 0;
-        for (int s2 = 0; s2 < nslot; s2++) {
-          double* d = reinterpret_cast<double*>(area + (size_t)s2 * slot_bytes);
-          const int* id2 = reinterpret_cast<int*>(d + 5 * P.nz_cap);
+        int ovf = 0;
+        for (int s2 = 0; s2 < nloc; s2++) {
+          double* d = reinterpret_cast<double*>(area + (size_t)s2 * slot_stride(SG));
+          const int* id2 = reinterpret_cast<int*>(d + 5 * MAXP * SG);
           double dd = dz0;
-          const int nz2 = slices_at(C.toolFluteLen, W.node_ap[s2], dd);
-          for (int k = 0; k < nz2; k++) if (id2[k] < 0) W.overflow = 1;
+          const int nz2 = slices_at(C.toolFluteLen, TL.node_ap[wv * spw + s2], dd);
+          for (int k = 0; k < nz2; k++) if (id2[k] < 0) ovf = 1;
         }
+        TL.overflow[wv] = ovf;
       }
-      wave_sync();
+      task_sync<WW>();
       const int nspec = uni(W.nspec);
-      if (uni(W.overflow)) { failed = true; tab_overflow = true; break; }
+      int any_ovf = 0;
+      for (int v = 0; v < WW; v++) any_ovf |= TL.overflow[v];
+      if (uni(any_ovf)) { failed = true; tab_overflow = true; break; }
       {
         // extend the full-slice columns, then (re)build this round's columns
         const int kf_new = max(kf_built, kneed);
@@ -400,7 +487,8 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       }
 
       wave_sync();  // the slice records are read; the step loop reuses their LDS
-      for (int s = 0; s < steps; s++) {
+      const int wsteps = nloc > 0 ? steps : 0;  // a wave without slots this round skips the steps
+      for (int s = 0; s < wsteps; s++) {
         const double xm = x * 1e3, ym = y * 1e3;  // rigid workpiece
         const double rvx = vxs, rvy = vys;
         const int prn = pr + 1 == spr ? 0 : pr + 1;
@@ -524,7 +612,7 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       }
 
       // ---- chatter index per slot (src/cutting.js:299-318)
-      if (in_slot && g == 0) W.eng[slot] = my_eng;
+      if (in_slot && g == 0) TL.eng[gslot] = my_eng;
       if (g == 0 && in_slot) {
         double index = 0, amp = 0, once = 0;
         const int n = tcount < 24 ? tcount : 24;
@@ -552,19 +640,19 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
           amp = __builtin_sqrt(w / mm2);
           index = once / (amp + 0.05);
         }
-        W.idx[slot] = index; W.amp[slot] = amp; W.once[slot] = once; W.nz[slot] = nz;
+        TL.idx[gslot] = index; TL.amp[gslot] = amp; TL.once[gslot] = once; TL.nz[gslot] = nz;
       }
-      wave_sync();
-      x_trials += nslot;
+      task_sync<WW>();
+      x_trials += nnode;
 
       // ---- walk the tree with the verdicts (wave-uniform)
       auto take = [&](int j) {
         p_trials++;
         p_steps += steps;
-        p_visits += (long long)N * W.nz[j] * steps;
-        p_eng += W.eng[j];
-        last_index = W.idx[j]; last_amp = W.amp[j]; last_once = W.once[j]; last_nz = W.nz[j];
-        last_chat = W.idx[j] > MC_CHATTER_IDX;
+        p_visits += (long long)N * TL.nz[j] * steps;
+        p_eng += TL.eng[j];
+        last_index = TL.idx[j]; last_amp = TL.amp[j]; last_once = TL.once[j]; last_nz = TL.nz[j];
+        last_chat = TL.idx[j] > MC_CHATTER_IDX;
       };
       if (amax_task) {
         take(0);
@@ -581,20 +669,23 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
         take(q++);                     // chatterAt(hi): must chatter
         if (!last_chat) { failed = true; break; }
       }
-      if (pend_lo || pend_hi) { wave_sync(); continue; }
-      int j = 0;
-      for (int l = 0; l < L; l++) {
+      // tree nodes evaluated this round: follow the verdicts from the root while the
+      // serial search's next node was among them
+      int j = ntree > 0 ? 0 : -1;
+      while (j >= 0) {
         if (!(it < iters && hi - lo > tol)) break;
-        take(q + j);
-        const double m = W.node_ap[q + j];
-        if (last_chat) { hi = m; j = 2 * j + 1; } else { lo = m; j = 2 * j + 2; }
+        take(tfirst + j);
+        const double m = TL.node_ap[tfirst + j];
+        int nx;
+        if (last_chat) { hi = m; nx = TL.node_kid[tfirst + j][0]; } else { lo = m; nx = TL.node_kid[tfirst + j][1]; }
         it++;
+        j = nx >= 0 && nx < ntree ? nx : -1;
       }
-      wave_sync();
+      task_sync<WW>();  // the next round's plan overwrites the shared block
     }
 
-    if (lane == 0) {
-      ResultDev& R = P.out[task];
+    if (lead) {
+      ResultDev& R = outp[task];
       R.alim = __builtin_sqrt(lo * hi);
       R.lo = lo; R.hi = hi;
       R.index = last_index; R.amp = last_amp; R.once = last_once;
@@ -603,12 +694,45 @@ __global__ __launch_bounds__(64) void cell_search(LaunchDev P, const TaskDev* __
       R.steps = p_steps; R.visits = p_visits; R.engaged = p_eng;
       R.t_start = t_start; R.t_end = wall_clock64();
     }
-    wave_sync();
+    task_sync<WW>();  // the next task index overwrites the shared block
   }
 }
 
-int mcd_launch_search(const LaunchDev& P, const TaskDev* d_tasks, int blocks, hipStream_t stream) {
-  const size_t lds = (size_t)mcd_search_lds_bytes(P.nv_cap, P.nz_cap);
-  hipLaunchKernelGGL(cell_search, dim3(blocks), dim3(64), lds, stream, P, d_tasks);
+// Persistent workgroups of four waves: the wide tasks first (a whole workgroup each),
+// then one task per wave until the queue is empty.
+__global__ __launch_bounds__(256) void cell_search(LaunchDev P) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned char* const wblk = lds + (size_t)w * per_wave_bytes();
+  TaskLds& TLw = *reinterpret_cast<TaskLds*>(wblk);
+  WaveLds<8>& W = *reinterpret_cast<WaveLds<8>*>(wblk + task_bytes());
+  unsigned char* const area = wblk + task_bytes() + wave_bytes<8>();
+  TaskLds& TL0 = *reinterpret_cast<TaskLds*>(lds);   // wave 0's task block: shared in wide tasks
+  const int64_t wid = (int64_t)blockIdx.x * 4 + w;   // this wave's delay lines, rings, table
+  float* const ring_base = P.ring + wid * (int64_t)8 * P.ring_cap;
+  float2* const hist_base = P.hist + wid * (int64_t)8 * HIST;
+  double2* const gsc = P.geo + wid * P.geo_cap;
+  unsigned char* const gbn = P.gbin + wid * P.geo_cap;
+  while (true) {
+    if (threadIdx.x == 0) TL0.task = atomicAdd(P.wqueue, 1);
+    __syncthreads();
+    const int task = uni(TL0.task);
+    __syncthreads();  // every wave has the index before wave 0 may reuse its block
+    if (task >= P.nwide) break;
+    run_task<4, 2>(P, P.wtasks, P.wout, task, TL0, W, area, w, lane, ring_base, hist_base, gsc, gbn);
+  }
+  while (true) {
+    if (lane == 0) TLw.task = atomicAdd(P.queue, 1);
+    wave_sync();
+    const int task = uni(TLw.task);
+    if (task >= P.ntasks) break;
+    run_task<1, 8>(P, P.tasks, P.out, task, TLw, W, area, 0, lane, ring_base, hist_base, gsc, gbn);
+  }
+}
+
+// blocks = persistent workgroups of four waves
+int mcd_launch_search(const LaunchDev& P, int blocks, hipStream_t stream) {
+  const size_t lds = (size_t)mcd_search_lds_bytes(P.nz_cap);
+  hipLaunchKernelGGL(cell_search, dim3(blocks), dim3(256), lds, stream, P);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

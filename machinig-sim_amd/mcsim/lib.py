@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG, "libmcsim.so")
 
 EXPORTS = [
     "mc_init", "mc_last_error", "mc_device_info", "mc_last_kernel_time", "mc_chatter_batch", "mc_bisect_batch",
-    "mc_bisect_bracket_batch", "mc_engine_trace", "mc_stock_create", "mc_stock_destroy",
+    "mc_bisect_bracket_batch", "mc_bisect_prior_batch", "mc_engine_trace", "mc_stock_create", "mc_stock_destroy",
     "mc_stock_dims", "mc_stock_set_wall_band", "mc_stock_read_h", "mc_stock_write_h",
     "mc_stock_read_side", "mc_stamp_batch", "mc_stock_sample", "mc_surrogate_features",
     "mc_mlp_create", "mc_mlp_destroy", "mc_mlp_nparams", "mc_mlp_params", "mc_mlp_set_scaling",
@@ -47,6 +47,7 @@ def load():
     L.mc_bisect_batch.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), C.POINTER(Limit)]
     L.mc_bisect_bracket_batch.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), P, P,
                                           C.c_double, C.POINTER(Limit), P]
+    L.mc_bisect_prior_batch.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), P, P, C.POINTER(Limit)]
     L.mc_engine_trace.argtypes = [C.POINTER(Cell), C.c_double, C.c_int64, P, P, P, P,
                                   C.POINTER(C.c_int64)]
     L.mc_stock_create.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(P)]
@@ -125,6 +126,17 @@ def bracket_batch(cells, lo, hi, tol, aMin=0.05, aMax=10.0, iters=10):
     _check(load().mc_bisect_bracket_batch(arr, n, C.byref(o), lo.ctypes.data, hi.ctypes.data,
                                           float(tol), out, fb.ctypes.data))
     return out, fb
+
+
+def prior_batch(cells, mu, sig, aMin=0.05, aMax=10.0, iters=10):
+    """timeDomainLimit guided by a prior ln(alim) ~ N(mu, sig^2); same results as bisect_batch"""
+    arr, n = _cells(cells)
+    o = BisectOpts(aMin, aMax, iters, 0)
+    mu = np.ascontiguousarray(np.broadcast_to(mu, (n,)), dtype=np.float64)
+    sig = np.ascontiguousarray(np.broadcast_to(sig, (n,)), dtype=np.float64)
+    out = (Limit * n)()
+    _check(load().mc_bisect_prior_batch(arr, n, C.byref(o), mu.ctypes.data, sig.ctypes.data, out))
+    return out
 
 
 def engine_trace(cell, ap, nsteps):

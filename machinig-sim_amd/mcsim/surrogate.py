@@ -17,7 +17,7 @@ from . import lib, workloads as W
 
 def sample_cells(n, seed=2, n_tools=None):
     """n cells: tools from grid_tools (seed), speeds uniform in [2000, 30000] rpm"""
-    n_tools = n_tools or max(16, n // 64)
+    n_tools = n_tools or max(16, n // 16)
     tools = W.grid_tools(n_tools, seed)
     rng = W.ParkMiller(seed * 7919 + 13)
     cells = []
@@ -54,3 +54,16 @@ def brackets(model, cells, width=0.15, aMin=0.05, aMax=10.0):
 def bracketed_search(model, cells, tol=0.05, width=0.15, **opts):
     lo, hi = brackets(model, cells, width, opts.get("aMin", 0.05), opts.get("aMax", 10.0))
     return lib.bracket_batch(cells, lo, hi, tol, **opts)
+
+
+def residual_sigma(model, X, y):
+    """spread of ln(alim) - prediction on held-out rows"""
+    r = y.astype(np.float64) - model.predict(X).astype(np.float64)
+    return float(np.sqrt(np.mean(r * r)))
+
+
+def guided_search(model, cells, sigma, **opts):
+    """timeDomainLimit with the surrogate as the speculation prior: identical results to
+    the plain search, fewer speculative rounds when the prediction is good"""
+    mu = model.predict(lib.surrogate_features(cells)).astype(np.float64)
+    return lib.prior_batch(cells, mu, np.full(len(cells), sigma), **opts)

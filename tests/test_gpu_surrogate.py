@@ -110,3 +110,22 @@ def test_surrogate_bracketed_search(lib):
           % (hit / len(test), agree / max(hit, 1), tp, ts), flush=True)
     assert hit > 0.3 * len(test)
     assert agree >= 0.8 * hit
+
+
+def test_prior_guided_search_is_identical_to_plain(lib):
+    """any prior (exact, noisy, wrong, absent) changes only which tree nodes are
+    speculated, never the result: limits, brackets and serial-path counts equal the plain
+    search's"""
+    from mcsim import workloads as W
+    cells, opts = W.sld_grid(n_speeds=16, n_tools=12, seed=5)
+    plain = lib.bisect_batch(cells, **opts)
+    n = len(cells)
+    rng = np.random.default_rng(4)
+    exact = np.array([math.log(r.alim) for r in plain])
+    for mu, sig in ((exact, np.full(n, 0.05)), (exact + rng.normal(0, 0.3, n), np.full(n, 0.3)),
+                    (np.full(n, math.log(9.5)), np.full(n, 0.01)), (exact, np.zeros(n))):
+        got = lib.prior_batch(cells, mu, sig, **opts)
+        for i in range(n):
+            a, b = got[i], plain[i]
+            assert (a.alim, a.lo, a.hi, a.open, a.trials, a.steps, a.visits, a.engaged) == \
+                   (b.alim, b.lo, b.hi, b.open, b.trials, b.steps, b.visits, b.engaged), i
