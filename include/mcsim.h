@@ -194,6 +194,30 @@ int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol);
 /* Stock.sample (src/stock.js:85-92): bilinear height at n points, -inf outside. */
 int mc_stock_sample(const mc_stock* s, const double* xy, int n, double* out);
 
+/* ---- Analytic (zero-order) stability lobes, host side ---- */
+
+/* stabilityLobes parameters (src/sld.js:23): flutes, Ktc, Krc, the engagement arc, the
+ * lobe count, the speed window and the modes (k in N/mm) */
+typedef struct {
+  int32_t N, lobes;
+  double Ktc, Krc, phiSt, phiEx, rpmMin, rpmMax;
+  int32_t nmx, nmy;
+  mc_mode mx[MC_MAX_MODES];
+  mc_mode my[MC_MAX_MODES];
+} mc_lobe_args;
+
+/* directionalCoeffs(Kr, phiSt, phiEx) (src/sld.js:6-15): out = [axx, axy, ayx, ayy] */
+int mc_directional_coeffs(double Kr, double phiSt, double phiEx, double* out);
+/* stabilityLobes (src/sld.js:23-61): the lobes' points, lobe after lobe (lobes with no
+ * point are left out), each sorted by speed; lobe_len[j] points in lobe j (0 past the
+ * last); *n_out points written (at most max_pts) */
+int mc_stability_lobes(const mc_lobe_args* a, double* rpm, double* alim, int max_pts, int* lobe_len, int* n_out);
+/* alimAt (src/sld.js:64-76): the lowest lobe covering S, linearly interpolated; +inf if none */
+int mc_alim_at(const double* rpm, const double* alim, const int* lobe_len, int nlobes, double S, double* out);
+/* alimAt of each cell at its own speed, on the lobes (of the cell's tool, modes, arc and
+ * coefficients) over [0.8 Smin, 1.2 Smax] of the cells sharing them; +inf if none covers */
+int mc_analytic_limits(const mc_cell* cells, int n, double* out);
+
 /* ---- Learned bracket surrogate (no counterpart in the reference) ----
  * An MLP 24 -> 256 -> 256 -> 256 -> 1 with GELU (tanh form) that predicts log(alim [mm])
  * of a cell from mc_surrogate_features(); its prediction p gives the bracket

@@ -10,7 +10,7 @@
 namespace mcd {
 
 constexpr int HIST = 2048;   // chatter-index history kept per group (the index reads <= 1416)
-constexpr int TS = 64;       // once-per-tooth sample ring (src/cutting.js:29)
+constexpr int TS = 32;       // once-per-tooth samples kept (chatterIndex reads the last 24, src/cutting.js:299-306)
 
 // Per-cell constants: CuttingEngine.configure (src/cutting.js:39-82) evaluated on the
 // host for the cell's speed and modes.  Everything that depends on the trial depth

@@ -408,6 +408,46 @@ MC_HD double pow_cell(double x, double y, double y1, double yr) {
   return x == 1.0 ? 1.0 : res;
 }
 
+// Math.atan (fdlibm s_atan: reduction to |x| < 7/16 around the break points 1/2, 1, 3/2
+// and infinity, then an odd polynomial in x^2 split into even and odd halves)
+MC_HD double atan_js(double x) {
+  constexpr double hi0 = 4.63647609000806093515e-01, hi1 = 7.85398163397448278999e-01,
+                   hi2 = 9.82793723247329054082e-01, hi3 = 1.57079632679489655800e+00;
+  constexpr double lo0 = 2.26987774529616870924e-17, lo1 = 3.06161699786838301793e-17,
+                   lo2 = 1.39033110312309984516e-17, lo3 = 6.12323399573676603587e-17;
+  constexpr double T0 = 3.33333333333329318027e-01, T1 = -1.99999999998764832476e-01,
+                   T2 = 1.42857142725034663711e-01, T3 = -1.11111104054623557880e-01,
+                   T4 = 9.09088713343650656196e-02, T5 = -7.69187620504482999495e-02,
+                   T6 = 6.66107313738753120669e-02, T7 = -5.83357013379057348645e-02,
+                   T8 = 4.97687799461593236017e-02, T9 = -3.65315727442169155270e-02,
+                   T10 = 1.62858201153657823623e-02;
+  const int32_t hx = hiw(x), ix = hx & 0x7fffffff;
+  if (ix >= 0x44100000) {  // |x| >= 2^66
+    if (ix > 0x7ff00000 || (ix == 0x7ff00000 && low(x) != 0)) return x + x;
+    return hx > 0 ? hi3 + lo3 : -hi3 - lo3;
+  }
+  int id;
+  if (ix < 0x3fdc0000) {   // |x| < 7/16
+    if (ix < 0x3e200000) return x;
+    id = -1;
+  } else {
+    x = fabs_(x);
+    if (ix < 0x3ff30000) {
+      if (ix < 0x3fe60000) { id = 0; x = (2.0 * x - 1.0) / (2.0 + x); }
+      else { id = 1; x = (x - 1.0) / (x + 1.0); }
+    } else if (ix < 0x40038000) { id = 2; x = (x - 1.5) / (1.0 + 1.5 * x); }
+    else { id = 3; x = -1.0 / x; }
+  }
+  const double z = x * x, w = z * z;
+  const double s1 = z * (T0 + w * (T2 + w * (T4 + w * (T6 + w * (T8 + w * T10)))));
+  const double s2 = w * (T1 + w * (T3 + w * (T5 + w * (T7 + w * T9))));
+  if (id < 0) return x - x * (s1 + s2);
+  const double ah = id == 0 ? hi0 : id == 1 ? hi1 : id == 2 ? hi2 : hi3;
+  const double al = id == 0 ? lo0 : id == 1 ? lo1 : id == 2 ? lo2 : lo3;
+  const double r = ah - ((x * (s1 + s2) - al) - x);
+  return hx < 0 ? -r : r;
+}
+
 // Math.hypot(a, b) as V8's builtin computes it
 MC_HD double hypot_js(double a, double b) {
   const double fa = fabs_(a), fb = fabs_(b);

@@ -662,45 +662,53 @@ int mc_mlp_predict(mc_mlp* m, const float* X, int n, float* out) {
   return MC_OK;
 }
 
-// The 24 surrogate inputs of a cell (raw; mc_mlp_train fits their scaling):
-//   0-5   x modes 1-2: log10 fn, zeta, log10 k   (zeros for an absent mode)
-//   6-11  y modes 1-2: the same
-//   12-15 Ktc, Krc, Kte, Kre
-//   16    helix angle [deg]   17 flutes   18 diameter [mm]
-//   19    immersion: engagement arc / pi   20 log10 rpm   21 revolutions per trial
-//   22-23 lobe phase of the most flexible mode (lowest k over both axes): sin and cos of
-//         2 pi fn / f_tooth, f_tooth = rpm * flutes / 60 (where the stability lobes of
-//         that mode repeat)
+// The 24 surrogate inputs of a cell (raw; mc_mlp_train fits their scaling).  Stability lobes
+// of a mode repeat with its phase eps = 2 pi fn / f_tooth (f_tooth = rpm * flutes / 60), so
+// each mode enters through its stiffness, damping ratio and that phase:
+//   0-15  x modes 1-2, y modes 1-2: log10 k, zeta, sin eps, cos eps (zeros for an absent mode)
+//   16    helix angle [deg]   17 flutes   18 diameter [mm]   19 immersion: arc / pi
+//   20    log10 rpm   21 revolutions per trial
+//   22    ln of the analytic (zero-order) limit at this speed, mc_analytic_limits, clipped
+//         to [1e-3, 1e3] mm: the lobe structure of the linear model, which the network
+//         corrects for process damping, the size effect and the time-domain verdict
+//   23    log10 fn of the most flexible mode (lowest k over both axes)
+// The cutting coefficients enter only through the analytic limit: the training
+// distribution (the lobe grid's) holds them fixed.
 int mc_surrogate_features(const mc_cell* cells, int n, float* out) {
   if (n < 0 || (n > 0 && (!cells || !out))) return mch::fail(MC_ERR_ARG, "bad arguments");
+  std::vector<double> zoa(n);
+  int st = mc_analytic_limits(cells, n, zoa.data());
+  if (st) return st;
   for (int i = 0; i < n; i++) {
     const mc_cell& c = cells[i];
     float* f = out + (size_t)i * IN;
     for (int j = 0; j < IN; j++) f[j] = 0.0f;
+    const double ftooth = (c.S > 100 ? c.S : 100.0) * c.tool.N / 60.0;
     for (int a = 0; a < 2; a++) {
       const mc_mode* md = a == 0 ? c.mx : c.my;
       const int nm = a == 0 ? c.nmx : c.nmy;
       for (int j = 0; j < 2 && j < nm; j++) {
         if (!(md[j].fn > 0) || !(md[j].k > 0)) return mch::fail(MC_ERR_ARG, "cell %d: mode needs fn > 0, k > 0", i);
-        f[a * 6 + j * 3 + 0] = (float)std::log10(md[j].fn);
-        f[a * 6 + j * 3 + 1] = (float)md[j].zeta;
-        f[a * 6 + j * 3 + 2] = (float)std::log10(md[j].k);
+        const double eps = 2 * 3.141592653589793 * (md[j].fn / ftooth);
+        float* m = f + a * 8 + j * 4;
+        m[0] = (float)std::log10(md[j].k);
+        m[1] = (float)md[j].zeta;
+        m[2] = (float)std::sin(eps);
+        m[3] = (float)std::cos(eps);
       }
     }
-    f[12] = (float)c.mat.Ktc; f[13] = (float)c.mat.Krc; f[14] = (float)c.mat.Kte; f[15] = (float)c.mat.Kre;
+    double kmin = INFINITY, fsoft = 1;
+    for (int j = 0; j < c.nmx && j < MC_MAX_MODES; j++) if (c.mx[j].k < kmin) { kmin = c.mx[j].k; fsoft = c.mx[j].fn; }
+    for (int j = 0; j < c.nmy && j < MC_MAX_MODES; j++) if (c.my[j].k < kmin) { kmin = c.my[j].k; fsoft = c.my[j].fn; }
+    const double za = zoa[i] < 1e-3 ? 1e-3 : zoa[i] > 1e3 || !(zoa[i] == zoa[i]) ? 1e3 : zoa[i];
+    f[22] = (float)std::log(za);
+    f[23] = (float)std::log10(fsoft);
     f[16] = (float)c.tool.helixDeg;
     f[17] = (float)c.tool.N;
     f[18] = (float)(2 * c.tool.R);
     f[19] = (float)((c.phiEx - c.phiSt) / 3.141592653589793);
     f[20] = (float)std::log10(c.S > 100 ? c.S : 100.0);
     f[21] = (float)c.revs;
-    double kmin = INFINITY, fsoft = 0;
-    for (int j = 0; j < c.nmx && j < MC_MAX_MODES; j++) if (c.mx[j].k < kmin) { kmin = c.mx[j].k; fsoft = c.mx[j].fn; }
-    for (int j = 0; j < c.nmy && j < MC_MAX_MODES; j++) if (c.my[j].k < kmin) { kmin = c.my[j].k; fsoft = c.my[j].fn; }
-    const double ftooth = (c.S > 100 ? c.S : 100.0) * c.tool.N / 60.0;
-    const double ph = 2 * 3.141592653589793 * (fsoft / ftooth);
-    f[22] = (float)std::sin(ph);
-    f[23] = (float)std::cos(ph);
     for (int j = 0; j < IN; j++)
       if (!std::isfinite(f[j])) return mch::fail(MC_ERR_ARG, "cell %d: feature %d is not finite", i, j);
   }

@@ -104,3 +104,10 @@ class TrainOpts(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
                 ("lr_final", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("eps", C.c_float), ("fit_scaling", C.c_int32), ("seed", C.c_uint64)]
+
+
+class LobeArgs(C.Structure):
+    _fields_ = [("N", C.c_int32), ("lobes", C.c_int32), ("Ktc", C.c_double), ("Krc", C.c_double),
+                ("phiSt", C.c_double), ("phiEx", C.c_double), ("rpmMin", C.c_double),
+                ("rpmMax", C.c_double), ("nmx", C.c_int32), ("nmy", C.c_int32),
+                ("mx", Mode * MC_MAX_MODES), ("my", Mode * MC_MAX_MODES)]

@@ -8,8 +8,8 @@ import os
 
 import numpy as np
 
-from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, TrainOpts, cell_array, MC_OK,
-                   MC_SURR_IN)
+from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, TrainOpts, LobeArgs, Mode, cell_array,
+                   MC_OK, MC_SURR_IN)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -21,6 +21,7 @@ EXPORTS = [
     "mc_stock_read_side", "mc_stamp_batch", "mc_stock_sample", "mc_surrogate_features",
     "mc_mlp_create", "mc_mlp_destroy", "mc_mlp_nparams", "mc_mlp_params", "mc_mlp_set_scaling",
     "mc_mlp_get_scaling", "mc_mlp_grads", "mc_mlp_train", "mc_mlp_predict",
+    "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
 ]
 
 _LIB = None
@@ -69,6 +70,10 @@ def load():
     L.mc_mlp_grads.argtypes = [P, P, P, C.c_int, P, P]
     L.mc_mlp_train.argtypes = [P, P, P, C.c_int, C.POINTER(TrainOpts), P]
     L.mc_mlp_predict.argtypes = [P, P, C.c_int, P]
+    L.mc_directional_coeffs.argtypes = [C.c_double, C.c_double, C.c_double, P]
+    L.mc_stability_lobes.argtypes = [C.POINTER(LobeArgs), P, P, C.c_int, P, P]
+    L.mc_alim_at.argtypes = [P, P, P, C.c_int, C.c_double, P]
+    L.mc_analytic_limits.argtypes = [C.POINTER(Cell), C.c_int, P]
     _LIB = L
     return L
 
@@ -272,3 +277,49 @@ class Mlp:
         out = np.empty(len(X), dtype=np.float32)
         _check(self.L.mc_mlp_predict(self.p, X.ctypes.data, len(X), out.ctypes.data))
         return out
+
+
+def directional_coeffs(Kr, phiSt, phiEx):
+    out = np.zeros(4)
+    _check(load().mc_directional_coeffs(float(Kr), float(phiSt), float(phiEx), out.ctypes.data))
+    return out
+
+
+def stability_lobes(N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin, rpmMax, modes_x, modes_y=None, cap=400000):
+    """stabilityLobes (src/sld.js:23): list of (rpm array, alim array) per non-empty lobe"""
+    modes_y = modes_y or modes_x
+    a = LobeArgs()
+    a.N, a.lobes, a.Ktc, a.Krc, a.phiSt, a.phiEx = int(N), int(lobes), Ktc, Krc, phiSt, phiEx
+    a.rpmMin, a.rpmMax, a.nmx, a.nmy = rpmMin, rpmMax, len(modes_x), len(modes_y)
+    for i, m in enumerate(modes_x):
+        a.mx[i] = Mode(*map(float, m))
+    for i, m in enumerate(modes_y):
+        a.my[i] = Mode(*map(float, m))
+    rpm, alim = np.zeros(cap), np.zeros(cap)
+    ll = np.zeros(max(1, lobes), dtype=np.int32)
+    n = C.c_int(0)
+    _check(load().mc_stability_lobes(C.byref(a), rpm.ctypes.data, alim.ctypes.data, cap, ll.ctypes.data, C.byref(n)))
+    out, off = [], 0
+    for k in ll[ll > 0]:
+        out.append((rpm[off:off + k].copy(), alim[off:off + k].copy()))
+        off += k
+    return out
+
+
+def alim_at(lobes, S):
+    """alimAt (src/sld.js:64) over the lobes stability_lobes returns"""
+    if not lobes:
+        return float("inf")
+    rpm = np.concatenate([l[0] for l in lobes])
+    alim = np.concatenate([l[1] for l in lobes])
+    ll = np.array([len(l[0]) for l in lobes], dtype=np.int32)
+    out = C.c_double()
+    _check(load().mc_alim_at(rpm.ctypes.data, alim.ctypes.data, ll.ctypes.data, len(ll), float(S), C.byref(out)))
+    return out.value
+
+
+def analytic_limits(cells):
+    arr, n = _cells(cells)
+    out = np.zeros(n)
+    _check(load().mc_analytic_limits(arr, n, out.ctypes.data))
+    return out
