@@ -331,7 +331,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   const int64_t geo_cap = pe_cap * (nz_cap + mcd_search_maxspec());
   if (geo_cap * 16 >= ((int64_t)1 << 31) || (int64_t)SEARCH_MAXSLOT * ring_cap * 4 >= ((int64_t)1 << 31))
     return fail(MC_ERR_LIMIT, "per-wave delay lines or geometry table exceed 2 GiB");
-  const int wpc = std::max(4, env_int("MC_WAVES_PER_CU", 12));   // 3 per SIMD
+  const int wpc = std::max(4, env_int("MC_WAVES_PER_CU", 8));   // 2 per SIMD (the VGPR limit)
   const int64_t want = ((int64_t)nn + 4 * (int64_t)nw + 3) / 4;      // workgroups of four waves
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)X.cus * wpc / 4));
   const int lds = mcd_search_lds_bytes(nz_cap);

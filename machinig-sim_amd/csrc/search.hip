@@ -700,7 +700,7 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
 
 // Persistent workgroups of four waves: the wide tasks first (a whole workgroup each),
 // then one task per wave until the queue is empty.
-__global__ __launch_bounds__(256, 3) void cell_search(LaunchDev P) {
+__global__ __launch_bounds__(256) void cell_search(LaunchDev P) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   unsigned char* const wblk = lds + (size_t)w * per_wave_bytes();
