@@ -188,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heightmap", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--train-cells", type=int, default=65536)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -203,6 +204,13 @@ def main():
     cells, opts = W.sld_grid(n_speeds=a.speeds, n_tools=a.tools)
     arr = cell_array(cells)
     print("device:", lib.device_info(), file=sys.stderr, flush=True)
+    tuning = None
+    if not a.no_autotune:
+        from mcsim import autotune
+        t0 = time.perf_counter()
+        best, best_ms, table = autotune.tune_search(arr, opts)
+        tuning = dict(best=best, kernel_ms=best_ms, seconds=time.perf_counter() - t0, candidates=table)
+        print("autotuned schedule %r (%.0f ms) in %.1f s" % (best, best_ms, tuning["seconds"]), file=sys.stderr, flush=True)
     for w in range(a.warmup):
         lib.bisect_batch(arr, **opts)
         print("warmup %d done" % w, file=sys.stderr, flush=True)
@@ -245,7 +253,7 @@ def main():
                    "bisection": opts, "value_path": path, "plain_cells_per_s": plain_value,
                    "plain_ms_per_step": ms, "plain_kernel_ms": kms_mean, "trials": trials,
                    "slice_visits": visits, "slice_visits_per_s": visits / (ms_step / 1e3),
-                   "surrogate": surr},
+                   "surrogate": surr, "schedule": tuning or lib.search_tuning()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic("cell_search"),

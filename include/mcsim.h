@@ -126,6 +126,22 @@ int mc_device_info(char* buf, int len);
  * events on the library's stream around that kernel alone. */
 int mc_last_kernel_time(double* ms);
 
+/* Launch schedule of the search (library-wide; no reference counterpart).  The trees of
+ * the cells whose trials have >= wide_frac x the longest trial's steps (at most wide_max
+ * of them) run first as wide tasks on four waves each; waves_per_cu persistent waves per
+ * compute unit; use_graph: replay the launch sequence from a captured hipGraph while its
+ * shape and buffers repeat.  Defaults {8, 256, 0.5, 1}.  mcsim/autotune.py picks them by
+ * timing candidates on the caller's batch. */
+typedef struct {
+  int32_t waves_per_cu;
+  int32_t wide_max;
+  double wide_frac;
+  int32_t use_graph;
+  int32_t reserved[3];
+} mc_search_tuning;
+int mc_set_search_tuning(const mc_search_tuning* t);
+int mc_get_search_tuning(mc_search_tuning* t);
+
 /* chatterAt (src/stabtd.js:14-22) for n (cell, depth) pairs: cells[i] at depth ap[i]. */
 int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* out);
 

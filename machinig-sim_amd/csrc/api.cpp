@@ -23,6 +23,7 @@ int mcd_launch_chatter(const mcd::LaunchDev& P, int G, int blocks, hipStream_t s
 // search.hip
 int mcd_search_lds_bytes(int nz_cap);
 int mcd_launch_search(const mcd::LaunchDev& P, int blocks, hipStream_t stream);
+int mcd_launch_reset_queue(int32_t* q, int n, hipStream_t stream);
 int mcd_search_maxspec();
 constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;   // slots per wave (a wide task: 4 waves x 2)
 // stock.hip
@@ -75,6 +76,40 @@ struct DevBuf {
   }
 };
 
+// pinned host staging (grow-only), so that a captured graph can copy from fixed addresses
+struct HostBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= n) return 0;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return fail(MC_ERR_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+    n = bytes;
+    return 0;
+  }
+};
+
+// the search launch sequence (upload, queue reset, kernel, download) captured as a graph,
+// reused while the launch shape and buffers stay the same
+struct SearchGraph {
+  bool valid = false;
+  mcd::LaunchDev P;
+  int blocks = 0;
+  size_t cell_bytes = 0, task_bytes = 0, out_bytes = 0;
+  const void *hc = nullptr, *ht = nullptr, *ho = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+};
+
 struct Ctx {
   int dev = -1;
   int cus = 0;
@@ -83,6 +118,9 @@ struct Ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   DevBuf cells, tasks, out, queue, ring, hist, trace, geo, gbin;
+  HostBuf h_cells, h_tasks, h_out;
+  SearchGraph sgraph;
+  mc_search_tuning tune = {8, 256, 0.5, 1, {0, 0, 0}};
   std::mutex mu;
 };
 
@@ -298,9 +336,8 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts.cost[a] > ts.cost[b]; });
   // wide tasks: the longest trees
-  const char* wf = getenv("MC_WIDE_FRAC");
-  const double wide_frac = wf && *wf ? atof(wf) : 0.5;
-  const int wide_max = std::max(0, env_int("MC_WIDE_MAX", 256));
+  const double wide_frac = X.tune.wide_frac;
+  const int wide_max = std::max(0, X.tune.wide_max);
   double smax = 0;
   for (int q = 0; q < n; q++) {
     const mcd::CellDev& c = ts.cells[ts.tasks[q].cell];
@@ -331,7 +368,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   const int64_t geo_cap = pe_cap * (nz_cap + mcd_search_maxspec());
   if (geo_cap * 16 >= ((int64_t)1 << 31) || (int64_t)SEARCH_MAXSLOT * ring_cap * 4 >= ((int64_t)1 << 31))
     return fail(MC_ERR_LIMIT, "per-wave delay lines or geometry table exceed 2 GiB");
-  const int wpc = std::max(4, env_int("MC_WAVES_PER_CU", 8));   // 2 per SIMD (the VGPR limit)
+  const int wpc = std::max(4, X.tune.waves_per_cu);   // 8: 2 per SIMD (the VGPR limit)
   const int64_t want = ((int64_t)nn + 4 * (int64_t)nw + 3) / 4;      // workgroups of four waves
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)X.cus * wpc / 4));
   const int lds = mcd_search_lds_bytes(nz_cap);
@@ -347,9 +384,11 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   if ((st = X.hist.ensure((size_t)waves * SEARCH_MAXSLOT * mcd::HIST * sizeof(float2)))) return st;
   if ((st = X.geo.ensure((size_t)waves * geo_cap * sizeof(double2)))) return st;
   if ((st = X.gbin.ensure((size_t)waves * geo_cap))) return st;
-  HIPCHK(hipMemcpyAsync(X.cells.p, ts.cells.data(), ts.cells.size() * sizeof(mcd::CellDev), hipMemcpyHostToDevice, X.stream));
-  HIPCHK(hipMemcpyAsync(X.tasks.p, sorted.data(), n * sizeof(mcd::TaskDev), hipMemcpyHostToDevice, X.stream));
-  HIPCHK(hipMemsetAsync(X.queue.p, 0, 256, X.stream));
+  const size_t cell_bytes = ts.cells.size() * sizeof(mcd::CellDev), task_bytes = n * sizeof(mcd::TaskDev),
+               out_bytes = n * sizeof(mcd::ResultDev);
+  if ((st = X.h_cells.ensure(cell_bytes)) || (st = X.h_tasks.ensure(task_bytes)) || (st = X.h_out.ensure(out_bytes))) return st;
+  std::memcpy(X.h_cells.p, ts.cells.data(), cell_bytes);
+  std::memcpy(X.h_tasks.p, sorted.data(), task_bytes);
   mcd::LaunchDev P;
   std::memset(&P, 0, sizeof P);
   P.geo = (double2*)X.geo.p;
@@ -369,16 +408,57 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   P.ring_cap = ring_cap;
   P.nz_cap = nz_cap;
   P.nv_cap = nv_cap;
-  HIPCHK(hipEventRecord(X.ev0, X.stream));
-  if (mcd_launch_search(P, blocks, X.stream))
-    return fail(MC_ERR_HIP, "search kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-  HIPCHK(hipEventRecord(X.ev1, X.stream));
-  std::vector<mcd::ResultDev> tmp(n);
-  HIPCHK(hipMemcpyAsync(tmp.data(), X.out.p, n * sizeof(mcd::ResultDev), hipMemcpyDeviceToHost, X.stream));
+  // the launch sequence: upload, queue reset (a kernel, so that a graph holds no memset
+  // node), the search, download.  Launched directly, the two events bracket the search
+  // kernel; replayed from a graph, they bracket the graph launch (the copies add ~1 ms)
+  auto enqueue = [&](bool events) -> int {
+    HIPCHK(hipMemcpyAsync(X.cells.p, X.h_cells.p, cell_bytes, hipMemcpyHostToDevice, X.stream));
+    HIPCHK(hipMemcpyAsync(X.tasks.p, X.h_tasks.p, task_bytes, hipMemcpyHostToDevice, X.stream));
+    if (mcd_launch_reset_queue(P.queue, 64, X.stream)) return fail(MC_ERR_HIP, "queue reset failed");
+    if (events) HIPCHK(hipEventRecord(X.ev0, X.stream));
+    if (mcd_launch_search(P, blocks, X.stream))
+      return fail(MC_ERR_HIP, "search kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (events) HIPCHK(hipEventRecord(X.ev1, X.stream));
+    HIPCHK(hipMemcpyAsync(X.h_out.p, X.out.p, out_bytes, hipMemcpyDeviceToHost, X.stream));
+    return MC_OK;
+  };
+  SearchGraph& G = X.sgraph;
+  bool launched = false;
+  if (X.tune.use_graph) {
+    const bool same = G.valid && std::memcmp(&G.P, &P, sizeof P) == 0 && G.blocks == blocks &&
+                      G.cell_bytes == cell_bytes && G.task_bytes == task_bytes && G.out_bytes == out_bytes &&
+                      G.hc == X.h_cells.p && G.ht == X.h_tasks.p && G.ho == X.h_out.p;
+    if (!same) {
+      G.reset();
+      if (hipStreamBeginCapture(X.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int est = enqueue(false);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(X.stream, &g);
+        if (est == MC_OK && ce == hipSuccess && g && hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0) == hipSuccess) {
+          G.graph = g;
+          G.valid = true;
+          G.P = P; G.blocks = blocks; G.cell_bytes = cell_bytes; G.task_bytes = task_bytes; G.out_bytes = out_bytes;
+          G.hc = X.h_cells.p; G.ht = X.h_tasks.p; G.ho = X.h_out.p;
+        } else {
+          if (g) (void)hipGraphDestroy(g);
+          G.exec = nullptr;
+          (void)hipGetLastError();   // capture not available: launch the sequence directly
+        }
+      }
+    }
+    if (G.valid) {
+      HIPCHK(hipEventRecord(X.ev0, X.stream));
+      HIPCHK(hipGraphLaunch(G.exec, X.stream));
+      HIPCHK(hipEventRecord(X.ev1, X.stream));
+      launched = true;
+    }
+  }
+  if (!launched && (st = enqueue(true))) return st;
   HIPCHK(hipStreamSynchronize(X.stream));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
   X.last_ms += ms;
+  const mcd::ResultDev* tmp = static_cast<const mcd::ResultDev*>(X.h_out.p);
   for (int q = 0; q < n; q++) res[q < nn ? narrow[q] : wide[q - nn]] = tmp[q];
   if (getenv("MC_TASK_TIMES")) {
     FILE* f = fopen(getenv("MC_TASK_TIMES"), "w");
@@ -543,6 +623,22 @@ struct mc_stock : StockImpl {};
 extern "C" {
 
 int mc_init(int device) { return ensure_init(device); }
+
+int mc_set_search_tuning(const mc_search_tuning* t) {
+  if (!t) return fail(MC_ERR_ARG, "null tuning");
+  if (t->waves_per_cu < 4 || t->waves_per_cu > 32 || t->wide_max < 0 || !(t->wide_frac >= 0) || t->wide_frac > 1e9)
+    return fail(MC_ERR_ARG, "need 4 <= waves_per_cu <= 32, wide_max >= 0, wide_frac >= 0");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  g_ctx.tune = *t;
+  return MC_OK;
+}
+
+int mc_get_search_tuning(mc_search_tuning* t) {
+  if (!t) return fail(MC_ERR_ARG, "null tuning");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  *t = g_ctx.tune;
+  return MC_OK;
+}
 
 const char* mc_last_error(void) { return g_err.c_str(); }
 

@@ -730,6 +730,14 @@ __global__ __launch_bounds__(256) void cell_search(LaunchDev P) {
   }
 }
 
+__global__ void reset_queue(int32_t* q, int n) {
+  if ((int)threadIdx.x < n) q[threadIdx.x] = 0;
+}
+int mcd_launch_reset_queue(int32_t* q, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, stream, q, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 // blocks = persistent workgroups of four waves
 int mcd_launch_search(const LaunchDev& P, int blocks, hipStream_t stream) {
   const size_t lds = (size_t)mcd_search_lds_bytes(P.nz_cap);

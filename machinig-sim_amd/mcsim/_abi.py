@@ -111,3 +111,8 @@ class LobeArgs(C.Structure):
                 ("phiSt", C.c_double), ("phiEx", C.c_double), ("rpmMin", C.c_double),
                 ("rpmMax", C.c_double), ("nmx", C.c_int32), ("nmy", C.c_int32),
                 ("mx", Mode * MC_MAX_MODES), ("my", Mode * MC_MAX_MODES)]
+
+
+class SearchTuning(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_int32), ("wide_max", C.c_int32), ("wide_frac", C.c_double),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32 * 3)]

@@ -8,8 +8,8 @@ import os
 
 import numpy as np
 
-from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, TrainOpts, LobeArgs, Mode, cell_array,
-                   MC_OK, MC_SURR_IN)
+from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, TrainOpts, LobeArgs, Mode, SearchTuning,
+                   cell_array, MC_OK, MC_SURR_IN)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -22,6 +22,7 @@ EXPORTS = [
     "mc_mlp_create", "mc_mlp_destroy", "mc_mlp_nparams", "mc_mlp_params", "mc_mlp_set_scaling",
     "mc_mlp_get_scaling", "mc_mlp_grads", "mc_mlp_train", "mc_mlp_predict",
     "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
+    "mc_set_search_tuning", "mc_get_search_tuning",
 ]
 
 _LIB = None
@@ -74,6 +75,8 @@ def load():
     L.mc_stability_lobes.argtypes = [C.POINTER(LobeArgs), P, P, C.c_int, P, P]
     L.mc_alim_at.argtypes = [P, P, P, C.c_int, C.c_double, P]
     L.mc_analytic_limits.argtypes = [C.POINTER(Cell), C.c_int, P]
+    L.mc_set_search_tuning.argtypes = [C.POINTER(SearchTuning)]
+    L.mc_get_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     _LIB = L
     return L
 
@@ -323,3 +326,14 @@ def analytic_limits(cells):
     out = np.zeros(n)
     _check(load().mc_analytic_limits(arr, n, out.ctypes.data))
     return out
+
+
+def search_tuning():
+    t = SearchTuning()
+    _check(load().mc_get_search_tuning(C.byref(t)))
+    return dict(waves_per_cu=t.waves_per_cu, wide_max=t.wide_max, wide_frac=t.wide_frac, use_graph=t.use_graph)
+
+
+def set_search_tuning(waves_per_cu=8, wide_max=256, wide_frac=0.5, use_graph=1):
+    t = SearchTuning(int(waves_per_cu), int(wide_max), float(wide_frac), int(use_graph))
+    _check(load().mc_set_search_tuning(C.byref(t)))

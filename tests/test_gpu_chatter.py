@@ -126,3 +126,35 @@ def test_mixed_batch_routes_every_cell_exactly(mc, orc):
         o = orc.time_domain_limit(c, **opts)
         assert (out[i].alim, out[i].lo, out[i].hi, out[i].open, out[i].trials, out[i].steps,
                 out[i].visits) == (o.alim, o.lo, o.hi, o.open, o.trials, o.steps, o.visits), i
+
+
+def test_graph_replays_match_eager_launches(mc):
+    """the captured launch sequence, replayed several times with eager work and
+    synchronisations in between, returns exactly what direct launches return"""
+    from mcsim import workloads as W
+    cells, opts = W.sld_grid(n_speeds=10, n_tools=8, seed=21)
+    keep = mc.search_tuning()
+    try:
+        mc.set_search_tuning(**dict(keep, use_graph=0))
+        eager = [(r.alim, r.lo, r.hi, r.trials, r.steps, r.visits) for r in mc.bisect_batch(cells, **opts)]
+        mc.set_search_tuning(**dict(keep, use_graph=1))
+        for rep in range(3):
+            got = [(r.alim, r.lo, r.hi, r.trials, r.steps, r.visits) for r in mc.bisect_batch(cells, **opts)]
+            assert got == eager, rep
+            mc.chatter_batch(cells[:4], [0.5] * 4)   # eager work of the same stream in between
+    finally:
+        mc.set_search_tuning(**keep)
+
+
+def test_autotuner_keeps_results(mc):
+    from mcsim import autotune, workloads as W
+    cells, opts = W.sld_grid(n_speeds=8, n_tools=6, seed=22)
+    keep = mc.search_tuning()
+    try:
+        ref = [(r.alim, r.lo, r.hi, r.trials) for r in mc.bisect_batch(cells, **opts)]
+        best, ms, table = autotune.tune_search(cells, opts, fracs=(1e9, 0.5, 0.3), wide_max=(16, 64))
+        print("autotune best", best, "%.2f ms" % ms, flush=True)
+        assert mc.search_tuning()["wide_frac"] == best["wide_frac"]
+        assert [(r.alim, r.lo, r.hi, r.trials) for r in mc.bisect_batch(cells, **opts)] == ref
+    finally:
+        mc.set_search_tuning(**keep)
