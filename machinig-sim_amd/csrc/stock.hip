@@ -10,8 +10,12 @@
 //   stamp_prep   per stamp: bounding box (the reference's floor/ceil rule), radii and
 //                rim height; per chunk of CHUNK stamps: union bounding box.
 //   stamp_tiles  one 16x16 node tile per workgroup: walks the chunks whose box meets the
-//                tile, compacts (in order) the stamps of a chunk that meet the tile into
-//                LDS, and lets each node thread apply them in order.
+//                tile, compacts (in order, with wave ballots) the stamps of a chunk that
+//                meet the tile and can still cut it into LDS, and lets each node thread
+//                apply them in order.  A stamp can cut a node only where H > zmin (its
+//                lowest reach), so stamps whose zmin is at or above the tile's highest node
+//                are dropped, and each wave skips the stamps at or above its own highest
+//                node: exact, since those stamps would change nothing.
 //
 // Removed volume per stamp (the value stamp() returns) is summed with one FP64 atomic per
 // wavefront and stamp, so its last bits depend on the summation order.
@@ -27,6 +31,7 @@ constexpr double EDGE_TIE = 2 * 0.0015;  // src/stock.js:6
 
 struct StampDev {
   double cx, cy, tipZ, R, R2, tie, Rw2, zRim;
+  double zmin;   // lowest z the stamp can cut to: both of its updates need H above it
   float errVal;
   int32_t line, type, i0, i1, j0, j1, index;
 };
@@ -52,6 +57,9 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, int n, double hx, do
     d.R2 = st.R * st.R;
     d.tie = d.R2 - EDGE_TIE * st.R;
     d.zRim = st.tipZ + (st.type == MC_TOOL_BALL ? st.R : 0);
+    // a ball cuts to z = (tipZ + R) - sqrt(R^2 - r2) >= (tipZ + R) - R, a flat one to tipZ; the
+    // wall update needs H > zRim >= zmin too, so a node with H <= zmin is left as it is
+    d.zmin = !(st.R >= 0) ? -__builtin_inf() : st.type == MC_TOOL_BALL ? (st.tipZ + st.R) - st.R : st.tipZ;
     const double a = __builtin_floor((st.cx - Rw) / hx), b = __builtin_ceil((st.cx + Rw) / hx);
     const double c = __builtin_floor((st.cy - Rw) / hy), e = __builtin_ceil((st.cy + Rw) / hy);
     const bool live = st.tipZ < zTop && a <= nx - 1 && c <= ny - 1 && b >= 0 && e >= 0;
@@ -81,7 +89,6 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
     float* __restrict__ wallD, int32_t* __restrict__ lineCut, int32_t* __restrict__ lineWall,
     double* __restrict__ vol) {
   __shared__ StampDev list[CHUNK];
-  __shared__ int cnt[TILE * TILE];
   const int t = threadIdx.x;
   const int ti0 = blockIdx.x * TILE, tj0 = blockIdx.y * TILE;
   const int ti1 = min(ti0 + TILE, nx) - 1, tj1 = min(tj0 + TILE, ny) - 1;
@@ -93,30 +100,43 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
   if (own) { h = H[k]; e = err[k]; wd = wallD[k]; lc = lineCut[k]; lw = lineWall[k]; }
   const double xi = i * hx, yj = j * hy;
   const int nchunks = (n + CHUNK - 1) / CHUNK;
+  const int lane = t & 63, wv = t >> 6;
+  __shared__ int wcount[TILE * TILE / 64];
+  __shared__ float wmaxs[TILE * TILE / 64];
   for (int c = 0; c < nchunks; c++) {
     const ChunkBox b = box[c];
     if (b.i1 < ti0 || b.i0 > ti1 || b.j1 < tj0 || b.j0 > tj1) continue;  // block-uniform
-    // order-preserving compaction of the chunk's stamps that meet this tile
+    // highest node of this wave and of the tile (H only falls, so a stale maximum is a
+    // safe upper bound for the rest of the chunk)
+    float wmax = own ? h : -__builtin_inff();
+    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
+    if (lane == 0) wmaxs[wv] = wmax;
+    // order-preserving compaction of the chunk's stamps that meet this tile and reach
+    // below its highest node
     const int s = c * CHUNK + t;
-    bool hit = false;
     StampDev d;
+    bool cand = false;
     if (s < n) {
       d = st[s];
-      hit = !(d.i1 < ti0 || d.i0 > ti1 || d.j1 < tj0 || d.j0 > tj1) && d.i0 <= d.i1 && d.j0 <= d.j1;
+      cand = !(d.i1 < ti0 || d.i0 > ti1 || d.j1 < tj0 || d.j0 > tj1) && d.i0 <= d.i1 && d.j0 <= d.j1;
     }
-    cnt[t] = hit;
     __syncthreads();
-    for (int off = 1; off < TILE * TILE; off <<= 1) {  // inclusive scan
-      const int v = t >= off ? cnt[t - off] : 0;
-      __syncthreads();
-      cnt[t] += v;
-      __syncthreads();
+    float tmax = wmaxs[0];
+    for (int v = 1; v < TILE * TILE / 64; v++) tmax = fmaxf(tmax, wmaxs[v]);
+    const bool hit = cand && (double)tmax > d.zmin;
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+    if (lane == 0) wcount[wv] = __builtin_popcountll(m);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int v = 0; v < TILE * TILE / 64; v++) {
+      before += v < wv ? wcount[v] : 0;
+      total += wcount[v];
     }
-    const int total = cnt[TILE * TILE - 1];
-    if (hit) list[cnt[t] - 1] = d;
+    if (hit) list[before + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0))] = d;
     __syncthreads();
     for (int q = 0; q < total; q++) {
       const StampDev& p = list[q];
+      if ((double)wmax <= p.zmin) continue;   // wave-uniform: no node of this wave can change
       double removed = 0;
       if (own && i >= p.i0 && i <= p.i1 && j >= p.j0 && j <= p.j1) {
         const double dx = xi - p.cx, dy = yj - p.cy;
@@ -140,7 +160,7 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
       if (vol && __any(removed != 0.0)) {
         double r = removed;
         for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
-        if ((t & 63) == 0 && r != 0.0) atomicAdd(&vol[p.index], r);
+        if (lane == 0 && r != 0.0) atomicAdd(&vol[p.index], r);
       }
     }
     __syncthreads();
