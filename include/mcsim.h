@@ -192,7 +192,7 @@ int mc_stock_write_h(mc_stock* s, const float* H);
 int mc_stock_read_side(const mc_stock* s, float* err, float* wallD, int32_t* lineCut,
                        int32_t* lineWall);
 
-/* one Stock.stamp call (src/stock.js:118): flat/ball, no LUT, no feed marks */
+/* one Stock.stamp call (src/stock.js:118): flat/ball; mc_stamp_opt adds marks and profiles */
 typedef struct {
   double cx, cy, tipZ, R;
   int32_t type;        /* MC_TOOL_FLAT | MC_TOOL_BALL */
@@ -207,8 +207,59 @@ typedef struct {
  * tipZ >= zTop are skipped (vol 0) like src/stock.js:119. */
 int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol);
 
+/* The mark and lut arguments of Stock.stamp (src/stock.js:118) for one stamp. */
+typedef struct {
+  double ux, uy;       /* feed direction (mark.ux, mark.uy) */
+  double f;            /* feed per revolution [mm] (mark.f); <= 0: no feed marks (mark = null) */
+  int32_t lut;         /* tip profile: index into the batch's tables, -1: none (lut = null) */
+  int32_t pad;
+} mc_stamp_opt;
+
+/* Stock.stamp with feed marks and tip profiles (src/stock.js:118-164), n stamps in order.
+ * opt (may be NULL: no marks, no profiles) holds one mc_stamp_opt per stamp.  Profile t
+ * is luts[lut_off[t] .. lut_off[t] + lut_len[t]) (f32 heights above the tip in radial
+ * steps of MC_LUT_DR, as tipLut returns); with a profile the cutter surface is
+ * tipZ + lut[min(len-1, round(r / MC_LUT_DR))] whatever the type.  Feed marks (markS /
+ * markF) are written for flat stamps without a profile.  Everything else as
+ * mc_stamp_batch, which this extends. */
+int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* opt, int n,
+                      const float* luts, const int32_t* lut_off, const int32_t* lut_len,
+                      int nluts, double* vol);
+
+/* feed-mark maps markS [mm] and markF [mm] (f32; src/stock.js:19-23); either may be NULL */
+int mc_stock_read_marks(const mc_stock* s, float* markS, float* markF);
+
+/* Stock.dirty (src/stock.js:81, :156-160): node rectangle {i0, j0, i1, j1} covering the
+ * bounding boxes of the stamps that changed a node since the last clear; empty as
+ * {INT32_MAX, INT32_MAX, -1, -1} (the reference's {Infinity, Infinity, -1, -1}). */
+int mc_stock_dirty(const mc_stock* s, int32_t rect[4]);
+int mc_stock_clear_dirty(mc_stock* s);
+
+/* Stock.blockMax over every block (src/stock.js:45-54): highest H of each MC_STOCK_B x
+ * MC_STOCK_B node block, bnx = ceil(nx / B) by bny = ceil(ny / B), row-major. */
+#define MC_STOCK_B 8
+int mc_stock_block_max(const mc_stock* s, float* bmax);
+
+/* Stock.maxInDisk (src/stock.js:57-80) for n queries q[4i..4i+3] = {cx, cy, r, above}
+ * (above = -INFINITY for the default): out[3i..3i+2] = {h, x, y}, h = -INFINITY (x = y = 0)
+ * when nothing qualifies.  Same value and location as the reference's block walk. */
+int mc_stock_max_in_disk(const mc_stock* s, const double* q, int n, double* out);
+
 /* Stock.sample (src/stock.js:85-92): bilinear height at n points, -inf outside. */
 int mc_stock_sample(const mc_stock* s, const double* xy, int n, double* out);
+
+/* ---- Tip profile table, host side ---- */
+
+#define MC_LUT_DR 0.02   /* radial step of the tip-shape table [mm] (src/assembly.js:8) */
+enum { MC_TIP_FLAT = 0, MC_TIP_BALL = 1, MC_TIP_BULL = 2, MC_TIP_EDGE = 3 };
+
+/* tipLut(t) (src/assembly.js:38-54): height of the cutter surface above the tip at radius
+ * i * MC_LUT_DR, i < n = ceil(D/2 / MC_LUT_DR) + 1.  shape MC_TIP_EDGE uses the imported
+ * envelope edge_r[0..edge_n) sampled every edge_dz from the tip (t.edge); cornerR is read
+ * for MC_TIP_BULL only.  *n_out receives n; out needs room for cap >= n floats (out may be
+ * NULL to ask for n). */
+int mc_tip_lut(int32_t shape, double D, double cornerR, const double* edge_r, int32_t edge_n,
+               double edge_dz, float* out, int32_t cap, int32_t* n_out);
 
 /* ---- Analytic (zero-order) stability lobes, host side ---- */
 

@@ -30,10 +30,19 @@ constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;   // slots per wave (a wide t
 size_t mcs_stampdev_bytes();
 size_t mcs_chunkbox_bytes();
 int mcs_chunk();
-int mcs_launch_stamps(const mc_stamp* d_in, int n, void* d_stampdev, void* d_box, int nx, int ny,
+size_t mcs_stampx_bytes();
+int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, const float* luts,
+                      const int32_t* lut_off, const int32_t* lut_len, const float* lut_min,
+                      void* d_stampdev, void* d_stampx, void* d_box, int32_t* touched, int nx, int ny,
                       double hx, double hy, double zTop, double zB, double band, float* H,
-                      float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, double* vol,
-                      hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
+                      float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, float* markS,
+                      float* markF, int32_t* rect, double* vol, hipStream_t stream, hipEvent_t ev0,
+                      hipEvent_t ev1);
+int mcs_launch_set_rect(int32_t* rect, hipStream_t stream);
+int mcs_launch_block_max(const float* H, int nx, int ny, int bnx, int bny, float* bmax,
+                         hipStream_t stream);
+int mcs_launch_max_in_disk(const float* H, const float* bmax, int nx, int ny, int bnx, double hx,
+                           double hy, const double* d_q, int n, double* d_out, hipStream_t stream);
 int mcs_launch_sample(const float* H, int nx, int ny, double hx, double hy, const double* d_xy,
                       int n, double* d_out, hipStream_t stream);
 int mcs_launch_fill(float* p, size_t n, float v, hipStream_t stream);
@@ -592,11 +601,14 @@ int check_opts(const mc_bisect_opts* o) {
 
 // ---- stock
 struct StockImpl {
-  int nx = 0, ny = 0;
+  int nx = 0, ny = 0, bnx = 0, bny = 0;
   double w = 0, l = 0, hx = 0, hy = 0, zTop = 0, zB = 0, band = 0;
-  float *H = nullptr, *err = nullptr, *wallD = nullptr;
+  float *H = nullptr, *err = nullptr, *wallD = nullptr, *markS = nullptr, *markF = nullptr;
   int32_t *lineCut = nullptr, *lineWall = nullptr;
-  DevBuf stamps, sdev, box, vol, xy, smp;
+  int32_t* rect = nullptr;   // Stock.dirty {i0, j0, i1, j1} on the device
+  float* bmax = nullptr;     // block maxima, valid while !bmax_stale
+  bool bmax_stale = true;
+  DevBuf stamps, opt, sdev, sx, box, touched, vol, xy, smp, lut, lutmeta, q, qout;
 };
 
 }  // namespace
@@ -869,17 +881,22 @@ int mc_stock_create(double w, double l, double h, double cell, mc_stock** out) {
   s->nx = (int)nxd; s->ny = (int)nyd;
   s->w = w; s->l = l; s->zTop = 0; s->zB = -h;
   s->hx = w / (s->nx - 1); s->hy = l / (s->ny - 1);
+  s->bnx = (s->nx + MC_STOCK_B - 1) / MC_STOCK_B; s->bny = (s->ny + MC_STOCK_B - 1) / MC_STOCK_B;
   const size_t n = (size_t)s->nx * s->ny;
   if (hipMalloc(&s->H, n * 4) != hipSuccess || hipMalloc(&s->err, n * 4) != hipSuccess ||
       hipMalloc(&s->wallD, n * 4) != hipSuccess || hipMalloc(&s->lineCut, n * 4) != hipSuccess ||
-      hipMalloc(&s->lineWall, n * 4) != hipSuccess) {
+      hipMalloc(&s->lineWall, n * 4) != hipSuccess || hipMalloc(&s->markS, n * 4) != hipSuccess ||
+      hipMalloc(&s->markF, n * 4) != hipSuccess || hipMalloc(&s->rect, 16) != hipSuccess ||
+      hipMalloc(&s->bmax, (size_t)s->bnx * s->bny * 4) != hipSuccess) {
     mc_stock_destroy(s);
     return fail(MC_ERR_NOMEM, "stock allocation failed");
   }
   hipStream_t q = g_ctx.stream;
   if (mcs_launch_fill(s->H, n, (float)s->zTop, q) || mcs_launch_fill(s->err, n, 0.0f, q) ||
       mcs_launch_fill(s->wallD, n, INFINITY, q) || mcs_launch_fill((float*)s->lineCut, n, 0.0f, q) ||
-      mcs_launch_fill((float*)s->lineWall, n, 0.0f, q) || hipStreamSynchronize(q) != hipSuccess) {
+      mcs_launch_fill((float*)s->lineWall, n, 0.0f, q) || mcs_launch_fill(s->markS, n, 0.0f, q) ||
+      mcs_launch_fill(s->markF, n, 0.0f, q) || mcs_launch_set_rect(s->rect, q) ||
+      hipStreamSynchronize(q) != hipSuccess) {
     mc_stock_destroy(s);
     return fail(MC_ERR_HIP, "stock initialisation failed");
   }
@@ -889,9 +906,11 @@ int mc_stock_create(double w, double l, double h, double cell, mc_stock** out) {
 
 int mc_stock_destroy(mc_stock* s) {
   if (!s) return MC_OK;
-  for (void* p : {(void*)s->H, (void*)s->err, (void*)s->wallD, (void*)s->lineCut, (void*)s->lineWall})
+  for (void* p : {(void*)s->H, (void*)s->err, (void*)s->wallD, (void*)s->lineCut, (void*)s->lineWall,
+                  (void*)s->markS, (void*)s->markF, (void*)s->rect, (void*)s->bmax})
     if (p) (void)hipFree(p);
-  for (DevBuf* b : {&s->stamps, &s->sdev, &s->box, &s->vol, &s->xy, &s->smp})
+  for (DevBuf* b : {&s->stamps, &s->opt, &s->sdev, &s->sx, &s->box, &s->touched, &s->vol, &s->xy,
+                    &s->smp, &s->lut, &s->lutmeta, &s->q, &s->qout})
     if (b->p) (void)hipFree(b->p);
   delete s;
   return MC_OK;
@@ -921,6 +940,7 @@ int mc_stock_read_h(const mc_stock* s, float* H) {
 int mc_stock_write_h(mc_stock* s, const float* H) {
   if (!s || !H) return fail(MC_ERR_ARG, "null argument");
   HIPCHK(hipMemcpy(s->H, H, (size_t)s->nx * s->ny * 4, hipMemcpyHostToDevice));
+  s->bmax_stale = true;
   return MC_OK;
 }
 
@@ -935,28 +955,79 @@ int mc_stock_read_side(const mc_stock* s, float* err, float* wallD, int32_t* lin
 }
 
 int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol) {
-  if (!s || n < 0 || (n > 0 && !stamps)) return fail(MC_ERR_ARG, "bad stamp batch");
+  return mc_stamp_batch_ex(s, stamps, nullptr, n, nullptr, nullptr, nullptr, 0, vol);
+}
+
+int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* opt, int n,
+                      const float* luts, const int32_t* lut_off, const int32_t* lut_len, int nluts,
+                      double* vol) {
+  if (!s || n < 0 || (n > 0 && !stamps) || nluts < 0 || (nluts > 0 && (!luts || !lut_off || !lut_len)))
+    return fail(MC_ERR_ARG, "bad stamp batch");
+  // profile tables: offsets and lengths in range, finite heights, and their minima (the
+  // lowest reach of a profiled stamp)
+  std::vector<float> lmin((size_t)nluts);
+  int64_t lut_total = 0;
+  for (int t = 0; t < nluts; t++) {
+    if (lut_off[t] < 0 || lut_len[t] < 1 || (int64_t)lut_off[t] + lut_len[t] > (int64_t)INT32_MAX)
+      return fail(MC_ERR_ARG, "profile %d: bad offset or length", t);
+    lut_total = std::max(lut_total, (int64_t)lut_off[t] + lut_len[t]);
+    float m = INFINITY;
+    for (int i = 0; i < lut_len[t]; i++) {
+      const float v = luts[lut_off[t] + i];
+      if (!std::isfinite(v)) return fail(MC_ERR_ARG, "profile %d: entry %d not finite", t, i);
+      m = std::min(m, v);
+    }
+    lmin[t] = m;
+  }
   for (int i = 0; i < n; i++) {
     const mc_stamp& p = stamps[i];
     if (!std::isfinite(p.cx) || !std::isfinite(p.cy) || !std::isfinite(p.tipZ) || !(p.R > 0) ||
         (p.type != MC_TOOL_FLAT && p.type != MC_TOOL_BALL))
       return fail(MC_ERR_ARG, "stamp %d: bad position, radius or type", i);
+    if (opt && (!std::isfinite(opt[i].ux) || !std::isfinite(opt[i].uy) || std::isnan(opt[i].f) ||
+                opt[i].lut < -1 || opt[i].lut >= nluts))
+      return fail(MC_ERR_ARG, "stamp %d: bad feed mark or profile index", i);
   }
   if (n == 0) return MC_OK;
   int st;
-  if ((st = s->stamps.ensure((size_t)n * sizeof(mc_stamp)))) return st;
   const int chunks = (n + mcs_chunk() - 1) / mcs_chunk();
+  if ((st = s->stamps.ensure((size_t)n * sizeof(mc_stamp)))) return st;
   if ((st = s->sdev.ensure((size_t)n * mcs_stampdev_bytes()))) return st;
   if ((st = s->box.ensure((size_t)chunks * mcs_chunkbox_bytes()))) return st;
+  if ((st = s->touched.ensure((size_t)n * 4))) return st;
   if (vol && (st = s->vol.ensure((size_t)n * sizeof(double)))) return st;
+  if (opt) {
+    if ((st = s->opt.ensure((size_t)n * sizeof(mc_stamp_opt)))) return st;
+    if ((st = s->sx.ensure((size_t)n * mcs_stampx_bytes()))) return st;
+    if ((st = s->lut.ensure((size_t)std::max<int64_t>(lut_total, 1) * 4))) return st;
+    if ((st = s->lutmeta.ensure((size_t)std::max(nluts, 1) * 12))) return st;
+  }
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   hipStream_t q = g_ctx.stream;
   HIPCHK(hipMemcpyAsync(s->stamps.p, stamps, (size_t)n * sizeof(mc_stamp), hipMemcpyHostToDevice, q));
+  int32_t *d_off = nullptr, *d_len = nullptr;
+  float* d_min = nullptr;
+  if (opt) {
+    HIPCHK(hipMemcpyAsync(s->opt.p, opt, (size_t)n * sizeof(mc_stamp_opt), hipMemcpyHostToDevice, q));
+    d_off = (int32_t*)s->lutmeta.p;
+    d_len = d_off + std::max(nluts, 1);
+    d_min = (float*)(d_len + std::max(nluts, 1));
+    if (nluts > 0) {
+      HIPCHK(hipMemcpyAsync(s->lut.p, luts, (size_t)lut_total * 4, hipMemcpyHostToDevice, q));
+      HIPCHK(hipMemcpyAsync(d_off, lut_off, (size_t)nluts * 4, hipMemcpyHostToDevice, q));
+      HIPCHK(hipMemcpyAsync(d_len, lut_len, (size_t)nluts * 4, hipMemcpyHostToDevice, q));
+      HIPCHK(hipMemcpyAsync(d_min, lmin.data(), (size_t)nluts * 4, hipMemcpyHostToDevice, q));
+    }
+  }
   if (vol && mcs_launch_fill((float*)s->vol.p, (size_t)n * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
-  if (mcs_launch_stamps((const mc_stamp*)s->stamps.p, n, s->sdev.p, s->box.p, s->nx, s->ny, s->hx, s->hy,
-                        s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut, s->lineWall,
-                        vol ? (double*)s->vol.p : nullptr, q, g_ctx.ev0, g_ctx.ev1))
+  if (mcs_launch_stamps((const mc_stamp*)s->stamps.p, opt ? (const mc_stamp_opt*)s->opt.p : nullptr, n,
+                        opt ? (const float*)s->lut.p : nullptr, d_off, d_len, d_min, s->sdev.p,
+                        opt ? s->sx.p : nullptr, s->box.p, (int32_t*)s->touched.p, s->nx, s->ny, s->hx,
+                        s->hy, s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut, s->lineWall,
+                        s->markS, s->markF, s->rect, vol ? (double*)s->vol.p : nullptr, q, g_ctx.ev0,
+                        g_ctx.ev1))
     return fail(MC_ERR_HIP, "stamp kernels failed: %s", hipGetErrorString(hipGetLastError()));
+  s->bmax_stale = true;
   if (vol) HIPCHK(hipMemcpyAsync(vol, s->vol.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, q));
   HIPCHK(hipStreamSynchronize(q));
   {
@@ -968,6 +1039,72 @@ int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol) {
     const double area = s->hx * s->hy;
     for (int i = 0; i < n; i++) vol[i] *= area;
   }
+  return MC_OK;
+}
+
+int mc_stock_read_marks(const mc_stock* s, float* markS, float* markF) {
+  if (!s) return fail(MC_ERR_ARG, "null stock");
+  const size_t b = (size_t)s->nx * s->ny * 4;
+  if (markS) HIPCHK(hipMemcpy(markS, s->markS, b, hipMemcpyDeviceToHost));
+  if (markF) HIPCHK(hipMemcpy(markF, s->markF, b, hipMemcpyDeviceToHost));
+  return MC_OK;
+}
+
+int mc_stock_dirty(const mc_stock* s, int32_t rect[4]) {
+  if (!s || !rect) return fail(MC_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpy(rect, s->rect, 16, hipMemcpyDeviceToHost));
+  return MC_OK;
+}
+
+int mc_stock_clear_dirty(mc_stock* s) {
+  if (!s) return fail(MC_ERR_ARG, "null stock");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  if (mcs_launch_set_rect(s->rect, g_ctx.stream)) return fail(MC_ERR_HIP, "clear kernel failed");
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return MC_OK;
+}
+
+// block maxima are recomputed from H after any change: the reference's lazily refreshed
+// maxima equal the current block maxima whenever a query reads them
+static int refresh_bmax(mc_stock* s, hipStream_t q) {
+  if (!s->bmax_stale) return MC_OK;
+  if (mcs_launch_block_max(s->H, s->nx, s->ny, s->bnx, s->bny, s->bmax, q))
+    return fail(MC_ERR_HIP, "block max kernel failed");
+  s->bmax_stale = false;
+  return MC_OK;
+}
+
+int mc_stock_block_max(const mc_stock* s, float* bmax) {
+  if (!s || !bmax) return fail(MC_ERR_ARG, "null argument");
+  mc_stock* m = const_cast<mc_stock*>(s);
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  int st;
+  if ((st = refresh_bmax(m, g_ctx.stream))) return st;
+  HIPCHK(hipMemcpyAsync(bmax, s->bmax, (size_t)s->bnx * s->bny * 4, hipMemcpyDeviceToHost, g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return MC_OK;
+}
+
+int mc_stock_max_in_disk(const mc_stock* s, const double* qv, int n, double* out) {
+  if (!s || n < 0 || (n > 0 && (!qv || !out))) return fail(MC_ERR_ARG, "bad query arguments");
+  for (int i = 0; i < n; i++)
+    if (!std::isfinite(qv[4 * i]) || !std::isfinite(qv[4 * i + 1]) || !std::isfinite(qv[4 * i + 2]) ||
+        std::isnan(qv[4 * i + 3]))
+      return fail(MC_ERR_ARG, "query %d: bad centre, radius or floor", i);
+  if (n == 0) return MC_OK;
+  mc_stock* m = const_cast<mc_stock*>(s);
+  int st;
+  if ((st = m->q.ensure((size_t)n * 32))) return st;
+  if ((st = m->qout.ensure((size_t)n * 24))) return st;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  hipStream_t q = g_ctx.stream;
+  if ((st = refresh_bmax(m, q))) return st;
+  HIPCHK(hipMemcpyAsync(m->q.p, qv, (size_t)n * 32, hipMemcpyHostToDevice, q));
+  if (mcs_launch_max_in_disk(s->H, s->bmax, s->nx, s->ny, s->bnx, s->hx, s->hy, (const double*)m->q.p, n,
+                             (double*)m->qout.p, q))
+    return fail(MC_ERR_HIP, "maxInDisk kernel failed");
+  HIPCHK(hipMemcpyAsync(out, m->qout.p, (size_t)n * 24, hipMemcpyDeviceToHost, q));
+  HIPCHK(hipStreamSynchronize(q));
   return MC_OK;
 }
 

@@ -2,7 +2,7 @@
 // in machinig-sim_amd/js (which keep the reference's exported signatures).
 //
 // Data crosses as ArrayBuffers laid out exactly like the C structs of include/mcsim.h
-// (the JS side packs mc_cell / mc_stamp records with a DataView), so the addon only
+// (the JS side packs mc_cell / mc_stamp / mc_stamp_opt records with a DataView), so the addon only
 // forwards pointers and maps a non-zero mc_status to a thrown Error carrying
 // mc_last_error().  bisectBatchAsync runs the search on a libuv worker thread and
 // returns a Promise, for the stability-sweep worker protocol.
@@ -341,8 +341,124 @@ napi_value StockWriteH(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-// stampBatch(handle, stamps: ArrayBuffer of n mc_stamp) -> Float64Array(n) volumes
+bool is_nullish(napi_env env, napi_value v) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  return t == napi_null || t == napi_undefined;
+}
+
+// stampBatch(handle, stamps: ArrayBuffer of n mc_stamp[, opts: ArrayBuffer of n mc_stamp_opt,
+//            luts: Float32Array, lutOff: Int32Array, lutLen: Int32Array]) -> Float64Array(n)
 napi_value StampBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  size_t argc = 6;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok || argc < 2) {
+    napi_throw_type_error(env, nullptr, "missing arguments");
+    return nullptr;
+  }
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* p;
+  size_t len;
+  if (!buffer_of(env, argv[1], &p, &len)) return nullptr;
+  const int n = (int)(len / sizeof(mc_stamp));
+  void* q[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t ql[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 4; k++)
+    if (argc > 2 + (size_t)k && !is_nullish(env, argv[2 + k]) && !buffer_of(env, argv[2 + k], &q[k], &ql[k]))
+      return nullptr;
+  if (q[0] && ql[0] < (size_t)n * sizeof(mc_stamp_opt)) {
+    napi_throw_range_error(env, nullptr, "one mc_stamp_opt per stamp expected");
+    return nullptr;
+  }
+  const int nluts = (int)(ql[2] / 4);
+  if (ql[3] / 4 != (size_t)nluts) {
+    napi_throw_range_error(env, nullptr, "lutOff and lutLen differ in length");
+    return nullptr;
+  }
+  // every profile must lie inside the luts array handed over
+  for (int t = 0; t < nluts; t++) {
+    const int64_t end = (int64_t)((int32_t*)q[2])[t] + ((int32_t*)q[3])[t];
+    if (((int32_t*)q[2])[t] < 0 || end > (int64_t)(ql[1] / 4)) {
+      napi_throw_range_error(env, nullptr, "profile outside the luts array");
+      return nullptr;
+    }
+  }
+  void* vol;
+  napi_value ab = new_ab(env, (size_t)n * 8 + 1, &vol);
+  int st = mc_stamp_batch_ex(s, (const mc_stamp*)p, (const mc_stamp_opt*)q[0], n, (const float*)q[1],
+                             (const int32_t*)q[2], (const int32_t*)q[3], nluts, (double*)vol);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta));
+  return ta;
+}
+
+// stockReadMarks(handle, markS, markF): fills the given Float32Arrays (either may be null)
+napi_value StockReadMarks(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  int32_t nx, ny;
+  mc_stock_dims(s, &nx, &ny, nullptr, nullptr);
+  void* p[2] = {nullptr, nullptr};
+  for (int k = 0; k < 2; k++) {
+    if (is_nullish(env, argv[1 + k])) continue;
+    size_t len;
+    if (!buffer_of(env, argv[1 + k], &p[k], &len)) return nullptr;
+    if (len < (size_t)nx * ny * 4) { napi_throw_range_error(env, nullptr, "array smaller than the grid"); return nullptr; }
+  }
+  int st = mc_stock_read_marks(s, (float*)p[0], (float*)p[1]);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// stockDirty(handle) -> Int32Array [i0, j0, i1, j1]
+napi_value StockDirty(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* out;
+  napi_value ab = new_ab(env, 16, &out);
+  int st = mc_stock_dirty(s, (int32_t*)out);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_int32_array, 4, ab, 0, &ta));
+  return ta;
+}
+
+napi_value StockClearDirty(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  int st = mc_stock_clear_dirty(s);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// stockBlockMax(handle) -> Float32Array(bnx * bny)
+napi_value StockBlockMax(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  int32_t nx, ny;
+  mc_stock_dims(s, &nx, &ny, nullptr, nullptr);
+  const size_t nb = (size_t)((nx + MC_STOCK_B - 1) / MC_STOCK_B) * ((ny + MC_STOCK_B - 1) / MC_STOCK_B);
+  void* out;
+  napi_value ab = new_ab(env, nb * 4, &out);
+  int st = mc_stock_block_max(s, (float*)out);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, nb, ab, 0, &ta));
+  return ta;
+}
+
+// stockMaxInDisk(handle, q: Float64Array(4n) {cx, cy, r, above}) -> Float64Array(3n) {h, x, y}
+napi_value StockMaxInDisk(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return nullptr;
   mc_stock* s = stock_of(env, argv[0]);
@@ -350,13 +466,36 @@ napi_value StampBatch(napi_env env, napi_callback_info info) {
   void* p;
   size_t len;
   if (!buffer_of(env, argv[1], &p, &len)) return nullptr;
-  const int n = (int)(len / sizeof(mc_stamp));
-  void* vol;
-  napi_value ab = new_ab(env, (size_t)n * 8 + 1, &vol);
-  int st = mc_stamp_batch(s, (const mc_stamp*)p, n, (double*)vol);
+  const int n = (int)(len / 32);
+  void* out;
+  napi_value ab = new_ab(env, (size_t)n * 24 + 1, &out);
+  int st = mc_stock_max_in_disk(s, (const double*)p, n, (double*)out);
   if (st) return throw_mc(env, st);
   napi_value ta;
-  NAPI_OK(napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta));
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)n * 3, ab, 0, &ta));
+  return ta;
+}
+
+// tipLut(shape, D, cornerR, edgeR: Float64Array | null, edgeDz) -> Float32Array
+napi_value TipLut(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return nullptr;
+  int32_t shape = 0;
+  napi_get_value_int32(env, argv[0], &shape);
+  void* er = nullptr;
+  size_t erl = 0;
+  if (!is_nullish(env, argv[3]) && !buffer_of(env, argv[3], &er, &erl)) return nullptr;
+  int32_t n = 0;
+  int st = mc_tip_lut(shape, num(env, argv[1]), num(env, argv[2]), (const double*)er, (int32_t)(erl / 8),
+                      num(env, argv[4]), nullptr, 0, &n);
+  if (st) return throw_mc(env, st);
+  void* out;
+  napi_value ab = new_ab(env, (size_t)n * 4, &out);
+  st = mc_tip_lut(shape, num(env, argv[1]), num(env, argv[2]), (const double*)er, (int32_t)(erl / 8),
+                  num(env, argv[4]), (float*)out, n, &n);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, (size_t)n, ab, 0, &ta));
   return ta;
 }
 
@@ -384,7 +523,7 @@ napi_value Sizes(napi_env env, napi_callback_info) {
   NAPI_OK(napi_create_object(env, &o));
   const struct { const char* k; size_t s; } t[] = {
       {"mc_cell", sizeof(mc_cell)}, {"mc_trial", sizeof(mc_trial)}, {"mc_limit", sizeof(mc_limit)},
-      {"mc_stamp", sizeof(mc_stamp)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
+      {"mc_stamp", sizeof(mc_stamp)}, {"mc_stamp_opt", sizeof(mc_stamp_opt)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
   for (auto& e : t) {
     napi_create_uint32(env, (uint32_t)e.s, &v);
     napi_set_named_property(env, o, e.k, v);
@@ -400,6 +539,8 @@ napi_value ModuleInit(napi_env env, napi_value exports) {
       {"engineTrace", EngineTrace}, {"stockCreate", StockCreate}, {"stockDestroy", StockDestroy},
       {"stockDims", StockDims}, {"stockSetWallBand", StockSetWallBand}, {"stockRead", StockRead},
       {"stockWriteH", StockWriteH}, {"stampBatch", StampBatch}, {"stockSample", StockSample},
+      {"stockReadMarks", StockReadMarks}, {"stockDirty", StockDirty}, {"stockClearDirty", StockClearDirty},
+      {"stockBlockMax", StockBlockMax}, {"stockMaxInDisk", StockMaxInDisk}, {"tipLut", TipLut},
       {"sizes", Sizes}};
   for (auto& f : fns) {
     napi_value fn;

@@ -1,54 +1,72 @@
-// stock.hip — heightmap stock removal (Stock.stamp, src/stock.js:118-164) on gfx950.
+// stock.hip — heightmap stock removal (Stock.stamp, src/stock.js:118-164) and the
+// stock's queries (blockMax, maxInDisk, sample: src/stock.js:45-92) on gfx950.
 //
 // The heightfield and its side maps stay resident in HBM.  A batch of n stamps is applied
 // with the reference's sequential semantics by giving every grid node one thread that
 // visits, in stamp order, exactly the stamps whose bounding box covers it; nodes are
-// independent, so the final H, err, wallD, lineCut and lineWall are bit-identical to n
-// successive Stock.stamp calls, while each node's state is read from and written to HBM
-// once per batch (the tile keeps it in registers).
+// independent, so the final H, err, wallD, lineCut, lineWall, markS and markF are
+// bit-identical to n successive Stock.stamp calls, while each node's state is read from
+// and written to HBM once per batch (the tile keeps it in registers).
 //
-//   stamp_prep   per stamp: bounding box (the reference's floor/ceil rule), radii and
-//                rim height; per chunk of CHUNK stamps: union bounding box.
+//   stamp_prep   per stamp: bounding box (the reference's floor/ceil rule), radii, rim
+//                height and lowest reach; per chunk of CHUNK stamps: union bounding box.
 //   stamp_tiles  one 16x16 node tile per workgroup: walks the chunks whose box meets the
 //                tile, compacts (in order, with wave ballots) the stamps of a chunk that
-//                meet the tile and can still cut it into LDS, and lets each node thread
-//                apply them in order.  A stamp can cut a node only where H > zmin (its
-//                lowest reach), so stamps whose zmin is at or above the tile's highest node
-//                are dropped, and each wave skips the stamps at or above its own highest
-//                node: exact, since those stamps would change nothing.
+//                meet the tile and can still change it into LDS, and lets each node thread
+//                apply them in order.  A stamp can change a node only where H > zmin (its
+//                lowest reach: the cut, the wall test and the feed-mark test all need it),
+//                so stamps whose zmin is at or above the tile's highest node are dropped,
+//                and each wave skips the stamps at or above its own highest node: exact,
+//                since those stamps would change nothing.  The EX variant adds tip
+//                profiles and feed marks.
+//   dirty_rect   union of the boxes of the stamps that changed a node (Stock.dirty).
 //
 // Removed volume per stamp (the value stamp() returns) is summed with one FP64 atomic per
 // wavefront and stamp, so its last bits depend on the summation order.
 #include <hip/hip_runtime.h>
 
 #include "../../include/mcsim.h"
+#include "mc_math.h"
 
 namespace mcs {
 
 constexpr int TILE = 16;      // 16 x 16 nodes per workgroup
 constexpr int CHUNK = 256;    // stamps compacted into LDS at a time
 constexpr double EDGE_TIE = 2 * 0.0015;  // src/stock.js:6
+constexpr double LUT_DR = MC_LUT_DR;     // src/assembly.js:8
 
 struct StampDev {
   double cx, cy, tipZ, R, R2, tie, Rw2, zRim;
-  double zmin;   // lowest z the stamp can cut to: both of its updates need H above it
+  double zmin;   // lowest z the stamp can reach: every update it makes needs H above it
   float errVal;
   int32_t line, type, i0, i1, j0, j1, index;
+};
+
+// tip profile and feed marks of a stamp (the EX variant only)
+struct StampX {
+  double ux, uy, mf;       // mf: mark spacing, 0 = no marks (src/stock.js:128)
+  int32_t lut_off, lut_n;  // profile table in the batch's LUT buffer, lut_n = 0: none
 };
 
 struct ChunkBox {
   int32_t i0, i1, j0, j1;
 };
 
-__global__ void stamp_prep(const mc_stamp* __restrict__ in, int n, double hx, double hy, int nx,
-                           int ny, double zTop, double band, StampDev* __restrict__ out,
-                           ChunkBox* __restrict__ box) {
+__global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* __restrict__ opt,
+                           const float* __restrict__ luts, const int32_t* __restrict__ lut_off,
+                           const int32_t* __restrict__ lut_len, const float* __restrict__ lut_min,
+                           int n, double hx, double hy, int nx, int ny, double zTop, double band,
+                           StampDev* __restrict__ out, StampX* __restrict__ outx,
+                           int32_t* __restrict__ touched, ChunkBox* __restrict__ box) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ int bi0[256], bi1[256], bj0[256], bj1[256];
   int i0 = 1 << 30, i1 = -1, j0 = 1 << 30, j1 = -1;
   if (s < n) {
     const mc_stamp st = in[s];
     StampDev d;
+    StampX x{0, 0, 0, 0, 0};
+    const int t = opt ? opt[s].lut : -1;
+    if (t >= 0) { x.lut_off = lut_off[t]; x.lut_n = lut_len[t]; }
     d.cx = st.cx; d.cy = st.cy; d.tipZ = st.tipZ; d.R = st.R;
     d.type = st.type; d.line = st.line; d.errVal = (float)st.errVal; d.index = s;
     const double m = 2 * (hx > hy ? hx : hy);
@@ -56,10 +74,18 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, int n, double hx, do
     d.Rw2 = Rw * Rw;
     d.R2 = st.R * st.R;
     d.tie = d.R2 - EDGE_TIE * st.R;
-    d.zRim = st.tipZ + (st.type == MC_TOOL_BALL ? st.R : 0);
-    // a ball cuts to z = (tipZ + R) - sqrt(R^2 - r2) >= (tipZ + R) - R, a flat one to tipZ; the
+    const bool ball = st.type == MC_TOOL_BALL;
+    d.zRim = st.tipZ + (t >= 0 ? (double)luts[x.lut_off + x.lut_n - 1] : ball ? st.R : 0);
+    if (opt && opt[s].f > 0 && st.type == MC_TOOL_FLAT && t < 0) {
+      x.ux = opt[s].ux; x.uy = opt[s].uy; x.mf = opt[s].f;
+    }
+    // lowest reach: a profile cuts to tipZ + lut[q] >= tipZ + min(lut) (rounding is
+    // monotone), a ball to (tipZ + R) - sqrt(R^2 - r2) >= (tipZ + R) - R, a flat one to
+    // tipZ, and a flat one with feed marks rewrites the marks where H > tipZ - 0.002; the
     // wall update needs H > zRim >= zmin too, so a node with H <= zmin is left as it is
-    d.zmin = !(st.R >= 0) ? -__builtin_inf() : st.type == MC_TOOL_BALL ? (st.tipZ + st.R) - st.R : st.tipZ;
+    d.zmin = t >= 0 ? st.tipZ + (double)lut_min[t]
+           : ball ? (st.tipZ + st.R) - st.R
+           : x.mf != 0 ? st.tipZ - 0.002 : st.tipZ;
     const double a = __builtin_floor((st.cx - Rw) / hx), b = __builtin_ceil((st.cx + Rw) / hx);
     const double c = __builtin_floor((st.cy - Rw) / hy), e = __builtin_ceil((st.cy + Rw) / hy);
     const bool live = st.tipZ < zTop && a <= nx - 1 && c <= ny - 1 && b >= 0 && e >= 0;
@@ -68,6 +94,8 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, int n, double hx, do
     d.j0 = live ? (int)(c > 0 ? c : 0) : 1;
     d.j1 = live ? (int)(e < ny - 1 ? e : ny - 1) : 0;
     out[s] = d;
+    if (outx) outx[s] = x;
+    touched[s] = 0;
     if (d.i0 <= d.i1 && d.j0 <= d.j1) { i0 = d.i0; i1 = d.i1; j0 = d.j0; j1 = d.j1; }
   }
   bi0[threadIdx.x] = i0; bi1[threadIdx.x] = i1; bj0[threadIdx.x] = j0; bj1[threadIdx.x] = j1;
@@ -83,21 +111,28 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, int n, double hx, do
   if (threadIdx.x == 0) box[blockIdx.x] = ChunkBox{bi0[0], bi1[0], bj0[0], bj1[0]};
 }
 
+// EX: tip profiles (luts) and feed marks (markS / markF) are live in this batch
+template <bool EX>
 __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
-    const StampDev* __restrict__ st, const ChunkBox* __restrict__ box, int n, int nx, int ny,
-    double hx, double hy, double zB, float* __restrict__ H, float* __restrict__ err,
-    float* __restrict__ wallD, int32_t* __restrict__ lineCut, int32_t* __restrict__ lineWall,
-    double* __restrict__ vol) {
+    const StampDev* __restrict__ st, const StampX* __restrict__ stx, const ChunkBox* __restrict__ box,
+    int n, int nx, int ny, double hx, double hy, double zB, float* __restrict__ H,
+    float* __restrict__ err, float* __restrict__ wallD, int32_t* __restrict__ lineCut,
+    int32_t* __restrict__ lineWall, float* __restrict__ markS, float* __restrict__ markF,
+    const float* __restrict__ luts, int32_t* __restrict__ touched, double* __restrict__ vol) {
   __shared__ StampDev list[CHUNK];
+  __shared__ StampX listx[EX ? CHUNK : 1];
   const int t = threadIdx.x;
   const int ti0 = blockIdx.x * TILE, tj0 = blockIdx.y * TILE;
   const int ti1 = min(ti0 + TILE, nx) - 1, tj1 = min(tj0 + TILE, ny) - 1;
   const int i = ti0 + (t % TILE), j = tj0 + (t / TILE);
   const bool own = i <= ti1 && j <= tj1;
   const int k = j * nx + i;
-  float h = 0, e = 0, wd = 0;
+  float h = 0, e = 0, wd = 0, ms = 0, mfv = 0;
   int lc = 0, lw = 0;
-  if (own) { h = H[k]; e = err[k]; wd = wallD[k]; lc = lineCut[k]; lw = lineWall[k]; }
+  if (own) {
+    h = H[k]; e = err[k]; wd = wallD[k]; lc = lineCut[k]; lw = lineWall[k];
+    if (EX) { ms = markS[k]; mfv = markF[k]; }
+  }
   const double xi = i * hx, yj = j * hy;
   const int nchunks = (n + CHUNK - 1) / CHUNK;
   const int lane = t & 63, wv = t >> 6;
@@ -114,16 +149,18 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
     // order-preserving compaction of the chunk's stamps that meet this tile and reach
     // below its highest node
     const int s = c * CHUNK + t;
-    StampDev d;
     bool cand = false;
+    double zmin = 0;
     if (s < n) {
-      d = st[s];
-      cand = !(d.i1 < ti0 || d.i0 > ti1 || d.j1 < tj0 || d.j0 > tj1) && d.i0 <= d.i1 && d.j0 <= d.j1;
+      const StampDev& d = st[s];
+      const int i0 = d.i0, i1 = d.i1, j0 = d.j0, j1 = d.j1;
+      cand = !(i1 < ti0 || i0 > ti1 || j1 < tj0 || j0 > tj1) && i0 <= i1 && j0 <= j1;
+      zmin = d.zmin;
     }
     __syncthreads();
     float tmax = wmaxs[0];
     for (int v = 1; v < TILE * TILE / 64; v++) tmax = fmaxf(tmax, wmaxs[v]);
-    const bool hit = cand && (double)tmax > d.zmin;
+    const bool hit = cand && (double)tmax > zmin;
     const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
     if (lane == 0) wcount[wv] = __builtin_popcountll(m);
     __syncthreads();
@@ -132,31 +169,54 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
       before += v < wv ? wcount[v] : 0;
       total += wcount[v];
     }
-    if (hit) list[before + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0))] = d;
+    if (hit) {
+      const int slot = before + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+      list[slot] = st[s];
+      if (EX) listx[slot] = stx[s];
+    }
     __syncthreads();
     for (int q = 0; q < total; q++) {
       const StampDev& p = list[q];
       if ((double)wmax <= p.zmin) continue;   // wave-uniform: no node of this wave can change
       double removed = 0;
+      bool changed = false;
       if (own && i >= p.i0 && i <= p.i1 && j >= p.j0 && j <= p.j1) {
         const double dx = xi - p.cx, dy = yj - p.cy;
         const double r2 = dx * dx + dy * dy;
         if (r2 >= p.tie) {
           if (r2 <= p.Rw2 && (double)h > p.zRim) {
             const double dd = __builtin_sqrt(r2) - p.R;
-            if (dd < (double)wd) { wd = (float)dd; lw = p.line; }
+            if (dd < (double)wd) { wd = (float)dd; lw = p.line; changed = true; }
           }
         } else {
-          const double z = p.type == MC_TOOL_BALL ? p.tipZ + p.R - __builtin_sqrt(p.R2 - r2) : p.tipZ;
+          double z;
+          if (EX && listx[q].lut_n > 0) {
+            const StampX& x = listx[q];
+            const double r = mcm::round_js(__builtin_sqrt(r2) / LUT_DR);
+            z = p.tipZ + (double)luts[x.lut_off + (r < x.lut_n - 1 ? (int)r : x.lut_n - 1)];
+          } else {
+            z = p.type == MC_TOOL_BALL ? p.tipZ + p.R - __builtin_sqrt(p.R2 - r2) : p.tipZ;
+          }
           if ((double)h > z) {
             removed = (double)h - z;
             h = (float)(z > zB ? z : zB);
             e = p.errVal;
             wd = __builtin_inff();
             lc = p.line;
+            changed = true;
+          }
+          // the last pass over a floor at this depth leaves its trailing-edge arcs
+          if (EX && listx[q].mf != 0 && (double)h > z - 0.002) {
+            const StampX& x = listx[q];
+            const double pp = -dx * x.uy + dy * x.ux;
+            const double u = p.R2 - pp * pp;
+            ms = (float)(xi * x.ux + yj * x.uy + __builtin_sqrt(u > 0 ? u : 0.0));
+            mfv = (float)x.mf;
+            changed = true;
           }
         }
       }
+      if (__any(changed) && lane == 0) touched[p.index] = 1;
       if (vol && __any(removed != 0.0)) {
         double r = removed;
         for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
@@ -165,7 +225,31 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
     }
     __syncthreads();
   }
-  if (own) { H[k] = h; err[k] = e; wallD[k] = wd; lineCut[k] = lc; lineWall[k] = lw; }
+  if (own) {
+    H[k] = h; err[k] = e; wallD[k] = wd; lineCut[k] = lc; lineWall[k] = lw;
+    if (EX) { markS[k] = ms; markF[k] = mfv; }
+  }
+}
+
+// Stock.dirty (src/stock.js:156-160): union of the boxes of the stamps that changed a node,
+// folded into rect = {i0, j0, i1, j1}
+__global__ void dirty_rect(const StampDev* __restrict__ st, const int32_t* __restrict__ touched,
+                           int n, int32_t* __restrict__ rect) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int i0 = 0x7fffffff, j0 = 0x7fffffff, i1 = -1, j1 = -1;
+  if (s < n && touched[s]) { i0 = st[s].i0; j0 = st[s].j0; i1 = st[s].i1; j1 = st[s].j1; }
+  for (int o = 32; o > 0; o >>= 1) {
+    i0 = min(i0, __shfl_xor(i0, o, 64)); j0 = min(j0, __shfl_xor(j0, o, 64));
+    i1 = max(i1, __shfl_xor(i1, o, 64)); j1 = max(j1, __shfl_xor(j1, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0 && i1 >= 0) {
+    atomicMin(&rect[0], i0); atomicMin(&rect[1], j0);
+    atomicMax(&rect[2], i1); atomicMax(&rect[3], j1);
+  }
+}
+
+__global__ void set_rect(int32_t* rect) {
+  if (threadIdx.x == 0) { rect[0] = 0x7fffffff; rect[1] = 0x7fffffff; rect[2] = -1; rect[3] = -1; }
 }
 
 }  // namespace mcs
@@ -188,6 +272,87 @@ __global__ void stock_sample(const float* __restrict__ H, int nx, int ny, double
   out[q] = (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty;
 }
 
+constexpr int BLK = MC_STOCK_B;   // block edge of the interference maxima (src/stock.js:35)
+
+// Stock.blockMax (src/stock.js:45-54) for every block: the reference's scan order and
+// strict '>' (so which of +0 / -0 wins is the reference's too)
+__global__ void block_max(const float* __restrict__ H, int nx, int ny, int bnx, int bny,
+                          float* __restrict__ bmax) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= bnx * bny) return;
+  const int bi = b % bnx, bj = b / bnx;
+  float m = -__builtin_inff();
+  const int ie = min(nx, bi * BLK + BLK), je = min(ny, bj * BLK + BLK);
+  for (int j = bj * BLK; j < je; j++)
+    for (int i = bi * BLK; i < ie; i++) {
+      const float v = H[j * nx + i];
+      if (v > m) m = v;
+    }
+  bmax[b] = m;
+}
+
+// Stock.maxInDisk (src/stock.js:57-80), one wavefront per query.  The reference walks the
+// blocks of the disk's bounding box in row-major order and lets a block replace the best
+// answer when its maximum m exceeds max(best, above): a block wholly inside the disk
+// answers m at its centre, an edge block its highest node inside the disk (first in scan
+// order).  Since m >= every node of its block, this equals: among the blocks with
+// m > above, the first (in walk order) whose candidate is the largest, provided that
+// candidate exceeds -inf.  Lanes take 64 blocks at a time; a (value, order) reduction
+// with strict '>' keeps the first of equal candidates.
+__global__ __launch_bounds__(256) void max_in_disk(
+    const float* __restrict__ H, const float* __restrict__ bmax, int nx, int ny, int bnx,
+    double hx, double hy, const double* __restrict__ q, int nq, double* __restrict__ out) {
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (w >= nq) return;   // wave-uniform
+  const double cx = q[4 * w], cy = q[4 * w + 1], r = q[4 * w + 2], above = q[4 * w + 3];
+  double a = __builtin_floor((cx - r) / hx), b = __builtin_ceil((cx + r) / hx);
+  double c = __builtin_floor((cy - r) / hy), d = __builtin_ceil((cy + r) / hy);
+  a = a > 0 ? a : 0; b = b < nx - 1 ? b : nx - 1;
+  c = c > 0 ? c : 0; d = d < ny - 1 ? d : ny - 1;
+  double best = -__builtin_inf(), bx = 0, by = 0;
+  if (a <= b && c <= d) {
+    const int i0 = (int)a, i1 = (int)b, j0 = (int)c, j1 = (int)d;
+    const int bi0 = i0 / BLK, bj0 = j0 / BLK;
+    const int nbi = i1 / BLK - bi0 + 1, nb = nbi * (j1 / BLK - bj0 + 1);
+    const double r2 = r * r;
+    for (int base = 0; base < nb; base += 64) {
+      const int t = base + lane;
+      double v = -__builtin_inf(), vx = 0, vy = 0;
+      if (t < nb) {
+        const int bi = bi0 + t % nbi, bj = bj0 + t / nbi;
+        const double m = bmax[bj * bnx + bi];
+        if (m > above) {
+          const double xa = (double)(bi * BLK) * hx - cx, xb = (double)min(nx - 1, bi * BLK + BLK - 1) * hx - cx;
+          const double ya = (double)(bj * BLK) * hy - cy, yb = (double)min(ny - 1, bj * BLK + BLK - 1) * hy - cy;
+          const double far = (xa * xa > xb * xb ? xa * xa : xb * xb) + (ya * ya > yb * yb ? ya * ya : yb * yb);
+          if (far <= r2) {
+            v = m; vx = cx + (xa + xb) / 2; vy = cy + (ya + yb) / 2;
+          } else {
+            const int ja = max(j0, bj * BLK), jb = min(j1, bj * BLK + BLK - 1);
+            const int ia = max(i0, bi * BLK), ib = min(i1, bi * BLK + BLK - 1);
+            for (int j = ja; j <= jb; j++) {
+              const double dy = j * hy - cy;
+              for (int i = ia; i <= ib; i++) {
+                const double dx = i * hx - cx, h = H[j * nx + i];
+                if (h > v && dx * dx + dy * dy <= r2) { v = h; vx = i * hx; vy = j * hy; }
+              }
+            }
+          }
+        }
+      }
+      // first of the largest: lane order is walk order
+      int order = t;
+      for (int o = 1; o < 64; o <<= 1) {
+        const double ov = __shfl_xor(v, o, 64), ox = __shfl_xor(vx, o, 64), oy = __shfl_xor(vy, o, 64);
+        const int oo = __shfl_xor(order, o, 64);
+        if (ov > v || (ov == v && oo < order)) { v = ov; vx = ox; vy = oy; order = oo; }
+      }
+      if (v > best) { best = v; bx = vx; by = vy; }
+    }
+  }
+  if (lane == 0) { out[3 * w] = best; out[3 * w + 1] = bx; out[3 * w + 2] = by; }
+}
+
 __global__ void fill_f32(float* p, size_t n, float v) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) p[q] = v;
@@ -198,22 +363,59 @@ __global__ void fill_f32(float* p, size_t n, float v) {
 // ---- host-side launchers used by api.cpp
 size_t mcs_stampdev_bytes() { return sizeof(mcs::StampDev); }
 size_t mcs_chunkbox_bytes() { return sizeof(mcs::ChunkBox); }
+size_t mcs_stampx_bytes() { return sizeof(mcs::StampX); }
 int mcs_chunk() { return mcs::CHUNK; }
 
-int mcs_launch_stamps(const mc_stamp* d_in, int n, void* d_stampdev, void* d_box, int nx, int ny,
+int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, const float* luts,
+                      const int32_t* lut_off, const int32_t* lut_len, const float* lut_min,
+                      void* d_stampdev, void* d_stampx, void* d_box, int32_t* touched, int nx, int ny,
                       double hx, double hy, double zTop, double zB, double band, float* H,
-                      float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, double* vol,
-                      hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
+                      float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, float* markS,
+                      float* markF, int32_t* rect, double* vol, hipStream_t stream, hipEvent_t ev0,
+                      hipEvent_t ev1) {
   if (n <= 0) return 0;
   const int chunks = (n + mcs::CHUNK - 1) / mcs::CHUNK;
-  hipLaunchKernelGGL(mcs::stamp_prep, dim3(chunks), dim3(mcs::CHUNK), 0, stream, d_in, n, hx, hy,
-                     nx, ny, zTop, band, (mcs::StampDev*)d_stampdev, (mcs::ChunkBox*)d_box);
+  const bool ex = d_opt != nullptr;
+  hipLaunchKernelGGL(mcs::stamp_prep, dim3(chunks), dim3(mcs::CHUNK), 0, stream, d_in, d_opt, luts,
+                     lut_off, lut_len, lut_min, n, hx, hy, nx, ny, zTop, band,
+                     (mcs::StampDev*)d_stampdev, ex ? (mcs::StampX*)d_stampx : nullptr, touched,
+                     (mcs::ChunkBox*)d_box);
   const dim3 grid((nx + mcs::TILE - 1) / mcs::TILE, (ny + mcs::TILE - 1) / mcs::TILE);
   if (ev0 && hipEventRecord(ev0, stream) != hipSuccess) return -2;
-  hipLaunchKernelGGL(mcs::stamp_tiles, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
-                     (const mcs::StampDev*)d_stampdev, (const mcs::ChunkBox*)d_box, n, nx, ny, hx,
-                     hy, zB, H, err, wallD, lineCut, lineWall, vol);
+  if (ex)
+    hipLaunchKernelGGL(mcs::stamp_tiles<true>, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
+                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)d_stampx,
+                       (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy, zB, H, err, wallD, lineCut,
+                       lineWall, markS, markF, luts, touched, vol);
+  else
+    hipLaunchKernelGGL(mcs::stamp_tiles<false>, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
+                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)nullptr,
+                       (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy, zB, H, err, wallD, lineCut,
+                       lineWall, markS, markF, luts, touched, vol);
   if (ev1 && hipEventRecord(ev1, stream) != hipSuccess) return -2;
+  hipLaunchKernelGGL(mcs::dirty_rect, dim3((n + 255) / 256), dim3(256), 0, stream,
+                     (const mcs::StampDev*)d_stampdev, (const int32_t*)touched, n, rect);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mcs_launch_set_rect(int32_t* rect, hipStream_t stream) {
+  hipLaunchKernelGGL(mcs::set_rect, dim3(1), dim3(64), 0, stream, rect);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mcs_launch_block_max(const float* H, int nx, int ny, int bnx, int bny, float* bmax,
+                         hipStream_t stream) {
+  const int nb = bnx * bny;
+  hipLaunchKernelGGL(mcs::block_max, dim3((nb + 255) / 256), dim3(256), 0, stream, H, nx, ny, bnx,
+                     bny, bmax);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mcs_launch_max_in_disk(const float* H, const float* bmax, int nx, int ny, int bnx, double hx,
+                           double hy, const double* d_q, int n, double* d_out, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(mcs::max_in_disk, dim3((n + 3) / 4), dim3(256), 0, stream, H, bmax, nx, ny,
+                     bnx, hx, hy, d_q, n, d_out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -1,16 +1,18 @@
 // Packing of the reference's plain argument objects into the C records of
-// include/mcsim.h (mc_cell 352 B, mc_stamp 48 B), little-endian, for the addon.
+// include/mcsim.h (mc_cell 352 B, mc_stamp 48 B, mc_stamp_opt 32 B), little-endian, for the
+// addon.
 import { modeSet } from './frf.js';
 import { native } from './native.js';
 
 export const MC_MAX_MODES = 4, MC_MAX_FLUTES = 8;
-export const CELL_BYTES = 352, TRIAL_BYTES = 40, LIMIT_BYTES = 56, STAMP_BYTES = 48;
+export const CELL_BYTES = 352, TRIAL_BYTES = 40, LIMIT_BYTES = 56, STAMP_BYTES = 48, STAMP_OPT_BYTES = 32;
 
 let checked = false;
 function checkSizes() {
   if (checked) return;
   const s = native().sizes();
-  if (s.mc_cell !== CELL_BYTES || s.mc_trial !== TRIAL_BYTES || s.mc_limit !== LIMIT_BYTES || s.mc_stamp !== STAMP_BYTES)
+  if (s.mc_cell !== CELL_BYTES || s.mc_trial !== TRIAL_BYTES || s.mc_limit !== LIMIT_BYTES || s.mc_stamp !== STAMP_BYTES ||
+      s.mc_stamp_opt !== STAMP_OPT_BYTES)
     throw new Error('mcsim.node record sizes do not match this module: ' + JSON.stringify(s));
   checked = true;
 }
@@ -88,4 +90,31 @@ export function packStamps(list) {
     dv.setInt32(o + 36, s.line | 0, true); dv.setFloat64(o + 40, s.errVal, true);
   });
   return buf;
+}
+
+// the mark / lut arguments of the stamps in list: {opts: ArrayBuffer of mc_stamp_opt, luts,
+// lutOff, lutLen}, each distinct profile table (by identity) packed once; null when no
+// stamp has either
+export function packStampOpts(list) {
+  if (!list.some(s => s.mark || s.lut)) return null;
+  checkSizes();
+  const buf = new ArrayBuffer(STAMP_OPT_BYTES * list.length), dv = new DataView(buf);
+  const index = new Map(), tabs = [];
+  list.forEach((s, i) => {
+    const o = i * STAMP_OPT_BYTES, m = s.mark;
+    dv.setFloat64(o, m ? m.ux : 0, true); dv.setFloat64(o + 8, m ? m.uy : 0, true);
+    dv.setFloat64(o + 16, m && m.f > 0 ? m.f : 0, true);
+    let t = -1;
+    if (s.lut) {
+      if (!index.has(s.lut)) { index.set(s.lut, tabs.length); tabs.push(s.lut); }
+      t = index.get(s.lut);
+    }
+    dv.setInt32(o + 24, t, true); dv.setInt32(o + 28, 0, true);
+  });
+  const lutOff = new Int32Array(tabs.length), lutLen = new Int32Array(tabs.length);
+  let total = 0;
+  tabs.forEach((t, i) => { lutOff[i] = total; lutLen[i] = t.length; total += t.length; });
+  const luts = new Float32Array(Math.max(1, total));
+  tabs.forEach((t, i) => luts.set(t, lutOff[i]));
+  return { opts: buf, luts, lutOff, lutLen };
 }

@@ -59,6 +59,16 @@ class Stamp(C.Structure):
                 ("errVal", C.c_double)]
 
 
+class StampOpt(C.Structure):
+    _fields_ = [("ux", C.c_double), ("uy", C.c_double), ("f", C.c_double),
+                ("lut", C.c_int32), ("pad", C.c_int32)]
+
+
+MC_STOCK_B = 8
+MC_LUT_DR = 0.02
+MC_TIP_FLAT, MC_TIP_BALL, MC_TIP_BULL, MC_TIP_EDGE = 0, 1, 2, 3
+
+
 def make_cell(tool, modes_x, modes_y, mat, S, arc, fz, revs, nonlinear=True):
     """Build a Cell from plain values.
 

@@ -8,8 +8,8 @@ import os
 
 import numpy as np
 
-from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, TrainOpts, LobeArgs, Mode, SearchTuning,
-                   cell_array, MC_OK, MC_SURR_IN)
+from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
+                   SearchTuning, cell_array, MC_OK, MC_SURR_IN, MC_STOCK_B)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -22,7 +22,8 @@ EXPORTS = [
     "mc_mlp_create", "mc_mlp_destroy", "mc_mlp_nparams", "mc_mlp_params", "mc_mlp_set_scaling",
     "mc_mlp_get_scaling", "mc_mlp_grads", "mc_mlp_train", "mc_mlp_predict",
     "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
-    "mc_set_search_tuning", "mc_get_search_tuning",
+    "mc_set_search_tuning", "mc_get_search_tuning", "mc_stamp_batch_ex", "mc_stock_read_marks",
+    "mc_stock_dirty", "mc_stock_clear_dirty", "mc_stock_block_max", "mc_stock_max_in_disk", "mc_tip_lut",
 ]
 
 _LIB = None
@@ -61,6 +62,14 @@ def load():
     L.mc_stock_read_side.argtypes = [P, P, P, P, P]
     L.mc_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
     L.mc_stock_sample.argtypes = [P, P, C.c_int, P]
+    L.mc_stamp_batch_ex.argtypes = [P, C.POINTER(Stamp), P, C.c_int, P, P, P, C.c_int, P]
+    L.mc_stock_read_marks.argtypes = [P, P, P]
+    L.mc_stock_dirty.argtypes = [P, P]
+    L.mc_stock_clear_dirty.argtypes = [P]
+    L.mc_stock_block_max.argtypes = [P, P]
+    L.mc_stock_max_in_disk.argtypes = [P, P, C.c_int, P]
+    L.mc_tip_lut.argtypes = [C.c_int32, C.c_double, C.c_double, P, C.c_int32, C.c_double, P, C.c_int32,
+                             C.POINTER(C.c_int32)]
     L.mc_surrogate_features.argtypes = [C.POINTER(Cell), C.c_int, P]
     L.mc_mlp_create.argtypes = [C.c_uint64, C.POINTER(P)]
     L.mc_mlp_destroy.argtypes = [P]
@@ -211,6 +220,59 @@ class Stock:
         out = np.empty(len(xy))
         _check(self.L.mc_stock_sample(self.p, xy.ctypes.data, len(xy), out.ctypes.data))
         return out
+
+    def stamps_ex(self, stamps, opts, n=None, luts=None, lut_off=None, lut_len=None, want_vol=True):
+        """Stamps with feed marks and tip profiles: opts is a StampOpt array (or None);
+        luts the concatenated f32 profiles with int32 offsets / lengths per profile."""
+        n = len(stamps) if n is None else n
+        vol = np.zeros(n) if want_vol else None
+        keep = [np.ascontiguousarray(a, dt) if a is not None else None
+                for a, dt in ((luts, np.float32), (lut_off, np.int32), (lut_len, np.int32))]
+        nl = 0 if keep[2] is None else len(keep[2])
+        ptr = [a.ctypes.data if a is not None and a.size else None for a in keep]
+        _check(self.L.mc_stamp_batch_ex(self.p, stamps, opts, n, ptr[0], ptr[1], ptr[2], nl,
+                                        vol.ctypes.data if want_vol else None))
+        return vol
+
+    def marks(self):
+        n = self.nx * self.ny
+        ms, mf = np.empty(n, np.float32), np.empty(n, np.float32)
+        _check(self.L.mc_stock_read_marks(self.p, ms.ctypes.data, mf.ctypes.data))
+        return ms, mf
+
+    def dirty(self):
+        r = (C.c_int32 * 4)()
+        _check(self.L.mc_stock_dirty(self.p, r))
+        return list(r)
+
+    def clear_dirty(self):
+        _check(self.L.mc_stock_clear_dirty(self.p))
+
+    def block_maxima(self):
+        nb = ((self.nx + MC_STOCK_B - 1) // MC_STOCK_B) * ((self.ny + MC_STOCK_B - 1) // MC_STOCK_B)
+        out = np.empty(nb, np.float32)
+        _check(self.L.mc_stock_block_max(self.p, out.ctypes.data))
+        return out
+
+    def max_in_disk(self, q):
+        """q: (n, 4) rows {cx, cy, r, above} -> (n, 3) rows {h, x, y}"""
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 4)
+        out = np.empty((len(q), 3))
+        _check(self.L.mc_stock_max_in_disk(self.p, q.ctypes.data, len(q), out.ctypes.data))
+        return out
+
+
+def tip_lut(shape, D, corner_r=0.0, edge_r=None, edge_dz=0.0):
+    """tipLut (src/assembly.js:38-54): f32 table of heights above the tip"""
+    L = load()
+    er = np.ascontiguousarray(edge_r if edge_r is not None else [], np.float64)
+    n = C.c_int32()
+    args = (int(shape), float(D), float(corner_r), er.ctypes.data if er.size else None, len(er),
+            float(edge_dz))
+    _check(L.mc_tip_lut(*args, None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.float32)
+    _check(L.mc_tip_lut(*args, out.ctypes.data, n.value, C.byref(n)))
+    return out
 
 
 def surrogate_features(cells):

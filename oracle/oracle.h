@@ -75,7 +75,23 @@ int32_t *or_stock_linewall(or_stock *s);
 double or_stock_stamp(or_stock *s, double cx, double cy, double tipZ, double R, int type,
                       double errVal, int line);
 void or_stamp_batch(or_stock *s, const mc_stamp *st, int n, double *vol);
+/* mark = {ux, uy, f} or NULL; lut = f32 profile of lutn entries or NULL */
+double or_stock_stamp_ex(or_stock *s, double cx, double cy, double tipZ, double R, int type,
+                         double errVal, int line, const double *mark, const float *lut,
+                         int lutn);
+void or_stamp_batch_ex(or_stock *s, const mc_stamp *st, const mc_stamp_opt *opt, int n,
+                       const float *luts, const int32_t *lut_off, const int32_t *lut_len,
+                       double *vol);
+float *or_stock_marks(or_stock *s);
+float *or_stock_markf(or_stock *s);
+void or_stock_dirty(const or_stock *s, int32_t *rect);
+void or_stock_clear_dirty(or_stock *s);
+float or_stock_block_max(or_stock *s, int32_t bi, int32_t bj);
+void or_stock_max_in_disk(or_stock *s, double cx, double cy, double r, double above, double *out);
 double or_stock_sample(const or_stock *s, double x, double y);
+/* tipLut (src/assembly.js:38-54): returns n; fills out when not NULL */
+int or_tip_lut(int shape, double D, double cornerR, const double *er, int en, double edz,
+               float *out);
 
 /* toolModel (src/tool.js:16-32) for a uniform carbide beam: k [N/mm], fn [Hz] */
 void or_tool_model(double D, double stickout, double kSpindle, double mHolder, double *k,

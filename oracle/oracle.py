@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
 
-from mcsim._abi import Cell, Trial, BisectOpts, Limit, Mode, Stamp  # noqa: E402
+from mcsim._abi import Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt  # noqa: E402
 
 _LIB = None
 
@@ -55,6 +55,17 @@ def lib():
             getattr(L, f).argtypes = [P]
             getattr(L, f).restype = P
         L.or_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
+        L.or_stamp_batch_ex.argtypes = [P, C.POINTER(Stamp), P, C.c_int, P, P, P, P]
+        for f in ("or_stock_marks", "or_stock_markf"):
+            getattr(L, f).argtypes = [P]
+            getattr(L, f).restype = P
+        L.or_stock_dirty.argtypes = [P, P]
+        L.or_stock_clear_dirty.argtypes = [P]
+        L.or_stock_block_max.argtypes = [P, C.c_int32, C.c_int32]
+        L.or_stock_block_max.restype = C.c_float
+        L.or_stock_max_in_disk.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double, P]
+        L.or_tip_lut.argtypes = [C.c_int, C.c_double, C.c_double, P, C.c_int, C.c_double, P]
+        L.or_tip_lut.restype = C.c_int
         L.or_stock_sample.argtypes = [P, C.c_double, C.c_double]
         L.or_stock_sample.restype = C.c_double
         L.or_tool_model.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, P, P]
@@ -177,6 +188,59 @@ class OracleStock:
 
     def sample(self, x, y):
         return self.L.or_stock_sample(self.p, float(x), float(y))
+
+    def markS(self):
+        return self._arr("or_stock_marks", C.c_float)
+
+    def markF(self):
+        return self._arr("or_stock_markf", C.c_float)
+
+    def stamps_ex(self, stamps_arr, opt_arr, n, luts=None, lut_off=None, lut_len=None):
+        """Stamps with feed marks / profiles: opt_arr is a StampOpt array (or None); luts
+        the concatenated f32 profiles, lut_off / lut_len int32 per profile."""
+        import numpy as np
+        vol = np.zeros(n)
+        keep = [np.ascontiguousarray(a, dt) if a is not None else None
+                for a, dt in ((luts, np.float32), (lut_off, np.int32), (lut_len, np.int32))]
+        ptr = [a.ctypes.data if a is not None else None for a in keep]
+        self.L.or_stamp_batch_ex(self.p, stamps_arr, opt_arr, n, ptr[0], ptr[1], ptr[2],
+                                 vol.ctypes.data)
+        return vol
+
+    def dirty(self):
+        r = (C.c_int32 * 4)()
+        self.L.or_stock_dirty(self.p, r)
+        return list(r)
+
+    def clear_dirty(self):
+        self.L.or_stock_clear_dirty(self.p)
+
+    def block_max(self, bi, bj):
+        return self.L.or_stock_block_max(self.p, int(bi), int(bj))
+
+    def block_maxima(self):
+        import numpy as np
+        bnx, bny = (self.nx + 7) // 8, (self.ny + 7) // 8
+        return np.array([[self.block_max(bi, bj) for bi in range(bnx)] for bj in range(bny)],
+                        dtype=np.float32).ravel()
+
+    def max_in_disk(self, cx, cy, r, above=float("-inf")):
+        out = (C.c_double * 3)()
+        self.L.or_stock_max_in_disk(self.p, float(cx), float(cy), float(r), float(above), out)
+        return tuple(out)
+
+
+def tip_lut(shape, D, corner_r=0.0, edge_r=None, edge_dz=0.0):
+    """tipLut (src/assembly.js:38-54) restated: f32 table."""
+    import numpy as np
+    er = np.ascontiguousarray(edge_r if edge_r is not None else [], np.float64)
+    L = lib()
+    n = L.or_tip_lut(int(shape), float(D), float(corner_r), er.ctypes.data, len(er),
+                     float(edge_dz), None)
+    out = np.zeros(n, np.float32)
+    L.or_tip_lut(int(shape), float(D), float(corner_r), er.ctypes.data, len(er), float(edge_dz),
+                 out.ctypes.data)
+    return out
 
 
 def tool_model(D, stickout, kSpindle, mHolder=0.05):

@@ -8,8 +8,10 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_chatter.json   chatterAt verdicts + chatter index / amplitudes
   golden_tdl.json       timeDomainLimit (alim, bracket)
   golden_trace.json     per-step Fx, Fy, x, y of the engine chatterAt builds
-  golden_stock.json     Stock.stamp sequences: final H / err / wallD / lineCut / lineWall,
-                        per-stamp volumes, bilinear samples
+  golden_stock.json     Stock.stamp sequences: final H / err / wallD / lineCut / lineWall /
+                        markS / markF, per-stamp volumes, dirty rectangles, bilinear
+                        samples, maxInDisk answers, block maxima
+  golden_tiplut.json    tipLut tables
   golden_lobes.json     stabilityLobes / alimAt / directionalCoeffs
   golden_toolmodel.json toolModel (k, fn)
 
@@ -111,7 +113,46 @@ def build_cases(quick):
         samples += [[0.0, 0.0], [w, l], [w, 0.0], [0.0, l]]
         stocks.append(dict(w=w, l=l, h=h, cell=cell, wallBand=2.0 if variant == 1 else 0,
                            stamps=stamps, samples=samples))
+    # sequences with feed marks, tip profiles (tipLut of the tools below), a clearDirty half
+    # way, interference queries (maxInDisk) and block maxima
+    tools = [dict(D=6.0, type="bull", cornerR=1.0), dict(D=4.0, type="ball"),
+             dict(D=8.0, type="flat", edge=dict(dz=0.25, r=[0.0, 0.9, 1.7, 2.4, 3.0, 3.5, 3.8, 4.0, 4.0])),
+             dict(D=5.0, type="bull", cornerR=4.0)]
+    for variant in range(2):
+        rng = W.ParkMiller(200 + variant)
+        w, l, h, cell = (10.0, 8.0, 5.0, 0.05) if variant == 0 else (14.0, 9.3, 4.0, 0.3)
+        stamps = []
+        for s in range(140):
+            kind = rng.integer(0, 3)
+            st = dict(cx=rng.uniform(-2, w + 2), cy=rng.uniform(-2, l + 2),
+                      tipZ=rng.uniform(-h - 1.0, 0.3), errVal=rng.uniform(0, 50), line=s + 1)
+            if kind == 3:
+                t = rng.integer(0, len(tools) - 1)
+                st.update(R=tools[t]["D"] / 2, type="ball" if rng.u() < 0.3 else "flat", lut=t)
+            else:
+                st.update(R=rng.uniform(0.3, 3.5), type="ball" if kind == 2 else "flat")
+            if kind in (0, 3) or rng.u() < 0.2:
+                a = rng.uniform(-JS_PI, JS_PI)
+                f = 0.0 if rng.u() < 0.1 else rng.uniform(0.02, 0.4)
+                st["mark"] = dict(ux=math.cos(a), uy=math.sin(a), f=f)
+            # repeat some stamps in place (no change the second time: dirty stays as it is)
+            stamps.append(st)
+            if rng.u() < 0.1:
+                stamps.append(dict(st, line=s + 1))
+        queries = []
+        for q in range(90):
+            r = rng.uniform(-0.2, 6.0) if q % 9 else rng.uniform(6.0, 12.0)
+            above = float("-inf") if rng.u() < 0.7 else rng.uniform(-h, 0.1)
+            queries.append([rng.uniform(-3, w + 3), rng.uniform(-3, l + 3), r,
+                            "-inf" if above == float("-inf") else above])
+        samples = [[rng.uniform(-1, w + 1), rng.uniform(-1, l + 1)] for _ in range(100)]
+        stocks.append(dict(w=w, l=l, h=h, cell=cell, wallBand=2.0 if variant == 1 else 0,
+                           stamps=stamps, samples=samples, tools=tools, queries=queries,
+                           clearAt=len(stamps) // 2))
     cases["stock"] = stocks
+    cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
+                               dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
+                               dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]
     # --- lobes
     lob = []
     lob.append(dict(params=dict(N=3, Ktc=700.0, Krc=210.0, phiSt=JS_PI / 2, phiEx=JS_PI, lobes=8,
@@ -157,7 +198,7 @@ def main():
         json.dump(cases, f)
     run(cin, cout)
     out = json.load(open(cout))
-    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel"):
+    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut"):
         rec = dict(cases=cases[key], results=out[key])
         if key + "_meta" in cases:
             rec["meta"] = cases[key + "_meta"]

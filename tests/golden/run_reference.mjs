@@ -16,6 +16,7 @@ const { chatterAt, timeDomainLimit } = await import(REF + 'stabtd.js');
 const { stabilityLobes, alimAt, directionalCoeffs } = await import(REF + 'sld.js');
 const { Stock } = await import(REF + 'stock.js');
 const { toolModel } = await import(REF + 'tool.js');
+const { tipLut } = await import(REF + 'assembly.js');
 CE = CuttingEngine; NB = BINS;
 
 const b64f32 = a => Buffer.from(new Float32Array(a).buffer).toString('base64');
@@ -72,18 +73,30 @@ if (cases.trace) {
   console.error('trace', (Date.now() - t0) / 1e3, 's');
 }
 
+const num = v => v === -Infinity ? '-inf' : v === Infinity ? 'inf' : v;
+const rect = d => [d.i0, d.j0, d.i1, d.j1].map(num);
 if (cases.stock) {
-  out.stock = cases.stock.map(({ w, l, h, cell, wallBand, stamps, samples }) => {
+  out.stock = cases.stock.map(({ w, l, h, cell, wallBand, stamps, samples, tools, queries, clearAt }) => {
     const s = new Stock(w, l, h, cell);
     if (wallBand) s.wallBand = wallBand;
-    const vol = [];
-    for (const st of stamps) {
+    const luts = (tools || []).map(tipLut);
+    const vol = [], dirty = [], disk = [];
+    const ask = () => { if (queries) for (const [x, y, r, a] of queries) {
+      const m = s.maxInDisk(x, y, r, a === '-inf' ? -Infinity : a); disk.push([num(m.h), m.x, m.y]); } };
+    stamps.forEach((st, q) => {
+      if (clearAt !== undefined && q === clearAt) { dirty.push(rect(s.dirty)); ask(); s.clearDirty(); }
       s.curLine = st.line;
-      vol.push(s.stamp(st.cx, st.cy, st.tipZ, st.R, st.type, st.errVal, null, null));
-    }
-    const smp = samples.map(([x, y]) => { const v = s.sample(x, y); return v === -Infinity ? '-inf' : v; });
+      vol.push(s.stamp(st.cx, st.cy, st.tipZ, st.R, st.type, st.errVal, st.mark || null,
+        st.lut === undefined ? null : luts[st.lut]));
+    });
+    dirty.push(rect(s.dirty));
+    ask();
+    const bmax = [];
+    for (let bj = 0; bj < s.bny; bj++) for (let bi = 0; bi < s.bnx; bi++) bmax.push(s.blockMax(bi, bj));
+    const smp = samples.map(([x, y]) => num(s.sample(x, y)));
     return { nx: s.nx, ny: s.ny, hx: s.hx, hy: s.hy, H: b64f32(s.H), err: b64f32(s.err), wallD: b64f32(s.wallD),
-      lineCut: b64i32(s.lineCut), lineWall: b64i32(s.lineWall), vol, removed: s.removed, sample: smp };
+      lineCut: b64i32(s.lineCut), lineWall: b64i32(s.lineWall), markS: b64f32(s.markS), markF: b64f32(s.markF),
+      vol, removed: s.removed, sample: smp, dirty, disk, bmax: b64f32(bmax), luts: luts.map(b64f32) };
   });
   console.error('stock', (Date.now() - t0) / 1e3, 's');
 }
@@ -97,6 +110,8 @@ if (cases.lobes) {
       dc: [dc.xx, dc.xy, dc.yx, dc.yy] };
   });
 }
+
+if (cases.tiplut) out.tiplut = cases.tiplut.map(t => b64f32(tipLut(t)));
 
 if (cases.toolModel) out.toolModel = cases.toolModel.map(t => { const m = toolModel(t); return { k: m.k, fn: m.fn }; });
 

@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "machinig-sim_amd"))
 
-from mcsim._abi import make_cell, Stamp, MC_TOOL_BALL, MC_TOOL_FLAT  # noqa: E402
+from mcsim._abi import make_cell, Stamp, StampOpt, MC_TOOL_BALL, MC_TOOL_FLAT  # noqa: E402
+from mcsim._abi import MC_TIP_FLAT, MC_TIP_BALL, MC_TIP_BULL, MC_TIP_EDGE  # noqa: E402
 
 
 def load(name):
@@ -52,3 +53,42 @@ def stamps_from_json(stamps):
         arr[i] = Stamp(s["cx"], s["cy"], s["tipZ"], s["R"],
                        MC_TOOL_BALL if s["type"] == "ball" else MC_TOOL_FLAT, s["line"], s["errVal"])
     return arr
+
+
+def opts_from_json(stamps):
+    """StampOpt array of a stamp list (mark / lut keys), or None when no stamp has one"""
+    if not any("mark" in s or "lut" in s for s in stamps):
+        return None
+    arr = (StampOpt * len(stamps))()
+    for i, s in enumerate(stamps):
+        m = s.get("mark") or dict(ux=0.0, uy=0.0, f=0.0)
+        arr[i] = StampOpt(m["ux"], m["uy"], m["f"], s.get("lut", -1), 0)
+    return arr
+
+
+def luts_from_json(b64_list):
+    """concatenated f32 profiles + int32 offsets and lengths"""
+    tabs = [f32(b) for b in b64_list]
+    off = np.cumsum([0] + [len(t) for t in tabs[:-1]]).astype(np.int32) if tabs else np.zeros(0, np.int32)
+    cat = np.concatenate(tabs).astype(np.float32) if tabs else np.zeros(0, np.float32)
+    return cat, off, np.array([len(t) for t in tabs], np.int32)
+
+
+def tip_shape(t):
+    """tipLut tool dict -> (shape, D, cornerR, edge_r, edge_dz) of mc_tip_lut"""
+    if t.get("edge"):
+        return MC_TIP_EDGE, t["D"], 0.0, t["edge"]["r"], t["edge"]["dz"]
+    kind = {"ball": MC_TIP_BALL, "bull": MC_TIP_BULL}.get(t.get("type"), MC_TIP_FLAT)
+    return kind, t["D"], t.get("cornerR", 0.0) or 0.0, None, 0.0
+
+
+def split_at(case):
+    """stamp list halves around the clearDirty point (the whole list when there is none)"""
+    st = case["stamps"]
+    c = case.get("clearAt")
+    return (st, []) if c is None else (st[:c], st[c:])
+
+
+def rect(v):
+    big = np.iinfo(np.int32).max
+    return [big if x == "inf" else int(x) for x in v]

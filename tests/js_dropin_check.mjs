@@ -32,6 +32,9 @@ async function main() {
       const d = m.directionalCoeffs(c.params.Krc / c.params.Ktc, c.params.phiSt, c.params.phiEx);
       check(sameBits([d.xx, d.xy, d.yx, d.yy], ref.dc), 'dc ' + k);
     });
+    const tl = load('tiplut');
+    tl.cases.forEach((t, k) => check(sameBits(m.tipLut(t), f32(tl.results[k])), 'tipLut ' + k));
+    check(m.LUT_DR === 0.02, 'LUT_DR');
   } else {
     console.log('device', m.native().deviceInfo());
     const ch = load('chatter');
@@ -63,10 +66,28 @@ async function main() {
     st.cases.forEach((c, k) => {
       const s = new m.Stock(c.w, c.l, c.h, c.cell), ref = st.results[k];
       if (c.wallBand) s.wallBand = c.wallBand;
-      const half = c.stamps.length >> 1;
-      const vol = [];
-      for (const p of c.stamps.slice(0, half)) { s.curLine = p.line; vol.push(s.stamp(p.cx, p.cy, p.tipZ, p.R, p.type, p.errVal)); }
-      vol.push(...s.stampBatch(c.stamps.slice(half)));
+      const luts = (c.tools || []).map(m.tipLut);
+      luts.forEach((L, i) => check(sameBits(L, f32(ref.luts[i])), 'stock tipLut ' + k + '/' + i));
+      const args = p => [p.cx, p.cy, p.tipZ, p.R, p.type, p.errVal, p.mark || null, p.lut === undefined ? null : luts[p.lut]];
+      const ent = p => ({ cx: p.cx, cy: p.cy, tipZ: p.tipZ, R: p.R, type: p.type, errVal: p.errVal, line: p.line,
+        mark: p.mark || null, lut: p.lut === undefined ? null : luts[p.lut] });
+      const half = c.clearAt === undefined ? c.stamps.length >> 1 : c.clearAt;
+      const vol = [], dirty = [], disk = [];
+      const ask = () => (c.queries || []).forEach(([x, y, r, a]) => { const q = s.maxInDisk(x, y, r, num(a)); disk.push([q.h, q.x, q.y]); });
+      // the first part one call at a time, the rest as one batch
+      for (const p of c.stamps.slice(0, half)) { s.curLine = p.line; vol.push(s.stamp(...args(p))); }
+      if (c.clearAt !== undefined) { const d = s.dirty; dirty.push([d.i0, d.j0, d.i1, d.j1]); ask(); s.clearDirty(); }
+      vol.push(...s.stampBatch(c.stamps.slice(half).map(ent)));
+      check(sameBits(s.markS, f32(ref.markS)) && sameBits(s.markF, f32(ref.markF)), 'marks ' + k);
+      if (c.queries) {
+        const d = s.dirty; dirty.push([d.i0, d.j0, d.i1, d.j1]); ask();
+        check(JSON.stringify(dirty) === JSON.stringify(ref.dirty.map(r => r.map(num))), 'dirty ' + k);
+        check(disk.length === ref.disk.length && disk.every((q, i) => Object.is(q[0], num(ref.disk[i][0])) &&
+          q[1] === ref.disk[i][1] && q[2] === ref.disk[i][2]), 'maxInDisk ' + k);
+        const bm = [];
+        for (let bj = 0; bj < s.bny; bj++) for (let bi = 0; bi < s.bnx; bi++) bm.push(s.blockMax(bi, bj));
+        check(sameBits(Float32Array.from(bm), f32(ref.bmax)), 'blockMax ' + k);
+      }
       check(sameBits(s.H, f32(ref.H)), 'H ' + k);
       check(sameBits(s.err, f32(ref.err)) && sameBits(s.wallD, f32(ref.wallD)), 'err/wallD ' + k);
       check(sameBits(s.lineCut, i32(ref.lineCut)) && sameBits(s.lineWall, i32(ref.lineWall)), 'lines ' + k);

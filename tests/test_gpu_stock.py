@@ -28,16 +28,33 @@ def test_stamps_golden(mc):
         assert (s.nx, s.ny, s.hx, s.hy) == (ref["nx"], ref["ny"], ref["hx"], ref["hy"])
         if case["wallBand"]:
             s.set_wall_band(case["wallBand"])
-        vol = s.stamps(G.stamps_from_json(case["stamps"]))
+        luts = G.luts_from_json(ref.get("luts", []))
+        vol, dirty, disk = [], [], []
+        qs = np.array([[x, y, r, G.num(a)] for x, y, r, a in case.get("queries", [])]).reshape(-1, 4)
+        for part, half in enumerate(G.split_at(case)):
+            if part == 1:
+                dirty.append(s.dirty())
+                disk += [tuple(r) for r in s.max_in_disk(qs)]
+                s.clear_dirty()
+            if half:
+                vol += list(s.stamps_ex(G.stamps_from_json(half), G.opts_from_json(half), None, *luts))
         e, w, lc, lw = s.side()
         assert same_bits(s.H(), G.f32(ref["H"]))
         assert same_bits(e, G.f32(ref["err"])) and same_bits(w, G.f32(ref["wallD"]))
         assert np.array_equal(lc, G.i32(ref["lineCut"])) and np.array_equal(lw, G.i32(ref["lineWall"]))
+        ms, mf = s.marks()
+        assert same_bits(ms, G.f32(ref["markS"])) and same_bits(mf, G.f32(ref["markF"]))
         np.testing.assert_allclose(vol, np.array(ref["vol"]), rtol=1e-12, atol=1e-12)
         xy = np.array(case["samples"])
         got = s.sample(xy)
         want = np.array([G.num(v) for v in ref["sample"]])
         assert np.array_equal(got, want)
+        if "queries" in case:
+            dirty.append(s.dirty())
+            assert dirty == [G.rect(d) for d in ref["dirty"]]
+            disk += [tuple(r) for r in s.max_in_disk(qs)]
+            assert disk == [(G.num(h), x, y) for h, x, y in ref["disk"]]
+            assert same_bits(s.block_maxima(), G.f32(ref["bmax"]))
         s.close()
 
 
@@ -62,3 +79,100 @@ def test_zigzag_prefix_matches_oracle(mc):
         sub = path.as_stamp_array(rec[a:a + 7919])
         s2.stamps(sub, want_vol=False)
     assert same_bits(s2.H(), o.H())
+
+
+def marked_profiled_run(mc, n_batches=6):
+    """A zigzag of flat stamps with feed marks, interleaved with profiled (bull and ball
+    tables) and plain ball stamps, applied in batches on the GPU and in the oracle"""
+    import oracle
+    from mcsim import workloads as W, path
+    from mcsim._abi import StampOpt, MC_TIP_BULL, MC_TIP_BALL
+    segs = W.zigzag_segments(120)
+    rec, _ = path.schedule(segs, 0.1)
+    arr = path.as_stamp_array(rec)
+    n = len(arr)
+    rng = np.random.default_rng(11)
+    tabs = [oracle.tip_lut(MC_TIP_BULL, 2 * arr[0].R, 1.5), oracle.tip_lut(MC_TIP_BALL, 2 * arr[0].R)]
+    luts = np.concatenate(tabs)
+    off = np.array([0, len(tabs[0])], np.int32)
+    ln = np.array([len(t) for t in tabs], np.int32)
+    opts = (StampOpt * n)()
+    kind = rng.integers(0, 10, n)
+    for i in range(n):
+        a = arr[i]
+        if kind[i] < 6:       # feed marks along the path direction
+            j = max(0, i - 1)
+            dx, dy = a.cx - arr[j].cx, a.cy - arr[j].cy
+            d = np.hypot(dx, dy) or 1.0
+            opts[i] = StampOpt(dx / d, dy / d, 0.12, -1, 0)
+        elif kind[i] < 8:     # tip profile
+            opts[i] = StampOpt(0.0, 0.0, 0.0, int(kind[i] - 6), 0)
+        else:
+            opts[i] = StampOpt(0.0, 0.0, 0.0, -1, 0)
+            if kind[i] == 9:
+                arr[i].type = 1
+    s = mc.Stock(51.1, 51.1, 20, 0.1)
+    o = oracle.OracleStock(51.1, 51.1, 20, 0.1)
+    cuts = np.linspace(0, n, n_batches + 1).astype(int)
+    rects = []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        sub = (type(arr[0]) * (b - a)).from_buffer_copy(arr, a * C_sizeof(arr[0]))
+        osub = (StampOpt * (b - a)).from_buffer_copy(opts, a * C_sizeof(opts[0]))
+        v = s.stamps_ex(sub, osub, None, luts, off, ln)
+        ov = o.stamps_ex(sub, osub, b - a, luts, off, ln)
+        np.testing.assert_allclose(v, ov, rtol=1e-11, atol=1e-9)
+        rects.append((s.dirty(), o.dirty()))
+        s.clear_dirty()
+        o.clear_dirty()
+    return s, o, rects
+
+
+def C_sizeof(x):
+    import ctypes
+    return ctypes.sizeof(x)
+
+
+def test_marks_and_profiles_match_oracle(mc):
+    s, o, rects = marked_profiled_run(mc)
+    assert same_bits(s.H(), o.H())
+    e, w, lc, lw = s.side()
+    assert same_bits(e, o.err()) and same_bits(w, o.wallD())
+    assert np.array_equal(lc, o.lineCut()) and np.array_equal(lw, o.lineWall())
+    ms, mf = s.marks()
+    assert same_bits(ms, o.markS()) and same_bits(mf, o.markF())
+    assert np.count_nonzero(mf) > 10000
+    for got, want in rects:
+        assert got == want
+    assert same_bits(s.block_maxima(), o.block_maxima())
+    # interference queries over the cut stock: value and reported position
+    rng = np.random.default_rng(3)
+    q = np.column_stack([rng.uniform(-5, 56, 400), rng.uniform(-5, 56, 400), rng.uniform(-0.5, 15, 400),
+                         np.where(rng.uniform(size=400) < 0.6, -np.inf, rng.uniform(-20, 0.5, 400))])
+    got = s.max_in_disk(q)
+    want = np.array([o.max_in_disk(*row) for row in q])
+    assert np.array_equal(got, want)
+    assert np.count_nonzero(np.isfinite(got[:, 0])) > 200
+    s.close()
+
+
+def test_dirty_rect_only_for_stamps_that_change(mc):
+    """Stock.dirty grows by a stamp's box only when the stamp changed a node: a repeated
+    stamp can still rewrite wall distances (its FP64 distance below their f32 copy), a
+    shallower one inside the pocket changes nothing"""
+    import oracle
+    s = mc.Stock(10.0, 10.0, 5.0, 0.1)
+    o = oracle.OracleStock(10.0, 10.0, 5.0, 0.1)
+    seq = [dict(cx=5.0, cy=5.0, tipZ=-2.0, R=1.0, type="flat", errVal=0.0, line=1),
+           dict(cx=5.0, cy=5.0, tipZ=-2.0, R=1.0, type="flat", errVal=0.0, line=2),
+           dict(cx=5.0, cy=5.0, tipZ=-1.0, R=0.5, type="flat", errVal=0.0, line=3)]
+    seen = []
+    for st in seq:
+        arr = G.stamps_from_json([st])
+        s.stamps(arr)
+        o.stamps(arr, 1)
+        assert s.dirty() == o.dirty()
+        seen.append(s.dirty())
+        s.clear_dirty()
+        o.clear_dirty()
+    assert seen[0][2] > seen[0][0] and seen[2] == [2 ** 31 - 1, 2 ** 31 - 1, -1, -1]
+    s.close()

@@ -37,6 +37,10 @@ def test_force_traces():
                 assert np.allclose(a, b, rtol=1e-9, atol=1e-12 * np.max(np.abs(b))), key
 
 
+def bits(a):
+    return np.asarray(a, np.float32).view(np.int32)
+
+
 def test_stock_maps_bit_exact():
     g = G.load("stock")
     for case, ref in zip(g["cases"], g["results"]):
@@ -44,8 +48,22 @@ def test_stock_maps_bit_exact():
         assert (s.nx, s.ny, s.hx, s.hy) == (ref["nx"], ref["ny"], ref["hx"], ref["hy"])
         if case["wallBand"]:
             s.set_wall_band(case["wallBand"])
-        vol = s.stamps(G.stamps_from_json(case["stamps"]), len(case["stamps"]))
-        assert np.array_equal(s.H().view(np.int32), G.f32(ref["H"]).view(np.int32))
+        luts = G.luts_from_json(ref.get("luts", []))
+        vol, dirty, disk = [], [], []
+
+        def ask():
+            for x, y, r, a in case.get("queries", []):
+                disk.append(s.max_in_disk(x, y, r, G.num(a)))
+
+        for part, half in enumerate(G.split_at(case)):
+            if part == 1:
+                dirty.append(s.dirty())
+                ask()
+                s.clear_dirty()
+            if half:
+                vol += list(s.stamps_ex(G.stamps_from_json(half), G.opts_from_json(half), len(half),
+                                        *luts))
+        assert np.array_equal(bits(s.H()), bits(G.f32(ref["H"])))
         assert np.array_equal(s.err(), G.f32(ref["err"]))
         assert np.array_equal(s.wallD(), G.f32(ref["wallD"]))
         assert np.array_equal(s.lineCut(), G.i32(ref["lineCut"]))
@@ -53,6 +71,68 @@ def test_stock_maps_bit_exact():
         assert np.array_equal(vol, np.array(ref["vol"]))
         for (x, y), v in zip(case["samples"], ref["sample"]):
             assert s.sample(x, y) == G.num(v)
+        assert np.array_equal(bits(s.markS()), bits(G.f32(ref["markS"])))
+        assert np.array_equal(bits(s.markF()), bits(G.f32(ref["markF"])))
+        if "queries" not in case:
+            continue
+        assert np.count_nonzero(s.markF()) > 100     # the marks were exercised
+        dirty.append(s.dirty())
+        assert dirty == [G.rect(d) for d in ref["dirty"]]
+        ask()
+        want = [(G.num(h), x, y) for h, x, y in ref["disk"]]
+        assert disk == want
+        assert sum(d[0] != float("-inf") for d in disk) > len(disk) // 2
+        assert np.array_equal(bits(s.block_maxima()), bits(G.f32(ref["bmax"])))
+
+
+def test_tip_lut_bit_exact():
+    g = G.load("tiplut")
+    for t, want in zip(g["cases"], g["results"]):
+        shape, D, rc, er, dz = G.tip_shape(t)
+        got = oracle.tip_lut(shape, D, rc, er, dz)
+        assert np.array_equal(bits(got), bits(G.f32(want))), t
+
+
+def test_stock_profile_reduces_to_ball():
+    """A ball's own tipLut table cuts to within the table's radial step of the ball
+    formula: the table path and the closed form bound each other."""
+    R = 2.0
+    lut = oracle.tip_lut(G.MC_TIP_BALL, 2 * R)
+    a = oracle.OracleStock(10.0, 10.0, 5.0, 0.05)
+    b = oracle.OracleStock(10.0, 10.0, 5.0, 0.05)
+    st = G.stamps_from_json([dict(cx=5.0, cy=5.0, tipZ=-2.5, R=R, type="ball", errVal=1.0, line=1)])
+    opt = G.opts_from_json([dict(lut=0)])
+    a.stamps_ex(st, opt, 1, lut, np.zeros(1, np.int32), np.array([len(lut)], np.int32))
+    b.stamps(st, 1)
+    Ha, Hb = a.H(), b.H()
+    assert np.count_nonzero(Ha < 0) == np.count_nonzero(Hb < 0)
+    # slope of a ball of radius 2 below r = 1.9 is < 3.1: one half-step of 0.01 in r
+    inner = np.zeros_like(Ha, bool).reshape(a.ny, a.nx)
+    yy, xx = np.mgrid[0:a.ny, 0:a.nx]
+    inner[(xx * a.hx - 5) ** 2 + (yy * a.hy - 5) ** 2 < 1.9 ** 2] = True
+    assert np.max(np.abs(Ha - Hb)[inner.ravel()]) < 0.031
+
+
+def test_max_in_disk_brute_force():
+    """maxInDisk's value is the highest node inside the disk whenever that node is above
+    `above` (the block walk only chooses which equal node reports its position)."""
+    s = oracle.OracleStock(9.0, 7.0, 5.0, 0.1)
+    rng = np.random.default_rng(5)
+    st = [dict(cx=float(rng.uniform(0, 9)), cy=float(rng.uniform(0, 7)), tipZ=float(rng.uniform(-4, -0.1)),
+               R=float(rng.uniform(0.3, 2)), type="flat", errVal=0.0, line=1) for _ in range(60)]
+    s.stamps(G.stamps_from_json(st), len(st))
+    H = s.H().reshape(s.ny, s.nx).astype(np.float64)
+    yy, xx = np.mgrid[0:s.ny, 0:s.nx]
+    for _ in range(300):
+        cx, cy, r = rng.uniform(-1, 10), rng.uniform(-1, 8), rng.uniform(0.05, 4)
+        inside = (xx * s.hx - cx) ** 2 + (yy * s.hy - cy) ** 2 <= r * r
+        h, x, y = s.max_in_disk(cx, cy, r)
+        if not inside.any():
+            assert h == float("-inf") and (x, y) == (0.0, 0.0)
+            continue
+        assert h == H[inside].max()
+        # the reported point is a node at that height, or the centre of a block inside the disk
+        assert (x - cx) ** 2 + (y - cy) ** 2 <= r * r + 1e-9
 
 
 def test_lobes_and_tool_model():
