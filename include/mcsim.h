@@ -248,6 +248,26 @@ int mc_stock_max_in_disk(const mc_stock* s, const double* q, int n, double* out)
 /* Stock.sample (src/stock.js:85-92): bilinear height at n points, -inf outside. */
 int mc_stock_sample(const mc_stock* s, const double* xy, int n, double* out);
 
+/* One engagement query of the simulator's sub-step (src/sim.js:386-388): the cutter of
+ * radius R at (cx, cy) fed along the unit vector (ux, uy), tip at tipZ, flutes fluteLen. */
+typedef struct {
+  double cx, cy, ux, uy, R, tipZ, fluteLen;
+} mc_engage_query;
+
+/* Its answers: hmax = fillEngagement's return (src/sim.js:301-319; -INFINITY when no
+ * leading bin meets material), ap = min(fluteLen, max(0, hmax - tipZ)) (:388), and
+ * immersion(tipZ) (:343-348): imm = 1 with the arc [phiSt, phiEx] [rad], or imm = 0 (null). */
+typedef struct {
+  double hmax, ap, phiSt, phiEx;
+  int32_t imm, pad;
+} mc_engage_result;
+
+/* Simulator.fillEngagement + immersion for n queries against the stock's current heights.
+ * Hb (may be NULL) receives MC_BINS f32 heights per query (engine.Hb: the stock height on
+ * the cutter periphery per angle bin, -inf behind the cutter or off the stock). */
+int mc_stock_engagement(const mc_stock* s, const mc_engage_query* q, int n, float* Hb,
+                        mc_engage_result* out);
+
 /* ---- Tip profile table, host side ---- */
 
 #define MC_LUT_DR 0.02   /* radial step of the tip-shape table [mm] (src/assembly.js:8) */

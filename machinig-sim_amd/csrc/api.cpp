@@ -31,14 +31,18 @@ size_t mcs_stampdev_bytes();
 size_t mcs_chunkbox_bytes();
 int mcs_chunk();
 size_t mcs_stampx_bytes();
+size_t mcs_cull_bytes();
 int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, const float* luts,
                       const int32_t* lut_off, const int32_t* lut_len, const float* lut_min,
-                      void* d_stampdev, void* d_stampx, void* d_box, int32_t* touched, int nx, int ny,
+                      void* d_stampdev, void* d_stampx, void* d_cull, void* d_box, int32_t* touched, int nx, int ny,
                       double hx, double hy, double zTop, double zB, double band, float* H,
                       float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, float* markS,
                       float* markF, int32_t* rect, double* vol, hipStream_t stream, hipEvent_t ev0,
                       hipEvent_t ev1);
 int mcs_launch_set_rect(int32_t* rect, hipStream_t stream);
+int mcs_launch_engagement(const float* H, int nx, int ny, double hx, double hy,
+                          const mc_engage_query* d_q, int n, float* d_Hb, mc_engage_result* d_out,
+                          hipStream_t stream);
 int mcs_launch_block_max(const float* H, int nx, int ny, int bnx, int bny, float* bmax,
                          hipStream_t stream);
 int mcs_launch_max_in_disk(const float* H, const float* bmax, int nx, int ny, int bnx, double hx,
@@ -608,7 +612,7 @@ struct StockImpl {
   int32_t* rect = nullptr;   // Stock.dirty {i0, j0, i1, j1} on the device
   float* bmax = nullptr;     // block maxima, valid while !bmax_stale
   bool bmax_stale = true;
-  DevBuf stamps, opt, sdev, sx, box, touched, vol, xy, smp, lut, lutmeta, q, qout;
+  DevBuf stamps, opt, sdev, sx, cull, box, touched, vol, xy, smp, lut, lutmeta, q, qout, eq, ehb, eout;
 };
 
 }  // namespace
@@ -909,8 +913,8 @@ int mc_stock_destroy(mc_stock* s) {
   for (void* p : {(void*)s->H, (void*)s->err, (void*)s->wallD, (void*)s->lineCut, (void*)s->lineWall,
                   (void*)s->markS, (void*)s->markF, (void*)s->rect, (void*)s->bmax})
     if (p) (void)hipFree(p);
-  for (DevBuf* b : {&s->stamps, &s->opt, &s->sdev, &s->sx, &s->box, &s->touched, &s->vol, &s->xy,
-                    &s->smp, &s->lut, &s->lutmeta, &s->q, &s->qout})
+  for (DevBuf* b : {&s->stamps, &s->opt, &s->sdev, &s->sx, &s->cull, &s->box, &s->touched, &s->vol, &s->xy,
+                    &s->smp, &s->lut, &s->lutmeta, &s->q, &s->qout, &s->eq, &s->ehb, &s->eout})
     if (b->p) (void)hipFree(b->p);
   delete s;
   return MC_OK;
@@ -993,6 +997,7 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
   const int chunks = (n + mcs_chunk() - 1) / mcs_chunk();
   if ((st = s->stamps.ensure((size_t)n * sizeof(mc_stamp)))) return st;
   if ((st = s->sdev.ensure((size_t)n * mcs_stampdev_bytes()))) return st;
+  if ((st = s->cull.ensure((size_t)n * mcs_cull_bytes()))) return st;
   if ((st = s->box.ensure((size_t)chunks * mcs_chunkbox_bytes()))) return st;
   if ((st = s->touched.ensure((size_t)n * 4))) return st;
   if (vol && (st = s->vol.ensure((size_t)n * sizeof(double)))) return st;
@@ -1022,7 +1027,7 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
   if (vol && mcs_launch_fill((float*)s->vol.p, (size_t)n * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
   if (mcs_launch_stamps((const mc_stamp*)s->stamps.p, opt ? (const mc_stamp_opt*)s->opt.p : nullptr, n,
                         opt ? (const float*)s->lut.p : nullptr, d_off, d_len, d_min, s->sdev.p,
-                        opt ? s->sx.p : nullptr, s->box.p, (int32_t*)s->touched.p, s->nx, s->ny, s->hx,
+                        opt ? s->sx.p : nullptr, s->cull.p, s->box.p, (int32_t*)s->touched.p, s->nx, s->ny, s->hx,
                         s->hy, s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut, s->lineWall,
                         s->markS, s->markF, s->rect, vol ? (double*)s->vol.p : nullptr, q, g_ctx.ev0,
                         g_ctx.ev1))
@@ -1104,6 +1109,33 @@ int mc_stock_max_in_disk(const mc_stock* s, const double* qv, int n, double* out
                              (double*)m->qout.p, q))
     return fail(MC_ERR_HIP, "maxInDisk kernel failed");
   HIPCHK(hipMemcpyAsync(out, m->qout.p, (size_t)n * 24, hipMemcpyDeviceToHost, q));
+  HIPCHK(hipStreamSynchronize(q));
+  return MC_OK;
+}
+
+int mc_stock_engagement(const mc_stock* s, const mc_engage_query* qv, int n, float* Hb,
+                        mc_engage_result* out) {
+  if (!s || n < 0 || (n > 0 && (!qv || !out))) return fail(MC_ERR_ARG, "bad engagement arguments");
+  for (int i = 0; i < n; i++) {
+    const mc_engage_query& q = qv[i];
+    if (!std::isfinite(q.cx) || !std::isfinite(q.cy) || !std::isfinite(q.ux) || !std::isfinite(q.uy) ||
+        !std::isfinite(q.R) || std::isnan(q.tipZ) || std::isnan(q.fluteLen))
+      return fail(MC_ERR_ARG, "engagement query %d: bad position, direction or radius", i);
+  }
+  if (n == 0) return MC_OK;
+  mc_stock* m = const_cast<mc_stock*>(s);
+  int st;
+  if ((st = m->eq.ensure((size_t)n * sizeof(mc_engage_query)))) return st;
+  if ((st = m->eout.ensure((size_t)n * sizeof(mc_engage_result)))) return st;
+  if (Hb && (st = m->ehb.ensure((size_t)n * MC_BINS * 4))) return st;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  hipStream_t q = g_ctx.stream;
+  HIPCHK(hipMemcpyAsync(m->eq.p, qv, (size_t)n * sizeof(mc_engage_query), hipMemcpyHostToDevice, q));
+  if (mcs_launch_engagement(s->H, s->nx, s->ny, s->hx, s->hy, (const mc_engage_query*)m->eq.p, n,
+                            Hb ? (float*)m->ehb.p : nullptr, (mc_engage_result*)m->eout.p, q))
+    return fail(MC_ERR_HIP, "engagement kernel failed");
+  HIPCHK(hipMemcpyAsync(out, m->eout.p, (size_t)n * sizeof(mc_engage_result), hipMemcpyDeviceToHost, q));
+  if (Hb) HIPCHK(hipMemcpyAsync(Hb, m->ehb.p, (size_t)n * MC_BINS * 4, hipMemcpyDeviceToHost, q));
   HIPCHK(hipStreamSynchronize(q));
   return MC_OK;
 }

@@ -476,6 +476,30 @@ napi_value StockMaxInDisk(napi_env env, napi_callback_info info) {
   return ta;
 }
 
+// stockEngagement(handle, q: Float64Array(7n) {cx, cy, ux, uy, R, tipZ, fluteLen},
+//                 Hb: Float32Array(180n) | null) -> ArrayBuffer of n mc_engage_result
+napi_value StockEngagement(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  mc_stock* s = stock_of(env, argv[0]);
+  if (!s) return nullptr;
+  void* p;
+  size_t len;
+  if (!buffer_of(env, argv[1], &p, &len)) return nullptr;
+  const int n = (int)(len / sizeof(mc_engage_query));
+  void* hb = nullptr;
+  size_t hbl = 0;
+  if (!is_nullish(env, argv[2])) {
+    if (!buffer_of(env, argv[2], &hb, &hbl)) return nullptr;
+    if (hbl < (size_t)n * MC_BINS * 4) { napi_throw_range_error(env, nullptr, "Hb needs 180 floats per query"); return nullptr; }
+  }
+  void* out;
+  napi_value ab = new_ab(env, (size_t)n * sizeof(mc_engage_result) + 1, &out);
+  int st = mc_stock_engagement(s, (const mc_engage_query*)p, n, (float*)hb, (mc_engage_result*)out);
+  if (st) return throw_mc(env, st);
+  return ab;
+}
+
 // tipLut(shape, D, cornerR, edgeR: Float64Array | null, edgeDz) -> Float32Array
 napi_value TipLut(napi_env env, napi_callback_info info) {
   napi_value argv[5];
@@ -523,7 +547,8 @@ napi_value Sizes(napi_env env, napi_callback_info) {
   NAPI_OK(napi_create_object(env, &o));
   const struct { const char* k; size_t s; } t[] = {
       {"mc_cell", sizeof(mc_cell)}, {"mc_trial", sizeof(mc_trial)}, {"mc_limit", sizeof(mc_limit)},
-      {"mc_stamp", sizeof(mc_stamp)}, {"mc_stamp_opt", sizeof(mc_stamp_opt)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
+      {"mc_stamp", sizeof(mc_stamp)}, {"mc_stamp_opt", sizeof(mc_stamp_opt)},
+      {"mc_engage_query", sizeof(mc_engage_query)}, {"mc_engage_result", sizeof(mc_engage_result)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
   for (auto& e : t) {
     napi_create_uint32(env, (uint32_t)e.s, &v);
     napi_set_named_property(env, o, e.k, v);
@@ -541,6 +566,7 @@ napi_value ModuleInit(napi_env env, napi_value exports) {
       {"stockWriteH", StockWriteH}, {"stampBatch", StampBatch}, {"stockSample", StockSample},
       {"stockReadMarks", StockReadMarks}, {"stockDirty", StockDirty}, {"stockClearDirty", StockClearDirty},
       {"stockBlockMax", StockBlockMax}, {"stockMaxInDisk", StockMaxInDisk}, {"tipLut", TipLut},
+      {"stockEngagement", StockEngagement},
       {"sizes", Sizes}};
   for (auto& f : fns) {
     napi_value fn;

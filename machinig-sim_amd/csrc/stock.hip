@@ -57,6 +57,7 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
                            const int32_t* __restrict__ lut_len, const float* __restrict__ lut_min,
                            int n, double hx, double hy, int nx, int ny, double zTop, double band,
                            StampDev* __restrict__ out, StampX* __restrict__ outx,
+                           int4* __restrict__ cull, float* __restrict__ cull_z,
                            int32_t* __restrict__ touched, ChunkBox* __restrict__ box) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ int bi0[256], bi1[256], bj0[256], bj1[256];
@@ -94,6 +95,10 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
     d.j0 = live ? (int)(c > 0 ? c : 0) : 1;
     d.j1 = live ? (int)(e < ny - 1 ? e : ny - 1) : 0;
     out[s] = d;
+    // compact copy for the tiles' culling pass: the box, and zmin rounded down to f32 (for
+    // an f32 height h, h > zmin exactly when h > RD_f32(zmin))
+    cull[s] = make_int4(d.i0, d.i1, d.j0, d.j1);
+    cull_z[s] = __double2float_rd(d.zmin);
     if (outx) outx[s] = x;
     touched[s] = 0;
     if (d.i0 <= d.i1 && d.j0 <= d.j1) { i0 = d.i0; i1 = d.i1; j0 = d.j0; j1 = d.j1; }
@@ -114,8 +119,8 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
 // EX: tip profiles (luts) and feed marks (markS / markF) are live in this batch
 template <bool EX>
 __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
-    const StampDev* __restrict__ st, const StampX* __restrict__ stx, const ChunkBox* __restrict__ box,
-    int n, int nx, int ny, double hx, double hy, double zB, float* __restrict__ H,
+    const StampDev* __restrict__ st, const StampX* __restrict__ stx, const int4* __restrict__ cull,
+    const float* __restrict__ cull_z, const ChunkBox* __restrict__ box, int n, int nx, int ny, double hx, double hy, double zB, float* __restrict__ H,
     float* __restrict__ err, float* __restrict__ wallD, int32_t* __restrict__ lineCut,
     int32_t* __restrict__ lineWall, float* __restrict__ markS, float* __restrict__ markF,
     const float* __restrict__ luts, int32_t* __restrict__ touched, double* __restrict__ vol) {
@@ -150,17 +155,16 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
     // below its highest node
     const int s = c * CHUNK + t;
     bool cand = false;
-    double zmin = 0;
+    float zmin = 0;
     if (s < n) {
-      const StampDev& d = st[s];
-      const int i0 = d.i0, i1 = d.i1, j0 = d.j0, j1 = d.j1;
-      cand = !(i1 < ti0 || i0 > ti1 || j1 < tj0 || j0 > tj1) && i0 <= i1 && j0 <= j1;
-      zmin = d.zmin;
+      const int4 b4 = cull[s];
+      cand = !(b4.y < ti0 || b4.x > ti1 || b4.w < tj0 || b4.z > tj1) && b4.x <= b4.y && b4.z <= b4.w;
+      zmin = cull_z[s];
     }
     __syncthreads();
     float tmax = wmaxs[0];
     for (int v = 1; v < TILE * TILE / 64; v++) tmax = fmaxf(tmax, wmaxs[v]);
-    const bool hit = cand && (double)tmax > zmin;
+    const bool hit = cand && tmax > zmin;
     const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
     if (lane == 0) wcount[wv] = __builtin_popcountll(m);
     __syncthreads();
@@ -257,19 +261,78 @@ __global__ void set_rect(int32_t* rect) {
 namespace mcs {
 
 // Stock.sample (src/stock.js:85-92): bilinear height, -inf outside the stock
-__global__ void stock_sample(const float* __restrict__ H, int nx, int ny, double hx, double hy,
-                             const double* __restrict__ xy, int n, double* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const double fx = xy[2 * q] / hx, fy = xy[2 * q + 1] / hy;
-  if (fx < 0 || fy < 0 || fx > nx - 1 || fy > ny - 1) { out[q] = -__builtin_inf(); return; }
+__device__ __forceinline__ double sample_h(const float* __restrict__ H, int nx, int ny, double hx,
+                                          double hy, double x, double y) {
+  const double fx = x / hx, fy = y / hy;
+  if (fx < 0 || fy < 0 || fx > nx - 1 || fy > ny - 1) return -__builtin_inf();
   int i = (int)fx, j = (int)fy;
   if (i > nx - 2) i = nx - 2;
   if (j > ny - 2) j = ny - 2;
   const double tx = fx - i, ty = fy - j;
   const double a = H[j * nx + i], b = H[j * nx + i + 1], c = H[(j + 1) * nx + i],
                d = H[(j + 1) * nx + i + 1];
-  out[q] = (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty;
+  return (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty;
+}
+
+__global__ void stock_sample(const float* __restrict__ H, int nx, int ny, double hx, double hy,
+                             const double* __restrict__ xy, int n, double* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  out[q] = sample_h(H, nx, ny, hx, hy, xy[2 * q], xy[2 * q + 1]);
+}
+
+// Simulator.fillEngagement (src/sim.js:301-319), immersion (:343-348) and the axial depth
+// (:387-388) for one query per wavefront: lane b (and b + 64, b + 128) samples the stock on
+// the cutter periphery for angle bin b.  Hb is stored as f32 like engine.Hb; the maximum
+// and the first / last leading bin above tipZ + 0.1 are wave reductions, exact since max
+// and min of f32 values and of bin indices do not depend on order.
+__global__ __launch_bounds__(256) void engagement(
+    const float* __restrict__ H, int nx, int ny, double hx, double hy,
+    const mc_engage_query* __restrict__ qs, int nq, float* __restrict__ Hb_out,
+    mc_engage_result* __restrict__ out) {
+  constexpr double PI = 3.141592653589793;
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (w >= nq) return;   // wave-uniform
+  const mc_engage_query q = qs[w];
+  const double cell = hx > hy ? hx : hy;
+  const double r = q.R - 0.5 * cell, ax = q.cx + q.ux * cell, ay = q.cy + q.uy * cell;
+  const double vx = -q.uy, vy = q.ux;
+  const double z = q.tipZ + 0.1;
+  float hmax = -__builtin_inff();
+  int first = MC_BINS, last = -1;
+  for (int b = lane; b < MC_BINS; b += 64) {
+    const double phi = (b + 0.5) / MC_BINS * 2 * PI;
+    double sn, cs;
+    mcm::sincos_js(phi, sn, cs);
+    const double h = sample_h(H, nx, ny, hx, hy, ax + r * (sn * q.ux + cs * vx), ay + r * (sn * q.uy + cs * vy));
+    const float f = sn > 0.02 ? (float)h : -__builtin_inff();
+    if (Hb_out) Hb_out[(size_t)w * MC_BINS + b] = f;
+    hmax = fmaxf(hmax, f);
+    if (b < MC_BINS / 2 && (double)f > z) { first = min(first, b); last = max(last, b); }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    hmax = fmaxf(hmax, __shfl_xor(hmax, o, 64));
+    first = min(first, __shfl_xor(first, o, 64));
+    last = max(last, __shfl_xor(last, o, 64));
+  }
+  if (lane == 0) {
+    mc_engage_result res;
+    res.hmax = hmax;
+    const double d = (double)hmax - q.tipZ;
+    const double a = d > 0 ? d : 0;     // max(0, -inf) = 0
+    res.ap = q.fluteLen < a ? q.fluteLen : a;
+    res.imm = last >= 0;
+    res.pad = 0;
+    res.phiSt = 0;
+    res.phiEx = 0;
+    if (last >= 0) {
+      const double st = (first + 0.5) / MC_BINS * 2 * PI - PI / MC_BINS;
+      const double ex = (last + 0.5) / MC_BINS * 2 * PI + PI / MC_BINS;
+      res.phiSt = st > 0 ? st : 0;
+      res.phiEx = ex < PI ? ex : PI;
+    }
+    out[w] = res;
+  }
 }
 
 constexpr int BLK = MC_STOCK_B;   // block edge of the interference maxima (src/stock.js:35)
@@ -364,11 +427,12 @@ __global__ void fill_f32(float* p, size_t n, float v) {
 size_t mcs_stampdev_bytes() { return sizeof(mcs::StampDev); }
 size_t mcs_chunkbox_bytes() { return sizeof(mcs::ChunkBox); }
 size_t mcs_stampx_bytes() { return sizeof(mcs::StampX); }
+size_t mcs_cull_bytes() { return sizeof(int4) + sizeof(float); }
 int mcs_chunk() { return mcs::CHUNK; }
 
 int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, const float* luts,
                       const int32_t* lut_off, const int32_t* lut_len, const float* lut_min,
-                      void* d_stampdev, void* d_stampx, void* d_box, int32_t* touched, int nx, int ny,
+                      void* d_stampdev, void* d_stampx, void* d_cull, void* d_box, int32_t* touched, int nx, int ny,
                       double hx, double hy, double zTop, double zB, double band, float* H,
                       float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, float* markS,
                       float* markF, int32_t* rect, double* vol, hipStream_t stream, hipEvent_t ev0,
@@ -378,19 +442,21 @@ int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, co
   const bool ex = d_opt != nullptr;
   hipLaunchKernelGGL(mcs::stamp_prep, dim3(chunks), dim3(mcs::CHUNK), 0, stream, d_in, d_opt, luts,
                      lut_off, lut_len, lut_min, n, hx, hy, nx, ny, zTop, band,
-                     (mcs::StampDev*)d_stampdev, ex ? (mcs::StampX*)d_stampx : nullptr, touched,
-                     (mcs::ChunkBox*)d_box);
+                     (mcs::StampDev*)d_stampdev, ex ? (mcs::StampX*)d_stampx : nullptr, (int4*)d_cull,
+                     (float*)((int4*)d_cull + n), touched, (mcs::ChunkBox*)d_box);
   const dim3 grid((nx + mcs::TILE - 1) / mcs::TILE, (ny + mcs::TILE - 1) / mcs::TILE);
   if (ev0 && hipEventRecord(ev0, stream) != hipSuccess) return -2;
   if (ex)
     hipLaunchKernelGGL(mcs::stamp_tiles<true>, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
-                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)d_stampx,
-                       (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy, zB, H, err, wallD, lineCut,
+                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)d_stampx, (const int4*)d_cull,
+                       (const float*)((int4*)d_cull + n), (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy,
+                       zB, H, err, wallD, lineCut,
                        lineWall, markS, markF, luts, touched, vol);
   else
     hipLaunchKernelGGL(mcs::stamp_tiles<false>, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
-                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)nullptr,
-                       (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy, zB, H, err, wallD, lineCut,
+                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)nullptr, (const int4*)d_cull,
+                       (const float*)((int4*)d_cull + n), (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy,
+                       zB, H, err, wallD, lineCut,
                        lineWall, markS, markF, luts, touched, vol);
   if (ev1 && hipEventRecord(ev1, stream) != hipSuccess) return -2;
   hipLaunchKernelGGL(mcs::dirty_rect, dim3((n + 255) / 256), dim3(256), 0, stream,
@@ -424,6 +490,15 @@ int mcs_launch_sample(const float* H, int nx, int ny, double hx, double hy, cons
   if (n <= 0) return 0;
   hipLaunchKernelGGL(mcs::stock_sample, dim3((n + 255) / 256), dim3(256), 0, stream, H, nx, ny,
                      hx, hy, d_xy, n, d_out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mcs_launch_engagement(const float* H, int nx, int ny, double hx, double hy,
+                          const mc_engage_query* d_q, int n, float* d_Hb, mc_engage_result* d_out,
+                          hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(mcs::engagement, dim3((n + 3) / 4), dim3(256), 0, stream, H, nx, ny, hx, hy, d_q,
+                     n, d_Hb, d_out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

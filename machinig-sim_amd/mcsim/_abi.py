@@ -64,6 +64,15 @@ class StampOpt(C.Structure):
                 ("lut", C.c_int32), ("pad", C.c_int32)]
 
 
+class EngageQuery(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("cx", "cy", "ux", "uy", "R", "tipZ", "fluteLen")]
+
+
+class EngageResult(C.Structure):
+    _fields_ = [("hmax", C.c_double), ("ap", C.c_double), ("phiSt", C.c_double),
+                ("phiEx", C.c_double), ("imm", C.c_int32), ("pad", C.c_int32)]
+
+
 MC_STOCK_B = 8
 MC_LUT_DR = 0.02
 MC_TIP_FLAT, MC_TIP_BALL, MC_TIP_BULL, MC_TIP_EDGE = 0, 1, 2, 3

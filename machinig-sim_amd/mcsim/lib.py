@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
-                   SearchTuning, cell_array, MC_OK, MC_SURR_IN, MC_STOCK_B)
+                   SearchTuning, EngageQuery, EngageResult, cell_array, MC_OK, MC_SURR_IN, MC_STOCK_B)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -24,6 +24,7 @@ EXPORTS = [
     "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
     "mc_set_search_tuning", "mc_get_search_tuning", "mc_stamp_batch_ex", "mc_stock_read_marks",
     "mc_stock_dirty", "mc_stock_clear_dirty", "mc_stock_block_max", "mc_stock_max_in_disk", "mc_tip_lut",
+    "mc_stock_engagement",
 ]
 
 _LIB = None
@@ -68,6 +69,7 @@ def load():
     L.mc_stock_clear_dirty.argtypes = [P]
     L.mc_stock_block_max.argtypes = [P, P]
     L.mc_stock_max_in_disk.argtypes = [P, P, C.c_int, P]
+    L.mc_stock_engagement.argtypes = [P, P, C.c_int, P, P]
     L.mc_tip_lut.argtypes = [C.c_int32, C.c_double, C.c_double, P, C.c_int32, C.c_double, P, C.c_int32,
                              C.POINTER(C.c_int32)]
     L.mc_surrogate_features.argtypes = [C.POINTER(Cell), C.c_int, P]
@@ -260,6 +262,19 @@ class Stock:
         out = np.empty((len(q), 3))
         _check(self.L.mc_stock_max_in_disk(self.p, q.ctypes.data, len(q), out.ctypes.data))
         return out
+
+    def engagement(self, queries, want_hb=True):
+        """Simulator.fillEngagement + immersion for a list of dicts (cx, cy, ux, uy, R, tipZ,
+        fluteLen=inf): (results EngageResult array, Hb f32 (n, 180) or None)"""
+        n = len(queries)
+        qa = (EngageQuery * max(1, n))()
+        for i, q in enumerate(queries):
+            qa[i] = EngageQuery(q["cx"], q["cy"], q["ux"], q["uy"], q["R"], q["tipZ"],
+                                q.get("fluteLen", float("inf")))
+        res = (EngageResult * max(1, n))()
+        Hb = np.empty((n, 180), np.float32) if want_hb else None
+        _check(self.L.mc_stock_engagement(self.p, qa, n, Hb.ctypes.data if want_hb else None, res))
+        return res, Hb
 
 
 def tip_lut(shape, D, corner_r=0.0, edge_r=None, edge_dz=0.0):

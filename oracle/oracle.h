@@ -89,6 +89,10 @@ void or_stock_clear_dirty(or_stock *s);
 float or_stock_block_max(or_stock *s, int32_t bi, int32_t bj);
 void or_stock_max_in_disk(or_stock *s, double cx, double cy, double r, double above, double *out);
 double or_stock_sample(const or_stock *s, double x, double y);
+/* engage.c: Simulator.fillEngagement / immersion (src/sim.js:301-348) */
+double or_fill_engagement(const or_stock *s, double cx, double cy, double ux, double uy, double R,
+                          float *Hb);
+int or_immersion(const float *Hb, double tipZ, double *st, double *ex);
 /* tipLut (src/assembly.js:38-54): returns n; fills out when not NULL */
 int or_tip_lut(int shape, double D, double cornerR, const double *er, int en, double edz,
                float *out);

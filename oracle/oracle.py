@@ -66,6 +66,10 @@ def lib():
         L.or_stock_max_in_disk.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double, P]
         L.or_tip_lut.argtypes = [C.c_int, C.c_double, C.c_double, P, C.c_int, C.c_double, P]
         L.or_tip_lut.restype = C.c_int
+        L.or_fill_engagement.argtypes = [P] + [C.c_double] * 5 + [P]
+        L.or_fill_engagement.restype = C.c_double
+        L.or_immersion.argtypes = [P, C.c_double, P, P]
+        L.or_immersion.restype = C.c_int
         L.or_stock_sample.argtypes = [P, C.c_double, C.c_double]
         L.or_stock_sample.restype = C.c_double
         L.or_tool_model.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, P, P]
@@ -228,6 +232,24 @@ class OracleStock:
         out = (C.c_double * 3)()
         self.L.or_stock_max_in_disk(self.p, float(cx), float(cy), float(r), float(above), out)
         return tuple(out)
+
+
+def fill_engagement(stock, cx, cy, ux, uy, R):
+    """Simulator.fillEngagement (src/sim.js:301-319) restated: (hmax, Hb f32[180])"""
+    import numpy as np
+    Hb = np.zeros(180, np.float32)
+    h = lib().or_fill_engagement(stock.p, float(cx), float(cy), float(ux), float(uy), float(R),
+                                 Hb.ctypes.data)
+    return h, Hb
+
+
+def immersion(Hb, tipZ):
+    """Simulator.immersion (src/sim.js:343-348) restated: (st, ex) or None"""
+    import numpy as np
+    Hb = np.ascontiguousarray(Hb, np.float32)
+    st, ex = C.c_double(), C.c_double()
+    ok = lib().or_immersion(Hb.ctypes.data, float(tipZ), C.byref(st), C.byref(ex))
+    return (st.value, ex.value) if ok else None
 
 
 def tip_lut(shape, D, corner_r=0.0, edge_r=None, edge_dz=0.0):

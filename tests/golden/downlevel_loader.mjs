@@ -2,7 +2,8 @@
 // container.  The reference's hot-path modules use `a ?? b` and `a?.b`, which V8 7.8
 // (Node 12) cannot parse.  This hook rewrites exactly those two operators, in memory,
 // into calls of two helpers with the same semantics (`__nc(a, b)` = a ?? b for
-// side-effect-free b, `__oc(a, 'k')` = a?.k) and leaves every other token untouched.
+// side-effect-free b, `__oc(a, 'k')` = a?.k, `__oc(a, e)` = a?.[e]) and leaves every other
+// token untouched.
 // Nothing is written to disk.
 const REF = process.env.MC_REFERENCE_SRC || '/root/reference/src/';
 
@@ -24,6 +25,13 @@ function rewriteOptional(s) {
   while ((k = s.indexOf('?.')) >= 0) {
     if (/\d/.test(s[k + 2] || '')) break;                 // a ternary `?.5`: not optional chaining
     const start = scanLeftMember(s, k - 1);
+    if (s[k + 2] === '[') {                               // a?.[expr]
+      let depth = 0, e = k + 2;
+      for (; e < s.length; e++) { if (s[e] === '[') depth++; else if (s[e] === ']' && --depth === 0) break; }
+      const left = s.slice(start, k), key = s.slice(k + 3, e);
+      s = s.slice(0, start) + `__oc(${left}, ${key})` + s.slice(e + 1);
+      continue;
+    }
     const m = s.slice(k + 2).match(/^[A-Za-z_$][\w$]*/);
     if (!m) throw new Error('unsupported ?. form near: ' + s.slice(start, k + 12));
     const left = s.slice(start, k);
@@ -49,6 +57,7 @@ function scanRightOperand(s, i) {
   let depth = 0, j = i;
   for (; j < s.length; j++) {
     const c = s[j];
+    if (c === "'" || c === '"') { j = s.indexOf(c, j + 1); if (j < 0) j = s.length; continue; }   // a string literal
     if (c === '(' || c === '[' || c === '{') { depth++; continue; }
     if (c === ')' || c === ']' || c === '}') { if (depth === 0) break; depth--; continue; }
     if (depth > 0) continue;

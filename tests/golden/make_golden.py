@@ -12,6 +12,7 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
                         markS / markF, per-stamp volumes, dirty rectangles, bilinear
                         samples, maxInDisk answers, block maxima
   golden_tiplut.json    tipLut tables
+  golden_engage.json    Simulator.fillEngagement / immersion on cut stocks
   golden_lobes.json     stabilityLobes / alimAt / directionalCoeffs
   golden_toolmodel.json toolModel (k, fn)
 
@@ -150,6 +151,23 @@ def build_cases(quick):
                            stamps=stamps, samples=samples, tools=tools, queries=queries,
                            clearAt=len(stamps) // 2))
     cases["stock"] = stocks
+    # engagement sampling of the simulator (fillEngagement / immersion) on the cut stocks of
+    # the two mark / profile sequences above
+    engage = []
+    for k, base in enumerate(stocks[3:5]):
+        rng = W.ParkMiller(300 + k)
+        w, l = base["w"], base["l"]
+        qs = []
+        for q in range(120):
+            a = rng.uniform(-JS_PI, JS_PI)
+            R = rng.uniform(0.3, 4.0)
+            qs.append(dict(cx=rng.uniform(-2, w + 2), cy=rng.uniform(-2, l + 2), ux=math.cos(a),
+                           uy=math.sin(a), R=R, tipZ=rng.uniform(-base["h"] - 0.5, 0.2)))
+        # axis-aligned feed directions, where -0 and exact zeros enter the frame
+        for ux, uy in ((1.0, 0.0), (-1.0, 0.0), (0.0, 1.0), (0.0, -1.0)):
+            qs.append(dict(cx=w / 2, cy=l / 2, ux=ux, uy=uy, R=2.0, tipZ=-1.0))
+        engage.append(dict(stock=3 + k, queries=qs))
+    cases["engage"] = engage
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]
@@ -198,7 +216,7 @@ def main():
         json.dump(cases, f)
     run(cin, cout)
     out = json.load(open(cout))
-    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut"):
+    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage"):
         rec = dict(cases=cases[key], results=out[key])
         if key + "_meta" in cases:
             rec["meta"] = cases[key + "_meta"]

@@ -111,6 +111,31 @@ if (cases.lobes) {
   });
 }
 
+// the simulator's engagement sampling, called as Simulator methods on a stand-in holding
+// just the fields they read (engine.Hb, activeTool.R, stock)
+if (cases.engage) {
+  const { Simulator } = await import(REF + 'sim.js');
+  out.engage = cases.engage.map(({ stock: si, queries }) => {
+    const c = cases.stock[si];
+    const s = new Stock(c.w, c.l, c.h, c.cell);
+    if (c.wallBand) s.wallBand = c.wallBand;
+    const luts = (c.tools || []).map(tipLut);
+    for (const st of c.stamps) {
+      s.curLine = st.line;
+      s.stamp(st.cx, st.cy, st.tipZ, st.R, st.type, st.errVal, st.mark || null, st.lut === undefined ? null : luts[st.lut]);
+    }
+    const Hb = [], hmax = [], imm = [];
+    for (const q of queries) {
+      const me = { engine: { Hb: new Float32Array(NB) }, activeTool: { R: q.R }, stock: s };
+      hmax.push(num(Simulator.prototype.fillEngagement.call(me, q.cx, q.cy, [q.ux, q.uy])));
+      Hb.push(...me.engine.Hb);
+      const m = Simulator.prototype.immersion.call(me, q.tipZ);
+      imm.push(m === null ? null : [m.st, m.ex]);
+    }
+    return { Hb: b64f32(Hb), hmax, imm };
+  });
+}
+
 if (cases.tiplut) out.tiplut = cases.tiplut.map(t => b64f32(tipLut(t)));
 
 if (cases.toolModel) out.toolModel = cases.toolModel.map(t => { const m = toolModel(t); return { k: m.k, fn: m.fn }; });

@@ -92,3 +92,13 @@ def split_at(case):
 def rect(v):
     big = np.iinfo(np.int32).max
     return [big if x == "inf" else int(x) for x in v]
+
+
+def stock_from_case(stock_mod, case, ref, use_ex=True):
+    """Build a stock (oracle or product class with stamps_ex) and apply a golden stamp case"""
+    s = stock_mod(case["w"], case["l"], case["h"], case["cell"])
+    if case["wallBand"]:
+        s.set_wall_band(case["wallBand"])
+    luts = luts_from_json(ref.get("luts", []))
+    st = case["stamps"]
+    return s, stamps_from_json(st), opts_from_json(st), luts

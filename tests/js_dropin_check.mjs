@@ -35,6 +35,16 @@ async function main() {
     const tl = load('tiplut');
     tl.cases.forEach((t, k) => check(sameBits(m.tipLut(t), f32(tl.results[k])), 'tipLut ' + k));
     check(m.LUT_DR === 0.02, 'LUT_DR');
+    // immersion is host code over engine.Hb: the reference's answers from its own Hb
+    const eg = load('engage');
+    eg.cases.forEach((c, k) => {
+      const Hb = f32(eg.results[k].Hb);
+      c.queries.forEach((q, i) => {
+        const me = { engine: { Hb: Hb.subarray(180 * i, 180 * i + 180) } };
+        const im = m.immersion.call(me, q.tipZ), want = eg.results[k].imm[i];
+        check(want === null ? im === null : im !== null && im.st === want[0] && im.ex === want[1], 'immersion ' + k + '/' + i);
+      });
+    });
   } else {
     console.log('device', m.native().deviceInfo());
     const ch = load('chatter');
@@ -94,6 +104,27 @@ async function main() {
       check(vol.every((v, i) => Math.abs(v - ref.vol[i]) <= 1e-12 * Math.max(1, Math.abs(ref.vol[i]))), 'vol ' + k);
       check(Math.abs(s.removed - ref.removed) <= 1e-9 * Math.max(1, ref.removed), 'removed ' + k);
       c.samples.forEach(([x, y], i) => check(Object.is(s.sample(x, y), num(ref.sample[i])), 'sample ' + k + '/' + i));
+      s.dispose();
+    });
+    // the simulator's engagement sampling as methods of a stand-in simulator on this Stock
+    const eg = load('engage');
+    eg.cases.forEach((c, k) => {
+      const sc = st.cases[c.stock], ref = eg.results[k];
+      const s = new m.Stock(sc.w, sc.l, sc.h, sc.cell);
+      if (sc.wallBand) s.wallBand = sc.wallBand;
+      const luts = (sc.tools || []).map(m.tipLut);
+      s.stampBatch(sc.stamps.map(p => ({ cx: p.cx, cy: p.cy, tipZ: p.tipZ, R: p.R, type: p.type, errVal: p.errVal,
+        line: p.line, mark: p.mark || null, lut: p.lut === undefined ? null : luts[p.lut] })));
+      const HbRef = f32(ref.Hb);
+      c.queries.forEach((q, i) => {
+        const me = { engine: { Hb: new Float32Array(180) }, activeTool: { R: q.R }, stock: s };
+        const h = m.fillEngagement.call(me, q.cx, q.cy, [q.ux, q.uy]);
+        check(Object.is(h, num(ref.hmax[i])) && sameBits(me.engine.Hb, HbRef.subarray(180 * i, 180 * i + 180)), 'fillEngagement ' + k + '/' + i);
+        const im = m.immersion.call(me, q.tipZ), want = ref.imm[i];
+        check(want === null ? im === null : im !== null && im.st === want[0] && im.ex === want[1], 'immersion ' + k + '/' + i);
+      });
+      const b = m.engagementBatch(s, c.queries);
+      check(b.every((r, i) => Object.is(r.hmax, num(ref.hmax[i]))), 'engagementBatch ' + k);
       s.dispose();
     });
     const { handle } = await import(path.join(here, '..', 'machinig-sim_amd', 'js', 'stabworker.js'));

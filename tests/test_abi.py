@@ -34,6 +34,9 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(mc_tool), sizeof(mc_mode), sizeof(mc_material),
          sizeof(mc_cell), sizeof(mc_trial), sizeof(mc_bisect_opts), sizeof(mc_limit), sizeof(mc_stamp));
   printf("%zu %zu %zu\n", offsetof(mc_cell, S), offsetof(mc_cell, revs), offsetof(mc_limit, steps));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(mc_stamp_opt), sizeof(mc_engage_query), sizeof(mc_engage_result),
+         sizeof(mc_train_opts), sizeof(mc_lobe_args), sizeof(mc_search_tuning));
+  printf("%zu %zu\n", offsetof(mc_stamp_opt, lut), offsetof(mc_engage_result, imm));
   return 0;
 }'''
     import tempfile
@@ -45,4 +48,7 @@ int main(void) {
     want = [C.sizeof(t) for t in (_abi.Tool, _abi.Mode, _abi.Material, _abi.Cell, _abi.Trial,
                                   _abi.BisectOpts, _abi.Limit, _abi.Stamp)]
     want += [_abi.Cell.S.offset, _abi.Cell.revs.offset, _abi.Limit.steps.offset]
+    want += [C.sizeof(t) for t in (_abi.StampOpt, _abi.EngageQuery, _abi.EngageResult, _abi.TrainOpts,
+                                   _abi.LobeArgs, _abi.SearchTuning)]
+    want += [_abi.StampOpt.lut.offset, _abi.EngageResult.imm.offset]
     assert [int(x) for x in out] == want

@@ -176,3 +176,51 @@ def test_dirty_rect_only_for_stamps_that_change(mc):
         o.clear_dirty()
     assert seen[0][2] > seen[0][0] and seen[2] == [2 ** 31 - 1, 2 ** 31 - 1, -1, -1]
     s.close()
+
+
+def test_engagement_golden(mc):
+    """Simulator.fillEngagement / immersion (src/sim.js:301-348) against the reference's own
+    answers on the cut golden stocks: Hb bit for bit, hmax, the immersion arc"""
+    g = G.load("engage")
+    st = G.load("stock")
+    for case, ref in zip(g["cases"], g["results"]):
+        sc, sr = st["cases"][case["stock"]], st["results"][case["stock"]]
+        s, arr, opt, luts = G.stock_from_case(mc.Stock, sc, sr)
+        s.stamps_ex(arr, opt, None, *luts)
+        res, Hb = s.engagement(case["queries"])
+        assert same_bits(Hb.ravel(), G.f32(ref["Hb"]))
+        for r, hm, im, q in zip(res, ref["hmax"], ref["imm"], case["queries"]):
+            assert r.hmax == G.num(hm)
+            assert r.ap == min(float("inf"), max(0.0, G.num(hm) - q["tipZ"]))
+            assert (im is None and r.imm == 0) or (r.imm == 1 and [r.phiSt, r.phiEx] == im)
+        s.close()
+
+
+def test_engagement_matches_oracle_on_zigzag(mc):
+    import oracle
+    from mcsim import workloads as W, path
+    segs = W.zigzag_segments(300)
+    rec, _ = path.schedule(segs, 0.1)
+    arr = path.as_stamp_array(rec)
+    s = mc.Stock(51.1, 51.1, 20, 0.1)
+    s.stamps(arr, want_vol=False)
+    o = oracle.OracleStock(51.1, 51.1, 20, 0.1)
+    o.stamps(arr, len(arr))
+    rng = np.random.default_rng(9)
+    qs = []
+    for _ in range(3000):
+        a = rng.uniform(-np.pi, np.pi)
+        qs.append(dict(cx=float(rng.uniform(-3, 54)), cy=float(rng.uniform(-3, 54)), ux=float(np.cos(a)),
+                       uy=float(np.sin(a)), R=float(rng.uniform(0.5, 6)), tipZ=float(rng.uniform(-4, 0.5)),
+                       fluteLen=float(rng.uniform(1, 30))))
+    res, Hb = s.engagement(qs)
+    n_imm = 0
+    for i, q in enumerate(qs):
+        h, hb = oracle.fill_engagement(o, q["cx"], q["cy"], q["ux"], q["uy"], q["R"])
+        assert same_bits(Hb[i], hb) and res[i].hmax == h
+        assert res[i].ap == min(q["fluteLen"], max(0.0, h - q["tipZ"]))
+        im = oracle.immersion(hb, q["tipZ"])
+        assert (im is None and res[i].imm == 0) or (res[i].imm == 1 and (res[i].phiSt, res[i].phiEx) == im)
+        n_imm += im is not None
+    assert n_imm > 300
+    s.close()

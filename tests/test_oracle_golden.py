@@ -167,3 +167,40 @@ def test_lobes_and_tool_model():
     for case, ref in zip(t["cases"], t["results"]):
         k, fn = oracle.tool_model(case["D"], case["stickout"], case["kSpindle"], case.get("mHolder", 0.05))
         assert abs(k / ref["k"] - 1) < 1e-14 and abs(fn / ref["fn"] - 1) < 1e-14
+
+
+def test_engagement_bit_exact():
+    """Simulator.fillEngagement / immersion on the cut golden stocks"""
+    g = G.load("engage")
+    st = G.load("stock")
+    for case, ref in zip(g["cases"], g["results"]):
+        sc, sr = st["cases"][case["stock"]], st["results"][case["stock"]]
+        s, arr, opt, luts = G.stock_from_case(oracle.OracleStock, sc, sr)
+        s.stamps_ex(arr, opt, len(arr), *luts)
+        Hb_ref = G.f32(ref["Hb"]).reshape(-1, 180)
+        for q, hm, im, hb in zip(case["queries"], ref["hmax"], ref["imm"], Hb_ref):
+            h, Hb = oracle.fill_engagement(s, q["cx"], q["cy"], q["ux"], q["uy"], q["R"])
+            assert np.array_equal(bits(Hb), bits(hb)) and h == G.num(hm), q
+            got = oracle.immersion(Hb, q["tipZ"])
+            assert (got is None and im is None) or list(got) == im, q
+        assert sum(im is not None for im in ref["imm"]) >= 5
+        assert sum(G.num(h) > -1e300 for h in ref["hmax"]) > 50
+
+
+def test_engagement_orientation():
+    """With feed along +x the leading bins (sin phi > 0.02) sample ahead of the cutter and
+    phi runs from the +y side (phi -> 0) to the -y side (phi -> pi): a stock cut away on
+    the +y side shows the cut height in the first bins only."""
+    s = oracle.OracleStock(20.0, 20.0, 5.0, 0.1)
+    st = G.stamps_from_json([dict(cx=10.0, cy=20.0 + 7.0, tipZ=-2.0, R=9.0, type="flat", errVal=0, line=1)])
+    s.stamps(st, 1)
+    h, Hb = oracle.fill_engagement(s, 10.0, 10.0, 1.0, 0.0, 3.0)
+    phi = (np.arange(180) + 0.5) / 180 * 2 * np.pi
+    lead = np.sin(phi) > 0.02
+    assert np.all(np.isneginf(Hb[~lead]))
+    # sample points: centre + u*cell + r*(sin(phi) u + cos(phi) v), v = (-uy, ux) = (0, 1)
+    y = 10.0 + 2.95 * np.cos(phi)
+    cut = lead & (np.hypot(10.1 + 2.95 * np.sin(phi) - 10.0, y - 27.0) < 8.9)
+    assert np.all(Hb[cut] == -2.0) and np.all(Hb[lead & (y < 17.9)] == 0.0)
+    assert h == 0.0
+    assert oracle.immersion(Hb, -1.0) is None or oracle.immersion(Hb, -1.0)[0] > 0
