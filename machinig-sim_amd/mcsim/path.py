@@ -1,5 +1,5 @@
-"""Stamp schedule along a G1 path: the sub-step positions Simulator.step visits
-(src/sim.js:351-377, with an unlimited frame budget), in the reference's arithmetic.
+"""Stamp schedule along a path: the sub-step positions Simulator.step visits
+(src/sim.js:351-377,448, with an unbounded frame budget), in the reference's arithmetic.
 
 segLen is Math.hypot of the three deltas (V8: scale by the largest magnitude, Kahan-
 compensated sum, sqrt, rescale); the running arc length segS accumulates ds in order;
@@ -37,6 +37,31 @@ def stamp_step(hx):
     return min(hx, 0.4) * 0.6
 
 
+def substeps(x, y, z, rapid, hx):
+    """Sub-step positions of Simulator.step over a discretised path (src/sim.js:363-377,448)
+    with an unbounded frame budget: segment k runs from point k-1 to point k in steps of
+    ds = min(dsMax, segLen - segS) (dsMax 1 mm on rapids, min(hx, 0.4) * 0.6 otherwise),
+    ending when segLen - segS <= 1e-9 before a step or segS >= segLen - 1e-9 after one.
+    Returns (px, py, pz, segment index) arrays."""
+    step = stamp_step(hx)
+    px, py, pz, seg = [], [], [], []
+    for k in range(1, len(x)):
+        ax, ay, az, bx, by, bz = x[k - 1], y[k - 1], z[k - 1], x[k], y[k], z[k]
+        seg_len = hypot_js(bx - ax, by - ay, bz - az)
+        ds_max = 1.0 if rapid[k] else step
+        s = 0.0
+        while not (seg_len - s <= 1e-9):
+            s += min(ds_max, seg_len - s)
+            f = s / seg_len if seg_len > 0 else 1.0
+            px.append(ax + (bx - ax) * f)
+            py.append(ay + (by - ay) * f)
+            pz.append(az + (bz - az) * f)
+            seg.append(k)
+            if s >= seg_len - 1e-9:
+                break
+    return np.array(px), np.array(py), np.array(pz), np.array(seg, dtype=np.int64)
+
+
 def schedule(segments, hx, err_val=0.0):
     """segments: (x0, y0, z0, x1, y1, z1, R, type, line) -> numpy record arrays of stamps
     (cx, cy, tipZ, R, type, line) and the index of each stamp's segment."""
@@ -56,6 +81,8 @@ def schedule(segments, hx, err_val=0.0):
             tt.append(typ)
             ll.append(line)
             seg.append(si)
+            if s >= seg_len - 1e-9:
+                break
     out = np.zeros(len(cx), dtype=[("cx", "<f8"), ("cy", "<f8"), ("tipZ", "<f8"), ("R", "<f8"),
                                    ("type", "<i4"), ("line", "<i4"), ("errVal", "<f8")])
     out["cx"], out["cy"], out["tipZ"], out["R"] = cx, cy, cz, rr

@@ -13,6 +13,8 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
                         samples, maxInDisk answers, block maxima
   golden_tiplut.json    tipLut tables
   golden_engage.json    Simulator.fillEngagement / immersion on cut stocks
+  golden_sim.json       Simulator runs of short programs: path, issued stamps, engagement
+                        samples, final maps
   golden_lobes.json     stabilityLobes / alimAt / directionalCoeffs
   golden_toolmodel.json toolModel (k, fn)
 
@@ -168,6 +170,19 @@ def build_cases(quick):
             qs.append(dict(cx=w / 2, cy=l / 2, ux=ux, uy=uy, R=2.0, tipZ=-1.0))
         engage.append(dict(stock=3 + k, queries=qs))
     cases["engage"] = engage
+    # the simulator itself on short programs: its path, the stamps it issues to the actual
+    # and the commanded stock, its engagement samples, and the final maps
+    progs = [
+        ["G90 G54 G17", "S8000 M3", "G0 X-6 Y4 Z5", "G1 Z-1.5 F300", "G1 X30 F900", "G1 Y9", "G1 X-6",
+         "G1 Y14", "G2 X10 Y14 R8", "G1 X30", "G0 Z5", "M30"],
+        ["G90 G54 G17", "S12000 M3", "G0 X-4 Y-4 Z3", "G1 Z-0.8 F400", "G1 X26 Y6 F1500", "G1 Y11",
+         "G3 X10 Y11 R8", "G1 X-4 Y16 Z-1.6", "G1 X26", "G0 Z5", "M30"],
+        ["G90 G54 G17", "S6000 M3", "G0 X-5 Y3 Z4", "G1 Z-2 F250", "G1 X28 F700", "G1 Y8 Z-2.5",
+         "G1 X-5", "G0 Z6", "G0 X12 Y12", "G1 Z-1 F200", "G2 X12 Y12 I0 J3 F600", "G0 Z6", "M30"],
+    ]
+    sim_tools = [dict(type="flat", D=10.0), dict(type="ball", D=6.0), dict(type="bull", D=8.0, cornerR=1.5)]
+    cases["sim"] = [dict(program="\n".join(p), tool=t, stock=dict(type="box", w=30.0, l=20.0, h=8.0),
+                         cell=0.25) for p, t in zip(progs, sim_tools)]
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]
@@ -216,7 +231,7 @@ def main():
         json.dump(cases, f)
     run(cin, cout)
     out = json.load(open(cout))
-    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage"):
+    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim"):
         rec = dict(cases=cases[key], results=out[key])
         if key + "_meta" in cases:
             rec["meta"] = cases[key + "_meta"]

@@ -4,6 +4,7 @@
 //
 // usage: node --experimental-loader ./downlevel_loader.mjs run_reference.mjs cases.json out.json
 import fs from 'fs';
+import zlib from 'zlib';
 
 const REF = process.env.MC_REFERENCE_SRC || '/root/reference/src/';
 const [casesPath, outPath] = process.argv.slice(2);
@@ -22,6 +23,8 @@ CE = CuttingEngine; NB = BINS;
 const b64f32 = a => Buffer.from(new Float32Array(a).buffer).toString('base64');
 const b64i32 = a => Buffer.from(new Int32Array(a).buffer).toString('base64');
 const b64f64 = a => Buffer.from(new Float64Array(a).buffer).toString('base64');
+// deflated (zlib) little-endian bytes, for the larger maps
+const z32 = (T, a) => zlib.deflateSync(Buffer.from(new T(a).buffer)).toString('base64');
 
 // the engine chatterAt builds (src/stabtd.js:15-20), kept open to read its chatter index
 function chatterEngine(c, ap) {
@@ -133,6 +136,52 @@ if (cases.engage) {
       imm.push(m === null ? null : [m.st, m.ex]);
     }
     return { Hb: b64f32(Hb), hmax, imm };
+  });
+}
+
+// whole simulator runs (one unbounded frame at a time), recording every stamp it issues to
+// the actual and the commanded stock and every engagement sample it takes
+if (cases.sim) {
+  const { Simulator } = await import(REF + 'sim.js');
+  out.sim = cases.sim.map(({ program, tool, stock, cell }) => {
+    const sim = new Simulator();
+    sim.cfg.cell = cell;
+    Object.assign(sim.cfg.tool, tool);
+    sim.load(program, { stock, origin: { x: 0, y: 0, z: 0 } });
+    const rec = { actual: [], ideal: [], engage: [] };
+    const tabs = [];
+    const lutId = l => { if (!l) return -1; let i = tabs.indexOf(l); if (i < 0) { tabs.push(l); i = tabs.length - 1; } return i; };
+    const wrap = (st, out) => {
+      const f = st.stamp.bind(st);
+      st.stamp = (cx, cy, tipZ, R, type, errVal, mark, lut) => {
+        const v = f(cx, cy, tipZ, R, type, errVal, mark, lut);
+        out.push([cx, cy, tipZ, R, type === 'ball' ? 1 : 0, errVal, mark ? mark.ux : 0, mark ? mark.uy : 0,
+          mark ? mark.f : 0, lutId(lut), st.curLine, v]);
+        return v;
+      };
+    };
+    wrap(sim.stock, rec.actual); wrap(sim.ideal, rec.ideal);
+    const fill = sim.fillEngagement.bind(sim);
+    sim.fillEngagement = (cx, cy, u) => {
+      const h = fill(cx, cy, u);
+      rec.engage.push([rec.actual.length, cx, cy, u[0], u[1], sim.activeTool.R, num(h)]);
+      return h;
+    };
+    let frames = 0;
+    while (!sim.done && frames < 100) { sim.step(1e6); frames++; }
+    const maps = s => ({ H: z32(Float32Array, s.H), err: z32(Float32Array, s.err), wallD: z32(Float32Array, s.wallD),
+      lineCut: z32(Int32Array, s.lineCut), lineWall: z32(Int32Array, s.lineWall), markS: z32(Float32Array, s.markS),
+      markF: z32(Float32Array, s.markF), removed: s.removed });
+    const col = (rows, k) => rows.map(r => r[k]);
+    const pack = rows => ({ cx: b64f64(col(rows, 0)), cy: b64f64(col(rows, 1)), tipZ: b64f64(col(rows, 2)),
+      R: b64f64(col(rows, 3)), type: col(rows, 4), errVal: b64f64(col(rows, 5)), ux: b64f64(col(rows, 6)),
+      uy: b64f64(col(rows, 7)), f: b64f64(col(rows, 8)), lut: col(rows, 9), line: col(rows, 10), vol: b64f64(col(rows, 11)) });
+    const P = sim.path;
+    return { frames, nx: sim.stock.nx, ny: sim.stock.ny, hx: sim.stock.hx, hy: sim.stock.hy, zBottom: sim.stock.zBottom,
+      path: { x: b64f64(P.map(p => p.x)), y: b64f64(P.map(p => p.y)), z: b64f64(P.map(p => p.z)),
+        rapid: P.map(p => (p.rapid ? 1 : 0)), line: P.map(p => p.line) },
+      actual: pack(rec.actual), ideal: pack(rec.ideal), luts: tabs.map(b64f32),
+      engage: rec.engage, stock: maps(sim.stock), commanded: maps(sim.ideal) };
   });
 }
 

@@ -31,6 +31,16 @@ def i32(s):
     return np.frombuffer(base64.b64decode(s), dtype="<i4")
 
 
+def zf32(s):
+    import zlib
+    return np.frombuffer(zlib.decompress(base64.b64decode(s)), dtype="<f4")
+
+
+def zi32(s):
+    import zlib
+    return np.frombuffer(zlib.decompress(base64.b64decode(s)), dtype="<i4")
+
+
 def num(v):
     if v == "inf":
         return float("inf")
@@ -102,3 +112,56 @@ def stock_from_case(stock_mod, case, ref, use_ex=True):
     luts = luts_from_json(ref.get("luts", []))
     st = case["stamps"]
     return s, stamps_from_json(st), opts_from_json(st), luts
+
+
+def sim_stamps(cols):
+    """The stamps a simulator run issued (columns of golden_sim.json) as Stamp / StampOpt
+    arrays, with the per-stamp volumes it got back"""
+    cx, cy, tz, R = f64(cols["cx"]), f64(cols["cy"]), f64(cols["tipZ"]), f64(cols["R"])
+    ev, ux, uy, f = f64(cols["errVal"]), f64(cols["ux"]), f64(cols["uy"]), f64(cols["f"])
+    n = len(cx)
+    arr = (Stamp * n)()
+    opt = (StampOpt * n)()
+    for i in range(n):
+        arr[i] = Stamp(cx[i], cy[i], tz[i], R[i], cols["type"][i], cols["line"][i], ev[i])
+        opt[i] = StampOpt(ux[i], uy[i], f[i], cols["lut"][i], 0)
+    return arr, opt, f64(cols["vol"])
+
+
+def sub(arr, a, b):
+    """elements a..b-1 of a ctypes array as a new array"""
+    import ctypes
+    t = type(arr[0]) if len(arr) else None
+    out = (t * (b - a))()
+    ctypes.memmove(out, ctypes.addressof(arr) + a * ctypes.sizeof(t), (b - a) * ctypes.sizeof(t))
+    return out
+
+
+def replay_sim(make_stock, stamp_fn, engage_fn, c, r):
+    """Replay a simulator run's stamps on a stock, answering its engagement samples at the
+    moments it took them; returns the actual and commanded stocks and the volumes"""
+    spec = c["stock"]
+    s = make_stock(spec["w"], spec["l"], spec["h"], c["cell"])
+    ideal = make_stock(spec["w"], spec["l"], spec["h"], c["cell"])
+    ideal.set_wall_band(2.0)
+    luts = luts_from_json(r["luts"])
+    arr, opt, vol_ref = sim_stamps(r["actual"])
+    iarr, iopt, ivol_ref = sim_stamps(r["ideal"])
+    vol, done = [], 0
+    for k, cx, cy, ux, uy, R, hm in r["engage"]:
+        if k > done:
+            vol += list(stamp_fn(s, sub(arr, done, k), sub(opt, done, k), luts))
+            done = k
+        assert engage_fn(s, cx, cy, ux, uy, R) == num(hm)
+    if len(arr) > done:
+        vol += list(stamp_fn(s, sub(arr, done, len(arr)), sub(opt, done, len(arr)), luts))
+    ivol = stamp_fn(ideal, iarr, iopt, luts)
+    return s, ideal, np.array(vol), vol_ref, np.asarray(ivol), ivol_ref
+
+
+def check_maps(stock_maps, ref, bits_of):
+    for key, want in (("H", zf32(ref["H"])), ("err", zf32(ref["err"])), ("wallD", zf32(ref["wallD"])),
+                      ("markS", zf32(ref["markS"])), ("markF", zf32(ref["markF"]))):
+        assert np.array_equal(bits_of(stock_maps[key]), bits_of(want)), key
+    assert np.array_equal(stock_maps["lineCut"], zi32(ref["lineCut"]))
+    assert np.array_equal(stock_maps["lineWall"], zi32(ref["lineWall"]))

@@ -224,3 +224,29 @@ def test_engagement_matches_oracle_on_zigzag(mc):
         n_imm += im is not None
     assert n_imm > 300
     s.close()
+
+
+def test_simulator_run_replayed(mc):
+    """Whole runs of the reference's simulator: the stamps it issued, replayed on the device
+    stock, give its final actual and commanded maps bit for bit and its per-stamp volumes;
+    the engagement kernel answers every sample it took, on the stock as it stood then"""
+    g = G.load("sim")
+
+    def stamp_fn(s, a, o, luts):
+        return s.stamps_ex(a, o, None, *luts)
+
+    def engage_fn(s, cx, cy, ux, uy, R):
+        res, _ = s.engagement([dict(cx=cx, cy=cy, ux=ux, uy=uy, R=R, tipZ=0.0)], want_hb=False)
+        return res[0].hmax
+
+    for c, r in zip(g["cases"], g["results"]):
+        s, ideal, vol, vol_ref, ivol, ivol_ref = G.replay_sim(mc.Stock, stamp_fn, engage_fn, c, r)
+        for st, ref in ((s, r["stock"]), (ideal, r["commanded"])):
+            e, w, lc, lw = st.side()
+            ms, mf = st.marks()
+            maps = dict(H=st.H(), err=e, wallD=w, markS=ms, markF=mf, lineCut=lc, lineWall=lw)
+            G.check_maps(maps, ref, lambda a: np.asarray(a, np.float32).view(np.int32))
+        np.testing.assert_allclose(vol, vol_ref, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(ivol, ivol_ref, rtol=1e-12, atol=1e-12)
+        s.close()
+        ideal.close()

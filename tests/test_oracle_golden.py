@@ -204,3 +204,39 @@ def test_engagement_orientation():
     assert np.all(Hb[cut] == -2.0) and np.all(Hb[lead & (y < 17.9)] == 0.0)
     assert h == 0.0
     assert oracle.immersion(Hb, -1.0) is None or oracle.immersion(Hb, -1.0)[0] > 0
+
+
+def test_simulator_schedule_bit_exact():
+    """The sub-step positions of Simulator.step (src/sim.js:363-377,448) from the path the
+    reference's own simulator discretised: the commanded stamps it issued"""
+    from mcsim import path
+    g = G.load("sim")
+    for r in g["results"]:
+        P = r["path"]
+        px, py, pz, _ = path.substeps(G.f64(P["x"]), G.f64(P["y"]), G.f64(P["z"]), P["rapid"], r["hx"])
+        I = r["ideal"]
+        assert np.array_equal(px, G.f64(I["cx"])) and np.array_equal(py, G.f64(I["cy"]))
+        assert np.array_equal(pz, G.f64(I["tipZ"]))
+
+
+def test_simulator_run_replayed_bit_exact():
+    """A whole simulator run: its stamps replayed on the oracle give the final actual and
+    commanded maps bit for bit, each stamp's volume, and every engagement sample it took
+    on the stock as it stood at that sub-step"""
+    g = G.load("sim")
+
+    def stamp_fn(s, a, o, luts):
+        return s.stamps_ex(a, o, len(a), *luts)
+
+    def engage_fn(s, cx, cy, ux, uy, R):
+        return oracle.fill_engagement(s, cx, cy, ux, uy, R)[0]
+
+    for c, r in zip(g["cases"], g["results"]):
+        s, ideal, vol, vol_ref, ivol, ivol_ref = G.replay_sim(oracle.OracleStock, stamp_fn, engage_fn, c, r)
+        assert (s.nx, s.ny, s.hx, s.hy) == (r["nx"], r["ny"], r["hx"], r["hy"])
+        for st, ref in ((s, r["stock"]), (ideal, r["commanded"])):
+            maps = dict(H=st.H(), err=st.err(), wallD=st.wallD(), markS=st.markS(), markF=st.markF(),
+                        lineCut=st.lineCut(), lineWall=st.lineWall())
+            G.check_maps(maps, ref, bits)
+        assert np.array_equal(vol, vol_ref) and np.array_equal(ivol, ivol_ref)
+        assert np.count_nonzero(vol) > 100 and len(r["engage"]) > 500
