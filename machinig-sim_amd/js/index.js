@@ -3,4 +3,6 @@ export { stabilityLobes, alimAt, directionalCoeffs } from './sld.js';
 export { modeSet, frf } from './frf.js';
 export { Stock, tipLut, LUT_DR } from './stock.js';
 export { engagementBatch, fillEngagement, immersion } from './engage.js';
+export { toolModel, isAssembly } from './tool.js';
+export { modesAt, tipFactor, bearingFactor, twoDofModes, speedDependent, SOFT_DEFAULT } from './spindle.js';
 export { native } from './native.js';

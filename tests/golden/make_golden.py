@@ -16,7 +16,8 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_sim.json       Simulator runs of short programs: path, issued stamps, engagement
                         samples, final maps
   golden_lobes.json     stabilityLobes / alimAt / directionalCoeffs
-  golden_toolmodel.json toolModel (k, fn)
+  golden_toolmodel.json toolModel (k, fn and the rest of its answer)
+  golden_spindle.json   modesAt / tipFactor / bearingFactor / speedDependent
 
 usage: python tests/golden/make_golden.py [--quick]
 """
@@ -206,6 +207,30 @@ def build_cases(quick):
                                kSpindle=50000),
                           dict(D=6, R=3, N=2, helixDeg=30, type="flat", fluteLen=15, stickout=22,
                                kSpindle=40000, mHolder=0.03)]
+    cases["toolModel"] += [dict(D=D, N=3, helixDeg=35, type="flat", fluteLen=2 * D, stickout=so, kSpindle=ks,
+                                **({"mHolder": mh} if mh is not None else {}))
+                           for D, so, ks, mh in ((3.0, 12.0, 30000, None), (12.0, 45.0, 80000, 0.08),
+                                                 (20.0, 60.0, 120000, 0.0), (8.0, 35.0, 60000, None))]
+    # speed-dependent modes: the speed model through the tool's chain, the two-mass model,
+    # and measured FRFs at several speeds
+    one = dict(k=2.0e4, fn=1800.0, zeta=0.03)
+    multi = dict(k=2.0e4, fn=1800.0, zeta=0.03, modesX=[dict(k=2.5e4, fn=1500.0, zeta=0.02),
+                 dict(k=6.0e4, fn=3100.0, zeta=0.04)], modesY=[dict(k=3.0e4, fn=1700.0, zeta=0.03)])
+    frfs = dict(k=2.0e4, fn=1800.0, zeta=0.03, speedFrfs=[
+        dict(S=20000, modesX=[dict(k=1.8e4, fn=1600.0, zeta=0.03), dict(k=5e4, fn=2900.0, zeta=0.05)]),
+        dict(S=5000, modesX=[dict(k=2.2e4, fn=1850.0, zeta=0.02), dict(k=5.5e4, fn=3000.0, zeta=0.04)],
+             modesY=[dict(k=2.4e4, fn=1900.0, zeta=0.03), dict(k=5.2e4, fn=3050.0, zeta=0.04)]),
+        dict(S=12000, modesX=[dict(k=2.0e4, fn=1700.0, zeta=0.025)])])
+    chain = dict(kTool=9000.0, kSp=50000.0)
+    two = dict(kTool=9000.0, kSp=50000.0, twoDof=dict(mTool=0.08, mSp=2.5, zSp=0.06))
+    speeds = [0, 3000, 5000, 8000.5, 12000, 16000, 20000, 26000, 40000]
+    softs = [None, dict(drop=0.25, Sref=30000), dict(drop=0.0), dict(drop=0.95, Sref=20000), dict(drop=0.4)]
+    sp = []
+    for modal, ch in ((one, None), (one, chain), (multi, chain), (frfs, chain), (dict(one, auto=True), two),
+                      (dict(multi, auto=True), chain)):
+        for soft in softs:
+            sp.append(dict(modal=modal, chain=ch, soft=soft, speeds=speeds))
+    cases["spindle"] = sp
     return cases
 
 
@@ -231,7 +256,7 @@ def main():
         json.dump(cases, f)
     run(cin, cout)
     out = json.load(open(cout))
-    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim"):
+    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle"):
         rec = dict(cases=cases[key], results=out[key])
         if key + "_meta" in cases:
             rec["meta"] = cases[key + "_meta"]

@@ -187,7 +187,19 @@ if (cases.sim) {
 
 if (cases.tiplut) out.tiplut = cases.tiplut.map(t => b64f32(tipLut(t)));
 
-if (cases.toolModel) out.toolModel = cases.toolModel.map(t => { const m = toolModel(t); return { k: m.k, fn: m.fn }; });
+if (cases.toolModel) out.toolModel = cases.toolModel.map(t => { const m = toolModel(t); return { k: m.k, fn: m.fn, De: m.De, I: m.I, kTool: m.kTool, mEff: m.mEff }; });
+
+if (cases.spindle) {
+  const sp = await import(REF + 'spindle.js');
+  out.spindle = cases.spindle.map(({ modal, chain, soft, speeds }) => {
+    const sf = soft === null ? undefined : soft;
+    return { modes: speeds.map(S => sp.modesAt(modal, S, chain, sf)),
+      bearing: speeds.map(S => sp.bearingFactor(S, sf)),
+      tip: chain ? speeds.map(S => sp.tipFactor(S, chain.kTool, chain.kSp, sf)) : null,
+      twoDof: chain && chain.twoDof ? sp.twoDofModes(chain.kTool, chain.twoDof.mTool, chain.kSp, chain.twoDof.mSp, modal.zeta, chain.twoDof.zSp) : null,
+      dependent: sp.speedDependent(modal, sf) };
+  });
+}
 
 fs.writeFileSync(outPath, JSON.stringify(out));
 console.error('done', (Date.now() - t0) / 1e3, 's');

@@ -35,6 +35,22 @@ async function main() {
     const tl = load('tiplut');
     tl.cases.forEach((t, k) => check(sameBits(m.tipLut(t), f32(tl.results[k])), 'tipLut ' + k));
     check(m.LUT_DR === 0.02, 'LUT_DR');
+    // toolModel and the speed-dependent modes are host code: the reference's doubles
+    const tm = load('toolmodel');
+    tm.cases.forEach((t, k) => {
+      const got = m.toolModel(t), want = tm.results[k];
+      check(['k', 'fn', 'De', 'I', 'kTool', 'mEff'].every(f => Object.is(got[f], want[f])), 'toolModel ' + k);
+    });
+    const sp = load('spindle');
+    const same = (a, b) => JSON.stringify(a) === JSON.stringify(b);
+    sp.cases.forEach((c, k) => {
+      const soft = c.soft === null ? undefined : c.soft, want = sp.results[k];
+      check(same(c.speeds.map(S => m.modesAt(c.modal, S, c.chain, soft)), want.modes), 'modesAt ' + k);
+      check(same(c.speeds.map(S => m.bearingFactor(S, soft)), want.bearing), 'bearingFactor ' + k);
+      if (c.chain) check(same(c.speeds.map(S => m.tipFactor(S, c.chain.kTool, c.chain.kSp, soft)), want.tip), 'tipFactor ' + k);
+      if (want.twoDof) check(same(m.twoDofModes(c.chain.kTool, c.chain.twoDof.mTool, c.chain.kSp, c.chain.twoDof.mSp, c.modal.zeta, c.chain.twoDof.zSp), want.twoDof), 'twoDofModes ' + k);
+      check(m.speedDependent(c.modal, soft) === want.dependent, 'speedDependent ' + k);
+    });
     // immersion is host code over engine.Hb: the reference's answers from its own Hb
     const eg = load('engage');
     eg.cases.forEach((c, k) => {
