@@ -33,6 +33,8 @@ export class Stock {
     this._wallBand = 0;
     this._mirror = null;
     this._bmax = null;
+    // wall errors are written by markWall only and read by the viewers: a host array
+    this.wallErr = new Float32Array(this.nx * this.ny);
     this.clearDirty();
   }
 
@@ -89,6 +91,26 @@ export class Stock {
     return list.map((_, i) => ({ h: o[3 * i], x: o[3 * i + 1], y: o[3 * i + 2] }));
   }
   inside(x, y) { return x >= 0 && y >= 0 && x <= this.w && y <= this.l; }
+
+  // Stock.markWall (src/stock.js:101-116): val on the wall cells just outside the cutter
+  // (R < r <= R + two cells) on the side facing (nx, ny); its box joins the dirty rectangle
+  markWall(cx, cy, R, nx, ny, val) {
+    const Ro = R + 2 * Math.max(this.hx, this.hy), hx = this.hx, hy = this.hy, W = this.wallErr;
+    const i0 = Math.max(0, Math.floor((cx - Ro) / hx)), i1 = Math.min(this.nx - 1, Math.ceil((cx + Ro) / hx));
+    const j0 = Math.max(0, Math.floor((cy - Ro) / hy)), j1 = Math.min(this.ny - 1, Math.ceil((cy + Ro) / hy));
+    for (let j = j0; j <= j1; j++) {
+      const dy = j * hy - cy;
+      for (let i = i0; i <= i1; i++) {
+        const dx = i * hx - cx, r = Math.hypot(dx, dy);
+        const outside = !(r <= R || r > Ro), facing = !((dx * nx + dy * ny) < 0.5 * r);
+        if (outside && facing) W[j * this.nx + i] = val;
+      }
+    }
+    if (i0 <= i1 && j0 <= j1) {
+      const d = this.dirty;
+      d.i0 = Math.min(d.i0, i0); d.j0 = Math.min(d.j0, j0); d.i1 = Math.max(d.i1, i1); d.j1 = Math.max(d.j1, j1);
+    }
+  }
 
   // remove material under the tool ('flat' | 'ball'; mark {ux, uy, f} for the floor's feed
   // marks, lut a tipLut table); returns the removed volume [mm^3]

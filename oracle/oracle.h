@@ -84,6 +84,9 @@ void or_stamp_batch_ex(or_stock *s, const mc_stamp *st, const mc_stamp_opt *opt,
                        double *vol);
 float *or_stock_marks(or_stock *s);
 float *or_stock_markf(or_stock *s);
+float *or_stock_wallerr(or_stock *s);
+void or_stock_mark_wall(or_stock *s, double cx, double cy, double R, double nx, double ny,
+                        double val);
 void or_stock_dirty(const or_stock *s, int32_t *rect);
 void or_stock_clear_dirty(or_stock *s);
 float or_stock_block_max(or_stock *s, int32_t bi, int32_t bj);

@@ -56,7 +56,8 @@ def lib():
             getattr(L, f).restype = P
         L.or_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
         L.or_stamp_batch_ex.argtypes = [P, C.POINTER(Stamp), P, C.c_int, P, P, P, P]
-        for f in ("or_stock_marks", "or_stock_markf"):
+        L.or_stock_mark_wall.argtypes = [P] + [C.c_double] * 6
+        for f in ("or_stock_marks", "or_stock_markf", "or_stock_wallerr"):
             getattr(L, f).argtypes = [P]
             getattr(L, f).restype = P
         L.or_stock_dirty.argtypes = [P, P]
@@ -198,6 +199,12 @@ class OracleStock:
 
     def markF(self):
         return self._arr("or_stock_markf", C.c_float)
+
+    def wallErr(self):
+        return self._arr("or_stock_wallerr", C.c_float)
+
+    def mark_wall(self, cx, cy, R, nx, ny, val):
+        self.L.or_stock_mark_wall(self.p, *[float(v) for v in (cx, cy, R, nx, ny, val)])
 
     def stamps_ex(self, stamps_arr, opt_arr, n, luts=None, lut_off=None, lut_len=None):
         """Stamps with feed marks / profiles: opt_arr is a StampOpt array (or None); luts

@@ -9,6 +9,7 @@
  *   maxInDisk       src/stock.js:57-80   (serial block walk)
  *   clearDirty      src/stock.js:81
  *   sample          src/stock.js:85-92
+ *   markWall        src/stock.js:101-116
  *   stamp           src/stock.js:118-164 (flat / ball / profile table, feed marks,
  *                                         dirty rectangle and block marks)
  *   tipLut          src/assembly.js:38-54
@@ -29,7 +30,7 @@
 struct or_stock {
   double w, l, zTop, zBottom, hx, hy, wallBand, removed;
   int32_t nx, ny, bnx, bny;
-  float *H, *err, *wallD, *markS, *markF, *bmax;
+  float *H, *err, *wallD, *markS, *markF, *bmax, *wallErr;
   int32_t *lineCut, *lineWall;
   unsigned char *bdirty;
   double di0, dj0, di1, dj1;   /* dirty rectangle: Infinity / -1 when empty */
@@ -57,6 +58,7 @@ or_stock *or_stock_new(double w, double l, double h, double cell) {
   s->wallD = (float *)malloc(n * sizeof(float));
   s->markS = (float *)calloc(n, sizeof(float));
   s->markF = (float *)calloc(n, sizeof(float));
+  s->wallErr = (float *)calloc(n, sizeof(float));
   s->lineCut = (int32_t *)calloc(n, sizeof(int32_t));
   s->lineWall = (int32_t *)calloc(n, sizeof(int32_t));
   for (k = 0; k < n; k++) { s->H[k] = (float)s->zTop; s->wallD[k] = INFINITY; }
@@ -71,7 +73,7 @@ or_stock *or_stock_new(double w, double l, double h, double cell) {
 
 void or_stock_free(or_stock *s) {
   if (!s) return;
-  free(s->H); free(s->err); free(s->wallD); free(s->markS); free(s->markF);
+  free(s->H); free(s->err); free(s->wallD); free(s->markS); free(s->markF); free(s->wallErr);
   free(s->lineCut); free(s->lineWall); free(s->bmax); free(s->bdirty); free(s);
 }
 
@@ -84,6 +86,28 @@ float *or_stock_err(or_stock *s) { return s->err; }
 float *or_stock_walld(or_stock *s) { return s->wallD; }
 float *or_stock_marks(or_stock *s) { return s->markS; }
 float *or_stock_markf(or_stock *s) { return s->markF; }
+float *or_stock_wallerr(or_stock *s) { return s->wallErr; }
+
+/* markWall (src/stock.js:101-116): val into wallErr on the cells within two cells outside
+ * the cutter on the side facing (nx, ny); the box joins the dirty rectangle */
+void or_stock_mark_wall(or_stock *s, double cx, double cy, double R, double nx, double ny,
+                        double val) {
+  const double w = 2 * dmax(s->hx, s->hy), Ro = R + w;
+  double a = floor((cx - Ro) / s->hx), b = ceil((cx + Ro) / s->hx);
+  double c = floor((cy - Ro) / s->hy), d = ceil((cy + Ro) / s->hy);
+  int32_t i, j;
+  a = dmax(0, a); b = dmin(s->nx - 1, b); c = dmax(0, c); d = dmin(s->ny - 1, d);
+  for (j = (int32_t)c; j <= (int32_t)d && c <= d; j++)
+    for (i = (int32_t)a; i <= (int32_t)b && a <= b; i++) {
+      const double dx = i * s->hx - cx, dy = j * s->hy - cy, r = or_hypot(dx, dy);
+      if (r <= R || r > Ro || (dx * nx + dy * ny) < 0.5 * r) continue;
+      s->wallErr[j * s->nx + i] = (float)val;
+    }
+  if (a <= b && c <= d) {
+    s->di0 = dmin(s->di0, a); s->dj0 = dmin(s->dj0, c);
+    s->di1 = dmax(s->di1, b); s->dj1 = dmax(s->dj1, d);
+  }
+}
 int32_t *or_stock_linecut(or_stock *s) { return s->lineCut; }
 int32_t *or_stock_linewall(or_stock *s) { return s->lineWall; }
 

@@ -150,9 +150,14 @@ def build_cases(quick):
             queries.append([rng.uniform(-3, w + 3), rng.uniform(-3, l + 3), r,
                             "-inf" if above == float("-inf") else above])
         samples = [[rng.uniform(-1, w + 1), rng.uniform(-1, l + 1)] for _ in range(100)]
+        walls = []
+        for q in range(40):
+            a = rng.uniform(-JS_PI, JS_PI)
+            walls.append([rng.uniform(-2, w + 2), rng.uniform(-2, l + 2), rng.uniform(0.3, 3.5),
+                          math.cos(a), math.sin(a), rng.uniform(-5, 30)])
         stocks.append(dict(w=w, l=l, h=h, cell=cell, wallBand=2.0 if variant == 1 else 0,
                            stamps=stamps, samples=samples, tools=tools, queries=queries,
-                           clearAt=len(stamps) // 2))
+                           clearAt=len(stamps) // 2, walls=walls))
     cases["stock"] = stocks
     # engagement sampling of the simulator (fillEngagement / immersion) on the cut stocks of
     # the two mark / profile sequences above

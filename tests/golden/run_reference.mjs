@@ -79,7 +79,8 @@ if (cases.trace) {
 const num = v => v === -Infinity ? '-inf' : v === Infinity ? 'inf' : v;
 const rect = d => [d.i0, d.j0, d.i1, d.j1].map(num);
 if (cases.stock) {
-  out.stock = cases.stock.map(({ w, l, h, cell, wallBand, stamps, samples, tools, queries, clearAt }) => {
+  out.stock = cases.stock.map(({ w, l, h, cell, wallBand, stamps, samples, tools, queries, clearAt, walls }) => {
+    const c = { walls };
     const s = new Stock(w, l, h, cell);
     if (wallBand) s.wallBand = wallBand;
     const luts = (tools || []).map(tipLut);
@@ -94,12 +95,20 @@ if (cases.stock) {
     });
     dirty.push(rect(s.dirty));
     ask();
+    // wall-error marks (markWall) after a clearDirty: the map and the rectangle they leave
+    let wallDirty = null;
+    if (c.walls) {
+      s.clearDirty();
+      for (const [x, y, R, nx, ny, v] of c.walls) s.markWall(x, y, R, nx, ny, v);
+      wallDirty = rect(s.dirty);
+    }
     const bmax = [];
     for (let bj = 0; bj < s.bny; bj++) for (let bi = 0; bi < s.bnx; bi++) bmax.push(s.blockMax(bi, bj));
     const smp = samples.map(([x, y]) => num(s.sample(x, y)));
     return { nx: s.nx, ny: s.ny, hx: s.hx, hy: s.hy, H: b64f32(s.H), err: b64f32(s.err), wallD: b64f32(s.wallD),
       lineCut: b64i32(s.lineCut), lineWall: b64i32(s.lineWall), markS: b64f32(s.markS), markF: b64f32(s.markF),
-      vol, removed: s.removed, sample: smp, dirty, disk, bmax: b64f32(bmax), luts: luts.map(b64f32) };
+      vol, removed: s.removed, sample: smp, dirty, disk, bmax: b64f32(bmax), luts: luts.map(b64f32),
+      wallErr: b64f32(s.wallErr), wallDirty };
   });
   console.error('stock', (Date.now() - t0) / 1e3, 's');
 }

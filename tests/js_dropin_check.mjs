@@ -113,6 +113,11 @@ async function main() {
         const bm = [];
         for (let bj = 0; bj < s.bny; bj++) for (let bi = 0; bi < s.bnx; bi++) bm.push(s.blockMax(bi, bj));
         check(sameBits(Float32Array.from(bm), f32(ref.bmax)), 'blockMax ' + k);
+        s.clearDirty();
+        for (const w of c.walls) s.markWall(...w);
+        const d2 = s.dirty;
+        check(sameBits(s.wallErr, f32(ref.wallErr)), 'markWall ' + k);
+        check(JSON.stringify([d2.i0, d2.j0, d2.i1, d2.j1]) === JSON.stringify(ref.wallDirty.map(num)), 'markWall dirty ' + k);
       }
       check(sameBits(s.H, f32(ref.H)), 'H ' + k);
       check(sameBits(s.err, f32(ref.err)) && sameBits(s.wallD, f32(ref.wallD)), 'err/wallD ' + k);

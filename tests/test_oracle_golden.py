@@ -83,6 +83,12 @@ def test_stock_maps_bit_exact():
         assert disk == want
         assert sum(d[0] != float("-inf") for d in disk) > len(disk) // 2
         assert np.array_equal(bits(s.block_maxima()), bits(G.f32(ref["bmax"])))
+        s.clear_dirty()
+        for w in case["walls"]:
+            s.mark_wall(*w)
+        assert np.array_equal(bits(s.wallErr()), bits(G.f32(ref["wallErr"])))
+        assert s.dirty() == G.rect(ref["wallDirty"])
+        assert np.count_nonzero(s.wallErr()) > 100
 
 
 def test_tip_lut_bit_exact():
