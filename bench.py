@@ -5,7 +5,7 @@ A step is one pass of the time-domain stability search (timeDomainLimit,
 {aMin 0.05, aMax 10, iters 10}) over every cell of the 256-speed x 64-tool lobe grid
 (16,384 cells, 2-4 flutes, 1-4 modes per axis, 20-80 revolutions per trial, process
 damping and size effect on).  It is timed twice: plain, and guided by the learned
-surrogate (trained first, untimed, on 65,536 cells of other seeded tools; its prediction
+surrogate (trained first, untimed, on 262,144 cells of other seeded tools; its prediction
 only steers which bisection nodes are speculated, so the results are the plain search's
 own, checked cell by cell).  value = SLD cells resolved per second by the faster of the
 two; both are in config.  The +-15 % bracket form of the surrogate is reported with its
@@ -141,7 +141,7 @@ def surrogate_leg(cells, opts, plain, steps, n_train, epochs):
     sigma = S.residual_sigma(model, Xva, yva)
     print("surrogate trained: %d epochs in %.2f s, held-out sigma(ln alim) %.3f" % (epochs, t_train, sigma),
           file=sys.stderr, flush=True)
-    prior_sigma = 2.0 * sigma
+    prior_sigma = sigma
     times, kms = [], []
     res = None
     for s in range(steps + 1):                  # one untimed warm-up
@@ -189,8 +189,8 @@ def main():
     ap.add_argument("--no-heightmap", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--train-cells", type=int, default=65536)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--train-cells", type=int, default=262144)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
     a = ap.parse_args()

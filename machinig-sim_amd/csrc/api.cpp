@@ -800,6 +800,9 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
     std::vector<mcd::ResultDev> spent(n);
     for (size_t q = 0; q < hinted.size(); q++) {
       const int i = hinted[q];
+      // a round that needed more geometry-table columns than the search kernel holds:
+      // the whole bracketed search reruns on the one-trial-per-group kernel below
+      if (hr[q].fell_back == 2) { serial.push_back(i); continue; }
       if (hr[q].fell_back) { plain.push_back(i); spent[i] = hr[q]; continue; }
       mc_limit& l = out[i];
       std::memset(&l, 0, sizeof l);

@@ -9,9 +9,11 @@
 // and written to HBM once per batch (the tile keeps it in registers).
 //
 //   stamp_prep   per stamp: bounding box (the reference's floor/ceil rule), radii, rim
-//                height and lowest reach; per chunk of CHUNK stamps: union bounding box.
-//   stamp_tiles  one 16x16 node tile per workgroup: walks the chunks whose box meets the
-//                tile, compacts (in order, with wave ballots) the stamps of a chunk that
+//                height and lowest reach; per chunk of CHUNK stamps: union bounding box
+//                and lowest reach.
+//   stamp_tiles  one 16x16 node tile per workgroup: lists, 256 chunks at a time, the chunks
+//                whose box meets the tile and whose lowest reach is below the tile's
+//                highest node, walks them, compacts (in order, with wave ballots) the stamps of a chunk that
 //                meet the tile and can still change it into LDS, and lets each node thread
 //                apply them in order.  A stamp can change a node only where H > zmin (its
 //                lowest reach: the cut, the wall test and the feed-mark test all need it),
@@ -50,6 +52,8 @@ struct StampX {
 
 struct ChunkBox {
   int32_t i0, i1, j0, j1;
+  float zmin;        // lowest reach of the chunk's stamps, rounded down to f32
+  int32_t pad[3];
 };
 
 __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* __restrict__ opt,
@@ -61,7 +65,9 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
                            int32_t* __restrict__ touched, ChunkBox* __restrict__ box) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ int bi0[256], bi1[256], bj0[256], bj1[256];
+  __shared__ float bz[256];
   int i0 = 1 << 30, i1 = -1, j0 = 1 << 30, j1 = -1;
+  float zlow = __builtin_inff();
   if (s < n) {
     const mc_stamp st = in[s];
     StampDev d;
@@ -99,21 +105,24 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
     // an f32 height h, h > zmin exactly when h > RD_f32(zmin))
     cull[s] = make_int4(d.i0, d.i1, d.j0, d.j1);
     cull_z[s] = __double2float_rd(d.zmin);
+    zlow = cull_z[s];
     if (outx) outx[s] = x;
     touched[s] = 0;
     if (d.i0 <= d.i1 && d.j0 <= d.j1) { i0 = d.i0; i1 = d.i1; j0 = d.j0; j1 = d.j1; }
   }
   bi0[threadIdx.x] = i0; bi1[threadIdx.x] = i1; bj0[threadIdx.x] = j0; bj1[threadIdx.x] = j1;
+  bz[threadIdx.x] = zlow;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       const int t = threadIdx.x;
       bi0[t] = min(bi0[t], bi0[t + w]); bi1[t] = max(bi1[t], bi1[t + w]);
       bj0[t] = min(bj0[t], bj0[t + w]); bj1[t] = max(bj1[t], bj1[t + w]);
+      bz[t] = fminf(bz[t], bz[t + w]);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) box[blockIdx.x] = ChunkBox{bi0[0], bi1[0], bj0[0], bj1[0]};
+  if (threadIdx.x == 0) box[blockIdx.x] = ChunkBox{bi0[0], bi1[0], bj0[0], bj1[0], bz[0], {0, 0, 0}};
 }
 
 // EX: tip profiles (luts) and feed marks (markS / markF) are live in this batch
@@ -143,91 +152,132 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
   const int lane = t & 63, wv = t >> 6;
   __shared__ int wcount[TILE * TILE / 64];
   __shared__ float wmaxs[TILE * TILE / 64];
-  for (int c = 0; c < nchunks; c++) {
-    const ChunkBox b = box[c];
-    if (b.i1 < ti0 || b.i0 > ti1 || b.j1 < tj0 || b.j0 > tj1) continue;  // block-uniform
-    // highest node of this wave and of the tile (H only falls, so a stale maximum is a
-    // safe upper bound for the rest of the chunk)
-    float wmax = own ? h : -__builtin_inff();
-    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
-    if (lane == 0) wmaxs[wv] = wmax;
-    // order-preserving compaction of the chunk's stamps that meet this tile and reach
-    // below its highest node
-    const int s = c * CHUNK + t;
-    bool cand = false;
-    float zmin = 0;
-    if (s < n) {
-      const int4 b4 = cull[s];
-      cand = !(b4.y < ti0 || b4.x > ti1 || b4.w < tj0 || b4.z > tj1) && b4.x <= b4.y && b4.z <= b4.w;
-      zmin = cull_z[s];
+  __shared__ int clist[TILE * TILE];   // chunks that meet the tile, in order
+  __shared__ float clz[TILE * TILE];   // and their lowest reach
+  __shared__ int ccount[TILE * TILE / 64];
+  for (int c0 = 0; c0 < nchunks; c0 += TILE * TILE) {
+    // the boxes of the next 256 chunks are read at once (one per thread) and the chunks that
+    // meet the tile listed in order, so that walking the chunk list costs one load latency
+    // per 256 chunks instead of one per chunk
+    {
+      // the tile's highest node now (H only falls): a chunk whose lowest reach is at or
+      // above it can change no node of the tile, now or later
+      float wm = own ? h : -__builtin_inff();
+      for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
+      if (lane == 0) wmaxs[wv] = wm;
+      const int cc = c0 + t;
+      bool meets = false;
+      ChunkBox b{};
+      if (cc < nchunks) {
+        b = box[cc];
+        meets = !(b.i1 < ti0 || b.i0 > ti1 || b.j1 < tj0 || b.j0 > tj1);
+      }
+      __syncthreads();
+      float tm = wmaxs[0];
+      for (int v = 1; v < TILE * TILE / 64; v++) tm = fmaxf(tm, wmaxs[v]);
+      meets = meets && tm > b.zmin;
+      const unsigned long long cm = __builtin_amdgcn_ballot_w64(meets);
+      if (lane == 0) ccount[wv] = __builtin_popcountll(cm);
+      __syncthreads();
+      int cbefore = 0;
+      for (int v = 0; v < wv; v++) cbefore += ccount[v];
+      if (meets) {
+        const int at = cbefore + __builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)cm, 0));
+        clist[at] = cc;
+        clz[at] = b.zmin;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    float tmax = wmaxs[0];
-    for (int v = 1; v < TILE * TILE / 64; v++) tmax = fmaxf(tmax, wmaxs[v]);
-    const bool hit = cand && tmax > zmin;
-    const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
-    if (lane == 0) wcount[wv] = __builtin_popcountll(m);
-    __syncthreads();
-    int before = 0, total = 0;
-    for (int v = 0; v < TILE * TILE / 64; v++) {
-      before += v < wv ? wcount[v] : 0;
-      total += wcount[v];
-    }
-    if (hit) {
-      const int slot = before + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-      list[slot] = st[s];
-      if (EX) listx[slot] = stx[s];
-    }
-    __syncthreads();
-    for (int q = 0; q < total; q++) {
-      const StampDev& p = list[q];
-      if ((double)wmax <= p.zmin) continue;   // wave-uniform: no node of this wave can change
-      double removed = 0;
-      bool changed = false;
-      if (own && i >= p.i0 && i <= p.i1 && j >= p.j0 && j <= p.j1) {
-        const double dx = xi - p.cx, dy = yj - p.cy;
-        const double r2 = dx * dx + dy * dy;
-        if (r2 >= p.tie) {
-          if (r2 <= p.Rw2 && (double)h > p.zRim) {
-            const double dd = __builtin_sqrt(r2) - p.R;
-            if (dd < (double)wd) { wd = (float)dd; lw = p.line; changed = true; }
-          }
-        } else {
-          double z;
-          if (EX && listx[q].lut_n > 0) {
-            const StampX& x = listx[q];
-            const double r = mcm::round_js(__builtin_sqrt(r2) / LUT_DR);
-            z = p.tipZ + (double)luts[x.lut_off + (r < x.lut_n - 1 ? (int)r : x.lut_n - 1)];
+    int ctotal = 0;
+    for (int v = 0; v < TILE * TILE / 64; v++) ctotal += ccount[v];
+    for (int k = 0; k < ctotal; k++) {
+      const int c = clist[k];
+      // highest node of this wave and of the tile (H only falls, so a stale maximum is a
+      // safe upper bound for the rest of the chunk)
+      float wmax = own ? h : -__builtin_inff();
+      for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
+      if (lane == 0) wmaxs[wv] = wmax;
+      // order-preserving compaction of the chunk's stamps that meet this tile and reach
+      // below its highest node
+      const int s = c * CHUNK + t;
+      __syncthreads();
+      float tmax = wmaxs[0];
+      for (int v = 1; v < TILE * TILE / 64; v++) tmax = fmaxf(tmax, wmaxs[v]);
+      // the tile may have fallen below the chunk's lowest reach since the list was made
+      if (!(tmax > clz[k])) { __syncthreads(); continue; }   // block-uniform
+      bool cand = false;
+      float zmin = 0;
+      if (s < n) {
+        const int4 b4 = cull[s];
+        cand = !(b4.y < ti0 || b4.x > ti1 || b4.w < tj0 || b4.z > tj1) && b4.x <= b4.y && b4.z <= b4.w;
+        zmin = cull_z[s];
+      }
+      const bool hit = cand && tmax > zmin;
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+      if (lane == 0) wcount[wv] = __builtin_popcountll(m);
+      __syncthreads();
+      int before = 0, total = 0;
+      for (int v = 0; v < TILE * TILE / 64; v++) {
+        before += v < wv ? wcount[v] : 0;
+        total += wcount[v];
+      }
+      if (hit) {
+        const int slot = before + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+        list[slot] = st[s];
+        if (EX) listx[slot] = stx[s];
+      }
+      __syncthreads();
+      for (int q = 0; q < total; q++) {
+        const StampDev& p = list[q];
+        if ((double)wmax <= p.zmin) continue;   // wave-uniform: no node of this wave can change
+        double removed = 0;
+        bool changed = false;
+        if (own && i >= p.i0 && i <= p.i1 && j >= p.j0 && j <= p.j1) {
+          const double dx = xi - p.cx, dy = yj - p.cy;
+          const double r2 = dx * dx + dy * dy;
+          if (r2 >= p.tie) {
+            if (r2 <= p.Rw2 && (double)h > p.zRim) {
+              const double dd = __builtin_sqrt(r2) - p.R;
+              if (dd < (double)wd) { wd = (float)dd; lw = p.line; changed = true; }
+            }
           } else {
-            z = p.type == MC_TOOL_BALL ? p.tipZ + p.R - __builtin_sqrt(p.R2 - r2) : p.tipZ;
-          }
-          if ((double)h > z) {
-            removed = (double)h - z;
-            h = (float)(z > zB ? z : zB);
-            e = p.errVal;
-            wd = __builtin_inff();
-            lc = p.line;
-            changed = true;
-          }
-          // the last pass over a floor at this depth leaves its trailing-edge arcs
-          if (EX && listx[q].mf != 0 && (double)h > z - 0.002) {
-            const StampX& x = listx[q];
-            const double pp = -dx * x.uy + dy * x.ux;
-            const double u = p.R2 - pp * pp;
-            ms = (float)(xi * x.ux + yj * x.uy + __builtin_sqrt(u > 0 ? u : 0.0));
-            mfv = (float)x.mf;
-            changed = true;
+            double z;
+            if (EX && listx[q].lut_n > 0) {
+              const StampX& x = listx[q];
+              const double r = mcm::round_js(__builtin_sqrt(r2) / LUT_DR);
+              z = p.tipZ + (double)luts[x.lut_off + (r < x.lut_n - 1 ? (int)r : x.lut_n - 1)];
+            } else {
+              z = p.type == MC_TOOL_BALL ? p.tipZ + p.R - __builtin_sqrt(p.R2 - r2) : p.tipZ;
+            }
+            if ((double)h > z) {
+              removed = (double)h - z;
+              h = (float)(z > zB ? z : zB);
+              e = p.errVal;
+              wd = __builtin_inff();
+              lc = p.line;
+              changed = true;
+            }
+            // the last pass over a floor at this depth leaves its trailing-edge arcs
+            if (EX && listx[q].mf != 0 && (double)h > z - 0.002) {
+              const StampX& x = listx[q];
+              const double pp = -dx * x.uy + dy * x.ux;
+              const double u = p.R2 - pp * pp;
+              ms = (float)(xi * x.ux + yj * x.uy + __builtin_sqrt(u > 0 ? u : 0.0));
+              mfv = (float)x.mf;
+              changed = true;
+            }
           }
         }
+        if (__any(changed) && lane == 0) touched[p.index] = 1;
+        if (vol && __any(removed != 0.0)) {
+          double r = removed;
+          for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+          if (lane == 0 && r != 0.0) atomicAdd(&vol[p.index], r);
+        }
       }
-      if (__any(changed) && lane == 0) touched[p.index] = 1;
-      if (vol && __any(removed != 0.0)) {
-        double r = removed;
-        for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
-        if (lane == 0 && r != 0.0) atomicAdd(&vol[p.index], r);
-      }
+      __syncthreads();
     }
-    __syncthreads();
+    __syncthreads();   // every wave has read ccount and clist before the next 256 chunks
   }
   if (own) {
     H[k] = h; err[k] = e; wallD[k] = wd; lineCut[k] = lc; lineWall[k] = lw;
