@@ -24,6 +24,7 @@ int mcd_launch_chatter(const mcd::LaunchDev& P, int G, int blocks, hipStream_t s
 int mcd_search_lds_bytes(int nz_cap);
 int mcd_launch_search(const mcd::LaunchDev& P, int blocks, hipStream_t stream);
 int mcd_launch_reset_queue(int32_t* q, int n, hipStream_t stream);
+int mcd_launch_clear_state(int32_t* s, int n, hipStream_t stream);
 int mcd_search_maxspec();
 constexpr int SEARCH_MAXP = 2, SEARCH_MAXSLOT = 8;   // slots per wave (a wide task: 4 waves x 2)
 // stock.hip
@@ -130,7 +131,7 @@ struct Ctx {
   hipStream_t stream = nullptr;   // default priority
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
-  DevBuf cells, tasks, out, queue, ring, hist, trace, geo, gbin;
+  DevBuf cells, tasks, out, queue, cstate, ring, hist, trace, geo, gbin;
   HostBuf h_cells, h_tasks, h_out;
   SearchGraph sgraph;
   mc_search_tuning tune = {8, 256, 0.5, 1, {0, 0, 0}};
@@ -393,6 +394,8 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   if ((st = X.tasks.ensure(n * sizeof(mcd::TaskDev)))) return st;
   if ((st = X.out.ensure(n * sizeof(mcd::ResultDev)))) return st;
   if ((st = X.queue.ensure(256))) return st;
+  // one verdict slot per cell of ts.cells (tasks index cells by their position there)
+  if ((st = X.cstate.ensure(std::max<size_t>(1, ts.cells.size()) * sizeof(int32_t)))) return st;
   if ((st = X.ring.ensure((size_t)waves * SEARCH_MAXSLOT * ring_cap * sizeof(float)))) return st;
   if ((st = X.hist.ensure((size_t)waves * SEARCH_MAXSLOT * mcd::HIST * sizeof(float2)))) return st;
   if ((st = X.geo.ensure((size_t)waves * geo_cap * sizeof(double2)))) return st;
@@ -411,6 +414,9 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   P.tasks = (const mcd::TaskDev*)X.tasks.p;
   P.out = (mcd::ResultDev*)X.out.p;
   P.queue = (int32_t*)X.queue.p;
+  P.cstate = (int32_t*)X.cstate.p;
+  if (!P.cstate || X.cstate.n < ts.cells.size() * sizeof(int32_t))
+    return fail(MC_ERR_HIP, "search state buffer missing");
   P.ntasks = nn;
   P.wtasks = P.tasks + nn;
   P.wout = P.out + nn;
@@ -428,6 +434,7 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
     HIPCHK(hipMemcpyAsync(X.cells.p, X.h_cells.p, cell_bytes, hipMemcpyHostToDevice, X.stream));
     HIPCHK(hipMemcpyAsync(X.tasks.p, X.h_tasks.p, task_bytes, hipMemcpyHostToDevice, X.stream));
     if (mcd_launch_reset_queue(P.queue, 64, X.stream)) return fail(MC_ERR_HIP, "queue reset failed");
+    if (mcd_launch_clear_state(P.cstate, (int)ts.cells.size(), X.stream)) return fail(MC_ERR_HIP, "state reset failed");
     if (events) HIPCHK(hipEventRecord(X.ev0, X.stream));
     if (mcd_launch_search(P, blocks, X.stream))
       return fail(MC_ERR_HIP, "search kernel launch failed: %s", hipGetErrorString(hipGetLastError()));

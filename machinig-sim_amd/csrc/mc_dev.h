@@ -65,6 +65,7 @@ struct LaunchDev {
   ResultDev* wout;       //   results, queue and count
   int32_t* wqueue;
   int32_t nwide;
+  int32_t* cstate;       // per cell: 0 aMax check pending, 1 it chattered, 2 it was stable
   double2* geo;          // edge geometry table (sin, cos), geo_cap entries per wave
   unsigned char* gbin;   // engagement-map bin of each table entry
   int64_t geo_cap;

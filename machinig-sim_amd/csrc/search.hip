@@ -303,6 +303,11 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
             if (k >= 1 && nvmax <= MAXP * sg && nvmax <= P.nv_cap) { nn = k; sgp = sg; spwp = spw; ntree = k; }
           }
         }
+        // a tree whose cell's aMax check came out stable is moot: the cell is open and the
+        // host reads only the check (src/stabtd.js:25), so the tree stops here
+        if (T.mode == TASK_BISECT && P.cstate &&
+            __hip_atomic_load(P.cstate + cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 2)
+          nn = -1;
         TL.plan_n = nn; TL.plan_sg = sgp; TL.plan_spw = spwp; TL.plan_first = first; TL.plan_ntree = ntree;
         TL.plan_evlo = elo; TL.plan_evhi = ehi;
       }
@@ -310,6 +315,7 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
       const int nnode = uni(TL.plan_n), SG = uni(TL.plan_sg), spw = uni(TL.plan_spw);
       const int tfirst = uni(TL.plan_first), ntree = uni(TL.plan_ntree);
       const bool ev_lo = uni(TL.plan_evlo) != 0, ev_hi = uni(TL.plan_evhi) != 0;
+      if (nnode < 0) break;                       // moot (open cell)
       if (nnode == 0) { failed = true; break; }  // the host sizes the trees so this never happens
       // this wave runs the task's slots w*spw .. w*spw + spw - 1 (lslot is the local index)
       const int lslot = lane / SG, g = lane % SG;
@@ -656,6 +662,8 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
       };
       if (amax_task) {
         take(0);
+        if (lead && P.cstate)
+          __hip_atomic_store(P.cstate + cell, last_chat ? 1 : 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
       int q = 0;
@@ -732,6 +740,16 @@ __global__ __launch_bounds__(256) void cell_search(LaunchDev P) {
 
 __global__ void reset_queue(int32_t* q, int n) {
   if ((int)threadIdx.x < n) q[threadIdx.x] = 0;
+}
+__global__ void clear_state(int32_t* s, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = 0;
+}
+int mcd_launch_clear_state(int32_t* s, int n, hipStream_t stream) {
+  if (n <= 0) return 0;
+  if (!s) return -2;
+  hipLaunchKernelGGL(clear_state, dim3((n + 255) / 256), dim3(256), 0, stream, s, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 int mcd_launch_reset_queue(int32_t* q, int n, hipStream_t stream) {
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, stream, q, n);
