@@ -538,7 +538,9 @@ int plain_search(const mc_cell* cells, const TaskSet& base, const std::vector<in
     a.cell = i; a.mode = mcd::TASK_SINGLE; a.ap = o->aMax;
     ts.tasks.push_back(a);
     ts.nzcap.push_back(nz_bound(cells[i], o->aMax));
-    ts.cost.push_back(steps * 1.0);
+    // the check is queued ahead of its own tree (and of trees up to 4/3 its size): an open
+    // cell's tree then stops as soon as the check has answered
+    ts.cost.push_back(steps * 2.0);
     mcd::TaskDev t;
     std::memset(&t, 0, sizeof t);
     t.cell = i; t.mode = mcd::TASK_BISECT; t.iters = o->iters; t.lo = o->aMin; t.hi = o->aMax; t.tol = -1;
