@@ -174,6 +174,27 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
 int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps,
                     double* Fx, double* Fy, double* x, double* y, int64_t* steps_out);
 
+/* The engine of one simulator sub-step (src/sim.js:252-257,386-416): CuttingEngine
+ * configured for the cell's tool (its own flute length, runout and runoutPhase, and fnMax),
+ * modes, material and speed, then setEngagement([ux, uy], fz, tipZ, ap) with engine.Hb =
+ * Hb (src/cutting.js:38-124), started from rest with the velocity (vx0, vy0) [m/s] set from
+ * outside.  Rigid workpiece, no end edges, flat or ball tools. */
+typedef struct {
+  double ux, uy;       /* unit feed direction */
+  double fz;           /* feed per tooth [mm] */
+  double tipZ;         /* tool tip height [mm] */
+  double ap;           /* engaged axial depth [mm] */
+  double vx0, vy0;     /* initial tool velocity [m/s] */
+  double fnMax;        /* configure's fnMax [Hz] (0: the modes alone) */
+  float Hb[MC_BINS];   /* material height at the cutter periphery per angle bin [mm] */
+} mc_engage_spec;
+
+/* Per-step Fx, Fy [N] and x, y [m] of that engine for nsteps steps (CuttingEngine.run,
+ * src/cutting.js:146-293).  The cell's arc, fz and revs are not used.  Any output may be
+ * NULL; *steps_out receives nsteps. */
+int mc_engine_trace_ex(const mc_cell* cell, const mc_engage_spec* eng, int64_t nsteps,
+                       double* Fx, double* Fy, double* x, double* y, int64_t* steps_out);
+
 /* ---- Path B: heightmap stock (src/stock.js) ---- */
 
 typedef struct mc_stock mc_stock;   /* device-resident heightfield + side maps */

@@ -196,7 +196,7 @@ int check_cell(const mc_cell& c, int i) {
   return MC_OK;
 }
 
-void make_celldev(const mc_cell& c, mcd::CellDev& d) {
+void make_celldev(const mc_cell& c, mcd::CellDev& d, double fnMax = 0) {
   std::memset(&d, 0, sizeof d);
   const int N = c.tool.N;
   const double S = js_max(100, c.S);
@@ -205,7 +205,7 @@ void make_celldev(const mc_cell& c, mcd::CellDev& d) {
   double fTop = -INFINITY;
   for (int j = 0; j < c.nmx; j++) fTop = js_max(fTop, c.mx[j].fn);
   for (int j = 0; j < c.nmy; j++) fTop = js_max(fTop, c.my[j].fn);
-  d.spt = (int)js_max(48, std::ceil(Ttooth * js_max(fTop, 0) * 24));
+  d.spt = (int)js_max(48, std::ceil(Ttooth * js_max(fTop, fnMax) * 24));
   d.dt = Ttooth / d.spt;
   d.spr = N * d.spt;
   d.dphi = d.omega * d.dt;
@@ -882,6 +882,56 @@ int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps, double* Fx, 
     if (y) y[s] = buf[4 * s + 3];
   }
   if (steps_out) *steps_out = ns;
+  return MC_OK;
+}
+
+int mc_engine_trace_ex(const mc_cell* cell, const mc_engage_spec* eng, int64_t nsteps, double* Fx,
+                       double* Fy, double* x, double* y, int64_t* steps_out) {
+  int st = ensure_init(-1);
+  if (st) return st;
+  if (!cell || !eng || nsteps < 0) return fail(MC_ERR_ARG, "bad trace arguments");
+  const mc_engage_spec& e = *eng;
+  if (!std::isfinite(e.ux) || !std::isfinite(e.uy) || !std::isfinite(e.fz) || !std::isfinite(e.tipZ) ||
+      !std::isfinite(e.ap) || !std::isfinite(e.vx0) || !std::isfinite(e.vy0) || !(e.fnMax >= 0) ||
+      !std::isfinite(cell->tool.runout) || !std::isfinite(cell->tool.runoutPhase))
+    return fail(MC_ERR_ARG, "bad engagement of the simulator's engine");
+  if ((st = check_cell(*cell, 0))) return st;
+  TaskSet ts;
+  ts.cells.resize(1);
+  mcd::CellDev& c = ts.cells[0];
+  make_celldev(*cell, c, e.fnMax);
+  if (!((double)c.spt * c.N <= 16777216.0)) return fail(MC_ERR_LIMIT, "fnMax gives too many steps per revolution");
+  // the simulator's engine: its own feed per tooth and map, the tool's runout
+  // (src/cutting.js:42-44, src/sim.js:255,403-404)
+  c.fz = e.fz;
+  c.hMax = js_max(0.1, 3 * e.fz);
+  for (int b = 0; b < MC_BINS; b++) c.Hb[b] = e.Hb[b];
+  for (int i = 0; i < c.N; i++)
+    c.dR[i] = cell->tool.runout * mcm::cos_js(2 * JS_PI * i / c.N - cell->tool.runoutPhase);
+  // one trial long enough for the trace
+  const int64_t per_rev = (int64_t)c.spt * c.N;
+  const int64_t revs = std::max<int64_t>(1, (nsteps + per_rev - 1) / per_rev);
+  if (revs > INT32_MAX || (double)per_rev * revs >= 2147483647.0) return fail(MC_ERR_LIMIT, "trace too long");
+  c.revs = (int32_t)revs;
+  ts.tasks.resize(1);
+  std::memset(&ts.tasks[0], 0, sizeof(mcd::TaskDev));
+  mcd::TaskDev& t = ts.tasks[0];
+  t.cell = 0; t.mode = mcd::TASK_SINGLE; t.ap = e.ap; t.sim = 1;
+  t.ux = e.ux; t.uy = e.uy; t.tipZ = e.tipZ; t.vx0 = e.vx0; t.vy0 = e.vy0;
+  ts.cost = {1.0};
+  ts.nzcap = {std::max(1, nz_bound(*cell, e.ap))};
+  g_ctx.last_ms = 0;
+  const int64_t ns = std::max<int64_t>(nsteps, 1);
+  std::vector<double> buf((size_t)ns * 4);
+  std::vector<mcd::ResultDev> res;
+  if ((st = run_tasks(ts, res, buf.data(), ns))) return st;
+  for (int64_t s = 0; s < nsteps; s++) {
+    if (Fx) Fx[s] = buf[4 * s];
+    if (Fy) Fy[s] = buf[4 * s + 1];
+    if (x) x[s] = buf[4 * s + 2];
+    if (y) y[s] = buf[4 * s + 3];
+  }
+  if (steps_out) *steps_out = nsteps;
   return MC_OK;
 }
 

@@ -107,8 +107,11 @@ __global__ __launch_bounds__(64) void chatter_search(LaunchDev P) {
   double fz = 0, dphi = 0, pitch = 0, binScale = 0, hMax = 0, dt = 0, db = 0;
   double Ktc = 0, Krc = 0, Kte = 0, Kre = 0, mcx = 0, pdK = 0;
   float cur[MAXP];
-  // feed direction of chatterAt: u = [1, 0], v = [-u1, u0] (src/stabtd.js:18, cutting.js:99)
-  const double ux = 1.0, uy = 0.0, vx = -0.0, vy = 1.0;
+  // feed direction: chatterAt's u = [1, 0], or the task's; v = [-u1, u0] (src/stabtd.js:18,
+  // cutting.js:99)
+  double ux = 1.0, uy = 0.0, vx = -0.0, vy = 1.0;
+  bool simeng = false;         // the task is a simulator sub-step's engine
+  double tipZs = 0, kvx = 0.01, kvy = 0.0;
   const double TWO_PI = mcm::TWO_PI;
 
   while (true) {
@@ -209,6 +212,11 @@ __global__ __launch_bounds__(64) void chatter_search(LaunchDev P) {
           mode = T.mode; iters = T.iters; aMin = T.aMin; aMax = T.aMax; tol = T.tol;
           lo_hint = T.lo; hi_hint = T.hi;
           it = 0; fell = 0; trials = 0; steps_acc = 0; visits_acc = 0; my_eng = 0;
+          simeng = mode == TASK_SINGLE && T.sim != 0;
+          ux = simeng ? T.ux : 1.0; uy = simeng ? T.uy : 0.0;
+          vx = -uy; vy = ux;
+          tipZs = T.tipZ;
+          kvx = simeng ? T.vx0 : 0.01; kvy = simeng ? T.vy0 : 0.0;
           if (mode == TASK_SINGLE) ap = T.ap;
           else if (mode == TASK_BISECT) { phase = 0; ap = aMax; }
           else { phase = 10; ap = lo_hint; }
@@ -228,7 +236,10 @@ __global__ __launch_bounds__(64) void chatter_search(LaunchDev P) {
       }
       if (alive) {
         // ---- fresh engine at depth ap: configure + setEngagement (src/stabtd.js:15-19)
-        const double fl = js_max(C->toolFluteLen, ap + 1);
+        // chatterAt lengthens the flutes to ap + 1 (src/stabtd.js:16); the simulator's
+        // engine keeps the tool's own (src/sim.js:255)
+        const double fl = simeng ? C->toolFluteLen : js_max(C->toolFluteLen, ap + 1);
+        const double tz = simeng ? tipZs : -ap;
         const double dz = js_max(0.25, fl / 48);
         const int maxSlices = (int)__builtin_ceil(fl / dz);
         nz = ap <= 0 ? 0 : (int)js_min((double)maxSlices, __builtin_ceil(ap / dz));
@@ -247,7 +258,7 @@ __global__ __launch_bounds__(64) void chatter_search(LaunchDev P) {
             r = __builtin_sqrt(js_max(1e-6, R * R - t * t));
             kap = r / R;
           }
-          L.sB[k] = b; L.sZ[k] = -ap + zr; L.sR[k] = r; L.sKap[k] = kap;
+          L.sB[k] = b; L.sZ[k] = tz + zr; L.sR[k] = r; L.sKap[k] = kap;
           L.sLag[k] = zr * C->tanB / R;
         }
         const int rn = nz * spr;
@@ -256,9 +267,11 @@ __global__ __launch_bounds__(64) void chatter_search(LaunchDev P) {
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < MC_MAX_MODES; j++) { qx[j] = 0; vxm[j] = 0; qy[j] = 0; vym[j] = 0; }
-        // the kick vx = 0.01 enters the first mode (src/stabtd.js:19, cutting.js:164)
-        vxm[0] = 0.0 + (0.01 - 0.0);
-        x = 0; y = 0; vxs = 0.01; vys = 0;
+        // a velocity set from outside (chatterAt's kick vx = 0.01) enters the first mode
+        // of each axis (src/stabtd.js:19, cutting.js:164)
+        vxm[0] = 0.0 + (kvx - 0.0);
+        vym[0] = 0.0 + (kvy - 0.0);
+        x = 0; y = 0; vxs = kvx; vys = kvy;
         pr = 0; ph = 0; dc = 0; ti = 0; tcount = 0; hpos = 0; hcount = 0;
         sTotal = sLeft = (long long)spt * N * C->revs;
 #pragma unroll

@@ -37,6 +37,11 @@ struct TaskDev {
   double aMin, aMax;  // bisection range
   double lo, hi, tol; // TASK_BRACKET hint and stopping width
   double mu, sig;     // prior on ln(alim) ~ N(mu, sig^2) guiding the speculation; sig <= 0: none
+  // sim != 0 (TASK_SINGLE only): the engine as the simulator drives it -- feed direction
+  // (ux, uy), tip height tipZ, the tool's own flute length, velocity (vx0, vy0) set from
+  // outside -- instead of chatterAt's u = (1, 0), tipZ = -ap, fluteLen >= ap + 1, vx = 0.01
+  double ux, uy, tipZ, vx0, vy0;
+  int32_t sim, pad2;
 };
 
 struct ResultDev {

@@ -64,6 +64,11 @@ class StampOpt(C.Structure):
                 ("lut", C.c_int32), ("pad", C.c_int32)]
 
 
+class EngageSpec(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("ux", "uy", "fz", "tipZ", "ap", "vx0", "vy0", "fnMax")] + \
+               [("Hb", C.c_float * 180)]
+
+
 class EngageQuery(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("cx", "cy", "ux", "uy", "R", "tipZ", "fluteLen")]
 

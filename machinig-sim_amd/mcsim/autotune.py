@@ -5,7 +5,7 @@ wide (four-wave) tasks, and how many persistent waves share the compute units.  
 setting depends on the batch (its mix of trial lengths and tree depths), so it is picked by
 timing candidates on the caller's own batch: a coarse pass over the wide-task threshold, then
 a refinement around the best, each candidate timed on the device (HIP events around the
-kernel) by the slowest of two launches.  The winner is installed with mc_set_search_tuning and returned with the table of
+kernel) by the slowest of three launches.  The winner is installed with mc_set_search_tuning and returned with the table of
 timings.  Results do not depend on the schedule; the tuner checks that every candidate
 returns the same limits.
 """
@@ -14,7 +14,7 @@ import numpy as np
 from . import lib
 
 
-def _time(cells, opts, tuning, repeats=2, prior=None):
+def _time(cells, opts, tuning, repeats=3, prior=None):
     """the slowest of `repeats` launches: the makespan depends on when each cell's check
     answers relative to its tree, so a schedule is judged by its bad runs, not its luckiest"""
     lib.set_search_tuning(**tuning)
@@ -27,7 +27,7 @@ def _time(cells, opts, tuning, repeats=2, prior=None):
 
 
 def tune_search(cells, opts, prior=None, fracs=(1e9, 0.6, 0.5, 0.4, 0.3), wide_max=(128, 256, 512),
-                waves_per_cu=(8,), repeats=2):
+                waves_per_cu=(8,), repeats=3):
     """time candidate schedules on this batch; install and return the fastest"""
     table = []
     ref = None

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
-                   SearchTuning, EngageQuery, EngageResult, cell_array, MC_OK, MC_SURR_IN, MC_STOCK_B)
+                   SearchTuning, EngageQuery, EngageResult, EngageSpec, cell_array, MC_OK, MC_SURR_IN,
+                   MC_STOCK_B)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -24,7 +25,7 @@ EXPORTS = [
     "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
     "mc_set_search_tuning", "mc_get_search_tuning", "mc_stamp_batch_ex", "mc_stock_read_marks",
     "mc_stock_dirty", "mc_stock_clear_dirty", "mc_stock_block_max", "mc_stock_max_in_disk", "mc_tip_lut",
-    "mc_stock_engagement",
+    "mc_stock_engagement", "mc_engine_trace_ex",
 ]
 
 _LIB = None
@@ -70,6 +71,8 @@ def load():
     L.mc_stock_block_max.argtypes = [P, P]
     L.mc_stock_max_in_disk.argtypes = [P, P, C.c_int, P]
     L.mc_stock_engagement.argtypes = [P, P, C.c_int, P, P]
+    L.mc_engine_trace_ex.argtypes = [C.POINTER(Cell), C.POINTER(EngageSpec), C.c_int64, P, P, P, P,
+                                     C.POINTER(C.c_int64)]
     L.mc_tip_lut.argtypes = [C.c_int32, C.c_double, C.c_double, P, C.c_int32, C.c_double, P, C.c_int32,
                              C.POINTER(C.c_int32)]
     L.mc_surrogate_features.argtypes = [C.POINTER(Cell), C.c_int, P]
@@ -163,6 +166,15 @@ def engine_trace(cell, ap, nsteps):
     n = C.c_int64(0)
     _check(load().mc_engine_trace(C.byref(cell), float(ap), int(nsteps), *[x.ctypes.data for x in a],
                                   C.byref(n)))
+    return [x[:n.value] for x in a]
+
+
+def engine_trace_ex(cell, spec, nsteps):
+    """per-step Fx, Fy, x, y of the engine as the simulator drives it (mc_engine_trace_ex)"""
+    a = [np.zeros(max(1, nsteps)) for _ in range(4)]
+    n = C.c_int64(0)
+    _check(load().mc_engine_trace_ex(C.byref(cell), C.byref(spec), int(nsteps), *[x.ctypes.data for x in a],
+                                     C.byref(n)))
     return [x[:n.value] for x in a]
 
 

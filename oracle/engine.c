@@ -75,7 +75,8 @@ static double js_min(double a, double b) {
 
 /* configure(tool, modal, mat, S, fnMax = 0), src/cutting.js:39-82 */
 static int eng_configure(Eng *e, const mc_tool *tool, double fluteLen, double runout,
-                         double runoutPhase, int nonlinear, const mc_cell *c, double S) {
+                         double runoutPhase, int nonlinear, const mc_cell *c, double S,
+                         double fnMax) {
   int i;
   double fTop;
   memset(e, 0, sizeof(*e));
@@ -91,7 +92,7 @@ static int eng_configure(Eng *e, const mc_tool *tool, double fluteLen, double ru
   fTop = -INFINITY;
   for (i = 0; i < c->nmx; i++) fTop = js_max(fTop, c->mx[i].fn);
   for (i = 0; i < c->nmy; i++) fTop = js_max(fTop, c->my[i].fn);
-  e->spt = (int)js_max(48, ceil(e->Ttooth * js_max(fTop, 0) * 24));
+  e->spt = (int)js_max(48, ceil(e->Ttooth * js_max(fTop, fnMax) * 24));
   e->dt = e->Ttooth / e->spt;
   e->nmx = c->nmx;
   e->nmy = c->nmy;
@@ -270,7 +271,7 @@ static void eng_chatter_index(const Eng *e, double *index, double *amp, double *
 /* the engine chatterAt builds: src/stabtd.js:15-19 */
 static int trial_setup(Eng *e, const mc_cell *c, double ap) {
   int b;
-  if (eng_configure(e, &c->tool, js_max(c->tool.fluteLen, ap + 1), 0, 0, c->nonlinear, c, c->S))
+  if (eng_configure(e, &c->tool, js_max(c->tool.fluteLen, ap + 1), 0, 0, c->nonlinear, c, c->S, 0))
     return -1;
   for (b = 0; b < BINS; b++) {
     double phi = (b + 0.5) / BINS * 2 * JS_PI;
@@ -377,6 +378,31 @@ int64_t or_engine_trace(const mc_cell *c, double ap, int64_t nsteps, double *Fx,
   if (trial_setup(e, c, ap)) { free(e); return -1; }
   total = (int64_t)e->spt * c->tool.N * c->revs;
   if (nsteps > total) nsteps = total;
+  eng_run(e, nsteps, NULL, Fx, Fy, x, y);
+  free(e->surf);
+  free(e);
+  return nsteps;
+}
+
+/* the engine of one simulator sub-step: configure with the tool's own flute length and
+ * runout (src/sim.js:252-257), setEngagement(u, fz, tipZ, ap) with the caller's map
+ * (src/sim.js:403-404), a velocity set from outside, then run step by step */
+int64_t or_engine_trace_ex(const mc_cell *c, const mc_engage_spec *s, int64_t nsteps, double *Fx,
+                           double *Fy, double *x, double *y) {
+  Eng *e;
+  if (!cell_ok(c) || !s || nsteps < 0) return -1;
+  e = (Eng *)malloc(sizeof(Eng));
+  if (!e) return -1;
+  if (eng_configure(e, &c->tool, c->tool.fluteLen, c->tool.runout, c->tool.runoutPhase,
+                    c->nonlinear, c, c->S, s->fnMax)) {
+    free(e);
+    return -1;
+  }
+  memcpy(e->Hb, s->Hb, sizeof(e->Hb));
+  eng_set_engagement(e, s->ux, s->uy, s->fz, s->tipZ, s->ap);
+  if (e->nz > MC_MAX_SLICES) { free(e->surf); free(e); return -1; }
+  e->vxs = s->vx0;
+  e->vys = s->vy0;
   eng_run(e, nsteps, NULL, Fx, Fy, x, y);
   free(e->surf);
   free(e);

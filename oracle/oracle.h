@@ -100,6 +100,9 @@ int or_immersion(const float *Hb, double tipZ, double *st, double *ex);
 int or_tip_lut(int shape, double D, double cornerR, const double *er, int en, double edz,
                float *out);
 
+int64_t or_engine_trace_ex(const mc_cell *c, const mc_engage_spec *s, int64_t nsteps, double *Fx,
+                           double *Fy, double *x, double *y);
+
 /* toolModel (src/tool.js:16-32) for a uniform carbide beam: k [N/mm], fn [Hz] */
 void or_tool_model(double D, double stickout, double kSpindle, double mHolder, double *k,
                    double *fn);

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
 
-from mcsim._abi import Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt  # noqa: E402
+from mcsim._abi import Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec  # noqa: E402
 
 _LIB = None
 
@@ -36,6 +36,8 @@ def lib():
                                        C.c_double, C.POINTER(Limit), C.POINTER(C.c_int32)]
         L.or_engine_trace.argtypes = [C.POINTER(Cell), C.c_double, C.c_int64, P, P, P, P]
         L.or_engine_trace.restype = C.c_int64
+        L.or_engine_trace_ex.argtypes = [C.POINTER(Cell), C.POINTER(EngageSpec), C.c_int64, P, P, P, P]
+        L.or_engine_trace_ex.restype = C.c_int64
         L.or_bisect_batch_mt.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), C.c_int,
                                          C.POINTER(Limit)]
         L.or_chatter_batch_mt.argtypes = [C.POINTER(Cell), P, C.c_int, C.c_int, C.POINTER(Trial)]
@@ -144,6 +146,15 @@ def engine_trace(cell, ap, nsteps):
     a = [np.zeros(nsteps) for _ in range(4)]
     n = lib().or_engine_trace(C.byref(cell), float(ap), int(nsteps),
                               *[x.ctypes.data for x in a])
+    return [x[:n] for x in a]
+
+
+def engine_trace_ex(cell, spec, nsteps):
+    """the engine of one simulator sub-step (src/sim.js:252-257,403-404) restated"""
+    import numpy as np
+    a = [np.zeros(max(1, nsteps)) for _ in range(4)]
+    n = lib().or_engine_trace_ex(C.byref(cell), C.byref(spec), int(nsteps), *[x.ctypes.data for x in a])
+    assert n == nsteps
     return [x[:n] for x in a]
 
 

@@ -18,6 +18,8 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_lobes.json     stabilityLobes / alimAt / directionalCoeffs
   golden_toolmodel.json toolModel (k, fn and the rest of its answer)
   golden_spindle.json   modesAt / tipFactor / bearingFactor / speedDependent
+  golden_enginex.json   per-step forces and displacements of the engine as the simulator
+                        drives it (feed direction, stock map, runout, fnMax)
 
 usage: python tests/golden/make_golden.py [--quick]
 """
@@ -236,6 +238,47 @@ def build_cases(quick):
         for soft in softs:
             sp.append(dict(modal=modal, chain=ch, soft=soft, speeds=speeds))
     cases["spindle"] = sp
+    # the engine as the simulator drives it: any feed direction, an engagement map from the
+    # stock, the tool's own flute length and runout, fnMax, a velocity set from outside
+    rng = W.ParkMiller(400)
+    eng = []
+    specs = [
+        dict(tool=dict(N=3, R=5.0, helixDeg=30.0, type="flat", fluteLen=25.0, runout=0.005, runoutPhase=0.3),
+             modes=[[2.0e4, 1800.0, 0.03]], modesY=None, S=9000.0, a=0.7, fz=0.08, tipZ=-2.0, ap=1.7,
+             v0=[0.002, 0.0], fnMax=0.0, nl=True, steps=3000, fill="mixed"),
+        dict(tool=dict(N=2, R=3.0, helixDeg=35.0, type="ball", fluteLen=12.0, runout=0.0, runoutPhase=0.0),
+             modes=[[1.5e4, 2400.0, 0.02], [4.0e4, 3900.0, 0.04]], modesY=[[1.8e4, 2600.0, 0.03]],
+             S=15000.0, a=JS_PI, fz=0.05, tipZ=-1.0, ap=1.2, v0=[0.0, 0.001], fnMax=5000.0, nl=False,
+             steps=4000, fill="mixed"),
+        dict(tool=dict(N=4, R=4.0, helixDeg=40.0, type="flat", fluteLen=20.0, runout=0.01, runoutPhase=1.1),
+             modes=[[3.0e4, 1500.0, 0.03], [6.0e4, 2800.0, 0.05]], modesY=None, S=4000.0, a=JS_PI / 2,
+             fz=0.05, tipZ=-3.5, ap=3.0, v0=[0.0, 0.0], fnMax=0.0, nl=True, steps=5000, fill="full"),
+        dict(tool=dict(N=3, R=6.0, helixDeg=30.0, type="flat", fluteLen=24.0, runout=0.003, runoutPhase=0.0),
+             modes=[[2.0e4, 1000.0, 0.03]], modesY=None, S=6000.0, a=-2.2, fz=0.1, tipZ=-1.5, ap=1.5,
+             v0=[0.01, -0.004], fnMax=0.0, nl=True, steps=3000, fill="none"),
+    ]
+    for sp_ in specs:
+        Hb = []
+        for b in range(180):
+            phi = (b + 0.5) / 180 * 2 * JS_PI
+            if math.sin(phi) <= 0.02:
+                Hb.append("-inf")
+            elif sp_["fill"] == "full":
+                Hb.append(0.0)
+            elif sp_["fill"] == "none":
+                Hb.append(sp_["tipZ"] - 0.5)
+            else:
+                u = rng.u()
+                Hb.append("-inf" if u < 0.1 else sp_["tipZ"] + rng.uniform(-0.3, sp_["ap"] + 0.5))
+        t = sp_["tool"]
+        mx = [dict(k=k, fn=fn, zeta=z) for k, fn, z in sp_["modes"]]
+        my = [dict(k=k, fn=fn, zeta=z) for k, fn, z in (sp_["modesY"] or sp_["modes"])]
+        modal = dict(k=mx[0]["k"], fn=mx[0]["fn"], zeta=mx[0]["zeta"], modesX=mx, modesY=my)
+        mat = dict(S50C, mc=0.10, Ci=2.0e4)
+        eng.append(dict(tool=t, modal=modal, mat=mat, S=sp_["S"], u=[math.cos(sp_["a"]), math.sin(sp_["a"])],
+                        fz=sp_["fz"], tipZ=sp_["tipZ"], ap=sp_["ap"], v0=sp_["v0"], fnMax=sp_["fnMax"],
+                        nonlinear=sp_["nl"], steps=sp_["steps"], Hb=Hb))
+    cases["enginex"] = eng
     return cases
 
 
@@ -261,7 +304,8 @@ def main():
         json.dump(cases, f)
     run(cin, cout)
     out = json.load(open(cout))
-    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle"):
+    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
+                "enginex"):
         rec = dict(cases=cases[key], results=out[key])
         if key + "_meta" in cases:
             rec["meta"] = cases[key + "_meta"]

@@ -194,6 +194,26 @@ if (cases.sim) {
   });
 }
 
+// the engine as Simulator.ensureEngine / step configure it (src/sim.js:252-257,403-404)
+if (cases.enginex) {
+  out.enginex = cases.enginex.map(c => {
+    const e = new CE();
+    e.configure({ N: c.tool.N, R: c.tool.R, helixDeg: c.tool.helixDeg, type: c.tool.type, fluteLen: c.tool.fluteLen,
+      nonlinear: c.nonlinear, runout: c.tool.runout, runoutPhase: c.tool.runoutPhase }, c.modal, c.mat, c.S, c.fnMax);
+    e.setEngagement(c.u, c.fz, c.tipZ, c.ap);
+    e.Hb.set(c.Hb.map(v => (v === '-inf' ? -Infinity : v)));
+    e.vx = c.v0[0]; e.vy = c.v0[1];
+    const Fx = [], Fy = [], x = [], y = [];
+    e.takeStats();
+    for (let s = 0; s < c.steps; s++) {
+      e.run(1);
+      const st = e.takeStats();
+      Fx.push(st.sumFx); Fy.push(st.sumFy); x.push(e.x); y.push(e.y);
+    }
+    return { Fx: b64f64(Fx), Fy: b64f64(Fy), x: b64f64(x), y: b64f64(y), nz: e.nz, spt: e.stepsPerTooth };
+  });
+}
+
 if (cases.tiplut) out.tiplut = cases.tiplut.map(t => b64f32(tipLut(t)));
 
 if (cases.toolModel) out.toolModel = cases.toolModel.map(t => { const m = toolModel(t); return { k: m.k, fn: m.fn, De: m.De, I: m.I, kTool: m.kTool, mEff: m.mEff }; });

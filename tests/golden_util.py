@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "machinig-sim_amd"))
 
-from mcsim._abi import make_cell, Stamp, StampOpt, MC_TOOL_BALL, MC_TOOL_FLAT  # noqa: E402
+from mcsim._abi import make_cell, Stamp, StampOpt, EngageSpec, MC_TOOL_BALL, MC_TOOL_FLAT  # noqa: E402
 from mcsim._abi import MC_TIP_FLAT, MC_TIP_BALL, MC_TIP_BULL, MC_TIP_EDGE  # noqa: E402
 
 
@@ -165,3 +165,19 @@ def check_maps(stock_maps, ref, bits_of):
         assert np.array_equal(bits_of(stock_maps[key]), bits_of(want)), key
     assert np.array_equal(stock_maps["lineCut"], zi32(ref["lineCut"]))
     assert np.array_equal(stock_maps["lineWall"], zi32(ref["lineWall"]))
+
+
+def enginex_case(c):
+    """(Cell, EngageSpec) of a golden simulator-engine case"""
+    md = c["modal"]
+    mx = [(m["k"], m["fn"], m["zeta"]) for m in md["modesX"]]
+    my = [(m["k"], m["fn"], m["zeta"]) for m in md["modesY"]]
+    cell = make_cell(c["tool"], mx, my, c["mat"], c["S"], (0.0, 3.141592653589793), c["fz"], 1, c["nonlinear"])
+    sp = EngageSpec()
+    sp.ux, sp.uy = c["u"]
+    sp.fz, sp.tipZ, sp.ap = c["fz"], c["tipZ"], c["ap"]
+    sp.vx0, sp.vy0 = c["v0"]
+    sp.fnMax = c["fnMax"]
+    for b, v in enumerate(c["Hb"]):
+        sp.Hb[b] = num(v)
+    return cell, sp

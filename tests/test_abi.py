@@ -37,6 +37,7 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(mc_stamp_opt), sizeof(mc_engage_query), sizeof(mc_engage_result),
          sizeof(mc_train_opts), sizeof(mc_lobe_args), sizeof(mc_search_tuning));
   printf("%zu %zu\n", offsetof(mc_stamp_opt, lut), offsetof(mc_engage_result, imm));
+  printf("%zu %zu\n", sizeof(mc_engage_spec), offsetof(mc_engage_spec, Hb));
   return 0;
 }'''
     import tempfile
@@ -51,4 +52,5 @@ int main(void) {
     want += [C.sizeof(t) for t in (_abi.StampOpt, _abi.EngageQuery, _abi.EngageResult, _abi.TrainOpts,
                                    _abi.LobeArgs, _abi.SearchTuning)]
     want += [_abi.StampOpt.lut.offset, _abi.EngageResult.imm.offset]
+    want += [C.sizeof(_abi.EngageSpec), _abi.EngageSpec.Hb.offset]
     assert [int(x) for x in out] == want

@@ -62,6 +62,24 @@ def test_engine_trace_bit_exact(mc, orc):
             assert np.allclose(fx, G.f64(ref["Fx"]), rtol=1e-9, atol=1e-9)
 
 
+def test_simulator_engine_traces(mc, orc):
+    """The engine as the simulator drives it (feed direction, stock map, runout, fnMax,
+    velocity set from outside) against the reference's traces and the oracle"""
+    g = G.load("enginex")
+    for case, ref in zip(g["cases"], g["results"]):
+        cell, spec = G.enginex_case(case)
+        got = mc.engine_trace_ex(cell, spec, case["steps"])
+        want = orc.engine_trace_ex(cell, spec, case["steps"])
+        for a, b in zip(got, want):          # the oracle: bit for bit
+            assert np.array_equal(a, b)
+        for a, key in zip(got, ("Fx", "Fy", "x", "y")):
+            r = G.f64(ref[key])
+            if not case["nonlinear"]:
+                assert np.array_equal(a, r), key
+            else:   # size effect: pow's last bits are not pinned to V8's
+                assert np.allclose(a, r, rtol=1e-9, atol=1e-12 * np.max(np.abs(r))), key
+
+
 def test_grid_subset_bisection_matches_oracle(mc, orc):
     from mcsim import workloads as W
     cells, opts = W.sld_grid(n_speeds=8, n_tools=6, seed=3)

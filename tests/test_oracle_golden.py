@@ -246,3 +246,20 @@ def test_simulator_run_replayed_bit_exact():
             G.check_maps(maps, ref, bits)
         assert np.array_equal(vol, vol_ref) and np.array_equal(ivol, ivol_ref)
         assert np.count_nonzero(vol) > 100 and len(r["engage"]) > 500
+
+
+def test_simulator_engine_traces():
+    """The engine as the simulator drives it (any feed direction, a stock map, runout, fnMax,
+    a velocity set from outside): per-step forces and displacements"""
+    g = G.load("enginex")
+    for case, ref in zip(g["cases"], g["results"]):
+        cell, spec = G.enginex_case(case)
+        assert cell.tool.runout == case["tool"]["runout"]
+        fx, fy, x, y = oracle.engine_trace_ex(cell, spec, case["steps"])
+        for a, key in ((fx, "Fx"), (fy, "Fy"), (x, "x"), (y, "y")):
+            b = G.f64(ref[key])
+            if not case["nonlinear"]:
+                assert np.array_equal(a, b), key
+            else:   # size effect: pow's last bits (see test_oracle_math)
+                assert np.allclose(a, b, rtol=1e-9, atol=1e-12 * np.max(np.abs(b))), key
+        assert np.any(fx != 0) or case["Hb"][45] != "-inf"
