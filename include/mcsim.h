@@ -195,6 +195,92 @@ typedef struct {
 int mc_engine_trace_ex(const mc_cell* cell, const mc_engage_spec* eng, int64_t nsteps,
                        double* Fx, double* Fy, double* x, double* y, int64_t* steps_out);
 
+/* ---- A device-resident CuttingEngine (src/cutting.js:21-320) ----
+ *
+ * The engine object the simulator keeps across sub-steps: its modal state, delay line
+ * (surf), wall mode, per-tooth chip volumes, statistics and sample rings live in device
+ * memory between calls, so a sequence of configure / setWear / setWork / setEngagement /
+ * setEndEngagement / run calls gives the reference's doubles after every run.  Each
+ * handle is independent; mc_engine_run_batch steps many engines in one launch. */
+#define MC_END_BINS    36    /* angle bins of the end-face map (src/cutting.js:16) */
+#define MC_END_RINGS    6    /* radial rings of the end-face map (src/cutting.js:17) */
+#define MC_ENGINE_HIST 4096  /* decimated history ring (src/cutting.js:26) */
+#define MC_ENGINE_RING 64    /* once-per-tooth and wall sample rings (src/cutting.js:27-31) */
+
+typedef struct mc_engine mc_engine;
+
+/* what configure derives (src/cutting.js:45-80), for the caller's reads */
+typedef struct {
+  double dt, omega, Ttooth, dz, S;
+  int32_t stepsPerTooth, stepsPerRev, decim, piStep, maxSlices, nmx, nmy, pad;
+} mc_engine_info;
+
+/* setWork(wall): a 1-DOF workpiece mode along the wall normal (src/cutting.js:88-96) */
+typedef struct { double k, m, zeta, nx, ny; } mc_wall;
+
+/* the engine's public scalar fields.  mc_engine_set_state writes x, y, vx, vy, w, wv, t,
+ * decim and hist_i (what callers assign: src/stabtd.js:19, src/sim.js:206,427-428,
+ * src/validate.js:81-83); the rest are read-only here */
+typedef struct {
+  double x, y, vx, vy;     /* tool deflection [m] and velocity [m/s] */
+  double w, wv;            /* wall deflection [m] and velocity [m/s] */
+  double t;                /* engine time [s] */
+  double runMeanW;         /* mean wall deflection over the last run [m] */
+  double wk, wnx, wny;     /* wall stiffness [N/m] (0: rigid) and normal */
+  int64_t stepCount;
+  int32_t pr, decim, nz, pad;
+  int64_t hist_i, hist_count, ts_i, ts_count, ws_i, ws_count, wsp_i, wsp_count;
+} mc_engine_state;
+
+/* the stats object (newStats, src/cutting.js:323-326), in its field order */
+typedef struct {
+  double n, sumF, maxF, sumFx, sumFy, sumFz, maxFz, sumT, maxT, sumX, sumY;
+  double minX, maxX, minY, maxY, sumZc, wZc, sumW, maxW, sumHb, sumB, sumRb, engTeeth;
+} mc_engine_stats;
+
+/* a chip event: pushed when a tooth leaves the cut with volume (src/cutting.js:238-241) */
+typedef struct { double vol, phi, z, ux, uy, vx, vy; } mc_chip_event;
+
+/* new CuttingEngine() (src/cutting.js:22-35) */
+int mc_engine_create(mc_engine** out);
+int mc_engine_destroy(mc_engine* e);
+/* configure(tool, modal, mat, S, fnMax) (src/cutting.js:39-82): cell->tool (N, type, R,
+ * helixDeg, fluteLen, runout [mm], runoutPhase), the mode lists, cell->mat, cell->S and
+ * cell->nonlinear; the cell's arc, fz and revs are not used.  info may be NULL. */
+int mc_engine_configure(mc_engine* e, const mc_cell* cell, double fnMax, mc_engine_info* info);
+/* setWear({Kte, Kre}) (src/cutting.js:85): edge coefficients used instead of the
+ * material's from now on */
+int mc_engine_set_wear(mc_engine* e, double Kte, double Kre);
+/* setWork(wall) (src/cutting.js:88-96); NULL is setWork(null), a rigid workpiece */
+int mc_engine_set_work(mc_engine* e, const mc_wall* wall);
+/* setEngagement([ux, uy], fz, tipZ, ap) (src/cutting.js:98-124).  sliceR / sliceKap: the
+ * radius and edge factor of each slice from the tool's own profile (tool.rAt), at least nz
+ * entries each; both NULL: the flat / ball rule.  *nz (may be NULL) receives the slices. */
+int mc_engine_set_engagement(mc_engine* e, double ux, double uy, double fz, double tipZ, double ap,
+                             const double* sliceR, const double* sliceKap, int32_t* nz);
+/* setEndEngagement(fza, rOut, zAt) (src/cutting.js:130-143).  rings: ringR, ringCos, ringZ,
+ * ringDs (MC_END_RINGS each, in that order) from the caller's zAt; NULL: zAt = () => 0. */
+int mc_engine_set_end_engagement(mc_engine* e, double fza, double rOut, const double* rings);
+/* the caller-filled maps engine.Hb (MC_BINS) and engine.Eb (MC_END_BINS x MC_END_RINGS,
+ * [bin][ring]); either may be NULL to leave it as it is */
+int mc_engine_set_maps(mc_engine* e, const float* Hb, const float* Eb);
+int mc_engine_get_state(const mc_engine* e, mc_engine_state* s);
+int mc_engine_set_state(mc_engine* e, const mc_engine_state* s);
+/* run(nSteps) (src/cutting.js:146-296); setEngagement must have followed configure */
+int mc_engine_run(mc_engine* e, int64_t nSteps);
+/* n engines, engine i stepping nSteps[i], in one launch; the handles must be distinct */
+int mc_engine_run_batch(mc_engine* const* e, const int64_t* nSteps, int n);
+/* the stats accumulated since the last take; take != 0 is takeStats() (src/cutting.js:320) */
+int mc_engine_read_stats(mc_engine* e, mc_engine_stats* out, int take);
+/* chip events pushed by the last run, in order: *n receives their number; at most cap are
+ * copied */
+int mc_engine_chip_events(const mc_engine* e, mc_chip_event* out, int64_t cap, int64_t* n);
+/* chatterIndex() (src/cutting.js:299-318): out[0] index, out[1] amp, out[2] once */
+int mc_engine_chatter_index(mc_engine* e, double out[3]);
+/* the sample rings: hist (5 x MC_ENGINE_HIST: Fx, Fy, Fz, x, y), toothSamples, wallSamples
+ * and wallSamplesPi (2 x MC_ENGINE_RING each: x, y); any may be NULL */
+int mc_engine_read_rings(const mc_engine* e, float* hist, float* ts, float* ws, float* wsp);
+
 /* ---- Path B: heightmap stock (src/stock.js) ---- */
 
 typedef struct mc_stock mc_stock;   /* device-resident heightfield + side maps */

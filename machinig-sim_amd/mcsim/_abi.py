@@ -78,6 +78,43 @@ class EngageResult(C.Structure):
                 ("phiEx", C.c_double), ("imm", C.c_int32), ("pad", C.c_int32)]
 
 
+MC_END_BINS = 36
+MC_END_RINGS = 6
+MC_ENGINE_HIST = 4096
+MC_ENGINE_RING = 64
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dt", "omega", "Ttooth", "dz", "S")] + \
+               [(n, C.c_int32) for n in ("stepsPerTooth", "stepsPerRev", "decim", "piStep", "maxSlices",
+                                         "nmx", "nmy", "pad")]
+
+
+class Wall(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("k", "m", "zeta", "nx", "ny")]
+
+
+class EngineState(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "y", "vx", "vy", "w", "wv", "t", "runMeanW", "wk", "wnx",
+                                          "wny")] + \
+               [("stepCount", C.c_int64)] + [(n, C.c_int32) for n in ("pr", "decim", "nz", "pad")] + \
+               [(n, C.c_int64) for n in ("hist_i", "hist_count", "ts_i", "ts_count", "ws_i", "ws_count",
+                                         "wsp_i", "wsp_count")]
+
+
+STATS_FIELDS = ("n", "sumF", "maxF", "sumFx", "sumFy", "sumFz", "maxFz", "sumT", "maxT", "sumX", "sumY",
+                "minX", "maxX", "minY", "maxY", "sumZc", "wZc", "sumW", "maxW", "sumHb", "sumB", "sumRb",
+                "engTeeth")
+
+
+class EngineStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in STATS_FIELDS]
+
+
+class ChipEvent(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("vol", "phi", "z", "ux", "uy", "vx", "vy")]
+
+
 MC_STOCK_B = 8
 MC_LUT_DR = 0.02
 MC_TIP_FLAT, MC_TIP_BALL, MC_TIP_BULL, MC_TIP_EDGE = 0, 1, 2, 3

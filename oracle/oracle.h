@@ -50,6 +50,25 @@ int or_bisect_batch_mt(const mc_cell *cells, int n, const mc_bisect_opts *o, int
 int or_chatter_batch_mt(const mc_cell *cells, const double *ap, int n, int nthreads,
                         mc_trial *out);
 
+/* cengine.c: the CuttingEngine object kept across calls (src/cutting.js:21-326) */
+typedef struct or_engine or_engine;
+or_engine *or_engine_create(void);
+void or_engine_destroy(or_engine *e);
+int or_engine_configure(or_engine *e, const mc_cell *c, double fnMax, mc_engine_info *info);
+void or_engine_set_wear(or_engine *e, double Kte, double Kre);
+void or_engine_set_work(or_engine *e, const mc_wall *w);
+int or_engine_set_engagement(or_engine *e, double ux, double uy, double fz, double tipZ, double ap,
+                             const double *sliceR, const double *sliceKap, int32_t *nz_out);
+void or_engine_set_end_engagement(or_engine *e, double fza, double rOut, const double *rings);
+void or_engine_set_maps(or_engine *e, const float *Hb, const float *Eb);
+void or_engine_get_state(const or_engine *e, mc_engine_state *s);
+void or_engine_set_state(or_engine *e, const mc_engine_state *s);
+int or_engine_run(or_engine *e, int64_t nSteps);
+void or_engine_stats(or_engine *e, mc_engine_stats *out, int take);
+int64_t or_engine_chip_events(const or_engine *e, mc_chip_event *out, int64_t cap);
+void or_engine_chatter_index(const or_engine *e, double out[3]);
+void or_engine_read_rings(const or_engine *e, float *hist, float *ts, float *ws, float *wsp);
+
 /* sld.c: zero-order lobes (src/sld.js) */
 void or_directional_coeffs(double Kr, double phiSt, double phiEx, double out[4]);
 /* writes up to max_pts (rpm, alim) pairs, lobe j's points contiguous; lobe_len[j] =

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
 
-from mcsim._abi import Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec  # noqa: E402
+from mcsim._abi import (Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec,  # noqa: E402
+                        EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST, MC_ENGINE_RING)
 
 _LIB = None
 
@@ -41,6 +42,22 @@ def lib():
         L.or_bisect_batch_mt.argtypes = [C.POINTER(Cell), C.c_int, C.POINTER(BisectOpts), C.c_int,
                                          C.POINTER(Limit)]
         L.or_chatter_batch_mt.argtypes = [C.POINTER(Cell), P, C.c_int, C.c_int, C.POINTER(Trial)]
+        L.or_engine_create.restype = P
+        L.or_engine_destroy.argtypes = [P]
+        L.or_engine_configure.argtypes = [P, C.POINTER(Cell), C.c_double, C.POINTER(EngineInfo)]
+        L.or_engine_set_wear.argtypes = [P, C.c_double, C.c_double]
+        L.or_engine_set_work.argtypes = [P, P]
+        L.or_engine_set_engagement.argtypes = [P] + [C.c_double] * 5 + [P, P, P]
+        L.or_engine_set_end_engagement.argtypes = [P, C.c_double, C.c_double, P]
+        L.or_engine_set_maps.argtypes = [P, P, P]
+        L.or_engine_get_state.argtypes = [P, C.POINTER(EngineState)]
+        L.or_engine_set_state.argtypes = [P, C.POINTER(EngineState)]
+        L.or_engine_run.argtypes = [P, C.c_int64]
+        L.or_engine_stats.argtypes = [P, C.POINTER(EngineStats), C.c_int]
+        L.or_engine_chip_events.argtypes = [P, P, C.c_int64]
+        L.or_engine_chip_events.restype = C.c_int64
+        L.or_engine_chatter_index.argtypes = [P, P]
+        L.or_engine_read_rings.argtypes = [P, P, P, P, P]
         L.or_directional_coeffs.argtypes = [C.c_double, C.c_double, C.c_double, P]
         L.or_stability_lobes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_int, C.c_double, C.c_double, C.POINTER(Mode), C.c_int,
@@ -156,6 +173,98 @@ def engine_trace_ex(cell, spec, nsteps):
     n = lib().or_engine_trace_ex(C.byref(cell), C.byref(spec), int(nsteps), *[x.ctypes.data for x in a])
     assert n == nsteps
     return [x[:n] for x in a]
+
+
+class OracleEngine:
+    """the CuttingEngine object restated (oracle/cengine.c), with the same methods as the
+    device engine's binding so one replay drives either"""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.or_engine_create()
+        assert self.h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.or_engine_destroy(self.h)
+            self.h = None
+
+    def configure(self, cell, fnMax=0.0):
+        info = EngineInfo()
+        st = self.L.or_engine_configure(self.h, C.byref(cell), float(fnMax), C.byref(info))
+        if st:
+            raise RuntimeError("or_engine_configure: %d" % st)
+        return info
+
+    def set_wear(self, Kte, Kre):
+        self.L.or_engine_set_wear(self.h, float(Kte), float(Kre))
+
+    def set_work(self, wall):
+        self.L.or_engine_set_work(self.h, C.byref(wall) if wall is not None else None)
+
+    def set_engagement(self, ux, uy, fz, tipZ, ap, sliceR=None, sliceKap=None):
+        import numpy as np
+        nz = C.c_int32()
+        r = np.ascontiguousarray(sliceR, dtype=np.float64) if sliceR is not None else None
+        k = np.ascontiguousarray(sliceKap, dtype=np.float64) if sliceKap is not None else None
+        st = self.L.or_engine_set_engagement(self.h, ux, uy, fz, tipZ, ap, r.ctypes.data if r is not None else None,
+                                             k.ctypes.data if k is not None else None, C.byref(nz))
+        if st:
+            raise RuntimeError("or_engine_set_engagement: %d" % st)
+        return nz.value
+
+    def set_end_engagement(self, fza, rOut, rings=None):
+        import numpy as np
+        g = np.ascontiguousarray(rings, dtype=np.float64) if rings is not None else None
+        self.L.or_engine_set_end_engagement(self.h, float(fza), float(rOut), g.ctypes.data if g is not None else None)
+
+    def set_maps(self, Hb=None, Eb=None):
+        import numpy as np
+        hb = np.ascontiguousarray(Hb, dtype=np.float32) if Hb is not None else None
+        eb = np.ascontiguousarray(Eb, dtype=np.float32) if Eb is not None else None
+        self.L.or_engine_set_maps(self.h, hb.ctypes.data if hb is not None else None,
+                                  eb.ctypes.data if eb is not None else None)
+
+    def state(self):
+        s = EngineState()
+        self.L.or_engine_get_state(self.h, C.byref(s))
+        return s
+
+    def set_state(self, **kw):
+        s = self.state()
+        for k, v in kw.items():
+            setattr(s, k, v)
+        self.L.or_engine_set_state(self.h, C.byref(s))
+
+    def run(self, n):
+        st = self.L.or_engine_run(self.h, int(n))
+        if st:
+            raise RuntimeError("or_engine_run: %d" % st)
+
+    def stats(self, take=False):
+        s = EngineStats()
+        self.L.or_engine_stats(self.h, C.byref(s), 1 if take else 0)
+        return s
+
+    def chip_events(self):
+        import numpy as np
+        n = self.L.or_engine_chip_events(self.h, None, 0)
+        out = (ChipEvent * max(1, n))()
+        self.L.or_engine_chip_events(self.h, out, n)
+        return np.array([[e.vol, e.phi, e.z, e.ux, e.uy, e.vx, e.vy] for e in out[:n]]).reshape(n, 7)
+
+    def chatter_index(self):
+        import numpy as np
+        out = np.zeros(3)
+        self.L.or_engine_chatter_index(self.h, out.ctypes.data)
+        return out
+
+    def rings(self):
+        import numpy as np
+        h = np.zeros(5 * MC_ENGINE_HIST, np.float32)
+        r = [np.zeros(2 * MC_ENGINE_RING, np.float32) for _ in range(3)]
+        self.L.or_engine_read_rings(self.h, h.ctypes.data, *[x.ctypes.data for x in r])
+        return h, r[0], r[1], r[2]
 
 
 class OracleStock:

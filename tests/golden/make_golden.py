@@ -20,8 +20,12 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_spindle.json   modesAt / tipFactor / bearingFactor / speedDependent
   golden_enginex.json   per-step forces and displacements of the engine as the simulator
                         drives it (feed direction, stock map, runout, fnMax)
+  golden_englog.json    the engine's whole call sequence inside simulator runs (configure,
+                        setWear, setWork, setEngagement, setEndEngagement, run, takeStats,
+                        chatterIndex) with its state, stats, rings and chip events after
+                        every run
 
-usage: python tests/golden/make_golden.py [--quick]
+usage: python tests/golden/make_golden.py [--quick] [--only key,key,...]
 """
 import json
 import math
@@ -191,6 +195,17 @@ def build_cases(quick):
     sim_tools = [dict(type="flat", D=10.0), dict(type="ball", D=6.0), dict(type="bull", D=8.0, cornerR=1.5)]
     cases["sim"] = [dict(program="\n".join(p), tool=t, stock=dict(type="box", w=30.0, l=20.0, h=8.0),
                          cell=0.25) for p, t in zip(progs, sim_tools)]
+    # the engine's call sequence in simulator runs: the programs above, and one that leaves a
+    # thin rib and finishes its sides, so the workpiece mode (setWork) is active
+    rib = ["G90 G54 G17", "S9000 M3", "G0 X-5 Y4 Z5", "G1 Z-4 F300", "G1 X35 F800", "G0 Z5", "G0 X-5 Y16",
+           "G1 Z-4 F300", "G1 X35 F800", "G1 Y14.6", "G1 X-5", "G0 Z5", "G0 X-5 Y5.4", "G1 Z-4 F300",
+           "G1 X35 F700", "G0 Z5", "M30"]
+    # (the linear force law first: bit-exact; then the size effect and process damping, whose
+    # Math.pow differs from the restated fdlibm pow in the last bit on a few % of inputs)
+    ribc = dict(program="\n".join(rib), tool=dict(type="flat", D=6.0), stock=dict(type="box", w=30.0, l=20.0, h=8.0),
+                cell=0.25, flex=True)
+    lin = [dict(c, nonlinear=False) for c in cases["sim"] + [ribc]]
+    cases["englog"] = lin + [dict(cases["sim"][2]), dict(ribc)]
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]
@@ -300,12 +315,17 @@ def main():
             for m in md["modesX"] + md["modesY"]:
                 m["k"], m["fn"] = tm["k"], tm["fn"]
             md["k"], md["fn"] = tm["k"], tm["fn"]
+    keys = ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
+            "enginex", "englog")
+    if "--only" in sys.argv:
+        keys = tuple(sys.argv[sys.argv.index("--only") + 1].split(","))
+        # engage cases read the stock cases
+        cases = {k: v for k, v in cases.items() if k in keys or (k == "stock" and "engage" in keys)}
     with open(cin, "w") as f:
         json.dump(cases, f)
     run(cin, cout)
     out = json.load(open(cout))
-    for key in ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
-                "enginex"):
+    for key in keys:
         rec = dict(cases=cases[key], results=out[key])
         if key + "_meta" in cases:
             rec["meta"] = cases[key + "_meta"]

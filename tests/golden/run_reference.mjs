@@ -194,6 +194,78 @@ if (cases.sim) {
   });
 }
 
+// the engine's own call sequence inside whole simulator runs: every configure, setWear,
+// setWork, setEngagement, setEndEngagement, run, takeStats and chatterIndex the simulator
+// makes, with the maps and the state it set before each run and the engine's state, stats,
+// rings and chip events after it
+if (cases.englog) {
+  const { Simulator } = await import(REF + 'sim.js');
+  const { modeSet } = await import(REF + 'frf.js');
+  const ST = ['n', 'sumF', 'maxF', 'sumFx', 'sumFy', 'sumFz', 'maxFz', 'sumT', 'maxT', 'sumX', 'sumY', 'minX',
+    'maxX', 'minY', 'maxY', 'sumZc', 'wZc', 'sumW', 'maxW', 'sumHb', 'sumB', 'sumRb', 'engTeeth'];
+  const rings = e => [e.hist.i, e.hist.count, e.toothSamples.i, e.toothSamples.count, e.wallSamples.i,
+    e.wallSamples.count, e.wallSamplesPi.i, e.wallSamplesPi.count];
+  out.englog = cases.englog.map(({ program, tool, stock, cell, flex, nonlinear }) => {
+    const sim = new Simulator();
+    sim.cfg.cell = cell;
+    if (flex !== undefined) sim.cfg.workpieceFlex = flex;
+    if (nonlinear !== undefined) sim.cfg.cuttingNonlinear = nonlinear;
+    Object.assign(sim.cfg.tool, tool);
+    sim.load(program, { stock, origin: { x: 0, y: 0, z: 0 } });
+    const e = sim.engine;
+    const ops = [], Hb = [], Eb = [], pre = [], post = [], ev = [], evn = [], takes = [], cis = [], sl = [], rg = [];
+    let inCfg = false;
+    const f = {};
+    for (const m of ['configure', 'setWear', 'setWork', 'setEngagement', 'setEndEngagement', 'run', 'takeStats', 'chatterIndex']) f[m] = e[m].bind(e);
+    e.configure = (t, modal, mat, S, fnMax = 0) => {
+      inCfg = true;
+      f.configure(t, modal, mat, S, fnMax);
+      inCfg = false;
+      const ms = modeSet(modal), md = L => L.map(m => [m.k, m.fn, m.zeta]);
+      ops.push(['cfg', t.N, t.R, t.helixDeg, t.type, t.fluteLen, t.nonlinear === false ? 0 : 1, t.runout || 0,
+        t.runoutPhase || 0, S, fnMax, md(ms.x), md(ms.y),
+        [mat.Ktc, mat.Krc, mat.Kte, mat.Kre, mat.Kac, mat.Kae, mat.mc != null ? mat.mc : 0, mat.Ci != null ? mat.Ci : 0],
+        [e.stepsPerTooth, e.stepsPerRev, e.decim, e.piStep, e.maxSlices, e.dz, e.dt, e.omega]]);
+    };
+    e.setWear = edge => { f.setWear(edge); ops.push(['wear', edge.Kte, edge.Kre]); };
+    // configure's own setWork(null) is part of configure
+    e.setWork = w => { f.setWork(w); if (!inCfg) ops.push(w ? ['work', w.k, w.m, w.zeta, w.nx, w.ny] : ['work']); };
+    e.setEngagement = (u, fz, tipZ, ap) => {
+      f.setEngagement(u, fz, tipZ, ap);
+      ops.push(['eng', u[0], u[1], fz, tipZ, ap, e.tool.rAt ? 1 : 0, e.nz]);
+      sl.push(...e.sliceR, ...e.sliceKap, ...e.sliceZ, ...e.sliceB, ...e.sliceLag);
+    };
+    e.setEndEngagement = (fza, rOut, zAt) => {
+      f.setEndEngagement(fza, rOut, zAt);
+      ops.push(['end', fza, rOut]);
+      rg.push(...e.ringR, ...e.ringCos, ...e.ringZ, ...e.ringDs);
+    };
+    e.run = n => {
+      Hb.push(...e.Hb); Eb.push(...e.Eb);
+      pre.push(e.x, e.y, e.vx, e.vy, e.w, e.wv);
+      const before = e.chipEvents.length;
+      f.run(n);
+      const st = e.stats;
+      post.push(e.x, e.y, e.vx, e.vy, e.w, e.wv, e.t, e.pr, e.runMeanW, e.stepCount, e.nz, ...rings(e), ...ST.map(k => st[k]));
+      const nev = e.chipEvents.length - before;
+      evn.push(nev);
+      for (let q = before; q < e.chipEvents.length; q++) { const c = e.chipEvents[q]; ev.push(c.vol, c.phi, c.z); }
+      ops.push(['run', n]);
+    };
+    e.takeStats = () => { const s = f.takeStats(); takes.push(...ST.map(k => s[k])); ops.push(['take']); return s; };
+    e.chatterIndex = () => { const c = f.chatterIndex(); cis.push(c.index, c.amp, c.once === undefined ? 0 : c.once); ops.push(['ci']); return c; };
+    let frames = 0;
+    while (!sim.done && frames < 100) { sim.step(1e6); frames++; }
+    const R = (b => ({ x: b64f32(b.x), y: b64f32(b.y) }));
+    return { ops, Hb: z32(Float32Array, Hb), Eb: z32(Float32Array, Eb), pre: z32(Float64Array, pre),
+      post: z32(Float64Array, post), ev: z32(Float64Array, ev), evn, takes: z32(Float64Array, takes),
+      cis: b64f64(cis), sl: z32(Float64Array, sl), rg: z32(Float64Array, rg), nst: ST.length,
+      final: { hist: z32(Float32Array, [...e.hist.Fx, ...e.hist.Fy, ...e.hist.Fz, ...e.hist.x, ...e.hist.y]),
+        ts: R(e.toothSamples), ws: R(e.wallSamples), wsp: R(e.wallSamplesPi) } };
+  });
+  console.error('englog', (Date.now() - t0) / 1e3, 's');
+}
+
 // the engine as Simulator.ensureEngine / step configure it (src/sim.js:252-257,403-404)
 if (cases.enginex) {
   out.enginex = cases.enginex.map(c => {

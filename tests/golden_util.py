@@ -181,3 +181,125 @@ def enginex_case(c):
     for b, v in enumerate(c["Hb"]):
         sp.Hb[b] = num(v)
     return cell, sp
+
+
+def zf64(s):
+    import zlib
+    return np.frombuffer(zlib.decompress(base64.b64decode(s)), dtype="<f8")
+
+
+ENGLOG_POST = ("x", "y", "vx", "vy", "w", "wv", "t", "pr", "runMeanW", "stepCount", "nz", "hist_i",
+               "hist_count", "ts_i", "ts_count", "ws_i", "ws_count", "wsp_i", "wsp_count")
+MAT_ORDER = ("Ktc", "Krc", "Kte", "Kre", "Kac", "Kae", "mc", "Ci")
+
+
+def _same(a, b, rtol=0.0):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.shape != b.shape:
+        return False
+    if rtol == 0.0:
+        return np.array_equal(a, b, equal_nan=True)
+    fin = np.isfinite(b)
+    if not np.array_equal(a[~fin], b[~fin], equal_nan=True):
+        return False
+    scale = np.max(np.abs(b[fin])) if np.any(fin) else 0.0
+    return bool(np.allclose(a[fin], b[fin], rtol=rtol, atol=rtol * scale))
+
+
+ENGLOG_EXACT = ("pr", "stepCount", "nz", "hist_i", "hist_count", "ts_i", "ts_count", "ws_i", "ws_count", "wsp_i",
+                "wsp_count")
+
+
+def replay_englog(eng, res, max_runs=None, rtol=0.0):
+    """Drive an engine (the oracle's OracleEngine or the library's Engine: same methods)
+    through a call sequence recorded from the reference simulator (golden_englog.json) and
+    compare, bit for bit, the engine's state, stats and chip events after every run, every
+    takeStats and chatterIndex answer, and the sample rings at the end.  Returns the list of
+    mismatches (empty when the engine reproduces the reference) and the number of runs."""
+    from mcsim._abi import Wall
+    nst = res["nst"]
+    Hb = zf32(res["Hb"]).reshape(-1, 180)
+    Eb = zf32(res["Eb"]).reshape(-1, 216)
+    pre = zf64(res["pre"]).reshape(-1, 6)
+    post = zf64(res["post"]).reshape(-1, len(ENGLOG_POST) + nst)
+    ev = zf64(res["ev"]).reshape(-1, 3)
+    evoff = np.concatenate([[0], np.cumsum(res["evn"])]).astype(int)
+    takes = zf64(res["takes"]).reshape(-1, nst)
+    cis = f64(res["cis"]).reshape(-1, 3)
+    sl = zf64(res["sl"])
+    rg = zf64(res["rg"]).reshape(-1, 24)
+    bad = []
+    r = ie = ir = it = ic = 0
+    flat = True
+    iex = [ENGLOG_POST.index(k) for k in ENGLOG_EXACT]
+    for op in res["ops"]:
+        kind = op[0]
+        if kind == "cfg":
+            _, N, R, hd, typ, fl, nl, ro, rph, S, fnMax, mxs, mys, mat, derived = op
+            flat = typ == "flat"
+            cell = make_cell(dict(N=N, R=R, helixDeg=hd, type=typ, fluteLen=fl, runout=ro, runoutPhase=rph),
+                             mxs, mys, dict(zip(MAT_ORDER, mat)), S, (0.0, 0.0), 0.0, 1, nl)
+            info = eng.configure(cell, fnMax)
+            got = [info.stepsPerTooth, info.stepsPerRev, info.decim, info.piStep, info.maxSlices, info.dz,
+                   info.dt, info.omega]
+            if not _same(got, derived):
+                bad.append(("configure", got, derived))
+        elif kind == "wear":
+            eng.set_wear(op[1], op[2])
+        elif kind == "work":
+            eng.set_work(Wall(*op[1:]) if len(op) > 1 else None)
+        elif kind == "eng":
+            _, ux, uy, fz, tipZ, ap, rat, nz = op
+            a = sl[ie:ie + 5 * nz].reshape(5, nz) if nz else np.zeros((5, 0))
+            ie += 5 * nz
+            got = eng.set_engagement(ux, uy, fz, tipZ, ap, a[0] if rat else None, a[1] if rat else None)
+            if got != nz:
+                bad.append(("setEngagement nz", got, nz))
+            flat = flat and not rat
+        elif kind == "end":
+            eng.set_end_engagement(op[1], op[2], None if flat else rg[ir])
+            ir += 1
+        elif kind == "run":
+            if max_runs is not None and r >= max_runs:
+                break
+            eng.set_maps(Hb[r], Eb[r])
+            x, y, vx, vy, w, wv = pre[r]
+            eng.set_state(x=x, y=y, vx=vx, vy=vy, w=w, wv=wv)
+            eng.run(op[1])
+            s = eng.state()
+            got = np.array([float(getattr(s, k)) for k in ENGLOG_POST])
+            want = post[r, :len(ENGLOG_POST)]
+            if not (_same(got[iex], want[iex]) and all(_same(g, w_, rtol) for g, w_ in zip(got, want))):
+                bad.append(("run %d state" % r, [(k, g, w_) for k, g, w_ in zip(ENGLOG_POST, got, want)
+                                                 if not _same(g, w_, rtol)]))
+            st = eng.stats()
+            gs = [getattr(st, f) for f, _ in st._fields_]
+            if not all(_same(g, w_, rtol) for g, w_ in zip(gs, post[r, len(ENGLOG_POST):])):
+                bad.append(("run %d stats" % r, [(f, g, w_) for (f, _), g, w_ in zip(st._fields_, gs, post[r, len(ENGLOG_POST):])
+                                                 if not _same(g, w_, rtol)]))
+            evs = eng.chip_events()
+            if not _same(evs[:, :3] if len(evs) else np.zeros((0, 3)), ev[evoff[r]:evoff[r + 1]], rtol):
+                bad.append(("run %d chip events" % r, len(evs), int(evoff[r + 1] - evoff[r])))
+            r += 1
+        elif kind == "take":
+            st = eng.stats(take=True)
+            if not all(_same(getattr(st, f), w_, rtol) for (f, _), w_ in zip(st._fields_, takes[it])):
+                bad.append(("takeStats %d" % it,))
+            it += 1
+        elif kind == "ci":
+            if not _same(eng.chatter_index(), cis[ic], rtol):
+                bad.append(("chatterIndex %d" % ic, list(eng.chatter_index()), list(cis[ic])))
+            ic += 1
+        if len(bad) > 8:
+            break
+    if max_runs is None and not bad:
+        h, ts, ws, wsp = eng.rings()
+        fin = res["final"]
+        if not _same(h, zf32(fin["hist"]), rtol * 1e3):
+            bad.append(("hist ring",))
+        for name, got in (("ts", ts), ("ws", ws), ("wsp", wsp)):
+            want = np.concatenate([f32(fin[name]["x"]), f32(fin[name]["y"])])
+            if not _same(got, want, rtol * 1e3):
+                bad.append((name + " ring",))
+    return bad, r

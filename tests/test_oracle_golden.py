@@ -263,3 +263,41 @@ def test_simulator_engine_traces():
             else:   # size effect: pow's last bits (see test_oracle_math)
                 assert np.allclose(a, b, rtol=1e-9, atol=1e-12 * np.max(np.abs(b))), key
         assert np.any(fx != 0) or case["Hb"][45] != "-inf"
+
+
+def test_engine_object_replays_simulator_calls():
+    """the CuttingEngine object across a simulator's whole call sequence (configure, setWear,
+    setWork, setEngagement with flat, ball and profile slices, setEndEngagement on plunges and
+    ramps, run, takeStats, chatterIndex; src/cutting.js:21-326 driven by src/sim.js:252-257,
+    380-445,645-662): state, stats, chip events and rings after every run.  Bit for bit with
+    the linear force law; within 1e-12 with the size effect, whose Math.pow differs from the
+    restated fdlibm pow in the last bit"""
+    g = G.load("englog")
+    walls = ends = 0
+    for case, res in zip(g["cases"], g["results"]):
+        rtol = 0.0 if case.get("nonlinear") is False else 1e-12
+        bad, runs = G.replay_englog(oracle.OracleEngine(), res, rtol=rtol)
+        assert not bad, (case["tool"], bad[:3])
+        assert runs == sum(1 for op in res["ops"] if op[0] == "run") > 100
+        walls += sum(1 for op in res["ops"] if op[0] == "work" and len(op) > 1)
+        ends += sum(1 for op in res["ops"] if op[0] == "end" and op[1] > 0)
+    assert walls > 0 and ends > 0
+
+
+def test_engine_object_mistakes_are_caught():
+    """a planted mistake in the replayed calls (a wall normal turned, the worn edge
+    coefficients changed) is detected by the replay"""
+    g = G.load("englog")
+    case, res = g["cases"][3], g["results"][3]
+    assert case.get("nonlinear") is False
+    ops = [list(op) for op in res["ops"]]
+    k = next(i for i, op in enumerate(ops) if op[0] == "work" and len(op) > 1)
+    ops[k][4], ops[k][5] = ops[k][5], ops[k][4]
+    bad, _ = G.replay_englog(oracle.OracleEngine(), dict(res, ops=ops))
+    assert bad
+    ops = [list(op) for op in res["ops"]]
+    for op in ops:
+        if op[0] == "wear":
+            op[1] *= 1.01
+    bad, _ = G.replay_englog(oracle.OracleEngine(), dict(res, ops=ops))
+    assert bad
