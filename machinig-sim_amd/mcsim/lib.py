@@ -10,7 +10,8 @@ import numpy as np
 
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
                    SearchTuning, EngageQuery, EngageResult, EngageSpec, cell_array, MC_OK, MC_SURR_IN,
-                   MC_STOCK_B)
+                   MC_STOCK_B, EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST,
+                   MC_ENGINE_RING)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -25,7 +26,10 @@ EXPORTS = [
     "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
     "mc_set_search_tuning", "mc_get_search_tuning", "mc_stamp_batch_ex", "mc_stock_read_marks",
     "mc_stock_dirty", "mc_stock_clear_dirty", "mc_stock_block_max", "mc_stock_max_in_disk", "mc_tip_lut",
-    "mc_stock_engagement", "mc_engine_trace_ex",
+    "mc_stock_engagement", "mc_engine_trace_ex", "mc_engine_create", "mc_engine_destroy", "mc_engine_configure",
+    "mc_engine_set_wear", "mc_engine_set_work", "mc_engine_set_engagement", "mc_engine_set_end_engagement",
+    "mc_engine_set_maps", "mc_engine_get_state", "mc_engine_set_state", "mc_engine_run", "mc_engine_run_batch",
+    "mc_engine_read_stats", "mc_engine_chip_events", "mc_engine_chatter_index", "mc_engine_read_rings",
 ]
 
 _LIB = None
@@ -89,6 +93,22 @@ def load():
     L.mc_stability_lobes.argtypes = [C.POINTER(LobeArgs), P, P, C.c_int, P, P]
     L.mc_alim_at.argtypes = [P, P, P, C.c_int, C.c_double, P]
     L.mc_analytic_limits.argtypes = [C.POINTER(Cell), C.c_int, P]
+    L.mc_engine_create.argtypes = [C.POINTER(P)]
+    L.mc_engine_destroy.argtypes = [P]
+    L.mc_engine_configure.argtypes = [P, C.POINTER(Cell), C.c_double, C.POINTER(EngineInfo)]
+    L.mc_engine_set_wear.argtypes = [P, C.c_double, C.c_double]
+    L.mc_engine_set_work.argtypes = [P, P]
+    L.mc_engine_set_engagement.argtypes = [P] + [C.c_double] * 5 + [P, P, P]
+    L.mc_engine_set_end_engagement.argtypes = [P, C.c_double, C.c_double, P]
+    L.mc_engine_set_maps.argtypes = [P, P, P]
+    L.mc_engine_get_state.argtypes = [P, C.POINTER(EngineState)]
+    L.mc_engine_set_state.argtypes = [P, C.POINTER(EngineState)]
+    L.mc_engine_run.argtypes = [P, C.c_int64]
+    L.mc_engine_run_batch.argtypes = [P, P, C.c_int]
+    L.mc_engine_read_stats.argtypes = [P, C.POINTER(EngineStats), C.c_int]
+    L.mc_engine_chip_events.argtypes = [P, P, C.c_int64, P]
+    L.mc_engine_chatter_index.argtypes = [P, P]
+    L.mc_engine_read_rings.argtypes = [P, P, P, P, P]
     L.mc_set_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     L.mc_get_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     _LIB = L
@@ -176,6 +196,99 @@ def engine_trace_ex(cell, spec, nsteps):
     _check(load().mc_engine_trace_ex(C.byref(cell), C.byref(spec), int(nsteps), *[x.ctypes.data for x in a],
                                      C.byref(n)))
     return [x[:n.value] for x in a]
+
+
+class Engine:
+    """Device-resident CuttingEngine (new CuttingEngine(), src/cutting.js:22-35): the state
+    survives between calls, as the simulator keeps its engine across sub-steps."""
+
+    def __init__(self):
+        self.L = load()
+        self.p = C.c_void_p()
+        _check(self.L.mc_engine_create(C.byref(self.p)))
+
+    def close(self):
+        if self.p:
+            self.L.mc_engine_destroy(self.p)
+            self.p = None
+
+    __del__ = close
+
+    def configure(self, cell, fnMax=0.0):
+        info = EngineInfo()
+        _check(self.L.mc_engine_configure(self.p, C.byref(cell), float(fnMax), C.byref(info)))
+        return info
+
+    def set_wear(self, Kte, Kre):
+        _check(self.L.mc_engine_set_wear(self.p, float(Kte), float(Kre)))
+
+    def set_work(self, wall):
+        _check(self.L.mc_engine_set_work(self.p, C.byref(wall) if wall is not None else None))
+
+    def set_engagement(self, ux, uy, fz, tipZ, ap, sliceR=None, sliceKap=None):
+        nz = C.c_int32()
+        r = np.ascontiguousarray(sliceR, dtype=np.float64) if sliceR is not None else None
+        k = np.ascontiguousarray(sliceKap, dtype=np.float64) if sliceKap is not None else None
+        _check(self.L.mc_engine_set_engagement(self.p, ux, uy, fz, tipZ, ap,
+                                               r.ctypes.data if r is not None else None,
+                                               k.ctypes.data if k is not None else None, C.byref(nz)))
+        return nz.value
+
+    def set_end_engagement(self, fza, rOut, rings=None):
+        g = np.ascontiguousarray(rings, dtype=np.float64) if rings is not None else None
+        _check(self.L.mc_engine_set_end_engagement(self.p, float(fza), float(rOut),
+                                                   g.ctypes.data if g is not None else None))
+
+    def set_maps(self, Hb=None, Eb=None):
+        hb = np.ascontiguousarray(Hb, dtype=np.float32) if Hb is not None else None
+        eb = np.ascontiguousarray(Eb, dtype=np.float32) if Eb is not None else None
+        _check(self.L.mc_engine_set_maps(self.p, hb.ctypes.data if hb is not None else None,
+                                         eb.ctypes.data if eb is not None else None))
+
+    def state(self):
+        s = EngineState()
+        _check(self.L.mc_engine_get_state(self.p, C.byref(s)))
+        return s
+
+    def set_state(self, **kw):
+        s = self.state()
+        for k, v in kw.items():
+            setattr(s, k, v)
+        _check(self.L.mc_engine_set_state(self.p, C.byref(s)))
+
+    def run(self, n):
+        _check(self.L.mc_engine_run(self.p, int(n)))
+
+    def stats(self, take=False):
+        s = EngineStats()
+        _check(self.L.mc_engine_read_stats(self.p, C.byref(s), 1 if take else 0))
+        return s
+
+    def chip_events(self):
+        n = C.c_int64()
+        _check(self.L.mc_engine_chip_events(self.p, None, 0, C.byref(n)))
+        out = (ChipEvent * max(1, n.value))()
+        _check(self.L.mc_engine_chip_events(self.p, out, n.value, C.byref(n)))
+        return np.array([[e.vol, e.phi, e.z, e.ux, e.uy, e.vx, e.vy] for e in out[:n.value]]).reshape(n.value, 7)
+
+    def chatter_index(self):
+        out = np.zeros(3)
+        _check(self.L.mc_engine_chatter_index(self.p, out.ctypes.data))
+        return out
+
+    def rings(self):
+        h = np.zeros(5 * MC_ENGINE_HIST, np.float32)
+        r = [np.zeros(2 * MC_ENGINE_RING, np.float32) for _ in range(3)]
+        _check(self.L.mc_engine_read_rings(self.p, h.ctypes.data, *[x.ctypes.data for x in r]))
+        return h, r[0], r[1], r[2]
+
+
+def engine_run_batch(engines, nsteps):
+    """run engine i for nsteps[i] steps, all in one launch"""
+    n = len(engines)
+    ptrs = (C.c_void_p * max(1, n))(*[e.p.value for e in engines])
+    steps = np.ascontiguousarray(nsteps, dtype=np.int64)
+    _check(load().mc_engine_run_batch(ptrs, steps.ctypes.data, n))
 
 
 class Stock:

@@ -211,13 +211,16 @@ ENGLOG_EXACT = ("pr", "stepCount", "nz", "hist_i", "hist_count", "ts_i", "ts_cou
                 "wsp_count")
 
 
-def replay_englog(eng, res, max_runs=None, rtol=0.0):
-    """Drive an engine (the oracle's OracleEngine or the library's Engine: same methods)
-    through a call sequence recorded from the reference simulator (golden_englog.json) and
-    compare, bit for bit, the engine's state, stats and chip events after every run, every
-    takeStats and chatterIndex answer, and the sample rings at the end.  Returns the list of
-    mismatches (empty when the engine reproduces the reference) and the number of runs."""
+def englog_replayer(eng, res, max_runs=None, rtol=0.0, bad=None):
+    """Generator over a call sequence recorded from the reference simulator
+    (golden_englog.json): applies the recorded calls to `eng` (the oracle's OracleEngine or
+    the library's Engine: same methods) up to each run, yields the run's step count for the
+    caller to run, then compares, bit for bit (or within rtol), the engine's state, stats
+    and chip events with the reference's; also every takeStats and chatterIndex answer and
+    the sample rings at the end.  Mismatches are appended to `bad`; the generator returns the
+    number of runs checked."""
     from mcsim._abi import Wall
+    bad = [] if bad is None else bad
     nst = res["nst"]
     Hb = zf32(res["Hb"]).reshape(-1, 180)
     Eb = zf32(res["Eb"]).reshape(-1, 216)
@@ -229,7 +232,6 @@ def replay_englog(eng, res, max_runs=None, rtol=0.0):
     cis = f64(res["cis"]).reshape(-1, 3)
     sl = zf64(res["sl"])
     rg = zf64(res["rg"]).reshape(-1, 24)
-    bad = []
     r = ie = ir = it = ic = 0
     flat = True
     iex = [ENGLOG_POST.index(k) for k in ENGLOG_EXACT]
@@ -266,7 +268,7 @@ def replay_englog(eng, res, max_runs=None, rtol=0.0):
             eng.set_maps(Hb[r], Eb[r])
             x, y, vx, vy, w, wv = pre[r]
             eng.set_state(x=x, y=y, vx=vx, vy=vy, w=w, wv=wv)
-            eng.run(op[1])
+            yield op[1]
             s = eng.state()
             got = np.array([float(getattr(s, k)) for k in ENGLOG_POST])
             want = post[r, :len(ENGLOG_POST)]
@@ -302,4 +304,18 @@ def replay_englog(eng, res, max_runs=None, rtol=0.0):
             want = np.concatenate([f32(fin[name]["x"]), f32(fin[name]["y"])])
             if not _same(got, want, rtol * 1e3):
                 bad.append((name + " ring",))
-    return bad, r
+    return r
+
+
+def replay_englog(eng, res, max_runs=None, rtol=0.0):
+    """replay one recorded call sequence on `eng` (see englog_replayer); returns the list of
+    mismatches (empty when the engine reproduces the reference) and the number of runs"""
+    bad = []
+    g = englog_replayer(eng, res, max_runs, rtol, bad)
+    try:
+        n = next(g)
+        while True:
+            eng.run(n)
+            n = g.send(None)
+    except StopIteration as stop:
+        return bad, stop.value

@@ -38,6 +38,10 @@ int main(void) {
          sizeof(mc_train_opts), sizeof(mc_lobe_args), sizeof(mc_search_tuning));
   printf("%zu %zu\n", offsetof(mc_stamp_opt, lut), offsetof(mc_engage_result, imm));
   printf("%zu %zu\n", sizeof(mc_engage_spec), offsetof(mc_engage_spec, Hb));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(mc_engine_info), sizeof(mc_wall), sizeof(mc_engine_state),
+         sizeof(mc_engine_stats), sizeof(mc_chip_event));
+  printf("%zu %zu %zu\n", offsetof(mc_engine_state, stepCount), offsetof(mc_engine_state, hist_i),
+         offsetof(mc_engine_info, stepsPerTooth));
   return 0;
 }'''
     import tempfile
@@ -53,4 +57,6 @@ int main(void) {
                                    _abi.LobeArgs, _abi.SearchTuning)]
     want += [_abi.StampOpt.lut.offset, _abi.EngageResult.imm.offset]
     want += [C.sizeof(_abi.EngageSpec), _abi.EngageSpec.Hb.offset]
+    want += [C.sizeof(t) for t in (_abi.EngineInfo, _abi.Wall, _abi.EngineState, _abi.EngineStats, _abi.ChipEvent)]
+    want += [_abi.EngineState.stepCount.offset, _abi.EngineState.hist_i.offset, _abi.EngineInfo.stepsPerTooth.offset]
     assert [int(x) for x in out] == want
