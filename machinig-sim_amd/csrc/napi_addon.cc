@@ -9,6 +9,7 @@
 #define NAPI_VERSION 6
 #include <node_api.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -542,6 +543,231 @@ napi_value StockSample(napi_env env, napi_callback_info info) {
   return ta;
 }
 
+// ---- the device-resident CuttingEngine (mc_engine_*) for js/cutting.js
+void engine_finalize(napi_env, void* data, void*) {
+  mc_engine** h = (mc_engine**)data;
+  if (h && *h) mc_engine_destroy(*h);
+  delete h;
+}
+
+mc_engine* engine_of(napi_env env, napi_value v) {
+  mc_engine** h = nullptr;
+  if (napi_get_value_external(env, v, (void**)&h) != napi_ok || !h || !*h) {
+    napi_throw_type_error(env, nullptr, "not a live engine handle");
+    return nullptr;
+  }
+  return *h;
+}
+
+napi_value f64_array(napi_env env, const double* v, size_t n) {
+  void* d;
+  napi_value ab = new_ab(env, n * 8, &d);
+  if (!ab) return nullptr;
+  std::memcpy(d, v, n * 8);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta));
+  return ta;
+}
+
+napi_value EngineCreate(napi_env env, napi_callback_info) {
+  mc_engine* e = nullptr;
+  int st = mc_engine_create(&e);
+  if (st) return throw_mc(env, st);
+  mc_engine** h = new mc_engine*(e);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, h, engine_finalize, nullptr, &ext));
+  return ext;
+}
+
+// engineConfigure(h, cell: one packed mc_cell, fnMax) -> Float64Array [dt, omega, Ttooth, dz, S,
+// stepsPerTooth, stepsPerRev, decim, piStep, maxSlices]
+napi_value EngineConfigure(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  void* cell;
+  size_t cb;
+  if (!buffer_of(env, argv[1], &cell, &cb)) return nullptr;
+  if (cb < sizeof(mc_cell)) { napi_throw_range_error(env, nullptr, "cell record too short"); return nullptr; }
+  mc_engine_info in;
+  int st = mc_engine_configure(e, (const mc_cell*)cell, num(env, argv[2]), &in);
+  if (st) return throw_mc(env, st);
+  const double v[10] = {in.dt, in.omega, in.Ttooth, in.dz, in.S, (double)in.stepsPerTooth, (double)in.stepsPerRev,
+                        (double)in.decim, (double)in.piStep, (double)in.maxSlices};
+  return f64_array(env, v, 10);
+}
+
+napi_value EngineSetWear(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  int st = mc_engine_set_wear(e, num(env, argv[1]), num(env, argv[2]));
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// engineSetWork(h, null | Float64Array [k, m, zeta, nx, ny])
+napi_value EngineSetWork(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  int st;
+  if (is_nullish(env, argv[1])) {
+    st = mc_engine_set_work(e, nullptr);
+  } else {
+    void* w;
+    size_t wb;
+    if (!buffer_of(env, argv[1], &w, &wb)) return nullptr;
+    if (wb < sizeof(mc_wall)) { napi_throw_range_error(env, nullptr, "wall needs 5 values"); return nullptr; }
+    st = mc_engine_set_work(e, (const mc_wall*)w);
+  }
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// engineSetEngagement(h, ux, uy, fz, tipZ, ap, sliceR | null, sliceKap | null) -> nz
+napi_value EngineSetEngagement(napi_env env, napi_callback_info info) {
+  napi_value argv[8];
+  if (!get_args(env, info, 8, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  void *r = nullptr, *k = nullptr;
+  size_t rb = 0, kb = 0;
+  if (!is_nullish(env, argv[6]) && !buffer_of(env, argv[6], &r, &rb)) return nullptr;
+  if (!is_nullish(env, argv[7]) && !buffer_of(env, argv[7], &k, &kb)) return nullptr;
+  if (r && rb < 8 * (size_t)MC_MAX_SLICES) { napi_throw_range_error(env, nullptr, "sliceR needs MC_MAX_SLICES entries"); return nullptr; }
+  if (k && kb < 8 * (size_t)MC_MAX_SLICES) { napi_throw_range_error(env, nullptr, "sliceKap needs MC_MAX_SLICES entries"); return nullptr; }
+  int32_t nz = 0;
+  int st = mc_engine_set_engagement(e, num(env, argv[1]), num(env, argv[2]), num(env, argv[3]), num(env, argv[4]),
+                                    num(env, argv[5]), (const double*)r, (const double*)k, &nz);
+  if (st) return throw_mc(env, st);
+  napi_value v;
+  NAPI_OK(napi_create_int32(env, nz, &v));
+  return v;
+}
+
+// engineSetEndEngagement(h, fza, rOut, rings: Float64Array(24) | null)
+napi_value EngineSetEndEngagement(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  void* g = nullptr;
+  size_t gb = 0;
+  if (!is_nullish(env, argv[3]) && !buffer_of(env, argv[3], &g, &gb)) return nullptr;
+  if (g && gb < 8 * 4 * (size_t)MC_END_RINGS) { napi_throw_range_error(env, nullptr, "rings need 24 values"); return nullptr; }
+  int st = mc_engine_set_end_engagement(e, num(env, argv[1]), num(env, argv[2]), (const double*)g);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// engineRun(h, nSteps, Hb: Float32Array(180), Eb: Float32Array(216), set: Float64Array [x, y,
+// vx, vy, w, wv, t, decim, hist_i], Kte, Kre) -> Float64Array: the state after the run [x, y,
+// vx, vy, w, wv, t, runMeanW, wk, wnx, wny, stepCount, pr, decim, nz, hist_i, hist_count, ts_i,
+// ts_count, ws_i, ws_count, wsp_i, wsp_count], the 23 stats, the number of chip events
+napi_value EngineRun(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  void *hb, *eb, *sv;
+  size_t hbb, ebb, svb;
+  if (!buffer_of(env, argv[2], &hb, &hbb) || !buffer_of(env, argv[3], &eb, &ebb) || !buffer_of(env, argv[4], &sv, &svb))
+    return nullptr;
+  if (hbb < 4 * (size_t)MC_BINS || ebb < 4 * (size_t)(MC_END_BINS * MC_END_RINGS) || svb < 8 * 9) {
+    napi_throw_range_error(env, nullptr, "engine maps or state too short");
+    return nullptr;
+  }
+  const double* set = (const double*)sv;
+  mc_engine_state s;
+  int st = mc_engine_get_state(e, &s);
+  if (!st) {
+    s.x = set[0]; s.y = set[1]; s.vx = set[2]; s.vy = set[3]; s.w = set[4]; s.wv = set[5]; s.t = set[6];
+    s.decim = (int32_t)set[7]; s.hist_i = (int64_t)set[8];
+    st = mc_engine_set_state(e, &s);
+  }
+  if (!st) st = mc_engine_set_maps(e, (const float*)hb, (const float*)eb);
+  if (!st) st = mc_engine_set_wear(e, num(env, argv[5]), num(env, argv[6]));
+  if (!st) st = mc_engine_run(e, (int64_t)num(env, argv[1]));
+  mc_engine_stats sts;
+  if (!st) st = mc_engine_get_state(e, &s);
+  if (!st) st = mc_engine_read_stats(e, &sts, 0);
+  int64_t nev = 0;
+  if (!st) st = mc_engine_chip_events(e, nullptr, 0, &nev);
+  if (st) return throw_mc(env, st);
+  double v[23 + 23 + 1];
+  const double sv2[23] = {s.x, s.y, s.vx, s.vy, s.w, s.wv, s.t, s.runMeanW, s.wk, s.wnx, s.wny, (double)s.stepCount,
+                          (double)s.pr, (double)s.decim, (double)s.nz, (double)s.hist_i, (double)s.hist_count,
+                          (double)s.ts_i, (double)s.ts_count, (double)s.ws_i, (double)s.ws_count,
+                          (double)s.wsp_i, (double)s.wsp_count};
+  std::memcpy(v, sv2, sizeof sv2);
+  std::memcpy(v + 23, &sts, sizeof sts);
+  v[46] = (double)nev;
+  return f64_array(env, v, 47);
+}
+
+// engineTakeStats(h): restart the device statistics (takeStats)
+napi_value EngineTakeStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  int st = mc_engine_read_stats(e, nullptr, 1);
+  if (st) return throw_mc(env, st);
+  return nullptr;
+}
+
+// engineChipEvents(h) -> Float64Array [vol, phi, z] per event of the last run
+napi_value EngineChipEvents(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  int64_t n = 0;
+  int st = mc_engine_chip_events(e, nullptr, 0, &n);
+  if (st) return throw_mc(env, st);
+  std::vector<mc_chip_event> ev((size_t)std::max<int64_t>(n, 1));
+  st = mc_engine_chip_events(e, ev.data(), n, &n);
+  if (st) return throw_mc(env, st);
+  std::vector<double> v((size_t)(3 * n));
+  for (int64_t i = 0; i < n; i++) { v[3 * i] = ev[i].vol; v[3 * i + 1] = ev[i].phi; v[3 * i + 2] = ev[i].z; }
+  return f64_array(env, v.data(), v.size());
+}
+
+napi_value EngineChatterIndex(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  double out[3];
+  int st = mc_engine_chatter_index(e, out);
+  if (st) return throw_mc(env, st);
+  return f64_array(env, out, 3);
+}
+
+// engineRings(h) -> Float32Array: hist (5 x 4096: Fx Fy Fz x y), then toothSamples, wallSamples,
+// wallSamplesPi (2 x 64 each: x, y)
+napi_value EngineRings(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  const size_t n = 5 * MC_ENGINE_HIST + 6 * MC_ENGINE_RING;
+  void* d;
+  napi_value ab = new_ab(env, n * 4, &d);
+  if (!ab) return nullptr;
+  float* f = (float*)d;
+  int st = mc_engine_read_rings(e, f, f + 5 * MC_ENGINE_HIST, f + 5 * MC_ENGINE_HIST + 2 * MC_ENGINE_RING,
+                                f + 5 * MC_ENGINE_HIST + 4 * MC_ENGINE_RING);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, n, ab, 0, &ta));
+  return ta;
+}
+
 napi_value Sizes(napi_env env, napi_callback_info) {
   napi_value o, v;
   NAPI_OK(napi_create_object(env, &o));
@@ -566,7 +792,11 @@ napi_value ModuleInit(napi_env env, napi_value exports) {
       {"stockWriteH", StockWriteH}, {"stampBatch", StampBatch}, {"stockSample", StockSample},
       {"stockReadMarks", StockReadMarks}, {"stockDirty", StockDirty}, {"stockClearDirty", StockClearDirty},
       {"stockBlockMax", StockBlockMax}, {"stockMaxInDisk", StockMaxInDisk}, {"tipLut", TipLut},
-      {"stockEngagement", StockEngagement},
+      {"stockEngagement", StockEngagement}, {"engineCreate", EngineCreate},
+      {"engineConfigure", EngineConfigure}, {"engineSetWear", EngineSetWear}, {"engineSetWork", EngineSetWork},
+      {"engineSetEngagement", EngineSetEngagement}, {"engineSetEndEngagement", EngineSetEndEngagement},
+      {"engineRun", EngineRun}, {"engineTakeStats", EngineTakeStats}, {"engineChipEvents", EngineChipEvents},
+      {"engineChatterIndex", EngineChatterIndex}, {"engineRings", EngineRings},
       {"sizes", Sizes}};
   for (auto& f : fns) {
     napi_value fn;

@@ -6,3 +6,4 @@ export { engagementBatch, fillEngagement, immersion } from './engage.js';
 export { toolModel, isAssembly } from './tool.js';
 export { modesAt, tipFactor, bearingFactor, twoDofModes, speedDependent, SOFT_DEFAULT } from './spindle.js';
 export { native } from './native.js';
+export { CuttingEngine, BINS, END_BINS, END_RINGS, H_REF } from './cutting.js';

@@ -2,6 +2,7 @@
 // from the reference and checks they return the reference's answers.
 // usage: node js_dropin_check.mjs [lobes|gpu]
 import fs from 'fs';
+import zlib from 'zlib';
 import path from 'path';
 import { fileURLToPath } from 'url';
 
@@ -148,6 +149,74 @@ async function main() {
       check(b.every((r, i) => Object.is(r.hmax, num(ref.hmax[i]))), 'engagementBatch ' + k);
       s.dispose();
     });
+    // the CuttingEngine class over the simulator's recorded call sequences (linear force law,
+    // flat and ball tools): state, stats, chip events, takeStats, chatterIndex and rings
+    const zdec = (s, T) => { const b = zlib.inflateSync(Buffer.from(s, 'base64')); return new T(b.buffer.slice(b.byteOffset, b.byteOffset + b.length)); };
+    const eq = (a, b) => a === b || (a !== a && b !== b);
+    const STATS = ['n', 'sumF', 'maxF', 'sumFx', 'sumFy', 'sumFz', 'maxFz', 'sumT', 'maxT', 'sumX', 'sumY', 'minX',
+      'maxX', 'minY', 'maxY', 'sumZc', 'wZc', 'sumW', 'maxW', 'sumHb', 'sumB', 'sumRb', 'engTeeth'];
+    const el = load('englog');
+    let replayed = 0;
+    el.cases.forEach((c, k) => {
+      if (c.nonlinear !== false || c.tool.type === 'bull') return;
+      const res = el.results[k];
+      const Hb = zdec(res.Hb, Float32Array), Eb = zdec(res.Eb, Float32Array), pre = zdec(res.pre, Float64Array);
+      const post = zdec(res.post, Float64Array), ev = zdec(res.ev, Float64Array), takes = zdec(res.takes, Float64Array);
+      const cis = f64(res.cis), K = 19 + res.nst;
+      const e = new m.CuttingEngine();
+      let r = 0, evo = 0, it = 0, ic = 0, bad = 0, tool = null;
+      for (const op of res.ops) {
+        if (op[0] === 'cfg') {
+          const [, N, R, hd, type, fl, nl, ro, rph, S, fnMax, mx, my, mt] = op;
+          tool = { N, R, helixDeg: hd, type, fluteLen: fl, nonlinear: nl === 1, runout: ro, runoutPhase: rph };
+          const modes = L => L.map(q => ({ k: q[0], fn: q[1], zeta: q[2] }));
+          e.configure(tool, { k: mx[0][0], fn: mx[0][1], zeta: mx[0][2], modesX: modes(mx), modesY: modes(my) },
+            { Ktc: mt[0], Krc: mt[1], Kte: mt[2], Kre: mt[3], Kac: mt[4], Kae: mt[5], mc: mt[6], Ci: mt[7] }, S, fnMax);
+          if (e.stepsPerTooth !== op[14][0] || e.decim !== op[14][2] || e.dt !== op[14][6]) bad++;
+        } else if (op[0] === 'wear') e.setWear({ Kte: op[1], Kre: op[2] });
+        else if (op[0] === 'work') e.setWork(op.length > 1 ? { k: op[1], m: op[2], zeta: op[3], nx: op[4], ny: op[5] } : null);
+        else if (op[0] === 'eng') { e.setEngagement([op[1], op[2]], op[3], op[4], op[5]); if (e.nz !== op[7]) bad++; }
+        else if (op[0] === 'end') {
+          const R = tool.R;
+          e.setEndEngagement(op[1], op[2], tool.type === 'ball' ? (q => R - Math.sqrt(Math.max(0, R * R - q * q))) : undefined);
+        } else if (op[0] === 'run') {
+          e.Hb.set(Hb.subarray(180 * r, 180 * r + 180)); e.Eb.set(Eb.subarray(216 * r, 216 * r + 216));
+          [e.x, e.y, e.vx, e.vy, e.w, e.wv] = pre.subarray(6 * r, 6 * r + 6);
+          const before = e.chipEvents.length;
+          e.run(op[1]);
+          const p = post.subarray(K * r, K * r + K);
+          const got = [e.x, e.y, e.vx, e.vy, e.w, e.wv, e.t, e.pr, e.runMeanW, e.stepCount, e.nz, e.hist.i, e.hist.count,
+            e.toothSamples.i, e.toothSamples.count, e.wallSamples.i, e.wallSamples.count, e.wallSamplesPi.i, e.wallSamplesPi.count];
+          if (!got.every((v, i) => eq(v, p[i]))) bad++;
+          if (!STATS.every((f, i) => eq(e.stats[f], p[19 + i]))) bad++;
+          const nev = res.evn[r], mine = e.chipEvents.slice(before);
+          if (mine.length !== nev || !mine.every((q, i) => eq(q.vol, ev[3 * (evo + i)]) && eq(q.phi, ev[3 * (evo + i) + 1]) &&
+            eq(q.z, ev[3 * (evo + i) + 2]) && q.ux === e.ux && q.vy === e.vy_)) bad++;
+          evo += nev;
+          r++;
+        } else if (op[0] === 'take') {
+          const s = e.takeStats();
+          if (!STATS.every((f, i) => eq(s[f], takes[res.nst * it + i]))) bad++;
+          it++;
+        } else if (op[0] === 'ci') {
+          const ci = e.chatterIndex();
+          if (!(eq(ci.index, cis[3 * ic]) && eq(ci.amp, cis[3 * ic + 1]) && eq(ci.once === undefined ? 0 : ci.once, cis[3 * ic + 2]))) bad++;
+          ic++;
+        }
+        if (bad) break;
+      }
+      const fin = res.final, hist = zdec(fin.hist, Float32Array);
+      if (!bad) {
+        const H = 4096;
+        ['Fx', 'Fy', 'Fz', 'x', 'y'].forEach((f, j) => { if (!sameBits(e.hist[f], hist.subarray(j * H, j * H + H))) bad++; });
+        [['ts', e.toothSamples], ['ws', e.wallSamples], ['wsp', e.wallSamplesPi]].forEach(([n, b]) => {
+          if (!sameBits(b.x, f32(fin[n].x)) || !sameBits(b.y, f32(fin[n].y))) bad++;
+        });
+      }
+      check(bad === 0 && r > 100, 'CuttingEngine replay ' + k + ' (' + c.tool.type + ', ' + r + ' runs)');
+      replayed++;
+    });
+    check(replayed === 3, 'CuttingEngine replays');
     const { handle } = await import(path.join(here, '..', 'machinig-sim_amd', 'js', 'stabworker.js'));
     const msgs = [];
     const c0 = td.cases[0].cell;
