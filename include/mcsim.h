@@ -388,6 +388,58 @@ enum { MC_TIP_FLAT = 0, MC_TIP_BALL = 1, MC_TIP_BULL = 2, MC_TIP_EDGE = 3 };
 int mc_tip_lut(int32_t shape, double D, double cornerR, const double* edge_r, int32_t edge_n,
                double edge_dz, float* out, int32_t cap, int32_t* n_out);
 
+/* ---- The simulator's sub-step pipeline on the device (src/sim.js:351-449) ----
+ *
+ * Simulator.step per sub-step: the engagement map from the actual stock (fillEngagement,
+ * and fillEndEngagement while descending), the axial depth, the engine set up and run for
+ * the sub-step's share of steps (or its vibration halved out of the cut), then the
+ * commanded stamp at p and the actual stamp at p plus the deflection, with the feed mark.
+ * One wavefront per program runs its whole sub-step list without returning to the host;
+ * mc_sim_run puts many programs in one launch.  The workpiece is rigid (workpieceFlex off,
+ * so setWork(null)), the flank wear fixed for the call (the engine's setWear and rw), and
+ * the tool flat or ball with the simulator's end-face shape. */
+
+/* one sub-step, as the simulator's schedule produces it (src/sim.js:363-381,389-405) */
+typedef struct {
+  double x, y, z;     /* tool position p [mm] */
+  double ux, uy;      /* the feed direction the simulator holds (this.u) */
+  double dtm;         /* machine time of the sub-step [s] */
+  double fzEff;       /* feed per tooth along the move's xy path [mm] (0 on vertical moves) */
+  double fza;         /* axial feed per tooth while descending [mm] (0 otherwise) */
+  double dxy;         /* xy length of the move [mm]: below 1e-6 a vertical move */
+  int32_t rapid;      /* a G0 move */
+  int32_t line;       /* program line (Stock.curLine) */
+} mc_substep;
+
+/* what a sub-step did */
+typedef struct {
+  double dx, dy;      /* deflection added to the actual stamp [mm] */
+  double ap;          /* engaged axial depth [mm] */
+  double errVal;      /* vibration written by the actual stamp [um] */
+  double vol;         /* volume the actual stamp removed [mm^3] */
+  double maxF;        /* largest |(Fx, Fy)| of the sub-step's run [N] (0 without a run) */
+  int32_t nsteps;     /* engine steps run (0: not cutting) */
+  int32_t cut;        /* 1 when the sub-step cut */
+} mc_substep_out;
+
+typedef struct {
+  mc_stock* actual;        /* Simulator.stock: cut by the deflected tool */
+  mc_stock* ideal;         /* Simulator.ideal: the commanded cut */
+  mc_engine* engine;       /* configured for this program's tool and speed, with its setWear */
+  const mc_substep* sub;   /* nsub sub-steps, in order */
+  mc_substep_out* out;     /* nsub results, or NULL */
+  double R;                /* tool radius [mm] */
+  double fluteLen;         /* [mm] */
+  double rw;               /* radial flank wear taken off the actual stamp [mm] (src/sim.js:439) */
+  double share;            /* steps a sub-step may run besides its own time: the frame
+                              budget's share, max(stepsPerTooth / 4, ...) (src/sim.js:410-412) */
+  int32_t nsub;
+  int32_t type;            /* MC_TOOL_FLAT | MC_TOOL_BALL */
+} mc_sim_program;
+
+/* n programs, one wavefront each, one launch; the stocks and engines must be distinct */
+int mc_sim_run(const mc_sim_program* progs, int n);
+
 /* ---- Analytic (zero-order) stability lobes, host side ---- */
 
 /* stabilityLobes parameters (src/sld.js:23): flutes, Ktc, Krc, the engagement arc, the

@@ -639,11 +639,22 @@ int fail(int code, const char* fmt, ...) {
 int init() { return ensure_init(-1); }
 hipStream_t stream() { return g_ctx.stream; }
 std::mutex& lock() { return g_ctx.mu; }
+int stock_view(mc_stock* s, StockView* v);
 int cus() { return g_ctx.cus; }
 void set_last_ms(double ms) { g_ctx.last_ms = ms; }
 }  // namespace mch
 
 struct mc_stock : StockImpl {};
+
+int mch::stock_view(mc_stock* s, StockView* v) {
+  if (!s || !v) return fail(MC_ERR_ARG, "null stock");
+  v->nx = s->nx; v->ny = s->ny; v->hx = s->hx; v->hy = s->hy; v->zTop = s->zTop; v->zBottom = s->zB;
+  v->band = s->band;
+  v->H = s->H; v->err = s->err; v->wallD = s->wallD; v->markS = s->markS; v->markF = s->markF;
+  v->lineCut = s->lineCut; v->lineWall = s->lineWall; v->rect = s->rect;
+  s->bmax_stale = true;
+  return MC_OK;
+}
 
 extern "C" {
 

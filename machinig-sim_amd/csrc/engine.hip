@@ -5,15 +5,7 @@
 // history and the tooth / wall sample rings.  A run of n steps is one workgroup of one
 // wavefront per engine; mc_engine_run_batch puts many engines in one launch.
 //
-// Per step, the lanes split the (tooth, slice) side-edge visits and the (tooth, ring)
-// end-edge visits; each lane evaluates its visits' engagement, chip thickness and forces,
-// writes the delay line and leaves every quantity the reference accumulates in LDS, at the
-// visit's position in the reference's loop order (tooth by tooth, its slices, then its end
-// rings).  Nine lanes then each sum one of the accumulations (Fx, Fy, Fz, torque, force
-// sum, its moment, and the three engagement statistics) serially in that order, and one
-// lane per tooth sums the tooth's chip volume and decides its chip event, so every FP64
-// sum is the reference's, bit for bit.  The modal and wall updates run redundantly on all
-// lanes; lane 0 keeps the step statistics and writes the sample rings.
+// The step loop itself is engine_steps (engine_dev.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,354 +17,16 @@
 #include "../../include/mcsim.h"
 #include "mc_host.h"
 #include "mc_math.h"
+#include "engine_dev.h"
+
+using namespace mce;
 
 namespace {
 
-constexpr int EBINS = MC_END_BINS, ERINGS = MC_END_RINGS;
-constexpr int HN = MC_ENGINE_HIST, RN = MC_ENGINE_RING;
-constexpr int MS = MC_MAX_SLICES;
-constexpr int SLOT_CAP = MC_MAX_FLUTES * (MS + ERINGS);
-constexpr double JS_PI = 3.141592653589793;
-constexpr double H_REF = 0.05, H_MIN = 0.002;
-
-enum { S_N, S_SUMF, S_MAXF, S_SUMFX, S_SUMFY, S_SUMFZ, S_MAXFZ, S_SUMT, S_MAXT, S_SUMX, S_SUMY, S_MINX,
-       S_MAXX, S_MINY, S_MAXY, S_SUMZC, S_WZC, S_SUMW, S_MAXW, S_SUMHB, S_SUMB, S_SUMRB, S_ENGT, S_COUNT };
-
-// accumulations kept per visit (the order of the chain lanes)
-enum { T_FX, T_FY, T_FZ, T_TQ, T_FS, T_FR2Z, T_HB, T_B, T_RB, T_VOL, T_PHI, T_Z, T_COUNT };
-
-// One engine.  The host keeps a mirror, uploads it before a run and reads it back after.
-struct EngDev {
-  // configure
-  double R, tanB, omega, dt, Ttooth, dz, S;
-  double kx[MC_MAX_MODES], mx[MC_MAX_MODES], cx[MC_MAX_MODES], qx[MC_MAX_MODES], vxm[MC_MAX_MODES];
-  double ky[MC_MAX_MODES], my[MC_MAX_MODES], cy[MC_MAX_MODES], qy[MC_MAX_MODES], vym[MC_MAX_MODES];
-  double dR[MC_MAX_FLUTES];
-  int32_t N, spt, spr, decim, nmx, nmy, maxSlices, piStep;
-  int32_t pr, nz, pad0, pad1;
-  int64_t stepCount;
-  // run constants, derived on the host from the material, wear and engagement
-  double Ktc, Krc, Kac, Kae, Kte, Kre, mc, pdK, dphi, hMax, pitch, binScale;
-  // state
-  double x, y, vx, vy, x0, y0, vx0, vy0, t, w, wv, runMeanW, wk, wm, wc, wnx, wny;
-  // setEngagement / setEndEngagement
-  double ux, uy, ex, ey, fz, tipZ, db;
-  double sB[MS], sZ[MS], sLag[MS], sR[MS], sKap[MS];
-  double fza, endR, ringR[ERINGS], ringCos[ERINGS], ringZ[ERINGS], ringDs[ERINGS];
-  float Hb[MC_BINS], Eb[EBINS * ERINGS];
-  // per tooth
-  double toothVol[MC_MAX_FLUTES], toothPhi[MC_MAX_FLUTES], toothZ[MC_MAX_FLUTES];
-  int32_t toothEngaged[MC_MAX_FLUTES];
-  // stats and rings
-  double st[S_COUNT];
-  int64_t hist_i, hist_count, ts_i, ts_count, ws_i, ws_count, wsp_i, wsp_count;
-  // this run
-  int64_t nsteps, nev, ev_cap;
-  float* surf;   // [spr][maxSlices]
-  float* hist;   // [5][HN]: Fx Fy Fz x y
-  float* rings;  // [6][RN]: ts x, ts y, ws x, ws y, wsp x, wsp y
-  double* ev;    // [ev_cap][3]: vol, phi, z
-};
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Math.max / Math.min: NaN wins, +0 is above -0
-__host__ __device__ __forceinline__ double jmax(double a, double b) {
-  if (a != a || b != b) return __builtin_nan("");
-  if (a == 0 && b == 0) return __builtin_signbit(a) ? b : a;
-  return a > b ? a : b;
-}
-__host__ __device__ __forceinline__ double jmin(double a, double b) {
-  if (a != a || b != b) return __builtin_nan("");
-  if (a == 0 && b == 0) return __builtin_signbit(a) ? a : b;
-  return a < b ? a : b;
-}
-__host__ __device__ __forceinline__ bool truthy(double v) { return v != 0 && v == v; }
-
 __global__ __launch_bounds__(64) void engine_run(EngDev* const* engines) {
+  __shared__ EngLds L;
   EngDev& E = *engines[blockIdx.x];
-  __shared__ double term[T_COUNT][SLOT_CAP];
-  __shared__ int teng[SLOT_CAP];
-  __shared__ double res[8];
-  const int lane = threadIdx.x;
-  const double TWO_PI = mcm::TWO_PI;
-
-  const int N = E.N, spt = E.spt, spr = E.spr, nz = E.nz, MSL = E.maxSlices, decim = E.decim;
-  const int nmx = E.nmx, nmy = E.nmy, piStep = E.piStep;
-  const double ux = E.ux, uy = E.uy, vx = E.ex, vy = E.ey, fz = E.fz, dt = E.dt, fza = E.fza, tipZ = E.tipZ;
-  const double Ktc = E.Ktc, Krc = E.Krc, Kac = E.Kac, Kae = E.Kae, Kte = E.Kte, Kre = E.Kre;
-  const double mc = E.mc, pdK = E.pdK, db = E.db, dphi = E.dphi, hMax = E.hMax, pitch = E.pitch,
-               binScale = E.binScale;
-  const double wnx = E.wnx, wny = E.wny, wk = E.wk, wm = E.wm, wc = E.wc;
-  const bool ends = truthy(fza);
-  const int per = nz + (ends ? ERINGS : 0);
-  const int NS = N * per;
-  const bool mcOn = truthy(mc), pdOn = truthy(pdK), wallOn = truthy(wk);
-  const int64_t nSteps = E.nsteps;
-  float* const surf = E.surf;
-
-  // modal state, redundantly on every lane; a state set from outside enters mode 0
-  double qx[MC_MAX_MODES], vxm[MC_MAX_MODES], qy[MC_MAX_MODES], vym[MC_MAX_MODES];
-#pragma unroll
-  for (int j = 0; j < MC_MAX_MODES; j++) { qx[j] = E.qx[j]; vxm[j] = E.vxm[j]; qy[j] = E.qy[j]; vym[j] = E.vym[j]; }
-  qx[0] += E.x - E.x0; qy[0] += E.y - E.y0;
-  vxm[0] += E.vx - E.vx0; vym[0] += E.vy - E.vy0;
-  double x = E.x, y = E.y, vxs = E.vx, vys = E.vy, t = E.t, w = E.w, wv = E.wv, wSum = 0;
-  int pr = E.pr;
-  int64_t stepCount = E.stepCount;
-  // chain lanes 6..8 carry the engagement statistics across steps; tooth lanes their tooth
-  const int tooth = lane - 16;
-  const bool isTooth = tooth >= 0 && tooth < N;
-  double chain = lane == 6 ? E.st[S_SUMHB] : lane == 7 ? E.st[S_SUMB] : lane == 8 ? E.st[S_SUMRB] : 0.0;
-  double tvol = isTooth ? E.toothVol[tooth] : 0.0, tphi = isTooth ? E.toothPhi[tooth] : 0.0,
-         tz = isTooth ? E.toothZ[tooth] : 0.0;
-  int tEng = isTooth ? E.toothEngaged[tooth] : 0;
-  // lane 0: the step statistics, rings and events
-  double st[S_COUNT];
-#pragma unroll
-  for (int k = 0; k < S_COUNT; k++) st[k] = E.st[k];
-  int64_t hist_i = E.hist_i, hist_count = E.hist_count, ts_i = E.ts_i, ts_count = E.ts_count;
-  int64_t ws_i = E.ws_i, ws_count = E.ws_count, wsp_i = E.wsp_i, wsp_count = E.wsp_count;
-  int64_t nev = 0;
-  const int64_t ev_cap = E.ev_cap;
-
-  for (int64_t s = 0; s < nSteps; s++) {
-    const double theta = pr * dphi;
-    const double rx = x - w * wnx, ry = y - w * wny;
-    const double xm = rx * 1e3, ym = ry * 1e3;
-    const double rvx = vxs - wv * wnx, rvy = vys - wv * wny;
-    // ---- visits (src/cutting.js:172-237)
-    for (int slot = lane; slot < NS; slot += 64) {
-      const int i = slot / per, r = slot - i * per;
-      double a[T_COUNT];
-#pragma unroll
-      for (int q = 0; q < T_COUNT; q++) a[q] = 0.0;
-      int eng = 0;
-      if (r < nz) {
-        const int k = r;
-        double phi = mcm::fmod_2pi((theta + i * pitch) - E.sLag[k]);
-        if (phi < 0) phi += TWO_PI;
-        double sn, cs;
-        mcm::sincos_js(phi, sn, cs);
-        const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-        const double rn = xm * nxg + ym * nyg + E.dR[i];
-        int pi = pr + i * spt;
-        if (pi >= spr) pi -= spr;
-        float* const cell = surf + (size_t)pi * MSL + k;
-        const double prev = (double)*cell;
-        const double feedH = fz * sn * E.sKap[k];
-        const int bin = (int)(phi * binScale);
-        const double hb = bin < MC_BINS ? (double)E.Hb[bin] : __builtin_nan("");
-        if (hb <= E.sZ[k] || sn <= 0) {
-          *cell = (float)rn;
-        } else {
-          double h = feedH + rn - prev;
-          if (h <= 0) {
-            *cell = (float)(prev - feedH);
-          } else {
-            *cell = (float)rn;
-            if (h > hMax) h = hMax;
-            const double se = mcOn ? mcm::pow_js(H_REF / (h > H_MIN ? h : H_MIN), mc) : 1.0;
-            const double bk = E.sB[k], sr = E.sR[k], sz = E.sZ[k];
-            const double Ft = Ktc * se * h * bk + Kte * bk;
-            double Fr = Krc * se * h * bk + Kre * bk;
-            const double Fa = Kac * se * h * bk + Kae * bk;
-            if (pdOn) Fr += pdK * (bk / db) / sr * (rvx * nxg + rvy * nyg);
-            const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-            a[T_FX] = -Ft * txg - Fr * nxg;
-            a[T_FY] = -Ft * tyg - Fr * nyg;
-            a[T_FZ] = -Fa;
-            a[T_TQ] = Ft * sr;
-            const double fr = mcm::hypot_js(Ft, Fr);
-            a[T_FS] = fr;
-            a[T_FR2Z] = fr * (sz - tipZ);
-            a[T_HB] = h * bk;
-            a[T_B] = bk;
-            a[T_RB] = sr * h * bk;
-            a[T_VOL] = h * bk * sr * dphi;
-            a[T_PHI] = phi;
-            a[T_Z] = sz;
-            eng = 1;
-          }
-        }
-      } else {
-        const int j = r - nz;
-        double phi = mcm::fmod_2pi(theta + i * pitch);
-        if (phi < 0) phi += TWO_PI;
-        const int rb = (int)(phi / TWO_PI * EBINS);
-        const double depth = rb < EBINS ? (double)E.Eb[rb * ERINGS + j] : __builtin_nan("");
-        if (!(depth <= 2e-4)) {
-          double sn, cs;
-          mcm::sincos_js(phi, sn, cs);
-          const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-          const double c = E.ringCos[j], ds = E.ringDs[j], rr = E.ringR[j], rz = E.ringZ[j];
-          const double h = (depth < fza ? depth : fza) * c;
-          const double se = mcOn ? mcm::pow_js(H_REF / (h > H_MIN ? h : H_MIN), mc) : 1.0;
-          const double Ft = Ktc * se * h * ds + Kte * ds;
-          const double Fn = Krc * se * h * ds + Kre * ds;
-          a[T_FX] = -Ft * txg;
-          a[T_FY] = -Ft * tyg;
-          a[T_FZ] = -(Fn * c);
-          a[T_TQ] = Ft * rr;
-          a[T_FS] = Ft;
-          a[T_FR2Z] = Ft * rz;
-          a[T_HB] = h * ds;
-          a[T_B] = ds;
-          a[T_RB] = rr * h * ds;
-          a[T_VOL] = h * ds * rr * dphi;
-          a[T_PHI] = phi;
-          a[T_Z] = tipZ + rz;
-          eng = 1;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < T_COUNT; q++) term[q][slot] = a[q];
-      teng[slot] = eng;
-    }
-    wave_sync();
-    // ---- serial sums in the reference's visit order
-    if (lane < 9) {
-      double acc = lane < 6 ? 0.0 : chain;
-      for (int q = 0; q < NS; q++) acc += term[lane][q];
-      if (lane < 6) res[lane] = acc; else chain = acc;
-    }
-    bool push = false;
-    double pvol = 0, pphi = 0, pz = 0;
-    bool engagedNow = false;
-    if (isTooth) {
-      int last = -1;
-      for (int r = 0; r < per; r++) {
-        const int q = tooth * per + r;
-        tvol += term[T_VOL][q];
-        if (teng[q]) last = q;
-      }
-      engagedNow = last >= 0;
-      if (engagedNow) {
-        tphi = term[T_PHI][last];
-        tz = term[T_Z][last];
-      } else if (tEng) {
-        if (tvol > 0) { push = true; pvol = tvol; pphi = tphi; pz = tz; }
-        tvol = 0;
-      }
-      tEng = engagedNow ? 1 : 0;
-    }
-    // chip events in tooth order (src/cutting.js:238-241)
-    const unsigned long long pmask = __ballot(push);
-    const unsigned long long emask = __ballot(engagedNow);
-    if (push) {
-      const int64_t at = nev + __popcll(pmask & ((1ull << lane) - 1));
-      if (at < ev_cap) {
-        double* q = E.ev + 3 * at;
-        q[0] = pvol; q[1] = pphi; q[2] = pz;
-      }
-    }
-    nev += __popcll(pmask);
-    wave_sync();
-    const double Fx = res[T_FX], Fy = res[T_FY], Fz = res[T_FZ], Tq = res[T_TQ], Fsum = res[T_FS],
-                 Fr2z = res[T_FR2Z];
-    // ---- modal integration, semi-implicit Euler (src/cutting.js:246-255)
-    x = 0; y = 0; vxs = 0; vys = 0;
-#pragma unroll
-    for (int j = 0; j < MC_MAX_MODES; j++) {
-      if (j < nmx) {
-        vxm[j] += (Fx - E.cx[j] * vxm[j] - E.kx[j] * qx[j]) / E.mx[j] * dt;
-        qx[j] += vxm[j] * dt;
-        x += qx[j];
-        vxs += vxm[j];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < MC_MAX_MODES; j++) {
-      if (j < nmy) {
-        vym[j] += (Fy - E.cy[j] * vym[j] - E.ky[j] * qy[j]) / E.my[j] * dt;
-        qy[j] += vym[j] * dt;
-        y += qy[j];
-        vys += vym[j];
-      }
-    }
-    // ---- wall mode (src/cutting.js:256-264)
-    if (wallOn) {
-      const double Fw = -(Fx * wnx + Fy * wny);
-      wv += (Fw - wc * wv - wk * w) / wm * dt;
-      w += wv * dt;
-      if (w > 5e-4) { w = 5e-4; if (wv > 0) wv = 0; }
-      else if (w < -2e-4) { w = -2e-4; if (wv < 0) wv = 0; }
-      st[S_SUMW] += w;
-      st[S_MAXW] = jmax(st[S_MAXW], __builtin_fabs(w));
-      wSum += w;
-    }
-    if (++pr == spr) pr = 0;
-    const int ph = pr % spt;
-    if (ph == 0) {
-      // the tooth now at phi = 0 generates the +v wall (src/cutting.js:265-272)
-      const double d0 = E.dR[(N - (pr / spt) % N) % N] * 1e-3;
-      if (lane == 0) {
-        E.rings[ts_i] = (float)(x - w * wnx);
-        E.rings[RN + ts_i] = (float)(y - w * wny);
-        E.rings[2 * RN + ws_i] = (float)(x - w * wnx + d0 * vx);
-        E.rings[3 * RN + ws_i] = (float)(y - w * wny + d0 * vy);
-      }
-      ts_i = (ts_i + 1) % RN; ts_count++;
-      ws_i = (ws_i + 1) % RN; ws_count++;
-    }
-    if (ph == piStep) {
-      const long long qn = (long long)mcm::round_js((JS_PI - pr * dphi) / pitch);
-      const int ip = (int)(((qn % N) + N) % N);
-      const double dp = E.dR[ip] * 1e-3;
-      if (lane == 0) {
-        E.rings[4 * RN + wsp_i] = (float)(x - w * wnx - dp * vx);
-        E.rings[5 * RN + wsp_i] = (float)(y - w * wny - dp * vy);
-      }
-      wsp_i = (wsp_i + 1) % RN; wsp_count++;
-    }
-    t += dt;
-    // ---- step statistics (src/cutting.js:279-287)
-    const double Fxy = mcm::hypot_js(Fx, Fy);
-    st[S_N] += 1;
-    st[S_SUMF] += Fxy; st[S_MAXF] = jmax(st[S_MAXF], Fxy);
-    st[S_SUMFX] += Fx; st[S_SUMFY] += Fy; st[S_SUMFZ] += Fz; st[S_MAXFZ] = jmax(st[S_MAXFZ], -Fz);
-    st[S_SUMT] += Tq; st[S_MAXT] = jmax(st[S_MAXT], Tq);
-    st[S_SUMX] += x; st[S_SUMY] += y;
-    st[S_MINX] = jmin(st[S_MINX], x); st[S_MAXX] = jmax(st[S_MAXX], x);
-    st[S_MINY] = jmin(st[S_MINY], y); st[S_MAXY] = jmax(st[S_MAXY], y);
-    if (Fsum > 0) { st[S_SUMZC] += Fr2z / Fsum * Fxy; st[S_WZC] += Fxy; }
-    st[S_ENGT] += (double)__popcll(emask);
-    if (++stepCount % decim == 0) {
-      if (lane == 0) {
-        E.hist[hist_i] = (float)Fx;
-        E.hist[HN + hist_i] = (float)Fy;
-        E.hist[2 * HN + hist_i] = (float)Fz;
-        E.hist[3 * HN + hist_i] = (float)((x - w * wnx) * 1e6);
-        E.hist[4 * HN + hist_i] = (float)((y - w * wny) * 1e6);
-      }
-      hist_i = (hist_i + 1) % HN; hist_count++;
-    }
-  }
-  wave_sync();
-  // ---- write the state back
-  if (lane == 6) E.st[S_SUMHB] = chain;
-  if (lane == 7) E.st[S_SUMB] = chain;
-  if (lane == 8) E.st[S_SUMRB] = chain;
-  if (isTooth) {
-    E.toothVol[tooth] = tvol; E.toothPhi[tooth] = tphi; E.toothZ[tooth] = tz; E.toothEngaged[tooth] = tEng;
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < S_COUNT; k++)
-      if (k != S_SUMHB && k != S_SUMB && k != S_SUMRB) E.st[k] = st[k];
-#pragma unroll
-    for (int j = 0; j < MC_MAX_MODES; j++) { E.qx[j] = qx[j]; E.vxm[j] = vxm[j]; E.qy[j] = qy[j]; E.vym[j] = vym[j]; }
-    E.x = E.x0 = x; E.y = E.y0 = y; E.vx = E.vx0 = vxs; E.vy = E.vy0 = vys;
-    E.t = t; E.pr = pr; E.w = w; E.wv = wv;
-    E.runMeanW = nSteps ? wSum / (double)nSteps : 0.0;
-    E.stepCount = stepCount;
-    E.hist_i = hist_i; E.hist_count = hist_count; E.ts_i = ts_i; E.ts_count = ts_count;
-    E.ws_i = ws_i; E.ws_count = ws_count; E.wsp_i = wsp_i; E.wsp_count = wsp_count;
-    E.nev = nev;
-  }
+  engine_steps(E, E.nsteps, L);
 }
 
 // chatterIndex() over the device rings (src/cutting.js:299-318), one lane
@@ -456,20 +110,8 @@ int zero_dev(float* p, int64_t n) {
   return MC_OK;
 }
 
-// the run constants run() derives from the material, wear and engagement (src/cutting.js:146-165)
 void run_constants(mc_engine* e) {
-  EngDev& h = e->h;
-  const bool nl = e->nonlinear != 0;
-  h.Ktc = e->mat.Ktc; h.Krc = e->mat.Krc; h.Kac = e->mat.Kac; h.Kae = e->mat.Kae;
-  h.Kte = e->wear ? e->wKte : e->mat.Kte;
-  h.Kre = e->wear ? e->wKre : e->mat.Kre;
-  h.mc = nl ? e->mat.mc : 0.0;
-  h.pdK = nl ? e->mat.Ci * h.db * 1e-3 / (h.omega * 1e-3) : 0.0;
-  const double TWO_PI = 2 * JS_PI;
-  h.binScale = MC_BINS / TWO_PI;
-  h.pitch = TWO_PI / h.N;
-  h.dphi = h.omega * h.dt;
-  h.hMax = jmax(0.1, 3 * h.fz);
+  eng_run_constants(e->h, e->mat, e->nonlinear != 0, e->wear ? e->wKte : e->mat.Kte, e->wear ? e->wKre : e->mat.Kre);
 }
 
 int ensure_events(mc_engine* e, int64_t nsteps) {
@@ -602,8 +244,7 @@ int mc_engine_configure(mc_engine* e, const mc_cell* c, double fnMax, mc_engine_
   if (int st = zero_dev(h.surf, (int64_t)h.maxSlices * h.spr)) return st;
   h.pr = 0;
   h.x = h.y = h.vx = h.vy = h.t = 0;
-  // setWork(null)
-  h.runMeanW = 0; h.wk = 0; h.wm = 1; h.wc = 0; h.w = 0; h.wv = 0; h.wnx = 1; h.wny = 0;
+  eng_set_work_null(h);
   h.fza = 0;
   for (int i = 0; i < MC_MAX_FLUTES; i++) { h.toothEngaged[i] = 0; h.toothVol[i] = h.toothPhi[i] = h.toothZ[i] = 0; }
   h.decim = (int)jmax(1, mcm::round_js(h.spt / 40.0));
@@ -635,7 +276,7 @@ int mc_engine_set_wear(mc_engine* e, double Kte, double Kre) {
 int mc_engine_set_work(mc_engine* e, const mc_wall* w) {
   if (int st = check_engine(e)) return st;
   EngDev& h = e->h;
-  if (!w) { h.runMeanW = 0; h.wk = 0; h.wm = 1; h.wc = 0; h.w = 0; h.wv = 0; h.wnx = 1; h.wny = 0; return MC_OK; }
+  if (!w) { eng_set_work_null(h); return MC_OK; }
   if (truthy(h.wk) && w->nx * h.wnx + w->ny * h.wny < 0.8) { h.w = 0; h.wv = 0; }
   h.wk = w->k * 1e3; h.wm = w->m; h.wc = 2 * w->zeta * std::sqrt(h.wk * w->m);
   h.wnx = w->nx; h.wny = w->ny;
@@ -648,26 +289,7 @@ int mc_engine_set_engagement(mc_engine* e, double ux, double uy, double fz, doub
   if (int st = check_engine(e)) return st;
   if (!e->configured) return mch::fail(MC_ERR_ARG, "setEngagement before configure");
   if ((sliceR == nullptr) != (sliceKap == nullptr)) return mch::fail(MC_ERR_ARG, "sliceR and sliceKap go together");
-  EngDev& h = e->h;
-  h.ux = ux; h.uy = uy; h.ex = -uy; h.ey = ux;
-  h.fz = fz; h.tipZ = tipZ;
-  const int nz = ap <= 0 ? 0 : (int)jmin(h.maxSlices, std::ceil(ap / h.dz));
-  h.nz = nz;
-  h.db = h.dz;
-  for (int k = 0; k < nz; k++) {
-    h.sB[k] = jmin(h.dz, ap - k * h.dz);
-    const double zr = k * h.dz + 0.5 * h.sB[k];
-    h.sZ[k] = tipZ + zr;
-    double r = h.R, kap = 1;
-    if (sliceR) { r = sliceR[k]; kap = sliceKap[k]; }
-    else if (e->type == MC_TOOL_BALL && zr < h.R) {
-      const double q = h.R - zr;
-      r = std::sqrt(jmax(1e-6, h.R * h.R - q * q));
-      kap = r / h.R;
-    }
-    h.sR[k] = r; h.sKap[k] = kap;
-    h.sLag[k] = zr * h.tanB / h.R;
-  }
+  const int nz = eng_set_engagement(e->h, ux, uy, fz, tipZ, ap, sliceR, sliceKap, e->type == MC_TOOL_BALL);
   e->engaged = true;
   if (nz_out) *nz_out = nz;
   return MC_OK;
@@ -676,21 +298,7 @@ int mc_engine_set_engagement(mc_engine* e, double ux, double uy, double fz, doub
 // setEndEngagement(fza, rOut, zAt), src/cutting.js:130-143
 int mc_engine_set_end_engagement(mc_engine* e, double fza, double rOut, const double* rings) {
   if (int st = check_engine(e)) return st;
-  EngDev& h = e->h;
-  h.fza = fza > 0 ? fza : 0;
-  h.endR = rOut;
-  const double dr = rOut / ERINGS;
-  for (int j = 0; j < ERINGS; j++) {
-    if (rings) {
-      h.ringR[j] = rings[j]; h.ringCos[j] = rings[ERINGS + j];
-      h.ringZ[j] = rings[2 * ERINGS + j]; h.ringDs[j] = rings[3 * ERINGS + j];
-    } else {
-      const double r = (j + 0.5) * dr;
-      const double slope = (0.0 - 0.0) / (0.5 * dr);
-      const double c = jmax(0.22, 1 / std::sqrt(1 + slope * slope));
-      h.ringR[j] = r; h.ringCos[j] = c; h.ringDs[j] = dr / c; h.ringZ[j] = 0;
-    }
-  }
+  eng_set_end_engagement(e->h, fza, rOut, rings, 0, 0.0);
   return MC_OK;
 }
 
@@ -828,3 +436,23 @@ int mc_engine_read_rings(const mc_engine* e, float* hist, float* ts, float* ws, 
 }
 
 }  // extern "C"
+
+// ---- for the sub-step pipeline (simrun.hip), called with the library lock held
+EngDev* mce_engine_mirror(mc_engine* e) { return e && e->configured ? &e->h : nullptr; }
+EngDev* mce_engine_device(mc_engine* e) { return e ? e->d : nullptr; }
+
+// the engine's run constants set and its record uploaded for a run on the device
+int mce_engine_prepare(mc_engine* e, int64_t nsteps) {
+  e->h.db = e->h.dz;   // setEngagement's db, which the process damping constant uses
+  run_constants(e);
+  e->h.nsteps = nsteps;
+  e->h.nev = 0;
+  return upload(e);
+}
+
+// the record read back after such a run; the pipeline has set an engagement
+int mce_engine_finish(mc_engine* e) {
+  e->ev_fresh = false;
+  e->engaged = true;
+  return download(e);
+}

@@ -5,6 +5,8 @@
 
 #include <mutex>
 
+#include "../../include/mcsim.h"
+
 namespace mch {
 int fail(int code, const char* fmt, ...);           // sets mc_last_error(), returns code
 int init();                                         // the implicit mc_init(0)
@@ -12,6 +14,16 @@ hipStream_t stream();                               // the library's stream (def
 std::mutex& lock();                                 // serialises calls on the device
 int cus();                                          // compute units of the device
 void set_last_ms(double ms);                        // mc_last_kernel_time()
+
+// a stock's device arrays, for kernels of other source files that write the stock
+struct StockView {
+  int nx, ny;
+  double hx, hy, zTop, zBottom, band;
+  float *H, *err, *wallD, *markS, *markF;
+  int32_t *lineCut, *lineWall, *rect;   // rect: the dirty rectangle {i0, j0, i1, j1}
+};
+// fills v; the caller is about to change the heights, so the block maxima are marked stale
+int stock_view(mc_stock* s, StockView* v);
 }  // namespace mch
 
 #define MCH_HIPCHK(call)                                                                    \

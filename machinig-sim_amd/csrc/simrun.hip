@@ -1,0 +1,311 @@
+// simrun.hip — the simulator's sub-step loop (src/sim.js:351-449) on the device.
+//
+// One wavefront per program runs its whole list of sub-steps: per sub-step it samples the
+// actual stock on the cutter periphery (fillEngagement, src/sim.js:301-319) and, while
+// descending, under the end face (fillEndEngagement, :324-341) into the engine's maps,
+// takes the axial depth, sets the engine up and runs the sub-step's share of steps
+// (engine_steps, the reference's run) or halves the vibration out of the cut, and stamps
+// the commanded stock at p and the actual stock at p plus the deflection (Stock.stamp,
+// src/stock.js:118-164), before the next sub-step reads the actual stock again.  Nothing
+// returns to the host between sub-steps; many programs share one launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mcsim.h"
+#include "engine_dev.h"
+#include "mc_host.h"
+#include "mc_math.h"
+
+// the library's engine handle (engine.hip); its first member is the host mirror
+struct mc_engine;
+
+using namespace mce;
+
+namespace {
+
+struct StockDev {
+  float *H, *err, *wallD, *markS, *markF;
+  int32_t *lineCut, *lineWall, *rect;
+  int32_t nx, ny;
+  double hx, hy, zTop, zB, band;
+};
+
+struct ProgDev {
+  EngDev* eng;
+  StockDev act, ide;
+  const mc_substep* sub;
+  mc_substep_out* out;
+  double R, fluteLen, rw, share;
+  int32_t nsub, ball, N, pad;
+};
+
+// Stock.sample (src/stock.js:85-92)
+__device__ __forceinline__ double sample_at(const StockDev& S, double x, double y) {
+  const double fx = x / S.hx, fy = y / S.hy;
+  if (fx < 0 || fy < 0 || fx > S.nx - 1 || fy > S.ny - 1) return -__builtin_inf();
+  int i = (int)fx, j = (int)fy;
+  if (i > S.nx - 2) i = S.nx - 2;
+  if (j > S.ny - 2) j = S.ny - 2;
+  const double tx = fx - i, ty = fy - j;
+  const float* H = S.H;
+  const double a = H[j * S.nx + i], b = H[j * S.nx + i + 1], c = H[(j + 1) * S.nx + i],
+               d = H[(j + 1) * S.nx + i + 1];
+  return (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = jmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Stock.stamp (src/stock.js:118-164) by one wavefront, flat or ball without a profile table:
+// the nodes of the box are independent, so the lanes take them in any order.  Returns the
+// removed volume [mm^3] (its sum is a wave reduction, not the reference's node order).
+__device__ double stamp_wave(const StockDev& S, double cx, double cy, double tipZ, double R, bool ball,
+                             double errVal, double mux, double muy, double mfeed, int32_t line) {
+  const int lane = threadIdx.x & 63;
+  if (tipZ >= S.zTop) return 0;
+  const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band), Rw2 = Rw * Rw;
+  const double fi0 = jmax(0, __builtin_floor((cx - Rw) / S.hx)), fi1 = jmin(S.nx - 1, __builtin_ceil((cx + Rw) / S.hx));
+  const double fj0 = jmax(0, __builtin_floor((cy - Rw) / S.hy)), fj1 = jmin(S.ny - 1, __builtin_ceil((cy + Rw) / S.hy));
+  if (!(fi0 <= fi1) || !(fj0 <= fj1)) return 0;
+  const int i0 = (int)fi0, i1 = (int)fi1, j0 = (int)fj0, j1 = (int)fj1;
+  const double R2 = R * R;
+  const double mf = mfeed > 0 && !ball ? mfeed : 0;
+  const double zRim = tipZ + (ball ? R : 0);
+  const int W = i1 - i0 + 1, total = W * (j1 - j0 + 1);
+  double vol = 0;
+  bool touched = false;
+  for (int q = lane; q < total; q += 64) {
+    const int j = j0 + q / W, i = i0 + q % W;
+    const double dy = j * S.hy - cy, dx = i * S.hx - cx;
+    const double r2 = dx * dx + dy * dy;
+    const int64_t k = (int64_t)j * S.nx + i;
+    if (r2 >= R2 - 0.003 * R) {
+      if (r2 <= Rw2 && (double)S.H[k] > zRim) {
+        const double d = __builtin_sqrt(r2) - R;
+        if (d < (double)S.wallD[k]) { S.wallD[k] = (float)d; S.lineWall[k] = line; touched = true; }
+      }
+      continue;
+    }
+    const double z = ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
+    double h = S.H[k];
+    if (h > z) {
+      vol += h - z;
+      const float nh = (float)jmax(z, S.zB);
+      S.H[k] = nh;
+      S.err[k] = (float)errVal;
+      S.wallD[k] = __builtin_inff();
+      S.lineCut[k] = line;
+      touched = true;
+      h = nh;
+    }
+    if (mf != 0 && h > z - 0.002) {
+      const double xs = i * S.hx, ys = j * S.hy, p = -dx * muy + dy * mux;
+      S.markS[k] = (float)(xs * mux + ys * muy + __builtin_sqrt(jmax(0, R2 - p * p)));
+      S.markF[k] = (float)mf;
+      touched = true;
+    }
+  }
+  if (__any(touched) && lane == 0) {
+    atomicMin(&S.rect[0], i0); atomicMin(&S.rect[1], j0);
+    atomicMax(&S.rect[2], i1); atomicMax(&S.rect[3], j1);
+  }
+  return wave_sum(vol) * S.hx * S.hy;
+}
+
+__global__ __launch_bounds__(64) void sim_run(const ProgDev* progs) {
+  __shared__ EngLds L;
+  const ProgDev& P = progs[blockIdx.x];
+  EngDev& E = *P.eng;
+  const int lane = threadIdx.x;
+  const double PI = 3.141592653589793;
+  const bool ball = P.ball != 0;
+  const double R = P.R;
+  const double cell = jmax(P.act.hx, P.act.hy);
+  double fzLast = 0;   // this.fz: the last cutting sub-step's feed per tooth
+  for (int k = 0; k < P.nsub; k++) {
+    const mc_substep s = P.sub[k];
+    const double vx = -s.uy, vy = s.ux;
+    // ---- fillEngagement (src/sim.js:301-319); a vertical move keeps the previous map
+    double hmax = -__builtin_inf();
+    if (!(s.dxy < 1e-6)) {
+      const double r = R - 0.5 * cell, ax = s.x + s.ux * cell, ay = s.y + s.uy * cell;
+      for (int b = lane; b < MC_BINS; b += 64) {
+        const double phi = (b + 0.5) / MC_BINS * 2 * PI;
+        double sn, cs;
+        mcm::sincos_js(phi, sn, cs);
+        const double h = sample_at(P.act, ax + r * (sn * s.ux + cs * vx), ay + r * (sn * s.uy + cs * vy));
+        const float f = sn > 0.02 ? (float)h : -__builtin_inff();
+        E.Hb[b] = f;
+        hmax = jmax(hmax, (double)f);
+      }
+      hmax = wave_max(hmax);
+    }
+    const double ap = jmin(P.fluteLen, jmax(0, hmax - s.z));
+    // ---- fillEndEngagement (src/sim.js:324-341) while descending
+    const bool haveEnd = s.fza > 0;
+    double dmax = 0, rOut = 0;
+    if (haveEnd) {
+      rOut = R - 0.5 * cell;
+      for (int q = lane; q < MC_END_BINS * MC_END_RINGS; q += 64) {
+        const int b = q / MC_END_RINGS, j = q - b * MC_END_RINGS;
+        const double phi = (b + 0.5) / MC_END_BINS * 2 * PI;
+        double sn, cs;
+        mcm::sincos_js(phi, sn, cs);
+        const double ex = sn * s.ux + cs * vx, ey = sn * s.uy + cs * vy;
+        const double r = (j + 0.5) / MC_END_RINGS * rOut;
+        const double zEdge = s.z + eng_zat(ball, R, r);
+        const double d = jmax(0, sample_at(P.act, s.x + r * ex, s.y + r * ey) - zEdge);
+        E.Eb[q] = (float)d;
+        dmax = jmax(dmax, d);
+      }
+      dmax = wave_max(dmax);
+    }
+    const bool endCut = haveEnd && dmax > 2e-4;
+    const bool cut = (ap > 0.01 || endCut) && !s.rapid;
+    int n = 0;
+    wave_sync();
+    if (cut) {
+      // setEngagement, setEndEngagement, setWork(null) (src/sim.js:403-408), on lane 0
+      if (lane == 0) {
+        eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball);
+        eng_set_end_engagement(E, endCut ? s.fza : 0, haveEnd ? rOut : 0, nullptr, ball, R);
+        eng_set_work_null(E);
+        E.hMax = jmax(0.1, 3 * E.fz);
+      }
+      wave_sync();
+      // the sub-step's dynamics: its own time, capped by the frame budget's share (:410-412)
+      const double want = jmax(1, mcm::round_js(s.dtm / E.dt));
+      n = (int)jmax(1, jmin(want, __builtin_floor(P.share)));
+      engine_steps(E, n, L);
+      fzLast = s.fzEff;
+    } else {
+      // out of the cut the vibration decays (:427-428)
+      if (lane == 0) { E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5; }
+      wave_sync();
+    }
+    // ---- deflection, then the commanded and the actual stamp (:433-440)
+    const double dx = (E.x - E.w * E.wnx) * 1e3, dy = (E.y - E.w * E.wny) * 1e3;
+    const double errVal = mcm::hypot_js(E.x, E.y) * 1e6;
+    const double maxF = cut ? E.runMaxF : 0.0;
+    stamp_wave(P.ide, s.x, s.y, s.z, R, ball, 0.0, 0.0, 0.0, 0.0, s.line);
+    wave_sync();
+    const double vol = stamp_wave(P.act, s.x + dx, s.y + dy, s.z + (ball ? P.rw : 0), R - P.rw, ball, errVal, s.ux, s.uy,
+                                  fzLast * P.N, s.line);
+    wave_sync();
+    if (P.out && lane == 0) {
+      mc_substep_out& o = P.out[k];
+      o.dx = dx; o.dy = dy; o.ap = ap; o.errVal = errVal; o.vol = vol; o.maxF = maxF; o.nsteps = n; o.cut = cut ? 1 : 0;
+    }
+  }
+}
+
+}  // namespace
+
+// engine.hip's accessors of a handle's device record and host mirror
+EngDev* mce_engine_device(mc_engine* e);
+EngDev* mce_engine_mirror(mc_engine* e);
+int mce_engine_prepare(mc_engine* e, int64_t nsteps);
+int mce_engine_finish(mc_engine* e);
+
+extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
+  if (n < 0 || (n > 0 && !progs)) return mch::fail(MC_ERR_ARG, "null programs or negative count");
+  if (n == 0) return MC_OK;
+  std::vector<ProgDev> pd((size_t)n);
+  std::vector<const void*> seen;
+  for (int p = 0; p < n; p++) {
+    const mc_sim_program& g = progs[p];
+    if (!g.actual || !g.ideal || !g.engine || g.actual == g.ideal)
+      return mch::fail(MC_ERR_ARG, "program %d: needs an engine and two distinct stocks", p);
+    if (g.nsub < 0 || (g.nsub > 0 && !g.sub)) return mch::fail(MC_ERR_ARG, "program %d: null sub-steps", p);
+    if (g.type != MC_TOOL_FLAT && g.type != MC_TOOL_BALL) return mch::fail(MC_ERR_ARG, "program %d: flat or ball tools", p);
+    if (!(g.R > 0) || !(g.rw >= 0) || !(g.rw < g.R) || !(g.fluteLen > 0) || !(g.share >= 1))
+      return mch::fail(MC_ERR_ARG, "program %d: invalid radius, wear, flute length or share", p);
+    for (const void* q : {(const void*)g.actual, (const void*)g.ideal, (const void*)g.engine})
+      for (const void* o : seen)
+        if (o == q) return mch::fail(MC_ERR_ARG, "program %d shares a stock or an engine with another", p);
+    seen.push_back(g.actual); seen.push_back(g.ideal); seen.push_back(g.engine);
+    for (int k = 0; k < g.nsub; k++) {
+      const mc_substep& s = g.sub[k];
+      if (!std::isfinite(s.x) || !std::isfinite(s.y) || !std::isfinite(s.z) || !std::isfinite(s.ux) ||
+          !std::isfinite(s.uy) || !(s.dtm >= 0) || !std::isfinite(s.fzEff) || !std::isfinite(s.fza) || !(s.dxy >= 0))
+        return mch::fail(MC_ERR_ARG, "program %d sub-step %d: non-finite input", p, k);
+    }
+    EngDev* m = mce_engine_mirror(g.engine);
+    if (!m) return mch::fail(MC_ERR_ARG, "program %d: the engine is not configured", p);
+    if (m->N * (MC_MAX_SLICES + MC_END_RINGS) > MC_MAX_FLUTES * (MC_MAX_SLICES + MC_END_RINGS))
+      return mch::fail(MC_ERR_LIMIT, "program %d: too many flutes", p);
+    ProgDev& d = pd[(size_t)p];
+    mch::StockView a, i;
+    if (int st = mch::stock_view(g.actual, &a)) return st;
+    if (int st = mch::stock_view(g.ideal, &i)) return st;
+    if (a.nx < 2 || a.ny < 2 || i.nx < 2 || i.ny < 2) return mch::fail(MC_ERR_ARG, "program %d: stocks need 2 x 2 nodes", p);
+    d.act = StockDev{a.H, a.err, a.wallD, a.markS, a.markF, a.lineCut, a.lineWall, a.rect, a.nx, a.ny, a.hx, a.hy,
+                     a.zTop, a.zBottom, a.band};
+    d.ide = StockDev{i.H, i.err, i.wallD, i.markS, i.markF, i.lineCut, i.lineWall, i.rect, i.nx, i.ny, i.hx, i.hy,
+                     i.zTop, i.zBottom, i.band};
+    d.sub = nullptr; d.out = nullptr;
+    d.R = g.R; d.fluteLen = g.fluteLen; d.rw = g.rw; d.share = g.share;
+    d.nsub = g.nsub; d.ball = g.type == MC_TOOL_BALL; d.N = m->N; d.pad = 0;
+  }
+  std::lock_guard<std::mutex> lk(mch::lock());
+  hipStream_t q = mch::stream();
+  // sub-steps in, results out: one device buffer for every program
+  size_t nin = 0;
+  for (int p = 0; p < n; p++) nin += (size_t)progs[p].nsub;
+  mc_substep* dsub = nullptr;
+  mc_substep_out* dout = nullptr;
+  ProgDev* dprog = nullptr;
+  auto release = [&]() {
+    if (dsub) (void)hipFree(dsub);
+    if (dout) (void)hipFree(dout);
+    if (dprog) (void)hipFree(dprog);
+  };
+  if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
+      hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
+      hipMalloc(&dprog, sizeof(ProgDev) * (size_t)n) != hipSuccess) {
+    release();
+    return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
+  }
+  size_t off = 0;
+  int st = MC_OK;
+  for (int p = 0; p < n && !st; p++) {
+    const mc_sim_program& g = progs[p];
+    pd[(size_t)p].sub = dsub + off;
+    pd[(size_t)p].out = dout + off;
+    if (g.nsub > 0 && hipMemcpyAsync(dsub + off, g.sub, sizeof(mc_substep) * (size_t)g.nsub, hipMemcpyHostToDevice, q) != hipSuccess)
+      st = mch::fail(MC_ERR_HIP, "sub-step upload");
+    off += (size_t)g.nsub;
+    // the engine's record with its run constants, uploaded; chip events are not kept here
+    if (!st) st = mce_engine_prepare(g.engine, 0);
+    pd[(size_t)p].eng = mce_engine_device(g.engine);
+  }
+  if (!st && hipMemcpyAsync(dprog, pd.data(), sizeof(ProgDev) * (size_t)n, hipMemcpyHostToDevice, q) != hipSuccess)
+    st = mch::fail(MC_ERR_HIP, "program upload");
+  if (!st) {
+    hipLaunchKernelGGL(sim_run, dim3(n), dim3(64), 0, q, (const ProgDev*)dprog);
+    if (hipGetLastError() != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline launch failed");
+  }
+  off = 0;
+  for (int p = 0; p < n && !st; p++) {
+    const mc_sim_program& g = progs[p];
+    if (g.out && g.nsub > 0 &&
+        hipMemcpyAsync(g.out, dout + off, sizeof(mc_substep_out) * (size_t)g.nsub, hipMemcpyDeviceToHost, q) != hipSuccess)
+      st = mch::fail(MC_ERR_HIP, "result download");
+    off += (size_t)g.nsub;
+    if (!st) st = mce_engine_finish(g.engine);
+  }
+  if (!st && hipStreamSynchronize(q) != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline failed");
+  release();
+  return st;
+}

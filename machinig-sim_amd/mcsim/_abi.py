@@ -115,6 +115,23 @@ class ChipEvent(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("vol", "phi", "z", "ux", "uy", "vx", "vy")]
 
 
+class Substep(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "ux", "uy", "dtm", "fzEff", "fza", "dxy")] + \
+               [("rapid", C.c_int32), ("line", C.c_int32)]
+
+
+class SubstepOut(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dx", "dy", "ap", "errVal", "vol", "maxF")] + \
+               [("nsteps", C.c_int32), ("cut", C.c_int32)]
+
+
+class SimProgram(C.Structure):
+    _fields_ = [("actual", C.c_void_p), ("ideal", C.c_void_p), ("engine", C.c_void_p),
+                ("sub", C.POINTER(Substep)), ("out", C.POINTER(SubstepOut)),
+                ("R", C.c_double), ("fluteLen", C.c_double), ("rw", C.c_double), ("share", C.c_double),
+                ("nsub", C.c_int32), ("type", C.c_int32)]
+
+
 MC_STOCK_B = 8
 MC_LUT_DR = 0.02
 MC_TIP_FLAT, MC_TIP_BALL, MC_TIP_BULL, MC_TIP_EDGE = 0, 1, 2, 3

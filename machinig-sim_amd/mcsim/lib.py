@@ -11,7 +11,7 @@ import numpy as np
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
                    SearchTuning, EngageQuery, EngageResult, EngageSpec, cell_array, MC_OK, MC_SURR_IN,
                    MC_STOCK_B, EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST,
-                   MC_ENGINE_RING)
+                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -30,6 +30,7 @@ EXPORTS = [
     "mc_engine_set_wear", "mc_engine_set_work", "mc_engine_set_engagement", "mc_engine_set_end_engagement",
     "mc_engine_set_maps", "mc_engine_get_state", "mc_engine_set_state", "mc_engine_run", "mc_engine_run_batch",
     "mc_engine_read_stats", "mc_engine_chip_events", "mc_engine_chatter_index", "mc_engine_read_rings",
+    "mc_sim_run",
 ]
 
 _LIB = None
@@ -109,6 +110,7 @@ def load():
     L.mc_engine_chip_events.argtypes = [P, P, C.c_int64, P]
     L.mc_engine_chatter_index.argtypes = [P, P]
     L.mc_engine_read_rings.argtypes = [P, P, P, P, P]
+    L.mc_sim_run.argtypes = [C.POINTER(SimProgram), C.c_int]
     L.mc_set_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     L.mc_get_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     _LIB = L
@@ -289,6 +291,25 @@ def engine_run_batch(engines, nsteps):
     ptrs = (C.c_void_p * max(1, n))(*[e.p.value for e in engines])
     steps = np.ascontiguousarray(nsteps, dtype=np.int64)
     _check(load().mc_engine_run_batch(ptrs, steps.ctypes.data, n))
+
+
+def sim_run(programs):
+    """the simulator's sub-step loop on the device for several programs in one launch.
+    programs: dicts with actual, ideal (Stock), engine (Engine), sub (Substep array), R,
+    fluteLen, rw, share, type (MC_TOOL_*); returns a SubstepOut array per program"""
+    n = len(programs)
+    arr = (SimProgram * max(1, n))()
+    outs = []
+    for i, g in enumerate(programs):
+        sub = g["sub"]
+        out = (SubstepOut * max(1, len(sub)))()
+        outs.append(out)
+        arr[i] = SimProgram(g["actual"].p.value, g["ideal"].p.value, g["engine"].p.value,
+                            C.cast(sub, C.POINTER(Substep)), C.cast(out, C.POINTER(SubstepOut)),
+                            float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g["share"]),
+                            len(sub), int(g["type"]))
+    _check(load().mc_sim_run(arr, n))
+    return [o[:len(g["sub"])] for o, g in zip(outs, programs)]
 
 
 class Stock:

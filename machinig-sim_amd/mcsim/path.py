@@ -96,3 +96,43 @@ def as_stamp_array(rec):
     rec = np.ascontiguousarray(rec)
     arr = (Stamp * len(rec)).from_buffer_copy(rec.tobytes())
     return arr
+
+
+def sim_substeps(x, y, z, rapid, line, F, S, N, hx):
+    """The sub-steps Simulator.step visits over a discretised path with an unbounded frame
+    budget (src/sim.js:363-381,389-405), as mc_substep records: position, the feed direction
+    the simulator holds (this.u, starting at [1, 0]), machine time ds / v, feed per tooth
+    along the xy path, axial feed per tooth while descending, the move's xy length, rapid
+    flag and program line.  F and S are the path points' feed [mm/min] and speed [rpm]
+    (overrides at 1); N the tool's flutes."""
+    from ._abi import Substep
+    step = stamp_step(hx)
+    rows = []
+    u = (1.0, 0.0)
+    for k in range(1, len(x)):
+        ax, ay, az, bx, by, bz = x[k - 1], y[k - 1], z[k - 1], x[k], y[k], z[k]
+        seg_len = hypot_js(bx - ax, by - ay, bz - az)
+        v = (15000 if rapid[k] else F[k]) / 60
+        ds_max = 1.0 if rapid[k] else step
+        s = 0.0
+        while True:
+            ds = min(ds_max, seg_len - s)
+            if seg_len - s <= 1e-9:
+                break
+            dtm = ds / v
+            s += ds
+            f = s / seg_len if seg_len > 0 else 1.0
+            px, py, pz = ax + (bx - ax) * f, ay + (by - ay) * f, az + (bz - az) * f
+            dxy = hypot_js(bx - ax, by - ay)
+            if dxy > 1e-6:
+                u = ((bx - ax) / dxy, (by - ay) / dxy)
+            fza = fz_eff = 0.0
+            if not rapid[k]:   # a rapid never cuts, so its feeds are not used
+                Sk, Fk = S[k], F[k]
+                fza = (Fk / (Sk * N)) * ((az - bz) / seg_len) if (bz < az and Sk >= 1) else 0.0
+                fz_eff = (Fk / (Sk * N)) * (dxy / seg_len) if dxy > 1e-6 else 0.0
+            rows.append(Substep(px, py, pz, u[0], u[1], dtm, fz_eff, fza, dxy, 1 if rapid[k] else 0, int(line[k])))
+            if s >= seg_len - 1e-9:
+                break
+    arr = (Substep * max(1, len(rows)))(*rows)
+    return arr, len(rows)

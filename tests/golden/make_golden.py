@@ -20,6 +20,9 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_spindle.json   modesAt / tipFactor / bearingFactor / speedDependent
   golden_enginex.json   per-step forces and displacements of the engine as the simulator
                         drives it (feed direction, stock map, runout, fnMax)
+  golden_simx.json      Simulator runs with a rigid workpiece and the linear force law: the
+                        path with feeds and speeds, the engine's configure / wear arguments,
+                        every stamp, final maps
   golden_englog.json    the engine's whole call sequence inside simulator runs (configure,
                         setWear, setWork, setEngagement, setEndEngagement, run, takeStats,
                         chatterIndex) with its state, stats, rings and chip events after
@@ -206,6 +209,9 @@ def build_cases(quick):
                 cell=0.25, flex=True)
     lin = [dict(c, nonlinear=False) for c in cases["sim"] + [ribc]]
     cases["englog"] = lin + [dict(cases["sim"][2]), dict(ribc)]
+    # rigid workpiece, linear force law, flat and ball tools: the device sub-step pipeline
+    cases["simx"] = [dict(cases["sim"][0]), dict(cases["sim"][1]), dict(ribc),
+                     dict(cases["sim"][0], tool=dict(type="ball", D=8.0))]
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]
@@ -316,7 +322,7 @@ def main():
                 m["k"], m["fn"] = tm["k"], tm["fn"]
             md["k"], md["fn"] = tm["k"], tm["fn"]
     keys = ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
-            "enginex", "englog")
+            "enginex", "englog", "simx")
     if "--only" in sys.argv:
         keys = tuple(sys.argv[sys.argv.index("--only") + 1].split(","))
         # engage cases read the stock cases

@@ -194,6 +194,64 @@ if (cases.sim) {
   });
 }
 
+// whole simulator runs with a rigid workpiece and the linear force law, for the device
+// sub-step pipeline: the path with its feeds and speeds, the engine's configure and wear
+// arguments, every stamp issued to the actual and the commanded stock, and the final maps
+if (cases.simx) {
+  const { Simulator } = await import(REF + 'sim.js');
+  const { modeSet } = await import(REF + 'frf.js');
+  out.simx = cases.simx.map(({ program, tool, stock, cell }) => {
+    const sim = new Simulator();
+    sim.cfg.cell = cell;
+    sim.cfg.workpieceFlex = false;
+    sim.cfg.cuttingNonlinear = false;
+    Object.assign(sim.cfg.tool, tool);
+    sim.load(program, { stock, origin: { x: 0, y: 0, z: 0 } });
+    const rec = { actual: [], ideal: [] };
+    const wrap = (st, o) => {
+      const f = st.stamp.bind(st);
+      st.stamp = (cx, cy, tipZ, R, type, errVal, mark, lut) => {
+        const v = f(cx, cy, tipZ, R, type, errVal, mark, lut);
+        o.push([cx, cy, tipZ, R, type === 'ball' ? 1 : 0, errVal, mark ? mark.f : 0, st.curLine, v]);
+        return v;
+      };
+    };
+    wrap(sim.stock, rec.actual); wrap(sim.ideal, rec.ideal);
+    const e = sim.engine, cfgs = [], wears = [], runs = [];
+    const fc = e.configure.bind(e), fw = e.setWear.bind(e), fr = e.run.bind(e);
+    e.configure = (t, modal, mat, S, fnMax = 0) => {
+      fc(t, modal, mat, S, fnMax);
+      const ms = modeSet(modal), md = L => L.map(m => [m.k, m.fn, m.zeta]);
+      cfgs.push([rec.actual.length, t.N, t.R, t.helixDeg, t.type, t.fluteLen, t.nonlinear === false ? 0 : 1, t.runout || 0,
+        t.runoutPhase || 0, S, fnMax, md(ms.x), md(ms.y),
+        [mat.Ktc, mat.Krc, mat.Kte, mat.Kre, mat.Kac, mat.Kae, mat.mc != null ? mat.mc : 0, mat.Ci != null ? mat.Ci : 0]]);
+    };
+    e.setWear = edge => {
+      fw(edge);
+      const last = wears[wears.length - 1];
+      if (!last || last[1] !== edge.Kte || last[2] !== edge.Kre) wears.push([rec.actual.length, edge.Kte, edge.Kre]);
+    };
+    e.run = n => { fr(n); runs.push([rec.actual.length, n]); };
+    let frames = 0;
+    while (!sim.done && frames < 100) { sim.step(1e6); frames++; }
+    const maps = s => ({ H: z32(Float32Array, s.H), err: z32(Float32Array, s.err), wallD: z32(Float32Array, s.wallD),
+      lineCut: z32(Int32Array, s.lineCut), lineWall: z32(Int32Array, s.lineWall), markS: z32(Float32Array, s.markS),
+      markF: z32(Float32Array, s.markF), removed: s.removed });
+    const col = (rows, k) => rows.map(r => r[k]);
+    const pack = rows => ({ cx: b64f64(col(rows, 0)), cy: b64f64(col(rows, 1)), tipZ: b64f64(col(rows, 2)),
+      R: b64f64(col(rows, 3)), type: col(rows, 4), errVal: b64f64(col(rows, 5)), f: b64f64(col(rows, 6)),
+      line: col(rows, 7), vol: b64f64(col(rows, 8)) });
+    const P = sim.path, t = sim.activeTool;
+    const clr = t.clearanceDeg != null ? t.clearanceDeg : (sim.cfg.tool.clearanceDeg != null ? sim.cfg.tool.clearanceDeg : 10);
+    return { frames, nx: sim.stock.nx, ny: sim.stock.ny, hx: sim.stock.hx, hy: sim.stock.hy, zBottom: sim.stock.zBottom,
+      toolR: t.R, toolN: t.N, fluteLen: t.fluteLen, type: t.type, VB: sim.VB || 0, clearanceDeg: clr,
+      path: { x: b64f64(P.map(p => p.x)), y: b64f64(P.map(p => p.y)), z: b64f64(P.map(p => p.z)),
+        rapid: P.map(p => (p.rapid ? 1 : 0)), line: P.map(p => p.line), F: P.map(p => p.F), S: P.map(p => p.S) },
+      cfgs, wears, runs, actual: pack(rec.actual), ideal: pack(rec.ideal), stock: maps(sim.stock), commanded: maps(sim.ideal) };
+  });
+  console.error('simx', (Date.now() - t0) / 1e3, 's');
+}
+
 // the engine's own call sequence inside whole simulator runs: every configure, setWear,
 // setWork, setEngagement, setEndEngagement, run, takeStats and chatterIndex the simulator
 // makes, with the maps and the state it set before each run and the engine's state, stats,

@@ -319,3 +319,19 @@ def replay_englog(eng, res, max_runs=None, rtol=0.0):
             n = g.send(None)
     except StopIteration as stop:
         return bad, stop.value
+
+
+def simx_program(res):
+    """the sub-step schedule of a recorded rigid-workpiece simulator run (golden_simx.json)"""
+    from mcsim import path
+    P = res["path"]
+    x, y, z = f64(P["x"]), f64(P["y"]), f64(P["z"])
+    return path.sim_substeps(x, y, z, P["rapid"], P["line"], P["F"], P["S"], res["toolN"], res["hx"])
+
+
+def simx_cell(res):
+    """the engine's configure arguments recorded in a simx run"""
+    op = res["cfgs"][0]
+    _, N, R, hd, typ, fl, nl, ro, rph, S, fnMax, mxs, mys, mat = op
+    return make_cell(dict(N=N, R=R, helixDeg=hd, type=typ, fluteLen=fl, runout=ro, runoutPhase=rph),
+                     mxs, mys, dict(zip(MAT_ORDER, mat)), S, (0.0, 0.0), 0.0, 1, nl), fnMax

@@ -1,0 +1,63 @@
+"""The simulator's sub-step loop on the device (mc_sim_run: engagement, engine run and both
+stamps per sub-step, src/sim.js:351-449) against whole runs of the reference Simulator with a
+rigid workpiece and the linear force law (golden_simx.json): the deflected stamp positions,
+their vibration values, the engine's step counts, and every map of the actual and the
+commanded stock bit for bit; the removed volume within 1e-12 (its sum order differs).  The
+four programs run in one launch, one wavefront each."""
+import numpy as np
+import pytest
+
+import golden_util as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _maps(stock):
+    H = stock.H()
+    err, wallD, lineCut, lineWall = stock.side()
+    markS, markF = stock.marks()
+    return dict(H=H, err=err, wallD=wallD, lineCut=lineCut, lineWall=lineWall, markS=markS, markF=markF)
+
+
+def test_sim_pipeline_matches_reference_runs():
+    from mcsim import lib
+    from mcsim._abi import MC_TOOL_BALL, MC_TOOL_FLAT
+    g = G.load("simx")
+    progs, keep = [], []
+    for case, res in zip(g["cases"], g["results"]):
+        st = case["stock"]
+        act = lib.Stock(st["w"], st["l"], st["h"], case["cell"])
+        ide = lib.Stock(st["w"], st["l"], st["h"], case["cell"])
+        ide.set_wall_band(2.0)
+        assert (act.nx, act.ny) == (res["nx"], res["ny"])
+        eng = lib.Engine()
+        cell, fnMax = G.simx_cell(res)
+        info = eng.configure(cell, fnMax)
+        _, kte, kre = res["wears"][0]
+        eng.set_wear(kte, kre)
+        sub, n = G.simx_program(res)
+        progs.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, R=res["toolR"], fluteLen=res["fluteLen"],
+                          rw=0.0, share=info.stepsPerTooth / 4,
+                          type=MC_TOOL_BALL if res["type"] == "ball" else MC_TOOL_FLAT))
+        keep.append((case, res, n))
+    print("sim pipeline: %d programs, %s sub-steps" % (len(progs), [k[2] for k in keep]), flush=True)
+    outs = lib.sim_run(progs)
+    for (case, res, n), out, pg in zip(keep, outs, progs):
+        sub = pg["sub"]
+        a = res["actual"]
+        cx = np.array([sub[k].x + out[k].dx for k in range(n)])
+        cy = np.array([sub[k].y + out[k].dy for k in range(n)])
+        assert np.array_equal(cx, G.f64(a["cx"])) and np.array_equal(cy, G.f64(a["cy"])), case["tool"]
+        assert np.array_equal([o.errVal for o in out], G.f64(a["errVal"])), case["tool"]
+        runs = dict((k, m) for k, m in res["runs"])
+        assert [o.nsteps for o in out] == [runs.get(k, 0) for k in range(n)], case["tool"]
+        assert sum(o.cut for o in out) == len(runs)
+        for name, stock, ref in (("actual", pg["actual"], res["stock"]), ("commanded", pg["ideal"], res["commanded"])):
+            got = _maps(stock)
+            for key in ("H", "err", "wallD", "markS", "markF"):
+                assert np.array_equal(got[key], G.zf32(ref[key]), equal_nan=True), (case["tool"], name, key)
+            for key in ("lineCut", "lineWall"):
+                assert np.array_equal(got[key], G.zi32(ref[key])), (case["tool"], name, key)
+        removed = sum(o.vol for o in out)
+        assert abs(removed - res["stock"]["removed"]) <= 1e-12 * max(1.0, res["stock"]["removed"]), case["tool"]
+        assert any(o.maxF > 0 for o in out)
