@@ -73,6 +73,7 @@ struct or_engine {
   /* chip events of the last run */
   mc_chip_event *ev;
   int64_t nev, ev_cap;
+  double runMaxF; /* max |(Fx, Fy)| over the last run */
 };
 
 static double jmax(double a, double b) {
@@ -279,6 +280,7 @@ int or_engine_run(or_engine *e, int64_t nSteps) {
     e->ev_cap = cap;
   }
   e->nev = 0;
+  e->runMaxF = 0;
   e->qx[0] += e->x - e->x0; e->qy[0] += e->y - e->y0;
   e->vx[0] += e->vxs - e->vx0; e->vy[0] += e->vys - e->vy0;
   x = e->x; y = e->y; vxs = e->vxs; vys = e->vys; t = e->t; pr = e->pr;
@@ -403,6 +405,7 @@ int or_engine_run(or_engine *e, int64_t nSteps) {
     }
     t += dt;
     Fxy = or_hypot(Fx, Fy);
+    e->runMaxF = jmax(e->runMaxF, Fxy);
     st[S_N]++; st[S_SUMF] += Fxy; st[S_MAXF] = jmax(st[S_MAXF], Fxy);
     st[S_SUMFX] += Fx; st[S_SUMFY] += Fy; st[S_SUMFZ] += Fz; st[S_MAXFZ] = jmax(st[S_MAXFZ], -Fz);
     st[S_SUMT] += T; st[S_MAXT] = jmax(st[S_MAXT], T);
@@ -472,3 +475,7 @@ void or_engine_read_rings(const or_engine *e, float *hist, float *ts, float *ws,
   if (ws) { memcpy(ws, e->wsx, sizeof(e->wsx)); memcpy(ws + RN, e->wsy, sizeof(e->wsy)); }
   if (wsp) { memcpy(wsp, e->wpx, sizeof(e->wpx)); memcpy(wsp + RN, e->wpy, sizeof(e->wpy)); }
 }
+
+double or_engine_dt(const or_engine *e) { return e->dt; }
+int or_engine_flutes(const or_engine *e) { return e->N; }
+double or_engine_run_max_f(const or_engine *e) { return e->runMaxF; }

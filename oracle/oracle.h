@@ -68,6 +68,10 @@ void or_engine_stats(or_engine *e, mc_engine_stats *out, int take);
 int64_t or_engine_chip_events(const or_engine *e, mc_chip_event *out, int64_t cap);
 void or_engine_chatter_index(const or_engine *e, double out[3]);
 void or_engine_read_rings(const or_engine *e, float *hist, float *ts, float *ws, float *wsp);
+double or_engine_dt(const or_engine *e);
+int or_engine_flutes(const or_engine *e);
+double or_engine_run_max_f(const or_engine *e);
+
 
 /* sld.c: zero-order lobes (src/sld.js) */
 void or_directional_coeffs(double Kr, double phiSt, double phiEx, double out[4]);
@@ -125,6 +129,12 @@ int64_t or_engine_trace_ex(const mc_cell *c, const mc_engage_spec *s, int64_t ns
 /* toolModel (src/tool.js:16-32) for a uniform carbide beam: k [N/mm], fn [Hz] */
 void or_tool_model(double D, double stickout, double kSpindle, double mHolder, double *k,
                    double *fn);
+
+/* simloop.c: the simulator's sub-step loop, rigid workpiece (src/sim.js:380-445) */
+double or_fill_end_engagement(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
+                              double R, int ball, float *Eb, double *rOut_out);
+int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
+               double fluteLen, double rw, double share, int type, mc_substep_out *out);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
 
 from mcsim._abi import (Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec,  # noqa: E402
-                        EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST, MC_ENGINE_RING)
+                        EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST, MC_ENGINE_RING,
+                        Substep, SubstepOut)
 
 _LIB = None
 
@@ -58,6 +59,8 @@ def lib():
         L.or_engine_chip_events.restype = C.c_int64
         L.or_engine_chatter_index.argtypes = [P, P]
         L.or_engine_read_rings.argtypes = [P, P, P, P, P]
+        L.or_sim_run.argtypes = [P, P, P, C.POINTER(Substep), C.c_int] + [C.c_double] * 4 + [C.c_int,
+                                                                                         C.POINTER(SubstepOut)]
         L.or_directional_coeffs.argtypes = [C.c_double, C.c_double, C.c_double, P]
         L.or_stability_lobes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_int, C.c_double, C.c_double, C.POINTER(Mode), C.c_int,
@@ -265,6 +268,16 @@ class OracleEngine:
         r = [np.zeros(2 * MC_ENGINE_RING, np.float32) for _ in range(3)]
         self.L.or_engine_read_rings(self.h, h.ctypes.data, *[x.ctypes.data for x in r])
         return h, r[0], r[1], r[2]
+
+
+def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ):
+    """the simulator's sub-step loop restated (oracle/simloop.c); returns SubstepOut records"""
+    out = (SubstepOut * max(1, n))()
+    st = lib().or_sim_run(actual.p, ideal.p, engine.h, C.cast(sub, C.POINTER(Substep)), int(n), float(R),
+                          float(fluteLen), float(rw), float(share), int(typ), out)
+    if st:
+        raise RuntimeError("or_sim_run: %d" % st)
+    return out[:n]
 
 
 class OracleStock:

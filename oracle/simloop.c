@@ -1,0 +1,122 @@
+/*
+ * oracle/simloop.c — TEST INFRASTRUCTURE ONLY (the oracle).  CPU restatement of the
+ * simulator's sub-step loop with a rigid workpiece and a fixed flank wear, serially in the
+ * reference's order (src/sim.js:380-445):
+ *
+ *   fillEngagement     src/sim.js:301-319  (or_fill_engagement, engage.c)
+ *   fillEndEngagement  src/sim.js:324-341  (below)
+ *   axial depth, end cut, the cut decision                 src/sim.js:388-401
+ *   setEngagement / setEndEngagement / setWork(null)       src/sim.js:403-408
+ *   the run's step count (the frame budget's share)        src/sim.js:409-414
+ *   the decay out of the cut                               src/sim.js:425-429
+ *   deflection, the commanded and the actual stamp         src/sim.js:433-440
+ *
+ * The engine is oracle/cengine.c's object, the stocks oracle/stock.c's.  The end-face shape
+ * zAt is the simulator's for a flat end (0) or a ball (src/sim.js:216-218).
+ */
+#include <math.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define PI 3.141592653589793
+
+static double zat(int ball, double R, double r) {
+  return ball ? R - sqrt(fmax(0.0, R * R - r * r)) : 0.0;
+}
+
+/* Simulator.fillEndEngagement (src/sim.js:324-341): Eb[b * 6 + j]; returns dmax, *rOut */
+double or_fill_end_engagement(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
+                              double R, int ball, float *Eb, double *rOut_out) {
+  int32_t nx, ny;
+  double hx, hy, rOut, dmax = 0, vx = -uy, vy = ux;
+  int b, j;
+  or_stock_dims(s, &nx, &ny, &hx, &hy);
+  rOut = R - 0.5 * (hx > hy ? hx : hy);
+  for (b = 0; b < MC_END_BINS; b++) {
+    const double phi = (b + 0.5) / MC_END_BINS * 2 * PI, sn = or_sin(phi), cs = or_cos(phi);
+    const double ex = sn * ux + cs * vx, ey = sn * uy + cs * vy;
+    for (j = 0; j < MC_END_RINGS; j++) {
+      const double r = (j + 0.5) / MC_END_RINGS * rOut;
+      const double zEdge = tipZ + zat(ball, R, r);
+      double d = or_stock_sample(s, cx + r * ex, cy + r * ey) - zEdge;
+      if (!(d > 0)) d = 0; /* Math.max(0, d): d is never NaN here */
+      Eb[b * MC_END_RINGS + j] = (float)d;
+      if (d > dmax) dmax = d;
+    }
+  }
+  *rOut_out = rOut;
+  return dmax;
+}
+
+/* setEndEngagement's rings (src/cutting.js:133-142) for the simulator's zAt */
+static void end_rings(double rOut, int ball, double R, double *g) {
+  const double dr = rOut / MC_END_RINGS;
+  int j;
+  for (j = 0; j < MC_END_RINGS; j++) {
+    const double r = (j + 0.5) * dr;
+    const double a = r + 0.25 * dr < rOut ? r + 0.25 * dr : rOut;
+    const double b = r - 0.25 * dr > 0 ? r - 0.25 * dr : 0;
+    const double slope = (zat(ball, R, a) - zat(ball, R, b)) / (0.5 * dr);
+    double c = 1 / sqrt(1 + slope * slope);
+    if (isnan(c)) c = NAN; else if (c < 0.22) c = 0.22;
+    g[j] = r; g[MC_END_RINGS + j] = c; g[2 * MC_END_RINGS + j] = zat(ball, R, r);
+    g[3 * MC_END_RINGS + j] = dr / c;
+  }
+}
+
+int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
+               double fluteLen, double rw, double share, int type, mc_substep_out *out) {
+  const int ball = type == MC_TOOL_BALL;
+  float Hb[MC_BINS], Eb[MC_END_BINS * MC_END_RINGS];
+  double fzLast = 0;
+  int k;
+  memset(Hb, 0, sizeof(Hb));
+  memset(Eb, 0, sizeof(Eb));
+  for (k = 0; k < n; k++) {
+    const mc_substep *s = &sub[k];
+    mc_engine_state st;
+    double hmax = -INFINITY, ap, dmax = 0, rOut = 0, dx, dy, errVal, vol, maxF = 0;
+    int haveEnd = s->fza > 0, endCut, cut, nsteps = 0;
+    if (!(s->dxy < 1e-6)) hmax = or_fill_engagement(act, s->x, s->y, s->ux, s->uy, R, Hb);
+    ap = hmax - s->z > 0 ? hmax - s->z : 0;
+    if (fluteLen < ap) ap = fluteLen;
+    if (haveEnd) dmax = or_fill_end_engagement(act, s->x, s->y, s->z, s->ux, s->uy, R, ball, Eb, &rOut);
+    endCut = haveEnd && dmax > 2e-4;
+    cut = (ap > 0.01 || endCut) && !s->rapid;
+    or_engine_get_state(e, &st);
+    if (cut) {
+      double g[4 * MC_END_RINGS], want;
+      or_engine_set_engagement(e, s->ux, s->uy, s->fzEff, s->z, ap, NULL, NULL, NULL);
+      end_rings(haveEnd ? rOut : 0, ball, R, g);
+      or_engine_set_end_engagement(e, endCut ? s->fza : 0, haveEnd ? rOut : 0, g);
+      or_engine_set_work(e, NULL);
+      or_engine_set_maps(e, Hb, Eb);
+      want = or_js_round(s->dtm / or_engine_dt(e));
+      if (want < 1) want = 1;
+      nsteps = (int)(want < floor(share) ? want : floor(share));
+      if (nsteps < 1) nsteps = 1;
+      or_engine_run(e, nsteps);
+      maxF = or_engine_run_max_f(e);
+      fzLast = s->fzEff;
+    } else {
+      st.x *= 0.5; st.y *= 0.5; st.vx *= 0.5; st.vy *= 0.5; st.w *= 0.5; st.wv *= 0.5;
+      or_engine_set_state(e, &st);
+    }
+    or_engine_get_state(e, &st);
+    dx = (st.x - st.w * st.wnx) * 1e3;
+    dy = (st.y - st.w * st.wny) * 1e3;
+    errVal = or_hypot(st.x, st.y) * 1e6;
+    or_stock_stamp_ex(ide, s->x, s->y, s->z, R, type, 0, s->line, NULL, NULL, 0);
+    {
+      const double mark[3] = {s->ux, s->uy, fzLast * or_engine_flutes(e)};
+      vol = or_stock_stamp_ex(act, s->x + dx, s->y + dy, s->z + (ball ? rw : 0), R - rw, type, errVal, s->line, mark,
+                              NULL, 0);
+    }
+    if (out) {
+      out[k].dx = dx; out[k].dy = dy; out[k].ap = ap; out[k].errVal = errVal; out[k].vol = vol;
+      out[k].maxF = maxF; out[k].nsteps = nsteps; out[k].cut = cut;
+    }
+  }
+  return MC_OK;
+}

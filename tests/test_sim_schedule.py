@@ -32,3 +32,36 @@ def test_sim_substeps_match_reference_runs():
             if k in runs:
                 last = sub[k].fzEff
             assert f[k] == last * res["toolN"], (case["tool"], k)
+
+
+def test_oracle_sim_loop_matches_reference_runs():
+    """the oracle's restatement of the sub-step loop (oracle/simloop.c) against the same
+    runs: deflected stamp positions, vibration values, step counts and every map, bit for bit"""
+    import oracle
+    from mcsim._abi import MC_TOOL_BALL, MC_TOOL_FLAT
+    g = G.load("simx")
+    for case, res in zip(g["cases"], g["results"]):
+        st = case["stock"]
+        act = oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"])
+        ide = oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"])
+        ide.set_wall_band(2.0)
+        eng = oracle.OracleEngine()
+        cell, fnMax = G.simx_cell(res)
+        info = eng.configure(cell, fnMax)
+        _, kte, kre = res["wears"][0]
+        eng.set_wear(kte, kre)
+        sub, n = G.simx_program(res)
+        out = oracle.sim_run(act, ide, eng, sub, n, res["toolR"], res["fluteLen"], 0.0, info.stepsPerTooth / 4,
+                             MC_TOOL_BALL if res["type"] == "ball" else MC_TOOL_FLAT)
+        a = res["actual"]
+        assert np.array_equal([sub[k].x + out[k].dx for k in range(n)], G.f64(a["cx"])), case["tool"]
+        assert np.array_equal([sub[k].y + out[k].dy for k in range(n)], G.f64(a["cy"])), case["tool"]
+        assert np.array_equal([o.errVal for o in out], G.f64(a["errVal"]))
+        assert np.array_equal([o.vol for o in out], G.f64(a["vol"])), case["tool"]
+        runs = dict((k, m) for k, m in res["runs"])
+        assert [o.nsteps for o in out] == [runs.get(k, 0) for k in range(n)]
+        for name, s_, ref in (("actual", act, res["stock"]), ("commanded", ide, res["commanded"])):
+            for key, fn in (("H", s_.H), ("err", s_.err), ("wallD", s_.wallD), ("markS", s_.markS), ("markF", s_.markF)):
+                assert np.array_equal(fn(), G.zf32(ref[key]), equal_nan=True), (case["tool"], name, key)
+            for key, fn in (("lineCut", s_.lineCut), ("lineWall", s_.lineWall)):
+                assert np.array_equal(fn(), G.zi32(ref[key])), (case["tool"], name, key)
