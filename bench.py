@@ -10,7 +10,8 @@ only steers which bisection nodes are speculated, so the results are the plain s
 own, checked cell by cell).  value = SLD cells resolved per second by the faster of the
 two; both are in config.  The +-15 % bracket form of the surrogate is reported with its
 error against plain bisection.  The heightmap path (512 x 512 at 0.1 mm, 5,000 seeded
-zigzag segments = 2.41 M stamps) is reported beside it in GB/s.
+zigzag segments = 2.41 M stamps) is reported beside it in GB/s, and the simulator's coupled
+sub-step loop (16 seeded raster programs, each on its own 2048 x 2048 stock) in sub-steps/s.
 
 Prints ONE JSON line.  usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -122,6 +123,103 @@ def heightmap(steps):
                      "tiles stay in LDS across the stamps of a chunk")
 
 
+def coupled(n_prog, n_lines, cpu_threads, cpu):
+    """The simulator's coupled sub-step loop (engagement -> engine run -> commanded and
+    deflected stamps, src/sim.js:351-449) for n_prog seeded raster programs, each on its own
+    2048 x 2048 stock at 0.05 mm (102.35 mm square) with the flat 10 mm tool, one launch of
+    mc_sim_run (one wavefront per program).  Reported in sub-steps per second; the oracle's
+    restated loop runs the same programs on host threads as the CPU baseline and checks the
+    device's deflections, step counts and final heights bit for bit."""
+    from mcsim._abi import make_cell, MC_TOOL_FLAT
+    width, cell = 102.35, 0.05
+    hx = width / 2047
+    progs = []
+    for p in range(n_prog):
+        x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, n_lines)
+        progs.append(path.sim_substeps(x, y, z, r, ln, F, S, W.COUPLED_TOOL["N"], hx))
+    ecell = make_cell(W.COUPLED_TOOL, W.COUPLED_MODES, None, W.MAT_A5052, 10000.0, (0.0, 0.0), 0.0, 1, True)
+    kte, kre = W.MAT_A5052["Kte"], W.MAT_A5052["Kre"]
+
+    def setup():
+        P = []
+        for sub, n in progs:
+            act, ide = lib.Stock(width, width, 20, cell), lib.Stock(width, width, 20, cell)
+            ide.set_wall_band(2.0)
+            eng = lib.Engine()
+            info = eng.configure(ecell)
+            eng.set_wear(kte, kre)
+            P.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, R=5.0, fluteLen=25.0, rw=0.0,
+                          share=info.stepsPerTooth / 4, type=MC_TOOL_FLAT))
+        return P
+
+    def release(P):
+        for g in P:
+            g["actual"].close(); g["ideal"].close(); g["engine"].close()
+
+    W0 = setup()
+    lib.sim_run(W0)          # warm-up launch (code objects, allocations)
+    release(W0)
+    P = setup()
+    t0 = time.perf_counter()
+    outs = lib.sim_run(P)
+    wall = time.perf_counter() - t0
+    kms = lib.last_kernel_ms()
+    nsub = sum(n for _, n in progs)
+    steps = sum(o.nsteps for out in outs for o in out)
+    cuts = sum(o.cut for out in outs for o in out)
+    # algorithmic bytes: both stamps of every sub-step read and write the f32 height of each
+    # node of their bounding boxes (the commanded stock's wall band is 2 mm)
+    nodes = 0
+    for sub, n in progs:
+        cxs = np.array([sub[k].x for k in range(n)]); cys = np.array([sub[k].y for k in range(n)])
+        for Rw in (5.0 + 2 * hx, 5.0 + 2.0):
+            i0 = np.maximum(0, np.floor((cxs - Rw) / hx)); i1 = np.minimum(2047, np.ceil((cxs + Rw) / hx))
+            j0 = np.maximum(0, np.floor((cys - Rw) / hx)); j1 = np.minimum(2047, np.ceil((cys + Rw) / hx))
+            nodes += np.where((i1 >= i0) & (j1 >= j0), (i1 - i0 + 1) * (j1 - j0 + 1), 0).sum()
+    res = dict(workload="coupled_%dprog_2048x2048_%dlines" % (n_prog, n_lines), programs=n_prog,
+               substeps=nsub, cutting_substeps=cuts, engine_steps=steps, wall_s=wall, kernel_ms=kms,
+               substeps_per_s=nsub / wall, engine_steps_per_s=steps / wall,
+               roofline={"bound": "hbm", "achieved": 8.0 * nodes / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": 8.0 * nodes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "kernel": "sim_run", "traffic": None},
+               note="algorithmic bytes = 8 B x bbox nodes of both stamps per sub-step; one workgroup per "
+                    "program runs its sub-steps in order (each reads the stock the previous one cut), so "
+                    "the kernel is bound by the latency of that chain, far below the HBM roof")
+    if cpu:
+        import threading
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        k = min(cpu_threads, n_prog)
+        outs_c = [None] * k
+        H_c = [None] * k
+
+        def work(i):
+            sub, n = progs[i]
+            act, ide = oracle.OracleStock(width, width, 20, cell), oracle.OracleStock(width, width, 20, cell)
+            ide.set_wall_band(2.0)
+            eng = oracle.OracleEngine()
+            info = eng.configure(ecell)
+            eng.set_wear(kte, kre)
+            outs_c[i] = oracle.sim_run(act, ide, eng, sub, n, 5.0, 25.0, 0.0, info.stepsPerTooth / 4, MC_TOOL_FLAT)
+            H_c[i] = act.H()
+        th = [threading.Thread(target=work, args=(i,)) for i in range(k)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        cw = time.perf_counter() - t0
+        same = all(all(a.dx == b.dx and a.dy == b.dy and a.nsteps == b.nsteps for a, b in zip(outs[i], outs_c[i]))
+                   and np.array_equal(P[i]["actual"].H(), H_c[i]) for i in range(k))
+        csub = sum(progs[i][1] for i in range(k))
+        res["cpu_baseline"] = dict(value=csub / cw, unit="sub-steps/s", cores=k, kind="port",
+                                   sample="%d of the %d programs on %d host threads (oracle/simloop.c), %.1f s" %
+                                   (k, n_prog, k, cw))
+        res["identical_to_oracle"] = bool(same)
+    release(P)
+    return res
+
+
 def surrogate_leg(cells, opts, plain, steps, n_train, epochs):
     """Train the bracket surrogate on cells the GPU search resolves (random tools and speeds
     of the lobe-grid distributions, not the grid's own), then time the surrogate-guided
@@ -189,6 +287,9 @@ def main():
     ap.add_argument("--no-heightmap", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-coupled", action="store_true")
+    ap.add_argument("--coupled-programs", type=int, default=16)
+    ap.add_argument("--coupled-lines", type=int, default=4)
     ap.add_argument("--train-cells", type=int, default=262144)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -269,6 +370,8 @@ def main():
     }
     if not a.no_heightmap:
         out["heightmap"] = heightmap(2)
+    if not a.no_coupled:
+        out["coupled"] = coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, opts, a.cpu_seconds)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]

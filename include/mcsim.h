@@ -122,8 +122,8 @@ const char* mc_last_error(void);
 /* Writes "<device name> <gcnArch> CUs=<n>" into buf. */
 int mc_device_info(char* buf, int len);
 /* Device time [ms] of the main kernel of the last batch entry point called on this
- * device (the chatter search, or the tile pass of mc_stamp_batch), measured with HIP
- * events on the library's stream around that kernel alone. */
+ * device (the chatter search, the tile pass of mc_stamp_batch, or the sub-step pipeline of
+ * mc_sim_run), measured with HIP events on the library's stream around that kernel alone. */
 int mc_last_kernel_time(double* ms);
 
 /* Launch schedule of the search (library-wide; no reference counterpart).  The trees of

@@ -91,6 +91,12 @@ struct EngLds {
   double term[T_COUNT][SLOT_CAP];
   int teng[SLOT_CAP];
   double res[8];
+  // the run's constants, staged from the engine record
+  double sLag[MS], sKap[MS], sZ[MS], sB[MS], sR[MS];
+  double dR[MC_MAX_FLUTES];
+  double modal[6 * MC_MAX_MODES];   // kx mx cx ky my cy
+  double ring[4 * ERINGS];          // ringR ringCos ringZ ringDs
+  float Hb[MC_BINS], Eb[EBINS * ERINGS];
 };
 
 // run(nSteps) of src/cutting.js:146-296 on engine E by the calling wavefront
@@ -114,6 +120,35 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
   const bool mcOn = truthy(mc), pdOn = truthy(pdK), wallOn = truthy(wk);
   double runMaxF = 0;
   float* const surf = E.surf;
+
+  // the run's constants into LDS: the engagement's slice tables, the maps, the tooth
+  // offsets, the end rings and the modal coefficients
+  for (int k = lane; k < nz; k += 64) {
+    L.sLag[k] = E.sLag[k]; L.sKap[k] = E.sKap[k]; L.sZ[k] = E.sZ[k]; L.sB[k] = E.sB[k]; L.sR[k] = E.sR[k];
+  }
+  for (int b = lane; b < MC_BINS; b += 64) L.Hb[b] = E.Hb[b];
+  for (int b = lane; b < EBINS * ERINGS; b += 64) L.Eb[b] = E.Eb[b];
+  if (lane < MC_MAX_FLUTES) L.dR[lane] = E.dR[lane];
+  if (lane < ERINGS) {
+    L.ring[lane] = E.ringR[lane]; L.ring[ERINGS + lane] = E.ringCos[lane];
+    L.ring[2 * ERINGS + lane] = E.ringZ[lane]; L.ring[3 * ERINGS + lane] = E.ringDs[lane];
+  }
+  if (lane < MC_MAX_MODES) {
+    L.modal[lane] = E.kx[lane]; L.modal[MC_MAX_MODES + lane] = E.mx[lane];
+    L.modal[2 * MC_MAX_MODES + lane] = E.cx[lane]; L.modal[3 * MC_MAX_MODES + lane] = E.ky[lane];
+    L.modal[4 * MC_MAX_MODES + lane] = E.my[lane]; L.modal[5 * MC_MAX_MODES + lane] = E.cy[lane];
+  }
+  // the delay-line cell a lane's first visit reads at its step, loaded a step ahead: the
+  // cell read at step s + 1 was written at least a tooth period before, never at step s
+  const int k0 = lane < NS ? lane % per : 0, i0s = lane < NS ? lane / per : 0;
+  const bool pre0 = lane < NS && k0 < nz;
+  auto cell_of = [&](int prr) {
+    int pi = prr + i0s * spt;
+    if (pi >= spr) pi -= spr;
+    return surf + (size_t)pi * MSL + k0;
+  };
+  float cur0 = pre0 ? *cell_of(E.pr) : 0.0f;
+  wave_sync();
 
   // modal state, redundantly on every lane; a state set from outside enters mode 0
   double qx[MC_MAX_MODES], vxm[MC_MAX_MODES], qy[MC_MAX_MODES], vym[MC_MAX_MODES];
@@ -145,6 +180,8 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
     const double rx = x - w * wnx, ry = y - w * wny;
     const double xm = rx * 1e3, ym = ry * 1e3;
     const double rvx = vxs - wv * wnx, rvy = vys - wv * wny;
+    const int prn = pr + 1 == spr ? 0 : pr + 1;
+    const float nxt0 = pre0 ? *cell_of(prn) : 0.0f;
     // ---- visits (src/cutting.js:172-237)
     for (int slot = lane; slot < NS; slot += 64) {
       const int i = slot / per, r = slot - i * per;
@@ -154,20 +191,20 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
       int eng = 0;
       if (r < nz) {
         const int k = r;
-        double phi = mcm::fmod_2pi((theta + i * pitch) - E.sLag[k]);
+        double phi = mcm::fmod_2pi((theta + i * pitch) - L.sLag[k]);
         if (phi < 0) phi += TWO_PI;
         double sn, cs;
         mcm::sincos_js(phi, sn, cs);
         const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-        const double rn = xm * nxg + ym * nyg + E.dR[i];
+        const double rn = xm * nxg + ym * nyg + L.dR[i];
         int pi = pr + i * spt;
         if (pi >= spr) pi -= spr;
         float* const cell = surf + (size_t)pi * MSL + k;
-        const double prev = (double)*cell;
-        const double feedH = fz * sn * E.sKap[k];
+        const double prev = slot == lane ? (double)cur0 : (double)*cell;
+        const double feedH = fz * sn * L.sKap[k];
         const int bin = (int)(phi * binScale);
-        const double hb = bin < MC_BINS ? (double)E.Hb[bin] : __builtin_nan("");
-        if (hb <= E.sZ[k] || sn <= 0) {
+        const double hb = bin < MC_BINS ? (double)L.Hb[bin] : __builtin_nan("");
+        if (hb <= L.sZ[k] || sn <= 0) {
           *cell = (float)rn;
         } else {
           double h = feedH + rn - prev;
@@ -177,7 +214,7 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
             *cell = (float)rn;
             if (h > hMax) h = hMax;
             const double se = mcOn ? mcm::pow_js(H_REF / (h > H_MIN ? h : H_MIN), mc) : 1.0;
-            const double bk = E.sB[k], sr = E.sR[k], sz = E.sZ[k];
+            const double bk = L.sB[k], sr = L.sR[k], sz = L.sZ[k];
             const double Ft = Ktc * se * h * bk + Kte * bk;
             double Fr = Krc * se * h * bk + Kre * bk;
             const double Fa = Kac * se * h * bk + Kae * bk;
@@ -204,12 +241,12 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
         double phi = mcm::fmod_2pi(theta + i * pitch);
         if (phi < 0) phi += TWO_PI;
         const int rb = (int)(phi / TWO_PI * EBINS);
-        const double depth = rb < EBINS ? (double)E.Eb[rb * ERINGS + j] : __builtin_nan("");
+        const double depth = rb < EBINS ? (double)L.Eb[rb * ERINGS + j] : __builtin_nan("");
         if (!(depth <= 2e-4)) {
           double sn, cs;
           mcm::sincos_js(phi, sn, cs);
           const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-          const double c = E.ringCos[j], ds = E.ringDs[j], rr = E.ringR[j], rz = E.ringZ[j];
+          const double c = L.ring[ERINGS + j], ds = L.ring[3 * ERINGS + j], rr = L.ring[j], rz = L.ring[2 * ERINGS + j];
           const double h = (depth < fza ? depth : fza) * c;
           const double se = mcOn ? mcm::pow_js(H_REF / (h > H_MIN ? h : H_MIN), mc) : 1.0;
           const double Ft = Ktc * se * h * ds + Kte * ds;
@@ -279,7 +316,7 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
 #pragma unroll
     for (int j = 0; j < MC_MAX_MODES; j++) {
       if (j < nmx) {
-        vxm[j] += (Fx - E.cx[j] * vxm[j] - E.kx[j] * qx[j]) / E.mx[j] * dt;
+        vxm[j] += (Fx - L.modal[2 * MC_MAX_MODES + j] * vxm[j] - L.modal[j] * qx[j]) / L.modal[MC_MAX_MODES + j] * dt;
         qx[j] += vxm[j] * dt;
         x += qx[j];
         vxs += vxm[j];
@@ -288,7 +325,8 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
 #pragma unroll
     for (int j = 0; j < MC_MAX_MODES; j++) {
       if (j < nmy) {
-        vym[j] += (Fy - E.cy[j] * vym[j] - E.ky[j] * qy[j]) / E.my[j] * dt;
+        vym[j] += (Fy - L.modal[5 * MC_MAX_MODES + j] * vym[j] - L.modal[3 * MC_MAX_MODES + j] * qy[j]) /
+                  L.modal[4 * MC_MAX_MODES + j] * dt;
         qy[j] += vym[j] * dt;
         y += qy[j];
         vys += vym[j];
@@ -306,10 +344,11 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
       wSum += w;
     }
     if (++pr == spr) pr = 0;
+    cur0 = nxt0;
     const int ph = pr % spt;
     if (ph == 0) {
       // the tooth now at phi = 0 generates the +v wall (src/cutting.js:265-272)
-      const double d0 = E.dR[(N - (pr / spt) % N) % N] * 1e-3;
+      const double d0 = L.dR[(N - (pr / spt) % N) % N] * 1e-3;
       if (lane == 0) {
         E.rings[ts_i] = (float)(x - w * wnx);
         E.rings[RN + ts_i] = (float)(y - w * wny);
@@ -322,7 +361,7 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
     if (ph == piStep) {
       const long long qn = (long long)mcm::round_js((JS_PI - pr * dphi) / pitch);
       const int ip = (int)(((qn % N) + N) % N);
-      const double dp = E.dR[ip] * 1e-3;
+      const double dp = L.dR[ip] * 1e-3;
       if (lane == 0) {
         E.rings[4 * RN + wsp_i] = (float)(x - w * wnx - dp * vx);
         E.rings[5 * RN + wsp_i] = (float)(y - w * wny - dp * vy);

@@ -1,13 +1,14 @@
 // simrun.hip — the simulator's sub-step loop (src/sim.js:351-449) on the device.
 //
-// One wavefront per program runs its whole list of sub-steps: per sub-step it samples the
+// One workgroup per program runs its whole list of sub-steps: per sub-step it samples the
 // actual stock on the cutter periphery (fillEngagement, src/sim.js:301-319) and, while
 // descending, under the end face (fillEndEngagement, :324-341) into the engine's maps,
 // takes the axial depth, sets the engine up and runs the sub-step's share of steps
 // (engine_steps, the reference's run) or halves the vibration out of the cut, and stamps
 // the commanded stock at p and the actual stock at p plus the deflection (Stock.stamp,
-// src/stock.js:118-164), before the next sub-step reads the actual stock again.  Nothing
-// returns to the host between sub-steps; many programs share one launch.
+// src/stock.js:118-164), before the next sub-step reads the actual stock again.  The stamps
+// and the samples use all the workgroup's threads, the engine's step loop its first
+// wavefront.  Nothing returns to the host between sub-steps; many programs share one launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,6 +59,8 @@ __device__ __forceinline__ double sample_at(const StockDev& S, double x, double 
   return (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty;
 }
 
+constexpr int NT = 256;   // threads per program: the stamps and samples use all, the engine wave 0
+
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = jmax(v, __shfl_xor(v, o, 64));
   return v;
@@ -66,68 +69,118 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// block-wide max / sum through LDS (red holds NT / 64 doubles); every thread gets the result
+__device__ __forceinline__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int w = 1; w < NT / 64; w++) m = jmax(m, red[w]);
+  return m;
+}
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = 0;
+  for (int w = 0; w < NT / 64; w++) m += red[w];
+  return m;
+}
 
-// Stock.stamp (src/stock.js:118-164) by one wavefront, flat or ball without a profile table:
-// the nodes of the box are independent, so the lanes take them in any order.  Returns the
-// removed volume [mm^3] (its sum is a wave reduction, not the reference's node order).
-__device__ double stamp_wave(const StockDev& S, double cx, double cy, double tipZ, double R, bool ball,
-                             double errVal, double mux, double muy, double mfeed, int32_t line) {
-  const int lane = threadIdx.x & 63;
-  if (tipZ >= S.zTop) return 0;
-  const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band), Rw2 = Rw * Rw;
+// the node box of Stock.stamp (src/stock.js:119-128): empty when the tool is above the top
+struct Box { int i0, i1, j0, j1; bool ok; };
+__device__ __forceinline__ Box stamp_box(const StockDev& S, double cx, double cy, double tipZ, double R) {
+  Box b{0, -1, 0, -1, false};
+  if (tipZ >= S.zTop) return b;
+  const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band);
   const double fi0 = jmax(0, __builtin_floor((cx - Rw) / S.hx)), fi1 = jmin(S.nx - 1, __builtin_ceil((cx + Rw) / S.hx));
   const double fj0 = jmax(0, __builtin_floor((cy - Rw) / S.hy)), fj1 = jmin(S.ny - 1, __builtin_ceil((cy + Rw) / S.hy));
-  if (!(fi0 <= fi1) || !(fj0 <= fj1)) return 0;
-  const int i0 = (int)fi0, i1 = (int)fi1, j0 = (int)fj0, j1 = (int)fj1;
+  if (!(fi0 <= fi1) || !(fj0 <= fj1)) return b;
+  b.i0 = (int)fi0; b.i1 = (int)fi1; b.j0 = (int)fj0; b.j1 = (int)fj1; b.ok = true;
+  return b;
+}
+
+// Stock.stamp's node loop (src/stock.js:129-155), flat or ball without a profile table, by
+// threads t = 0..nt-1 of the workgroup: the nodes are independent, so they are taken in any
+// order, U at a time with their loads issued before any store.  Returns the thread's part
+// of the removed height sum; a node changed sets *touched.
+__device__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ, double R, bool ball,
+                             double errVal, double mux, double muy, double mfeed, int32_t line, int t, int nt,
+                             int* touched) {
+  if (!bx.ok) return 0;
+  const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band), Rw2 = Rw * Rw;
   const double R2 = R * R;
   const double mf = mfeed > 0 && !ball ? mfeed : 0;
   const double zRim = tipZ + (ball ? R : 0);
-  const int W = i1 - i0 + 1, total = W * (j1 - j0 + 1);
+  const int W = bx.i1 - bx.i0 + 1, total = W * (bx.j1 - bx.j0 + 1);
+  constexpr int U = 8;
   double vol = 0;
-  bool touched = false;
-  for (int q = lane; q < total; q += 64) {
-    const int j = j0 + q / W, i = i0 + q % W;
-    const double dy = j * S.hy - cy, dx = i * S.hx - cx;
-    const double r2 = dx * dx + dy * dy;
-    const int64_t k = (int64_t)j * S.nx + i;
-    if (r2 >= R2 - 0.003 * R) {
-      if (r2 <= Rw2 && (double)S.H[k] > zRim) {
-        const double d = __builtin_sqrt(r2) - R;
-        if (d < (double)S.wallD[k]) { S.wallD[k] = (float)d; S.lineWall[k] = line; touched = true; }
+  bool tch = false;
+  for (int q0 = t; q0 < total; q0 += U * nt) {
+    int64_t k[U];
+    float hv[U], wd[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = q0 + u * nt;
+      const int qq = q < total ? q : q0;
+      const int j = bx.j0 + qq / W, i = bx.i0 + qq % W;
+      k[u] = (int64_t)j * S.nx + i;
+      hv[u] = S.H[k[u]];
+      wd[u] = S.wallD[k[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = q0 + u * nt;
+      if (q >= total) continue;
+      const int j = bx.j0 + q / W, i = bx.i0 + q % W;
+      const double dy = j * S.hy - cy, dx = i * S.hx - cx;
+      const double r2 = dx * dx + dy * dy;
+      if (r2 >= R2 - 0.003 * R) {
+        if (r2 <= Rw2 && (double)hv[u] > zRim) {
+          const double d = __builtin_sqrt(r2) - R;
+          if (d < (double)wd[u]) { S.wallD[k[u]] = (float)d; S.lineWall[k[u]] = line; tch = true; }
+        }
+        continue;
       }
-      continue;
-    }
-    const double z = ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
-    double h = S.H[k];
-    if (h > z) {
-      vol += h - z;
-      const float nh = (float)jmax(z, S.zB);
-      S.H[k] = nh;
-      S.err[k] = (float)errVal;
-      S.wallD[k] = __builtin_inff();
-      S.lineCut[k] = line;
-      touched = true;
-      h = nh;
-    }
-    if (mf != 0 && h > z - 0.002) {
-      const double xs = i * S.hx, ys = j * S.hy, p = -dx * muy + dy * mux;
-      S.markS[k] = (float)(xs * mux + ys * muy + __builtin_sqrt(jmax(0, R2 - p * p)));
-      S.markF[k] = (float)mf;
-      touched = true;
+      const double z = ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
+      double h = hv[u];
+      if (h > z) {
+        vol += h - z;
+        const float nh = (float)jmax(z, S.zB);
+        S.H[k[u]] = nh;
+        S.err[k[u]] = (float)errVal;
+        S.wallD[k[u]] = __builtin_inff();
+        S.lineCut[k[u]] = line;
+        tch = true;
+        h = nh;
+      }
+      if (mf != 0 && h > z - 0.002) {
+        const double xs = i * S.hx, ys = j * S.hy, p = -dx * muy + dy * mux;
+        S.markS[k[u]] = (float)(xs * mux + ys * muy + __builtin_sqrt(jmax(0, R2 - p * p)));
+        S.markF[k[u]] = (float)mf;
+        tch = true;
+      }
     }
   }
-  if (__any(touched) && lane == 0) {
-    atomicMin(&S.rect[0], i0); atomicMin(&S.rect[1], j0);
-    atomicMax(&S.rect[2], i1); atomicMax(&S.rect[3], j1);
-  }
-  return wave_sum(vol) * S.hx * S.hy;
+  if (tch) *touched = 1;
+  return vol;
 }
 
-__global__ __launch_bounds__(64) void sim_run(const ProgDev* progs) {
+// the dirty rectangle joins a changed stamp's box (src/stock.js:156-160)
+__device__ __forceinline__ void join_rect(const StockDev& S, const Box& b) {
+  atomicMin(&S.rect[0], b.i0); atomicMin(&S.rect[1], b.j0);
+  atomicMax(&S.rect[2], b.i1); atomicMax(&S.rect[3], b.j1);
+}
+
+__global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   __shared__ EngLds L;
+  __shared__ double red[NT / 64];
+  __shared__ int flags[2];   // the commanded / actual stamp changed a node
   const ProgDev& P = progs[blockIdx.x];
   EngDev& E = *P.eng;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const double PI = 3.141592653589793;
   const bool ball = P.ball != 0;
   const double R = P.R;
@@ -140,7 +193,7 @@ __global__ __launch_bounds__(64) void sim_run(const ProgDev* progs) {
     double hmax = -__builtin_inf();
     if (!(s.dxy < 1e-6)) {
       const double r = R - 0.5 * cell, ax = s.x + s.ux * cell, ay = s.y + s.uy * cell;
-      for (int b = lane; b < MC_BINS; b += 64) {
+      for (int b = tid; b < MC_BINS; b += NT) {
         const double phi = (b + 0.5) / MC_BINS * 2 * PI;
         double sn, cs;
         mcm::sincos_js(phi, sn, cs);
@@ -149,7 +202,7 @@ __global__ __launch_bounds__(64) void sim_run(const ProgDev* progs) {
         E.Hb[b] = f;
         hmax = jmax(hmax, (double)f);
       }
-      hmax = wave_max(hmax);
+      hmax = block_max(hmax, red);
     }
     const double ap = jmin(P.fluteLen, jmax(0, hmax - s.z));
     // ---- fillEndEngagement (src/sim.js:324-341) while descending
@@ -157,7 +210,7 @@ __global__ __launch_bounds__(64) void sim_run(const ProgDev* progs) {
     double dmax = 0, rOut = 0;
     if (haveEnd) {
       rOut = R - 0.5 * cell;
-      for (int q = lane; q < MC_END_BINS * MC_END_RINGS; q += 64) {
+      for (int q = tid; q < MC_END_BINS * MC_END_RINGS; q += NT) {
         const int b = q / MC_END_RINGS, j = q - b * MC_END_RINGS;
         const double phi = (b + 0.5) / MC_END_BINS * 2 * PI;
         double sn, cs;
@@ -169,41 +222,56 @@ __global__ __launch_bounds__(64) void sim_run(const ProgDev* progs) {
         E.Eb[q] = (float)d;
         dmax = jmax(dmax, d);
       }
-      dmax = wave_max(dmax);
+      dmax = block_max(dmax, red);
     }
     const bool endCut = haveEnd && dmax > 2e-4;
     const bool cut = (ap > 0.01 || endCut) && !s.rapid;
     int n = 0;
-    wave_sync();
+    // the commanded stamp does not depend on the dynamics: waves 1.. stamp it while wave 0
+    // runs the engine
+    const Box bi = stamp_box(P.ide, s.x, s.y, s.z, R);
+    if (tid == 0) { flags[0] = 0; flags[1] = 0; }
+    __threadfence_block();
+    __syncthreads();
     if (cut) {
-      // setEngagement, setEndEngagement, setWork(null) (src/sim.js:403-408), on lane 0
-      if (lane == 0) {
+      // setEngagement, setEndEngagement, setWork(null) (src/sim.js:403-408), on thread 0
+      if (tid == 0) {
         eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball);
         eng_set_end_engagement(E, endCut ? s.fza : 0, haveEnd ? rOut : 0, nullptr, ball, R);
         eng_set_work_null(E);
         E.hMax = jmax(0.1, 3 * E.fz);
       }
-      wave_sync();
+      __threadfence_block();
+      __syncthreads();
       // the sub-step's dynamics: its own time, capped by the frame budget's share (:410-412)
       const double want = jmax(1, mcm::round_js(s.dtm / E.dt));
       n = (int)jmax(1, jmin(want, __builtin_floor(P.share)));
-      engine_steps(E, n, L);
+      if (tid < 64) engine_steps(E, n, L);
+      else stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64, &flags[0]);
       fzLast = s.fzEff;
     } else {
       // out of the cut the vibration decays (:427-428)
-      if (lane == 0) { E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5; }
-      wave_sync();
+      if (tid == 0) { E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5; }
+      if (tid >= 64) stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64, &flags[0]);
     }
-    // ---- deflection, then the commanded and the actual stamp (:433-440)
+    __threadfence_block();
+    __syncthreads();
+    // ---- deflection, then the actual stamp (:433-440)
     const double dx = (E.x - E.w * E.wnx) * 1e3, dy = (E.y - E.w * E.wny) * 1e3;
     const double errVal = mcm::hypot_js(E.x, E.y) * 1e6;
     const double maxF = cut ? E.runMaxF : 0.0;
-    stamp_wave(P.ide, s.x, s.y, s.z, R, ball, 0.0, 0.0, 0.0, 0.0, s.line);
-    wave_sync();
-    const double vol = stamp_wave(P.act, s.x + dx, s.y + dy, s.z + (ball ? P.rw : 0), R - P.rw, ball, errVal, s.ux, s.uy,
-                                  fzLast * P.N, s.line);
-    wave_sync();
-    if (P.out && lane == 0) {
+    const double ax = s.x + dx, ay = s.y + dy, az = s.z + (ball ? P.rw : 0), aR = R - P.rw;
+    const Box ba = stamp_box(P.act, ax, ay, az, aR);
+    const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, errVal, s.ux, s.uy, fzLast * P.N, s.line, tid, NT,
+                                   &flags[1]);
+    const double vol = block_sum(part, red) * P.act.hx * P.act.hy;
+    if (tid == 0) {
+      if (flags[0]) join_rect(P.ide, bi);
+      if (flags[1]) join_rect(P.act, ba);
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (P.out && tid == 0) {
       mc_substep_out& o = P.out[k];
       o.dx = dx; o.dy = dy; o.ap = ap; o.errVal = errVal; o.vol = vol; o.maxF = maxF; o.nsteps = n; o.cut = cut ? 1 : 0;
     }
@@ -292,9 +360,14 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   }
   if (!st && hipMemcpyAsync(dprog, pd.data(), sizeof(ProgDev) * (size_t)n, hipMemcpyHostToDevice, q) != hipSuccess)
     st = mch::fail(MC_ERR_HIP, "program upload");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = mch::fail(MC_ERR_HIP, "timing events");
   if (!st) {
-    hipLaunchKernelGGL(sim_run, dim3(n), dim3(64), 0, q, (const ProgDev*)dprog);
+    (void)hipEventRecord(e0, q);
+    hipLaunchKernelGGL(sim_run, dim3(n), dim3(NT), 0, q, (const ProgDev*)dprog);
     if (hipGetLastError() != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline launch failed");
+    (void)hipEventRecord(e1, q);
   }
   off = 0;
   for (int p = 0; p < n && !st; p++) {
@@ -306,6 +379,12 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (!st) st = mce_engine_finish(g.engine);
   }
   if (!st && hipStreamSynchronize(q) != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline failed");
+  if (!st) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) mch::set_last_ms(ms);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   release();
   return st;
 }

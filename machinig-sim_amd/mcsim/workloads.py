@@ -146,3 +146,42 @@ def zigzag_segments(n_segments=5000, width=51.1, seed=1):
             direction = -direction
         layer += 1
     return segs[:n_segments]
+
+
+# ---------------------------------------------------------------- coupled sub-step programs
+
+
+def coupled_program(seed, width, n_lines=4, feed=1000.0, speed=10000.0):
+    """One seeded raster program for the simulator's coupled sub-step loop: a rapid to a
+    start point beside the stock, a plunge at a third of the feed (end edges cut), then
+    n_lines raster passes along x across the stock at a depth d ~ U(0.5, 3) with a stepover
+    ~ U(2, 5) along y, alternating direction.  Returns the discretised path as lists
+    (x, y, z, rapid, line, F, S), point 0 being the start of the first move."""
+    rng = ParkMiller(seed)
+    d = rng.uniform(0.5, 3.0)
+    step = rng.uniform(2.0, 5.0)
+    y = rng.uniform(4.0, 10.0)
+    pts = [(-6.0, y, 5.0, 1), (-6.0, y, 2.0, 1), (2.0, y, 2.0, 1), (2.0, y, -d, 0)]
+    F = [None, None, None, feed / 3]
+    direction = 1
+    x = 2.0
+    for k in range(n_lines):
+        x = width - 2.0 if direction > 0 else 2.0
+        pts.append((x, y, -d, 0)); F.append(feed)
+        if k + 1 < n_lines:
+            y += step
+            pts.append((x, y, -d, 0)); F.append(feed)
+        direction = -direction
+    xs = [p[0] for p in pts]
+    ys = [p[1] for p in pts]
+    zs = [p[2] for p in pts]
+    rapid = [p[3] for p in pts]
+    line = list(range(1, len(pts) + 1))
+    S = [speed] * len(pts)
+    return xs, ys, zs, rapid, line, F, S
+
+
+# the coupled programs' tool, modes and material (the simulator's defaults: a 10 mm
+# three-flute end mill on one mode per axis, A5052 with the size effect and process damping)
+COUPLED_TOOL = dict(N=3, R=5.0, helixDeg=30.0, type="flat", fluteLen=25.0, runout=0.005, runoutPhase=0.0)
+COUPLED_MODES = [(2.0e4, 1800.0, 0.03)]
