@@ -181,7 +181,7 @@ def coupled(n_prog, n_lines, cpu_threads, cpu):
                substeps_per_s=nsub / wall, engine_steps_per_s=steps / wall,
                roofline={"bound": "hbm", "achieved": 8.0 * nodes / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": 8.0 * nodes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                         "kernel": "sim_run", "traffic": None},
+                         "kernel": "sim_run", "traffic": profiled_traffic("sim_run")},
                note="algorithmic bytes = 8 B x bbox nodes of both stamps per sub-step; one workgroup per "
                     "program runs its sub-steps in order (each reads the stock the previous one cut), so "
                     "the kernel is bound by the latency of that chain, far below the HBM roof")
