@@ -68,8 +68,10 @@ struct EngDev {
   double* ev;    // [ev_cap][3]: vol, phi, z
 };
 
+// the step loop runs on one wavefront: its lanes' LDS and delay-line accesses are ordered at
+// wavefront scope (a workgroup-scope fence would wait for every store to leave the CU)
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
