@@ -440,6 +440,14 @@ typedef struct {
 /* n programs, one wavefront each, one launch; the stocks and engines must be distinct */
 int mc_sim_run(const mc_sim_program* progs, int n);
 
+/* ---- V8's Math.* on the host (no reference counterpart as an entry point) ----
+ * The fdlibm-method functions the library's kernels use, for host code that must give the
+ * reference's doubles (the path discretizer's arcs, src/gcode.js:443-460): out[i] =
+ * f(x[i]) or f(x[i], y[i]) (atan2(x, y) is Math.atan2(x[i], y[i])). */
+enum { MC_JS_SIN = 0, MC_JS_COS = 1, MC_JS_TAN = 2, MC_JS_ATAN = 3, MC_JS_ATAN2 = 4, MC_JS_ACOS = 5,
+       MC_JS_POW = 6, MC_JS_HYPOT = 7 };
+int mc_js_math(int32_t fn, const double* x, const double* y, double* out, int64_t n);
+
 /* ---- Analytic (zero-order) stability lobes, host side ---- */
 
 /* stabilityLobes parameters (src/sld.js:23): flutes, Ktc, Krc, the engagement arc, the

@@ -479,6 +479,89 @@ MC_HD double fmod_2pi(double x) {
   return __builtin_copysign(r, x);
 }
 
+// Math.atan2 (fdlibm e_atan2 as FreeBSD's msun keeps it, which V8 ports: the quadrant from
+// the signs, the special values, then atan(|y/x|) corrected by pi with its low part)
+MC_HD double atan2_js(double y, double x) {
+  constexpr double tiny = 1.0e-300, pi_o_4 = 7.8539816339744827900e-01, pi_o_2 = 1.5707963267948965580e+00,
+                   pi = 3.1415926535897931160e+00, pi_lo = 1.2246467991473531772e-16;
+  const int32_t hx = hiw(x), ix = hx & 0x7fffffff, hy = hiw(y), iy = hy & 0x7fffffff;
+  const uint32_t lx = low(x), ly = low(y);
+  if ((uint32_t)(ix | ((lx | (0u - lx)) >> 31)) > 0x7ff00000u || (uint32_t)(iy | ((ly | (0u - ly)) >> 31)) > 0x7ff00000u)
+    return x + y;   // a NaN
+  if (((hx - 0x3ff00000) | (int32_t)lx) == 0) return atan_js(y);   // x = 1
+  int m = ((hy >> 31) & 1) | ((hx >> 30) & 2);                      // 2 sign(x) + sign(y)
+  if ((iy | (int32_t)ly) == 0) {                                     // y = 0
+    if (m < 2) return y;
+    return m == 2 ? pi + tiny : -pi - tiny;
+  }
+  if ((ix | (int32_t)lx) == 0) return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;   // x = 0
+  if (ix == 0x7ff00000) {                                            // x infinite
+    if (iy == 0x7ff00000) {
+      if (m == 0) return pi_o_4 + tiny;
+      if (m == 1) return -pi_o_4 - tiny;
+      if (m == 2) return 3.0 * pi_o_4 + tiny;
+      return -3.0 * pi_o_4 - tiny;
+    }
+    if (m == 0) return 0.0;
+    if (m == 1) return -0.0;
+    if (m == 2) return pi + tiny;
+    return -pi - tiny;
+  }
+  if (iy == 0x7ff00000) return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;   // y infinite
+  const int k = (iy - ix) >> 20;
+  double z;
+  if (k > 60) { z = pi_o_2 + 0.5 * pi_lo; m &= 1; }   // |y/x| > 2^60: +-pi/2 whatever x's sign
+  else if (hx < 0 && k < -60) z = 0.0;           // |y|/x < -2^60
+  else z = atan_js(fabs_(y / x));
+  if (m == 0) return z;
+  if (m == 1) return -z;
+  if (m == 2) return pi - (z - pi_lo);
+  return (z - pi_lo) - pi;
+}
+
+// Math.acos (fdlibm e_acos: a rational approximation of asin around 0 for |x| < 1/2, and
+// through sqrt((1 -+ x) / 2) beyond, with pi/2 split in high and low parts)
+MC_HD double acos_js(double x) {
+  constexpr double one = 1.0, pi = 3.14159265358979311600e+00, pio2_hi = 1.57079632679489655800e+00,
+                   pio2_lo = 6.12323399573676603587e-17;
+  constexpr double pS0 = 1.66666666666666657415e-01, pS1 = -3.25565818622400915405e-01,
+                   pS2 = 2.01212532134862925881e-01, pS3 = -4.00555345006794114027e-02,
+                   pS4 = 7.91534994289814532176e-04, pS5 = 3.47933107596021167570e-05;
+  constexpr double qS1 = -2.40339491173441421878e+00, qS2 = 2.02094576023350569471e+00,
+                   qS3 = -6.88283971605453293030e-01, qS4 = 7.70381505559019352791e-02;
+  const int32_t hx = hiw(x), ix = hx & 0x7fffffff;
+  if (ix >= 0x3ff00000) {                                            // |x| >= 1
+    if (((ix - 0x3ff00000) | (int32_t)low(x)) == 0) return hx > 0 ? 0.0 : pi + 2.0 * pio2_lo;
+    return (x - x) / (x - x);                                        // NaN
+  }
+  if (ix < 0x3fe00000) {                                             // |x| < 1/2
+    if (ix <= 0x3c600000) return pio2_hi + pio2_lo;
+    const double z = x * x;
+    const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    const double q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    const double r = p / q;
+    return pio2_hi - (x - (pio2_lo - x * r));
+  }
+  if (hx < 0) {                                                      // x < -1/2
+    const double z = (one + x) * 0.5;
+    const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    const double q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    const double s = __builtin_sqrt(z);
+    const double r = p / q;
+    const double w = r * s - pio2_lo;
+    return pi - 2.0 * (s + w);
+  }
+  const double z = (one - x) * 0.5;                                  // x > 1/2
+  const double s = __builtin_sqrt(z);
+  const double df = clear_lo(s);
+  const double c = (z - df * df) / (s + df);
+  const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+  const double q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+  const double r = p / q;
+  const double w = r * s + c;
+  return 2.0 * (df + w);
+}
+
 // Math.round for the engine's positive arguments: halves round up
 MC_HD double round_js(double x) {
   double r = __builtin_floor(x);

@@ -30,7 +30,7 @@ EXPORTS = [
     "mc_engine_set_wear", "mc_engine_set_work", "mc_engine_set_engagement", "mc_engine_set_end_engagement",
     "mc_engine_set_maps", "mc_engine_get_state", "mc_engine_set_state", "mc_engine_run", "mc_engine_run_batch",
     "mc_engine_read_stats", "mc_engine_chip_events", "mc_engine_chatter_index", "mc_engine_read_rings",
-    "mc_sim_run",
+    "mc_sim_run", "mc_js_math",
 ]
 
 _LIB = None
@@ -111,6 +111,7 @@ def load():
     L.mc_engine_chatter_index.argtypes = [P, P]
     L.mc_engine_read_rings.argtypes = [P, P, P, P, P]
     L.mc_sim_run.argtypes = [C.POINTER(SimProgram), C.c_int]
+    L.mc_js_math.argtypes = [C.c_int32, P, P, P, C.c_int64]
     L.mc_set_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     L.mc_get_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     _LIB = L
@@ -291,6 +292,19 @@ def engine_run_batch(engines, nsteps):
     ptrs = (C.c_void_p * max(1, n))(*[e.p.value for e in engines])
     steps = np.ascontiguousarray(nsteps, dtype=np.int64)
     _check(load().mc_engine_run_batch(ptrs, steps.ctypes.data, n))
+
+
+JS_MATH = dict(sin=0, cos=1, tan=2, atan=3, atan2=4, acos=5, pow=6, hypot=7)
+
+
+def js_math(fn, x, y=None):
+    """V8's Math.<fn> on the host (mc_js_math): elementwise over x (and y); no GPU call"""
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    yy = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64) if y is not None else None
+    out = np.empty_like(x)
+    _check(load().mc_js_math(JS_MATH[fn], x.ctypes.data, yy.ctypes.data if yy is not None else None,
+                             out.ctypes.data, x.size))
+    return out
 
 
 def sim_run(programs):

@@ -136,3 +136,63 @@ def sim_substeps(x, y, z, rapid, line, F, S, N, hx):
                 break
     arr = (Substep * max(1, len(rows)))(*rows)
     return arr, len(rows)
+
+
+CHORD_TOL = 0.001   # largest chord error of an arc [mm] (src/gcode.js:433)
+
+
+def discretize(moves, max_seg=0.5):
+    """The path discretizer (src/gcode.js:435-466) in the reference's arithmetic: moves ->
+    polyline points, each carrying the attributes of the segment ending at it.  An arc is
+    cut into segs = max(2 (1 for a null arc), ceil(|sweep| r / max_seg), ceil(|sweep| /
+    dAng)) chords, dAng the angle whose chord lies CHORD_TOL inside the arc; its last point
+    snaps to the move's end.  atan2 / acos / cos / sin are V8's (the library's host math).
+
+    moves: dicts with from, to ([x, y, z]), rapid, F, S, line, and arc None or {ca, cb, cw,
+    axes ([a, b, n]), sweep (None unless set by cutter compensation)}.
+    Returns a dict of lists x, y, z, rapid, F, S, line."""
+    from . import lib
+    out = dict(x=[], y=[], z=[], rapid=[], F=[], S=[], line=[])
+
+    def put(p, rapid, F, S, line):
+        out["x"].append(p[0]); out["y"].append(p[1]); out["z"].append(p[2])
+        out["rapid"].append(1 if rapid else 0); out["F"].append(F); out["S"].append(S); out["line"].append(line)
+
+    if not moves:
+        return out
+    m0 = moves[0]
+    put(list(m0["from"]), True, m0["F"], m0["S"], m0["line"])
+    AX = {"x": 0, "y": 1, "z": 2}
+    two_pi = 2 * math.pi
+    for mv in moves:
+        frm, to, arc = mv["from"], mv["to"], mv.get("arc")
+        if not arc:
+            put(list(to), mv["rapid"], mv["F"], mv["S"], mv["line"])
+            continue
+        a, b, n = (AX[k] for k in (arc.get("axes") or ["x", "y", "z"]))
+        ca, cb = arc["ca"], arc["cb"]
+        r = hypot_js(frm[a] - ca, frm[b] - cb)
+        a0, a1 = lib.js_math("atan2", [frm[b] - cb, to[b] - cb], [frm[a] - ca, to[a] - ca])
+        a0, a1 = float(a0), float(a1)
+        sweep = a1 - a0
+        if arc.get("sweep") is not None:
+            sweep = arc["sweep"]
+        elif arc["cw"]:
+            if sweep >= -1e-9:
+                sweep -= two_pi
+        elif sweep <= 1e-9:
+            sweep += two_pi
+        d_ang = 2 * float(lib.js_math("acos", [max(-1.0, 1 - CHORD_TOL / max(r, CHORD_TOL))])[0])
+        segs = max(1 if abs(sweep) * r < 1e-9 else 2, math.ceil(abs(sweep) * r / max_seg),
+                   math.ceil(abs(sweep) / d_ang))
+        ang = [a0 + sweep * i / segs for i in range(1, segs + 1)]
+        cs, sn = lib.js_math("cos", ang), lib.js_math("sin", ang)
+        for i in range(1, segs + 1):
+            p = [0.0, 0.0, 0.0]
+            p[a] = ca + r * float(cs[i - 1])
+            p[b] = cb + r * float(sn[i - 1])
+            p[n] = frm[n] + (to[n] - frm[n]) * i / segs
+            if i == segs:
+                p = list(to)   # snap to the exact end point
+            put(p, mv["rapid"], mv["F"], mv["S"], mv["line"])
+    return out

@@ -23,6 +23,7 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_simx.json      Simulator runs with a rigid workpiece and the linear force law: the
                         path with feeds and speeds, the engine's configure / wear arguments,
                         every stamp, final maps
+  golden_paths.json     the G-code parser's moves and the path discretizer's polylines
   golden_englog.json    the engine's whole call sequence inside simulator runs (configure,
                         setWear, setWork, setEngagement, setEndEngagement, run, takeStats,
                         chatterIndex) with its state, stats, rings and chip events after
@@ -209,6 +210,13 @@ def build_cases(quick):
                 cell=0.25, flex=True)
     lin = [dict(c, nonlinear=False) for c in cases["sim"] + [ribc]]
     cases["englog"] = lin + [dict(cases["sim"][2]), dict(ribc)]
+    # the path discretizer: the simulator programs and one with full circles, R arcs,
+    # helical arcs, XZ / YZ plane arcs and a tiny arc, at two chord lengths
+    arcs = ["G90 G54 G17", "S9000 M3", "G0 X10 Y10 Z5", "G1 Z-1 F300", "G2 X10 Y10 I5 J0 F800",
+            "G3 X20 Y10 R5", "G2 X25 Y15 Z-2 I0 J5", "G18 G2 X30 Z-1 I2.5 K0", "G19 G3 Y20 Z-2 J2.5 K0",
+            "G17 G1 X35 Y22", "G3 X35.01 Y22.01 R0.01", "G2 X30 Y22 R-3", "G0 Z5", "M30"]
+    cases["paths"] = [dict(program=c["program"], maxSegs=[0.5, 0.2]) for c in cases["sim"]] + \
+        [dict(program="\n".join(arcs), maxSegs=[0.5, 0.2, 2.0])]
     # rigid workpiece, linear force law, flat and ball tools: the device sub-step pipeline
     cases["simx"] = [dict(cases["sim"][0]), dict(cases["sim"][1]), dict(ribc),
                      dict(cases["sim"][0], tool=dict(type="ball", D=8.0))]
@@ -322,7 +330,7 @@ def main():
                 m["k"], m["fn"] = tm["k"], tm["fn"]
             md["k"], md["fn"] = tm["k"], tm["fn"]
     keys = ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
-            "enginex", "englog", "simx")
+            "enginex", "englog", "simx", "paths")
     if "--only" in sys.argv:
         keys = tuple(sys.argv[sys.argv.index("--only") + 1].split(","))
         # engage cases read the stock cases

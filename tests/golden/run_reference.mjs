@@ -252,6 +252,26 @@ if (cases.simx) {
   console.error('simx', (Date.now() - t0) / 1e3, 's');
 }
 
+// the parser's moves and the path discretizer (arcs into chords) on short programs
+if (cases.paths) {
+  const { parseGcode, discretize } = await import(REF + 'gcode.js');
+  out.paths = cases.paths.map(({ program, maxSegs }) => {
+    const parsed = parseGcode(program);
+    const moves = parsed.moves.map(m => ({ from: [m.from.x, m.from.y, m.from.z], to: [m.to.x, m.to.y, m.to.z],
+      rapid: m.rapid ? 1 : 0, F: m.F, S: m.S, line: m.line, tool: m.tool === undefined ? null : m.tool,
+      arc: m.arc ? { ca: m.arc.ca, cb: m.arc.cb, cw: m.arc.cw ? 1 : 0, axes: m.arc.axes || ['x', 'y', 'z'],
+        sweep: m.arc.sweep === undefined ? null : m.arc.sweep } : null }));
+    const pts = maxSegs.map(ms => {
+      const P = discretize(parsed.moves, ms);
+      return { x: b64f64(P.map(p => p.x)), y: b64f64(P.map(p => p.y)), z: b64f64(P.map(p => p.z)),
+        rapid: P.map(p => (p.rapid ? 1 : 0)), F: P.map(p => (p.F === undefined ? null : p.F)),
+        S: P.map(p => (p.S === undefined ? null : p.S)), line: P.map(p => p.line) };
+    });
+    return { moves, pts, warnings: parsed.warnings ? parsed.warnings.length : 0 };
+  });
+  console.error('paths', (Date.now() - t0) / 1e3, 's');
+}
+
 // the engine's own call sequence inside whole simulator runs: every configure, setWear,
 // setWork, setEngagement, setEndEngagement, run, takeStats and chatterIndex the simulator
 // makes, with the maps and the state it set before each run and the engine's state, stats,

@@ -9,6 +9,7 @@ const F = {
   sin: i => Math.sin(x[i]), cos: i => Math.cos(x[i]), tan: i => Math.tan(x[i]),
   pow: i => Math.pow(x[i], y[i]), sq: i => x[i] ** 2, hypot: i => Math.hypot(x[i], y[i]),
   atan: i => Math.atan(x[i]), round: i => Math.round(x[i]), fmod: i => x[i] % y[i],
+  atan2: i => Math.atan2(x[i], y[i]), acos: i => Math.acos(x[i]),
 }[fn];
 for (let i = 0; i < x.length; i++) out[i] = F(i);
 fs.writeFileSync(of, Buffer.from(out.buffer));

@@ -91,3 +91,34 @@ def test_constants_match_their_bit_patterns():
             6.93147182464599609375e-01: 0x3FE62E4300000000, 9.61796700954437255859e-01: 0x3FEEC709E0000000}
     for v, h in want.items():
         assert struct.unpack("<Q", struct.pack("<d", v))[0] == h
+
+
+@needs_node
+def test_library_host_math_matches_node():
+    """the library's own host math (mc_js_math: the fdlibm ports its kernels and host code use)
+    against Node's Math.*, bit for bit, including atan2's quadrants and special values and
+    acos's three ranges"""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
+    from mcsim import lib
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-30, 30, 40000), rng.normal(0, 1e-3, 5000), rng.uniform(-1e6, 1e6, 5000),
+                        [0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-310, 1e300]])
+    y = np.concatenate([rng.uniform(-30, 30, 40000), rng.normal(0, 1e-3, 5000), rng.uniform(-1e6, 1e6, 5000),
+                        [0.0, -0.0, -2.0, 0.0, np.inf, np.inf, 1.0, -1e-310, -1e300]])
+    for fn in ("atan2",):
+        a, b = lib.js_math(fn, x, y), node_math(fn, x, y)
+        assert np.array_equal(bits(a), bits(b)), fn
+    # atan2 with the special operands in every pairing
+    sp = np.array([0.0, -0.0, 1.0, -1.0, 2.5, -2.5, np.inf, -np.inf, 1e-300, -1e300])
+    xx, yy = np.meshgrid(sp, sp)
+    a, b = lib.js_math("atan2", xx.ravel(), yy.ravel()), node_math("atan2", xx.ravel(), yy.ravel())
+    assert np.array_equal(bits(a), bits(b))
+    c = np.concatenate([rng.uniform(-1, 1, 60000), rng.uniform(-0.5, 0.5, 20000), rng.uniform(0.99, 1, 10000),
+                        rng.uniform(-1, -0.99, 10000), [0.0, -0.0, 0.5, -0.5, 1.0, -1.0, 1.5, 1e-20, np.nan]])
+    a, b = lib.js_math("acos", c), node_math("acos", c)
+    nan = np.isnan(a) & np.isnan(b)   # outside [-1, 1]: a NaN either way (its payload is V8's own)
+    assert np.array_equal(bits(a)[~nan], bits(b)[~nan]) and np.array_equal(np.isnan(a), np.isnan(b))
+    for fn, lo, hi in (("sin", -10, 10), ("cos", -10, 10), ("atan", -20, 20)):
+        v = rng.uniform(lo, hi, 20000)
+        assert np.array_equal(bits(lib.js_math(fn, v)), bits(node_math(fn, v))), fn
