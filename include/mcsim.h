@@ -43,7 +43,7 @@ typedef enum {
   MC_ERR_NOMEM = 5    /* device allocation failed */
 } mc_status;
 
-enum { MC_TOOL_FLAT = 0, MC_TOOL_BALL = 1 };
+enum { MC_TOOL_FLAT = 0, MC_TOOL_BALL = 1, MC_TOOL_BULL = 2 };   /* BULL: mc_sim_run only */
 
 /* ---- Path A data ---- */
 
@@ -397,7 +397,8 @@ int mc_tip_lut(int32_t shape, double D, double cornerR, const double* edge_r, in
  * One wavefront per program runs its whole sub-step list without returning to the host;
  * mc_sim_run puts many programs in one launch.  The workpiece is rigid (workpieceFlex off,
  * so setWork(null)), the flank wear fixed for the call (the engine's setWear and rw), and
- * the tool flat or ball with the simulator's end-face shape. */
+ * the tool flat, ball or bull-nose (its profile cutterRadiusAt in the engine's slices, its
+ * tip table in the stamps and the end face, as the simulator sets them up, src/sim.js:190-218). */
 
 /* one sub-step, as the simulator's schedule produces it (src/sim.js:363-381,389-405) */
 typedef struct {
@@ -434,7 +435,11 @@ typedef struct {
   double share;            /* steps a sub-step may run besides its own time: the frame
                               budget's share, max(stepsPerTooth / 4, ...) (src/sim.js:410-412) */
   int32_t nsub;
-  int32_t type;            /* MC_TOOL_FLAT | MC_TOOL_BALL */
+  int32_t type;            /* MC_TOOL_FLAT | MC_TOOL_BALL | MC_TOOL_BULL */
+  const float* lut;        /* MC_TOOL_BULL: the tool's tipLut (mc_tip_lut), lutn entries */
+  int32_t lutn;
+  int32_t pad;
+  double cornerR;          /* MC_TOOL_BULL: corner radius [mm] (cutterRadiusAt, src/assembly.js:23-30) */
 } mc_sim_program;
 
 /* n programs, one wavefront each, one launch; the stocks and engines must be distinct */

@@ -298,7 +298,8 @@ int mc_engine_set_engagement(mc_engine* e, double ux, double uy, double fz, doub
 // setEndEngagement(fza, rOut, zAt), src/cutting.js:130-143
 int mc_engine_set_end_engagement(mc_engine* e, double fza, double rOut, const double* rings) {
   if (int st = check_engine(e)) return st;
-  eng_set_end_engagement(e->h, fza, rOut, rings, 0, 0.0);
+  const ToolShape flat{0, 0.0, 0.0, nullptr, 0};
+  eng_set_end_engagement(e->h, fza, rOut, rings, flat);
   return MC_OK;
 }
 

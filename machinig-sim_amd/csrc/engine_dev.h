@@ -425,10 +425,30 @@ __host__ __device__ __forceinline__ void eng_set_work_null(EngDev& h) {
   h.runMeanW = 0; h.wk = 0; h.wm = 1; h.wc = 0; h.w = 0; h.wv = 0; h.wnx = 1; h.wny = 0;
 }
 
-// setEngagement(u, fz, tipZ, ap), src/cutting.js:98-124: the flat / ball slices, or the
-// caller's profile radii sR / sK (tool.rAt)
+// the simulator's tool shapes as the engine and the stamps see them: flat, ball, or bull-nose
+// with corner radius rc and its tip table (lut of lutn heights every MC_LUT_DR)
+struct ToolShape {
+  int ball;           // the engine's ball rule and the stamp's sphere
+  double Rt;          // tool radius
+  double rc;          // bull-nose corner radius (> 0: the profile rAt below)
+  const float* lut;   // tip table or null
+  int lutn;
+};
+
+// cutterRadiusAt (src/assembly.js:23-30) of a bull-nose tool
+__host__ __device__ __forceinline__ double bull_rat(double R, double rc, double z) {
+  if (rc > 0 && z < rc) {
+    const double q = rc - z;
+    return R - rc + __builtin_sqrt(jmax(0, rc * rc - q * q));
+  }
+  return R;
+}
+
+// setEngagement(u, fz, tipZ, ap), src/cutting.js:98-124: the flat / ball slices, the caller's
+// profile radii sR / sK (tool.rAt), or a bull-nose profile (rc > 0) evaluated here
 __host__ __device__ __forceinline__ int eng_set_engagement(EngDev& h, double ux, double uy, double fz, double tipZ,
-                                                          double ap, const double* sR, const double* sK, bool ball) {
+                                                          double ap, const double* sR, const double* sK, bool ball,
+                                                          double rc = 0) {
   h.ux = ux; h.uy = uy; h.ex = -uy; h.ey = ux;
   h.fz = fz; h.tipZ = tipZ;
   const int nz = ap <= 0 ? 0 : (int)jmin(h.maxSlices, __builtin_ceil(ap / h.dz));
@@ -440,7 +460,11 @@ __host__ __device__ __forceinline__ int eng_set_engagement(EngDev& h, double ux,
     h.sZ[k] = tipZ + zr;
     double r = h.R, kap = 1;
     if (sR) { r = sR[k]; kap = sK[k]; }
-    else if (ball && zr < h.R) {
+    else if (rc > 0) {
+      r = jmax(1e-3, bull_rat(h.R, rc, zr));
+      const double dr = (bull_rat(h.R, rc, zr + 0.05) - bull_rat(h.R, rc, jmax(0, zr - 0.05))) / (zr < 0.05 ? 0.05 + zr : 0.1);
+      kap = 1 / __builtin_sqrt(1 + dr * dr);
+    } else if (ball && zr < h.R) {
       const double q = h.R - zr;
       r = __builtin_sqrt(jmax(1e-6, h.R * h.R - q * q));
       kap = r / h.R;
@@ -451,16 +475,20 @@ __host__ __device__ __forceinline__ int eng_set_engagement(EngDev& h, double ux,
   return nz;
 }
 
-// the end-face shape zAt of the simulator's tools (src/sim.js:216-218): 0 for a flat end,
-// the sphere for a ball (radius Rt)
-__host__ __device__ __forceinline__ double eng_zat(int ball, double Rt, double r) {
-  return ball ? Rt - __builtin_sqrt(jmax(0, Rt * Rt - r * r)) : 0.0;
+// the end-face shape zAt of the simulator's tools (src/sim.js:216-218): the tip table when
+// there is one, the sphere for a ball, 0 for a flat end
+__host__ __device__ __forceinline__ double eng_zat(const ToolShape& t, double r) {
+  if (t.lut) {
+    const double i = mcm::round_js(r / MC_LUT_DR);
+    return (double)t.lut[(int)jmin(t.lutn - 1, i)];
+  }
+  return t.ball ? t.Rt - __builtin_sqrt(jmax(0, t.Rt * t.Rt - r * r)) : 0.0;
 }
 
 // setEndEngagement(fza, rOut, zAt), src/cutting.js:130-143: rings from the caller (R, cos,
-// z, ds) or from zAt = eng_zat(ball, Rt, .)
+// z, ds) or from zAt = eng_zat(shape, .)
 __host__ __device__ __forceinline__ void eng_set_end_engagement(EngDev& h, double fza, double rOut,
-                                                               const double* rings, int ball, double Rt) {
+                                                               const double* rings, const ToolShape& t) {
   h.fza = fza > 0 ? fza : 0;
   h.endR = rOut;
   const double dr = rOut / ERINGS;
@@ -470,10 +498,10 @@ __host__ __device__ __forceinline__ void eng_set_end_engagement(EngDev& h, doubl
       h.ringZ[j] = rings[2 * ERINGS + j]; h.ringDs[j] = rings[3 * ERINGS + j];
     } else {
       const double r = (j + 0.5) * dr;
-      const double slope = (eng_zat(ball, Rt, jmin(rOut, r + 0.25 * dr)) - eng_zat(ball, Rt, jmax(0, r - 0.25 * dr))) / (0.5 * dr);
+      const double slope = (eng_zat(t, jmin(rOut, r + 0.25 * dr)) - eng_zat(t, jmax(0, r - 0.25 * dr))) / (0.5 * dr);
       const double c = jmax(0.22, 1 / __builtin_sqrt(1 + slope * slope));
       h.ringR[j] = r; h.ringCos[j] = c; h.ringDs[j] = dr / c;
-      h.ringZ[j] = eng_zat(ball, Rt, r);
+      h.ringZ[j] = eng_zat(t, r);
     }
   }
 }

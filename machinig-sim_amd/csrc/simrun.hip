@@ -41,8 +41,9 @@ struct ProgDev {
   StockDev act, ide;
   const mc_substep* sub;
   mc_substep_out* out;
-  double R, fluteLen, rw, share;
-  int32_t nsub, ball, N, pad;
+  const float* lut;   // bull-nose: the tip table (null otherwise)
+  double R, fluteLen, rw, share, rc;
+  int32_t nsub, ball, N, lutn;
 };
 
 // Stock.sample (src/stock.js:85-92)
@@ -102,18 +103,18 @@ __device__ __forceinline__ Box stamp_box(const StockDev& S, double cx, double cy
   return b;
 }
 
-// Stock.stamp's node loop (src/stock.js:129-155), flat or ball without a profile table, by
+// Stock.stamp's node loop (src/stock.js:129-155), flat, ball or a tip table (lut), by
 // threads t = 0..nt-1 of the workgroup: the nodes are independent, so they are taken in any
 // order, U at a time with their loads issued before any store.  Returns the thread's part
 // of the removed height sum; a node changed sets *touched.
 __device__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ, double R, bool ball,
-                             double errVal, double mux, double muy, double mfeed, int32_t line, int t, int nt,
-                             int* touched) {
+                             const float* lut, int lutn, double errVal, double mux, double muy, double mfeed,
+                             int32_t line, int t, int nt, int* touched) {
   if (!bx.ok) return 0;
   const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band), Rw2 = Rw * Rw;
   const double R2 = R * R;
-  const double mf = mfeed > 0 && !ball ? mfeed : 0;
-  const double zRim = tipZ + (ball ? R : 0);
+  const double mf = mfeed > 0 && !ball && !lut ? mfeed : 0;
+  const double zRim = tipZ + (lut ? (double)lut[lutn - 1] : ball ? R : 0);
   const int W = bx.i1 - bx.i0 + 1, total = W * (bx.j1 - bx.j0 + 1);
   constexpr int U = 8;
   double vol = 0;
@@ -144,7 +145,8 @@ __device__ double stamp_part(const StockDev& S, const Box& bx, double cx, double
         }
         continue;
       }
-      const double z = ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
+      const double z = lut ? tipZ + (double)lut[(int)jmin(lutn - 1, mcm::round_js(__builtin_sqrt(r2) / MC_LUT_DR))]
+                       : ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
       double h = hv[u];
       if (h > z) {
         vol += h - z;
@@ -184,6 +186,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   const double PI = 3.141592653589793;
   const bool ball = P.ball != 0;
   const double R = P.R;
+  const ToolShape shape{P.ball, R, P.rc, P.lut, P.lutn};
   const double cell = jmax(P.act.hx, P.act.hy);
   double fzLast = 0;   // this.fz: the last cutting sub-step's feed per tooth
   for (int k = 0; k < P.nsub; k++) {
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
         mcm::sincos_js(phi, sn, cs);
         const double ex = sn * s.ux + cs * vx, ey = sn * s.uy + cs * vy;
         const double r = (j + 0.5) / MC_END_RINGS * rOut;
-        const double zEdge = s.z + eng_zat(ball, R, r);
+        const double zEdge = s.z + eng_zat(shape, r);
         const double d = jmax(0, sample_at(P.act, s.x + r * ex, s.y + r * ey) - zEdge);
         E.Eb[q] = (float)d;
         dmax = jmax(dmax, d);
@@ -236,8 +239,8 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     if (cut) {
       // setEngagement, setEndEngagement, setWork(null) (src/sim.js:403-408), on thread 0
       if (tid == 0) {
-        eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball);
-        eng_set_end_engagement(E, endCut ? s.fza : 0, haveEnd ? rOut : 0, nullptr, ball, R);
+        eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball, P.rc);
+        eng_set_end_engagement(E, endCut ? s.fza : 0, haveEnd ? rOut : 0, nullptr, shape);
         eng_set_work_null(E);
         E.hMax = jmax(0.1, 3 * E.fz);
       }
@@ -247,12 +250,15 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
       const double want = jmax(1, mcm::round_js(s.dtm / E.dt));
       n = (int)jmax(1, jmin(want, __builtin_floor(P.share)));
       if (tid < 64) engine_steps(E, n, L);
-      else stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64, &flags[0]);
+      else stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, P.lut, P.lutn, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64,
+                      &flags[0]);
       fzLast = s.fzEff;
     } else {
       // out of the cut the vibration decays (:427-428)
       if (tid == 0) { E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5; }
-      if (tid >= 64) stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64, &flags[0]);
+      if (tid >= 64)
+        stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, P.lut, P.lutn, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64,
+                   &flags[0]);
     }
     __threadfence_block();
     __syncthreads();
@@ -262,8 +268,8 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     const double maxF = cut ? E.runMaxF : 0.0;
     const double ax = s.x + dx, ay = s.y + dy, az = s.z + (ball ? P.rw : 0), aR = R - P.rw;
     const Box ba = stamp_box(P.act, ax, ay, az, aR);
-    const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, errVal, s.ux, s.uy, fzLast * P.N, s.line, tid, NT,
-                                   &flags[1]);
+    const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, P.lut, P.lutn, errVal, s.ux, s.uy, fzLast * P.N,
+                                   s.line, tid, NT, &flags[1]);
     const double vol = block_sum(part, red) * P.act.hx * P.act.hy;
     if (tid == 0) {
       if (flags[0]) join_rect(P.ide, bi);
@@ -296,7 +302,14 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (!g.actual || !g.ideal || !g.engine || g.actual == g.ideal)
       return mch::fail(MC_ERR_ARG, "program %d: needs an engine and two distinct stocks", p);
     if (g.nsub < 0 || (g.nsub > 0 && !g.sub)) return mch::fail(MC_ERR_ARG, "program %d: null sub-steps", p);
-    if (g.type != MC_TOOL_FLAT && g.type != MC_TOOL_BALL) return mch::fail(MC_ERR_ARG, "program %d: flat or ball tools", p);
+    if (g.type != MC_TOOL_FLAT && g.type != MC_TOOL_BALL && g.type != MC_TOOL_BULL)
+      return mch::fail(MC_ERR_ARG, "program %d: flat, ball or bull-nose tools", p);
+    if (g.type == MC_TOOL_BULL) {
+      if (!g.lut || g.lutn < 1 || g.lutn > (1 << 20) || !(g.cornerR >= 0))
+        return mch::fail(MC_ERR_ARG, "program %d: a bull-nose tool needs its tip table and corner radius", p);
+      for (int i = 0; i < g.lutn; i++)
+        if (!std::isfinite(g.lut[i])) return mch::fail(MC_ERR_ARG, "program %d: non-finite tip table", p);
+    }
     if (!(g.R > 0) || !(g.rw >= 0) || !(g.rw < g.R) || !(g.fluteLen > 0) || !(g.share >= 1))
       return mch::fail(MC_ERR_ARG, "program %d: invalid radius, wear, flute length or share", p);
     for (const void* q : {(const void*)g.actual, (const void*)g.ideal, (const void*)g.engine})
@@ -324,31 +337,45 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
                      i.zTop, i.zBottom, i.band};
     d.sub = nullptr; d.out = nullptr;
     d.R = g.R; d.fluteLen = g.fluteLen; d.rw = g.rw; d.share = g.share;
-    d.nsub = g.nsub; d.ball = g.type == MC_TOOL_BALL; d.N = m->N; d.pad = 0;
+    d.nsub = g.nsub; d.ball = g.type == MC_TOOL_BALL; d.N = m->N;
+    d.lut = nullptr; d.lutn = g.type == MC_TOOL_BULL ? g.lutn : 0;
+    d.rc = g.type == MC_TOOL_BULL ? std::min(g.R, g.cornerR) : 0.0;
   }
   std::lock_guard<std::mutex> lk(mch::lock());
   hipStream_t q = mch::stream();
   // sub-steps in, results out: one device buffer for every program
-  size_t nin = 0;
-  for (int p = 0; p < n; p++) nin += (size_t)progs[p].nsub;
+  size_t nin = 0, nlut = 0;
+  for (int p = 0; p < n; p++) {
+    nin += (size_t)progs[p].nsub;
+    if (progs[p].type == MC_TOOL_BULL) nlut += (size_t)progs[p].lutn;
+  }
+  float* dlut = nullptr;
   mc_substep* dsub = nullptr;
   mc_substep_out* dout = nullptr;
   ProgDev* dprog = nullptr;
   auto release = [&]() {
+    if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
     if (dout) (void)hipFree(dout);
     if (dprog) (void)hipFree(dprog);
   };
   if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
-      hipMalloc(&dprog, sizeof(ProgDev) * (size_t)n) != hipSuccess) {
+      hipMalloc(&dprog, sizeof(ProgDev) * (size_t)n) != hipSuccess ||
+      hipMalloc(&dlut, sizeof(float) * std::max<size_t>(1, nlut)) != hipSuccess) {
     release();
     return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
   }
-  size_t off = 0;
+  size_t off = 0, loff = 0;
   int st = MC_OK;
   for (int p = 0; p < n && !st; p++) {
     const mc_sim_program& g = progs[p];
+    if (g.type == MC_TOOL_BULL) {
+      pd[(size_t)p].lut = dlut + loff;
+      if (hipMemcpyAsync(dlut + loff, g.lut, sizeof(float) * (size_t)g.lutn, hipMemcpyHostToDevice, q) != hipSuccess)
+        st = mch::fail(MC_ERR_HIP, "tip table upload");
+      loff += (size_t)g.lutn;
+    }
     pd[(size_t)p].sub = dsub + off;
     pd[(size_t)p].out = dout + off;
     if (g.nsub > 0 && hipMemcpyAsync(dsub + off, g.sub, sizeof(mc_substep) * (size_t)g.nsub, hipMemcpyHostToDevice, q) != hipSuccess)

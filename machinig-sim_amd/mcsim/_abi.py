@@ -11,7 +11,7 @@ MC_MAX_SLICES = 64
 MC_CHATTER_IDX = 0.45
 
 MC_OK, MC_ERR_ARG, MC_ERR_LIMIT, MC_ERR_NODEV, MC_ERR_HIP, MC_ERR_NOMEM = range(6)
-MC_TOOL_FLAT, MC_TOOL_BALL = 0, 1
+MC_TOOL_FLAT, MC_TOOL_BALL, MC_TOOL_BULL = 0, 1, 2
 
 
 class Tool(C.Structure):
@@ -129,7 +129,8 @@ class SimProgram(C.Structure):
     _fields_ = [("actual", C.c_void_p), ("ideal", C.c_void_p), ("engine", C.c_void_p),
                 ("sub", C.POINTER(Substep)), ("out", C.POINTER(SubstepOut)),
                 ("R", C.c_double), ("fluteLen", C.c_double), ("rw", C.c_double), ("share", C.c_double),
-                ("nsub", C.c_int32), ("type", C.c_int32)]
+                ("nsub", C.c_int32), ("type", C.c_int32), ("lut", C.POINTER(C.c_float)), ("lutn", C.c_int32),
+                ("pad", C.c_int32), ("cornerR", C.c_double)]
 
 
 MC_STOCK_B = 8

@@ -309,21 +309,28 @@ def js_math(fn, x, y=None):
 
 def sim_run(programs):
     """the simulator's sub-step loop on the device for several programs in one launch.
-    programs: dicts with actual, ideal (Stock), engine (Engine), sub (Substep array), R,
-    fluteLen, rw, share, type (MC_TOOL_*); returns a SubstepOut array per program"""
+    programs: dicts with actual, ideal (Stock), engine (Engine), sub (Substep array) and
+    optionally nsub (its length), R, fluteLen, rw, share, type (MC_TOOL_*), and for a
+    bull-nose tool lut (its tipLut) and cornerR; returns a SubstepOut list per program"""
     n = len(programs)
     arr = (SimProgram * max(1, n))()
-    outs = []
+    outs, keep = [], []
     for i, g in enumerate(programs):
         sub = g["sub"]
-        out = (SubstepOut * max(1, len(sub)))()
-        outs.append(out)
+        ns = int(g.get("nsub", len(sub)))
+        out = (SubstepOut * max(1, ns))()
+        outs.append((out, ns))
+        lut = g.get("lut")
+        lut_arr = np.ascontiguousarray(lut, dtype=np.float32) if lut is not None else None
+        keep.append(lut_arr)
         arr[i] = SimProgram(g["actual"].p.value, g["ideal"].p.value, g["engine"].p.value,
                             C.cast(sub, C.POINTER(Substep)), C.cast(out, C.POINTER(SubstepOut)),
                             float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g["share"]),
-                            len(sub), int(g["type"]))
+                            ns, int(g["type"]),
+                            lut_arr.ctypes.data_as(C.POINTER(C.c_float)) if lut_arr is not None else None,
+                            len(lut_arr) if lut_arr is not None else 0, 0, float(g.get("cornerR", 0.0)))
     _check(load().mc_sim_run(arr, n))
-    return [o[:len(g["sub"])] for o, g in zip(outs, programs)]
+    return [o[:ns] for o, ns in outs]
 
 
 class Stock:

@@ -71,6 +71,8 @@ void or_engine_read_rings(const or_engine *e, float *hist, float *ts, float *ws,
 double or_engine_dt(const or_engine *e);
 int or_engine_flutes(const or_engine *e);
 double or_engine_run_max_f(const or_engine *e);
+double or_engine_dz(const or_engine *e);
+int or_engine_max_slices(const or_engine *e);
 
 
 /* sld.c: zero-order lobes (src/sld.js) */
@@ -134,7 +136,8 @@ void or_tool_model(double D, double stickout, double kSpindle, double mHolder, d
 double or_fill_end_engagement(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
                               double R, int ball, float *Eb, double *rOut_out);
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
-               double fluteLen, double rw, double share, int type, mc_substep_out *out);
+               double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
+               mc_substep_out *out);
 
 #ifdef __cplusplus
 }

@@ -59,8 +59,8 @@ def lib():
         L.or_engine_chip_events.restype = C.c_int64
         L.or_engine_chatter_index.argtypes = [P, P]
         L.or_engine_read_rings.argtypes = [P, P, P, P, P]
-        L.or_sim_run.argtypes = [P, P, P, C.POINTER(Substep), C.c_int] + [C.c_double] * 4 + [C.c_int,
-                                                                                         C.POINTER(SubstepOut)]
+        L.or_sim_run.argtypes = [P, P, P, C.POINTER(Substep), C.c_int] + [C.c_double] * 4 + \
+            [C.c_int, P, C.c_int, C.c_double, C.POINTER(SubstepOut)]
         L.or_directional_coeffs.argtypes = [C.c_double, C.c_double, C.c_double, P]
         L.or_stability_lobes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_int, C.c_double, C.c_double, C.POINTER(Mode), C.c_int,
@@ -270,11 +270,15 @@ class OracleEngine:
         return h, r[0], r[1], r[2]
 
 
-def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ):
+def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None, cornerR=0.0):
     """the simulator's sub-step loop restated (oracle/simloop.c); returns SubstepOut records"""
+    import numpy as np
     out = (SubstepOut * max(1, n))()
+    la = np.ascontiguousarray(lut, dtype=np.float32) if lut is not None else None
     st = lib().or_sim_run(actual.p, ideal.p, engine.h, C.cast(sub, C.POINTER(Substep)), int(n), float(R),
-                          float(fluteLen), float(rw), float(share), int(typ), out)
+                          float(fluteLen), float(rw), float(share), int(typ),
+                          la.ctypes.data if la is not None else None, len(la) if la is not None else 0,
+                          float(cornerR), out)
     if st:
         raise RuntimeError("or_sim_run: %d" % st)
     return out[:n]

@@ -11,8 +11,11 @@
  *   the decay out of the cut                               src/sim.js:425-429
  *   deflection, the commanded and the actual stamp         src/sim.js:433-440
  *
- * The engine is oracle/cengine.c's object, the stocks oracle/stock.c's.  The end-face shape
- * zAt is the simulator's for a flat end (0) or a ball (src/sim.js:216-218).
+ * The engine is oracle/cengine.c's object, the stocks oracle/stock.c's.  The tool is flat,
+ * ball or bull-nose; the end-face shape zAt is the simulator's (src/sim.js:216-218: the tip
+ * table when the tool has one, the sphere for a ball, 0 for a flat end), and a bull-nose
+ * tool's slices follow cutterRadiusAt (src/assembly.js:23-30) as tool.rAt
+ * (src/sim.js:190-191, src/cutting.js:113-119).
  */
 #include <math.h>
 #include <string.h>
@@ -21,13 +24,30 @@
 
 #define PI 3.141592653589793
 
-static double zat(int ball, double R, double r) {
-  return ball ? R - sqrt(fmax(0.0, R * R - r * r)) : 0.0;
+typedef struct { int ball; double R; const float *lut; int lutn; double rc; } Shape;
+
+static double zat(const Shape *t, double r) {
+  if (t->lut) {
+    double i = or_js_round(r / MC_LUT_DR);
+    if (i > t->lutn - 1) i = t->lutn - 1;
+    return t->lut[(int)i];
+  }
+  return t->ball ? t->R - sqrt(fmax(0.0, t->R * t->R - r * r)) : 0.0;
+}
+
+/* cutterRadiusAt (src/assembly.js:23-30) for a bull-nose corner rc */
+static double bull_rat(double R, double rc, double z) {
+  if (rc > 0 && z < rc) {
+    const double q = rc - z, v = rc * rc - q * q;
+    return R - rc + sqrt(v > 0 ? v : 0);
+  }
+  return R;
 }
 
 /* Simulator.fillEndEngagement (src/sim.js:324-341): Eb[b * 6 + j]; returns dmax, *rOut */
-double or_fill_end_engagement(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
-                              double R, int ball, float *Eb, double *rOut_out) {
+static double fill_end(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
+                       const Shape *t, float *Eb, double *rOut_out) {
+  const double R = t->R;
   int32_t nx, ny;
   double hx, hy, rOut, dmax = 0, vx = -uy, vy = ux;
   int b, j;
@@ -38,7 +58,7 @@ double or_fill_end_engagement(const or_stock *s, double cx, double cy, double ti
     const double ex = sn * ux + cs * vx, ey = sn * uy + cs * vy;
     for (j = 0; j < MC_END_RINGS; j++) {
       const double r = (j + 0.5) / MC_END_RINGS * rOut;
-      const double zEdge = tipZ + zat(ball, R, r);
+      const double zEdge = tipZ + zat(t, r);
       double d = or_stock_sample(s, cx + r * ex, cy + r * ey) - zEdge;
       if (!(d > 0)) d = 0; /* Math.max(0, d): d is never NaN here */
       Eb[b * MC_END_RINGS + j] = (float)d;
@@ -49,25 +69,35 @@ double or_fill_end_engagement(const or_stock *s, double cx, double cy, double ti
   return dmax;
 }
 
+double or_fill_end_engagement(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
+                              double R, int ball, float *Eb, double *rOut_out) {
+  const Shape t = {ball, R, NULL, 0, 0};
+  return fill_end(s, cx, cy, tipZ, ux, uy, &t, Eb, rOut_out);
+}
+
 /* setEndEngagement's rings (src/cutting.js:133-142) for the simulator's zAt */
-static void end_rings(double rOut, int ball, double R, double *g) {
+static void end_rings(double rOut, const Shape *t, double *g) {
   const double dr = rOut / MC_END_RINGS;
   int j;
   for (j = 0; j < MC_END_RINGS; j++) {
     const double r = (j + 0.5) * dr;
     const double a = r + 0.25 * dr < rOut ? r + 0.25 * dr : rOut;
     const double b = r - 0.25 * dr > 0 ? r - 0.25 * dr : 0;
-    const double slope = (zat(ball, R, a) - zat(ball, R, b)) / (0.5 * dr);
+    const double slope = (zat(t, a) - zat(t, b)) / (0.5 * dr);
     double c = 1 / sqrt(1 + slope * slope);
     if (isnan(c)) c = NAN; else if (c < 0.22) c = 0.22;
-    g[j] = r; g[MC_END_RINGS + j] = c; g[2 * MC_END_RINGS + j] = zat(ball, R, r);
+    g[j] = r; g[MC_END_RINGS + j] = c; g[2 * MC_END_RINGS + j] = zat(t, r);
     g[3 * MC_END_RINGS + j] = dr / c;
   }
 }
 
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
-               double fluteLen, double rw, double share, int type, mc_substep_out *out) {
-  const int ball = type == MC_TOOL_BALL;
+               double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
+               mc_substep_out *out) {
+  const int ball = type == MC_TOOL_BALL, bull = type == MC_TOOL_BULL;
+  const Shape shape = {ball, R, bull ? lut : NULL, bull ? lutn : 0, bull ? (cornerR < R ? cornerR : R) : 0};
+  const float *slut = bull ? lut : NULL;
+  const int stype = ball ? MC_TOOL_BALL : MC_TOOL_FLAT;
   float Hb[MC_BINS], Eb[MC_END_BINS * MC_END_RINGS];
   double fzLast = 0;
   int k;
@@ -81,14 +111,31 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     if (!(s->dxy < 1e-6)) hmax = or_fill_engagement(act, s->x, s->y, s->ux, s->uy, R, Hb);
     ap = hmax - s->z > 0 ? hmax - s->z : 0;
     if (fluteLen < ap) ap = fluteLen;
-    if (haveEnd) dmax = or_fill_end_engagement(act, s->x, s->y, s->z, s->ux, s->uy, R, ball, Eb, &rOut);
+    if (haveEnd) dmax = fill_end(act, s->x, s->y, s->z, s->ux, s->uy, &shape, Eb, &rOut);
     endCut = haveEnd && dmax > 2e-4;
     cut = (ap > 0.01 || endCut) && !s->rapid;
     or_engine_get_state(e, &st);
     if (cut) {
       double g[4 * MC_END_RINGS], want;
-      or_engine_set_engagement(e, s->ux, s->uy, s->fzEff, s->z, ap, NULL, NULL, NULL);
-      end_rings(haveEnd ? rOut : 0, ball, R, g);
+      if (bull) {
+        /* setEngagement's rAt slices (src/cutting.js:112-119) */
+        double sr[MC_MAX_SLICES], sk[MC_MAX_SLICES];
+        const double dz = or_engine_dz(e);
+        const int ms = or_engine_max_slices(e);
+        int k, nz = ap <= 0 ? 0 : (int)fmin(ms, ceil(ap / dz));
+        for (k = 0; k < nz; k++) {
+          const double b = dz < ap - k * dz ? dz : ap - k * dz, zr = k * dz + 0.5 * b;
+          const double r0 = bull_rat(R, shape.rc, zr);
+          const double lo = zr - 0.05 > 0 ? zr - 0.05 : 0;
+          const double dr = (bull_rat(R, shape.rc, zr + 0.05) - bull_rat(R, shape.rc, lo)) / (zr < 0.05 ? 0.05 + zr : 0.1);
+          sr[k] = r0 > 1e-3 ? r0 : 1e-3;
+          sk[k] = 1 / sqrt(1 + dr * dr);
+        }
+        or_engine_set_engagement(e, s->ux, s->uy, s->fzEff, s->z, ap, sr, sk, NULL);
+      } else {
+        or_engine_set_engagement(e, s->ux, s->uy, s->fzEff, s->z, ap, NULL, NULL, NULL);
+      }
+      end_rings(haveEnd ? rOut : 0, &shape, g);
       or_engine_set_end_engagement(e, endCut ? s->fza : 0, haveEnd ? rOut : 0, g);
       or_engine_set_work(e, NULL);
       or_engine_set_maps(e, Hb, Eb);
@@ -107,11 +154,11 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     dx = (st.x - st.w * st.wnx) * 1e3;
     dy = (st.y - st.w * st.wny) * 1e3;
     errVal = or_hypot(st.x, st.y) * 1e6;
-    or_stock_stamp_ex(ide, s->x, s->y, s->z, R, type, 0, s->line, NULL, NULL, 0);
+    or_stock_stamp_ex(ide, s->x, s->y, s->z, R, stype, 0, s->line, NULL, slut, shape.lutn);
     {
       const double mark[3] = {s->ux, s->uy, fzLast * or_engine_flutes(e)};
-      vol = or_stock_stamp_ex(act, s->x + dx, s->y + dy, s->z + (ball ? rw : 0), R - rw, type, errVal, s->line, mark,
-                              NULL, 0);
+      vol = or_stock_stamp_ex(act, s->x + dx, s->y + dy, s->z + (ball ? rw : 0), R - rw, stype, errVal, s->line, mark,
+                              slut, shape.lutn);
     }
     if (out) {
       out[k].dx = dx; out[k].dy = dy; out[k].ap = ap; out[k].errVal = errVal; out[k].vol = vol;

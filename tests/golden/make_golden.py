@@ -217,9 +217,9 @@ def build_cases(quick):
             "G17 G1 X35 Y22", "G3 X35.01 Y22.01 R0.01", "G2 X30 Y22 R-3", "G0 Z5", "M30"]
     cases["paths"] = [dict(program=c["program"], maxSegs=[0.5, 0.2]) for c in cases["sim"]] + \
         [dict(program="\n".join(arcs), maxSegs=[0.5, 0.2, 2.0])]
-    # rigid workpiece, linear force law, flat and ball tools: the device sub-step pipeline
+    # rigid workpiece, linear force law, flat, ball and bull-nose tools: the device sub-step pipeline
     cases["simx"] = [dict(cases["sim"][0]), dict(cases["sim"][1]), dict(ribc),
-                     dict(cases["sim"][0], tool=dict(type="ball", D=8.0))]
+                     dict(cases["sim"][0], tool=dict(type="ball", D=8.0)), dict(cases["sim"][2])]
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]

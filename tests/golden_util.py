@@ -335,3 +335,14 @@ def simx_cell(res):
     _, N, R, hd, typ, fl, nl, ro, rph, S, fnMax, mxs, mys, mat = op
     return make_cell(dict(N=N, R=R, helixDeg=hd, type=typ, fluteLen=fl, runout=ro, runoutPhase=rph),
                      mxs, mys, dict(zip(MAT_ORDER, mat)), S, (0.0, 0.0), 0.0, 1, nl), fnMax
+
+
+def simx_tool(case, res, tip_lut):
+    """(type, lut, cornerR) of a simx run's tool for mc_sim_run / the oracle's loop; tip_lut is
+    the library's or the oracle's tipLut"""
+    from mcsim._abi import MC_TOOL_BALL, MC_TOOL_BULL, MC_TOOL_FLAT
+    t = res["type"]
+    if t == "bull":
+        rc = case["tool"]["cornerR"]
+        return MC_TOOL_BULL, tip_lut(MC_TIP_BULL, 2 * res["toolR"], rc), rc
+    return (MC_TOOL_BALL if t == "ball" else MC_TOOL_FLAT), None, 0.0

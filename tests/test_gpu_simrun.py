@@ -3,7 +3,7 @@ stamps per sub-step, src/sim.js:351-449) against whole runs of the reference Sim
 rigid workpiece and the linear force law (golden_simx.json): the deflected stamp positions,
 their vibration values, the engine's step counts, and every map of the actual and the
 commanded stock bit for bit; the removed volume within 1e-12 (its sum order differs).  The
-four programs run in one launch, one wavefront each."""
+five programs (flat, ball and bull-nose tools) run in one launch, one workgroup each."""
 import numpy as np
 import pytest
 
@@ -21,7 +21,6 @@ def _maps(stock):
 
 def test_sim_pipeline_matches_reference_runs():
     from mcsim import lib
-    from mcsim._abi import MC_TOOL_BALL, MC_TOOL_FLAT
     g = G.load("simx")
     progs, keep = [], []
     for case, res in zip(g["cases"], g["results"]):
@@ -36,9 +35,9 @@ def test_sim_pipeline_matches_reference_runs():
         _, kte, kre = res["wears"][0]
         eng.set_wear(kte, kre)
         sub, n = G.simx_program(res)
-        progs.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, R=res["toolR"], fluteLen=res["fluteLen"],
-                          rw=0.0, share=info.stepsPerTooth / 4,
-                          type=MC_TOOL_BALL if res["type"] == "ball" else MC_TOOL_FLAT))
+        typ, lut, rc = G.simx_tool(case, res, lib.tip_lut)
+        progs.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=n, R=res["toolR"], fluteLen=res["fluteLen"],
+                          rw=0.0, share=info.stepsPerTooth / 4, type=typ, lut=lut, cornerR=rc))
         keep.append((case, res, n))
     print("sim pipeline: %d programs, %s sub-steps" % (len(progs), [k[2] for k in keep]), flush=True)
     outs = lib.sim_run(progs)
