@@ -473,6 +473,14 @@ int mc_directional_coeffs(double Kr, double phiSt, double phiEx, double* out);
 int mc_stability_lobes(const mc_lobe_args* a, double* rpm, double* alim, int max_pts, int* lobe_len, int* n_out);
 /* alimAt (src/sld.js:64-76): the lowest lobe covering S, linearly interpolated; +inf if none */
 int mc_alim_at(const double* rpm, const double* alim, const int* lobe_len, int nlobes, double S, double* out);
+/* Simulator.lobes' composite curve for a speed-dependent tool tip (src/sim.js:270-281, the
+ * loop that replaces): node i's diagram, stabilityLobes(nodes[i]) with that node's modes
+ * (modalNow at the node's centre), read by alimAt at S = rpmMin, rpmMin + step, ... while
+ * S < min(hi, rpmMax); a speed no lobe covers reads 1e3.  The points go out node after node
+ * (nodes of several curves may share one call).  *n_out = the points; rpm = alim = NULL with
+ * max_pts = 0 only asks for that count. */
+int mc_composite_lobes(const mc_lobe_args* nodes, int n, double step, double hi, double* rpm, double* alim,
+                       int max_pts, int* n_out);
 /* alimAt of each cell at its own speed, on the lobes (of the cell's tool, modes, arc and
  * coefficients) over [0.8 Smin, 1.2 Smax] of the cells sharing them; +inf if none covers */
 int mc_analytic_limits(const mc_cell* cells, int n, double* out);

@@ -107,6 +107,27 @@ void lobe(const mc_lobe_args& a, const std::vector<Sample>& pts, int j, double l
   }
 }
 
+// the lowest lobe covering S, linearly interpolated (alimAt's inner loops, src/sld.js:66-73)
+void alim_on(const std::vector<LobePt>& L, double S, double& best) {
+  for (size_t i = 1; i < L.size(); i++) {
+    const double ar = L[i - 1].rpm, br = L[i].rpm;
+    if (S >= ar && S <= br) {
+      const double d = br - ar;
+      const double t = (S - ar) / (d != 0 ? d : 1);
+      const double v = L[i - 1].alim + (L[i].alim - L[i - 1].alim) * t;
+      if (v < best) best = v;
+    }
+  }
+}
+
+// speeds a composite node reads: rpmMin, rpmMin + step, ... below min(hi, rpmMax)
+int64_t node_points(const mc_lobe_args& a, double step, double hi) {
+  const double top = std::min(hi, a.rpmMax);
+  int64_t n = 0;
+  for (double S = a.rpmMin; S < top; S += step) n++;
+  return n;
+}
+
 int check_args(const mc_lobe_args* a) {
   if (!a) return mch::fail(MC_ERR_ARG, "null lobe arguments");
   if (a->N < 1 || a->nmx < 1 || a->nmx > MC_MAX_MODES || a->nmy < 1 || a->nmy > MC_MAX_MODES)
@@ -171,6 +192,55 @@ int mc_alim_at(const double* rpm, const double* alim, const int* lobe_len, int n
     off += lobe_len[j];
   }
   *out = best;
+  return MC_OK;
+}
+
+// Simulator.lobes' composite curve (src/sim.js:270-281): every node's diagram is built and
+// read independently, so the nodes (of one curve or of many curves) run on host threads and
+// each writes its own stretch of the output
+int mc_composite_lobes(const mc_lobe_args* nodes, int n, double step, double hi, double* rpm, double* alim,
+                       int max_pts, int* n_out) {
+  if (n < 0 || (n > 0 && !nodes) || !n_out) return mch::fail(MC_ERR_ARG, "bad arguments");
+  if (!(step > 0) || !std::isfinite(step) || std::isnan(hi)) return mch::fail(MC_ERR_ARG, "need a finite step > 0");
+  std::vector<int64_t> off(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    int st = check_args(&nodes[i]);
+    if (st) return st;
+    const mc_lobe_args& a = nodes[i];
+    if (!std::isfinite(a.rpmMin) || !std::isfinite(a.rpmMax)) return mch::fail(MC_ERR_ARG, "node %d: bad speed window", i);
+    if (std::min(hi, a.rpmMax) - a.rpmMin > step * 1e7) return mch::fail(MC_ERR_ARG, "node %d: over 1e7 speeds", i);
+    off[i + 1] = off[i] + node_points(a, step, hi);
+    if (off[i + 1] > 0x7fffffff) return mch::fail(MC_ERR_ARG, "too many points");
+  }
+  *n_out = (int)off[n];
+  if (!rpm && !alim && max_pts == 0) return MC_OK;  // size query
+  if (!rpm || !alim || max_pts < off[n]) return mch::fail(MC_ERR_ARG, "the curve needs %lld points", (long long)off[n]);
+  const int nthreads = std::max(1, std::min<int>(n, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthreads; t++) {
+    pool.emplace_back([&, t]() {
+      std::vector<Sample> pts;
+      std::vector<LobePt> L;
+      std::vector<double> best;
+      for (int i = t; i < n; i += nthreads) {
+        const mc_lobe_args& a = nodes[i];
+        const int64_t m = off[i + 1] - off[i];
+        best.assign((size_t)m, INFINITY);
+        boundary(a, pts);
+        for (int j = 0; j < a.lobes; j++) {
+          lobe(a, pts, j, a.rpmMin * 0.8, a.rpmMax * 1.2, L);
+          double S = a.rpmMin;
+          for (int64_t q = 0; q < m; q++, S += step) alim_on(L, S, best[q]);
+        }
+        double S = a.rpmMin;
+        for (int64_t q = 0; q < m; q++, S += step) {
+          rpm[off[i] + q] = S;
+          alim[off[i] + q] = best[q] == INFINITY ? 1e3 : best[q];
+        }
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
   return MC_OK;
 }
 

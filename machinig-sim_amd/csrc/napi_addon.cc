@@ -524,6 +524,29 @@ napi_value TipLut(napi_env env, napi_callback_info info) {
   return ta;
 }
 
+// compositeLobes(nodes: mc_lobe_args records, step, hi) -> Float64Array(2n): n speeds, then
+// their n depths
+napi_value CompositeLobes(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  void* nd = nullptr;
+  size_t nl = 0;
+  if (!buffer_of(env, argv[0], &nd, &nl)) return nullptr;
+  const int n = (int)(nl / sizeof(mc_lobe_args));
+  const double step = num(env, argv[1]), hi = num(env, argv[2]);
+  int cnt = 0;
+  int st = mc_composite_lobes((const mc_lobe_args*)nd, n, step, hi, nullptr, nullptr, 0, &cnt);
+  if (st) return throw_mc(env, st);
+  void* out;
+  napi_value ab = new_ab(env, (size_t)cnt * 16, &out);
+  if (!ab) return nullptr;
+  st = mc_composite_lobes((const mc_lobe_args*)nd, n, step, hi, (double*)out, (double*)out + cnt, cnt, &cnt);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)cnt * 2, ab, 0, &ta));
+  return ta;
+}
+
 // stockSample(handle, xy: Float64Array(2n)) -> Float64Array(n)
 napi_value StockSample(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -774,7 +797,7 @@ napi_value Sizes(napi_env env, napi_callback_info) {
   const struct { const char* k; size_t s; } t[] = {
       {"mc_cell", sizeof(mc_cell)}, {"mc_trial", sizeof(mc_trial)}, {"mc_limit", sizeof(mc_limit)},
       {"mc_stamp", sizeof(mc_stamp)}, {"mc_stamp_opt", sizeof(mc_stamp_opt)},
-      {"mc_engage_query", sizeof(mc_engage_query)}, {"mc_engage_result", sizeof(mc_engage_result)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
+      {"mc_engage_query", sizeof(mc_engage_query)}, {"mc_engage_result", sizeof(mc_engage_result)}, {"mc_lobe_args", sizeof(mc_lobe_args)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
   for (auto& e : t) {
     napi_create_uint32(env, (uint32_t)e.s, &v);
     napi_set_named_property(env, o, e.k, v);
@@ -797,7 +820,7 @@ napi_value ModuleInit(napi_env env, napi_value exports) {
       {"engineSetEngagement", EngineSetEngagement}, {"engineSetEndEngagement", EngineSetEndEngagement},
       {"engineRun", EngineRun}, {"engineTakeStats", EngineTakeStats}, {"engineChipEvents", EngineChipEvents},
       {"engineChatterIndex", EngineChatterIndex}, {"engineRings", EngineRings},
-      {"sizes", Sizes}};
+      {"compositeLobes", CompositeLobes}, {"sizes", Sizes}};
   for (auto& f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

@@ -1,5 +1,5 @@
 export { chatterAt, timeDomainLimit, timeDomainLimits, timeDomainLimitsAsync, timeDomainLimitsBracketed, CHATTER_IDX } from './stabtd.js';
-export { stabilityLobes, alimAt, directionalCoeffs } from './sld.js';
+export { stabilityLobes, alimAt, directionalCoeffs, compositeLobes } from './sld.js';
 export { modeSet, frf } from './frf.js';
 export { Stock, tipLut, LUT_DR } from './stock.js';
 export { engagementBatch, fillEngagement, immersion } from './engage.js';

@@ -1,18 +1,20 @@
 // Packing of the reference's plain argument objects into the C records of
-// include/mcsim.h (mc_cell 352 B, mc_stamp 48 B, mc_stamp_opt 32 B), little-endian, for the
+// include/mcsim.h (mc_cell 352 B, mc_stamp 48 B, mc_stamp_opt 32 B, mc_lobe_args 256 B),
+// little-endian, for the
 // addon.
 import { modeSet } from './frf.js';
 import { native } from './native.js';
 
 export const MC_MAX_MODES = 4, MC_MAX_FLUTES = 8;
 export const CELL_BYTES = 352, TRIAL_BYTES = 40, LIMIT_BYTES = 56, STAMP_BYTES = 48, STAMP_OPT_BYTES = 32;
+export const LOBE_ARGS_BYTES = 256;
 
 let checked = false;
 function checkSizes() {
   if (checked) return;
   const s = native().sizes();
   if (s.mc_cell !== CELL_BYTES || s.mc_trial !== TRIAL_BYTES || s.mc_limit !== LIMIT_BYTES || s.mc_stamp !== STAMP_BYTES ||
-      s.mc_stamp_opt !== STAMP_OPT_BYTES)
+      s.mc_stamp_opt !== STAMP_OPT_BYTES || s.mc_lobe_args !== LOBE_ARGS_BYTES)
     throw new Error('mcsim.node record sizes do not match this module: ' + JSON.stringify(s));
   checked = true;
 }
@@ -117,4 +119,27 @@ export function packStampOpts(list) {
   const luts = new Float32Array(Math.max(1, total));
   tabs.forEach((t, i) => luts.set(t, lutOff[i]));
   return { opts: buf, luts, lutOff, lutLen };
+}
+
+// one stabilityLobes parameter set (src/sld.js:23: N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin,
+// rpmMax and the modal fields modeSet reads) as an mc_lobe_args record
+export function writeLobeArgs(dv, off, p) {
+  checkSizes();
+  const ms = modeSet(p);
+  if (ms.x.length > MC_MAX_MODES || ms.y.length > MC_MAX_MODES)
+    throw new RangeError('at most ' + MC_MAX_MODES + ' modes per axis');
+  dv.setInt32(off, p.N, true);
+  dv.setInt32(off + 4, p.lobes === undefined ? 8 : p.lobes, true);
+  const D = [p.Ktc, p.Krc, p.phiSt, p.phiEx, p.rpmMin === undefined ? 1000 : p.rpmMin,
+    p.rpmMax === undefined ? 40000 : p.rpmMax];
+  for (let i = 0; i < 6; i++) dv.setFloat64(off + 8 + 8 * i, D[i], true);
+  dv.setInt32(off + 56, ms.x.length, true);
+  dv.setInt32(off + 60, ms.y.length, true);
+  const put = (list, at) => list.forEach((m, i) => {
+    dv.setFloat64(at + 24 * i, m.k, true);
+    dv.setFloat64(at + 24 * i + 8, m.fn, true);
+    dv.setFloat64(at + 24 * i + 16, m.zeta, true);
+  });
+  put(ms.x, off + 64);
+  put(ms.y, off + 64 + 24 * MC_MAX_MODES);
 }

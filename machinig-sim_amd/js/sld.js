@@ -3,6 +3,8 @@
 // and cheap: evaluated on the host, once per diagram, with the same arithmetic order as the
 // reference so the lobes are identical.
 import { modeSet, frf } from './frf.js';
+import { native } from './native.js';
+import { LOBE_ARGS_BYTES, writeLobeArgs } from './pack.js';
 
 // averaged directional factors of the x/y force for an immersion [phiSt, phiEx]
 export function directionalCoeffs(Kr, phiSt, phiEx) {
@@ -80,4 +82,23 @@ export function alimAt(lobes, rpm) {
     }
   }
   return best;
+}
+
+// The composite curve Simulator.lobes builds for a speed-dependent tool tip (src/sim.js:270-281):
+// one diagram per `node` min^-1 of speed, from the modal set at the node's centre
+// (modalAt(S), the simulator's modalNow), read every `step` min^-1 within its own node.
+// base: stabilityLobes' other parameters (N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin, rpmMax).
+// Returns [curve] with curve = [{rpm, alim}], 1e3 where no lobe covers a speed.  All nodes
+// go to the library in one call (mc_composite_lobes, host threads).
+export function compositeLobes(base, modalAt, node = 1000, step = 25) {
+  const lo = base.rpmMin * 0.8, hi = base.rpmMax * 1.1;
+  const starts = [];
+  for (let S0 = lo; S0 < hi; S0 += node) starts.push(S0);
+  const dv = new DataView(new ArrayBuffer(starts.length * LOBE_ARGS_BYTES));
+  starts.forEach((S0, i) => writeLobeArgs(dv, i * LOBE_ARGS_BYTES,
+    Object.assign({}, modalAt(S0 + node / 2), base, { rpmMin: S0, rpmMax: S0 + node })));
+  const r = native().compositeLobes(dv.buffer, step, hi), n = r.length / 2;
+  const curve = new Array(n);
+  for (let i = 0; i < n; i++) curve[i] = { rpm: r[i], alim: r[n + i] };
+  return [curve];
 }

@@ -11,7 +11,7 @@ import numpy as np
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
                    SearchTuning, EngageQuery, EngageResult, EngageSpec, cell_array, MC_OK, MC_SURR_IN,
                    MC_STOCK_B, EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST,
-                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram)
+                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram, MC_MAX_MODES)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -23,7 +23,7 @@ EXPORTS = [
     "mc_stock_read_side", "mc_stamp_batch", "mc_stock_sample", "mc_surrogate_features",
     "mc_mlp_create", "mc_mlp_destroy", "mc_mlp_nparams", "mc_mlp_params", "mc_mlp_set_scaling",
     "mc_mlp_get_scaling", "mc_mlp_grads", "mc_mlp_train", "mc_mlp_predict",
-    "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits",
+    "mc_directional_coeffs", "mc_stability_lobes", "mc_alim_at", "mc_analytic_limits", "mc_composite_lobes",
     "mc_set_search_tuning", "mc_get_search_tuning", "mc_stamp_batch_ex", "mc_stock_read_marks",
     "mc_stock_dirty", "mc_stock_clear_dirty", "mc_stock_block_max", "mc_stock_max_in_disk", "mc_tip_lut",
     "mc_stock_engagement", "mc_engine_trace_ex", "mc_engine_create", "mc_engine_destroy", "mc_engine_configure",
@@ -94,6 +94,7 @@ def load():
     L.mc_stability_lobes.argtypes = [C.POINTER(LobeArgs), P, P, C.c_int, P, P]
     L.mc_alim_at.argtypes = [P, P, P, C.c_int, C.c_double, P]
     L.mc_analytic_limits.argtypes = [C.POINTER(Cell), C.c_int, P]
+    L.mc_composite_lobes.argtypes = [C.POINTER(LobeArgs), C.c_int, C.c_double, C.c_double, P, P, C.c_int, P]
     L.mc_engine_create.argtypes = [C.POINTER(P)]
     L.mc_engine_destroy.argtypes = [P]
     L.mc_engine_configure.argtypes = [P, C.POINTER(Cell), C.c_double, C.POINTER(EngineInfo)]
@@ -534,14 +535,7 @@ def directional_coeffs(Kr, phiSt, phiEx):
 
 def stability_lobes(N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin, rpmMax, modes_x, modes_y=None, cap=400000):
     """stabilityLobes (src/sld.js:23): list of (rpm array, alim array) per non-empty lobe"""
-    modes_y = modes_y or modes_x
-    a = LobeArgs()
-    a.N, a.lobes, a.Ktc, a.Krc, a.phiSt, a.phiEx = int(N), int(lobes), Ktc, Krc, phiSt, phiEx
-    a.rpmMin, a.rpmMax, a.nmx, a.nmy = rpmMin, rpmMax, len(modes_x), len(modes_y)
-    for i, m in enumerate(modes_x):
-        a.mx[i] = Mode(*map(float, m))
-    for i, m in enumerate(modes_y):
-        a.my[i] = Mode(*map(float, m))
+    a = lobe_args(N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin, rpmMax, modes_x, modes_y)
     rpm, alim = np.zeros(cap), np.zeros(cap)
     ll = np.zeros(max(1, lobes), dtype=np.int32)
     n = C.c_int(0)
@@ -563,6 +557,35 @@ def alim_at(lobes, S):
     out = C.c_double()
     _check(load().mc_alim_at(rpm.ctypes.data, alim.ctypes.data, ll.ctypes.data, len(ll), float(S), C.byref(out)))
     return out.value
+
+
+def lobe_args(N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin, rpmMax, modes_x, modes_y=None):
+    """one mc_lobe_args record (stabilityLobes' parameters, src/sld.js:23)"""
+    modes_y = modes_y or modes_x
+    if not (1 <= len(modes_x) <= MC_MAX_MODES and 1 <= len(modes_y) <= MC_MAX_MODES):
+        raise ValueError("1..%d modes per axis" % MC_MAX_MODES)
+    a = LobeArgs()
+    a.N, a.lobes, a.Ktc, a.Krc, a.phiSt, a.phiEx = int(N), int(lobes), Ktc, Krc, phiSt, phiEx
+    a.rpmMin, a.rpmMax, a.nmx, a.nmy = rpmMin, rpmMax, len(modes_x), len(modes_y)
+    for i, m in enumerate(modes_x):
+        a.mx[i] = Mode(*map(float, m))
+    for i, m in enumerate(modes_y):
+        a.my[i] = Mode(*map(float, m))
+    return a
+
+
+def composite_lobes(nodes, step=25.0, hi=44000.0):
+    """Simulator.lobes' composite curve (src/sim.js:270-281) from its speed nodes (LobeArgs
+    records, see lobe_args; each node's window [rpmMin, rpmMax) is read every step below hi):
+    (rpm, alim) arrays, node after node; 1e3 where no lobe covers a speed"""
+    n = len(nodes)
+    arr = (LobeArgs * n)(*nodes)
+    cnt = C.c_int(0)
+    _check(load().mc_composite_lobes(arr, n, float(step), float(hi), None, None, 0, C.byref(cnt)))
+    rpm, alim = np.zeros(cnt.value), np.zeros(cnt.value)
+    _check(load().mc_composite_lobes(arr, n, float(step), float(hi), rpm.ctypes.data, alim.ctypes.data, cnt.value,
+                                     C.byref(cnt)))
+    return rpm, alim
 
 
 def analytic_limits(cells):

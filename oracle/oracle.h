@@ -85,6 +85,13 @@ int or_stability_lobes(int N, double Ktc, double Krc, double phiSt, double phiEx
                        int *lobe_len);
 double or_alim_at(const double *rpm, const double *alim, const int *lobe_len, int nlobes,
                   double S);
+/* Simulator.lobes' composite curve (src/sim.js:270-281), one speed node per entry (modes of
+ * node i at mx + i * mstride, my + i * mstride); returns the points (counted only when rpm
+ * is NULL) */
+int or_composite_lobes(int n, const int *N, const double *Ktc, const double *Krc, const double *phiSt,
+                       const double *phiEx, const int *lobes, const double *S0, const double *S1,
+                       const mc_mode *mx, const int *nmx, const mc_mode *my, const int *nmy, int mstride,
+                       double step, double hi, double *rpm, double *alim);
 
 /* stock.c: Stock (src/stock.js) */
 typedef struct or_stock or_stock;

@@ -67,6 +67,8 @@ def lib():
                                          C.POINTER(Mode), C.c_int, P, P, C.c_int, P]
         L.or_alim_at.argtypes = [P, P, P, C.c_int, C.c_double]
         L.or_alim_at.restype = C.c_double
+        L.or_composite_lobes.argtypes = [C.c_int] + [P] * 12 + [C.c_int, C.c_double, C.c_double, P, P]
+        L.or_composite_lobes.restype = C.c_int
         L.or_stock_new.argtypes = [C.c_double] * 4
         L.or_stock_new.restype = P
         L.or_stock_free.argtypes = [P]
@@ -282,6 +284,30 @@ def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None
     if st:
         raise RuntimeError("or_sim_run: %d" % st)
     return out[:n]
+
+
+def composite_lobes(nodes, step, hi):
+    """Simulator.lobes' composite curve (src/sim.js:270-281) over speed nodes, each a dict of
+    N, Ktc, Krc, phiSt, phiEx, lobes, rpmMin, rpmMax, mx, my (lists of (k, fn, zeta)):
+    (rpm, alim) arrays"""
+    import numpy as np
+    n = len(nodes)
+    I = lambda k: np.array([d[k] for d in nodes], dtype=np.int32)
+    F = lambda k: np.array([d[k] for d in nodes], dtype=np.float64)
+    mx = np.zeros((n, 4, 3))
+    my = np.zeros((n, 4, 3))
+    for i, d in enumerate(nodes):
+        mx[i, :len(d["mx"])] = d["mx"]
+        my[i, :len(d["my"])] = d["my"]
+    nmx = np.array([len(d["mx"]) for d in nodes], dtype=np.int32)
+    nmy = np.array([len(d["my"]) for d in nodes], dtype=np.int32)
+    args = [I("N"), F("Ktc"), F("Krc"), F("phiSt"), F("phiEx"), I("lobes"), F("rpmMin"), F("rpmMax"),
+            mx, nmx, my, nmy]
+    ptrs = [a.ctypes.data for a in args]
+    tot = lib().or_composite_lobes(n, *ptrs, 4, float(step), float(hi), None, None)
+    rpm, alim = np.zeros(tot), np.zeros(tot)
+    lib().or_composite_lobes(n, *ptrs, 4, float(step), float(hi), rpm.ctypes.data, alim.ctypes.data)
+    return rpm, alim
 
 
 class OracleStock:

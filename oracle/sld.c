@@ -152,3 +152,39 @@ double or_alim_at(const double *rpm, const double *alim, const int *lobe_len, in
   }
   return best;
 }
+
+/* The composite curve of Simulator.lobes for a speed-dependent tool tip (src/sim.js:270-281):
+ * node i's diagram (stabilityLobes over [S0, S0 + node] with that node's modes) read by alimAt
+ * at S = S0, S0 + step, ... while S < min(hi, S0 + node); a speed no lobe covers reads 1e3.
+ * Returns the points written (all of them, however large, when rpm is NULL). */
+int or_composite_lobes(int n, const int *N, const double *Ktc, const double *Krc, const double *phiSt,
+                       const double *phiEx, const int *lobes, const double *S0, const double *S1,
+                       const mc_mode *mx, const int *nmx, const mc_mode *my, const int *nmy, int mstride,
+                       double step, double hi, double *rpm, double *alim) {
+  const int cap = 1 << 20;
+  double *lr = malloc(sizeof(double) * cap), *la = malloc(sizeof(double) * cap);
+  int *ll = malloc(sizeof(int) * 100000);
+  int i, total = 0;
+  for (i = 0; i < n; i++) {
+    const double top = hi < S1[i] ? hi : S1[i];
+    double S;
+    int nl = 0, j;
+    if (rpm) {
+      or_stability_lobes(N[i], Ktc[i], Krc[i], phiSt[i], phiEx[i], lobes[i], S0[i], S1[i], mx + i * mstride, nmx[i],
+                         my + i * mstride, nmy[i], lr, la, cap, ll);
+      for (j = 0; j < lobes[i]; j++) if (ll[j]) nl = j + 1;
+    }
+    for (S = S0[i]; S < top; S += step) {
+      if (rpm) {
+        const double a = or_alim_at(lr, la, ll, nl, S);
+        rpm[total] = S;
+        alim[total] = a == INFINITY ? 1e3 : a;
+      }
+      total++;
+    }
+  }
+  free(lr);
+  free(la);
+  free(ll);
+  return total;
+}

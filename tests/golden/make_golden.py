@@ -24,6 +24,8 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
                         path with feeds and speeds, the engine's configure / wear arguments,
                         every stamp, final maps
   golden_paths.json     the G-code parser's moves and the path discretizer's polylines
+  golden_clobes.json    Simulator.lobes: the composite (speed-dependent) lobe curve and the
+                        diagram at rest, with the speeds and modes its nodes were built from
   golden_englog.json    the engine's whole call sequence inside simulator runs (configure,
                         setWear, setWork, setEngagement, setEndEngagement, run, takeStats,
                         chatterIndex) with its state, stats, rings and chip events after
@@ -267,6 +269,19 @@ def build_cases(quick):
         for soft in softs:
             sp.append(dict(modal=modal, chain=ch, soft=soft, speeds=speeds))
     cases["spindle"] = sp
+    # Simulator.lobes: the composite curve of a speed-dependent tool tip (one diagram per
+    # 1000 rpm node) and the diagram at rest, over several engagement arcs and tip models
+    prog = "(STOCK 50 50 10)\nG0 X0 Y0 Z5\nM30"
+    arcs = [[0.0, JS_PI, 0], [0.0, JS_PI, 1], [JS_PI / 2, JS_PI, 0], [0.0, JS_PI / 3, 0]]
+    cases["clobes"] = [
+        dict(program=prog, arcs=arcs),
+        dict(program=prog, tool=dict(stickout=15, fluteLen=12), arcs=[arcs[0], arcs[2]]),
+        dict(program=prog, tool=dict(N=2, D=8, stickout=40), material="S50C", arcs=arcs[2:]),
+        dict(program=prog, modal=dict(spindleMode=dict(on=True, fn=1200, zeta=0.05)), arcs=[arcs[0], arcs[2]]),
+        dict(program=prog, modal=dict(auto=False, k=2.0e4, fn=1800.0, zeta=0.03, speedFrfs=frfs["speedFrfs"]),
+             arcs=[arcs[0], arcs[3]]),
+        dict(program=prog, soft=dict(drop=0.0), arcs=arcs[:1]),
+    ]
     # the engine as the simulator drives it: any feed direction, an engagement map from the
     # stock, the tool's own flute length and runout, fnMax, a velocity set from outside
     rng = W.ParkMiller(400)
@@ -330,7 +345,7 @@ def main():
                 m["k"], m["fn"] = tm["k"], tm["fn"]
             md["k"], md["fn"] = tm["k"], tm["fn"]
     keys = ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
-            "enginex", "englog", "simx", "paths")
+            "enginex", "englog", "simx", "paths", "clobes")
     if "--only" in sys.argv:
         keys = tuple(sys.argv[sys.argv.index("--only") + 1].split(","))
         # engage cases read the stock cases

@@ -380,6 +380,37 @@ if (cases.spindle) {
   });
 }
 
+// Simulator.lobes (src/sim.js:262-286): the composite curve of a speed-dependent tool tip and
+// the diagram at rest, per engagement arc; the modalNow calls each lobes() call makes are
+// recorded (speed and the x / y modes) so the nodes' inputs are known exactly
+if (cases.clobes) {
+  const { Simulator } = await import(REF + 'sim.js');
+  const { modeSet } = await import(REF + 'frf.js');
+  const { speedDependent } = await import(REF + 'spindle.js');
+  const md = L => L.map(m => [m.k, m.fn, m.zeta]);
+  const lob = L => L.map(l => ({ rpm: z32(Float64Array, l.map(p => p.rpm)), alim: z32(Float64Array, l.map(p => p.alim)) }));
+  out.clobes = cases.clobes.map(({ program, tool, modal, soft, material, arcs }) => {
+    const sim = new Simulator();
+    if (material) sim.cfg.material = material;
+    if (tool) Object.assign(sim.cfg.tool, tool);
+    if (modal) Object.assign(sim.cfg.modal, modal);
+    if (soft !== undefined) sim.cfg.spindleSoft = soft;
+    sim.load(program);
+    const orig = sim.modalNow.bind(sim);
+    let calls = [];
+    sim.modalNow = S => { const m = orig(S); const ms = modeSet(m); calls.push([S, md(ms.x), md(ms.y)]); return m; };
+    const runs = arcs.map(([phiSt, phiEx, staticOnly]) => {
+      calls = [];
+      const L = sim.lobes(phiSt, phiEx, !!staticOnly);
+      return { lobes: lob(L), calls };
+    });
+    return { N: sim.activeTool.N, Ktc: sim.mat.Ktc, Krc: sim.mat.Krc, chain: sim.chain(), modal: sim.cfg.modal,
+      soft: sim.cfg.spindleSoft === undefined ? null : sim.cfg.spindleSoft,
+      dependent: speedDependent(sim.cfg.modal, sim.cfg.spindleSoft), runs };
+  });
+  console.error('clobes', (Date.now() - t0) / 1e3, 's');
+}
+
 fs.writeFileSync(outPath, JSON.stringify(out));
 console.error('done', (Date.now() - t0) / 1e3, 's');
 }

@@ -33,6 +33,26 @@ async function main() {
       const d = m.directionalCoeffs(c.params.Krc / c.params.Ktc, c.params.phiSt, c.params.phiEx);
       check(sameBits([d.xx, d.xy, d.yx, d.yy], ref.dc), 'dc ' + k);
     });
+    // Simulator.lobes with this build's modules: modalNow from modesAt on the recorded modal,
+    // chain and softening; the composite curve from compositeLobes, the diagram at rest (and
+    // a speed-independent tip) from stabilityLobes of modalNow(0)
+    const cl = load('clobes');
+    const z64 = s => { const b = zlib.inflateSync(Buffer.from(s, 'base64')); return new Float64Array(b.buffer, b.byteOffset, b.length / 8); };
+    cl.cases.forEach((c, k) => {
+      const res = cl.results[k], soft = res.soft === null ? undefined : res.soft, md = res.modal;
+      const modalNow = S => { const ms = m.modesAt(md, S, res.chain, soft); return { k: md.k, fn: md.fn, zeta: md.zeta, modesX: ms.x, modesY: ms.y }; };
+      c.arcs.forEach(([phiSt, phiEx, staticOnly], a) => {
+        const base = { N: res.N, Ktc: res.Ktc, Krc: res.Krc, phiSt, phiEx, lobes: 30, rpmMin: 2000, rpmMax: 40000 };
+        const dep = !staticOnly && m.speedDependent(md, soft);
+        const got = dep ? m.compositeLobes(base, modalNow) : m.stabilityLobes(Object.assign({}, modalNow(0), base));
+        const want = res.runs[a].lobes;
+        check(got.length === want.length, 'composite lobe count ' + k + '/' + a);
+        got.forEach((L, j) => {
+          check(sameBits(Float64Array.from(L.map(p => p.rpm)), z64(want[j].rpm)), 'composite rpm ' + k + '/' + a + '/' + j);
+          check(sameBits(Float64Array.from(L.map(p => p.alim)), z64(want[j].alim)), 'composite alim ' + k + '/' + a + '/' + j);
+        });
+      });
+    });
     const tl = load('tiplut');
     tl.cases.forEach((t, k) => check(sameBits(m.tipLut(t), f32(tl.results[k])), 'tipLut ' + k));
     check(m.LUT_DR === 0.02, 'LUT_DR');

@@ -71,3 +71,98 @@ def test_analytic_limits_of_cells():
                                     [(c.my[j].k, c.my[j].fn, c.my[j].zeta) for j in range(c.nmy)])
         for q, x in enumerate(group):
             assert got[t * 16 + q] == lib.alim_at(lobes, x.S)
+
+
+# ---- Simulator.lobes (src/sim.js:262-286): the composite curve and the diagram at rest ----
+
+BASE = dict(lobes=30, rpmMin=2000, rpmMax=40000)
+
+
+def _node(res, S0, mx, my, phiSt, phiEx):
+    """the speed node the reference's loop builds at S0 (its modes: modalNow(S0 + 500))"""
+    return dict(N=res["N"], Ktc=res["Ktc"], Krc=res["Krc"], phiSt=phiSt, phiEx=phiEx, lobes=BASE["lobes"],
+                rpmMin=S0, rpmMax=S0 + 1000, mx=mx, my=my)
+
+
+def _clobes_runs():
+    """(case result, run, arc, composite?) for every lobes() call the fixtures recorded"""
+    g = G.load("clobes")
+    for case, res in zip(g["cases"], g["results"]):
+        for arc, run in zip(case["arcs"], res["runs"]):
+            yield res, run, arc, bool(res["dependent"] and not arc[2])
+
+
+def _nodes(res, run, arc):
+    # modalNow(S0 + LOBE_NODE / 2) per node: S0 = S - 500 exactly (integers)
+    return [_node(res, S - 500, mx, my, arc[0], arc[1]) for S, mx, my in run["calls"]]
+
+
+def test_composite_lobes_match_reference():
+    """mc_composite_lobes on the recorded nodes gives Simulator.lobes' composite curve bit for
+    bit; the diagram at rest (and a tip whose dynamics do not depend on speed) is
+    stabilityLobes of modalNow(0) over 2000-40000 rpm"""
+    from mcsim import lib
+    seen = [0, 0]
+    for res, run, arc, comp in _clobes_runs():
+        ref = run["lobes"]
+        if comp:
+            assert len(run["calls"]) == 43 and len(ref) == 1
+            nodes = [lib.lobe_args(d["N"], d["Ktc"], d["Krc"], d["phiSt"], d["phiEx"], d["lobes"], d["rpmMin"],
+                                   d["rpmMax"], d["mx"], d["my"]) for d in _nodes(res, run, arc)]
+            rpm, alim = lib.composite_lobes(nodes, 25.0, BASE["rpmMax"] * 1.1)
+            assert np.array_equal(rpm, G.zf64(ref[0]["rpm"])) and np.array_equal(alim, G.zf64(ref[0]["alim"]))
+            seen[0] += 1
+        elif run["calls"]:
+            S, mx, my = run["calls"][0]
+            assert S == 0
+            lob = lib.stability_lobes(res["N"], res["Ktc"], res["Krc"], arc[0], arc[1], BASE["lobes"], BASE["rpmMin"],
+                                      BASE["rpmMax"], mx, my)
+            assert len(lob) == len(ref)
+            for (r, a), want in zip(lob, ref):
+                assert np.array_equal(r, G.zf64(want["rpm"])) and np.array_equal(a, G.zf64(want["alim"]))
+            seen[1] += 1
+    assert seen == [11, 2]
+
+
+def test_composite_lobes_oracle_pinned():
+    """the oracle's composite restatement against the same recorded curves; then planted
+    mistakes (the next node's modes, a window read to its end inclusive, the uncovered
+    speeds left at +inf) each break it"""
+    import oracle
+    for res, run, arc, comp in _clobes_runs():
+        if not comp:
+            continue
+        ref = run["lobes"][0]
+        want_r, want_a = G.zf64(ref["rpm"]), G.zf64(ref["alim"])
+        nodes = _nodes(res, run, arc)
+        rpm, alim = oracle.composite_lobes(nodes, 25.0, BASE["rpmMax"] * 1.1)
+        assert np.array_equal(rpm, want_r) and np.array_equal(alim, want_a)
+        shifted = [dict(d, mx=e["mx"], my=e["my"]) for d, e in zip(nodes, nodes[1:] + nodes[-1:])]
+        assert not np.array_equal(oracle.composite_lobes(shifted, 25.0, 44000.0)[1], want_a)
+        wide = [dict(d, rpmMax=d["rpmMax"] + 1e-9 * d["rpmMax"]) for d in nodes]
+        assert len(oracle.composite_lobes(wide, 25.0, 44000.0)[0]) != len(want_r)
+        if np.any(want_a == 1e3):
+            assert np.all(alim[want_a == 1e3] == 1e3) and not np.any(np.isinf(alim))
+
+
+def test_composite_lobes_batched_and_checked():
+    """the nodes of several curves in one call give each curve as its own call does; the size
+    query, a too-small output and a bad step are reported"""
+    import ctypes as C
+    from mcsim import lib
+    from mcsim._abi import LobeArgs
+    curves = []
+    for res, run, arc, comp in _clobes_runs():
+        if comp:
+            curves.append([lib.lobe_args(d["N"], d["Ktc"], d["Krc"], d["phiSt"], d["phiEx"], d["lobes"], d["rpmMin"],
+                                         d["rpmMax"], d["mx"], d["my"]) for d in _nodes(res, run, arc)])
+    one = [lib.composite_lobes(c, 25.0, 44000.0) for c in curves]
+    rpm, alim = lib.composite_lobes([a for c in curves for a in c], 25.0, 44000.0)
+    assert np.array_equal(rpm, np.concatenate([r for r, _ in one]))
+    assert np.array_equal(alim, np.concatenate([a for _, a in one]))
+    arr = (LobeArgs * 2)(*curves[0][:2])
+    n = C.c_int(0)
+    assert lib.load().mc_composite_lobes(arr, 2, 25.0, 44000.0, None, None, 0, C.byref(n)) == 0 and n.value == 80
+    out = np.zeros(80)
+    assert lib.load().mc_composite_lobes(arr, 2, 25.0, 44000.0, out.ctypes.data, out.ctypes.data, 79, C.byref(n)) != 0
+    assert lib.load().mc_composite_lobes(arr, 2, 0.0, 44000.0, None, None, 0, C.byref(n)) != 0
