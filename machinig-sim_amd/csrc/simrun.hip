@@ -103,18 +103,19 @@ __device__ __forceinline__ Box stamp_box(const StockDev& S, double cx, double cy
   return b;
 }
 
-// Stock.stamp's node loop (src/stock.js:129-155), flat, ball or a tip table (lut), by
+// Stock.stamp's node loop (src/stock.js:129-155), flat, ball or a tip table (LUT), by
 // threads t = 0..nt-1 of the workgroup: the nodes are independent, so they are taken in any
 // order, U at a time with their loads issued before any store.  Returns the thread's part
 // of the removed height sum; a node changed sets *touched.
-__device__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ, double R, bool ball,
-                             const float* lut, int lutn, double errVal, double mux, double muy, double mfeed,
-                             int32_t line, int t, int nt, int* touched) {
-  if (!bx.ok) return 0;
+template <bool LUT>
+__device__ __forceinline__ double stamp_nodes(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
+                                              double R, bool ball, const float* lut, int lutn, double errVal,
+                                              double mux, double muy, double mfeed, int32_t line, int t, int nt,
+                                              int* touched) {
   const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band), Rw2 = Rw * Rw;
   const double R2 = R * R;
-  const double mf = mfeed > 0 && !ball && !lut ? mfeed : 0;
-  const double zRim = tipZ + (lut ? (double)lut[lutn - 1] : ball ? R : 0);
+  const double mf = mfeed > 0 && !ball && !LUT ? mfeed : 0;
+  const double zRim = tipZ + (LUT ? (double)lut[lutn - 1] : ball ? R : 0);
   const int W = bx.i1 - bx.i0 + 1, total = W * (bx.j1 - bx.j0 + 1);
   constexpr int U = 8;
   double vol = 0;
@@ -145,7 +146,7 @@ __device__ double stamp_part(const StockDev& S, const Box& bx, double cx, double
         }
         continue;
       }
-      const double z = lut ? tipZ + (double)lut[(int)jmin(lutn - 1, mcm::round_js(__builtin_sqrt(r2) / MC_LUT_DR))]
+      const double z = LUT ? tipZ + (double)lut[(int)jmin(lutn - 1, mcm::round_js(__builtin_sqrt(r2) / MC_LUT_DR))]
                        : ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
       double h = hv[u];
       if (h > z) {
@@ -168,6 +169,17 @@ __device__ double stamp_part(const StockDev& S, const Box& bx, double cx, double
   }
   if (tch) *touched = 1;
   return vol;
+}
+
+// The tip-table form runs in a code path of its own: mixed into the flat / ball loop it
+// raised the kernel's register pressure into scratch spills.
+__device__ __forceinline__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
+                                             double R, bool ball, const float* lut, int lutn, double errVal,
+                                             double mux, double muy, double mfeed, int32_t line, int t, int nt,
+                                             int* touched) {
+  if (!bx.ok) return 0;
+  if (lut) return stamp_nodes<true>(S, bx, cx, cy, tipZ, R, ball, lut, lutn, errVal, mux, muy, mfeed, line, t, nt, touched);
+  return stamp_nodes<false>(S, bx, cx, cy, tipZ, R, ball, nullptr, 0, errVal, mux, muy, mfeed, line, t, nt, touched);
 }
 
 // the dirty rectangle joins a changed stamp's box (src/stock.js:156-160)
