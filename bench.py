@@ -17,6 +17,7 @@ Prints ONE JSON line.  usage: python bench.py [--gpus 1] [--steps K] [--warmup W
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -123,21 +124,52 @@ def heightmap(steps):
                      "tiles stay in LDS across the stamps of a chunk")
 
 
-def coupled(n_prog, n_lines, cpu_threads, cpu):
+def line_cells(progs, outs, revs=40):
+    """The machining report's per-line stability question (src/report.js:27-28 keeps each
+    line's heaviest cut with an engagement arc; src/main.js:1045 asks for the limit at the
+    line's speed): one cell per cutting line of every program, from the pipeline's axial
+    depths and the engagement arc the simulator shows after each sub-step."""
+    from mcsim._abi import make_cell
+    cells, keys = [], []
+    for p, ((sub, n), out) in enumerate(zip(progs, outs)):
+        best = {}
+        st, ex = 0.0, math.pi
+        for k in range(n):
+            o = out[k]
+            if o.arc:
+                st, ex = o.phiSt, o.phiEx
+            if not o.cut:
+                continue
+            r = best.get(sub[k].line)
+            if ex > st and (r is None or o.ap >= r[0]):
+                best[sub[k].line] = (o.ap, st, ex, sub[k].fzEff)
+        for ln in sorted(best):
+            ap, st_, ex_, fz = best[ln]
+            if not ap > 0.01:          # an end cut alone: no flute engagement to limit
+                continue
+            cells.append(make_cell(W.COUPLED_TOOL, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (st_, ex_),
+                                   fz, revs, True))
+            keys.append((p, ln, ap))
+    return cells, keys
+
+
+def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     """The simulator's coupled sub-step loop (engagement -> engine run -> commanded and
     deflected stamps, src/sim.js:351-449) for n_prog seeded raster programs, each on its own
     2048 x 2048 stock at 0.05 mm (102.35 mm square) with the flat 10 mm tool, one launch of
-    mc_sim_run (one wavefront per program).  Reported in sub-steps per second; the oracle's
+    mc_sim_run (one workgroup per program).  Reported in sub-steps per second; the oracle's
     restated loop runs the same programs on host threads as the CPU baseline and checks the
-    device's deflections, step counts and final heights bit for bit."""
+    device's deflections, step counts and final heights bit for bit.  Then every line's
+    stability check (line_cells) runs as one batched time-domain search, guided by the
+    surrogate when one was trained (checked identical to the plain search)."""
     from mcsim._abi import make_cell, MC_TOOL_FLAT
     width, cell = 102.35, 0.05
     hx = width / 2047
     progs = []
     for p in range(n_prog):
-        x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, n_lines)
+        x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, n_lines, speed=W.COUPLED_SPEED)
         progs.append(path.sim_substeps(x, y, z, r, ln, F, S, W.COUPLED_TOOL["N"], hx))
-    ecell = make_cell(W.COUPLED_TOOL, W.COUPLED_MODES, None, W.MAT_A5052, 10000.0, (0.0, 0.0), 0.0, 1, True)
+    ecell = make_cell(W.COUPLED_TOOL, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (0.0, 0.0), 0.0, 1, True)
     kte, kre = W.MAT_A5052["Kte"], W.MAT_A5052["Kre"]
 
     def setup():
@@ -185,6 +217,23 @@ def coupled(n_prog, n_lines, cpu_threads, cpu):
                note="algorithmic bytes = 8 B x bbox nodes of both stamps per sub-step; one workgroup per "
                     "program runs its sub-steps in order (each reads the stock the previous one cut), so "
                     "the kernel is bound by the latency of that chain, far below the HBM roof")
+    lc, keys = line_cells(progs, outs)
+    t0 = time.perf_counter()
+    plain = lib.bisect_batch(lc, **opts)
+    t_plain = time.perf_counter() - t0
+    checks = dict(lines=len(lc), plain_s=t_plain, plain_kernel_ms=lib.last_kernel_ms(),
+                  lines_over_limit=sum(1 for (p, ln, ap), r in zip(keys, plain) if ap > r.alim),
+                  alim_mm_min=min(r.alim for r in plain), alim_mm_max=max(r.alim for r in plain))
+    if surrogate is not None:
+        from mcsim import surrogate as S
+        model, sigma = surrogate
+        t0 = time.perf_counter()
+        guided = S.guided_search(model, lc, sigma, **opts)
+        checks.update(guided_s=time.perf_counter() - t0, guided_kernel_ms=lib.last_kernel_ms(),
+                      identical_to_plain=all((g.alim, g.lo, g.hi, g.open) == (q.alim, q.lo, q.hi, q.open)
+                                             for g, q in zip(guided, plain)))
+    res["line_checks"] = checks
+    res["pipeline_s"] = wall + checks.get("guided_s", t_plain)
     if cpu:
         import threading
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -262,7 +311,7 @@ def surrogate_leg(cells, opts, plain, steps, n_train, epochs):
     err = np.array([abs(br[i].alim - plain[i].alim) for i in range(len(cells))])
     pred = model.predict(lib.surrogate_features(cells))
     lnp = np.array([math.log(r.alim) for r in plain])
-    return res, dict(
+    return res, (model, prior_sigma), dict(
         guided_cells_per_s=len(cells) / float(np.mean(times)), guided_ms_per_step=1e3 * float(np.mean(times)),
         guided_kernel_ms=float(np.mean(kms)), identical_to_plain=identical, prior_sigma=prior_sigma,
         training_cells=n_train, heldout_sigma_ln_alim=sigma, grid_sigma_ln_alim=float(np.sqrt(np.mean((pred - lnp) ** 2))),
@@ -328,9 +377,9 @@ def main():
     n = len(cells)
     plain_value = n / (ms / 1e3)
     value, ms_step, kernel_ms, path = plain_value, ms, kms_mean, "plain"
-    surr = None
+    surr = model = None
     if not a.no_surrogate:
-        gres, surr, gk = surrogate_leg(cells, opts, res, a.steps, a.train_cells, a.epochs)
+        gres, model, surr, gk = surrogate_leg(cells, opts, res, a.steps, a.train_cells, a.epochs)
         if surr["identical_to_plain"] and surr["guided_cells_per_s"] > plain_value:
             value, ms_step, kernel_ms, path = surr["guided_cells_per_s"], surr["guided_ms_per_step"], gk, "surrogate-guided"
     # the roofline is that of the plain launch of cell_search (the one the rocprofv3
@@ -371,7 +420,7 @@ def main():
     if not a.no_heightmap:
         out["heightmap"] = heightmap(2)
     if not a.no_coupled:
-        out["coupled"] = coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu)
+        out["coupled"] = coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu, opts, model)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, opts, a.cpu_seconds)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]

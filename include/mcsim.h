@@ -419,8 +419,14 @@ typedef struct {
   double errVal;      /* vibration written by the actual stamp [um] */
   double vol;         /* volume the actual stamp removed [mm^3] */
   double maxF;        /* largest |(Fx, Fy)| of the sub-step's run [N] (0 without a run) */
+  double phiSt, phiEx; /* the engagement arc the sub-step shows (src/sim.js:421-423): immersion
+                          (:343-348) when it cuts deeper than 0.01 mm and a leading bin stands
+                          above tipZ + 0.1, [0, pi] for an end cut alone; 0, 0 otherwise */
   int32_t nsteps;     /* engine steps run (0: not cutting) */
   int32_t cut;        /* 1 when the sub-step cut */
+  int32_t arc;        /* 1 when the sub-step sets phiSt / phiEx (the simulator keeps the
+                         previous arc otherwise); the report's per-line engagement reads it */
+  int32_t pad;
 } mc_substep_out;
 
 typedef struct {

@@ -241,6 +241,23 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     }
     const bool endCut = haveEnd && dmax > 2e-4;
     const bool cut = (ap > 0.01 || endCut) && !s.rapid;
+    // the engagement arc the simulator shows (src/sim.js:421-423): immersion(tipZ) over the
+    // leading bins of the map (:343-348) when the flutes cut, [0, pi] for an end cut alone
+    double phiSt = 0, phiEx = 0;
+    int arc = 0;
+    if (cut && ap > 0.01) {
+      const bool up = tid < MC_BINS / 2 && (double)E.Hb[tid] > s.z + 0.1;
+      const double last = block_max(up ? (double)tid : -1.0, red);
+      const double first = -block_max(up ? -(double)tid : -1e9, red);
+      if (last >= 0) {
+        phiSt = jmax(0, (first + 0.5) / MC_BINS * 2 * PI - PI / MC_BINS);
+        phiEx = jmin(PI, (last + 0.5) / MC_BINS * 2 * PI + PI / MC_BINS);
+        arc = 1;
+      }
+    } else if (cut && endCut) {
+      phiEx = PI;
+      arc = 1;
+    }
     int n = 0;
     // the commanded stamp does not depend on the dynamics: waves 1.. stamp it while wave 0
     // runs the engine
@@ -292,6 +309,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     if (P.out && tid == 0) {
       mc_substep_out& o = P.out[k];
       o.dx = dx; o.dy = dy; o.ap = ap; o.errVal = errVal; o.vol = vol; o.maxF = maxF; o.nsteps = n; o.cut = cut ? 1 : 0;
+      o.phiSt = phiSt; o.phiEx = phiEx; o.arc = arc; o.pad = 0;
     }
   }
 }

@@ -121,8 +121,8 @@ class Substep(C.Structure):
 
 
 class SubstepOut(C.Structure):
-    _fields_ = [(n, C.c_double) for n in ("dx", "dy", "ap", "errVal", "vol", "maxF")] + \
-               [("nsteps", C.c_int32), ("cut", C.c_int32)]
+    _fields_ = [(n, C.c_double) for n in ("dx", "dy", "ap", "errVal", "vol", "maxF", "phiSt", "phiEx")] + \
+               [("nsteps", C.c_int32), ("cut", C.c_int32), ("arc", C.c_int32), ("pad", C.c_int32)]
 
 
 class SimProgram(C.Structure):

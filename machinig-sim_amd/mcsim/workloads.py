@@ -185,3 +185,4 @@ def coupled_program(seed, width, n_lines=4, feed=1000.0, speed=10000.0):
 # three-flute end mill on one mode per axis, A5052 with the size effect and process damping)
 COUPLED_TOOL = dict(N=3, R=5.0, helixDeg=30.0, type="flat", fluteLen=25.0, runout=0.005, runoutPhase=0.0)
 COUPLED_MODES = [(2.0e4, 1800.0, 0.03)]
+COUPLED_SPEED = 10000.0   # min^-1, every line of the coupled programs

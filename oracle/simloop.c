@@ -8,6 +8,7 @@
  *   axial depth, end cut, the cut decision                 src/sim.js:388-401
  *   setEngagement / setEndEngagement / setWork(null)       src/sim.js:403-408
  *   the run's step count (the frame budget's share)        src/sim.js:409-414
+ *   the engagement arc it shows (immersion, :343-348)      src/sim.js:421-423
  *   the decay out of the cut                               src/sim.js:425-429
  *   deflection, the commanded and the actual stamp         src/sim.js:433-440
  *
@@ -107,13 +108,18 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     const mc_substep *s = &sub[k];
     mc_engine_state st;
     double hmax = -INFINITY, ap, dmax = 0, rOut = 0, dx, dy, errVal, vol, maxF = 0;
-    int haveEnd = s->fza > 0, endCut, cut, nsteps = 0;
+    int haveEnd = s->fza > 0, endCut, cut, nsteps = 0, arc;
+    double phiSt, phiEx;
     if (!(s->dxy < 1e-6)) hmax = or_fill_engagement(act, s->x, s->y, s->ux, s->uy, R, Hb);
     ap = hmax - s->z > 0 ? hmax - s->z : 0;
     if (fluteLen < ap) ap = fluteLen;
     if (haveEnd) dmax = fill_end(act, s->x, s->y, s->z, s->ux, s->uy, &shape, Eb, &rOut);
     endCut = haveEnd && dmax > 2e-4;
     cut = (ap > 0.01 || endCut) && !s->rapid;
+    /* the engagement arc the simulator shows (src/sim.js:421-423) */
+    phiSt = 0; phiEx = 0; arc = 0;
+    if (cut && ap > 0.01) arc = or_immersion(Hb, s->z, &phiSt, &phiEx);
+    else if (cut && endCut) { phiEx = PI; arc = 1; }
     or_engine_get_state(e, &st);
     if (cut) {
       double g[4 * MC_END_RINGS], want;
@@ -163,6 +169,7 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     if (out) {
       out[k].dx = dx; out[k].dy = dy; out[k].ap = ap; out[k].errVal = errVal; out[k].vol = vol;
       out[k].maxF = maxF; out[k].nsteps = nsteps; out[k].cut = cut;
+      out[k].phiSt = arc ? phiSt : 0; out[k].phiEx = arc ? phiEx : 0; out[k].arc = arc; out[k].pad = 0;
     }
   }
   return MC_OK;

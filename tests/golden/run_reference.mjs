@@ -217,6 +217,10 @@ if (cases.simx) {
       };
     };
     wrap(sim.stock, rec.actual); wrap(sim.ideal, rec.ideal);
+    // the engagement arc the simulator shows (src/sim.js:421-423) at each actual stamp
+    const darc = [];
+    const fa = sim.stock.stamp;
+    sim.stock.stamp = (...a) => { darc.push(sim.display.phiSt, sim.display.phiEx); return fa(...a); };
     const e = sim.engine, cfgs = [], wears = [], runs = [];
     const fc = e.configure.bind(e), fw = e.setWear.bind(e), fr = e.run.bind(e);
     e.configure = (t, modal, mat, S, fnMax = 0) => {
@@ -243,7 +247,7 @@ if (cases.simx) {
       line: col(rows, 7), vol: b64f64(col(rows, 8)) });
     const P = sim.path, t = sim.activeTool;
     const clr = t.clearanceDeg != null ? t.clearanceDeg : (sim.cfg.tool.clearanceDeg != null ? sim.cfg.tool.clearanceDeg : 10);
-    return { frames, nx: sim.stock.nx, ny: sim.stock.ny, hx: sim.stock.hx, hy: sim.stock.hy, zBottom: sim.stock.zBottom,
+    return { frames, darc: b64f64(darc), nx: sim.stock.nx, ny: sim.stock.ny, hx: sim.stock.hx, hy: sim.stock.hy, zBottom: sim.stock.zBottom,
       toolR: t.R, toolN: t.N, fluteLen: t.fluteLen, type: t.type, VB: sim.VB || 0, clearanceDeg: clr,
       path: { x: b64f64(P.map(p => p.x)), y: b64f64(P.map(p => p.y)), z: b64f64(P.map(p => p.z)),
         rapid: P.map(p => (p.rapid ? 1 : 0)), line: P.map(p => p.line), F: P.map(p => p.F), S: P.map(p => p.S) },

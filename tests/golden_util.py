@@ -346,3 +346,15 @@ def simx_tool(case, res, tip_lut):
         rc = case["tool"]["cornerR"]
         return MC_TOOL_BULL, tip_lut(MC_TIP_BULL, 2 * res["toolR"], rc), rc
     return (MC_TOOL_BALL if t == "ball" else MC_TOOL_FLAT), None, 0.0
+
+
+def display_arcs(out):
+    """the simulator's display arc after each sub-step (src/sim.js:421-423), from the
+    pipeline's per-sub-step arc: [0, pi] until a sub-step sets one (the reset display,
+    src/sim.js:734), then the last one set; flattened (phiSt, phiEx) pairs"""
+    st, ex, arcs = 0.0, 3.141592653589793, []
+    for o in out:
+        if o.arc:
+            st, ex = o.phiSt, o.phiEx
+        arcs += [st, ex]
+    return np.array(arcs)

@@ -42,6 +42,8 @@ int main(void) {
          sizeof(mc_engine_stats), sizeof(mc_chip_event));
   printf("%zu %zu %zu\n", offsetof(mc_engine_state, stepCount), offsetof(mc_engine_state, hist_i),
          offsetof(mc_engine_info, stepsPerTooth));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(mc_substep), sizeof(mc_substep_out), offsetof(mc_substep_out, arc),
+         sizeof(mc_sim_program), offsetof(mc_sim_program, cornerR));
   return 0;
 }'''
     import tempfile
@@ -59,4 +61,6 @@ int main(void) {
     want += [C.sizeof(_abi.EngageSpec), _abi.EngageSpec.Hb.offset]
     want += [C.sizeof(t) for t in (_abi.EngineInfo, _abi.Wall, _abi.EngineState, _abi.EngineStats, _abi.ChipEvent)]
     want += [_abi.EngineState.stepCount.offset, _abi.EngineState.hist_i.offset, _abi.EngineInfo.stepsPerTooth.offset]
+    want += [C.sizeof(_abi.Substep), C.sizeof(_abi.SubstepOut), _abi.SubstepOut.arc.offset, C.sizeof(_abi.SimProgram),
+             _abi.SimProgram.cornerR.offset]
     assert [int(x) for x in out] == want

@@ -50,6 +50,7 @@ def test_sim_pipeline_matches_reference_runs():
         assert np.array_equal([o.errVal for o in out], G.f64(a["errVal"])), case["tool"]
         runs = dict((k, m) for k, m in res["runs"])
         assert [o.nsteps for o in out] == [runs.get(k, 0) for k in range(n)], case["tool"]
+        assert np.array_equal(G.display_arcs(out), G.f64(res["darc"])), case["tool"]
         assert sum(o.cut for o in out) == len(runs)
         for name, stock, ref in (("actual", pg["actual"], res["stock"]), ("commanded", pg["ideal"], res["commanded"])):
             got = _maps(stock)

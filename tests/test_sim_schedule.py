@@ -61,6 +61,8 @@ def test_oracle_sim_loop_matches_reference_runs():
         assert np.array_equal([o.vol for o in out], G.f64(a["vol"])), case["tool"]
         runs = dict((k, m) for k, m in res["runs"])
         assert [o.nsteps for o in out] == [runs.get(k, 0) for k in range(n)]
+        assert np.array_equal(G.display_arcs(out), G.f64(res["darc"])), case["tool"]
+        assert any(o.arc for o in out)
         for name, s_, ref in (("actual", act, res["stock"]), ("commanded", ide, res["commanded"])):
             for key, fn in (("H", s_.H), ("err", s_.err), ("wallD", s_.wallD), ("markS", s_.markS), ("markF", s_.markF)):
                 assert np.array_equal(fn(), G.zf32(ref[key]), equal_nan=True), (case["tool"], name, key)
