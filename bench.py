@@ -124,14 +124,14 @@ def heightmap(steps):
                      "tiles stay in LDS across the stamps of a chunk")
 
 
-def line_cells(progs, outs, revs=40):
+def line_cells(progs, outs, tools, revs=40):
     """The machining report's per-line stability question (src/report.js:27-28 keeps each
     line's heaviest cut with an engagement arc; src/main.js:1045 asks for the limit at the
     line's speed): one cell per cutting line of every program, from the pipeline's axial
     depths and the engagement arc the simulator shows after each sub-step."""
     from mcsim._abi import make_cell
     cells, keys = [], []
-    for p, ((sub, n), out) in enumerate(zip(progs, outs)):
+    for p, ((sub, n), out, tool) in enumerate(zip(progs, outs, tools)):
         best = {}
         st, ex = 0.0, math.pi
         for k in range(n):
@@ -147,8 +147,8 @@ def line_cells(progs, outs, revs=40):
             ap, st_, ex_, fz = best[ln]
             if not ap > 0.01:          # an end cut alone: no flute engagement to limit
                 continue
-            cells.append(make_cell(W.COUPLED_TOOL, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (st_, ex_),
-                                   fz, revs, True))
+            cells.append(make_cell(tool, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (st_, ex_), fz, revs,
+                                   True))
             keys.append((p, ln, ap))
     return cells, keys
 
@@ -156,40 +156,45 @@ def line_cells(progs, outs, revs=40):
 def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     """The simulator's coupled sub-step loop (engagement -> engine run -> commanded and
     deflected stamps, src/sim.js:351-449) for n_prog seeded raster programs, each on its own
-    2048 x 2048 stock at 0.05 mm (102.35 mm square) with the flat 10 mm tool, one launch of
+    2048 x 2048 stock at 0.05 mm (102.35 mm square), flat 12 mm and ball 8 mm tools taking
+    turns, two modes per axis, 3 to 6 passes (workloads.coupled_plan), one launch of
     mc_sim_run (one workgroup per program).  Reported in sub-steps per second; the oracle's
     restated loop runs the same programs on host threads as the CPU baseline and checks the
     device's deflections, step counts and final heights bit for bit.  Then every line's
     stability check (line_cells) runs as one batched time-domain search, guided by the
     surrogate when one was trained (checked identical to the plain search)."""
-    from mcsim._abi import make_cell, MC_TOOL_FLAT
+    from mcsim._abi import make_cell, MC_TOOL_FLAT, MC_TOOL_BALL
     width, cell = 102.35, 0.05
     hx = width / 2047
-    progs = []
+    progs, tools, passes = [], [], []
     for p in range(n_prog):
-        x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, n_lines, speed=W.COUPLED_SPEED)
-        progs.append(path.sim_substeps(x, y, z, r, ln, F, S, W.COUPLED_TOOL["N"], hx))
-    ecell = make_cell(W.COUPLED_TOOL, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (0.0, 0.0), 0.0, 1, True)
+        tool, m = W.coupled_plan(p, n_lines)
+        x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, m, speed=W.COUPLED_SPEED)
+        progs.append(path.sim_substeps(x, y, z, r, ln, F, S, tool["N"], hx))
+        tools.append(tool)
+        passes.append(m)
+    ecells = [make_cell(t, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (0.0, 0.0), 0.0, 1, True) for t in tools]
+    ttype = [MC_TOOL_BALL if t["type"] == "ball" else MC_TOOL_FLAT for t in tools]
     kte, kre = W.MAT_A5052["Kte"], W.MAT_A5052["Kre"]
 
-    def setup():
+    def setup(limit=None):
         P = []
-        for sub, n in progs:
+        for (sub, n), t, ec, ty in zip(progs, tools, ecells, ttype):
             act, ide = lib.Stock(width, width, 20, cell), lib.Stock(width, width, 20, cell)
             ide.set_wall_band(2.0)
             eng = lib.Engine()
-            info = eng.configure(ecell)
+            info = eng.configure(ec)
             eng.set_wear(kte, kre)
-            P.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, R=5.0, fluteLen=25.0, rw=0.0,
-                          share=info.stepsPerTooth / 4, type=MC_TOOL_FLAT))
+            P.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=n if limit is None else min(n, limit),
+                          R=t["R"], fluteLen=t["fluteLen"], rw=0.0, share=info.stepsPerTooth / 4, type=ty))
         return P
 
     def release(P):
         for g in P:
             g["actual"].close(); g["ideal"].close(); g["engine"].close()
 
-    W0 = setup()
-    lib.sim_run(W0)          # warm-up launch (code objects, allocations)
+    W0 = setup(400)
+    lib.sim_run(W0)          # warm-up launch (code objects, allocations) on the programs' first sub-steps
     release(W0)
     P = setup()
     t0 = time.perf_counter()
@@ -202,13 +207,14 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     # algorithmic bytes: both stamps of every sub-step read and write the f32 height of each
     # node of their bounding boxes (the commanded stock's wall band is 2 mm)
     nodes = 0
-    for sub, n in progs:
+    for (sub, n), t in zip(progs, tools):
         cxs = np.array([sub[k].x for k in range(n)]); cys = np.array([sub[k].y for k in range(n)])
-        for Rw in (5.0 + 2 * hx, 5.0 + 2.0):
+        for Rw in (t["R"] + 2 * hx, t["R"] + 2.0):
             i0 = np.maximum(0, np.floor((cxs - Rw) / hx)); i1 = np.minimum(2047, np.ceil((cxs + Rw) / hx))
             j0 = np.maximum(0, np.floor((cys - Rw) / hx)); j1 = np.minimum(2047, np.ceil((cys + Rw) / hx))
             nodes += np.where((i1 >= i0) & (j1 >= j0), (i1 - i0 + 1) * (j1 - j0 + 1), 0).sum()
-    res = dict(workload="coupled_%dprog_2048x2048_%dlines" % (n_prog, n_lines), programs=n_prog,
+    res = dict(workload="coupled_%dprog_2048x2048_%dto%dpasses_flat12_ball8" % (n_prog, min(passes), max(passes)),
+               programs=n_prog, passes=passes,
                substeps=nsub, cutting_substeps=cuts, engine_steps=steps, wall_s=wall, kernel_ms=kms,
                substeps_per_s=nsub / wall, engine_steps_per_s=steps / wall,
                roofline={"bound": "hbm", "achieved": 8.0 * nodes / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
@@ -217,7 +223,7 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
                note="algorithmic bytes = 8 B x bbox nodes of both stamps per sub-step; one workgroup per "
                     "program runs its sub-steps in order (each reads the stock the previous one cut), so "
                     "the kernel is bound by the latency of that chain, far below the HBM roof")
-    lc, keys = line_cells(progs, outs)
+    lc, keys = line_cells(progs, outs, tools)
     t0 = time.perf_counter()
     plain = lib.bisect_batch(lc, **opts)
     t_plain = time.perf_counter() - t0
@@ -247,9 +253,10 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
             act, ide = oracle.OracleStock(width, width, 20, cell), oracle.OracleStock(width, width, 20, cell)
             ide.set_wall_band(2.0)
             eng = oracle.OracleEngine()
-            info = eng.configure(ecell)
+            info = eng.configure(ecells[i])
             eng.set_wear(kte, kre)
-            outs_c[i] = oracle.sim_run(act, ide, eng, sub, n, 5.0, 25.0, 0.0, info.stepsPerTooth / 4, MC_TOOL_FLAT)
+            outs_c[i] = oracle.sim_run(act, ide, eng, sub, n, tools[i]["R"], tools[i]["fluteLen"], 0.0,
+                                       info.stepsPerTooth / 4, ttype[i])
             H_c[i] = act.H()
         th = [threading.Thread(target=work, args=(i,)) for i in range(k)]
         t0 = time.perf_counter()
@@ -258,7 +265,8 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
         for t in th:
             t.join()
         cw = time.perf_counter() - t0
-        same = all(all(a.dx == b.dx and a.dy == b.dy and a.nsteps == b.nsteps for a, b in zip(outs[i], outs_c[i]))
+        same = all(all(a.dx == b.dx and a.dy == b.dy and a.nsteps == b.nsteps and a.arc == b.arc and
+                       a.phiSt == b.phiSt and a.phiEx == b.phiEx for a, b in zip(outs[i], outs_c[i]))
                    and np.array_equal(P[i]["actual"].H(), H_c[i]) for i in range(k))
         csub = sum(progs[i][1] for i in range(k))
         res["cpu_baseline"] = dict(value=csub / cw, unit="sub-steps/s", cores=k, kind="port",

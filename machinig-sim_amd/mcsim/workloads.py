@@ -181,8 +181,15 @@ def coupled_program(seed, width, n_lines=4, feed=1000.0, speed=10000.0):
     return xs, ys, zs, rapid, line, F, S
 
 
-# the coupled programs' tool, modes and material (the simulator's defaults: a 10 mm
-# three-flute end mill on one mode per axis, A5052 with the size effect and process damping)
-COUPLED_TOOL = dict(N=3, R=5.0, helixDeg=30.0, type="flat", fluteLen=25.0, runout=0.005, runoutPhase=0.0)
-COUPLED_MODES = [(2.0e4, 1800.0, 0.03)]
+# the coupled programs' tools, modes and material: a 12 mm flat end mill and an 8 mm ball
+# end mill taking turns, two modes per axis, A5052 with the size effect and process damping;
+# program p runs n_lines - 1 + p % 4 passes, so the programs differ in length
+COUPLED_TOOLS = [dict(N=3, R=6.0, helixDeg=30.0, type="flat", fluteLen=30.0, runout=0.005, runoutPhase=0.0),
+                 dict(N=2, R=4.0, helixDeg=30.0, type="ball", fluteLen=20.0, runout=0.005, runoutPhase=0.0)]
+COUPLED_MODES = [(2.0e4, 1800.0, 0.03), (6.0e4, 3200.0, 0.04)]
 COUPLED_SPEED = 10000.0   # min^-1, every line of the coupled programs
+
+
+def coupled_plan(p, n_lines):
+    """(tool, passes) of coupled program p"""
+    return COUPLED_TOOLS[p % 2], n_lines - 1 + p % 4
