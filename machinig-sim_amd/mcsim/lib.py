@@ -319,6 +319,8 @@ def sim_run(programs):
     for i, g in enumerate(programs):
         sub = g["sub"]
         ns = int(g.get("nsub", len(sub)))
+        if not 0 <= ns <= len(sub):
+            raise ValueError("program %d: nsub %d outside [0, %d] (its sub-step array)" % (i, ns, len(sub)))
         out = (SubstepOut * max(1, ns))()
         outs.append((out, ns))
         lut = g.get("lut")

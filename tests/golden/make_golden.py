@@ -38,6 +38,7 @@ import math
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -329,8 +330,11 @@ def build_cases(quick):
 def main():
     quick = "--quick" in sys.argv
     cases = build_cases(quick)
-    work = os.path.join(ROOT, "_work")
-    os.makedirs(work, exist_ok=True)
+    with tempfile.TemporaryDirectory() as work:
+        generate(cases, quick, work)
+
+
+def generate(cases, quick, work):
     # chatter cases use toolModel's values for the steel tool: ask the reference first
     cin = os.path.join(work, "golden_cases.json")
     cout = os.path.join(work, "golden_out.json")

@@ -12,6 +12,7 @@ import os
 import re
 import subprocess
 import sys
+import tempfile
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -28,17 +29,16 @@ def main():
     every = int(sys.argv[1]) if len(sys.argv) > 1 else 4
     cells, opts = W.sld_40rpm()
     cells = cells[::every]
-    work = os.path.join(ROOT, "_work")
-    os.makedirs(work, exist_ok=True)
-    cin, cout = os.path.join(work, "time_cases.json"), os.path.join(work, "time_out.json")
-    with open(cin, "w") as f:
-        json.dump({"tdl": [dict(cell=cell_json(c), opts=dict(opts)) for c in cells]}, f)
-    cmd = ["node", "--experimental-loader", os.path.join(HERE, "downlevel_loader.mjs"),
-           os.path.join(HERE, "run_reference.mjs"), cin, cout]
-    r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True, check=True)
-    m = re.search(r"^tdl \d+ ([0-9.]+) s$", r.stderr, re.M)
-    t_ref = float(m.group(1))
-    ref = json.load(open(cout))["tdl"]
+    with tempfile.TemporaryDirectory() as work:
+        cin, cout = os.path.join(work, "time_cases.json"), os.path.join(work, "time_out.json")
+        with open(cin, "w") as f:
+            json.dump({"tdl": [dict(cell=cell_json(c), opts=dict(opts)) for c in cells]}, f)
+        cmd = ["node", "--experimental-loader", os.path.join(HERE, "downlevel_loader.mjs"),
+               os.path.join(HERE, "run_reference.mjs"), cin, cout]
+        r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True, check=True)
+        m = re.search(r"^tdl \d+ ([0-9.]+) s$", r.stderr, re.M)
+        t_ref = float(m.group(1))
+        ref = json.load(open(cout))["tdl"]
     import oracle
     t0 = time.perf_counter()
     got = [oracle.time_domain_limit(c, **opts) for c in cells]
