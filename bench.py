@@ -46,27 +46,58 @@ def flop_alg(cells, res):
     return tot
 
 
-def cpu_baseline(cells, opts, target_s):
+def reference_limits(workload):
+    """The reference's own answers on the default lobe grid (tests/golden/golden_tdl_grid.json,
+    every cell resolved by its timeDomainLimit under Node in the build container), or None
+    for another workload."""
+    if workload != "sld_grid_256speeds_x_64tools":
+        return None
+    import base64
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "golden_tdl_grid.json")) as f:
+            g = json.load(f)
+    except OSError:
+        return None
+    arr = lambda k: np.frombuffer(base64.b64decode(g[k]), dtype="<f8")
+    return dict(alim=arr("alim"), lo=arr("lo"), hi=arr("hi"), node_s=g["node_s"], procs=g["procs"], node=g["node"])
+
+
+def same_limits(res, ref):
+    """cells whose (alim, lo, hi) differ from the reference's"""
+    alim = np.array([r.alim for r in res]); lo = np.array([r.lo for r in res]); hi = np.array([r.hi for r in res])
+    return int(np.count_nonzero((alim != ref["alim"]) | (lo != ref["lo"]) | (hi != ref["hi"])))
+
+
+def cpu_baseline(cells, opts, target_s, gpu_res=None):
     """The oracle (FP64 CPU restatement) on a bounded sample of the same cells: a seeded
     shuffle of the grid, run in chunks of 8 cells per thread (one cell per thread at a
-    time) until target_s seconds have elapsed."""
+    time) until target_s seconds have elapsed.  With gpu_res, each sampled cell's answer is
+    compared with the GPU's (alim, bracket, trials, steps, visits)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(threads, 64))
     order = np.random.RandomState(12345).permutation(len(cells))
-    done, dt, pos = 0, 0.0, 0
+    done, dt, pos, diff = 0, 0.0, 0, 0
     chunk = 8 * threads
     while dt < target_s and pos < len(order):
         idx = order[pos:pos + chunk]
         pos += len(idx)
         arr = cell_array([cells[i] for i in idx])
         t0 = time.perf_counter()
-        oracle.bisect_batch(arr, len(idx), threads=threads, **opts)
+        got = oracle.bisect_batch(arr, len(idx), threads=threads, **opts)
         dt += time.perf_counter() - t0
         done += len(idx)
-    return dict(value=done / dt, unit="SLD cells/s", cores=threads, kind="port",
-                sample="%d of %d cells (seeded shuffle), %.1f s on %d threads" % (done, len(cells), dt, threads))
+        if gpu_res is not None:
+            diff += sum(1 for i, o in zip(idx, got)
+                        if (o.alim, o.lo, o.hi, o.open, o.trials, o.steps, o.visits) !=
+                        (gpu_res[i].alim, gpu_res[i].lo, gpu_res[i].hi, gpu_res[i].open, gpu_res[i].trials,
+                         gpu_res[i].steps, gpu_res[i].visits))
+    out = dict(value=done / dt, unit="SLD cells/s", cores=threads, kind="port",
+               sample="%d of %d cells (seeded shuffle), %.1f s on %d threads" % (done, len(cells), dt, threads))
+    if gpu_res is not None:
+        out["cells_differing_from_gpu"] = diff
+    return out
 
 
 def calibrate_copy(nbytes):
@@ -393,6 +424,16 @@ def main():
     # the roofline is that of the plain launch of cell_search (the one the rocprofv3
     # summaries in profiles/ cover); the guided launch is the same kernel with another
     # speculation order, and its time is in config.surrogate
+    ref = reference_limits(out_workload := "sld_grid_%dspeeds_x_%dtools" % (a.speeds, a.tools))
+    ref_check = None
+    if ref is not None:
+        ref_check = dict(cells=n, plain_cells_differing=same_limits(res, ref),
+                         reference="timeDomainLimit of the reference (Node %s, %d processes, %.0f s for the grid: "
+                                   "%.1f cells/s) from tests/golden/golden_tdl_grid.json" %
+                                   (ref["node"], ref["procs"], ref["node_s"], n / ref["node_s"]),
+                         reference_cells_per_s=n / ref["node_s"])
+        if surr is not None:
+            ref_check["guided_cells_differing"] = same_limits(gres, ref)
     flops = flop_alg(cells, res)
     achieved = flops / (kms_mean * 1e-3) / 1e12
     trials = sum(r.trials for r in res)
@@ -410,18 +451,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Park-Miller tool/mode grid; surrogate trained on other seeded tools)",
-        "config": {"workload": "sld_grid_%dspeeds_x_%dtools" % (a.speeds, a.tools), "cells": n,
+        "config": {"workload": out_workload, "cells": n,
                    "bisection": opts, "value_path": path, "plain_cells_per_s": plain_value,
                    "plain_ms_per_step": ms, "plain_kernel_ms": kms_mean, "trials": trials,
                    "slice_visits": visits, "slice_visits_per_s": visits / (ms_step / 1e3),
-                   "surrogate": surr, "schedule": tuning or lib.search_tuning()},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                   "surrogate": surr, "schedule": tuning or lib.search_tuning(),
+                   "reference_check": ref_check,
+                   "identical_to_reference": None if ref_check is None else
+                   (ref_check["plain_cells_differing"] == 0 and ref_check.get("guided_cells_differing", 0) == 0)},
+        "roofline": {"bound": "mfma", "pipe": "FP64 VALU", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic("cell_search"),
                      "kernel": "cell_search", "launch": "plain", "kernel_ms": kms_mean, "flop_alg": flops,
-                     "roof": "FP64 compute (MI355X FP64 vector rate 78.6 TFLOP/s, equal to its "
-                             "FP64 matrix rate); the kernel is FP64 VALU work, a serial "
-                             "recurrence per trial with no matrix shape",
+                     "roof": "compute bound, on the FP64 vector (VALU) pipe: MI355X's FP64 vector rate "
+                             "78.6 TFLOP/s (equal to its FP64 matrix rate); the kernel issues no matrix "
+                             "instructions: each trial is a serial recurrence with no matrix shape",
                      "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated)"},
     }
@@ -430,7 +474,8 @@ def main():
     if not a.no_coupled:
         out["coupled"] = coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu, opts, model)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cells, opts, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cells, opts, a.cpu_seconds, res)
+        out["config"]["identical_to_oracle_sample"] = out["cpu_baseline"]["cells_differing_from_gpu"] == 0
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

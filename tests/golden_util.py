@@ -358,3 +358,23 @@ def display_arcs(out):
             st, ex = o.phiSt, o.phiEx
         arcs += [st, ex]
     return np.array(arcs)
+
+
+def grid_limits():
+    """The reference's timeDomainLimit over the whole 256-speed x 64-tool lobe grid
+    (golden_tdl_grid.json, cells in mcsim.workloads.sld_grid() order): dict of arrays
+    alim, lo, hi (+inf for open cells) and open, plus the options and Node's wall time."""
+    g = load("tdl_grid")
+    out = dict(alim=f64(g["alim"]), lo=f64(g["lo"]), hi=f64(g["hi"]),
+               open=np.frombuffer(base64.b64decode(g["open"]), dtype=np.uint8).astype(bool))
+    out.update(opts=g["opts"], n=g["n"], node_s=g["node_s"], procs=g["procs"])
+    return out
+
+
+# Cells of the lobe grid whose answer differs from the reference's because V8 7.8's Math.pow
+# (Node 12) and the fdlibm e_pow restated by the oracle and the kernels differ in the last
+# bit on a few % of inputs (the size effect (0.05/h)^mc of src/cutting.js:197): found by
+# running the oracle over all 16,384 cells against golden_tdl_grid.json.  14077 ends one
+# bisection step (one bracket) from the reference and 13924 two; 5395 is stable at aMax for the oracle (open) and chatters
+# there for the reference (alim 1.856 mm).  The GPU must give the oracle's answers on them.
+POW_ULP_CELLS = (5395, 13924, 14077)
