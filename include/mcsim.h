@@ -333,6 +333,19 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
                       const float* luts, const int32_t* lut_off, const int32_t* lut_len,
                       int nluts, double* vol);
 
+/* mc_stamp_batch without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during a call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The per-stamp records live in a
+ * fixed scratch array of the library (MC_NOALLOC_SCRATCH_MB, held by each device the library
+ * runs on until the process ends) instead of the stock's grow-only buffers, so at most
+ * MC_NOALLOC_MAX_STAMPS stamps per call (more: MC_ERR_LIMIT, nothing applied).  Calls no
+ * hipMalloc / hipFree.  Results as mc_stamp_batch.  Every call on a device shares that
+ * device's array, which keeps the last call's contents; calls that overlapped on one device
+ * would race for it, so the library's call lock serialises them within a process. */
+#define MC_NOALLOC_MAX_STAMPS 65536
+#define MC_NOALLOC_SCRATCH_MB 13
+int mc_stamp_batch_noalloc(mc_stock* s, const mc_stamp* stamps, int n, double* vol);
+
 /* feed-mark maps markS [mm] and markF [mm] (f32; src/stock.js:19-23); either may be NULL */
 int mc_stock_read_marks(const mc_stock* s, float* markS, float* markF);
 

@@ -51,6 +51,9 @@ int mcs_launch_max_in_disk(const float* H, const float* bmax, int nx, int ny, in
 int mcs_launch_sample(const float* H, int nx, int ny, double hx, double hy, const double* d_xy,
                       int n, double* d_out, hipStream_t stream);
 int mcs_launch_fill(float* p, size_t n, float v, hipStream_t stream);
+int mcs_scratch_setup(int device);
+int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull, void** box, int32_t** touched,
+                        double** vol);
 
 namespace {
 
@@ -159,6 +162,8 @@ int ensure_init(int device) {
   HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&g_ctx.ev0));
   HIPCHK(hipEventCreate(&g_ctx.ev1));
+  // this device's copy of the no-allocation entry point's scratch array
+  if (mcs_scratch_setup(device)) return fail(MC_ERR_HIP, "hipGetSymbolAddress of the stamp scratch failed");
   g_ctx.cus = prop.multiProcessorCount;
   g_ctx.name = prop.name;
   g_ctx.arch = prop.gcnArchName;
@@ -1107,6 +1112,49 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
     return fail(MC_ERR_HIP, "stamp kernels failed: %s", hipGetErrorString(hipGetLastError()));
   s->bmax_stale = true;
   if (vol) HIPCHK(hipMemcpyAsync(vol, s->vol.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, q));
+  HIPCHK(hipStreamSynchronize(q));
+  {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1));
+    g_ctx.last_ms = ms;
+  }
+  if (vol) {
+    const double area = s->hx * s->hy;
+    for (int i = 0; i < n; i++) vol[i] *= area;
+  }
+  return MC_OK;
+}
+
+int mc_stamp_batch_noalloc(mc_stock* s, const mc_stamp* stamps, int n, double* vol) {
+  if (!s || n < 0 || (n > 0 && !stamps)) return fail(MC_ERR_ARG, "bad stamp batch");
+  if (n > MC_NOALLOC_MAX_STAMPS)
+    return fail(MC_ERR_LIMIT, "%d stamps: the no-allocation batch takes at most MC_NOALLOC_MAX_STAMPS = %d", n,
+                MC_NOALLOC_MAX_STAMPS);
+  for (int i = 0; i < n; i++) {
+    const mc_stamp& p = stamps[i];
+    if (!std::isfinite(p.cx) || !std::isfinite(p.cy) || !std::isfinite(p.tipZ) || !(p.R > 0) ||
+        (p.type != MC_TOOL_FLAT && p.type != MC_TOOL_BALL))
+      return fail(MC_ERR_ARG, "stamp %d: bad position, radius or type", i);
+  }
+  if (n == 0) return MC_OK;
+  mc_stamp* d_in;
+  void *d_dev, *d_cull, *d_box;
+  int32_t* d_touched;
+  double* d_vol;
+  if (mcs_scratch_regions(g_ctx.dev, &d_in, &d_dev, &d_cull, &d_box, &d_touched, &d_vol))
+    return fail(MC_ERR_NODEV, "the library is not initialised on a device");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  hipStream_t q = g_ctx.stream;
+  HIPCHK(hipMemcpyAsync(d_in, stamps, (size_t)n * sizeof(mc_stamp), hipMemcpyHostToDevice, q));
+  // the volumes start at zero (a fill kernel: no memset node if a caller captures this);
+  // stamp_prep rewrites the records, boxes and touched flags of the n stamps
+  if (vol && mcs_launch_fill((float*)d_vol, (size_t)n * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
+  if (mcs_launch_stamps(d_in, nullptr, n, nullptr, nullptr, nullptr, nullptr, d_dev, nullptr, d_cull, d_box, d_touched,
+                        s->nx, s->ny, s->hx, s->hy, s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut,
+                        s->lineWall, s->markS, s->markF, s->rect, vol ? d_vol : nullptr, q, g_ctx.ev0, g_ctx.ev1))
+    return fail(MC_ERR_HIP, "stamp kernels failed: %s", hipGetErrorString(hipGetLastError()));
+  s->bmax_stale = true;
+  if (vol) HIPCHK(hipMemcpyAsync(vol, d_vol, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, q));
   HIPCHK(hipStreamSynchronize(q));
   {
     float ms = 0;

@@ -27,6 +27,8 @@
 // wavefront and stamp, so its last bits depend on the summation order.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "../../include/mcsim.h"
 #include "mc_math.h"
 
@@ -472,6 +474,65 @@ __global__ void fill_f32(float* p, size_t n, float v) {
 }
 
 }  // namespace mcs
+
+// ---- scratch of the no-allocation stamp batch (mc_stamp_batch_noalloc)
+// The uploaded stamps, their expanded and culling records, the chunk boxes, the touched
+// flags and the per-stamp volumes of at most MC_NOALLOC_MAX_STAMPS stamps, in one file-scope
+// device array.  Each device the library runs on holds its own copy until the process ends;
+// on the host the array's name is only a stand-in, so kernels get the address that
+// hipGetSymbolAddress returns for the current device (resolved once per device at set-up).
+namespace mcs {
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_MAX = MC_NOALLOC_MAX_STAMPS;
+constexpr size_t NA_CHUNKS = (NA_MAX + CHUNK - 1) / CHUNK;
+constexpr size_t NA_OFF_DEV = align256(sizeof(mc_stamp) * NA_MAX);
+constexpr size_t NA_OFF_CULL = align256(NA_OFF_DEV + sizeof(StampDev) * NA_MAX);
+constexpr size_t NA_OFF_BOX = align256(NA_OFF_CULL + (sizeof(int4) + sizeof(float)) * NA_MAX);
+constexpr size_t NA_OFF_TOUCHED = align256(NA_OFF_BOX + sizeof(ChunkBox) * NA_CHUNKS);
+constexpr size_t NA_OFF_VOL = align256(NA_OFF_TOUCHED + sizeof(int32_t) * NA_MAX);
+constexpr size_t NA_BYTES = align256(NA_OFF_VOL + sizeof(double) * NA_MAX);
+constexpr int NA_DEVICES = 64;
+static_assert(NA_BYTES <= ((size_t)MC_NOALLOC_SCRATCH_MB << 20), "scratch larger than the documented size");
+}  // namespace mcs
+
+static __device__ __align__(256) unsigned char scratch[mcs::NA_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[mcs::NA_DEVICES] = {};
+}  // namespace
+
+// resolve this device's copy of the scratch array; called with `device` current
+int mcs_scratch_setup(int device) {
+  if (device < 0 || device >= mcs::NA_DEVICES) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -2;
+  g_scratch_ptr[device] = (unsigned char*)p;
+  return 0;
+}
+
+// the regions of the device's scratch (null when set-up has not resolved it)
+int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull, void** box, int32_t** touched,
+                        double** vol) {
+  if (device < 0 || device >= mcs::NA_DEVICES) return -1;
+  unsigned char* p;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    p = g_scratch_ptr[device];
+  }
+  if (!p) return -1;
+  *in = (mc_stamp*)p;
+  *stampdev = p + mcs::NA_OFF_DEV;
+  *cull = p + mcs::NA_OFF_CULL;
+  *box = p + mcs::NA_OFF_BOX;
+  *touched = (int32_t*)(p + mcs::NA_OFF_TOUCHED);
+  *vol = (double*)(p + mcs::NA_OFF_VOL);
+  return 0;
+}
+
+size_t mcs_scratch_bytes() { return mcs::NA_BYTES; }
 
 // ---- host-side launchers used by api.cpp
 size_t mcs_stampdev_bytes() { return sizeof(mcs::StampDev); }

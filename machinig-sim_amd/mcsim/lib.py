@@ -30,7 +30,7 @@ EXPORTS = [
     "mc_engine_set_wear", "mc_engine_set_work", "mc_engine_set_engagement", "mc_engine_set_end_engagement",
     "mc_engine_set_maps", "mc_engine_get_state", "mc_engine_set_state", "mc_engine_run", "mc_engine_run_batch",
     "mc_engine_read_stats", "mc_engine_chip_events", "mc_engine_chatter_index", "mc_engine_read_rings",
-    "mc_sim_run", "mc_js_math",
+    "mc_sim_run", "mc_js_math", "mc_stamp_batch_noalloc",
 ]
 
 _LIB = None
@@ -68,6 +68,7 @@ def load():
     L.mc_stock_write_h.argtypes = [P, P]
     L.mc_stock_read_side.argtypes = [P, P, P, P, P]
     L.mc_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
+    L.mc_stamp_batch_noalloc.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
     L.mc_stock_sample.argtypes = [P, P, C.c_int, P]
     L.mc_stamp_batch_ex.argtypes = [P, C.POINTER(Stamp), P, C.c_int, P, P, P, C.c_int, P]
     L.mc_stock_read_marks.argtypes = [P, P, P]
