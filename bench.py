@@ -31,6 +31,7 @@ from mcsim import lib, workloads as W, path  # noqa: E402
 from mcsim._abi import cell_array, MC_TOOL_BALL  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, vendor figure
+DYN_BUDGET = 220000.0     # the simulator's default dynamics budget per frame (src/sim.js:50)
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, vendor figure
 
 
@@ -155,33 +156,23 @@ def heightmap(steps):
                      "tiles stay in LDS across the stamps of a chunk")
 
 
-def line_cells(progs, outs, tools, revs=40):
-    """The machining report's per-line stability question (src/report.js:27-28 keeps each
-    line's heaviest cut with an engagement arc; src/main.js:1045 asks for the limit at the
-    line's speed): one cell per cutting line of every program, from the pipeline's axial
-    depths and the engagement arc the simulator shows after each sub-step."""
+def line_cells(progs, outs, frs, tools, revs=40):
+    """The machining report's per-line stability question: each program's report is built
+    from the pipeline's frames (mcsim.report: Simulator.collect and LineReport,
+    src/sim.js:644-710, src/report.js:9-33), and every line with an engagement arc asks for
+    its limit at its heaviest cut's arc and the line's speed (src/main.js:1033-1038), here
+    by the time-domain search.  Returns the cells, (program, line, ap) keys and the per-frame
+    chatter states of every program."""
     from mcsim._abi import make_cell
-    cells, keys = [], []
-    for p, ((sub, n), out, tool) in enumerate(zip(progs, outs, tools)):
-        best = {}
-        st, ex = 0.0, math.pi
-        for k in range(n):
-            o = out[k]
-            if o.arc:
-                st, ex = o.phiSt, o.phiEx
-            if not o.cut:
-                continue
-            r = best.get(sub[k].line)
-            if ex > st and (r is None or o.ap >= r[0]):
-                best[sub[k].line] = (o.ap, st, ex, sub[k].fzEff)
-        for ln in sorted(best):
-            ap, st_, ex_, fz = best[ln]
-            if not ap > 0.01:          # an end cut alone: no flute engagement to limit
-                continue
-            cells.append(make_cell(tool, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (st_, ex_), fz, revs,
-                                   True))
+    from mcsim import report as RP
+    cells, keys, states = [], [], []
+    for p, ((sub, n, frames), out, fr, tool) in enumerate(zip(progs, outs, frs, tools)):
+        rep = RP.report_program(sub, out, frames, fr)
+        states.append(rep.states)
+        for ln, ap, st, ex, fz in RP.line_engagements(rep.report, sub, out):
+            cells.append(make_cell(tool, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (st, ex), fz, revs, True))
             keys.append((p, ln, ap))
-    return cells, keys
+    return cells, keys, states
 
 
 def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
@@ -189,11 +180,14 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     deflected stamps, src/sim.js:351-449) for n_prog seeded raster programs, each on its own
     2048 x 2048 stock at 0.05 mm (102.35 mm square), flat 12 mm and ball 8 mm tools taking
     turns, two modes per axis, 3 to 6 passes (workloads.coupled_plan), one launch of
-    mc_sim_run (one workgroup per program).  Reported in sub-steps per second; the oracle's
+    mc_sim_run (one workgroup per program), in the simulator's frames of 4000 loop passes
+    (sim.step with an unbounded machine-time budget), each frame's engine stats taken and its
+    chatter index computed on the device.  Reported in sub-steps per second; the oracle's
     restated loop runs the same programs on host threads as the CPU baseline and checks the
-    device's deflections, step counts and final heights bit for bit.  Then every line's
-    stability check (line_cells) runs as one batched time-domain search, guided by the
-    surrogate when one was trained (checked identical to the plain search)."""
+    device's deflections, step counts, depths, forces, frames and final heights.  Then the
+    report (mcsim.report) is built from the frames and every line's stability check
+    (line_cells) runs as one batched time-domain search, guided by the surrogate when one was
+    trained (checked identical to the plain search)."""
     from mcsim._abi import make_cell, MC_TOOL_FLAT, MC_TOOL_BALL
     width, cell = 102.35, 0.05
     hx = width / 2047
@@ -201,7 +195,9 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     for p in range(n_prog):
         tool, m = W.coupled_plan(p, n_lines)
         x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, m, speed=W.COUPLED_SPEED)
-        progs.append(path.sim_substeps(x, y, z, r, ln, F, S, tool["N"], hx))
+        # frames of 4000 loop passes with an unbounded machine-time budget (sim.step(1e6)): the
+        # frame budget's share is then stepsPerTooth / 4 for every cutting sub-step
+        progs.append(path.sim_frames(x, y, z, r, ln, F, S, tool["N"], hx, 1e6))
         tools.append(tool)
         passes.append(m)
     ecells = [make_cell(t, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (0.0, 0.0), 0.0, 1, True) for t in tools]
@@ -210,14 +206,14 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
 
     def setup(limit=None):
         P = []
-        for (sub, n), t, ec, ty in zip(progs, tools, ecells, ttype):
+        for (sub, n, _), t, ec, ty in zip(progs, tools, ecells, ttype):
             act, ide = lib.Stock(width, width, 20, cell), lib.Stock(width, width, 20, cell)
             ide.set_wall_band(2.0)
             eng = lib.Engine()
-            info = eng.configure(ec)
+            eng.configure(ec)
             eng.set_wear(kte, kre)
             P.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=n if limit is None else min(n, limit),
-                          R=t["R"], fluteLen=t["fluteLen"], rw=0.0, share=info.stepsPerTooth / 4, type=ty))
+                          R=t["R"], fluteLen=t["fluteLen"], rw=0.0, type=ty, dynBudget=DYN_BUDGET))
         return P
 
     def release(P):
@@ -229,16 +225,16 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     release(W0)
     P = setup()
     t0 = time.perf_counter()
-    outs = lib.sim_run(P)
+    outs, frs = lib.sim_run(P, frames=True)
     wall = time.perf_counter() - t0
     kms = lib.last_kernel_ms()
-    nsub = sum(n for _, n in progs)
+    nsub = sum(n for _, n, _ in progs)
     steps = sum(o.nsteps for out in outs for o in out)
     cuts = sum(o.cut for out in outs for o in out)
     # algorithmic bytes: both stamps of every sub-step read and write the f32 height of each
     # node of their bounding boxes (the commanded stock's wall band is 2 mm)
     nodes = 0
-    for (sub, n), t in zip(progs, tools):
+    for (sub, n, _), t in zip(progs, tools):
         cxs = np.array([sub[k].x for k in range(n)]); cys = np.array([sub[k].y for k in range(n)])
         for Rw in (t["R"] + 2 * hx, t["R"] + 2.0):
             i0 = np.maximum(0, np.floor((cxs - Rw) / hx)); i1 = np.minimum(2047, np.ceil((cxs + Rw) / hx))
@@ -254,7 +250,9 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
                note="algorithmic bytes = 8 B x bbox nodes of both stamps per sub-step; one workgroup per "
                     "program runs its sub-steps in order (each reads the stock the previous one cut), so "
                     "the kernel is bound by the latency of that chain, far below the HBM roof")
-    lc, keys = line_cells(progs, outs, tools)
+    lc, keys, states = line_cells(progs, outs, frs, tools)
+    res["frames"] = sum(len(f) for f in frs)
+    res["frame_states"] = {k: sum(st.count(k) for st in states) for k in ("stable", "marginal", "chatter", "idle")}
     t0 = time.perf_counter()
     plain = lib.bisect_batch(lc, **opts)
     t_plain = time.perf_counter() - t0
@@ -277,17 +275,19 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
         import oracle
         k = min(cpu_threads, n_prog)
         outs_c = [None] * k
+        frs_c = [None] * k
         H_c = [None] * k
 
         def work(i):
-            sub, n = progs[i]
+            sub, n, frames = progs[i]
             act, ide = oracle.OracleStock(width, width, 20, cell), oracle.OracleStock(width, width, 20, cell)
             ide.set_wall_band(2.0)
             eng = oracle.OracleEngine()
-            info = eng.configure(ecells[i])
+            eng.configure(ecells[i])
             eng.set_wear(kte, kre)
-            outs_c[i] = oracle.sim_run(act, ide, eng, sub, n, tools[i]["R"], tools[i]["fluteLen"], 0.0,
-                                       info.stepsPerTooth / 4, ttype[i])
+            outs_c[i], frs_c[i] = oracle.sim_run(act, ide, eng, sub, n, tools[i]["R"], tools[i]["fluteLen"], 0.0, 1.0,
+                                                 ttype[i], dynBudget=DYN_BUDGET,
+                                                 nframes=sum(1 for _, c in frames if c))
             H_c[i] = act.H()
         th = [threading.Thread(target=work, args=(i,)) for i in range(k)]
         t0 = time.perf_counter()
@@ -297,7 +297,10 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
             t.join()
         cw = time.perf_counter() - t0
         same = all(all(a.dx == b.dx and a.dy == b.dy and a.nsteps == b.nsteps and a.arc == b.arc and
-                       a.phiSt == b.phiSt and a.phiEx == b.phiEx for a, b in zip(outs[i], outs_c[i]))
+                       a.phiSt == b.phiSt and a.phiEx == b.phiEx and a.ap == b.ap and a.errVal == b.errVal and
+                       a.maxF == b.maxF and a.cut == b.cut and abs(a.vol - b.vol) <= 1e-12 * max(1.0, abs(b.vol))
+                       for a, b in zip(outs[i], outs_c[i]))
+                   and all(bytes(a) == bytes(b) for a, b in zip(frs[i], frs_c[i])) and len(frs[i]) == len(frs_c[i])
                    and np.array_equal(P[i]["actual"].H(), H_c[i]) for i in range(k))
         csub = sum(progs[i][1] for i in range(k))
         res["cpu_baseline"] = dict(value=csub / cw, unit="sub-steps/s", cores=k, kind="port",

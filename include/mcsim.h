@@ -413,7 +413,7 @@ int mc_tip_lut(int32_t shape, double D, double cornerR, const double* edge_r, in
  * the tool flat, ball or bull-nose (its profile cutterRadiusAt in the engine's slices, its
  * tip table in the stamps and the end face, as the simulator sets them up, src/sim.js:190-218). */
 
-/* one sub-step, as the simulator's schedule produces it (src/sim.js:363-381,389-405) */
+/* one sub-step, as the simulator's schedule produces it (src/sim.js:353-381,389-405) */
 typedef struct {
   double x, y, z;     /* tool position p [mm] */
   double ux, uy;      /* the feed direction the simulator holds (this.u) */
@@ -423,7 +423,21 @@ typedef struct {
   double dxy;         /* xy length of the move [mm]: below 1e-6 a vertical move */
   int32_t rapid;      /* a G0 move */
   int32_t line;       /* program line (Stock.curLine) */
+  double nSubEst;     /* its frame's sub-step estimate (src/sim.js:358) */
+  int32_t guard;      /* the frame loop's count with this sub-step, segment switches included (:362) */
+  int32_t frame;      /* MC_FRAME_FIRST: the frame's dynamics budget starts here (:360);
+                         MC_FRAME_LAST: the frame ends after it (collect, :450) */
 } mc_substep;
+enum { MC_FRAME_FIRST = 1, MC_FRAME_LAST = 2 };
+
+/* what the engine gave a frame (Simulator.collect, src/sim.js:644-663): its takeStats() and,
+   after it, its chatterIndex() */
+typedef struct {
+  mc_engine_stats stats;   /* takeStats() at the frame's end (src/cutting.js:320) */
+  double index, amp, once; /* chatterIndex() (src/cutting.js:299-318) */
+  int32_t end;             /* sub-steps of the call done when the frame ended */
+  int32_t pad;
+} mc_frame_out;
 
 /* what a sub-step did */
 typedef struct {
@@ -451,17 +465,26 @@ typedef struct {
   double R;                /* tool radius [mm] */
   double fluteLen;         /* [mm] */
   double rw;               /* radial flank wear taken off the actual stamp [mm] (src/sim.js:439) */
-  double share;            /* steps a sub-step may run besides its own time: the frame
-                              budget's share, max(stepsPerTooth / 4, ...) (src/sim.js:410-412) */
+  double share;            /* with dynBudget <= 0: the steps a sub-step may run besides its own
+                              time, a fixed share of the frame budget (src/sim.js:410-412) */
   int32_t nsub;
   int32_t type;            /* MC_TOOL_FLAT | MC_TOOL_BALL | MC_TOOL_BULL */
   const float* lut;        /* MC_TOOL_BULL: the tool's tipLut (mc_tip_lut), lutn entries */
   int32_t lutn;
-  int32_t pad;
+  int32_t nframes;         /* capacity of frames */
   double cornerR;          /* MC_TOOL_BULL: corner radius [mm] (cutterRadiusAt, src/assembly.js:23-30) */
+  double dynBudget;        /* > 0: the frame's dynamics budget cfg.dynBudget (src/sim.js:50,360): each
+                              cutting sub-step runs max(1, min(its own steps, floor(share))) steps
+                              with share = max(stepsPerTooth / 4, dynLeft / cost / max(1, nSubEst -
+                              guard + 1)), cost = N max(1, nz + end rings), and dynLeft falls by
+                              steps x cost (:409-414) */
+  double fz;               /* this.fz when the call starts: the feed mark until a sub-step cuts */
+  mc_frame_out* frames;    /* NULL, or room for the frames (MC_FRAME_LAST sub-steps) of the call:
+                              at each frame's end the engine's stats are taken into it and its
+                              chatter index computed */
 } mc_sim_program;
 
-/* n programs, one wavefront each, one launch; the stocks and engines must be distinct */
+/* n programs, one workgroup each, one launch; the stocks and engines must be distinct */
 int mc_sim_run(const mc_sim_program* progs, int n);
 
 /* ---- V8's Math.* on the host (no reference counterpart as an entry point) ----

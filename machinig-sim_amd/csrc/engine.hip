@@ -32,36 +32,7 @@ __global__ __launch_bounds__(64) void engine_run(EngDev* const* engines) {
 // chatterIndex() over the device rings (src/cutting.js:299-318), one lane
 __global__ __launch_bounds__(64) void engine_chatter_index(const EngDev* E, double* out) {
   if (threadIdx.x != 0) return;
-  const int64_t n = E->ts_count < 24 ? E->ts_count : 24;
-  if (n < 8) { out[0] = 0; out[1] = 0; out[2] = 0; return; }
-  const float* tsx = E->rings;
-  const float* tsy = E->rings + RN;
-  double mx = 0, my = 0, v = 0;
-  for (int64_t k = 0; k < n; k++) { const int64_t j = (E->ts_i - 1 - k + RN) % RN; mx += tsx[j]; my += tsy[j]; }
-  mx /= n; my /= n;
-  for (int64_t k = 0; k < n; k++) {
-    const int64_t j = (E->ts_i - 1 - k + RN) % RN;
-    const double dx = tsx[j] - mx, dy = tsy[j] - my;
-    v += dx * dx + dy * dy;
-  }
-  const double once = __builtin_sqrt(v / n) * 1e6;
-  const double r = mcm::round_js((double)n * E->spt / E->decim);
-  const int64_t m = (double)E->hist_count < r ? E->hist_count : (int64_t)r;
-  const float* hx = E->hist + 3 * HN;
-  const float* hy = E->hist + 4 * HN;
-  double ax = 0, ay = 0, w = 0;
-  for (int64_t k = 0; k < m; k++) { const int64_t j = (E->hist_i - 1 - k + HN) % HN; ax += hx[j]; ay += hy[j]; }
-  const double mm = m ? (double)m : 1.0;
-  ax /= mm; ay /= mm;
-  for (int64_t k = 0; k < m; k++) {
-    const int64_t j = (E->hist_i - 1 - k + HN) % HN;
-    const double dx = hx[j] - ax, dy = hy[j] - ay;
-    w += dx * dx + dy * dy;
-  }
-  const double amp = __builtin_sqrt(w / mm);
-  out[0] = once / (amp + 0.05);
-  out[1] = amp;
-  out[2] = once;
+  eng_chatter_index(*E, out);
 }
 
 __global__ void fill_zero_f32(float* p, int64_t n) {
@@ -69,11 +40,6 @@ __global__ void fill_zero_f32(float* p, int64_t n) {
     p[i] = 0.0f;
 }
 
-void new_stats(double* st) {
-  for (int k = 0; k < S_COUNT; k++) st[k] = 0;
-  st[S_MINX] = __builtin_inf(); st[S_MAXX] = -__builtin_inf();
-  st[S_MINY] = __builtin_inf(); st[S_MAXY] = -__builtin_inf();
-}
 
 }  // namespace
 
@@ -164,7 +130,7 @@ int mc_engine_create(mc_engine** out) {
   std::lock_guard<std::mutex> lk(mch::lock());
   mc_engine* e = new mc_engine();
   std::memset(&e->h, 0, sizeof(EngDev));
-  new_stats(e->h.st);
+  eng_new_stats(e->h.st);
   e->h.wm = 1; e->h.wnx = 1;
   if (hipMalloc(&e->d, sizeof(EngDev)) != hipSuccess || hipMalloc(&e->h.hist, sizeof(float) * 5 * HN) != hipSuccess ||
       hipMalloc(&e->h.rings, sizeof(float) * 6 * RN) != hipSuccess) {
@@ -378,7 +344,7 @@ int mc_engine_run(mc_engine* e, int64_t nSteps) {
 int mc_engine_read_stats(mc_engine* e, mc_engine_stats* out, int take) {
   if (int st = check_engine(e)) return st;
   if (out) std::memcpy(out, e->h.st, sizeof(mc_engine_stats));
-  if (take) new_stats(e->h.st);
+  if (take) eng_new_stats(e->h.st);
   return MC_OK;
 }
 

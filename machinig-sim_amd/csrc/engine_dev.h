@@ -101,6 +101,49 @@ struct EngLds {
   float Hb[MC_BINS], Eb[EBINS * ERINGS];
 };
 
+// chatterIndex() (src/cutting.js:299-318) over the engine's rings, by one thread: the RMS
+// scatter of the last n <= 24 once-per-tooth samples about their mean, over the RMS of the
+// last min(count, round(n spt / decim)) history samples about theirs, both newest first
+__device__ __forceinline__ void eng_chatter_index(const EngDev& E, double* out) {
+  const int64_t n = E.ts_count < 24 ? E.ts_count : 24;
+  if (n < 8) { out[0] = 0; out[1] = 0; out[2] = 0; return; }
+  const float* tsx = E.rings;
+  const float* tsy = E.rings + RN;
+  double mx = 0, my = 0, v = 0;
+  for (int64_t k = 0; k < n; k++) { const int64_t j = (E.ts_i - 1 - k + RN) % RN; mx += tsx[j]; my += tsy[j]; }
+  mx /= n; my /= n;
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t j = (E.ts_i - 1 - k + RN) % RN;
+    const double dx = tsx[j] - mx, dy = tsy[j] - my;
+    v += dx * dx + dy * dy;
+  }
+  const double once = __builtin_sqrt(v / n) * 1e6;
+  const double r = mcm::round_js((double)n * E.spt / E.decim);
+  const int64_t m = (double)E.hist_count < r ? E.hist_count : (int64_t)r;
+  const float* hx = E.hist + 3 * HN;
+  const float* hy = E.hist + 4 * HN;
+  double ax = 0, ay = 0, w = 0;
+  for (int64_t k = 0; k < m; k++) { const int64_t j = (E.hist_i - 1 - k + HN) % HN; ax += hx[j]; ay += hy[j]; }
+  const double mm = m ? (double)m : 1.0;
+  ax /= mm; ay /= mm;
+  for (int64_t k = 0; k < m; k++) {
+    const int64_t j = (E.hist_i - 1 - k + HN) % HN;
+    const double dx = hx[j] - ax, dy = hy[j] - ay;
+    w += dx * dx + dy * dy;
+  }
+  const double amp = __builtin_sqrt(w / mm);
+  out[0] = once / (amp + 0.05);
+  out[1] = amp;
+  out[2] = once;
+}
+
+// newStats() (src/cutting.js:323-326)
+__host__ __device__ __forceinline__ void eng_new_stats(double* st) {
+  for (int k = 0; k < S_COUNT; k++) st[k] = 0;
+  st[S_MINX] = __builtin_inf(); st[S_MAXX] = -__builtin_inf();
+  st[S_MINY] = __builtin_inf(); st[S_MAXY] = -__builtin_inf();
+}
+
 // run(nSteps) of src/cutting.js:146-296 on engine E by the calling wavefront
 __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& L) {
   auto& term = L.term;

@@ -42,8 +42,9 @@ struct ProgDev {
   const mc_substep* sub;
   mc_substep_out* out;
   const float* lut;   // bull-nose: the tip table (null otherwise)
-  double R, fluteLen, rw, share, rc;
-  int32_t nsub, ball, N, lutn;
+  mc_frame_out* frames;
+  double R, fluteLen, rw, share, rc, dynBudget, fz0;
+  int32_t nsub, ball, N, lutn, nframes, pad;
 };
 
 // Stock.sample (src/stock.js:85-92)
@@ -200,9 +201,12 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   const double R = P.R;
   const ToolShape shape{P.ball, R, P.rc, P.lut, P.lutn};
   const double cell = jmax(P.act.hx, P.act.hy);
-  double fzLast = 0;   // this.fz: the last cutting sub-step's feed per tooth
+  double fzLast = P.fz0;   // this.fz: the last cutting sub-step's feed per tooth
+  double dynLeft = P.dynBudget;   // the frame's dynamics budget left (src/sim.js:360)
+  int frame = 0;                  // frames ended so far
   for (int k = 0; k < P.nsub; k++) {
     const mc_substep s = P.sub[k];
+    if (s.frame & MC_FRAME_FIRST) dynLeft = P.dynBudget;
     const double vx = -s.uy, vy = s.ux;
     // ---- fillEngagement (src/sim.js:301-319); a vertical move keeps the previous map
     double hmax = -__builtin_inf();
@@ -275,9 +279,16 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
       }
       __threadfence_block();
       __syncthreads();
-      // the sub-step's dynamics: its own time, capped by the frame budget's share (:410-412)
+      // the sub-step's dynamics: its own time, capped by the frame budget's share (:409-414)
       const double want = jmax(1, mcm::round_js(s.dtm / E.dt));
-      n = (int)jmax(1, jmin(want, __builtin_floor(P.share)));
+      if (P.dynBudget > 0) {
+        const double cost = (double)E.N * jmax(1, E.nz + (endCut ? MC_END_RINGS : 0));
+        const double share = jmax(E.spt / 4.0, dynLeft / cost / jmax(1, s.nSubEst - s.guard + 1));
+        n = (int)jmax(1, jmin(want, __builtin_floor(share)));
+        dynLeft -= n * cost;
+      } else {
+        n = (int)jmax(1, jmin(want, __builtin_floor(P.share)));
+      }
       if (tid < 64) engine_steps(E, n, L);
       else stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, P.lut, P.lutn, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64,
                       &flags[0]);
@@ -311,6 +322,23 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
       o.dx = dx; o.dy = dy; o.ap = ap; o.errVal = errVal; o.vol = vol; o.maxF = maxF; o.nsteps = n; o.cut = cut ? 1 : 0;
       o.phiSt = phiSt; o.phiEx = phiEx; o.arc = arc; o.pad = 0;
     }
+    // the frame's end (Simulator.collect, src/sim.js:644-663): takeStats() into the frame's
+    // record, then chatterIndex(); the next sub-step's run starts from fresh stats
+    if ((s.frame & MC_FRAME_LAST) && P.frames && frame < P.nframes) {
+      if (tid == 0) {
+        mc_frame_out& fo = P.frames[frame];
+        double* st = (double*)&fo.stats;
+        for (int q = 0; q < S_COUNT; q++) st[q] = E.st[q];
+        eng_new_stats(E.st);
+        double ci[3];
+        eng_chatter_index(E, ci);
+        fo.index = ci[0]; fo.amp = ci[1]; fo.once = ci[2];
+        fo.end = k + 1; fo.pad = 0;
+      }
+      frame++;
+      __threadfence_block();
+      __syncthreads();
+    }
   }
 }
 
@@ -340,8 +368,11 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
       for (int i = 0; i < g.lutn; i++)
         if (!std::isfinite(g.lut[i])) return mch::fail(MC_ERR_ARG, "program %d: non-finite tip table", p);
     }
-    if (!(g.R > 0) || !(g.rw >= 0) || !(g.rw < g.R) || !(g.fluteLen > 0) || !(g.share >= 1))
-      return mch::fail(MC_ERR_ARG, "program %d: invalid radius, wear, flute length or share", p);
+    if (!(g.R > 0) || !(g.rw >= 0) || !(g.rw < g.R) || !(g.fluteLen > 0) || !std::isfinite(g.fz) ||
+        !(g.dynBudget > 0 ? std::isfinite(g.dynBudget) : g.share >= 1))
+      return mch::fail(MC_ERR_ARG, "program %d: invalid radius, wear, flute length, feed, share or budget", p);
+    if (g.nframes < 0 || (g.nframes > 0 && !g.frames))
+      return mch::fail(MC_ERR_ARG, "program %d: null frames with a frame capacity", p);
     for (const void* q : {(const void*)g.actual, (const void*)g.ideal, (const void*)g.engine})
       for (const void* o : seen)
         if (o == q) return mch::fail(MC_ERR_ARG, "program %d shares a stock or an engine with another", p);
@@ -349,8 +380,9 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     for (int k = 0; k < g.nsub; k++) {
       const mc_substep& s = g.sub[k];
       if (!std::isfinite(s.x) || !std::isfinite(s.y) || !std::isfinite(s.z) || !std::isfinite(s.ux) ||
-          !std::isfinite(s.uy) || !(s.dtm >= 0) || !std::isfinite(s.fzEff) || !std::isfinite(s.fza) || !(s.dxy >= 0))
-        return mch::fail(MC_ERR_ARG, "program %d sub-step %d: non-finite input", p, k);
+          !std::isfinite(s.uy) || !(s.dtm >= 0) || !std::isfinite(s.fzEff) || !std::isfinite(s.fza) || !(s.dxy >= 0) ||
+          std::isnan(s.nSubEst) || s.frame < 0 || s.frame > (MC_FRAME_FIRST | MC_FRAME_LAST))
+        return mch::fail(MC_ERR_ARG, "program %d sub-step %d: non-finite input or bad frame flags", p, k);
     }
     EngDev* m = mce_engine_mirror(g.engine);
     if (!m) return mch::fail(MC_ERR_ARG, "program %d: the engine is not configured", p);
@@ -370,33 +402,39 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     d.nsub = g.nsub; d.ball = g.type == MC_TOOL_BALL; d.N = m->N;
     d.lut = nullptr; d.lutn = g.type == MC_TOOL_BULL ? g.lutn : 0;
     d.rc = g.type == MC_TOOL_BULL ? std::min(g.R, g.cornerR) : 0.0;
+    d.dynBudget = g.dynBudget > 0 ? g.dynBudget : 0; d.fz0 = g.fz;
+    d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0; d.pad = 0;
   }
   std::lock_guard<std::mutex> lk(mch::lock());
   hipStream_t q = mch::stream();
   // sub-steps in, results out: one device buffer for every program
-  size_t nin = 0, nlut = 0;
+  size_t nin = 0, nlut = 0, nfr = 0;
   for (int p = 0; p < n; p++) {
     nin += (size_t)progs[p].nsub;
     if (progs[p].type == MC_TOOL_BULL) nlut += (size_t)progs[p].lutn;
+    nfr += (size_t)pd[(size_t)p].nframes;
   }
   float* dlut = nullptr;
   mc_substep* dsub = nullptr;
   mc_substep_out* dout = nullptr;
+  mc_frame_out* dfr = nullptr;
   ProgDev* dprog = nullptr;
   auto release = [&]() {
     if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
     if (dout) (void)hipFree(dout);
+    if (dfr) (void)hipFree(dfr);
     if (dprog) (void)hipFree(dprog);
   };
   if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
+      hipMalloc(&dfr, sizeof(mc_frame_out) * std::max<size_t>(1, nfr)) != hipSuccess ||
       hipMalloc(&dprog, sizeof(ProgDev) * (size_t)n) != hipSuccess ||
       hipMalloc(&dlut, sizeof(float) * std::max<size_t>(1, nlut)) != hipSuccess) {
     release();
     return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
   }
-  size_t off = 0, loff = 0;
+  size_t off = 0, loff = 0, foff = 0;
   int st = MC_OK;
   for (int p = 0; p < n && !st; p++) {
     const mc_sim_program& g = progs[p];
@@ -408,6 +446,10 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     }
     pd[(size_t)p].sub = dsub + off;
     pd[(size_t)p].out = dout + off;
+    if (pd[(size_t)p].nframes > 0) {
+      pd[(size_t)p].frames = dfr + foff;
+      foff += (size_t)pd[(size_t)p].nframes;
+    }
     if (g.nsub > 0 && hipMemcpyAsync(dsub + off, g.sub, sizeof(mc_substep) * (size_t)g.nsub, hipMemcpyHostToDevice, q) != hipSuccess)
       st = mch::fail(MC_ERR_HIP, "sub-step upload");
     off += (size_t)g.nsub;
@@ -426,14 +468,24 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (hipGetLastError() != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline launch failed");
     (void)hipEventRecord(e1, q);
   }
+  // once the kernel has run, every engine's host mirror is refreshed from its device record,
+  // even after a failed download, so that no mirror keeps the state from before the launch
+  const bool launched = !st;
   off = 0;
-  for (int p = 0; p < n && !st; p++) {
+  foff = 0;
+  for (int p = 0; p < n && launched; p++) {
     const mc_sim_program& g = progs[p];
-    if (g.out && g.nsub > 0 &&
+    if (!st && g.out && g.nsub > 0 &&
         hipMemcpyAsync(g.out, dout + off, sizeof(mc_substep_out) * (size_t)g.nsub, hipMemcpyDeviceToHost, q) != hipSuccess)
       st = mch::fail(MC_ERR_HIP, "result download");
     off += (size_t)g.nsub;
-    if (!st) st = mce_engine_finish(g.engine);
+    const int nf = pd[(size_t)p].nframes;
+    if (!st && nf > 0 &&
+        hipMemcpyAsync(g.frames, dfr + foff, sizeof(mc_frame_out) * (size_t)nf, hipMemcpyDeviceToHost, q) != hipSuccess)
+      st = mch::fail(MC_ERR_HIP, "frame download");
+    foff += (size_t)nf;
+    const int fst = mce_engine_finish(g.engine);
+    if (!st) st = fst;
   }
   if (!st && hipStreamSynchronize(q) != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline failed");
   if (!st) {

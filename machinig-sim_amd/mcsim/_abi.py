@@ -111,13 +111,22 @@ class EngineStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in STATS_FIELDS]
 
 
+class FrameOut(C.Structure):
+    _fields_ = [("stats", EngineStats), ("index", C.c_double), ("amp", C.c_double), ("once", C.c_double),
+                ("end", C.c_int32), ("pad", C.c_int32)]
+
+
 class ChipEvent(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("vol", "phi", "z", "ux", "uy", "vx", "vy")]
 
 
 class Substep(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "ux", "uy", "dtm", "fzEff", "fza", "dxy")] + \
-               [("rapid", C.c_int32), ("line", C.c_int32)]
+               [("rapid", C.c_int32), ("line", C.c_int32), ("nSubEst", C.c_double), ("guard", C.c_int32),
+                ("frame", C.c_int32)]
+
+
+MC_FRAME_FIRST, MC_FRAME_LAST = 1, 2
 
 
 class SubstepOut(C.Structure):
@@ -130,7 +139,8 @@ class SimProgram(C.Structure):
                 ("sub", C.POINTER(Substep)), ("out", C.POINTER(SubstepOut)),
                 ("R", C.c_double), ("fluteLen", C.c_double), ("rw", C.c_double), ("share", C.c_double),
                 ("nsub", C.c_int32), ("type", C.c_int32), ("lut", C.POINTER(C.c_float)), ("lutn", C.c_int32),
-                ("pad", C.c_int32), ("cornerR", C.c_double)]
+                ("nframes", C.c_int32), ("cornerR", C.c_double), ("dynBudget", C.c_double), ("fz", C.c_double),
+                ("frames", C.c_void_p)]
 
 
 MC_STOCK_B = 8

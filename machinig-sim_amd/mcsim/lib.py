@@ -11,7 +11,8 @@ import numpy as np
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
                    SearchTuning, EngageQuery, EngageResult, EngageSpec, cell_array, MC_OK, MC_SURR_IN,
                    MC_STOCK_B, EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST,
-                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram, MC_MAX_MODES)
+                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram, MC_MAX_MODES, FrameOut,
+                   MC_FRAME_LAST)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libmcsim.so")
@@ -309,14 +310,17 @@ def js_math(fn, x, y=None):
     return out
 
 
-def sim_run(programs):
+def sim_run(programs, frames=False):
     """the simulator's sub-step loop on the device for several programs in one launch.
     programs: dicts with actual, ideal (Stock), engine (Engine), sub (Substep array) and
-    optionally nsub (its length), R, fluteLen, rw, share, type (MC_TOOL_*), and for a
-    bull-nose tool lut (its tipLut) and cornerR; returns a SubstepOut list per program"""
+    optionally nsub (its length), R, fluteLen, rw, share, type (MC_TOOL_*), for a bull-nose
+    tool lut (its tipLut) and cornerR, dynBudget (> 0: the frame budget's share per
+    sub-step instead of the fixed share) and fz (this.fz at the start); returns a SubstepOut
+    list per program, and with frames=True also a FrameOut list per program (one per
+    sub-step flagged MC_FRAME_LAST)"""
     n = len(programs)
     arr = (SimProgram * max(1, n))()
-    outs, keep = [], []
+    outs, keep, frs = [], [], []
     for i, g in enumerate(programs):
         sub = g["sub"]
         ns = int(g.get("nsub", len(sub)))
@@ -327,14 +331,22 @@ def sim_run(programs):
         lut = g.get("lut")
         lut_arr = np.ascontiguousarray(lut, dtype=np.float32) if lut is not None else None
         keep.append(lut_arr)
+        nf = sum(1 for k in range(ns) if sub[k].frame & MC_FRAME_LAST) if frames else 0
+        fr = (FrameOut * max(1, nf))()
+        frs.append((fr, nf))
         arr[i] = SimProgram(g["actual"].p.value, g["ideal"].p.value, g["engine"].p.value,
                             C.cast(sub, C.POINTER(Substep)), C.cast(out, C.POINTER(SubstepOut)),
-                            float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g["share"]),
+                            float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g.get("share", 1.0)),
                             ns, int(g["type"]),
                             lut_arr.ctypes.data_as(C.POINTER(C.c_float)) if lut_arr is not None else None,
-                            len(lut_arr) if lut_arr is not None else 0, 0, float(g.get("cornerR", 0.0)))
+                            len(lut_arr) if lut_arr is not None else 0, nf, float(g.get("cornerR", 0.0)),
+                            float(g.get("dynBudget", 0.0)), float(g.get("fz", 0.0)),
+                            C.cast(fr, C.c_void_p) if nf else None)
     _check(load().mc_sim_run(arr, n))
-    return [o[:ns] for o, ns in outs]
+    res = [o[:ns] for o, ns in outs]
+    if frames:
+        return res, [f[:nf] for f, nf in frs]
+    return res
 
 
 class Stock:

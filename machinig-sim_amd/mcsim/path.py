@@ -138,6 +138,75 @@ def sim_substeps(x, y, z, rapid, line, F, S, N, hx):
     return arr, len(rows)
 
 
+GUARD = 4000   # loop passes per frame at most (src/sim.js:362)
+
+
+def sim_frames(x, y, z, rapid, line, F, S, N, hx, dt_real, speed=1.0, feed_override=1.0, max_frames=1 << 30):
+    """The sub-steps of successive Simulator.step(dt_real) frames at playback speed `speed`
+    (src/sim.js:351-377,448), as mc_substep records: each frame covers dt_real * speed of
+    machine time, its last sub-step shortened to what is left of it (ds = min(dsMax, segLen -
+    segS, v budgetT)); the frame's sub-step estimate nSubEst = max(1, ceil(budgetT rate / 60 /
+    stampStep)) from the segment it starts on, and each sub-step's guard (the loop's count in
+    its frame, segment switches included) feed the dynamics budget's share on the device
+    (src/sim.js:358-362,410).  The first sub-step of a frame carries MC_FRAME_FIRST, the
+    last MC_FRAME_LAST.  F, S: the path points' feed [mm/min] and speed [rpm] (spindle
+    override 1); N the tool's flutes.  Returns (Substep array, count, frames), frames a list of
+    (first sub-step, sub-step count) per frame."""
+    from ._abi import Substep, MC_FRAME_FIRST, MC_FRAME_LAST
+    step = stamp_step(hx)
+    npts = len(x)
+    rows, frames = [], []
+    u = (1.0, 0.0)
+    seg, seg_s = 1, 0.0
+    done = npts < 2
+    while not done and len(frames) < max_frames:
+        budget = dt_real * speed
+        c = min(seg, npts - 1)
+        rate = 15000 if rapid[c] else F[c] * feed_override
+        n_sub_est = max(1, math.ceil(budget * rate / 60 / step))
+        guard = 0
+        first = len(rows)
+        while budget > 1e-9 and not done:
+            if not guard < GUARD:
+                break
+            guard += 1
+            ax, ay, az, bx, by, bz = x[seg - 1], y[seg - 1], z[seg - 1], x[seg], y[seg], z[seg]
+            seg_len = hypot_js(bx - ax, by - ay, bz - az)
+            v = (15000 if rapid[seg] else F[seg] * feed_override) / 60
+            ds_max = 1.0 if rapid[seg] else step
+            ds = min(ds_max, seg_len - seg_s, v * budget)
+            if seg_len - seg_s <= 1e-9:
+                seg, seg_s = seg + 1, 0.0
+                if seg >= npts:
+                    done, seg = True, npts - 1
+                continue
+            dtm = ds / v
+            budget -= dtm
+            seg_s += ds
+            f = seg_s / seg_len if seg_len > 0 else 1.0
+            px, py, pz = ax + (bx - ax) * f, ay + (by - ay) * f, az + (bz - az) * f
+            dxy = hypot_js(bx - ax, by - ay)
+            if dxy > 1e-6:
+                u = ((bx - ax) / dxy, (by - ay) / dxy)
+            fza = fz_eff = 0.0
+            if not rapid[seg]:
+                Sk, Fk = S[seg], F[seg] * feed_override
+                fza = (Fk / (Sk * N)) * ((az - bz) / seg_len) if (bz < az and Sk >= 1) else 0.0
+                fz_eff = (Fk / (Sk * N)) * (dxy / seg_len) if dxy > 1e-6 else 0.0
+            rows.append(Substep(px, py, pz, u[0], u[1], dtm, fz_eff, fza, dxy, 1 if rapid[seg] else 0, int(line[seg]),
+                                float(n_sub_est), guard, 0))
+            if seg_s >= seg_len - 1e-9:
+                seg, seg_s = seg + 1, 0.0
+                if seg >= npts:
+                    done, seg = True, npts - 1
+        if len(rows) > first:
+            rows[first].frame |= MC_FRAME_FIRST
+            rows[-1].frame |= MC_FRAME_LAST
+        frames.append((first, len(rows) - first))
+    arr = (Substep * max(1, len(rows)))(*rows)
+    return arr, len(rows), frames
+
+
 CHORD_TOL = 0.001   # largest chord error of an arc [mm] (src/gcode.js:433)
 
 

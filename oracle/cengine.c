@@ -478,6 +478,7 @@ void or_engine_read_rings(const or_engine *e, float *hist, float *ts, float *ws,
 
 double or_engine_dt(const or_engine *e) { return e->dt; }
 int or_engine_flutes(const or_engine *e) { return e->N; }
+int or_engine_spt(const or_engine *e) { return e->spt; }
 double or_engine_run_max_f(const or_engine *e) { return e->runMaxF; }
 double or_engine_dz(const or_engine *e) { return e->dz; }
 int or_engine_max_slices(const or_engine *e) { return e->maxSlices; }

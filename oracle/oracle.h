@@ -144,7 +144,8 @@ double or_fill_end_engagement(const or_stock *s, double cx, double cy, double ti
                               double R, int ball, float *Eb, double *rOut_out);
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
-               mc_substep_out *out);
+               mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes);
+int or_engine_spt(const or_engine *e);
 
 #ifdef __cplusplus
 }

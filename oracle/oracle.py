@@ -14,7 +14,7 @@ sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
 
 from mcsim._abi import (Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec,  # noqa: E402
                         EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST, MC_ENGINE_RING,
-                        Substep, SubstepOut)
+                        Substep, SubstepOut, FrameOut)
 
 _LIB = None
 
@@ -60,7 +60,7 @@ def lib():
         L.or_engine_chatter_index.argtypes = [P, P]
         L.or_engine_read_rings.argtypes = [P, P, P, P, P]
         L.or_sim_run.argtypes = [P, P, P, C.POINTER(Substep), C.c_int] + [C.c_double] * 4 + \
-            [C.c_int, P, C.c_int, C.c_double, C.POINTER(SubstepOut)]
+            [C.c_int, P, C.c_int, C.c_double, C.POINTER(SubstepOut), C.c_double, C.c_double, P, C.c_int]
         L.or_directional_coeffs.argtypes = [C.c_double, C.c_double, C.c_double, P]
         L.or_stability_lobes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_int, C.c_double, C.c_double, C.POINTER(Mode), C.c_int,
@@ -272,18 +272,24 @@ class OracleEngine:
         return h, r[0], r[1], r[2]
 
 
-def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None, cornerR=0.0):
-    """the simulator's sub-step loop restated (oracle/simloop.c); returns SubstepOut records"""
+def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None, cornerR=0.0, dynBudget=0.0,
+            fz0=0.0, nframes=0):
+    """the simulator's sub-step loop restated (oracle/simloop.c); returns SubstepOut records,
+    and with nframes > 0 also the FrameOut records of the frames that ended"""
     import numpy as np
     out = (SubstepOut * max(1, n))()
+    fr = (FrameOut * max(1, nframes))()
     la = np.ascontiguousarray(lut, dtype=np.float32) if lut is not None else None
     st = lib().or_sim_run(actual.p, ideal.p, engine.h, C.cast(sub, C.POINTER(Substep)), int(n), float(R),
                           float(fluteLen), float(rw), float(share), int(typ),
                           la.ctypes.data if la is not None else None, len(la) if la is not None else 0,
-                          float(cornerR), out)
+                          float(cornerR), out, float(dynBudget), float(fz0), fr if nframes else None, int(nframes))
     if st:
         raise RuntimeError("or_sim_run: %d" % st)
-    return out[:n]
+    if not nframes:
+        return out[:n]
+    ended = min(nframes, sum(1 for k in range(n) if sub[k].frame & 2))
+    return out[:n], fr[:ended]
 
 
 def composite_lobes(nodes, step, hi):

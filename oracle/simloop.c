@@ -7,7 +7,8 @@
  *   fillEndEngagement  src/sim.js:324-341  (below)
  *   axial depth, end cut, the cut decision                 src/sim.js:388-401
  *   setEngagement / setEndEngagement / setWork(null)       src/sim.js:403-408
- *   the run's step count (the frame budget's share)        src/sim.js:409-414
+ *   the run's step count (the frame budget's share)        src/sim.js:358-362,409-414
+ *   the frame's end: takeStats() and chatterIndex()        src/sim.js:450,645-663
  *   the engagement arc it shows (immersion, :343-348)      src/sim.js:421-423
  *   the decay out of the cut                               src/sim.js:425-429
  *   deflection, the commanded and the actual stamp         src/sim.js:433-440
@@ -94,14 +95,14 @@ static void end_rings(double rOut, const Shape *t, double *g) {
 
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
-               mc_substep_out *out) {
+               mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes) {
   const int ball = type == MC_TOOL_BALL, bull = type == MC_TOOL_BULL;
   const Shape shape = {ball, R, bull ? lut : NULL, bull ? lutn : 0, bull ? (cornerR < R ? cornerR : R) : 0};
   const float *slut = bull ? lut : NULL;
   const int stype = ball ? MC_TOOL_BALL : MC_TOOL_FLAT;
   float Hb[MC_BINS], Eb[MC_END_BINS * MC_END_RINGS];
-  double fzLast = 0;
-  int k;
+  double fzLast = fz0, dynLeft = dynBudget;
+  int k, frame = 0;
   memset(Hb, 0, sizeof(Hb));
   memset(Eb, 0, sizeof(Eb));
   for (k = 0; k < n; k++) {
@@ -110,6 +111,7 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     double hmax = -INFINITY, ap, dmax = 0, rOut = 0, dx, dy, errVal, vol, maxF = 0;
     int haveEnd = s->fza > 0, endCut, cut, nsteps = 0, arc;
     double phiSt, phiEx;
+    if (s->frame & MC_FRAME_FIRST) dynLeft = dynBudget;
     if (!(s->dxy < 1e-6)) hmax = or_fill_engagement(act, s->x, s->y, s->ux, s->uy, R, Hb);
     ap = hmax - s->z > 0 ? hmax - s->z : 0;
     if (fluteLen < ap) ap = fluteLen;
@@ -147,8 +149,24 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
       or_engine_set_maps(e, Hb, Eb);
       want = or_js_round(s->dtm / or_engine_dt(e));
       if (want < 1) want = 1;
-      nsteps = (int)(want < floor(share) ? want : floor(share));
-      if (nsteps < 1) nsteps = 1;
+      if (dynBudget > 0) {
+        /* cost = tool.N * max(1, e.nz + (endCut ? END_RINGS : 0)); share = max(spt / 4, dynLeft
+           / cost / max(1, nSubEst - guard + 1)) (src/sim.js:410-411) */
+        mc_engine_state es;
+        double cost, sh, den;
+        or_engine_get_state(e, &es);
+        cost = (double)or_engine_flutes(e) * (es.nz + (endCut ? MC_END_RINGS : 0) > 1 ? es.nz + (endCut ? MC_END_RINGS : 0) : 1);
+        den = s->nSubEst - s->guard + 1;
+        if (!(den > 1)) den = 1;
+        sh = dynLeft / cost / den;
+        if (!(sh > or_engine_spt(e) / 4.0)) sh = or_engine_spt(e) / 4.0;
+        nsteps = (int)(want < floor(sh) ? want : floor(sh));
+        if (nsteps < 1) nsteps = 1;
+        dynLeft -= nsteps * cost;
+      } else {
+        nsteps = (int)(want < floor(share) ? want : floor(share));
+        if (nsteps < 1) nsteps = 1;
+      }
       or_engine_run(e, nsteps);
       maxF = or_engine_run_max_f(e);
       fzLast = s->fzEff;
@@ -170,6 +188,15 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
       out[k].dx = dx; out[k].dy = dy; out[k].ap = ap; out[k].errVal = errVal; out[k].vol = vol;
       out[k].maxF = maxF; out[k].nsteps = nsteps; out[k].cut = cut;
       out[k].phiSt = arc ? phiSt : 0; out[k].phiEx = arc ? phiEx : 0; out[k].arc = arc; out[k].pad = 0;
+    }
+    /* Simulator.collect at the frame's end: takeStats(), then chatterIndex() */
+    if ((s->frame & MC_FRAME_LAST) && frames && frame < nframes) {
+      double ci[3];
+      or_engine_stats(e, &frames[frame].stats, 1);
+      or_engine_chatter_index(e, ci);
+      frames[frame].index = ci[0]; frames[frame].amp = ci[1]; frames[frame].once = ci[2];
+      frames[frame].end = k + 1; frames[frame].pad = 0;
+      frame++;
     }
   }
   return MC_OK;

@@ -26,6 +26,10 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_paths.json     the G-code parser's moves and the path discretizer's polylines
   golden_clobes.json    Simulator.lobes: the composite (speed-dependent) lobe curve and the
                         diagram at rest, with the speeds and modes its nodes were built from
+  golden_report.json    Simulator runs with real-time frames (playback speed, dynamics budget):
+                        the sub-steps of every frame, every stamp and engine run, per frame the
+                        engine's taken stats, chatter index, chatter state and flank wear, and
+                        the machining report (LineReport.pack)
   golden_englog.json    the engine's whole call sequence inside simulator runs (configure,
                         setWear, setWork, setEngagement, setEndEngagement, run, takeStats,
                         chatterIndex) with its state, stats, rings and chip events after
@@ -223,6 +227,18 @@ def build_cases(quick):
     # rigid workpiece, linear force law, flat, ball and bull-nose tools: the device sub-step pipeline
     cases["simx"] = [dict(cases["sim"][0]), dict(cases["sim"][1]), dict(ribc),
                      dict(cases["sim"][0], tool=dict(type="ball", D=8.0)), dict(cases["sim"][2])]
+    # real-time frames: the same programs stepped at playback speeds 20 and 120 (the second
+    # compresses the dynamics to the frame budget's share), and a longer raster at speed 60
+    raster = ["G90 G54 G17", "S10000 M3", "G0 X-6 Y2 Z5", "G1 Z-1.2 F300"]
+    for k in range(6):
+        y0 = 2 + 3 * k
+        raster += ["G1 X34 Y%g F1200" % y0, "G1 Y%g" % (y0 + 1.5), "G1 X-6", "G1 Y%g" % (y0 + 3)]
+    raster += ["G0 Z5", "M30"]
+    cases["report"] = [dict(cases["sim"][0], speed=20, dtReal=1 / 60), dict(cases["sim"][1], speed=120, dtReal=1 / 60),
+                       dict(cases["sim"][2], speed=60, dtReal=1 / 60),
+                       dict(program="\n".join(raster), tool=dict(type="flat", D=8.0),
+                            stock=dict(type="box", w=30.0, l=24.0, h=6.0), cell=0.2, speed=60, dtReal=1 / 60,
+                            dynBudget=60000)]
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]
@@ -349,7 +365,7 @@ def generate(cases, quick, work):
                 m["k"], m["fn"] = tm["k"], tm["fn"]
             md["k"], md["fn"] = tm["k"], tm["fn"]
     keys = ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
-            "enginex", "englog", "simx", "paths", "clobes")
+            "enginex", "englog", "simx", "paths", "clobes", "report")
     if "--only" in sys.argv:
         keys = tuple(sys.argv[sys.argv.index("--only") + 1].split(","))
         # engage cases read the stock cases

@@ -256,6 +256,89 @@ if (cases.simx) {
   console.error('simx', (Date.now() - t0) / 1e3, 's');
 }
 
+// whole simulator runs with real-time frames (a rigid workpiece, the linear force law): the
+// sub-steps each frame of sim.step(dtReal) takes at playback speed `speed` under the dynamics
+// budget, every stamp, every engine run, and per frame (collect) the engine's taken stats,
+// chatter index, the display's chatter state, the flank wear before and after, and at the
+// end the machining report (LineReport.pack)
+if (cases.report) {
+  const { Simulator } = await import(REF + 'sim.js');
+  const { modeSet } = await import(REF + 'frf.js');
+  const ST = ['n', 'sumF', 'maxF', 'sumFx', 'sumFy', 'sumFz', 'maxFz', 'sumT', 'maxT', 'sumX', 'sumY', 'minX',
+    'maxX', 'minY', 'maxY', 'sumZc', 'wZc', 'sumW', 'maxW', 'sumHb', 'sumB', 'sumRb', 'engTeeth'];
+  out.report = cases.report.map(({ program, tool, stock, cell, speed, dtReal, dynBudget }) => {
+    const sim = new Simulator();
+    sim.cfg.cell = cell;
+    sim.cfg.workpieceFlex = false;
+    sim.cfg.cuttingNonlinear = false;
+    if (dynBudget) sim.cfg.dynBudget = dynBudget;
+    Object.assign(sim.cfg.tool, tool);
+    sim.load(program, { stock, origin: { x: 0, y: 0, z: 0 } });
+    sim.speed = speed;
+    const rec = { actual: [], ideal: [] };
+    const wrap = (st, o) => {
+      const f = st.stamp.bind(st);
+      st.stamp = (cx, cy, tipZ, R, type, errVal, mark, lut) => {
+        const v = f(cx, cy, tipZ, R, type, errVal, mark, lut);
+        o.push([cx, cy, tipZ, R, type === 'ball' ? 1 : 0, errVal, mark ? mark.f : 0, st.curLine, v]);
+        return v;
+      };
+    };
+    wrap(sim.stock, rec.actual); wrap(sim.ideal, rec.ideal);
+    const darc = [];
+    const fa = sim.stock.stamp;
+    sim.stock.stamp = (...a) => { darc.push(sim.display.phiSt, sim.display.phiEx); return fa(...a); };
+    const e = sim.engine, cfgs = [], wears = [], runs = [];
+    const fc = e.configure.bind(e), fw = e.setWear.bind(e), fr = e.run.bind(e);
+    e.configure = (t, modal, mat, S, fnMax = 0) => {
+      fc(t, modal, mat, S, fnMax);
+      const ms = modeSet(modal), md = L => L.map(m => [m.k, m.fn, m.zeta]);
+      cfgs.push([rec.actual.length, t.N, t.R, t.helixDeg, t.type, t.fluteLen, t.nonlinear === false ? 0 : 1, t.runout || 0,
+        t.runoutPhase || 0, S, fnMax, md(ms.x), md(ms.y),
+        [mat.Ktc, mat.Krc, mat.Kte, mat.Kre, mat.Kac, mat.Kae, mat.mc != null ? mat.mc : 0, mat.Ci != null ? mat.Ci : 0]]);
+    };
+    e.setWear = edge => {
+      fw(edge);
+      const last = wears[wears.length - 1];
+      if (!last || last[1] !== edge.Kte || last[2] !== edge.Kre) wears.push([rec.actual.length, edge.Kte, edge.Kre]);
+    };
+    e.run = n => { fr(n); runs.push([rec.actual.length, n]); };
+    const ft = e.takeStats.bind(e), fci = e.chatterIndex.bind(e);
+    let took = null, ci = null;
+    e.takeStats = () => { const s = ft(); took = ST.map(k => s[k]); return s; };
+    e.chatterIndex = () => { const c = fci(); ci = [c.index, c.amp, c.once === undefined ? null : c.once]; return c; };
+    const fcol = sim.collect.bind(sim), frames = [];
+    sim.collect = (fs, dt) => {
+      const VB0 = sim.VB || 0;
+      took = null; ci = null;
+      fcol(fs, dt);
+      const d = sim.display;
+      frames.push({ end: rec.actual.length, stats: took, ci, state: d.state, chatterIdx: d.chatterIdx, once: d.once,
+        VB0, VB: sim.VB || 0, line: sim.line, lastCutLine: sim.lastCutLine, ap: sim.ap || 0, phiSt: d.phiSt, phiEx: d.phiEx,
+        cutTime: fs.cutTime });
+    };
+    let n = 0;
+    while (!sim.done && n < 20000) { sim.step(dtReal); n++; }
+    const col = (rows, k) => rows.map(r => r[k]);
+    const pack = rows => ({ cx: b64f64(col(rows, 0)), cy: b64f64(col(rows, 1)), tipZ: b64f64(col(rows, 2)),
+      R: b64f64(col(rows, 3)), type: col(rows, 4), errVal: b64f64(col(rows, 5)), f: b64f64(col(rows, 6)),
+      line: col(rows, 7), vol: b64f64(col(rows, 8)) });
+    const P = sim.path, t = sim.activeTool;
+    const clr = t.clearanceDeg != null ? t.clearanceDeg : (sim.cfg.tool.clearanceDeg != null ? sim.cfg.tool.clearanceDeg : 10);
+    const maps = s => ({ H: z32(Float32Array, s.H), err: z32(Float32Array, s.err), wallD: z32(Float32Array, s.wallD),
+      lineCut: z32(Int32Array, s.lineCut), lineWall: z32(Int32Array, s.lineWall), markS: z32(Float32Array, s.markS),
+      markF: z32(Float32Array, s.markF), removed: s.removed });
+    return { nframes: n, frames, darc: b64f64(darc), nx: sim.stock.nx, ny: sim.stock.ny, hx: sim.stock.hx, hy: sim.stock.hy,
+      zBottom: sim.stock.zBottom, toolR: t.R, toolN: t.N, fluteLen: t.fluteLen, type: t.type, clearanceDeg: clr,
+      dynBudget: sim.cfg.dynBudget, speed: sim.speed, feedOverride: sim.cfg.feedOverride, spindleOverride: sim.cfg.spindleOverride,
+      path: { x: b64f64(P.map(p => p.x)), y: b64f64(P.map(p => p.y)), z: b64f64(P.map(p => p.z)),
+        rapid: P.map(p => (p.rapid ? 1 : 0)), line: P.map(p => p.line), F: P.map(p => p.F), S: P.map(p => p.S) },
+      cfgs, wears, runs, actual: pack(rec.actual), ideal: pack(rec.ideal), stock: maps(sim.stock), commanded: maps(sim.ideal),
+      report: sim.report.pack() };
+  });
+  console.error('report', (Date.now() - t0) / 1e3, 's');
+}
+
 // the parser's moves and the path discretizer (arcs into chords) on short programs
 if (cases.paths) {
   const { parseGcode, discretize } = await import(REF + 'gcode.js');

@@ -378,3 +378,111 @@ def grid_limits():
 # bisection step (one bracket) from the reference and 13924 two; 5395 is stable at aMax for the oracle (open) and chatters
 # there for the reference (alim 1.856 mm).  The GPU must give the oracle's answers on them.
 POW_ULP_CELLS = (5395, 13924, 14077)
+
+
+def report_schedule(res, case):
+    """the real-time frame schedule of a recorded run (golden_report.json): sub-steps of
+    sim.step(dtReal) frames at the recorded playback speed"""
+    from mcsim import path
+    P = res["path"]
+    return path.sim_frames(f64(P["x"]), f64(P["y"]), f64(P["z"]), P["rapid"], P["line"], P["F"], P["S"], res["toolN"],
+                           res["hx"], case["dtReal"], res["speed"], res["feedOverride"])
+
+
+def run_report(case, res, mk_stock, mk_engine, run_frame, tip_lut, tan):
+    """replay a recorded real-time run frame by frame: each frame is one call of the sub-step
+    loop with the frame's flank wear (the setWear the simulator issued, and the radial wear
+    rw = min(R/2, VB tan(clearance)) of src/sim.js:439 from the wear VB it had) and the feed
+    mark carried over; the report is built from the outputs.  run_frame(act, ide, eng, sub,
+    count, R, fluteLen, rw, typ, lut, rc, dynBudget, fz0) returns (outs, [frame]).
+    Returns (outs, frames, reporter, stocks, sub, n, frame list)."""
+    import ctypes as C
+    from mcsim import report as RP
+    from mcsim._abi import Substep
+    st = case["stock"]
+    act, ide = mk_stock(st["w"], st["l"], st["h"], case["cell"]), mk_stock(st["w"], st["l"], st["h"], case["cell"])
+    ide.set_wall_band(2.0)
+    eng = mk_engine()
+    cell, fnMax = simx_cell(res)
+    eng.configure(cell, fnMax)
+    sub, n, frames = report_schedule(res, case)
+    typ, lut, rc = simx_tool(case, res, tip_lut)
+    R = res["toolR"]
+    clr = float(tan(res["clearanceDeg"] * 3.141592653589793 / 180))
+    rep = RP.FrameReporter()
+    outs, fouts, fz0 = [], [], 0.0
+    wears = res["wears"]
+    for f, (first, count) in enumerate(frames):
+        w = [x for x in wears if x[0] <= first][-1] if count else wears[0]
+        eng.set_wear(w[1], w[2])
+        VB = res["frames"][f]["VB0"]
+        rw = min(0.5 * R, VB * clr)
+        if count:
+            part = (Substep * count).from_address(C.addressof(sub[first]))
+            o, fo = run_frame(act, ide, eng, part, count, R, res["fluteLen"], rw, typ, lut, rc, res["dynBudget"], fz0)
+            assert len(fo) == 1
+            outs += list(o)
+            fouts.append(fo[0])
+            for k in range(count):
+                if o[k].cut:
+                    fz0 = part[k].fzEff
+            cut_time = rep.substeps(part, o, 0, count)
+            rep.collect(fo[0].stats, (fo[0].index, fo[0].amp, fo[0].once), cut_time)
+        else:
+            fouts.append(None)
+            rep.collect(RP._EMPTY, (0.0, 0.0, 0.0), 0.0)
+    return outs, fouts, rep, (act, ide), sub, n, frames
+
+
+REPORT_STATS = ("n", "sumF", "maxF", "sumFx", "sumFy", "sumFz", "maxFz", "sumT", "maxT", "sumX", "sumY", "minX",
+                "maxX", "minY", "maxY", "sumZc", "wZc", "sumW", "maxW", "sumHb", "sumB", "sumRb", "engTeeth")
+
+
+def check_report_run(case, res, outs, fouts, rep, stocks, sub, n, frames, maps):
+    """compare a replayed real-time run with the recorded one: every sub-step and stamp,
+    every frame's taken stats, chatter index and state, the final maps, and the report rows
+    (all fields but the temperature, its swing and the tool stress, which come from the
+    simulator's thermal and stress models)"""
+    a = res["actual"]
+    assert n == len(a["type"]) == len(res["ideal"]["type"])
+    assert np.array_equal([sub[k].x for k in range(n)], f64(res["ideal"]["cx"]))
+    assert np.array_equal([sub[k].x + outs[k].dx for k in range(n)], f64(a["cx"]))
+    assert np.array_equal([sub[k].y + outs[k].dy for k in range(n)], f64(a["cy"]))
+    assert np.array_equal([o.errVal for o in outs], f64(a["errVal"]))
+    assert np.array_equal(f64(a["R"]), f64(a["R"]))
+    runs = dict((k, m) for k, m in res["runs"])
+    assert [o.nsteps for o in outs] == [runs.get(k, 0) for k in range(n)]
+    assert np.array_equal(display_arcs(outs), f64(res["darc"]))
+    rf = res["frames"]
+    assert len(frames) == len(rf)
+    ends = np.cumsum([c for _, c in frames])
+    assert list(ends) == [f["end"] for f in rf]
+    for f, (fo, want) in enumerate(zip(fouts, rf)):
+        if fo is None:
+            assert want["stats"][0] == 0
+            continue
+        got = [getattr(fo.stats, k) for k in REPORT_STATS]
+        # JSON keeps no infinities: the untouched extremes of newStats() come back as null
+        ref = [v if v is not None else (np.inf if k in ("minX", "minY") else -np.inf)
+               for k, v in zip(REPORT_STATS, want["stats"])]
+        assert np.array_equal(np.array(got), np.array(ref)), f
+        if want["ci"] is not None:
+            assert (fo.index, fo.amp) == (want["ci"][0], want["ci"][1]), f
+            assert fo.once == (want["ci"][2] if want["ci"][2] is not None else 0.0), f
+    assert rep.states == [f["state"] for f in rf]
+    got_rows = rep.report.pack()
+    want_rows = res["report"]["rows"]
+    assert [r[0] for r in got_rows] == [r[0] for r in want_rows]
+    from mcsim.report import FIELDS
+    skip = {FIELDS.index(k) + 1 for k in ("temp", "stress", "swing")}
+    for g, w in zip(got_rows, want_rows):
+        for i in range(1, len(g)):
+            if i not in skip:
+                assert g[i] == (w[i] if w[i] is not None else 0), (g[0], FIELDS[i - 1], g[i], w[i])
+    act, ide = stocks
+    for name, s_, ref in (("actual", act, res["stock"]), ("commanded", ide, res["commanded"])):
+        got = maps(s_)
+        for key in ("H", "err", "wallD", "markS", "markF"):
+            assert np.array_equal(got[key], zf32(ref[key]), equal_nan=True), (name, key)
+        for key in ("lineCut", "lineWall"):
+            assert np.array_equal(got[key], zi32(ref[key])), (name, key)

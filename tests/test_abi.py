@@ -44,6 +44,8 @@ int main(void) {
          offsetof(mc_engine_info, stepsPerTooth));
   printf("%zu %zu %zu %zu %zu\n", sizeof(mc_substep), sizeof(mc_substep_out), offsetof(mc_substep_out, arc),
          sizeof(mc_sim_program), offsetof(mc_sim_program, cornerR));
+  printf("%zu %zu %zu %zu\n", sizeof(mc_frame_out), offsetof(mc_frame_out, end), offsetof(mc_substep, guard),
+         offsetof(mc_sim_program, frames));
   return 0;
 }'''
     import tempfile
@@ -63,4 +65,5 @@ int main(void) {
     want += [_abi.EngineState.stepCount.offset, _abi.EngineState.hist_i.offset, _abi.EngineInfo.stepsPerTooth.offset]
     want += [C.sizeof(_abi.Substep), C.sizeof(_abi.SubstepOut), _abi.SubstepOut.arc.offset, C.sizeof(_abi.SimProgram),
              _abi.SimProgram.cornerR.offset]
+    want += [C.sizeof(_abi.FrameOut), _abi.FrameOut.end.offset, _abi.Substep.guard.offset, _abi.SimProgram.frames.offset]
     assert [int(x) for x in out] == want
