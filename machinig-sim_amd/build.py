@@ -21,16 +21,22 @@ HIP_FLAGS = [
 SOURCES = ["chatter.hip", "search.hip", "stock.hip", "engine.hip", "simrun.hip", "mlp.hip", "lobes.cpp", "tiplut.cpp", "jsmath.cpp", "api.cpp"]
 
 
-def build(verbose=False):
-    objs = []
+def build(verbose=False, jobs=None):
+    """compile every source (in parallel, they are independent) and link libmcsim.so"""
+    from concurrent.futures import ThreadPoolExecutor
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd))
-        subprocess.run(cmd, check=True)
+        cmds.append([HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        for cmd, r in zip(cmds, ex.map(lambda c: subprocess.run(c, capture_output=not verbose), cmds)):
+            if verbose:
+                print(" ".join(cmd))
+            if r.returncode != 0:
+                sys.stderr.write((r.stderr or b"").decode(errors="replace"))
+                raise subprocess.CalledProcessError(r.returncode, cmd)
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs
     subprocess.run(cmd, check=True)
     for o in objs:

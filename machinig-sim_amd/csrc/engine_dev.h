@@ -88,17 +88,25 @@ __host__ __device__ __forceinline__ double jmin(double a, double b) {
 }
 __host__ __device__ __forceinline__ bool truthy(double v) { return v != 0 && v == v; }
 
+// a chunk of steps is staged in LDS: the edge angle with its sine and cosine of every
+// (step, visit), and the delay-line cells of its slice visits; GCAP entries at most
+constexpr int GCAP = 1024;
+
 // LDS of one engine's wavefront: the per-visit accumulations and the step sums
 struct EngLds {
   double term[T_COUNT][SLOT_CAP];
   int teng[SLOT_CAP];
   double res[8];
-  // the run's constants, staged from the engine record
+  // the run's constants, staged from the engine record: the slice tables, the edge force
+  // terms Kte b, Kre b (slices, then rings with ds), Kae b, the process-damping factor
+  // pdK (b / db) / r of each slice
   double sLag[MS], sKap[MS], sZ[MS], sB[MS], sR[MS];
+  double cTe[MS + ERINGS], cRe[MS + ERINGS], cAe[MS], cPd[MS];
   double dR[MC_MAX_FLUTES];
-  double modal[6 * MC_MAX_MODES];   // kx mx cx ky my cy
   double ring[4 * ERINGS];          // ringR ringCos ringZ ringDs
   float Hb[MC_BINS], Eb[EBINS * ERINGS];
+  double gphi[GCAP], gsn[GCAP], gcs[GCAP];
+  float win[GCAP];
 };
 
 // chatterIndex() (src/cutting.js:299-318) over the engine's rings, by one thread: the RMS
@@ -144,7 +152,15 @@ __host__ __device__ __forceinline__ void eng_new_stats(double* st) {
   st[S_MINY] = __builtin_inf(); st[S_MAXY] = -__builtin_inf();
 }
 
-// run(nSteps) of src/cutting.js:146-296 on engine E by the calling wavefront
+// run(nSteps) of src/cutting.js:146-296 on engine E by the calling wavefront.
+//
+// The steps run in chunks of at most one tooth period.  Before a chunk, every lane computes
+// its share of the chunk's edge angles (the slice angle (theta + i pitch - lag) mod 2 pi and
+// the end-edge angle, with their fdlibm sine and cosine) and loads the delay-line cells the
+// chunk's slice visits read: within one tooth period no cell is visited twice, so the
+// chunk reads and writes them in LDS and stores them back after it.  A step's own chain is
+// then the chip thickness, the force law, the serial sums, the modes (one lane per mode,
+// summed in mode order) and the statistics (one lane per field).
 __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& L) {
   auto& term = L.term;
   auto& teng = L.teng;
@@ -163,13 +179,18 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
   const int per = nz + (ends ? ERINGS : 0);
   const int NS = N * per;
   const bool mcOn = truthy(mc), pdOn = truthy(pdK), wallOn = truthy(wk);
+  // the size effect's pow: the branch-free form where it equals pow_js (x = H_REF / h lies in
+  // [H_REF / hMax, 25] and the exponent in pow_cell's domain), else pow_js itself
+  const bool cellPow = mcOn && mcm::pow_cell_ok(mc) && hMax <= 1e27;
+  const double my1 = mcm::clear_lo(mc), myr = mc - my1;
   double runMaxF = 0;
   float* const surf = E.surf;
 
-  // the run's constants into LDS: the engagement's slice tables, the maps, the tooth
-  // offsets, the end rings and the modal coefficients
   for (int k = lane; k < nz; k += 64) {
-    L.sLag[k] = E.sLag[k]; L.sKap[k] = E.sKap[k]; L.sZ[k] = E.sZ[k]; L.sB[k] = E.sB[k]; L.sR[k] = E.sR[k];
+    const double bk = E.sB[k], sr = E.sR[k];
+    L.sLag[k] = E.sLag[k]; L.sKap[k] = E.sKap[k]; L.sZ[k] = E.sZ[k]; L.sB[k] = bk; L.sR[k] = sr;
+    L.cTe[k] = Kte * bk; L.cRe[k] = Kre * bk; L.cAe[k] = Kae * bk;
+    L.cPd[k] = pdK * (bk / db) / sr;
   }
   for (int b = lane; b < MC_BINS; b += 64) L.Hb[b] = E.Hb[b];
   for (int b = lane; b < EBINS * ERINGS; b += 64) L.Eb[b] = E.Eb[b];
@@ -177,33 +198,23 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
   if (lane < ERINGS) {
     L.ring[lane] = E.ringR[lane]; L.ring[ERINGS + lane] = E.ringCos[lane];
     L.ring[2 * ERINGS + lane] = E.ringZ[lane]; L.ring[3 * ERINGS + lane] = E.ringDs[lane];
+    L.cTe[MS + lane] = Kte * E.ringDs[lane]; L.cRe[MS + lane] = Kre * E.ringDs[lane];
   }
-  if (lane < MC_MAX_MODES) {
-    L.modal[lane] = E.kx[lane]; L.modal[MC_MAX_MODES + lane] = E.mx[lane];
-    L.modal[2 * MC_MAX_MODES + lane] = E.cx[lane]; L.modal[3 * MC_MAX_MODES + lane] = E.ky[lane];
-    L.modal[4 * MC_MAX_MODES + lane] = E.my[lane]; L.modal[5 * MC_MAX_MODES + lane] = E.cy[lane];
-  }
-  // the delay-line cell a lane's first visit reads at its step, loaded a step ahead: the
-  // cell read at step s + 1 was written at least a tooth period before, never at step s
-  const int k0 = lane < NS ? lane % per : 0, i0s = lane < NS ? lane / per : 0;
-  const bool pre0 = lane < NS && k0 < nz;
-  auto cell_of = [&](int prr) {
-    int pi = prr + i0s * spt;
-    if (pi >= spr) pi -= spr;
-    return surf + (size_t)pi * MSL + k0;
-  };
-  float cur0 = pre0 ? *cell_of(E.pr) : 0.0f;
-  wave_sync();
 
-  // modal state, redundantly on every lane; a state set from outside enters mode 0
-  double qx[MC_MAX_MODES], vxm[MC_MAX_MODES], qy[MC_MAX_MODES], vym[MC_MAX_MODES];
-#pragma unroll
-  for (int j = 0; j < MC_MAX_MODES; j++) { qx[j] = E.qx[j]; vxm[j] = E.vxm[j]; qy[j] = E.qy[j]; vym[j] = E.vym[j]; }
-  qx[0] += E.x - E.x0; qy[0] += E.y - E.y0;
-  vxm[0] += E.vx - E.vx0; vym[0] += E.vy - E.vy0;
+  // modes: lane j < nmx integrates x-mode j, lane MC_MAX_MODES + j y-mode j; a state set from
+  // outside since the last run enters the first mode (src/cutting.js:164)
+  const bool xmode = lane < nmx, ymode = lane >= MC_MAX_MODES && lane < MC_MAX_MODES + nmy;
+  const int mj = lane < MC_MAX_MODES ? lane : (lane - MC_MAX_MODES) & (MC_MAX_MODES - 1);
+  double mk = 0, mmass = 1, mcd = 0, mq = 0, mv = 0;
+  if (xmode) { mk = E.kx[mj]; mmass = E.mx[mj]; mcd = E.cx[mj]; mq = E.qx[mj]; mv = E.vxm[mj]; }
+  if (ymode) { mk = E.ky[mj]; mmass = E.my[mj]; mcd = E.cy[mj]; mq = E.qy[mj]; mv = E.vym[mj]; }
+  if (lane == 0) { mq += E.x - E.x0; mv += E.vx - E.vx0; }
+  if (lane == MC_MAX_MODES) { mq += E.y - E.y0; mv += E.vy - E.vy0; }
   double x = E.x, y = E.y, vxs = E.vx, vys = E.vy, t = E.t, w = E.w, wv = E.wv, wSum = 0;
   int pr = E.pr;
+  int ph = pr % spt, tq = pr / spt;   // pr % stepsPerTooth and pr / stepsPerTooth, kept as counters
   int64_t stepCount = E.stepCount;
+  int dleft = decim - (int)(stepCount % decim);   // steps until ++stepCount % decim == 0
   // chain lanes 6..8 carry the engagement statistics across steps; tooth lanes their tooth
   const int tooth = lane - 16;
   const bool isTooth = tooth >= 0 && tooth < N;
@@ -211,246 +222,284 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
   double tvol = isTooth ? E.toothVol[tooth] : 0.0, tphi = isTooth ? E.toothPhi[tooth] : 0.0,
          tz = isTooth ? E.toothZ[tooth] : 0.0;
   int tEng = isTooth ? E.toothEngaged[tooth] : 0;
-  // lane 0: the step statistics, rings and events
-  double st[S_COUNT];
-#pragma unroll
-  for (int k = 0; k < S_COUNT; k++) st[k] = E.st[k];
+  // the step statistics: lane f keeps field f (the three engagement sums are the chain
+  // lanes'); each lane adds, maximises or minimises its own operand of the step
+  enum { OP_NONE, OP_ADD, OP_MAX, OP_MIN };
+  int op = OP_NONE, src = 0, cond = 0;   // src: operand; cond 1: Fsum > 0, 2: the wall mode is on
+  switch (lane) {
+    case S_N: op = OP_ADD; src = 0; break;
+    case S_SUMF: op = OP_ADD; src = 1; break;
+    case S_MAXF: op = OP_MAX; src = 1; break;
+    case S_SUMFX: op = OP_ADD; src = 2; break;
+    case S_SUMFY: op = OP_ADD; src = 3; break;
+    case S_SUMFZ: op = OP_ADD; src = 4; break;
+    case S_MAXFZ: op = OP_MAX; src = 5; break;
+    case S_SUMT: op = OP_ADD; src = 6; break;
+    case S_MAXT: op = OP_MAX; src = 6; break;
+    case S_SUMX: op = OP_ADD; src = 7; break;
+    case S_SUMY: op = OP_ADD; src = 8; break;
+    case S_MINX: op = OP_MIN; src = 7; break;
+    case S_MAXX: op = OP_MAX; src = 7; break;
+    case S_MINY: op = OP_MIN; src = 8; break;
+    case S_MAXY: op = OP_MAX; src = 8; break;
+    case S_SUMZC: op = OP_ADD; src = 9; cond = 1; break;
+    case S_WZC: op = OP_ADD; src = 1; cond = 1; break;
+    case S_SUMW: op = OP_ADD; src = 10; cond = 2; break;
+    case S_MAXW: op = OP_MAX; src = 11; cond = 2; break;
+    case S_ENGT: op = OP_ADD; src = 12; break;
+    default: break;
+  }
+  if (cond == 2 && !wallOn) op = OP_NONE;
+  double stv = op != OP_NONE ? E.st[lane] : 0.0;
   int64_t hist_i = E.hist_i, hist_count = E.hist_count, ts_i = E.ts_i, ts_count = E.ts_count;
   int64_t ws_i = E.ws_i, ws_count = E.ws_count, wsp_i = E.wsp_i, wsp_count = E.wsp_count;
   int64_t nev = 0;
   const int64_t ev_cap = E.ev_cap;
+  // steps per chunk: at most a tooth period (no delay-line cell twice) and GCAP entries
+  int chMax = spt < 64 ? spt : 64;
+  if (NS > 0 && GCAP / NS < chMax) chMax = GCAP / NS;
+  wave_sync();
 
-  for (int64_t s = 0; s < nSteps; s++) {
-    const double theta = pr * dphi;
-    const double rx = x - w * wnx, ry = y - w * wny;
-    const double xm = rx * 1e3, ym = ry * 1e3;
-    const double rvx = vxs - wv * wnx, rvy = vys - wv * wny;
-    const int prn = pr + 1 == spr ? 0 : pr + 1;
-    const float nxt0 = pre0 ? *cell_of(prn) : 0.0f;
-    // ---- visits (src/cutting.js:172-237)
-    for (int slot = lane; slot < NS; slot += 64) {
+  for (int64_t s0 = 0; s0 < nSteps; s0 += chMax) {
+    const int ch = nSteps - s0 < chMax ? (int)(nSteps - s0) : chMax;
+    const int prc = pr;
+    // ---- the chunk's edge angles and delay-line cells
+    for (int e = lane; e < ch * NS; e += 64) {
+      const int sc = e / NS, slot = e - sc * NS;
       const int i = slot / per, r = slot - i * per;
-      double a[T_COUNT];
-#pragma unroll
-      for (int q = 0; q < T_COUNT; q++) a[q] = 0.0;
-      int eng = 0;
+      int prs = prc + sc;
+      if (prs >= spr) prs -= spr;
+      const double theta = prs * dphi;
+      double phi = r < nz ? mcm::fmod_2pi((theta + i * pitch) - L.sLag[r]) : mcm::fmod_2pi(theta + i * pitch);
+      if (phi < 0) phi += TWO_PI;
+      double sn, cs;
+      mcm::sincos_js(phi, sn, cs);
+      L.gphi[e] = phi; L.gsn[e] = sn; L.gcs[e] = cs;
       if (r < nz) {
-        const int k = r;
-        double phi = mcm::fmod_2pi((theta + i * pitch) - L.sLag[k]);
-        if (phi < 0) phi += TWO_PI;
-        double sn, cs;
-        mcm::sincos_js(phi, sn, cs);
-        const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
-        const double rn = xm * nxg + ym * nyg + L.dR[i];
-        int pi = pr + i * spt;
+        int pi = prs + i * spt;
         if (pi >= spr) pi -= spr;
-        float* const cell = surf + (size_t)pi * MSL + k;
-        const double prev = slot == lane ? (double)cur0 : (double)*cell;
-        const double feedH = fz * sn * L.sKap[k];
-        const int bin = (int)(phi * binScale);
-        const double hb = bin < MC_BINS ? (double)L.Hb[bin] : __builtin_nan("");
-        if (hb <= L.sZ[k] || sn <= 0) {
-          *cell = (float)rn;
-        } else {
-          double h = feedH + rn - prev;
-          if (h <= 0) {
-            *cell = (float)(prev - feedH);
+        L.win[e] = surf[(size_t)pi * MSL + r];
+      }
+    }
+    wave_sync();
+    for (int sc = 0; sc < ch; sc++) {
+      const double rx = x - w * wnx, ry = y - w * wny;
+      const double xm = rx * 1e3, ym = ry * 1e3;
+      const double rvx = vxs - wv * wnx, rvy = vys - wv * wny;
+      // ---- visits (src/cutting.js:172-237)
+      for (int slot = lane; slot < NS; slot += 64) {
+        const int e = sc * NS + slot;
+        const int i = slot / per, r = slot - i * per;
+        double a[T_COUNT];
+#pragma unroll
+        for (int q = 0; q < T_COUNT; q++) a[q] = 0.0;
+        int eng = 0;
+        const double phi = L.gphi[e], sn = L.gsn[e], cs = L.gcs[e];
+        if (r < nz) {
+          const int k = r;
+          const double nxg = sn * ux + cs * vx, nyg = sn * uy + cs * vy;
+          const double rn = xm * nxg + ym * nyg + L.dR[i];
+          const double prev = (double)L.win[e];
+          const double feedH = fz * sn * L.sKap[k];
+          const int bin = (int)(phi * binScale);
+          const double hb = bin < MC_BINS ? (double)L.Hb[bin] : __builtin_nan("");
+          if (hb <= L.sZ[k] || sn <= 0) {
+            L.win[e] = (float)rn;
           } else {
-            *cell = (float)rn;
-            if (h > hMax) h = hMax;
-            const double se = mcOn ? mcm::pow_js(H_REF / (h > H_MIN ? h : H_MIN), mc) : 1.0;
-            const double bk = L.sB[k], sr = L.sR[k], sz = L.sZ[k];
-            const double Ft = Ktc * se * h * bk + Kte * bk;
-            double Fr = Krc * se * h * bk + Kre * bk;
-            const double Fa = Kac * se * h * bk + Kae * bk;
-            if (pdOn) Fr += pdK * (bk / db) / sr * (rvx * nxg + rvy * nyg);
+            double h = feedH + rn - prev;
+            if (h <= 0) {
+              L.win[e] = (float)(prev - feedH);
+            } else {
+              L.win[e] = (float)rn;
+              if (h > hMax) h = hMax;
+              const double xs = H_REF / (h > H_MIN ? h : H_MIN);
+              const double se = !mcOn ? 1.0 : cellPow ? mcm::pow_cell(xs, mc, my1, myr) : mcm::pow_js(xs, mc);
+              const double bk = L.sB[k], sr = L.sR[k], sz = L.sZ[k];
+              const double Ft = Ktc * se * h * bk + L.cTe[k];
+              double Fr = Krc * se * h * bk + L.cRe[k];
+              const double Fa = Kac * se * h * bk + L.cAe[k];
+              if (pdOn) Fr += L.cPd[k] * (rvx * nxg + rvy * nyg);
+              const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
+              a[T_FX] = -Ft * txg - Fr * nxg;
+              a[T_FY] = -Ft * tyg - Fr * nyg;
+              a[T_FZ] = -Fa;
+              a[T_TQ] = Ft * sr;
+              const double fr = mcm::hypot_js(Ft, Fr);
+              a[T_FS] = fr;
+              a[T_FR2Z] = fr * (sz - tipZ);
+              a[T_HB] = h * bk;
+              a[T_B] = bk;
+              a[T_RB] = sr * h * bk;
+              a[T_VOL] = h * bk * sr * dphi;
+              a[T_PHI] = phi;
+              a[T_Z] = sz;
+              eng = 1;
+            }
+          }
+        } else {
+          const int j = r - nz;
+          const int rb = (int)(phi / TWO_PI * EBINS);
+          const double depth = rb < EBINS ? (double)L.Eb[rb * ERINGS + j] : __builtin_nan("");
+          if (!(depth <= 2e-4)) {
             const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-            a[T_FX] = -Ft * txg - Fr * nxg;
-            a[T_FY] = -Ft * tyg - Fr * nyg;
-            a[T_FZ] = -Fa;
-            a[T_TQ] = Ft * sr;
-            const double fr = mcm::hypot_js(Ft, Fr);
-            a[T_FS] = fr;
-            a[T_FR2Z] = fr * (sz - tipZ);
-            a[T_HB] = h * bk;
-            a[T_B] = bk;
-            a[T_RB] = sr * h * bk;
-            a[T_VOL] = h * bk * sr * dphi;
+            const double c = L.ring[ERINGS + j], ds = L.ring[3 * ERINGS + j], rr = L.ring[j], rz = L.ring[2 * ERINGS + j];
+            const double h = (depth < fza ? depth : fza) * c;
+            const double xs = H_REF / (h > H_MIN ? h : H_MIN);
+            const double se = !mcOn ? 1.0 : cellPow ? mcm::pow_cell(xs, mc, my1, myr) : mcm::pow_js(xs, mc);
+            const double Ft = Ktc * se * h * ds + L.cTe[MS + j];
+            const double Fn = Krc * se * h * ds + L.cRe[MS + j];
+            a[T_FX] = -Ft * txg;
+            a[T_FY] = -Ft * tyg;
+            a[T_FZ] = -(Fn * c);
+            a[T_TQ] = Ft * rr;
+            a[T_FS] = Ft;
+            a[T_FR2Z] = Ft * rz;
+            a[T_HB] = h * ds;
+            a[T_B] = ds;
+            a[T_RB] = rr * h * ds;
+            a[T_VOL] = h * ds * rr * dphi;
             a[T_PHI] = phi;
-            a[T_Z] = sz;
+            a[T_Z] = tipZ + rz;
             eng = 1;
           }
         }
-      } else {
-        const int j = r - nz;
-        double phi = mcm::fmod_2pi(theta + i * pitch);
-        if (phi < 0) phi += TWO_PI;
-        const int rb = (int)(phi / TWO_PI * EBINS);
-        const double depth = rb < EBINS ? (double)L.Eb[rb * ERINGS + j] : __builtin_nan("");
-        if (!(depth <= 2e-4)) {
-          double sn, cs;
-          mcm::sincos_js(phi, sn, cs);
-          const double txg = cs * ux - sn * vx, tyg = cs * uy - sn * vy;
-          const double c = L.ring[ERINGS + j], ds = L.ring[3 * ERINGS + j], rr = L.ring[j], rz = L.ring[2 * ERINGS + j];
-          const double h = (depth < fza ? depth : fza) * c;
-          const double se = mcOn ? mcm::pow_js(H_REF / (h > H_MIN ? h : H_MIN), mc) : 1.0;
-          const double Ft = Ktc * se * h * ds + Kte * ds;
-          const double Fn = Krc * se * h * ds + Kre * ds;
-          a[T_FX] = -Ft * txg;
-          a[T_FY] = -Ft * tyg;
-          a[T_FZ] = -(Fn * c);
-          a[T_TQ] = Ft * rr;
-          a[T_FS] = Ft;
-          a[T_FR2Z] = Ft * rz;
-          a[T_HB] = h * ds;
-          a[T_B] = ds;
-          a[T_RB] = rr * h * ds;
-          a[T_VOL] = h * ds * rr * dphi;
-          a[T_PHI] = phi;
-          a[T_Z] = tipZ + rz;
-          eng = 1;
+#pragma unroll
+        for (int q = 0; q < T_COUNT; q++) term[q][slot] = a[q];
+        teng[slot] = eng;
+      }
+      wave_sync();
+      // ---- serial sums in the reference's visit order
+      if (lane < 9) {
+        double acc = lane < 6 ? 0.0 : chain;
+        for (int q = 0; q < NS; q++) acc += term[lane][q];
+        if (lane < 6) res[lane] = acc; else chain = acc;
+      }
+      bool push = false;
+      double pvol = 0, pphi = 0, pz = 0;
+      bool engagedNow = false;
+      if (isTooth) {
+        int last = -1;
+        for (int r = 0; r < per; r++) {
+          const int q = tooth * per + r;
+          tvol += term[T_VOL][q];
+          if (teng[q]) last = q;
+        }
+        engagedNow = last >= 0;
+        if (engagedNow) {
+          tphi = term[T_PHI][last];
+          tz = term[T_Z][last];
+        } else if (tEng) {
+          if (tvol > 0) { push = true; pvol = tvol; pphi = tphi; pz = tz; }
+          tvol = 0;
+        }
+        tEng = engagedNow ? 1 : 0;
+      }
+      // chip events in tooth order (src/cutting.js:238-241)
+      const unsigned long long pmask = __ballot(push);
+      const unsigned long long emask = __ballot(engagedNow);
+      if (push) {
+        const int64_t at = nev + __popcll(pmask & ((1ull << lane) - 1));
+        if (at < ev_cap) {
+          double* q = E.ev + 3 * at;
+          q[0] = pvol; q[1] = pphi; q[2] = pz;
         }
       }
-#pragma unroll
-      for (int q = 0; q < T_COUNT; q++) term[q][slot] = a[q];
-      teng[slot] = eng;
+      nev += __popcll(pmask);
+      wave_sync();
+      const double Fx = res[T_FX], Fy = res[T_FY], Fz = res[T_FZ], Tq = res[T_TQ], Fsum = res[T_FS],
+                   Fr2z = res[T_FR2Z];
+      // ---- modal integration, semi-implicit Euler (src/cutting.js:246-255): each mode on its
+      // lane, the displacements and velocities summed in mode order
+      if (xmode || ymode) {
+        mv += ((lane < MC_MAX_MODES ? Fx : Fy) - mcd * mv - mk * mq) / mmass * dt;
+        mq += mv * dt;
+      }
+      x = 0; y = 0; vxs = 0; vys = 0;
+      for (int j = 0; j < nmx; j++) { x += __shfl(mq, j, 64); vxs += __shfl(mv, j, 64); }
+      for (int j = 0; j < nmy; j++) { y += __shfl(mq, MC_MAX_MODES + j, 64); vys += __shfl(mv, MC_MAX_MODES + j, 64); }
+      // ---- wall mode (src/cutting.js:256-264)
+      if (wallOn) {
+        const double Fw = -(Fx * wnx + Fy * wny);
+        wv += (Fw - wc * wv - wk * w) / wm * dt;
+        w += wv * dt;
+        if (w > 5e-4) { w = 5e-4; if (wv > 0) wv = 0; }
+        else if (w < -2e-4) { w = -2e-4; if (wv < 0) wv = 0; }
+        wSum += w;
+      }
+      if (++pr == spr) pr = 0;
+      if (++ph == spt) { ph = 0; if (++tq == N) tq = 0; }
+      if (ph == 0) {
+        // the tooth now at phi = 0 generates the +v wall (src/cutting.js:265-272)
+        const double d0 = L.dR[(N - tq) % N] * 1e-3;
+        if (lane == 0) {
+          E.rings[ts_i] = (float)(x - w * wnx);
+          E.rings[RN + ts_i] = (float)(y - w * wny);
+          E.rings[2 * RN + ws_i] = (float)(x - w * wnx + d0 * vx);
+          E.rings[3 * RN + ws_i] = (float)(y - w * wny + d0 * vy);
+        }
+        ts_i = (ts_i + 1) % RN; ts_count++;
+        ws_i = (ws_i + 1) % RN; ws_count++;
+      }
+      if (ph == piStep) {
+        const long long qn = (long long)mcm::round_js((JS_PI - pr * dphi) / pitch);
+        const int ip = (int)(((qn % N) + N) % N);
+        const double dp = L.dR[ip] * 1e-3;
+        if (lane == 0) {
+          E.rings[4 * RN + wsp_i] = (float)(x - w * wnx - dp * vx);
+          E.rings[5 * RN + wsp_i] = (float)(y - w * wny - dp * vy);
+        }
+        wsp_i = (wsp_i + 1) % RN; wsp_count++;
+      }
+      t += dt;
+      // ---- step statistics (src/cutting.js:279-287), lane f updating field f
+      const double Fxy = mcm::hypot_js(Fx, Fy);
+      runMaxF = jmax(runMaxF, Fxy);
+      if (op != OP_NONE && (cond != 1 || Fsum > 0)) {
+        const double o = src == 0 ? 1.0 : src == 1 ? Fxy : src == 2 ? Fx : src == 3 ? Fy : src == 4 ? Fz
+                       : src == 5 ? -Fz : src == 6 ? Tq : src == 7 ? x : src == 8 ? y
+                       : src == 9 ? Fr2z / Fsum * Fxy : src == 10 ? w : src == 11 ? __builtin_fabs(w)
+                       : (double)__popcll(emask);
+        stv = op == OP_ADD ? stv + o : op == OP_MAX ? jmax(stv, o) : jmin(stv, o);
+      }
+      stepCount++;
+      if (--dleft == 0) {
+        dleft = decim;
+        if (lane == 0) {
+          E.hist[hist_i] = (float)Fx;
+          E.hist[HN + hist_i] = (float)Fy;
+          E.hist[2 * HN + hist_i] = (float)Fz;
+          E.hist[3 * HN + hist_i] = (float)((x - w * wnx) * 1e6);
+          E.hist[4 * HN + hist_i] = (float)((y - w * wny) * 1e6);
+        }
+        hist_i = (hist_i + 1) % HN; hist_count++;
+      }
     }
     wave_sync();
-    // ---- serial sums in the reference's visit order
-    if (lane < 9) {
-      double acc = lane < 6 ? 0.0 : chain;
-      for (int q = 0; q < NS; q++) acc += term[lane][q];
-      if (lane < 6) res[lane] = acc; else chain = acc;
-    }
-    bool push = false;
-    double pvol = 0, pphi = 0, pz = 0;
-    bool engagedNow = false;
-    if (isTooth) {
-      int last = -1;
-      for (int r = 0; r < per; r++) {
-        const int q = tooth * per + r;
-        tvol += term[T_VOL][q];
-        if (teng[q]) last = q;
-      }
-      engagedNow = last >= 0;
-      if (engagedNow) {
-        tphi = term[T_PHI][last];
-        tz = term[T_Z][last];
-      } else if (tEng) {
-        if (tvol > 0) { push = true; pvol = tvol; pphi = tphi; pz = tz; }
-        tvol = 0;
-      }
-      tEng = engagedNow ? 1 : 0;
-    }
-    // chip events in tooth order (src/cutting.js:238-241)
-    const unsigned long long pmask = __ballot(push);
-    const unsigned long long emask = __ballot(engagedNow);
-    if (push) {
-      const int64_t at = nev + __popcll(pmask & ((1ull << lane) - 1));
-      if (at < ev_cap) {
-        double* q = E.ev + 3 * at;
-        q[0] = pvol; q[1] = pphi; q[2] = pz;
+    // ---- the chunk's delay-line cells back to the engine's record
+    for (int e = lane; e < ch * NS; e += 64) {
+      const int sc = e / NS, slot = e - sc * NS;
+      const int i = slot / per, r = slot - i * per;
+      if (r < nz) {
+        int prs = prc + sc;
+        if (prs >= spr) prs -= spr;
+        int pi = prs + i * spt;
+        if (pi >= spr) pi -= spr;
+        surf[(size_t)pi * MSL + r] = L.win[e];
       }
     }
-    nev += __popcll(pmask);
     wave_sync();
-    const double Fx = res[T_FX], Fy = res[T_FY], Fz = res[T_FZ], Tq = res[T_TQ], Fsum = res[T_FS],
-                 Fr2z = res[T_FR2Z];
-    // ---- modal integration, semi-implicit Euler (src/cutting.js:246-255)
-    x = 0; y = 0; vxs = 0; vys = 0;
-#pragma unroll
-    for (int j = 0; j < MC_MAX_MODES; j++) {
-      if (j < nmx) {
-        vxm[j] += (Fx - L.modal[2 * MC_MAX_MODES + j] * vxm[j] - L.modal[j] * qx[j]) / L.modal[MC_MAX_MODES + j] * dt;
-        qx[j] += vxm[j] * dt;
-        x += qx[j];
-        vxs += vxm[j];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < MC_MAX_MODES; j++) {
-      if (j < nmy) {
-        vym[j] += (Fy - L.modal[5 * MC_MAX_MODES + j] * vym[j] - L.modal[3 * MC_MAX_MODES + j] * qy[j]) /
-                  L.modal[4 * MC_MAX_MODES + j] * dt;
-        qy[j] += vym[j] * dt;
-        y += qy[j];
-        vys += vym[j];
-      }
-    }
-    // ---- wall mode (src/cutting.js:256-264)
-    if (wallOn) {
-      const double Fw = -(Fx * wnx + Fy * wny);
-      wv += (Fw - wc * wv - wk * w) / wm * dt;
-      w += wv * dt;
-      if (w > 5e-4) { w = 5e-4; if (wv > 0) wv = 0; }
-      else if (w < -2e-4) { w = -2e-4; if (wv < 0) wv = 0; }
-      st[S_SUMW] += w;
-      st[S_MAXW] = jmax(st[S_MAXW], __builtin_fabs(w));
-      wSum += w;
-    }
-    if (++pr == spr) pr = 0;
-    cur0 = nxt0;
-    const int ph = pr % spt;
-    if (ph == 0) {
-      // the tooth now at phi = 0 generates the +v wall (src/cutting.js:265-272)
-      const double d0 = L.dR[(N - (pr / spt) % N) % N] * 1e-3;
-      if (lane == 0) {
-        E.rings[ts_i] = (float)(x - w * wnx);
-        E.rings[RN + ts_i] = (float)(y - w * wny);
-        E.rings[2 * RN + ws_i] = (float)(x - w * wnx + d0 * vx);
-        E.rings[3 * RN + ws_i] = (float)(y - w * wny + d0 * vy);
-      }
-      ts_i = (ts_i + 1) % RN; ts_count++;
-      ws_i = (ws_i + 1) % RN; ws_count++;
-    }
-    if (ph == piStep) {
-      const long long qn = (long long)mcm::round_js((JS_PI - pr * dphi) / pitch);
-      const int ip = (int)(((qn % N) + N) % N);
-      const double dp = L.dR[ip] * 1e-3;
-      if (lane == 0) {
-        E.rings[4 * RN + wsp_i] = (float)(x - w * wnx - dp * vx);
-        E.rings[5 * RN + wsp_i] = (float)(y - w * wny - dp * vy);
-      }
-      wsp_i = (wsp_i + 1) % RN; wsp_count++;
-    }
-    t += dt;
-    // ---- step statistics (src/cutting.js:279-287)
-    const double Fxy = mcm::hypot_js(Fx, Fy);
-    runMaxF = jmax(runMaxF, Fxy);
-    st[S_N] += 1;
-    st[S_SUMF] += Fxy; st[S_MAXF] = jmax(st[S_MAXF], Fxy);
-    st[S_SUMFX] += Fx; st[S_SUMFY] += Fy; st[S_SUMFZ] += Fz; st[S_MAXFZ] = jmax(st[S_MAXFZ], -Fz);
-    st[S_SUMT] += Tq; st[S_MAXT] = jmax(st[S_MAXT], Tq);
-    st[S_SUMX] += x; st[S_SUMY] += y;
-    st[S_MINX] = jmin(st[S_MINX], x); st[S_MAXX] = jmax(st[S_MAXX], x);
-    st[S_MINY] = jmin(st[S_MINY], y); st[S_MAXY] = jmax(st[S_MAXY], y);
-    if (Fsum > 0) { st[S_SUMZC] += Fr2z / Fsum * Fxy; st[S_WZC] += Fxy; }
-    st[S_ENGT] += (double)__popcll(emask);
-    if (++stepCount % decim == 0) {
-      if (lane == 0) {
-        E.hist[hist_i] = (float)Fx;
-        E.hist[HN + hist_i] = (float)Fy;
-        E.hist[2 * HN + hist_i] = (float)Fz;
-        E.hist[3 * HN + hist_i] = (float)((x - w * wnx) * 1e6);
-        E.hist[4 * HN + hist_i] = (float)((y - w * wny) * 1e6);
-      }
-      hist_i = (hist_i + 1) % HN; hist_count++;
-    }
   }
-  wave_sync();
   // ---- write the state back
   if (lane == 6) E.st[S_SUMHB] = chain;
   if (lane == 7) E.st[S_SUMB] = chain;
   if (lane == 8) E.st[S_SUMRB] = chain;
+  if (op != OP_NONE) E.st[lane] = stv;
   if (isTooth) {
     E.toothVol[tooth] = tvol; E.toothPhi[tooth] = tphi; E.toothZ[tooth] = tz; E.toothEngaged[tooth] = tEng;
   }
+  if (xmode) { E.qx[mj] = mq; E.vxm[mj] = mv; }
+  if (ymode) { E.qy[mj] = mq; E.vym[mj] = mv; }
   if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < S_COUNT; k++)
-      if (k != S_SUMHB && k != S_SUMB && k != S_SUMRB) E.st[k] = st[k];
-#pragma unroll
-    for (int j = 0; j < MC_MAX_MODES; j++) { E.qx[j] = qx[j]; E.vxm[j] = vxm[j]; E.qy[j] = qy[j]; E.vym[j] = vym[j]; }
     E.x = E.x0 = x; E.y = E.y0 = y; E.vx = E.vx0 = vxs; E.vy = E.vy0 = vys;
     E.t = t; E.pr = pr; E.w = w; E.wv = wv;
     E.runMeanW = nSteps ? wSum / (double)nSteps : 0.0;
