@@ -626,7 +626,7 @@ struct StockImpl {
   int32_t* rect = nullptr;   // Stock.dirty {i0, j0, i1, j1} on the device
   float* bmax = nullptr;     // block maxima, valid while !bmax_stale
   bool bmax_stale = true;
-  DevBuf stamps, opt, sdev, sx, cull, box, touched, vol, xy, smp, lut, lutmeta, q, qout, eq, ehb, eout;
+  DevBuf stamps, opt, sdev, sx, cull, box, touched, vol, xy, smp, lut, lutmeta, q, qout, eq, ehb, eout, strip;
 };
 
 }  // namespace
@@ -658,6 +658,13 @@ int mch::stock_view(mc_stock* s, StockView* v) {
   v->H = s->H; v->err = s->err; v->wallD = s->wallD; v->markS = s->markS; v->markF = s->markF;
   v->lineCut = s->lineCut; v->lineWall = s->lineWall; v->rect = s->rect;
   s->bmax_stale = true;
+  return MC_OK;
+}
+
+int mch::stock_strips(mc_stock* s, float** smax, int* snx, int* sny) {
+  const int nxs = (s->nx + MCH_STRIP_W - 1) / MCH_STRIP_W, nys = (s->ny + MCH_STRIP_H - 1) / MCH_STRIP_H;
+  if (int st = s->strip.ensure((size_t)nxs * nys * 4)) return st;
+  *smax = (float*)s->strip.p; *snx = nxs; *sny = nys;
   return MC_OK;
 }
 
@@ -992,7 +999,7 @@ int mc_stock_destroy(mc_stock* s) {
                   (void*)s->markS, (void*)s->markF, (void*)s->rect, (void*)s->bmax})
     if (p) (void)hipFree(p);
   for (DevBuf* b : {&s->stamps, &s->opt, &s->sdev, &s->sx, &s->cull, &s->box, &s->touched, &s->vol, &s->xy,
-                    &s->smp, &s->lut, &s->lutmeta, &s->q, &s->qout, &s->eq, &s->ehb, &s->eout})
+                    &s->smp, &s->lut, &s->lutmeta, &s->q, &s->qout, &s->eq, &s->ehb, &s->eout, &s->strip})
     if (b->p) (void)hipFree(b->p);
   delete s;
   return MC_OK;

@@ -109,6 +109,14 @@ struct EngLds {
   float win[GCAP];
 };
 
+// a double of lane `src`, broadcast to every lane
+__device__ __forceinline__ double read_lane(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // chatterIndex() (src/cutting.js:299-318) over the engine's rings, by one thread: the RMS
 // scatter of the last n <= 24 once-per-tooth samples about their mean, over the RMS of the
 // last min(count, round(n spt / decim)) history samples about theirs, both newest first
@@ -368,10 +376,19 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
         teng[slot] = eng;
       }
       wave_sync();
-      // ---- serial sums in the reference's visit order
+      // ---- serial sums in the reference's visit order, eight terms loaded at a time (the
+      // sums start at +0 or at a positive chain, so they are never -0 and the +0 padding past
+      // the last visit leaves them unchanged)
       if (lane < 9) {
         double acc = lane < 6 ? 0.0 : chain;
-        for (int q = 0; q < NS; q++) acc += term[lane][q];
+        const double* tl = term[lane];
+        for (int q0 = 0; q0 < NS; q0 += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = q0 + u < NS ? tl[q0 + u] : 0.0;
+#pragma unroll
+          for (int u = 0; u < 8; u++) acc += v[u];
+        }
         if (lane < 6) res[lane] = acc; else chain = acc;
       }
       bool push = false;
@@ -414,9 +431,20 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
         mv += ((lane < MC_MAX_MODES ? Fx : Fy) - mcd * mv - mk * mq) / mmass * dt;
         mq += mv * dt;
       }
-      x = 0; y = 0; vxs = 0; vys = 0;
-      for (int j = 0; j < nmx; j++) { x += __shfl(mq, j, 64); vxs += __shfl(mv, j, 64); }
-      for (int j = 0; j < nmy; j++) { y += __shfl(mq, MC_MAX_MODES + j, 64); vys += __shfl(mv, MC_MAX_MODES + j, 64); }
+      {
+        // every mode's displacement and velocity read from its lane, then summed from +0 in mode
+        // order (+0 for the modes past nmx / nmy: x starts at +0 and a sum of +0 with anything
+        // is never -0, so the padding leaves it unchanged)
+        double q[2 * MC_MAX_MODES], v[2 * MC_MAX_MODES];
+#pragma unroll
+        for (int j = 0; j < 2 * MC_MAX_MODES; j++) { q[j] = read_lane(mq, j); v[j] = read_lane(mv, j); }
+        x = 0; y = 0; vxs = 0; vys = 0;
+#pragma unroll
+        for (int j = 0; j < MC_MAX_MODES; j++) {
+          x += j < nmx ? q[j] : 0.0; vxs += j < nmx ? v[j] : 0.0;
+          y += j < nmy ? q[MC_MAX_MODES + j] : 0.0; vys += j < nmy ? v[MC_MAX_MODES + j] : 0.0;
+        }
+      }
       // ---- wall mode (src/cutting.js:256-264)
       if (wallOn) {
         const double Fw = -(Fx * wnx + Fy * wny);

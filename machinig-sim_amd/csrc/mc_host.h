@@ -24,6 +24,11 @@ struct StockView {
 };
 // fills v; the caller is about to change the heights, so the block maxima are marked stale
 int stock_view(mc_stock* s, StockView* v);
+// the stock's strip maxima for the sub-step kernel: one float per strip of MCH_STRIP_W x
+// MCH_STRIP_H nodes, snx strips per row of strips (allocated on first use; their values
+// are the kernel's to set)
+constexpr int MCH_STRIP_W = 32, MCH_STRIP_H = 2;
+int stock_strips(mc_stock* s, float** smax, int* snx, int* sny);
 }  // namespace mch
 
 #define MCH_HIPCHK(call)                                                                    \

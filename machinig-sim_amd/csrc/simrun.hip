@@ -32,13 +32,14 @@ namespace {
 struct StockDev {
   float *H, *err, *wallD, *markS, *markF;
   int32_t *lineCut, *lineWall, *rect;
-  int32_t nx, ny;
+  float* smax;   // exact maxima of the SW x SH node strips, snx per row of strips, sny rows
+  int32_t nx, ny, snx, sny;
   double hx, hy, zTop, zB, band;
 };
 
 struct ProgDev {
   EngDev* eng;
-  StockDev act, ide;
+  StockDev act;
   const mc_substep* sub;
   mc_substep_out* out;
   const float* lut;   // bull-nose: the tip table (null otherwise)
@@ -172,14 +173,176 @@ __device__ __forceinline__ double stamp_nodes(const StockDev& S, const Box& bx, 
   return vol;
 }
 
+// the strips the stamps skip by their maxima: SW x SH nodes, a wavefront's 64 lanes, whose
+// loads cover two rows of 128 contiguous bytes
+constexpr int SW = mch::MCH_STRIP_W, SH = mch::MCH_STRIP_H;
+static_assert(SW * SH == 64, "a strip is one wavefront");
+
+// each strip's highest node, one wavefront per strip (4 per workgroup)
+__global__ __launch_bounds__(256) void strip_max(const float* __restrict__ H, int nx, int ny, int snx, int sny,
+                                                 float* __restrict__ smax) {
+  const int strip = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (strip >= snx * sny) return;
+  const int si = strip % snx, sj = strip / snx;
+  const int i = si * SW + (lane & (SW - 1)), j = sj * SH + lane / SW;
+  float h = i < nx && j < ny ? H[(int64_t)j * nx + i] : -__builtin_inff();
+  for (int o = 32; o > 0; o >>= 1) h = fmaxf(h, __shfl_xor(h, o, 64));
+  if (lane == 0) smax[strip] = h;
+}
+
+// Stock.stamp's node loop (src/stock.js:129-155), flat, ball or a tip table (LUT), over the
+// SW x SH node strips that meet the box, one strip per wavefront at a time with a lane per
+// node (waves w = 0..nw-1 of the workgroup take strips w, w + nw, ...).
+// Every update of a stamp needs the node above the stamp's lowest reach over it: the cut
+// H > z, the wall distance H > zRim >= z, the feed mark H > tipZ - 0.002.  So a strip whose
+// maximum is at or below a lower bound of that reach over the block is skipped, exactly:
+// tipZ for a flat end (tipZ - 0.002 with feed marks), tipZ + min(lut) for a tip table, and
+// for a ball the sphere's height at the block's point nearest the stamp's axis, less a
+// margin far above the rounding of the per-node height; strips beyond the wall band are
+// skipped too.  The nodes are independent, so the strips go in any order; a strip with a
+// lowered node gets its maximum again from its nodes, so the maxima stay exact (heights
+// only fall).  Returns the thread's part of the removed height sum; a node changed sets
+// *touched.
+template <bool LUT>
+__device__ __forceinline__ double stamp_blocks(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
+                                               double R, bool ball, const float* lut, int lutn, double lutMin,
+                                               double errVal, double mux, double muy, double mfeed, int32_t line,
+                                               int wave, int nw, int* touched) {
+  const int lane = threadIdx.x & 63;
+  const double Rw = R + jmax(2 * jmax(S.hx, S.hy), S.band), Rw2 = Rw * Rw;
+  const double R2 = R * R, tie = R2 - 0.003 * R;
+  const double mf = mfeed > 0 && !ball && !LUT ? mfeed : 0;
+  const double zRim = tipZ + (LUT ? (double)lut[lutn - 1] : ball ? R : 0);
+  // the stamp's lowest reach where it is uniform over the stamp
+  const double zflat = LUT ? tipZ + lutMin : mf != 0 ? tipZ - 0.002 : tipZ;
+  const int bi0 = bx.i0 / SW, bi1 = bx.i1 / SW, bj0 = bx.j0 / SH, bj1 = bx.j1 / SH;
+  const int bw = bi1 - bi0 + 1, nb = bw * (bj1 - bj0 + 1);
+  double vol = 0;
+  bool tch = false;
+  for (int b0 = wave * 64; b0 < nb; b0 += nw * 64) {
+    // lane l tests block b0 + l: its maximum against the stamp's lowest reach over it
+    bool live = false;
+    {
+      const int q = b0 + lane;
+      if (q < nb) {
+        const int bi = bi0 + q % bw, bj = bj0 + q / bw;
+        // the block's part of the box, and its nearest point to the stamp's axis (a lower
+        // bound of every node's r2, less a margin above the rounding of r2)
+        const int ia = bi * SW > bx.i0 ? bi * SW : bx.i0, ib = bi * SW + SW - 1 < bx.i1 ? bi * SW + SW - 1 : bx.i1;
+        const int ja = bj * SH > bx.j0 ? bj * SH : bx.j0, jb = bj * SH + SH - 1 < bx.j1 ? bj * SH + SH - 1 : bx.j1;
+        const double ex = jmax(0, jmax(ia * S.hx - cx, cx - ib * S.hx));
+        const double ey = jmax(0, jmax(ja * S.hy - cy, cy - jb * S.hy));
+        const double rmin2 = (ex * ex + ey * ey) * (1 - 1e-12);
+        // beyond the wall band nothing changes; past the cutting radius only the wall
+        // distance, which needs H > zRim; inside it the cut (and marks) need H above the
+        // stamp's lowest reach over the block
+        const double zlow = rmin2 >= tie ? zRim
+                          : ball && !LUT ? tipZ + R - __builtin_sqrt(R2 - rmin2) - 1e-9 : zflat;
+        live = rmin2 <= Rw2 && (double)S.smax[bj * S.snx + bi] > zlow;
+      }
+    }
+    unsigned long long m = __ballot(live);
+    // the live blocks, eight at a time: their nodes' loads issued before any store
+    while (m) {
+      constexpr int U = 8;
+      int qb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        qb[u] = m ? b0 + __builtin_ctzll(m) : -1;
+        m &= m ? m - 1 : 0ull;
+      }
+      int64_t k[U];
+      int ni[U], nj[U];
+      bool inS[U], inB[U];
+      double h[U];
+      float wd[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int q = qb[u] >= 0 ? qb[u] : qb[0];
+        const int bi = bi0 + q % bw, bj = bj0 + q / bw;
+        ni[u] = bi * SW + (lane & (SW - 1)); nj[u] = bj * SH + lane / SW;
+        inS[u] = qb[u] >= 0 && ni[u] < S.nx && nj[u] < S.ny;
+        inB[u] = inS[u] && ni[u] >= bx.i0 && ni[u] <= bx.i1 && nj[u] >= bx.j0 && nj[u] <= bx.j1;
+        // every lane loads (a node clamped into the stock where its own is outside), so that the
+        // loads issue as one batch; the values of lanes outside are not used
+        k[u] = (int64_t)(nj[u] < S.ny ? nj[u] : S.ny - 1) * S.nx + (ni[u] < S.nx ? ni[u] : S.nx - 1);
+        const float hv = S.H[k[u]];
+        wd[u] = S.wallD[k[u]];
+        h[u] = inS[u] ? (double)hv : -__builtin_inf();
+      }
+      bool cutb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        cutb[u] = false;
+        if (qb[u] < 0) continue;   // wave-uniform
+        const int i = ni[u], j = nj[u];
+        bool cut = false;
+        if (inB[u]) {
+          const double dy = j * S.hy - cy, dx = i * S.hx - cx;
+          const double r2 = dx * dx + dy * dy;
+          if (r2 >= tie) {
+            if (r2 <= Rw2 && h[u] > zRim) {
+              const double d = __builtin_sqrt(r2) - R;
+              if (d < (double)wd[u]) { S.wallD[k[u]] = (float)d; S.lineWall[k[u]] = line; tch = true; }
+            }
+          } else {
+            const double z = LUT ? tipZ + (double)lut[(int)jmin(lutn - 1, mcm::round_js(__builtin_sqrt(r2) / MC_LUT_DR))]
+                           : ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
+            if (h[u] > z) {
+              vol += h[u] - z;
+              const float nh = (float)jmax(z, S.zB);
+              S.H[k[u]] = nh;
+              S.err[k[u]] = (float)errVal;
+              S.wallD[k[u]] = __builtin_inff();
+              S.lineCut[k[u]] = line;
+              tch = true;
+              cut = true;
+              h[u] = nh;
+            }
+            if (mf != 0 && h[u] > z - 0.002) {
+              const double xs = i * S.hx, ys = j * S.hy, p = -dx * muy + dy * mux;
+              S.markS[k[u]] = (float)(xs * mux + ys * muy + __builtin_sqrt(jmax(0, R2 - p * p)));
+              S.markF[k[u]] = (float)mf;
+              tch = true;
+            }
+          }
+        }
+        cutb[u] = __any(cut);
+      }
+      // a block with a lowered node gets its maximum again from its nodes (f32 heights: max is
+      // exact in any order), the blocks' reductions interleaved; heights only fall, so an
+      // untouched block keeps its maximum
+      float hm[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) hm[u] = (float)h[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int u = 0; u < U; u++) hm[u] = fmaxf(hm[u], __shfl_xor(hm[u], o, 64));
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (cutb[u] && lane == 0) S.smax[(nj[u] / SH) * S.snx + ni[u] / SW] = hm[u];
+    }
+  }
+  if (tch) *touched = 1;
+  return vol;
+}
+
 // The tip-table form runs in a code path of its own: mixed into the flat / ball loop it
-// raised the kernel's register pressure into scratch spills.
+// raised the kernel's register pressure into scratch spills.  A ball's stamps skip strips by
+// their maxima (the floor a ball leaves lies above most of the next stamp's sphere); a flat
+// end rewrites every floor node under it (the feed marks, and the cut wherever the f32
+// floor height rounded above tipZ), so its stamps take every node of the box.
 __device__ __forceinline__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
                                              double R, bool ball, const float* lut, int lutn, double errVal,
                                              double mux, double muy, double mfeed, int32_t line, int t, int nt,
                                              int* touched) {
   if (!bx.ok) return 0;
   if (lut) return stamp_nodes<true>(S, bx, cx, cy, tipZ, R, ball, lut, lutn, errVal, mux, muy, mfeed, line, t, nt, touched);
+  if (ball)
+    return stamp_blocks<false>(S, bx, cx, cy, tipZ, R, true, nullptr, 0, 0.0, errVal, mux, muy, mfeed, line, t >> 6,
+                               nt >> 6, touched);
   return stamp_nodes<false>(S, bx, cx, cy, tipZ, R, ball, nullptr, 0, errVal, mux, muy, mfeed, line, t, nt, touched);
 }
 
@@ -192,7 +355,7 @@ __device__ __forceinline__ void join_rect(const StockDev& S, const Box& b) {
 __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   __shared__ EngLds L;
   __shared__ double red[NT / 64];
-  __shared__ int flags[2];   // the commanded / actual stamp changed a node
+  __shared__ int flags[2];   // flags[1]: the actual stamp changed a node
   const ProgDev& P = progs[blockIdx.x];
   EngDev& E = *P.eng;
   const int tid = threadIdx.x;
@@ -265,7 +428,6 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     int n = 0;
     // the commanded stamp does not depend on the dynamics: waves 1.. stamp it while wave 0
     // runs the engine
-    const Box bi = stamp_box(P.ide, s.x, s.y, s.z, R);
     if (tid == 0) { flags[0] = 0; flags[1] = 0; }
     __threadfence_block();
     __syncthreads();
@@ -290,15 +452,10 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
         n = (int)jmax(1, jmin(want, __builtin_floor(P.share)));
       }
       if (tid < 64) engine_steps(E, n, L);
-      else stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, P.lut, P.lutn, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64,
-                      &flags[0]);
       fzLast = s.fzEff;
     } else {
       // out of the cut the vibration decays (:427-428)
       if (tid == 0) { E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5; }
-      if (tid >= 64)
-        stamp_part(P.ide, bi, s.x, s.y, s.z, R, ball, P.lut, P.lutn, 0.0, 0.0, 0.0, 0.0, s.line, tid - 64, NT - 64,
-                   &flags[0]);
     }
     __threadfence_block();
     __syncthreads();
@@ -308,13 +465,10 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     const double maxF = cut ? E.runMaxF : 0.0;
     const double ax = s.x + dx, ay = s.y + dy, az = s.z + (ball ? P.rw : 0), aR = R - P.rw;
     const Box ba = stamp_box(P.act, ax, ay, az, aR);
-    const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, P.lut, P.lutn, errVal, s.ux, s.uy, fzLast * P.N,
-                                   s.line, tid, NT, &flags[1]);
+    const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, P.lut, P.lutn, errVal, s.ux, s.uy,
+                                   fzLast * P.N, s.line, tid, NT, &flags[1]);
     const double vol = block_sum(part, red) * P.act.hx * P.act.hy;
-    if (tid == 0) {
-      if (flags[0]) join_rect(P.ide, bi);
-      if (flags[1]) join_rect(P.act, ba);
-    }
+    if (tid == 0 && flags[1]) join_rect(P.act, ba);
     __threadfence_block();
     __syncthreads();
     if (P.out && tid == 0) {
@@ -393,10 +547,12 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (int st = mch::stock_view(g.actual, &a)) return st;
     if (int st = mch::stock_view(g.ideal, &i)) return st;
     if (a.nx < 2 || a.ny < 2 || i.nx < 2 || i.ny < 2) return mch::fail(MC_ERR_ARG, "program %d: stocks need 2 x 2 nodes", p);
-    d.act = StockDev{a.H, a.err, a.wallD, a.markS, a.markF, a.lineCut, a.lineWall, a.rect, a.nx, a.ny, a.hx, a.hy,
-                     a.zTop, a.zBottom, a.band};
-    d.ide = StockDev{i.H, i.err, i.wallD, i.markS, i.markF, i.lineCut, i.lineWall, i.rect, i.nx, i.ny, i.hx, i.hy,
-                     i.zTop, i.zBottom, i.band};
+    float* sa = nullptr;
+    int snxa = 0, snya = 0;
+    if (g.type == MC_TOOL_BALL)
+      if (int st = mch::stock_strips(g.actual, &sa, &snxa, &snya)) return st;
+    d.act = StockDev{a.H, a.err, a.wallD, a.markS, a.markF, a.lineCut, a.lineWall, a.rect, sa, a.nx, a.ny, snxa, snya,
+                     a.hx, a.hy, a.zTop, a.zBottom, a.band};
     d.sub = nullptr; d.out = nullptr;
     d.R = g.R; d.fluteLen = g.fluteLen; d.rw = g.rw; d.share = g.share;
     d.nsub = g.nsub; d.ball = g.type == MC_TOOL_BALL; d.N = m->N;
@@ -405,8 +561,37 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     d.dynBudget = g.dynBudget > 0 ? g.dynBudget : 0; d.fz0 = g.fz;
     d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0; d.pad = 0;
   }
+  // The commanded stamps (Simulator.ideal, src/sim.js:436) depend only on the schedule and
+  // nothing in the chain reads the commanded stock, so each program's are applied as one
+  // batch with Stock.stamp's sequential semantics (mc_stamp_batch_ex) instead of inside the
+  // chain: R, the tool's type, the sub-step's position and line, errVal 0, no feed marks, and
+  // a bull-nose tool's tip table.
+  for (int p = 0; p < n; p++) {
+    const mc_sim_program& g = progs[p];
+    if (g.nsub == 0) continue;
+    std::vector<mc_stamp> cmd((size_t)g.nsub);
+    std::vector<mc_stamp_opt> opt(g.type == MC_TOOL_BULL ? (size_t)g.nsub : 0);
+    for (int k = 0; k < g.nsub; k++) {
+      const mc_substep& s = g.sub[k];
+      cmd[(size_t)k] = mc_stamp{s.x, s.y, s.z, g.R, g.type == MC_TOOL_BALL ? MC_TOOL_BALL : MC_TOOL_FLAT, s.line, 0.0};
+      if (!opt.empty()) opt[(size_t)k] = mc_stamp_opt{0.0, 0.0, 0.0, 0, 0};
+    }
+    const int32_t lo = 0, ln = g.lutn;
+    const int st = opt.empty() ? mc_stamp_batch(g.ideal, cmd.data(), g.nsub, nullptr)
+                               : mc_stamp_batch_ex(g.ideal, cmd.data(), opt.data(), g.nsub, g.lut, &lo, &ln, 1, nullptr);
+    if (st) return st;
+  }
   std::lock_guard<std::mutex> lk(mch::lock());
   hipStream_t q = mch::stream();
+  // a ball's stamps skip node strips by their maxima and keep them exact: take them from
+  // the heights first
+  for (int p = 0; p < n; p++) {
+    const StockDev& S = pd[(size_t)p].act;
+    if (!S.smax) continue;
+    const int ns = S.snx * S.sny;
+    hipLaunchKernelGGL(strip_max, dim3((ns + 3) / 4), dim3(256), 0, q, S.H, S.nx, S.ny, S.snx, S.sny, S.smax);
+    if (hipGetLastError() != hipSuccess) return mch::fail(MC_ERR_HIP, "strip maxima launch failed");
+  }
   // sub-steps in, results out: one device buffer for every program
   size_t nin = 0, nlut = 0, nfr = 0;
   for (int p = 0; p < n; p++) {
