@@ -31,8 +31,12 @@ constexpr double H_REF = 0.05, H_MIN = 0.002;
 enum { S_N, S_SUMF, S_MAXF, S_SUMFX, S_SUMFY, S_SUMFZ, S_MAXFZ, S_SUMT, S_MAXT, S_SUMX, S_SUMY, S_MINX,
        S_MAXX, S_MINY, S_MAXY, S_SUMZC, S_WZC, S_SUMW, S_MAXW, S_SUMHB, S_SUMB, S_SUMRB, S_ENGT, S_COUNT };
 
-// accumulations kept per visit (the order of the chain lanes)
-enum { T_FX, T_FY, T_FZ, T_TQ, T_FS, T_FR2Z, T_HB, T_B, T_RB, T_VOL, T_PHI, T_Z, T_COUNT };
+// accumulations kept per visit: the step's sums (lanes 0..T_SUMS-1 of the chain; the last
+// three carry across steps), then the tooth's chip volume, angle and height
+enum { T_FX, T_FY, T_FZ, T_TQ, T_HB, T_B, T_RB, T_VOL, T_PHI, T_Z, T_COUNT };
+constexpr int T_SUMS = T_RB + 1;
+// per-step records of a chunk, for the statistics taken after it
+enum { R_FXY, R_FX, R_FY, R_FZ, R_TQ, R_X, R_Y, R_W, R_ENG, R_ZC, R_ZCOK, R_COUNT };
 
 // One engine.  The host keeps a mirror, uploads it before a run and reads it back after.
 struct EngDev {
@@ -96,7 +100,7 @@ constexpr int GCAP = 1024;
 struct EngLds {
   double term[T_COUNT][SLOT_CAP];
   int teng[SLOT_CAP];
-  double res[8];
+  double res[T_SUMS];
   // the run's constants, staged from the engine record: the slice tables, the edge force
   // terms Kte b, Kre b (slices, then rings with ds), Kae b, the process-damping factor
   // pdK (b / db) / r of each slice
@@ -107,7 +111,29 @@ struct EngLds {
   float Hb[MC_BINS], Eb[EBINS * ERINGS];
   double gphi[GCAP], gsn[GCAP], gcs[GCAP];
   float win[GCAP];
+  // a chunk's tangential and radial force of every (step, visit), for the force magnitudes
+  // taken after it, and its per-step records
+  double dFt[GCAP], dFr[GCAP];
+  double rec[R_COUNT][64];
 };
+
+// the rare paths of the step loop, kept out of line so that the loop's code stays small:
+// pow_js where the branch-free pow_cell does not apply, hypot_js of non-finite operands
+__device__ __attribute__((noinline)) double pow_js_outlined(double x, double y) { return mcm::pow_js(x, y); }
+__device__ __attribute__((noinline)) double hypot_js_outlined(double a, double b) { return mcm::hypot_js(a, b); }
+
+// Math.hypot(a, b) for finite a, b without branches: V8's scaled, compensated sum (as
+// mcm::hypot_js), which for two terms is sqrt((|a|/m)^2 + (|b|/m)^2) m with m = max(|a|, |b|)
+// (the compensation term is exactly 0 after the first); non-finite operands go to hypot_js
+__device__ __forceinline__ double hypot_finite(double a, double b) {
+  const double fa = __builtin_fabs(a), fb = __builtin_fabs(b);
+  if (!(fa < __builtin_inf() && fb < __builtin_inf())) return hypot_js_outlined(a, b);
+  const double mx = fa > fb ? fa : fb;
+  const double m = mx == 0 ? 1.0 : mx;
+  const double na = fa / m, nb = fb / m;
+  const double r = __builtin_sqrt(na * na + nb * nb) * m;
+  return mx == 0 ? 0.0 : r;
+}
 
 // a double of lane `src`, broadcast to every lane
 __device__ __forceinline__ double read_lane(double v, int src) {
@@ -165,14 +191,17 @@ __host__ __device__ __forceinline__ void eng_new_stats(double* st) {
 // The steps run in chunks of at most one tooth period.  Before a chunk, every lane computes
 // its share of the chunk's edge angles (the slice angle (theta + i pitch - lag) mod 2 pi and
 // the end-edge angle, with their fdlibm sine and cosine) and loads the delay-line cells the
-// chunk's slice visits read: within one tooth period no cell is visited twice, so the
-// chunk reads and writes them in LDS and stores them back after it.  A step's own chain is
-// then the chip thickness, the force law, the serial sums, the modes (one lane per mode,
-// summed in mode order) and the statistics (one lane per field).
+// chunk's slice visits read: within one tooth period no cell is visited twice, so the chunk
+// reads and writes them in LDS and stores them back after it.  A step's own chain is then
+// the chip thickness, the force law, the serial sums, the chip events and the modes (one
+// lane per mode, summed in mode order).  What no later step reads is taken after the chunk,
+// from the chunk's records, in the reference's order: the force magnitudes |(Ft, Fr)| and
+// their sums, |(Fx, Fy)|, the statistics (one lane per field), the rings and the history.
 __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& L) {
   auto& term = L.term;
   auto& teng = L.teng;
   auto& res = L.res;
+  auto& rec = L.rec;
   const int lane = threadIdx.x & 63;
   const double TWO_PI = mcm::TWO_PI;
 
@@ -223,38 +252,38 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
   int ph = pr % spt, tq = pr / spt;   // pr % stepsPerTooth and pr / stepsPerTooth, kept as counters
   int64_t stepCount = E.stepCount;
   int dleft = decim - (int)(stepCount % decim);   // steps until ++stepCount % decim == 0
-  // chain lanes 6..8 carry the engagement statistics across steps; tooth lanes their tooth
+  // sum lanes T_HB..T_RB carry the engagement statistics across steps; tooth lanes their tooth
   const int tooth = lane - 16;
   const bool isTooth = tooth >= 0 && tooth < N;
-  double chain = lane == 6 ? E.st[S_SUMHB] : lane == 7 ? E.st[S_SUMB] : lane == 8 ? E.st[S_SUMRB] : 0.0;
+  double chain = lane == T_HB ? E.st[S_SUMHB] : lane == T_B ? E.st[S_SUMB] : lane == T_RB ? E.st[S_SUMRB] : 0.0;
   double tvol = isTooth ? E.toothVol[tooth] : 0.0, tphi = isTooth ? E.toothPhi[tooth] : 0.0,
          tz = isTooth ? E.toothZ[tooth] : 0.0;
   int tEng = isTooth ? E.toothEngaged[tooth] : 0;
-  // the step statistics: lane f keeps field f (the three engagement sums are the chain
-  // lanes'); each lane adds, maximises or minimises its own operand of the step
+  // the step statistics: lane f keeps field f (the three engagement sums are the sum lanes');
+  // each lane adds, maximises or minimises its own operand of each step, a record field
   enum { OP_NONE, OP_ADD, OP_MAX, OP_MIN };
-  int op = OP_NONE, src = 0, cond = 0;   // src: operand; cond 1: Fsum > 0, 2: the wall mode is on
+  int op = OP_NONE, src = 0, cond = 0;   // cond 1: Fsum > 0, 2: the wall mode is on
   switch (lane) {
-    case S_N: op = OP_ADD; src = 0; break;
-    case S_SUMF: op = OP_ADD; src = 1; break;
-    case S_MAXF: op = OP_MAX; src = 1; break;
-    case S_SUMFX: op = OP_ADD; src = 2; break;
-    case S_SUMFY: op = OP_ADD; src = 3; break;
-    case S_SUMFZ: op = OP_ADD; src = 4; break;
-    case S_MAXFZ: op = OP_MAX; src = 5; break;
-    case S_SUMT: op = OP_ADD; src = 6; break;
-    case S_MAXT: op = OP_MAX; src = 6; break;
-    case S_SUMX: op = OP_ADD; src = 7; break;
-    case S_SUMY: op = OP_ADD; src = 8; break;
-    case S_MINX: op = OP_MIN; src = 7; break;
-    case S_MAXX: op = OP_MAX; src = 7; break;
-    case S_MINY: op = OP_MIN; src = 8; break;
-    case S_MAXY: op = OP_MAX; src = 8; break;
-    case S_SUMZC: op = OP_ADD; src = 9; cond = 1; break;
-    case S_WZC: op = OP_ADD; src = 1; cond = 1; break;
-    case S_SUMW: op = OP_ADD; src = 10; cond = 2; break;
-    case S_MAXW: op = OP_MAX; src = 11; cond = 2; break;
-    case S_ENGT: op = OP_ADD; src = 12; break;
+    case S_N: op = OP_ADD; src = -1; break;
+    case S_SUMF: op = OP_ADD; src = R_FXY; break;
+    case S_MAXF: op = OP_MAX; src = R_FXY; break;
+    case S_SUMFX: op = OP_ADD; src = R_FX; break;
+    case S_SUMFY: op = OP_ADD; src = R_FY; break;
+    case S_SUMFZ: op = OP_ADD; src = R_FZ; break;
+    case S_MAXFZ: op = OP_MAX; src = -2; break;
+    case S_SUMT: op = OP_ADD; src = R_TQ; break;
+    case S_MAXT: op = OP_MAX; src = R_TQ; break;
+    case S_SUMX: op = OP_ADD; src = R_X; break;
+    case S_SUMY: op = OP_ADD; src = R_Y; break;
+    case S_MINX: op = OP_MIN; src = R_X; break;
+    case S_MAXX: op = OP_MAX; src = R_X; break;
+    case S_MINY: op = OP_MIN; src = R_Y; break;
+    case S_MAXY: op = OP_MAX; src = R_Y; break;
+    case S_SUMZC: op = OP_ADD; src = R_ZC; cond = 1; break;
+    case S_WZC: op = OP_ADD; src = R_FXY; cond = 1; break;
+    case S_SUMW: op = OP_ADD; src = R_W; cond = 2; break;
+    case S_MAXW: op = OP_MAX; src = -3; cond = 2; break;
+    case S_ENGT: op = OP_ADD; src = R_ENG; break;
     default: break;
   }
   if (cond == 2 && !wallOn) op = OP_NONE;
@@ -263,9 +292,13 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
   int64_t ws_i = E.ws_i, ws_count = E.ws_count, wsp_i = E.wsp_i, wsp_count = E.wsp_count;
   int64_t nev = 0;
   const int64_t ev_cap = E.ev_cap;
-  // steps per chunk: at most a tooth period (no delay-line cell twice) and GCAP entries
+  // steps per chunk: at most a tooth period (no delay-line cell twice), 64 and GCAP entries
   int chMax = spt < 64 ? spt : 64;
   if (NS > 0 && GCAP / NS < chMax) chMax = GCAP / NS;
+  // this lane's first visit: its tooth and its slice (or end ring), the same every step
+  const int iL = per > 0 ? lane / per : 0, rL = per > 0 ? lane - iL * per : 0;
+  // the sum lanes' and the tooth lanes' rows of the visit terms
+  const double* trow = lane < T_SUMS ? term[lane] : term[T_VOL];
   wave_sync();
 
   for (int64_t s0 = 0; s0 < nSteps; s0 += chMax) {
@@ -297,10 +330,11 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
       // ---- visits (src/cutting.js:172-237)
       for (int slot = lane; slot < NS; slot += 64) {
         const int e = sc * NS + slot;
-        const int i = slot / per, r = slot - i * per;
+        const int i = slot == lane ? iL : slot / per, r = slot == lane ? rL : slot - (slot / per) * per;
         double a[T_COUNT];
 #pragma unroll
         for (int q = 0; q < T_COUNT; q++) a[q] = 0.0;
+        double ft = 0, fr = 0;
         int eng = 0;
         const double phi = L.gphi[e], sn = L.gsn[e], cs = L.gcs[e];
         if (r < nz) {
@@ -321,7 +355,7 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
               L.win[e] = (float)rn;
               if (h > hMax) h = hMax;
               const double xs = H_REF / (h > H_MIN ? h : H_MIN);
-              const double se = !mcOn ? 1.0 : cellPow ? mcm::pow_cell(xs, mc, my1, myr) : mcm::pow_js(xs, mc);
+              const double se = !mcOn ? 1.0 : cellPow ? mcm::pow_cell(xs, mc, my1, myr) : pow_js_outlined(xs, mc);
               const double bk = L.sB[k], sr = L.sR[k], sz = L.sZ[k];
               const double Ft = Ktc * se * h * bk + L.cTe[k];
               double Fr = Krc * se * h * bk + L.cRe[k];
@@ -332,15 +366,13 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
               a[T_FY] = -Ft * tyg - Fr * nyg;
               a[T_FZ] = -Fa;
               a[T_TQ] = Ft * sr;
-              const double fr = mcm::hypot_js(Ft, Fr);
-              a[T_FS] = fr;
-              a[T_FR2Z] = fr * (sz - tipZ);
               a[T_HB] = h * bk;
               a[T_B] = bk;
               a[T_RB] = sr * h * bk;
               a[T_VOL] = h * bk * sr * dphi;
               a[T_PHI] = phi;
               a[T_Z] = sz;
+              ft = Ft; fr = Fr;
               eng = 1;
             }
           }
@@ -353,54 +385,51 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
             const double c = L.ring[ERINGS + j], ds = L.ring[3 * ERINGS + j], rr = L.ring[j], rz = L.ring[2 * ERINGS + j];
             const double h = (depth < fza ? depth : fza) * c;
             const double xs = H_REF / (h > H_MIN ? h : H_MIN);
-            const double se = !mcOn ? 1.0 : cellPow ? mcm::pow_cell(xs, mc, my1, myr) : mcm::pow_js(xs, mc);
+            const double se = !mcOn ? 1.0 : cellPow ? mcm::pow_cell(xs, mc, my1, myr) : pow_js_outlined(xs, mc);
             const double Ft = Ktc * se * h * ds + L.cTe[MS + j];
             const double Fn = Krc * se * h * ds + L.cRe[MS + j];
             a[T_FX] = -Ft * txg;
             a[T_FY] = -Ft * tyg;
             a[T_FZ] = -(Fn * c);
             a[T_TQ] = Ft * rr;
-            a[T_FS] = Ft;
-            a[T_FR2Z] = Ft * rz;
             a[T_HB] = h * ds;
             a[T_B] = ds;
             a[T_RB] = rr * h * ds;
             a[T_VOL] = h * ds * rr * dphi;
             a[T_PHI] = phi;
             a[T_Z] = tipZ + rz;
+            ft = Ft;
             eng = 1;
           }
         }
 #pragma unroll
         for (int q = 0; q < T_COUNT; q++) term[q][slot] = a[q];
         teng[slot] = eng;
+        L.dFt[e] = ft; L.dFr[e] = fr;
       }
       wave_sync();
-      // ---- serial sums in the reference's visit order, eight terms loaded at a time (the
-      // sums start at +0 or at a positive chain, so they are never -0 and the +0 padding past
-      // the last visit leaves them unchanged)
-      if (lane < 9) {
-        double acc = lane < 6 ? 0.0 : chain;
-        const double* tl = term[lane];
-        for (int q0 = 0; q0 < NS; q0 += 8) {
-          double v[8];
+      // ---- serial sums in the reference's visit order (sum lanes), and a tooth lane's chip
+      // volume (src/cutting.js:213) and last engaged visit over its own visits, eight terms
+      // loaded at a time (the sums start at +0 or at a non-negative carried sum, so they are
+      // never -0 and the +0 padding past the last visit leaves them unchanged)
+      double acc = lane < T_HB ? 0.0 : lane < T_SUMS ? chain : isTooth ? tvol : 0.0;
+      int last = -1;
+      const int qa = isTooth ? tooth * per : 0, qb = isTooth ? qa + per : NS;
+      for (int q0 = qa; q0 < qb; q0 += 8) {
+        double v[8];
+        int ev[8];
 #pragma unroll
-          for (int u = 0; u < 8; u++) v[u] = q0 + u < NS ? tl[q0 + u] : 0.0;
+        for (int u = 0; u < 8; u++) { v[u] = q0 + u < qb ? trow[q0 + u] : 0.0; ev[u] = q0 + u < qb ? teng[q0 + u] : 0; }
 #pragma unroll
-          for (int u = 0; u < 8; u++) acc += v[u];
-        }
-        if (lane < 6) res[lane] = acc; else chain = acc;
+        for (int u = 0; u < 8; u++) { acc += v[u]; last = ev[u] ? q0 + u : last; }
       }
+      if (lane < T_HB) res[lane] = acc;
+      else if (lane < T_SUMS) chain = acc;
       bool push = false;
       double pvol = 0, pphi = 0, pz = 0;
       bool engagedNow = false;
       if (isTooth) {
-        int last = -1;
-        for (int r = 0; r < per; r++) {
-          const int q = tooth * per + r;
-          tvol += term[T_VOL][q];
-          if (teng[q]) last = q;
-        }
+        tvol = acc;
         engagedNow = last >= 0;
         if (engagedNow) {
           tphi = term[T_PHI][last];
@@ -423,8 +452,7 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
       }
       nev += __popcll(pmask);
       wave_sync();
-      const double Fx = res[T_FX], Fy = res[T_FY], Fz = res[T_FZ], Tq = res[T_TQ], Fsum = res[T_FS],
-                   Fr2z = res[T_FR2Z];
+      const double Fx = res[T_FX], Fy = res[T_FY];
       // ---- modal integration, semi-implicit Euler (src/cutting.js:246-255): each mode on its
       // lane, the displacements and velocities summed in mode order
       if (xmode || ymode) {
@@ -452,57 +480,139 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
         w += wv * dt;
         if (w > 5e-4) { w = 5e-4; if (wv > 0) wv = 0; }
         else if (w < -2e-4) { w = -2e-4; if (wv < 0) wv = 0; }
-        wSum += w;
       }
-      if (++pr == spr) pr = 0;
-      if (++ph == spt) { ph = 0; if (++tq == N) tq = 0; }
-      if (ph == 0) {
-        // the tooth now at phi = 0 generates the +v wall (src/cutting.js:265-272)
-        const double d0 = L.dR[(N - tq) % N] * 1e-3;
-        if (lane == 0) {
-          E.rings[ts_i] = (float)(x - w * wnx);
-          E.rings[RN + ts_i] = (float)(y - w * wny);
-          E.rings[2 * RN + ws_i] = (float)(x - w * wnx + d0 * vx);
-          E.rings[3 * RN + ws_i] = (float)(y - w * wny + d0 * vy);
-        }
-        ts_i = (ts_i + 1) % RN; ts_count++;
-        ws_i = (ws_i + 1) % RN; ws_count++;
-      }
-      if (ph == piStep) {
-        const long long qn = (long long)mcm::round_js((JS_PI - pr * dphi) / pitch);
-        const int ip = (int)(((qn % N) + N) % N);
-        const double dp = L.dR[ip] * 1e-3;
-        if (lane == 0) {
-          E.rings[4 * RN + wsp_i] = (float)(x - w * wnx - dp * vx);
-          E.rings[5 * RN + wsp_i] = (float)(y - w * wny - dp * vy);
-        }
-        wsp_i = (wsp_i + 1) % RN; wsp_count++;
-      }
-      t += dt;
-      // ---- step statistics (src/cutting.js:279-287), lane f updating field f
-      const double Fxy = mcm::hypot_js(Fx, Fy);
-      runMaxF = jmax(runMaxF, Fxy);
-      if (op != OP_NONE && (cond != 1 || Fsum > 0)) {
-        const double o = src == 0 ? 1.0 : src == 1 ? Fxy : src == 2 ? Fx : src == 3 ? Fy : src == 4 ? Fz
-                       : src == 5 ? -Fz : src == 6 ? Tq : src == 7 ? x : src == 8 ? y
-                       : src == 9 ? Fr2z / Fsum * Fxy : src == 10 ? w : src == 11 ? __builtin_fabs(w)
-                       : (double)__popcll(emask);
-        stv = op == OP_ADD ? stv + o : op == OP_MAX ? jmax(stv, o) : jmin(stv, o);
-      }
-      stepCount++;
-      if (--dleft == 0) {
-        dleft = decim;
-        if (lane == 0) {
-          E.hist[hist_i] = (float)Fx;
-          E.hist[HN + hist_i] = (float)Fy;
-          E.hist[2 * HN + hist_i] = (float)Fz;
-          E.hist[3 * HN + hist_i] = (float)((x - w * wnx) * 1e6);
-          E.hist[4 * HN + hist_i] = (float)((y - w * wny) * 1e6);
-        }
-        hist_i = (hist_i + 1) % HN; hist_count++;
+      // ---- the step's record
+      if (lane == 0) {
+        rec[R_FX][sc] = Fx; rec[R_FY][sc] = Fy; rec[R_FZ][sc] = res[T_FZ]; rec[R_TQ][sc] = res[T_TQ];
+        rec[R_X][sc] = x; rec[R_Y][sc] = y; rec[R_W][sc] = w; rec[R_ENG][sc] = (double)__popcll(emask);
       }
     }
     wave_sync();
+    // ---- after the chunk: the force magnitudes of every visit, |(Ft, Fr)| on a slice and Ft
+    // on an end ring, with their moments about the tip (src/cutting.js:207-208,230), four
+    // visits per lane at a time
+    for (int e0 = lane; e0 < ch * NS; e0 += 4 * 64) {
+      double fs[4], fm[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = e0 + u * 64;
+        const int ee = e < ch * NS ? e : e0;
+        const int slot = ee % NS, r = slot % per;
+        const double a = L.dFt[ee], b = L.dFr[ee];
+        fs[u] = r < nz ? hypot_finite(a, b) : a;
+        fm[u] = r < nz ? L.sZ[r] - tipZ : L.ring[2 * ERINGS + r - nz];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = e0 + u * 64;
+        if (e < ch * NS) { L.dFt[e] = fs[u]; L.dFr[e] = fs[u] * fm[u]; }
+      }
+    }
+    wave_sync();
+    // lane sc: the step's Fsum and Fr2z summed in visit order from +0, |(Fx, Fy)|, and the
+    // moment centre term of the statistics; and whether the step takes a sample of the rings
+    // or of the history
+    {
+      const bool isStep = lane < ch;
+      const int sl = isStep ? lane : 0;
+      double fsum = 0, fr2z = 0;
+      for (int q0 = 0; q0 < NS; q0 += 8) {
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          a[u] = q0 + u < NS ? L.dFt[sl * NS + q0 + u] : 0.0;
+          b[u] = q0 + u < NS ? L.dFr[sl * NS + q0 + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { fsum += a[u]; fr2z += b[u]; }
+      }
+      if (isStep) {
+        const double Fxy = hypot_finite(rec[R_FX][sl], rec[R_FY][sl]);
+        rec[R_FXY][sl] = Fxy;
+        rec[R_ZCOK][sl] = fsum > 0 ? 1.0 : 0.0;
+        rec[R_ZC][sl] = fsum > 0 ? fr2z / fsum * Fxy : 0.0;
+      }
+    }
+    wave_sync();
+    // the statistics (src/cutting.js:279-287), lane f over the chunk's steps in order, eight
+    // records loaded at a time
+    for (int s8 = 0; s8 < ch; s8 += 8) {
+      double o[8], fxy[8];
+      bool ok[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int sc = s8 + u < ch ? s8 + u : s8;
+        fxy[u] = rec[R_FXY][sc];
+        ok[u] = s8 + u < ch && op != OP_NONE && (cond != 1 || rec[R_ZCOK][sc] != 0);
+        o[u] = src >= 0 ? rec[src][sc] : src == -1 ? 1.0 : src == -2 ? -rec[R_FZ][sc] : __builtin_fabs(rec[R_W][sc]);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (s8 + u < ch) runMaxF = jmax(runMaxF, fxy[u]);
+        if (ok[u]) stv = op == OP_ADD ? stv + o[u] : op == OP_MAX ? jmax(stv, o[u]) : jmin(stv, o[u]);
+      }
+    }
+    // the rings and the history: step sc's pr, its phase in the tooth period and its count;
+    // the samples are numbered in step order with ballots and written by their steps' lanes
+    {
+      const bool isStep = lane < ch;
+      const int sl = isStep ? lane : 0;
+      int prs = prc + sl + 1;
+      if (prs >= spr) prs -= spr;
+      const int phs = prs % spt, tqs = prs / spt;
+      const bool tsS = isStep && phs == 0, wpS = isStep && phs == piStep;
+      const bool hiS = isStep && sl + 1 >= dleft && (sl + 1 - dleft) % decim == 0;
+      const unsigned long long mTs = __ballot(tsS), mWp = __ballot(wpS), mHi = __ballot(hiS);
+      const unsigned long long below = (1ull << lane) - 1;
+      const double xs = rec[R_X][sl], ys = rec[R_Y][sl], ws = rec[R_W][sl];
+      if (tsS) {
+        // the tooth now at phi = 0 generates the +v wall (src/cutting.js:265-272)
+        const double d0 = L.dR[(N - tqs) % N] * 1e-3;
+        const int64_t a = (ts_i + __popcll(mTs & below)) % RN, b = (ws_i + __popcll(mTs & below)) % RN;
+        E.rings[a] = (float)(xs - ws * wnx);
+        E.rings[RN + a] = (float)(ys - ws * wny);
+        E.rings[2 * RN + b] = (float)(xs - ws * wnx + d0 * vx);
+        E.rings[3 * RN + b] = (float)(ys - ws * wny + d0 * vy);
+      }
+      if (wpS) {
+        const long long qn = (long long)mcm::round_js((JS_PI - prs * dphi) / pitch);
+        const int ip = (int)(((qn % N) + N) % N);
+        const double dp = L.dR[ip] * 1e-3;
+        const int64_t a = (wsp_i + __popcll(mWp & below)) % RN;
+        E.rings[4 * RN + a] = (float)(xs - ws * wnx - dp * vx);
+        E.rings[5 * RN + a] = (float)(ys - ws * wny - dp * vy);
+      }
+      if (hiS) {
+        const int64_t a = (hist_i + __popcll(mHi & below)) % HN;
+        E.hist[a] = (float)rec[R_FX][sl];
+        E.hist[HN + a] = (float)rec[R_FY][sl];
+        E.hist[2 * HN + a] = (float)rec[R_FZ][sl];
+        E.hist[3 * HN + a] = (float)((xs - ws * wnx) * 1e6);
+        E.hist[4 * HN + a] = (float)((ys - ws * wny) * 1e6);
+      }
+      const int nts = __popcll(mTs), nwp = __popcll(mWp), nhi = __popcll(mHi);
+      ts_i = (ts_i + nts) % RN; ts_count += nts;
+      ws_i = (ws_i + nts) % RN; ws_count += nts;
+      wsp_i = (wsp_i + nwp) % RN; wsp_count += nwp;
+      hist_i = (hist_i + nhi) % HN; hist_count += nhi;
+      dleft = nhi ? decim - (ch - (dleft + (nhi - 1) * decim)) : dleft - ch;
+      stepCount += ch;
+      pr = prc + ch;
+      if (pr >= spr) pr -= spr;
+      ph = pr % spt; tq = pr / spt;
+    }
+    // the time and the mean wall deflection, summed step by step
+    for (int s8 = 0; s8 < ch; s8 += 8) {
+      double wl[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) wl[u] = rec[R_W][s8 + u < ch ? s8 + u : s8];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (s8 + u < ch) {
+          t += dt;
+          if (wallOn) wSum += wl[u];
+        }
+    }
     // ---- the chunk's delay-line cells back to the engine's record
     for (int e = lane; e < ch * NS; e += 64) {
       const int sc = e / NS, slot = e - sc * NS;
@@ -518,9 +628,9 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
     wave_sync();
   }
   // ---- write the state back
-  if (lane == 6) E.st[S_SUMHB] = chain;
-  if (lane == 7) E.st[S_SUMB] = chain;
-  if (lane == 8) E.st[S_SUMRB] = chain;
+  if (lane == T_HB) E.st[S_SUMHB] = chain;
+  if (lane == T_B) E.st[S_SUMB] = chain;
+  if (lane == T_RB) E.st[S_SUMRB] = chain;
   if (op != OP_NONE) E.st[lane] = stv;
   if (isTooth) {
     E.toothVol[tooth] = tvol; E.toothPhi[tooth] = tphi; E.toothZ[tooth] = tz; E.toothEngaged[tooth] = tEng;
