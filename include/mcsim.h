@@ -125,6 +125,13 @@ int mc_device_info(char* buf, int len);
  * device (the chatter search, the tile pass of mc_stamp_batch, or the sub-step pipeline of
  * mc_sim_run), measured with HIP events on the library's stream around that kernel alone. */
 int mc_last_kernel_time(double* ms);
+/* Trials of the last search entry point called on this device (mc_bisect_batch,
+ * mc_bisect_prior_batch, mc_bisect_bracket_batch): *serial the trials on the serial path (the
+ * sum of the results' mc_limit.trials), *executed every trial the kernels ran: the speculated
+ * tree nodes off that path included, and the rounds a tree ran before its cell's aMax check
+ * came out stable (a number that depends on timing).  No reference counterpart: it measures
+ * the speculation a prior saves. */
+int mc_last_search_trials(int64_t* executed, int64_t* serial);
 
 /* Launch schedule of the search (library-wide; no reference counterpart).  The trees of
  * the cells whose trials have >= wide_frac x the longest trial's steps (at most wide_max
@@ -438,6 +445,57 @@ typedef struct {
   int32_t end;             /* sub-steps of the call done when the frame ended */
   int32_t pad;
 } mc_frame_out;
+
+/* ---- The path format and the sub-step schedule (host) ----
+ *
+ * The program's moves become a polyline (arcs cut into chords, src/gcode.js:433-466) and
+ * the polyline the sub-steps Simulator.step visits (src/sim.js:351-381,389-405), in the
+ * reference's arithmetic (V8's hypot, atan2, acos, cos, sin), so that a JavaScript or C
+ * caller hands mc_sim_run the simulator's own sub-steps. */
+
+/* one program move, as the reference's parser emits it (src/gcode.js moves) */
+typedef struct {
+  double from[3], to[3];   /* end points [mm] */
+  double F, S;             /* feed [mm/min] and speed [rpm] */
+  int32_t rapid, line;     /* a G0 move; program line */
+  int32_t arc;             /* 1: a circular move about (ca, cb) in the plane of axes[0], axes[1] */
+  int32_t cw;              /* clockwise (G2) */
+  int32_t axes[3];         /* plane axes a, b and the normal n: 0 x, 1 y, 2 z */
+  int32_t has_sweep;       /* 1: sweep set (by cutter compensation) instead of derived */
+  double ca, cb, sweep;
+} mc_move;
+
+/* one polyline point, carrying the attributes of the segment that ends at it */
+typedef struct {
+  double x, y, z, F, S;
+  int32_t rapid, line;
+  int32_t move;            /* the move whose segment ends here (the first point: move 0's start) */
+  int32_t pad;
+} mc_path_point;
+
+/* the path discretizer (src/gcode.js:435-466): the first move's start (as a rapid), then
+ * each line move's end, each arc's chords (segs = max(2, or 1 for a null arc;
+ * ceil(|sweep| r / max_seg); ceil(|sweep| / dAng)), dAng the angle whose chord lies 0.001 mm
+ * inside the arc; the last chord ends on the move's end).  out NULL and cap 0 ask for n. */
+int mc_path_discretize(const mc_move* moves, int n, double max_seg, mc_path_point* out, int cap, int* n_out);
+
+typedef struct {
+  double hx;             /* stock grid spacing [mm]: stampStep = min(hx, 0.4) * 0.6 (src/sim.js:355) */
+  double dt_real;        /* frame length [s] of Simulator.step(dt); 0 for one unbounded run */
+  double speed;          /* playback speed (machine seconds per real second) */
+  double feed_override;
+  int32_t flutes;        /* the tool's flutes (feed per tooth) */
+  int32_t max_frames;    /* frames at most (dt_real > 0) */
+} mc_schedule_opts;
+
+/* the sub-steps of a polyline (src/sim.js:351-381,389-405).  dt_real 0: every sub-step of
+ * the path with an unbounded budget (no frames; nSubEst, guard, frame 0).  dt_real > 0:
+ * successive frames of dt_real * speed machine seconds, the frame loop's guard of 4000
+ * passes, each frame's sub-step estimate and its first / last flags, and frames[2 f],
+ * frames[2 f + 1] = the first sub-step and the sub-step count of frame f.  out NULL and cap
+ * 0 (frames NULL, frame_cap 0) ask for the counts. */
+int mc_sim_schedule(const mc_path_point* pts, int npts, const mc_schedule_opts* opts, mc_substep* out, int cap,
+                    int* n_sub, int32_t* frames, int frame_cap, int* n_frames);
 
 /* what a sub-step did */
 typedef struct {

@@ -18,7 +18,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["chatter.hip", "search.hip", "stock.hip", "engine.hip", "simrun.hip", "mlp.hip", "lobes.hip", "tiplut.cpp", "jsmath.cpp", "api.cpp"]
+SOURCES = ["chatter.hip", "search.hip", "stock.hip", "engine.hip", "simrun.hip", "mlp.hip", "lobes.hip", "tiplut.cpp", "jsmath.cpp", "path.cpp", "api.cpp"]
 
 
 def build(verbose=False, jobs=None):

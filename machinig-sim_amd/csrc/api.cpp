@@ -134,6 +134,7 @@ struct Ctx {
   hipStream_t stream = nullptr;   // default priority
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
+  int64_t last_exec = 0, last_serial = 0;   // trials of the last search call: executed, on the serial path
   DevBuf cells, tasks, out, queue, cstate, ring, hist, trace, geo, gbin;
   HostBuf h_cells, h_tasks, h_out;
   SearchGraph sgraph;
@@ -484,7 +485,10 @@ int run_search(TaskSet& ts, std::vector<mcd::ResultDev>& res) {
   HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
   X.last_ms += ms;
   const mcd::ResultDev* tmp = static_cast<const mcd::ResultDev*>(X.h_out.p);
-  for (int q = 0; q < n; q++) res[q < nn ? narrow[q] : wide[q - nn]] = tmp[q];
+  for (int q = 0; q < n; q++) {
+    res[q < nn ? narrow[q] : wide[q - nn]] = tmp[q];
+    X.last_exec += tmp[q].pad;     // trials the task ran, the speculated ones included
+  }
   if (getenv("MC_TASK_TIMES")) {
     FILE* f = fopen(getenv("MC_TASK_TIMES"), "w");
     if (f) {
@@ -696,6 +700,13 @@ int mc_last_kernel_time(double* ms) {
   return MC_OK;
 }
 
+int mc_last_search_trials(int64_t* executed, int64_t* serial) {
+  if (!executed || !serial) return fail(MC_ERR_ARG, "null pointer");
+  *executed = g_ctx.last_exec;
+  *serial = g_ctx.last_serial;
+  return MC_OK;
+}
+
 int mc_device_info(char* buf, int len) {
   int st = ensure_init(-1);
   if (st) return st;
@@ -711,6 +722,7 @@ int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* ou
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
   g_ctx.last_ms = 0;
+  g_ctx.last_exec = g_ctx.last_serial = 0;
   ts.tasks.resize(n);
   ts.cost.resize(n);
   ts.nzcap.resize(n);
@@ -731,6 +743,23 @@ int mc_chatter_batch(const mc_cell* cells, const double* ap, int n, mc_trial* ou
   return MC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// the trial counts of a search call (mc_last_search_trials): the results' serial trials;
+// the cells in `serial` ran on the one-trial-per-group kernel, which runs exactly those
+int finish_counts(const mc_limit* out, int n, const std::vector<int>& serial) {
+  int64_t s = 0, x = 0;
+  for (int i = 0; i < n; i++) s += out[i].trials;
+  for (int i : serial) x += out[i].trials;
+  g_ctx.last_serial = s;
+  g_ctx.last_exec += x;
+  return MC_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_limit* out) {
   int st = ensure_init(-1);
   if (st) return st;
@@ -739,6 +768,7 @@ int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
   g_ctx.last_ms = 0;
+  g_ctx.last_exec = g_ctx.last_serial = 0;
   std::vector<int> spec, serial;
   split_cells(cells, ts, n, opts->aMax, spec, serial);
   if (!spec.empty()) {
@@ -746,7 +776,7 @@ int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_
     if ((st = plain_search(cells, ts, spec, opts, lim))) return st;
     for (size_t q = 0; q < spec.size(); q++) out[spec[q]] = lim[q];
   }
-  if (serial.empty()) return MC_OK;
+  if (serial.empty()) return finish_counts(out, n, serial);
   TaskSet ss;
   ss.cells = ts.cells;
   for (int i : serial) {
@@ -760,7 +790,7 @@ int mc_bisect_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, mc_
   std::vector<mcd::ResultDev> res;
   if ((st = run_tasks(ss, res, nullptr, 0))) return st;
   for (size_t q = 0; q < serial.size(); q++) to_limit(res[q], out[serial[q]]);
-  return MC_OK;
+  return finish_counts(out, n, serial);
 }
 
 int mc_bisect_prior_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts, const double* mu,
@@ -772,6 +802,7 @@ int mc_bisect_prior_batch(const mc_cell* cells, int n, const mc_bisect_opts* opt
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
   g_ctx.last_ms = 0;
+  g_ctx.last_exec = g_ctx.last_serial = 0;
   std::vector<int> spec, serial;
   split_cells(cells, ts, n, opts->aMax, spec, serial);
   if (!spec.empty()) {
@@ -779,7 +810,7 @@ int mc_bisect_prior_batch(const mc_cell* cells, int n, const mc_bisect_opts* opt
     if ((st = plain_search(cells, ts, spec, opts, lim, mu, sig))) return st;
     for (size_t q = 0; q < spec.size(); q++) out[spec[q]] = lim[q];
   }
-  if (serial.empty()) return MC_OK;
+  if (serial.empty()) return finish_counts(out, n, serial);
   // the one-trial-per-group kernel runs the serial search; a prior has nothing to guide
   TaskSet ss;
   ss.cells = ts.cells;
@@ -794,7 +825,7 @@ int mc_bisect_prior_batch(const mc_cell* cells, int n, const mc_bisect_opts* opt
   std::vector<mcd::ResultDev> res;
   if ((st = run_tasks(ss, res, nullptr, 0))) return st;
   for (size_t q = 0; q < serial.size(); q++) to_limit(res[q], out[serial[q]]);
-  return MC_OK;
+  return finish_counts(out, n, serial);
 }
 
 int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* opts,
@@ -807,6 +838,7 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
   TaskSet ts;
   if ((st = prepare(cells, n, ts))) return st;
   g_ctx.last_ms = 0;
+  g_ctx.last_exec = g_ctx.last_serial = 0;
   std::vector<int> spec, serial;
   split_cells(cells, ts, n, opts->aMax, spec, serial);
   if (!spec.empty()) {
@@ -854,7 +886,7 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
       if (fell_back) fell_back[i] = 1;
     }
   }
-  if (serial.empty()) return MC_OK;
+  if (serial.empty()) return finish_counts(out, n, serial);
   TaskSet ss;
   ss.cells = ts.cells;
   for (int i : serial) {
@@ -876,7 +908,7 @@ int mc_bisect_bracket_batch(const mc_cell* cells, int n, const mc_bisect_opts* o
     to_limit(res[q], out[i]);
     if (fell_back) fell_back[i] = ss.tasks[q].mode == mcd::TASK_BISECT ? 1 : res[q].fell_back;
   }
-  return MC_OK;
+  return finish_counts(out, n, serial);
 }
 
 int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps, double* Fx, double* Fy,
@@ -887,6 +919,7 @@ int mc_engine_trace(const mc_cell* cell, double ap, int64_t nsteps, double* Fx, 
   TaskSet ts;
   if ((st = prepare(cell, 1, ts))) return st;
   g_ctx.last_ms = 0;
+  g_ctx.last_exec = g_ctx.last_serial = 0;
   const mcd::CellDev& c = ts.cells[0];
   const int64_t total = (int64_t)c.spt * c.N * c.revs;
   const int64_t ns = std::min(nsteps, total);
@@ -944,6 +977,7 @@ int mc_engine_trace_ex(const mc_cell* cell, const mc_engage_spec* eng, int64_t n
   ts.cost = {1.0};
   ts.nzcap = {std::max(1, nz_bound(*cell, e.ap))};
   g_ctx.last_ms = 0;
+  g_ctx.last_exec = g_ctx.last_serial = 0;
   const int64_t ns = std::max<int64_t>(nsteps, 1);
   std::vector<double> buf((size_t)ns * 4);
   std::vector<mcd::ResultDev> res;

@@ -62,6 +62,12 @@ bool buffer_of(napi_env env, napi_value v, void** data, size_t* len) {
   return false;
 }
 
+bool napi_is_exception_pending_bool(napi_env env) {
+  bool p = false;
+  napi_is_exception_pending(env, &p);
+  return p;
+}
+
 double num(napi_env env, napi_value v) {
   double d = 0;
   napi_get_value_double(env, v, &d);
@@ -691,6 +697,28 @@ napi_value EngineSetEndEngagement(napi_env env, napi_callback_info info) {
 // vx, vy, w, wv, t, decim, hist_i], Kte, Kre) -> Float64Array: the state after the run [x, y,
 // vx, vy, w, wv, t, runMeanW, wk, wnx, wny, stepCount, pr, decim, nz, hist_i, hist_count, ts_i,
 // ts_count, ws_i, ws_count, wsp_i, wsp_count], the 23 stats, the number of chip events
+// the engine's state, statistics and chip-event count: [x, y, vx, vy, w, wv, t, runMeanW, wk,
+// wnx, wny, stepCount, pr, decim, nz, hist_i, hist_count, ts_i, ts_count, ws_i, ws_count,
+// wsp_i, wsp_count, the 23 mc_engine_stats fields, events]
+napi_value engine_snapshot(napi_env env, mc_engine* e) {
+  mc_engine_state s;
+  mc_engine_stats sts;
+  int st = mc_engine_get_state(e, &s);
+  if (!st) st = mc_engine_read_stats(e, &sts, 0);
+  int64_t nev = 0;
+  if (!st) st = mc_engine_chip_events(e, nullptr, 0, &nev);
+  if (st) return throw_mc(env, st);
+  double v[23 + 23 + 1];
+  const double sv2[23] = {s.x, s.y, s.vx, s.vy, s.w, s.wv, s.t, s.runMeanW, s.wk, s.wnx, s.wny, (double)s.stepCount,
+                          (double)s.pr, (double)s.decim, (double)s.nz, (double)s.hist_i, (double)s.hist_count,
+                          (double)s.ts_i, (double)s.ts_count, (double)s.ws_i, (double)s.ws_count,
+                          (double)s.wsp_i, (double)s.wsp_count};
+  std::memcpy(v, sv2, sizeof sv2);
+  std::memcpy(v + 23, &sts, sizeof sts);
+  v[46] = (double)nev;
+  return f64_array(env, v, 47);
+}
+
 napi_value EngineRun(napi_env env, napi_callback_info info) {
   napi_value argv[7];
   if (!get_args(env, info, 7, argv)) return nullptr;
@@ -715,21 +743,18 @@ napi_value EngineRun(napi_env env, napi_callback_info info) {
   if (!st) st = mc_engine_set_maps(e, (const float*)hb, (const float*)eb);
   if (!st) st = mc_engine_set_wear(e, num(env, argv[5]), num(env, argv[6]));
   if (!st) st = mc_engine_run(e, (int64_t)num(env, argv[1]));
-  mc_engine_stats sts;
-  if (!st) st = mc_engine_get_state(e, &s);
-  if (!st) st = mc_engine_read_stats(e, &sts, 0);
-  int64_t nev = 0;
-  if (!st) st = mc_engine_chip_events(e, nullptr, 0, &nev);
   if (st) return throw_mc(env, st);
-  double v[23 + 23 + 1];
-  const double sv2[23] = {s.x, s.y, s.vx, s.vy, s.w, s.wv, s.t, s.runMeanW, s.wk, s.wnx, s.wny, (double)s.stepCount,
-                          (double)s.pr, (double)s.decim, (double)s.nz, (double)s.hist_i, (double)s.hist_count,
-                          (double)s.ts_i, (double)s.ts_count, (double)s.ws_i, (double)s.ws_count,
-                          (double)s.wsp_i, (double)s.wsp_count};
-  std::memcpy(v, sv2, sizeof sv2);
-  std::memcpy(v + 23, &sts, sizeof sts);
-  v[46] = (double)nev;
-  return f64_array(env, v, 47);
+  return engine_snapshot(env, e);
+}
+
+// engineState(h) -> the Float64Array(47) engineRun returns, without running: the state a
+// device pipeline (simRun) left
+napi_value EngineState(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mc_engine* e = engine_of(env, argv[0]);
+  if (!e) return nullptr;
+  return engine_snapshot(env, e);
 }
 
 // engineTakeStats(h): restart the device statistics (takeStats)
@@ -791,13 +816,265 @@ napi_value EngineRings(napi_env env, napi_callback_info info) {
   return ta;
 }
 
+
+// bisectPriorBatch(cells, aMin, aMax, iters, mu: Float64Array(n), sig: Float64Array(n))
+//   -> ArrayBuffer of mc_limit (the plain search's results; the prior only steers speculation)
+napi_value BisectPriorBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return nullptr;
+  void *cells, *mu, *sig;
+  size_t cb, mb, sb;
+  if (!buffer_of(env, argv[0], &cells, &cb) || !buffer_of(env, argv[4], &mu, &mb) || !buffer_of(env, argv[5], &sig, &sb))
+    return nullptr;
+  const int n = (int)(cb / sizeof(mc_cell));
+  if (mb / 8 < (size_t)n || sb / 8 < (size_t)n) { napi_throw_range_error(env, nullptr, "prior arrays too short"); return nullptr; }
+  mc_bisect_opts o{num(env, argv[1]), num(env, argv[2]), (int32_t)num(env, argv[3]), 0};
+  void* out;
+  napi_value res = new_ab(env, (size_t)n * sizeof(mc_limit) + 1, &out);
+  int st = mc_bisect_prior_batch((const mc_cell*)cells, n, &o, (const double*)mu, (const double*)sig, (mc_limit*)out);
+  if (st) return throw_mc(env, st);
+  return res;
+}
+
+// surrogateFeatures(cells) -> Float32Array(n * MC_SURR_IN)
+napi_value SurrogateFeatures(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  void* cells;
+  size_t cb;
+  if (!buffer_of(env, argv[0], &cells, &cb)) return nullptr;
+  const int n = (int)(cb / sizeof(mc_cell));
+  void* out;
+  napi_value ab = new_ab(env, (size_t)n * MC_SURR_IN * 4 + 4, &out);
+  int st = mc_surrogate_features((const mc_cell*)cells, n, (float*)out);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, (size_t)n * MC_SURR_IN, ab, 0, &ta));
+  return ta;
+}
+
+// ---- the surrogate network: an External wrapping mc_mlp*
+void mlp_finalize(napi_env, void* data, void*) {
+  mc_mlp** h = (mc_mlp**)data;
+  if (h && *h) mc_mlp_destroy(*h);
+  delete h;
+}
+
+mc_mlp* mlp_of(napi_env env, napi_value v) {
+  mc_mlp** h = nullptr;
+  if (napi_get_value_external(env, v, (void**)&h) != napi_ok || !h || !*h) {
+    napi_throw_type_error(env, nullptr, "not a live surrogate handle");
+    return nullptr;
+  }
+  return *h;
+}
+
+// mlpCreate(params: Float32Array(mc_mlp_nparams()), xMean, xStd: Float32Array(MC_SURR_IN), yMean, yStd)
+//   -> handle of a network with those weights and that input / output scaling
+napi_value MlpCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return nullptr;
+  void *pp, *xm, *xs;
+  size_t pb, mb, sb;
+  if (!buffer_of(env, argv[0], &pp, &pb) || !buffer_of(env, argv[1], &xm, &mb) || !buffer_of(env, argv[2], &xs, &sb))
+    return nullptr;
+  if (pb < (size_t)mc_mlp_nparams() * 4 || mb < MC_SURR_IN * 4 || sb < MC_SURR_IN * 4) {
+    napi_throw_range_error(env, nullptr, "surrogate parameters or scaling too short");
+    return nullptr;
+  }
+  mc_mlp* m = nullptr;
+  int st = mc_mlp_create(1, &m);
+  if (!st) st = mc_mlp_params(m, (float*)pp, 1);
+  if (!st) st = mc_mlp_set_scaling(m, (const float*)xm, (const float*)xs, (float)num(env, argv[3]), (float)num(env, argv[4]));
+  if (st) {
+    if (m) mc_mlp_destroy(m);
+    return throw_mc(env, st);
+  }
+  mc_mlp** h = new mc_mlp*(m);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, h, mlp_finalize, nullptr, &ext));
+  return ext;
+}
+
+// mlpPredict(h, X: Float32Array(n * MC_SURR_IN)) -> Float32Array(n): ln alim
+napi_value MlpPredict(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mc_mlp* m = mlp_of(env, argv[0]);
+  if (!m) return nullptr;
+  void* X;
+  size_t xb;
+  if (!buffer_of(env, argv[1], &X, &xb)) return nullptr;
+  const int n = (int)(xb / (4 * MC_SURR_IN));
+  void* out;
+  napi_value ab = new_ab(env, (size_t)n * 4 + 4, &out);
+  int st = mc_mlp_predict(m, (const float*)X, n, (float*)out);
+  if (st) return throw_mc(env, st);
+  napi_value ta;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, (size_t)n, ab, 0, &ta));
+  return ta;
+}
+
+napi_value MlpNparams(napi_env env, napi_callback_info) {
+  napi_value v;
+  NAPI_OK(napi_create_int32(env, mc_mlp_nparams(), &v));
+  return v;
+}
+
+// pathDiscretize(moves: ArrayBuffer of mc_move, maxSeg) -> ArrayBuffer of mc_path_point
+napi_value PathDiscretize(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void* mv;
+  size_t mb;
+  if (!buffer_of(env, argv[0], &mv, &mb)) return nullptr;
+  const int n = (int)(mb / sizeof(mc_move));
+  const double maxSeg = num(env, argv[1]);
+  int np = 0;
+  int st = mc_path_discretize((const mc_move*)mv, n, maxSeg, nullptr, 0, &np);
+  if (st) return throw_mc(env, st);
+  void* out;
+  napi_value ab = new_ab(env, (size_t)np * sizeof(mc_path_point), &out);
+  if (np > 0) st = mc_path_discretize((const mc_move*)mv, n, maxSeg, (mc_path_point*)out, np, &np);
+  if (st) return throw_mc(env, st);
+  return ab;
+}
+
+// simSchedule(pts: ArrayBuffer of mc_path_point, hx, dtReal, speed, feedOverride, flutes, maxFrames)
+//   -> [ArrayBuffer of mc_substep, Int32Array of (first, count) per frame]
+napi_value SimSchedule(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return nullptr;
+  void* pts;
+  size_t pb;
+  if (!buffer_of(env, argv[0], &pts, &pb)) return nullptr;
+  const int np = (int)(pb / sizeof(mc_path_point));
+  const double mf = num(env, argv[6]);
+  mc_schedule_opts o{num(env, argv[1]), num(env, argv[2]), num(env, argv[3]), num(env, argv[4]),
+                     (int32_t)num(env, argv[5]), (int32_t)std::min(mf, 2147483647.0)};
+  int ns = 0, nf = 0;
+  int st = mc_sim_schedule((const mc_path_point*)pts, np, &o, nullptr, 0, &ns, nullptr, 0, &nf);
+  if (st) return throw_mc(env, st);
+  void *sub, *fr;
+  napi_value sab = new_ab(env, (size_t)ns * sizeof(mc_substep), &sub);
+  napi_value fab = new_ab(env, (size_t)nf * 8, &fr);
+  if (ns > 0 || nf > 0)
+    st = mc_sim_schedule((const mc_path_point*)pts, np, &o, (mc_substep*)sub, ns, &ns, o.dt_real > 0 ? (int32_t*)fr : nullptr,
+                         o.dt_real > 0 ? nf : 0, &nf);
+  if (st) return throw_mc(env, st);
+  napi_value fta, arr;
+  NAPI_OK(napi_create_typedarray(env, napi_int32_array, (size_t)nf * 2, fab, 0, &fta));
+  NAPI_OK(napi_create_array_with_length(env, 2, &arr));
+  NAPI_OK(napi_set_element(env, arr, 0, sab));
+  NAPI_OK(napi_set_element(env, arr, 1, fta));
+  return arr;
+}
+
+double prop_num(napi_env env, napi_value o, const char* k, double def) {
+  bool has = false;
+  napi_value v;
+  napi_valuetype t;
+  if (napi_has_named_property(env, o, k, &has) != napi_ok || !has) return def;
+  if (napi_get_named_property(env, o, k, &v) != napi_ok || napi_typeof(env, v, &t) != napi_ok) return def;
+  if (t != napi_number) return def;
+  return num(env, v);
+}
+
+bool prop_val(napi_env env, napi_value o, const char* k, napi_value* v) {
+  bool has = false;
+  napi_valuetype t;
+  if (napi_has_named_property(env, o, k, &has) != napi_ok || !has) return false;
+  if (napi_get_named_property(env, o, k, v) != napi_ok || napi_typeof(env, *v, &t) != napi_ok) return false;
+  return t != napi_undefined && t != napi_null;
+}
+
+// simRun(programs: [{actual, ideal (stock handles), engine (engine handle), sub (ArrayBuffer of
+//   mc_substep), R, fluteLen, rw, share, type, lut (Float32Array, bull-nose), cornerR,
+//   dynBudget, fz, frames (true: the frames' stats and chatter index)}])
+//   -> [[ArrayBuffer of mc_substep_out, ArrayBuffer of mc_frame_out], ...]
+// the programs' sub-step loops in one device call (mc_sim_run)
+napi_value SimRun(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  bool isarr = false;
+  napi_is_array(env, argv[0], &isarr);
+  if (!isarr) { napi_throw_type_error(env, nullptr, "expected an array of programs"); return nullptr; }
+  uint32_t n = 0;
+  NAPI_OK(napi_get_array_length(env, argv[0], &n));
+  std::vector<mc_sim_program> progs(n);
+  std::vector<napi_value> outs(n), frs(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value o, v;
+    NAPI_OK(napi_get_element(env, argv[0], i, &o));
+    mc_sim_program& g = progs[i];
+    std::memset(&g, 0, sizeof g);
+    if (!prop_val(env, o, "actual", &v) || !(g.actual = stock_of(env, v))) return nullptr;
+    if (!prop_val(env, o, "ideal", &v) || !(g.ideal = stock_of(env, v))) return nullptr;
+    if (!prop_val(env, o, "engine", &v) || !(g.engine = engine_of(env, v))) return nullptr;
+    void* sub;
+    size_t sb;
+    if (!prop_val(env, o, "sub", &v) || !buffer_of(env, v, &sub, &sb)) {
+      if (!napi_is_exception_pending_bool(env)) napi_throw_type_error(env, nullptr, "program without sub-steps");
+      return nullptr;
+    }
+    g.sub = (const mc_substep*)sub;
+    g.nsub = (int32_t)prop_num(env, o, "nsub", (double)(sb / sizeof(mc_substep)));
+    if (g.nsub < 0 || (size_t)g.nsub > sb / sizeof(mc_substep)) {
+      napi_throw_range_error(env, nullptr, "nsub outside the sub-step array");
+      return nullptr;
+    }
+    g.R = prop_num(env, o, "R", 0);
+    g.fluteLen = prop_num(env, o, "fluteLen", 0);
+    g.rw = prop_num(env, o, "rw", 0);
+    g.share = prop_num(env, o, "share", 1);
+    g.type = (int32_t)prop_num(env, o, "type", MC_TOOL_FLAT);
+    g.cornerR = prop_num(env, o, "cornerR", 0);
+    g.dynBudget = prop_num(env, o, "dynBudget", 0);
+    g.fz = prop_num(env, o, "fz", 0);
+    if (prop_val(env, o, "lut", &v)) {
+      void* lut;
+      size_t lb;
+      if (!buffer_of(env, v, &lut, &lb)) return nullptr;
+      g.lut = (const float*)lut;
+      g.lutn = (int32_t)(lb / 4);
+    }
+    bool want = false;
+    if (prop_val(env, o, "frames", &v)) napi_get_value_bool(env, v, &want);
+    int nf = 0;
+    if (want)
+      for (int k = 0; k < g.nsub; k++) nf += (g.sub[k].frame & MC_FRAME_LAST) ? 1 : 0;
+    void *op, *fp;
+    outs[i] = new_ab(env, (size_t)g.nsub * sizeof(mc_substep_out), &op);
+    frs[i] = new_ab(env, (size_t)nf * sizeof(mc_frame_out), &fp);
+    if (!outs[i] || !frs[i]) { napi_throw_error(env, nullptr, "out of memory"); return nullptr; }
+    g.out = (mc_substep_out*)op;
+    g.frames = nf ? (mc_frame_out*)fp : nullptr;
+    g.nframes = nf;
+  }
+  int st = mc_sim_run(progs.data(), (int)n);
+  if (st) return throw_mc(env, st);
+  napi_value res;
+  NAPI_OK(napi_create_array_with_length(env, n, &res));
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value pair;
+    NAPI_OK(napi_create_array_with_length(env, 2, &pair));
+    NAPI_OK(napi_set_element(env, pair, 0, outs[i]));
+    NAPI_OK(napi_set_element(env, pair, 1, frs[i]));
+    NAPI_OK(napi_set_element(env, res, i, pair));
+  }
+  return res;
+}
+
 napi_value Sizes(napi_env env, napi_callback_info) {
   napi_value o, v;
   NAPI_OK(napi_create_object(env, &o));
   const struct { const char* k; size_t s; } t[] = {
       {"mc_cell", sizeof(mc_cell)}, {"mc_trial", sizeof(mc_trial)}, {"mc_limit", sizeof(mc_limit)},
       {"mc_stamp", sizeof(mc_stamp)}, {"mc_stamp_opt", sizeof(mc_stamp_opt)},
-      {"mc_engage_query", sizeof(mc_engage_query)}, {"mc_engage_result", sizeof(mc_engage_result)}, {"mc_lobe_args", sizeof(mc_lobe_args)}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
+      {"mc_engage_query", sizeof(mc_engage_query)}, {"mc_engage_result", sizeof(mc_engage_result)}, {"mc_lobe_args", sizeof(mc_lobe_args)},
+      {"mc_move", sizeof(mc_move)}, {"mc_path_point", sizeof(mc_path_point)}, {"mc_substep", sizeof(mc_substep)},
+      {"mc_substep_out", sizeof(mc_substep_out)}, {"mc_frame_out", sizeof(mc_frame_out)},
+      {"MC_SURR_IN", MC_SURR_IN}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
   for (auto& e : t) {
     napi_create_uint32(env, (uint32_t)e.s, &v);
     napi_set_named_property(env, o, e.k, v);
@@ -820,7 +1097,10 @@ napi_value ModuleInit(napi_env env, napi_value exports) {
       {"engineSetEngagement", EngineSetEngagement}, {"engineSetEndEngagement", EngineSetEndEngagement},
       {"engineRun", EngineRun}, {"engineTakeStats", EngineTakeStats}, {"engineChipEvents", EngineChipEvents},
       {"engineChatterIndex", EngineChatterIndex}, {"engineRings", EngineRings},
-      {"compositeLobes", CompositeLobes}, {"sizes", Sizes}};
+      {"compositeLobes", CompositeLobes}, {"engineState", EngineState}, {"bisectPriorBatch", BisectPriorBatch},
+      {"surrogateFeatures", SurrogateFeatures}, {"mlpCreate", MlpCreate}, {"mlpPredict", MlpPredict},
+      {"mlpNparams", MlpNparams}, {"pathDiscretize", PathDiscretize}, {"simSchedule", SimSchedule},
+      {"simRun", SimRun}, {"sizes", Sizes}};
   for (auto& f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok ||

@@ -158,7 +158,12 @@ export class CuttingEngine {
     const mat = this.mat;
     const Kte = this.Kte != null ? this.Kte : mat.Kte, Kre = this.Kre != null ? this.Kre : mat.Kre;
     const set = Float64Array.of(this.x, this.y, this.vx, this.vy, this.w, this.wv, this.t, this.decim, this.hist.i);
-    const v = native().engineRun(this._h, nSteps, this.Hb, this.Eb, set, Kte, Kre);
+    this._sync(native().engineRun(this._h, nSteps, this.Hb, this.Eb, set, Kte, Kre), true);
+  }
+
+  // the host fields from the device's state vector (engineRun / engineState); events: the
+  // run's chip events are appended with the engagement's directions
+  _sync(v, events) {
     this.x = this.x0 = v[0]; this.y = this.y0 = v[1]; this.vx = this.vx0 = v[2]; this.vy = this.vy0 = v[3];
     this.w = v[4]; this.wv = v[5]; this.t = v[6]; this.runMeanW = v[7];
     this.stepCount = v[11]; this.pr = v[12];
@@ -170,7 +175,7 @@ export class CuttingEngine {
     for (let k = 0; k < STATS.length; k++) st[STATS[k]] = v[23 + k];
     this._ringCache = null;
     const nev = v[46];
-    if (nev > 0) {
+    if (events && nev > 0) {
       const ev = native().engineChipEvents(this._h);
       const ux = this.ux, uy = this.uy, vx = this.vx_, vy = this.vy_;
       for (let q = 0; q < nev; q++) this.chipEvents.push({ vol: ev[3 * q], phi: ev[3 * q + 1], z: ev[3 * q + 2], ux, uy, vx, vy });

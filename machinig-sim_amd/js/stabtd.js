@@ -56,3 +56,17 @@ export function timeDomainLimitsBracketed(points, lo, hi, tol, opts) {
   const lim = readLimits(buf, n), fb = new Int32Array(buf, n * 56, n);
   return lim.map((r, i) => Object.assign(asResult(r), { fellBack: fb[i] !== 0 }));
 }
+
+// timeDomainLimits steered by the surrogate: the same results as timeDomainLimits (same
+// trials on the search path, same verdicts, same brackets), with the speculative search
+// ordered by the prior ln alim ~ N(prediction, sigma^2) so that fewer speculative trials run
+// when the prediction is good.  surrogate: a Surrogate (surrogate.js); opts.sigma overrides
+// the spread stored with the model.
+export function timeDomainLimitsGuided(points, surrogate, opts) {
+  const o = optsOf(opts), n = points.length;
+  const cells = packCells(points.map(p => cellOf(p.tool, p.modal, p.mat, p.S, p.arc, p.fz, o.revs, o.nonlinear)));
+  const mu = surrogate.predictCells(cells);
+  const s = opts && opts.sigma !== undefined ? opts.sigma : surrogate.sigma;
+  const sig = new Float64Array(n).fill(s > 0 ? s : 0.3);
+  return readLimits(native().bisectPriorBatch(cells, o.aMin, o.aMax, o.iters, mu, sig), n).map(asResult);
+}

@@ -129,6 +129,23 @@ class Substep(C.Structure):
 MC_FRAME_FIRST, MC_FRAME_LAST = 1, 2
 
 
+class Move(C.Structure):
+    _fields_ = [("from_", C.c_double * 3), ("to", C.c_double * 3), ("F", C.c_double), ("S", C.c_double),
+                ("rapid", C.c_int32), ("line", C.c_int32), ("arc", C.c_int32), ("cw", C.c_int32),
+                ("axes", C.c_int32 * 3), ("has_sweep", C.c_int32), ("ca", C.c_double), ("cb", C.c_double),
+                ("sweep", C.c_double)]
+
+
+class PathPoint(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "F", "S")] + [("rapid", C.c_int32), ("line", C.c_int32),
+                                                                         ("move", C.c_int32), ("pad", C.c_int32)]
+
+
+class ScheduleOpts(C.Structure):
+    _fields_ = [("hx", C.c_double), ("dt_real", C.c_double), ("speed", C.c_double), ("feed_override", C.c_double),
+                ("flutes", C.c_int32), ("max_frames", C.c_int32)]
+
+
 class SubstepOut(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dx", "dy", "ap", "errVal", "vol", "maxF", "phiSt", "phiEx")] + \
                [("nsteps", C.c_int32), ("cut", C.c_int32), ("arc", C.c_int32), ("pad", C.c_int32)]

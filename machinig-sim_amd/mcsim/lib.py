@@ -31,7 +31,8 @@ EXPORTS = [
     "mc_engine_set_wear", "mc_engine_set_work", "mc_engine_set_engagement", "mc_engine_set_end_engagement",
     "mc_engine_set_maps", "mc_engine_get_state", "mc_engine_set_state", "mc_engine_run", "mc_engine_run_batch",
     "mc_engine_read_stats", "mc_engine_chip_events", "mc_engine_chatter_index", "mc_engine_read_rings",
-    "mc_sim_run", "mc_js_math", "mc_stamp_batch_noalloc",
+    "mc_sim_run", "mc_js_math", "mc_stamp_batch_noalloc", "mc_path_discretize", "mc_sim_schedule",
+    "mc_last_search_trials",
 ]
 
 _LIB = None
@@ -115,6 +116,9 @@ def load():
     L.mc_engine_read_rings.argtypes = [P, P, P, P, P]
     L.mc_sim_run.argtypes = [C.POINTER(SimProgram), C.c_int]
     L.mc_js_math.argtypes = [C.c_int32, P, P, P, C.c_int64]
+    L.mc_last_search_trials.argtypes = [P, P]
+    L.mc_path_discretize.argtypes = [P, C.c_int, C.c_double, P, C.c_int, P]
+    L.mc_sim_schedule.argtypes = [P, C.c_int, P, P, C.c_int, P, P, C.c_int, P]
     L.mc_set_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     L.mc_get_search_tuning.argtypes = [C.POINTER(SearchTuning)]
     _LIB = L
@@ -136,6 +140,14 @@ def last_kernel_ms():
     ms = C.c_double()
     _check(load().mc_last_kernel_time(C.byref(ms)))
     return ms.value
+
+
+def last_search_trials():
+    """(executed, serial) trials of the last search call: every trial the speculative kernel
+    ran, and those on the serial bisection path"""
+    ex, se = C.c_int64(), C.c_int64()
+    _check(load().mc_last_search_trials(C.byref(ex), C.byref(se)))
+    return ex.value, se.value
 
 
 def _cells(cells):
@@ -540,6 +552,35 @@ class Mlp:
         out = np.empty(len(X), dtype=np.float32)
         _check(self.L.mc_mlp_predict(self.p, X.ctypes.data, len(X), out.ctypes.data))
         return out
+
+    MAGIC = b"MCSURR01"
+
+    def save(self, path, sigma=0.0):
+        """the model file the JS Surrogate reads: MAGIC, int32 nparams, int32 inputs, float32
+        y_mean, y_std, sigma (the held-out spread of ln alim), 4 zero bytes, then float32
+        x_mean[inputs], x_std[inputs], params[nparams], little-endian"""
+        import struct
+        xm, xs, ym, ys = self.scaling()
+        with open(path, "wb") as f:
+            f.write(self.MAGIC + struct.pack("<iifffi", self.nparams, MC_SURR_IN, ym, ys, float(sigma), 0))
+            for a in (xm, xs, self.params()):
+                f.write(np.ascontiguousarray(a, dtype="<f4").tobytes())
+
+    @classmethod
+    def load(cls, path):
+        """(model, sigma) from a file save() wrote"""
+        import struct
+        raw = open(path, "rb").read()
+        if raw[:8] != cls.MAGIC:
+            raise ValueError("not a surrogate model file")
+        npar, nin, ym, ys, sigma, _ = struct.unpack("<iifffi", raw[8:32])
+        m = cls()
+        if npar != m.nparams or nin != MC_SURR_IN:
+            raise ValueError("model file for another network shape")
+        v = np.frombuffer(raw, dtype="<f4", offset=32)
+        m.set_scaling(v[:nin], v[nin:2 * nin], ym, ys)
+        m.set_params(v[2 * nin:2 * nin + npar])
+        return m, sigma
 
 
 def directional_coeffs(Kr, phiSt, phiEx):

@@ -55,6 +55,8 @@ def lib():
         L.or_engine_set_state.argtypes = [P, C.POINTER(EngineState)]
         L.or_engine_run.argtypes = [P, C.c_int64]
         L.or_engine_stats.argtypes = [P, C.POINTER(EngineStats), C.c_int]
+        L.or_engine_run_max_f.argtypes = [P]
+        L.or_engine_run_max_f.restype = C.c_double
         L.or_engine_chip_events.argtypes = [P, P, C.c_int64]
         L.or_engine_chip_events.restype = C.c_int64
         L.or_engine_chatter_index.argtypes = [P, P]
@@ -250,6 +252,10 @@ class OracleEngine:
         s = EngineStats()
         self.L.or_engine_stats(self.h, C.byref(s), 1 if take else 0)
         return s
+
+    def run_max_f(self):
+        """largest |(Fx, Fy)| over the last run (the sub-step loop's maxF)"""
+        return self.L.or_engine_run_max_f(self.h)
 
     def chip_events(self):
         import numpy as np

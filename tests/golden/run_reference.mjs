@@ -125,6 +125,29 @@ if (cases.lobes) {
 
 // the simulator's engagement sampling, called as Simulator methods on a stand-in holding
 // just the fields they read (engine.Hb, activeTool.R, stock)
+// a long stamp batch read from a binary file (mc_stamp records: cx, cy, tipZ, R, type, line,
+// errVal; 48 bytes, little-endian), applied in order; the answer is each final map's SHA-256
+// and the removed volume (the maps of a 512 x 512 stock are too large to keep as data)
+if (cases.zigzag) {
+  const crypto = await import('crypto');
+  const sha = a => crypto.createHash('sha256').update(Buffer.from(a.buffer, a.byteOffset, a.byteLength)).digest('hex');
+  out.zigzag = cases.zigzag.map(({ w, l, h, cell, stampsFile, n }) => {
+    const s = new Stock(w, l, h, cell), b = fs.readFileSync(stampsFile);
+    const dv = new DataView(b.buffer, b.byteOffset, b.byteLength);
+    const t1 = Date.now();
+    for (let q = 0; q < n; q++) {
+      const o = 48 * q;
+      s.curLine = dv.getInt32(o + 36, true);
+      s.stamp(dv.getFloat64(o, true), dv.getFloat64(o + 8, true), dv.getFloat64(o + 16, true), dv.getFloat64(o + 24, true),
+        dv.getInt32(o + 32, true) === 1 ? 'ball' : 'flat', dv.getFloat64(o + 40, true), null, null);
+    }
+    const ms = Date.now() - t1;
+    return { nx: s.nx, ny: s.ny, n, H: sha(s.H), err: sha(s.err), wallD: sha(s.wallD), lineCut: sha(s.lineCut),
+      lineWall: sha(s.lineWall), markS: sha(s.markS), markF: sha(s.markF), removed: s.removed, node_ms: ms };
+  });
+  console.error('zigzag', (Date.now() - t0) / 1e3, 's');
+}
+
 if (cases.engage) {
   const { Simulator } = await import(REF + 'sim.js');
   out.engage = cases.engage.map(({ stock: si, queries }) => {

@@ -1,6 +1,6 @@
 // Drives the JavaScript drop-in modules (machinig-sim_amd/js) on the golden cases written
 // from the reference and checks they return the reference's answers.
-// usage: node js_dropin_check.mjs [lobes|gpu]
+// usage: node js_dropin_check.mjs [lobes|gpu [surrogate model file]]
 import fs from 'fs';
 import zlib from 'zlib';
 import path from 'path';
@@ -16,6 +16,39 @@ const num = v => (v === 'inf' ? Infinity : v === '-inf' ? -Infinity : v);
 let fails = 0;
 const check = (ok, what) => { if (!ok) { fails++; console.log('FAIL', what); } };
 const sameBits = (a, b) => a.length === b.length && a.every((v, i) => Object.is(v, b[i]));
+const zdec = (s, T) => { const b = zlib.inflateSync(Buffer.from(s, 'base64')); return new T(b.buffer.slice(b.byteOffset, b.byteOffset + b.length)); };
+const col = (L, k, T = Float64Array) => T.from(L.map(p => p[k]));
+
+// the recorded run's polyline through simSchedule (every sub-step, unbounded budget)
+function schedule(m, res) {
+  const P = res.path, X = f64(P.x), Y = f64(P.y), Z = f64(P.z);
+  const pts = Array.from(X, (x, i) => ({ x, y: Y[i], z: Z[i], rapid: P.rapid[i], F: P.F[i], S: P.S[i], line: P.line[i] }));
+  return m.simSchedule(pts, { hx: res.hx, N: res.toolN });
+}
+
+function clobesCheck(m, what) {
+    // Simulator.lobes with this build's modules: modalNow from modesAt on the recorded modal,
+  // chain and softening; the composite curve from compositeLobes, the diagram at rest (and
+  // a speed-independent tip) from stabilityLobes of modalNow(0)
+  const cl = load('clobes');
+  const z64 = s => { const b = zlib.inflateSync(Buffer.from(s, 'base64')); return new Float64Array(b.buffer, b.byteOffset, b.length / 8); };
+  cl.cases.forEach((c, k) => {
+    const res = cl.results[k], soft = res.soft === null ? undefined : res.soft, md = res.modal;
+    const modalNow = S => { const ms = m.modesAt(md, S, res.chain, soft); return { k: md.k, fn: md.fn, zeta: md.zeta, modesX: ms.x, modesY: ms.y }; };
+    c.arcs.forEach(([phiSt, phiEx, staticOnly], a) => {
+      const base = { N: res.N, Ktc: res.Ktc, Krc: res.Krc, phiSt, phiEx, lobes: 30, rpmMin: 2000, rpmMax: 40000 };
+      const dep = !staticOnly && m.speedDependent(md, soft);
+      if (dep !== (what === 'gpu')) return;   // the composite curve is device work
+      const got = dep ? m.compositeLobes(base, modalNow) : m.stabilityLobes(Object.assign({}, modalNow(0), base));
+      const want = res.runs[a].lobes;
+      check(got.length === want.length, 'composite lobe count ' + k + '/' + a);
+      got.forEach((L, j) => {
+        check(sameBits(Float64Array.from(L.map(p => p.rpm)), z64(want[j].rpm)), 'composite rpm ' + k + '/' + a + '/' + j);
+        check(sameBits(Float64Array.from(L.map(p => p.alim)), z64(want[j].alim)), 'composite alim ' + k + '/' + a + '/' + j);
+      });
+    });
+  });
+}
 
 async function main() {
   const m = await import(mod);
@@ -33,26 +66,7 @@ async function main() {
       const d = m.directionalCoeffs(c.params.Krc / c.params.Ktc, c.params.phiSt, c.params.phiEx);
       check(sameBits([d.xx, d.xy, d.yx, d.yy], ref.dc), 'dc ' + k);
     });
-    // Simulator.lobes with this build's modules: modalNow from modesAt on the recorded modal,
-    // chain and softening; the composite curve from compositeLobes, the diagram at rest (and
-    // a speed-independent tip) from stabilityLobes of modalNow(0)
-    const cl = load('clobes');
-    const z64 = s => { const b = zlib.inflateSync(Buffer.from(s, 'base64')); return new Float64Array(b.buffer, b.byteOffset, b.length / 8); };
-    cl.cases.forEach((c, k) => {
-      const res = cl.results[k], soft = res.soft === null ? undefined : res.soft, md = res.modal;
-      const modalNow = S => { const ms = m.modesAt(md, S, res.chain, soft); return { k: md.k, fn: md.fn, zeta: md.zeta, modesX: ms.x, modesY: ms.y }; };
-      c.arcs.forEach(([phiSt, phiEx, staticOnly], a) => {
-        const base = { N: res.N, Ktc: res.Ktc, Krc: res.Krc, phiSt, phiEx, lobes: 30, rpmMin: 2000, rpmMax: 40000 };
-        const dep = !staticOnly && m.speedDependent(md, soft);
-        const got = dep ? m.compositeLobes(base, modalNow) : m.stabilityLobes(Object.assign({}, modalNow(0), base));
-        const want = res.runs[a].lobes;
-        check(got.length === want.length, 'composite lobe count ' + k + '/' + a);
-        got.forEach((L, j) => {
-          check(sameBits(Float64Array.from(L.map(p => p.rpm)), z64(want[j].rpm)), 'composite rpm ' + k + '/' + a + '/' + j);
-          check(sameBits(Float64Array.from(L.map(p => p.alim)), z64(want[j].alim)), 'composite alim ' + k + '/' + a + '/' + j);
-        });
-      });
-    });
+    clobesCheck(m, what);
     const tl = load('tiplut');
     tl.cases.forEach((t, k) => check(sameBits(m.tipLut(t), f32(tl.results[k])), 'tipLut ' + k));
     check(m.LUT_DR === 0.02, 'LUT_DR');
@@ -82,8 +96,28 @@ async function main() {
         check(want === null ? im === null : im !== null && im.st === want[0] && im.ex === want[1], 'immersion ' + k + '/' + i);
       });
     });
+    // the path format and the sub-step schedule (host C++ behind the addon): discretize on the
+    // reference parser's moves, simSchedule on the recorded simulator runs' polylines
+    const gp = load('paths');
+    const obj = v => ({ x: v[0], y: v[1], z: v[2] });
+    gp.cases.forEach((c, k) => {
+      const moves = gp.results[k].moves.map(mv => Object.assign({}, mv, { from: obj(mv.from), to: obj(mv.to) }));
+      c.maxSegs.forEach((ms, j) => {
+        const pts = m.discretize(moves, ms), ref = gp.results[k].pts[j];
+        check(sameBits(col(pts, 'x'), f64(ref.x)) && sameBits(col(pts, 'y'), f64(ref.y)) && sameBits(col(pts, 'z'), f64(ref.z)) &&
+          pts.every((p, i) => p.rapid === !!ref.rapid[i] && p.line === ref.line[i] && p.F === ref.F[i] && p.S === ref.S[i]),
+        'discretize ' + k + '/' + j);
+      });
+    });
+    const sx = load('simx');
+    sx.results.forEach((res, k) => {
+      const sub = m.readSubsteps(schedule(m, res).sub), ide = res.ideal;
+      check(sub.length === ide.type.length && sameBits(col(sub, 'x'), f64(ide.cx)) && sameBits(col(sub, 'y'), f64(ide.cy)) &&
+        sameBits(col(sub, 'z'), f64(ide.tipZ)) && sub.every((q, i) => q.line === ide.line[i]), 'simSchedule ' + k);
+    });
   } else {
     console.log('device', m.native().deviceInfo());
+    clobesCheck(m, what);
     const ch = load('chatter');
     ch.cases.forEach((c, k) => {
       const x = c.cell;
@@ -171,7 +205,6 @@ async function main() {
     });
     // the CuttingEngine class over the simulator's recorded call sequences (linear force law,
     // flat and ball tools): state, stats, chip events, takeStats, chatterIndex and rings
-    const zdec = (s, T) => { const b = zlib.inflateSync(Buffer.from(s, 'base64')); return new T(b.buffer.slice(b.byteOffset, b.byteOffset + b.length)); };
     const eq = (a, b) => a === b || (a !== a && b !== b);
     const STATS = ['n', 'sumF', 'maxF', 'sumFx', 'sumFy', 'sumFz', 'maxFz', 'sumT', 'maxT', 'sumX', 'sumY', 'minX',
       'maxX', 'minY', 'maxY', 'sumZc', 'wZc', 'sumW', 'maxW', 'sumHb', 'sumB', 'sumRb', 'engTeeth'];
@@ -242,6 +275,52 @@ async function main() {
     const c0 = td.cases[0].cell;
     await handle({ key: 7, points: [{ S: c0.S, modal: c0.modal }], tool: c0.tool, mat: c0.mat, arc: c0.arc, fz: c0.fz, nonlinear: c0.nonlinear }, r => msgs.push(r));
     check(msgs.length === 2 && msgs[1].done === true && msgs[0].key === 7, 'stabworker protocol');
+    // the surrogate-guided search gives the plain search's answers whatever the prior
+    if (process.argv[3]) {
+      const sg = m.Surrogate.load(process.argv[3]);
+      for (const key of Object.keys(groups)) {
+        const ks = groups[key], c0 = td.cases[ks[0]];
+        const o = Object.assign({}, c0.opts, { revs: c0.cell.revs, nonlinear: c0.cell.nonlinear });
+        const res = m.timeDomainLimitsGuided(ks.map(k => byOpts[k]), sg, o);
+        res.forEach((r, i) => {
+          const ref = td.results[ks[i]];
+          check(r.alim === ref.alim && r.bracket[0] === ref.lo && r.bracket[1] === num(ref.hi), 'timeDomainLimitsGuided ' + ks[i]);
+        });
+      }
+      check(sg.predict([byOpts[0]], { revs: c0.revs, nonlinear: c0.nonlinear }).every(Number.isFinite), 'surrogate predict');
+    }
+    // the simulator's sub-step loop through simRun: the recorded rigid-workpiece runs (flat,
+    // ball, bull-nose) in one call, their deflected stamps, vibration values, step counts and
+    // both stocks' maps as the reference's
+    const sx = load('simx'), progs = [];
+    sx.cases.forEach((c, k) => {
+      const res = sx.results[k], st = c.stock;
+      const act = new m.Stock(st.w, st.l, st.h, c.cell), ide = new m.Stock(st.w, st.l, st.h, c.cell);
+      ide.wallBand = 2.0;
+      const [, N, R, hd, typ, fl, nl, ro, rph, S, fnMax, mx, my, mt] = res.cfgs[0];
+      const e = new m.CuttingEngine(), modes = L => L.map(q => ({ k: q[0], fn: q[1], zeta: q[2] }));
+      e.configure({ N, R, helixDeg: hd, type: typ, fluteLen: fl, nonlinear: nl === 1, runout: ro, runoutPhase: rph },
+        { k: mx[0][0], fn: mx[0][1], zeta: mx[0][2], modesX: modes(mx), modesY: modes(my) },
+        { Ktc: mt[0], Krc: mt[1], Kte: mt[2], Kre: mt[3], Kac: mt[4], Kae: mt[5], mc: mt[6], Ci: mt[7] }, S, fnMax);
+      e.setWear({ Kte: res.wears[0][1], Kre: res.wears[0][2] });
+      const tool = { R: res.toolR, fluteLen: res.fluteLen, type: res.type };
+      if (res.type === 'bull') Object.assign(tool, { cornerR: c.tool.cornerR, lut: m.tipLut({ type: 'bull', D: 2 * res.toolR, cornerR: c.tool.cornerR }) });
+      progs.push({ stock: act, ideal: ide, engine: e, schedule: schedule(m, res), tool, share: e.stepsPerTooth / 4 });
+    });
+    const ran = m.simRun(progs);
+    ran.forEach((r, k) => {
+      const res = sx.results[k], p = progs[k], sub = m.readSubsteps(p.schedule.sub), a = res.actual;
+      const runs = new Map(res.runs.map(([i, n]) => [i, n]));
+      check(sameBits(Float64Array.from(sub, (q, i) => q.x + r.out[i].dx), f64(a.cx)) &&
+        sameBits(Float64Array.from(sub, (q, i) => q.y + r.out[i].dy), f64(a.cy)), 'simRun stamps ' + k);
+      check(sameBits(col(r.out, 'errVal'), f64(a.errVal)), 'simRun errVal ' + k);
+      check(r.out.every((o, i) => o.nsteps === (runs.get(i) || 0)), 'simRun steps ' + k);
+      check(sameBits(p.stock.H, zdec(res.stock.H, Float32Array)) && sameBits(p.ideal.H, zdec(res.commanded.H, Float32Array)),
+        'simRun maps ' + k);
+      check(Math.abs(p.stock.removed - res.stock.removed) <= 1e-12 * Math.max(1, res.stock.removed), 'simRun removed ' + k);
+      check(p.engine.t > 0 && p.engine.stepCount > 0, 'simRun engine state ' + k);
+      p.stock.dispose(); p.ideal.dispose();
+    });
   }
   console.log(fails ? fails + ' failures' : 'all ' + what + ' checks passed');
   process.exit(fails ? 1 : 0);

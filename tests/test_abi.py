@@ -46,6 +46,8 @@ int main(void) {
          sizeof(mc_sim_program), offsetof(mc_sim_program, cornerR));
   printf("%zu %zu %zu %zu\n", sizeof(mc_frame_out), offsetof(mc_frame_out, end), offsetof(mc_substep, guard),
          offsetof(mc_sim_program, frames));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(mc_move), offsetof(mc_move, axes), offsetof(mc_move, sweep),
+         sizeof(mc_path_point), sizeof(mc_schedule_opts));
   return 0;
 }'''
     import tempfile
@@ -66,4 +68,6 @@ int main(void) {
     want += [C.sizeof(_abi.Substep), C.sizeof(_abi.SubstepOut), _abi.SubstepOut.arc.offset, C.sizeof(_abi.SimProgram),
              _abi.SimProgram.cornerR.offset]
     want += [C.sizeof(_abi.FrameOut), _abi.FrameOut.end.offset, _abi.Substep.guard.offset, _abi.SimProgram.frames.offset]
+    want += [C.sizeof(_abi.Move), _abi.Move.axes.offset, _abi.Move.sweep.offset, C.sizeof(_abi.PathPoint),
+             C.sizeof(_abi.ScheduleOpts)]
     assert [int(x) for x in out] == want

@@ -61,3 +61,48 @@ def test_sim_pipeline_matches_reference_runs():
         removed = sum(o.vol for o in out)
         assert abs(removed - res["stock"]["removed"]) <= 1e-12 * max(1.0, res["stock"]["removed"]), case["tool"]
         assert any(o.maxF > 0 for o in out)
+
+
+def test_sim_pipeline_matches_oracle_per_substep():
+    """every field the pipeline reports per sub-step against the oracle's loop
+    (oracle/simloop.c) on the same recorded programs: deflection, axial depth, vibration
+    value, the run's largest |(Fx, Fy)| (maxF), the displayed arc, step count and cut flag
+    bit for bit; the stamp's removed volume within 1e-12 relative (the device sums the
+    stamp's nodes in another order).  The oracle's maxF is pinned on the reference's
+    statistics (test_oracle_golden.py), its volumes on the reference's stamps."""
+    import oracle
+    from mcsim import lib
+    g = G.load("simx")
+    progs, orc = [], []
+    for case, res in zip(g["cases"], g["results"]):
+        st = case["stock"]
+        cell, fnMax = G.simx_cell(res)
+        _, kte, kre = res["wears"][0]
+        sub, n = G.simx_program(res)
+        act, ide = lib.Stock(st["w"], st["l"], st["h"], case["cell"]), lib.Stock(st["w"], st["l"], st["h"], case["cell"])
+        ide.set_wall_band(2.0)
+        eng = lib.Engine()
+        info = eng.configure(cell, fnMax)
+        eng.set_wear(kte, kre)
+        typ, lut, rc = G.simx_tool(case, res, lib.tip_lut)
+        progs.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=n, R=res["toolR"], fluteLen=res["fluteLen"],
+                          rw=0.0, share=info.stepsPerTooth / 4, type=typ, lut=lut, cornerR=rc))
+        oa, oi = oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"]), oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"])
+        oi.set_wall_band(2.0)
+        oe = oracle.OracleEngine()
+        oe.configure(cell, fnMax)
+        oe.set_wear(kte, kre)
+        typ_o, lut_o, _ = G.simx_tool(case, res, oracle.tip_lut)
+        orc.append(oracle.sim_run(oa, oi, oe, sub, n, res["toolR"], res["fluteLen"], 0.0, info.stepsPerTooth / 4,
+                                  typ_o, lut_o, rc))
+    outs = lib.sim_run(progs)
+    cuts = 0
+    for (case, res), out, want in zip(zip(g["cases"], g["results"]), outs, orc):
+        assert len(out) == len(want)
+        for k, (a, b) in enumerate(zip(out, want)):
+            assert (a.dx, a.dy, a.ap, a.errVal, a.maxF, a.phiSt, a.phiEx, a.nsteps, a.cut, a.arc) == \
+                   (b.dx, b.dy, b.ap, b.errVal, b.maxF, b.phiSt, b.phiEx, b.nsteps, b.cut, b.arc), (case["tool"], k)
+            assert abs(a.vol - b.vol) <= 1e-12 * max(1.0, abs(b.vol)), (case["tool"], k)
+            cuts += a.cut
+        assert np.array_equal([o.vol for o in want], G.f64(res["actual"]["vol"]))
+    assert cuts > 0

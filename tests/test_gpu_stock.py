@@ -250,3 +250,28 @@ def test_simulator_run_replayed(mc):
         np.testing.assert_allclose(ivol, ivol_ref, rtol=1e-12, atol=1e-12)
         s.close()
         ideal.close()
+
+
+def test_full_zigzag_batch_matches_reference(mc):
+    """the whole heightmap workload (the 5,000-segment zigzag, about 2.4 million flat stamps
+    on the 512 x 512 stock) in one batch: every final map is the reference's Stock.stamp
+    result byte for byte (SHA-256 of each map against tests/golden/golden_zigzag.json, which
+    the reference wrote), and the removed volume equals its within 1e-12 relative"""
+    import hashlib
+    import json
+    import os
+    from mcsim import path, workloads as W
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden_zigzag.json")))
+    rec, _ = path.schedule(W.zigzag_segments(5000), 0.1)
+    assert len(rec) == g["n"]
+    s = mc.Stock(51.1, 51.1, 20, 0.1)
+    assert (s.nx, s.ny) == (g["nx"], g["ny"])
+    vol = s.stamps(path.as_stamp_array(rec))
+    err, wallD, lineCut, lineWall = s.side()
+    markS, markF = s.marks()
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    got = dict(H=sha(s.H()), err=sha(err), wallD=sha(wallD), lineCut=sha(lineCut), lineWall=sha(lineWall),
+               markS=sha(markS), markF=sha(markF))
+    assert got == {k: g[k] for k in got}
+    assert abs(float(np.sum(vol)) - g["removed"]) <= 1e-12 * g["removed"]
+    s.close()

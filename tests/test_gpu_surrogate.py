@@ -129,3 +129,41 @@ def test_prior_guided_search_is_identical_to_plain(lib):
             a, b = got[i], plain[i]
             assert (a.alim, a.lo, a.hi, a.open, a.trials, a.steps, a.visits, a.engaged) == \
                    (b.alim, b.lo, b.hi, b.open, b.trials, b.steps, b.visits, b.engaged), i
+
+
+def test_prior_saves_speculative_trials(lib):
+    """a good prior (the exact answer within its spread) makes the speculative search run
+    fewer trials than the prior-free search over the same cells, while the serial path, and
+    so every result, is the same; a confidently wrong prior costs trials instead"""
+    from mcsim import workloads as W
+    cells, opts = W.sld_grid(n_speeds=32, n_tools=16, seed=7)
+    plain = lib.bisect_batch(cells, **opts)
+    ex_plain, se_plain = lib.last_search_trials()
+    n = len(cells)
+    exact = np.array([math.log(r.alim) for r in plain])
+    lib.prior_batch(cells, exact, np.full(n, 0.05), **opts)
+    ex_good, se_good = lib.last_search_trials()
+    lib.prior_batch(cells, np.full(n, math.log(9.5)), np.full(n, 0.01), **opts)
+    ex_bad, se_bad = lib.last_search_trials()
+    print("executed trials: plain %d, good prior %d, wrong prior %d (serial %d)" % (ex_plain, ex_good, ex_bad, se_plain))
+    assert se_plain == se_good == se_bad == sum(r.trials for r in plain)
+    assert ex_plain >= se_plain and ex_good >= se_good
+    assert ex_good < ex_plain
+    assert ex_bad > ex_good
+
+
+def test_model_file_round_trip(tmp_path):
+    """Mlp.save / Mlp.load (the file the JS Surrogate reads): weights, scaling and sigma come
+    back, and the predictions are the same bits"""
+    from mcsim import lib, workloads as W
+    m = lib.Mlp(5)
+    xm = np.linspace(-1, 1, lib.MC_SURR_IN).astype(np.float32)
+    m.set_scaling(xm, np.full(lib.MC_SURR_IN, 2.0, np.float32), 0.25, 1.5)
+    cells, _ = W.sld_grid(n_speeds=8, n_tools=4)
+    X = lib.surrogate_features(cells)
+    path = str(tmp_path / "m.bin")
+    m.save(path, sigma=0.37)
+    m2, sigma = lib.Mlp.load(path)
+    assert abs(sigma - 0.37) < 1e-7
+    assert np.array_equal(m2.params(), m.params())
+    assert np.array_equal(m2.predict(X), m.predict(X))

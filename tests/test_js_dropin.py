@@ -1,5 +1,6 @@
 """The JavaScript drop-in modules (machinig-sim_amd/js: stabtd.js, sld.js, stock.js,
-stabworker.js over the N-API addon) return the reference's answers on the golden cases."""
+cutting.js, path.js, simrun.js, surrogate.js, stabworker.js over the N-API addon) return the
+reference's answers on the golden cases."""
 import os
 import shutil
 import subprocess
@@ -11,8 +12,8 @@ NODE = shutil.which("node")
 needs_node = pytest.mark.skipif(NODE is None, reason="node not installed")
 
 
-def run(what, timeout):
-    r = subprocess.run([NODE, os.path.join(HERE, "js_dropin_check.mjs"), what], capture_output=True,
+def run(what, timeout, *extra):
+    r = subprocess.run([NODE, os.path.join(HERE, "js_dropin_check.mjs"), what, *extra], capture_output=True,
                        text=True, timeout=timeout)
     print(r.stdout[-3000:], r.stderr[-3000:])
     return r
@@ -26,6 +27,12 @@ def test_lobes_dropin_matches_reference():
 
 @needs_node
 @pytest.mark.gpu
-def test_gpu_dropin_matches_reference():
-    r = run("gpu", 280)
+def test_gpu_dropin_matches_reference(tmp_path):
+    """the device entry points through the JS modules, and timeDomainLimitsGuided with a
+    surrogate model file (an untrained network: the guided search's answers do not depend
+    on the prior)"""
+    from mcsim import lib
+    model = str(tmp_path / "surrogate.bin")
+    lib.Mlp(3).save(model, sigma=0.4)
+    r = run("gpu", 280, model)
     assert r.returncode == 0 and "all gpu checks passed" in r.stdout

@@ -301,3 +301,73 @@ def test_engine_object_mistakes_are_caught():
             op[1] *= 1.01
     bad, _ = G.replay_englog(oracle.OracleEngine(), dict(res, ops=ops))
     assert bad
+
+
+def test_run_max_force_pinned_by_engine_stats():
+    """the oracle's per-run largest |(Fx, Fy)| (the sub-step loop's maxF) against the
+    reference's statistics over the recorded simulator call sequences: after every run
+    stats.maxF = Math.max(its value before the run, the run's maximum) (src/cutting.js:280-281),
+    and a run that starts from fresh statistics (after configure or takeStats) sets it alone"""
+    import oracle
+
+    class Probe:
+        def __init__(self):
+            self.e = oracle.OracleEngine()
+            self.checked = self.fresh = 0
+
+        def __getattr__(self, k):
+            return getattr(self.e, k)
+
+        def run(self, n):
+            before = self.e.stats().maxF
+            self.e.run(n)
+            m, after = self.e.run_max_f(), self.e.stats().maxF
+            assert after == max(before, m)
+            assert m <= after
+            if before == 0:
+                assert m == after
+                self.fresh += 1
+            self.checked += 1
+
+    g = G.load("englog")
+    fresh = 0
+    for case, res in zip(g["cases"], g["results"]):
+        if case.get("nonlinear", True) is not False or case["tool"]["type"] == "bull":
+            continue
+        p = Probe()
+        bad, runs = G.replay_englog(p, res, max_runs=300)
+        assert not bad and runs > 0 and p.checked == runs
+        fresh += p.fresh
+    assert fresh > 0
+
+
+def test_bracket_limit_pinned_by_reference_brackets():
+    """the oracle's bracketed search (no reference counterpart) against timeDomainLimit's
+    recorded answers (src/stabtd.js:24-32): handed the reference's own final bracket (its
+    low end a depth the reference found stable, its high end one it found chattering) and a
+    tolerance the bracket already meets, it confirms both ends (two trials) and answers the
+    reference's alim; with a tighter tolerance its bracket nests inside the reference's; a
+    hint whose low end chatters, or an invalid hint, falls back to the plain search and
+    answers exactly what timeDomainLimit answered"""
+    g = G.load("tdl")
+    seen = 0
+    for case, ref in zip(g["cases"], g["results"]):
+        o = case["opts"]
+        cell = G.cell_from_json(case["cell"])
+        lo, hi = ref["lo"], G.num(ref["hi"])
+        kw = dict(aMin=o["aMin"], aMax=o["aMax"], iters=o["iters"])
+        plain = oracle.time_domain_limit(cell, o["aMin"], o["aMax"], o["iters"])
+        if hi == float("inf") or lo == o["aMin"]:
+            continue   # open at aMax, or no stable depth tried: the bracket ends are not both verdicts
+        r, fb = oracle.bracket_limit(cell, lo, hi, hi - lo, **kw)
+        assert fb == 0 and r.trials == 2 and (r.alim, r.lo, r.hi, r.open) == (ref["alim"], lo, hi, 0)
+        r, fb = oracle.bracket_limit(cell, lo, hi, (hi - lo) / 4, **kw)
+        assert fb == 0 and lo <= r.lo < r.hi <= hi and r.hi - r.lo <= (hi - lo) / 4 and r.trials > 2
+        r, fb = oracle.bracket_limit(cell, hi, min(o["aMax"], 2 * hi), 1e-3, **kw)
+        assert fb == 1 and (r.alim, r.lo, r.hi) == (ref["alim"], lo, hi) and r.trials == plain.trials + 1
+        r, fb = oracle.bracket_limit(cell, hi, lo, 1e-3, **kw)
+        assert fb == 1 and (r.alim, r.lo, r.hi) == (ref["alim"], lo, hi) and r.trials == plain.trials
+        seen += 1
+        if seen == 6:
+            break
+    assert seen == 6

@@ -68,3 +68,46 @@ def test_oracle_sim_loop_matches_reference_runs():
                 assert np.array_equal(fn(), G.zf32(ref[key]), equal_nan=True), (case["tool"], name, key)
             for key, fn in (("lineCut", s_.lineCut), ("lineWall", s_.lineWall)):
                 assert np.array_equal(fn(), G.zi32(ref[key])), (case["tool"], name, key)
+
+
+def test_schedule_abi_checks():
+    """mc_sim_schedule and mc_path_discretize at the C ABI: size queries, short outputs and
+    bad options are reported; a path of one point has no sub-steps; max_frames stops the
+    frame loop; frames tile the sub-steps in order"""
+    import ctypes as C
+    from mcsim import lib, path
+    from mcsim._abi import ScheduleOpts, Substep
+    L = lib.load()
+    pts, n = path.path_points([0.0, 10.0, 10.0], [0.0, 0.0, 5.0], [1.0, -1.0, -1.0], [1, 0, 0], [1, 2, 3],
+                              [800.0, 800.0, 600.0], [8000.0, 8000.0, 8000.0])
+    ns, nf = C.c_int(-1), C.c_int(-1)
+    o = ScheduleOpts(0.1, 1 / 60, 1.0, 1.0, 3, 1000)
+    assert L.mc_sim_schedule(pts, n, C.byref(o), None, 0, C.byref(ns), None, 0, C.byref(nf)) == 0
+    assert ns.value > 0 and nf.value > 0
+    arr = (Substep * ns.value)()
+    fr = (C.c_int32 * (2 * nf.value))()
+    assert L.mc_sim_schedule(pts, n, C.byref(o), arr, ns.value - 1, C.byref(ns), fr, nf.value, C.byref(nf)) != 0
+    assert L.mc_sim_schedule(pts, n, C.byref(o), arr, ns.value, C.byref(ns), fr, nf.value - 1, C.byref(nf)) != 0
+    assert L.mc_sim_schedule(pts, n, C.byref(o), arr, ns.value, C.byref(ns), fr, nf.value, C.byref(nf)) == 0
+    nxt = 0
+    for f in range(nf.value):
+        assert fr[2 * f] == nxt
+        nxt += fr[2 * f + 1]
+    assert nxt == ns.value
+    for bad in (ScheduleOpts(0.0, 0.0, 1.0, 1.0, 3, 0), ScheduleOpts(0.1, -1.0, 1.0, 1.0, 3, 0),
+                ScheduleOpts(0.1, 0.01, 0.0, 1.0, 3, 10), ScheduleOpts(0.1, 0.0, 1.0, 1.0, 0, 0)):
+        assert L.mc_sim_schedule(pts, n, C.byref(bad), None, 0, C.byref(ns), None, 0, C.byref(nf)) != 0
+    o2 = ScheduleOpts(0.1, 1 / 60, 1.0, 1.0, 3, 3)
+    assert L.mc_sim_schedule(pts, n, C.byref(o2), None, 0, C.byref(ns), None, 0, C.byref(nf)) == 0 and nf.value == 3
+    assert L.mc_sim_schedule(pts, 1, C.byref(o), None, 0, C.byref(ns), None, 0, C.byref(nf)) == 0
+    assert ns.value == 0 and nf.value == 0
+    assert path.sim_substeps([1.0], [2.0], [3.0], [1], [1], [0.0], [0.0], 2, 0.1)[1] == 0
+    assert path.discretize([]) == dict(x=[], y=[], z=[], rapid=[], F=[], S=[], line=[])
+    mv = dict(to=[1.0, 0.0, 0.0], F=100.0, S=1000.0, rapid=0, line=1, arc=dict(ca=0.0, cb=0.0, cw=0, axes=["x", "x", "z"]))
+    mv["from"] = [0.0, 1.0, 0.0]
+    arr = path.moves_array([mv])
+    assert L.mc_path_discretize(arr, 1, 0.5, None, 0, C.byref(ns)) != 0
+    mv["arc"]["axes"] = ["x", "y", "z"]
+    arr = path.moves_array([mv])
+    assert L.mc_path_discretize(arr, 1, 0.0, None, 0, C.byref(ns)) != 0
+    assert L.mc_path_discretize(arr, 1, 0.5, None, 0, C.byref(ns)) == 0 and ns.value > 2
