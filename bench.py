@@ -123,37 +123,109 @@ def profiled_traffic(kernel):
         return None
 
 
-def heightmap(steps):
+def map_digests(stock):
+    """SHA-256 of each map of a library stock (the form tests/golden/golden_zigzag.json holds)"""
+    import hashlib
+    err, wallD, lineCut, lineWall = stock.side()
+    markS, markF = stock.marks()
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    return dict(H=sha(stock.H()), err=sha(err), wallD=sha(wallD), lineCut=sha(lineCut), lineWall=sha(lineWall),
+                markS=sha(markS), markF=sha(markF))
+
+
+def heightmap(steps, cpu=True, cpu_seconds=10.0):
+    """Config 1: the seeded 5,000-segment zigzag (flat R5 / ball R3 layers) on the 512 x 512
+    stock at 0.1 mm, stamped every min(hx, 0.4) * 0.6 mm along the path (2.41 M stamps), in
+    one batch with the config's outputs: the final maps and the removed volume per segment
+    (mc_stamp_batch_lines).  Checked against the reference's own answer for the whole batch
+    (the SHA-256 of every final map and the total removed volume, tests/golden/golden_zigzag.json).
+
+    Roofline (HBM): the algorithmic bytes are those any implementation must move once: the
+    stamp records in (48 B each), the five touched maps in and out (H, err, wallD, lineCut,
+    lineWall: 40 B per node) and the per-segment volumes out.  The reference's own loop reads
+    and writes 8 B per bounding-box node and stamp (bbox_bytes); the tiles' exact culling
+    skips most of those, so that figure is reported beside the roofline, not as it.
+    cpu_baseline: the oracle (oracle/stock.c) stamping a prefix of the same batch on one host
+    thread for about cpu_seconds, its maps compared with the GPU's on the same prefix; the
+    reference's Node time for the whole batch (measured where the fixture was written) is
+    reported beside it."""
+    import json as _json
     segs = W.zigzag_segments(5000)
     rec, _ = path.schedule(segs, 0.1)
     arr = path.as_stamp_array(rec)
     n = len(rec)
-    # algorithmic bytes: each stamp reads and writes the f32 height of every node of its
-    # bounding box once (src/stock.js:132-155)
+    nlines = int(rec["line"].max()) + 1
+    nodes = 512 * 512
+    alg_bytes = 48.0 * n + 40.0 * nodes + 8.0 * nlines
     hx = 0.1
     Rw = rec["R"] + 2 * hx
     i0 = np.maximum(0, np.floor((rec["cx"] - Rw) / hx)); i1 = np.minimum(511, np.ceil((rec["cx"] + Rw) / hx))
     j0 = np.maximum(0, np.floor((rec["cy"] - Rw) / hx)); j1 = np.minimum(511, np.ceil((rec["cy"] + Rw) / hx))
-    nodes = np.where((i1 >= i0) & (j1 >= j0), (i1 - i0 + 1) * (j1 - j0 + 1), 0).sum()
-    alg_bytes = 8.0 * nodes
+    bbox_bytes = 8.0 * np.where((i1 >= i0) & (j1 >= j0), (i1 - i0 + 1) * (j1 - j0 + 1), 0).sum()
     times, kms = [], []
+    lv = dg = None
     for _ in range(steps + 1):
         s = lib.Stock(51.1, 51.1, 20, 0.1)
         t0 = time.perf_counter()
-        s.stamps(arr, want_vol=True)
+        lv = s.stamps_lines(arr, nlines)
         times.append(time.perf_counter() - t0)
         kms.append(lib.last_kernel_ms())
+        dg = map_digests(s)
         s.close()
     wall = float(np.mean(times[1:]))
     k = float(np.mean(kms[1:]))
+    # the drop-in path: the same batch with a volume per stamp (what Stock.stamp returns)
+    s = lib.Stock(51.1, 51.1, 20, 0.1)
+    t0 = time.perf_counter()
+    vs = s.stamps(arr)
+    t_stamp = time.perf_counter() - t0
+    k_stamp = lib.last_kernel_ms()
+    dg_stamp = map_digests(s)
+    s.close()
+    per_line = np.bincount(rec["line"], weights=vs, minlength=nlines)
+    line_rel = float(np.max(np.abs(lv - per_line) / np.maximum(1e-30, np.abs(per_line))))
+    with open(os.path.join(ROOT, "tests", "golden", "golden_zigzag.json")) as f:
+        g = _json.load(f)
+    ident_ref = all(dg[key] == g[key] and dg_stamp[key] == g[key] for key in dg)
+    vol_rel = abs(float(lv.sum()) - g["removed"]) / g["removed"]
     gbps = alg_bytes / (k * 1e-3) / 1e9
-    return dict(workload="zigzag_512x512_5000seg", stamps=n, ms_per_batch=wall * 1e3,
-                kernel_ms=k, stamps_per_s=n / wall,
-                roofline={"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": gbps / HBM_PEAK_GBS, "traffic": profiled_traffic("stamp_tiles"),
-                          "kernel": "stamp_tiles"},
-                note="algorithmic bytes = 8 B x bbox nodes per stamp (f32 H read + write); "
-                     "tiles stay in LDS across the stamps of a chunk")
+    out = dict(workload="zigzag_512x512_5000seg", stamps=n, segments=nlines - 1, ms_per_batch=wall * 1e3,
+               kernel_ms=k, stamps_per_s=n / wall, gb_per_s=gbps,
+               per_stamp_volumes={"ms_per_batch": t_stamp * 1e3, "kernel_ms": k_stamp, "stamps_per_s": n / t_stamp},
+               roofline={"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbps / HBM_PEAK_GBS, "traffic": profiled_traffic("stamp_tiles"),
+                         "kernel": "stamp_tiles", "alg_bytes": alg_bytes,
+                         "alg_def": "48 B per stamp record in + 40 B per node (H, err, wallD, lineCut, lineWall "
+                                    "in and out) + 8 B per segment volume out",
+                         "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"},
+               bbox_bytes=bbox_bytes, bbox_gb_per_s=bbox_bytes / (k * 1e-3) / 1e9,
+               identical_to_reference=ident_ref, removed_rel_diff_vs_reference=vol_rel,
+               segment_volumes_rel_diff_vs_per_stamp=line_rel,
+               reference_node={"stamps_per_s": n / (g["node_ms"] * 1e-3), "seconds": g["node_ms"] * 1e-3,
+                               "sample": "the whole batch by the reference's Stock.stamp under Node 12, one thread, "
+                                         "in the build container (tests/golden/golden_zigzag.json)"})
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        o = oracle.OracleStock(51.1, 51.1, 20, 0.1)
+        done, step, t0 = 0, 20000, time.perf_counter()
+        while done < n and time.perf_counter() - t0 < cpu_seconds:
+            b = min(n, done + step)
+            o.stamps(path.as_stamp_array(rec[done:b]), b - done)
+            done = b
+        cw = time.perf_counter() - t0
+        s = lib.Stock(51.1, 51.1, 20, 0.1)
+        s.stamps_lines(path.as_stamp_array(rec[:done]), nlines)
+        same = bool(np.array_equal(s.H(), o.H()) and np.array_equal(s.side()[0], o.err()) and
+                    np.array_equal(s.side()[2], o.lineCut()))
+        s.close()
+        rate = done / cw
+        out["cpu_baseline"] = dict(value=rate, unit="stamps/s", cores=1, kind="port",
+                                   gb_per_s=rate * alg_bytes / n / 1e9,
+                                   sample="the first %d of the %d stamps on one host thread (oracle/stock.c), %.1f s"
+                                          % (done, n, cw), identical_to_gpu=same)
+        out["speedup_vs_cpu"] = out["stamps_per_s"] / rate
+    return out
 
 
 def line_cells(progs, outs, frs, tools, revs=40):
@@ -383,6 +455,8 @@ def main():
     ap.add_argument("--coupled-lines", type=int, default=4)
     ap.add_argument("--train-cells", type=int, default=262144)
     ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--heightmap-only", action="store_true",
+                    help="only the heightmap leg (for its profiles): one JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
     a = ap.parse_args()
@@ -393,6 +467,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if a.heightmap_only:
+        print(json.dumps(heightmap(a.steps, not a.no_cpu, a.cpu_seconds)))
+        return
     cells, opts = W.sld_grid(n_speeds=a.speeds, n_tools=a.tools)
     arr = cell_array(cells)
     print("device:", lib.device_info(), file=sys.stderr, flush=True)
@@ -473,7 +550,7 @@ def main():
                                        "copy-calibrated)"},
     }
     if not a.no_heightmap:
-        out["heightmap"] = heightmap(2)
+        out["heightmap"] = heightmap(2, not a.no_cpu)
     if not a.no_coupled:
         out["coupled"] = coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu, opts, model)
     if not a.no_cpu:

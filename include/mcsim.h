@@ -340,6 +340,13 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
                       const float* luts, const int32_t* lut_off, const int32_t* lut_len,
                       int nluts, double* vol);
 
+/* mc_stamp_batch with the removed volume summed per program line: line_vol[l] = the sum of
+ * what Stock.stamp returns (src/stock.js:162) over the batch's stamps with line l, 0 for a
+ * line without stamps; every stamp's line must lie in [0, nlines).  This is the simulator's
+ * per-segment removed volume (the stamps of one path segment carry its line), summed on the
+ * device; its last bits depend on the summation order. */
+int mc_stamp_batch_lines(mc_stock* s, const mc_stamp* stamps, int n, double* line_vol, int nlines);
+
 /* mc_stamp_batch without any device allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory during a call (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The per-stamp records live in a

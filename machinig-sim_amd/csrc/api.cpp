@@ -35,10 +35,10 @@ size_t mcs_stampx_bytes();
 size_t mcs_cull_bytes();
 int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, const float* luts,
                       const int32_t* lut_off, const int32_t* lut_len, const float* lut_min,
-                      void* d_stampdev, void* d_stampx, void* d_cull, void* d_box, int32_t* touched, int nx, int ny,
+                      void* d_stampdev, void* d_stampx, void* d_cull, void* d_box, int nx, int ny,
                       double hx, double hy, double zTop, double zB, double band, float* H,
                       float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, float* markS,
-                      float* markF, int32_t* rect, double* vol, hipStream_t stream, hipEvent_t ev0,
+                      float* markF, int32_t* rect, double* vol, bool vol_lines, hipStream_t stream, hipEvent_t ev0,
                       hipEvent_t ev1);
 int mcs_launch_set_rect(int32_t* rect, hipStream_t stream);
 int mcs_launch_engagement(const float* H, int nx, int ny, double hx, double hy,
@@ -52,8 +52,7 @@ int mcs_launch_sample(const float* H, int nx, int ny, double hx, double hy, cons
                       int n, double* d_out, hipStream_t stream);
 int mcs_launch_fill(float* p, size_t n, float v, hipStream_t stream);
 int mcs_scratch_setup(int device);
-int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull, void** box, int32_t** touched,
-                        double** vol);
+int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull, void** box, double** vol);
 
 namespace {
 
@@ -630,7 +629,7 @@ struct StockImpl {
   int32_t* rect = nullptr;   // Stock.dirty {i0, j0, i1, j1} on the device
   float* bmax = nullptr;     // block maxima, valid while !bmax_stale
   bool bmax_stale = true;
-  DevBuf stamps, opt, sdev, sx, cull, box, touched, vol, xy, smp, lut, lutmeta, q, qout, eq, ehb, eout, strip;
+  DevBuf stamps, opt, sdev, sx, cull, box, vol, xy, smp, lut, lutmeta, q, qout, eq, ehb, eout, strip;
 };
 
 }  // namespace
@@ -1032,7 +1031,7 @@ int mc_stock_destroy(mc_stock* s) {
   for (void* p : {(void*)s->H, (void*)s->err, (void*)s->wallD, (void*)s->lineCut, (void*)s->lineWall,
                   (void*)s->markS, (void*)s->markF, (void*)s->rect, (void*)s->bmax})
     if (p) (void)hipFree(p);
-  for (DevBuf* b : {&s->stamps, &s->opt, &s->sdev, &s->sx, &s->cull, &s->box, &s->touched, &s->vol, &s->xy,
+  for (DevBuf* b : {&s->stamps, &s->opt, &s->sdev, &s->sx, &s->cull, &s->box, &s->vol, &s->xy,
                     &s->smp, &s->lut, &s->lutmeta, &s->q, &s->qout, &s->eq, &s->ehb, &s->eout, &s->strip})
     if (b->p) (void)hipFree(b->p);
   delete s;
@@ -1081,11 +1080,21 @@ int mc_stamp_batch(mc_stock* s, const mc_stamp* stamps, int n, double* vol) {
   return mc_stamp_batch_ex(s, stamps, nullptr, n, nullptr, nullptr, nullptr, 0, vol);
 }
 
-int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* opt, int n,
-                      const float* luts, const int32_t* lut_off, const int32_t* lut_len, int nluts,
-                      double* vol) {
-  if (!s || n < 0 || (n > 0 && !stamps) || nluts < 0 || (nluts > 0 && (!luts || !lut_off || !lut_len)))
+}  // extern "C"
+
+namespace {
+// the stamp batch: vol receives the removed volume per stamp (nlines 0) or per program line
+// (nlines > 0 entries, every stamp's line in [0, nlines))
+int stamp_batch(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* opt, int n, const float* luts,
+                const int32_t* lut_off, const int32_t* lut_len, int nluts, double* vol, int nlines) {
+  if (!s || n < 0 || (n > 0 && !stamps) || nluts < 0 || (nluts > 0 && (!luts || !lut_off || !lut_len)) || nlines < 0)
     return fail(MC_ERR_ARG, "bad stamp batch");
+  if (nlines > 0) {
+    if (!vol) return fail(MC_ERR_ARG, "per-line volumes need an output");
+    for (int i = 0; i < n; i++)
+      if (stamps[i].line < 0 || stamps[i].line >= nlines)
+        return fail(MC_ERR_ARG, "stamp %d: line %d outside [0, %d)", i, stamps[i].line, nlines);
+  }
   // profile tables: offsets and lengths in range, finite heights, and their minima (the
   // lowest reach of a profiled stamp)
   std::vector<float> lmin((size_t)nluts);
@@ -1118,8 +1127,8 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
   if ((st = s->sdev.ensure((size_t)n * mcs_stampdev_bytes()))) return st;
   if ((st = s->cull.ensure((size_t)n * mcs_cull_bytes()))) return st;
   if ((st = s->box.ensure((size_t)chunks * mcs_chunkbox_bytes()))) return st;
-  if ((st = s->touched.ensure((size_t)n * 4))) return st;
-  if (vol && (st = s->vol.ensure((size_t)n * sizeof(double)))) return st;
+  const size_t nvol = nlines > 0 ? (size_t)nlines : (size_t)n;
+  if (vol && (st = s->vol.ensure(nvol * sizeof(double)))) return st;
   if (opt) {
     if ((st = s->opt.ensure((size_t)n * sizeof(mc_stamp_opt)))) return st;
     if ((st = s->sx.ensure((size_t)n * mcs_stampx_bytes()))) return st;
@@ -1143,16 +1152,16 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
       HIPCHK(hipMemcpyAsync(d_min, lmin.data(), (size_t)nluts * 4, hipMemcpyHostToDevice, q));
     }
   }
-  if (vol && mcs_launch_fill((float*)s->vol.p, (size_t)n * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
+  if (vol && mcs_launch_fill((float*)s->vol.p, nvol * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
   if (mcs_launch_stamps((const mc_stamp*)s->stamps.p, opt ? (const mc_stamp_opt*)s->opt.p : nullptr, n,
                         opt ? (const float*)s->lut.p : nullptr, d_off, d_len, d_min, s->sdev.p,
-                        opt ? s->sx.p : nullptr, s->cull.p, s->box.p, (int32_t*)s->touched.p, s->nx, s->ny, s->hx,
+                        opt ? s->sx.p : nullptr, s->cull.p, s->box.p, s->nx, s->ny, s->hx,
                         s->hy, s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut, s->lineWall,
-                        s->markS, s->markF, s->rect, vol ? (double*)s->vol.p : nullptr, q, g_ctx.ev0,
+                        s->markS, s->markF, s->rect, vol ? (double*)s->vol.p : nullptr, nlines > 0, q, g_ctx.ev0,
                         g_ctx.ev1))
     return fail(MC_ERR_HIP, "stamp kernels failed: %s", hipGetErrorString(hipGetLastError()));
   s->bmax_stale = true;
-  if (vol) HIPCHK(hipMemcpyAsync(vol, s->vol.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, q));
+  if (vol) HIPCHK(hipMemcpyAsync(vol, s->vol.p, nvol * sizeof(double), hipMemcpyDeviceToHost, q));
   HIPCHK(hipStreamSynchronize(q));
   {
     float ms = 0;
@@ -1161,9 +1170,23 @@ int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* o
   }
   if (vol) {
     const double area = s->hx * s->hy;
-    for (int i = 0; i < n; i++) vol[i] *= area;
+    for (size_t i = 0; i < nvol; i++) vol[i] *= area;
   }
   return MC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mc_stamp_batch_ex(mc_stock* s, const mc_stamp* stamps, const mc_stamp_opt* opt, int n,
+                      const float* luts, const int32_t* lut_off, const int32_t* lut_len, int nluts,
+                      double* vol) {
+  return stamp_batch(s, stamps, opt, n, luts, lut_off, lut_len, nluts, vol, 0);
+}
+
+int mc_stamp_batch_lines(mc_stock* s, const mc_stamp* stamps, int n, double* line_vol, int nlines) {
+  if (nlines < 1) return fail(MC_ERR_ARG, "need nlines >= 1");
+  return stamp_batch(s, stamps, nullptr, n, nullptr, nullptr, nullptr, 0, line_vol, nlines);
 }
 
 int mc_stamp_batch_noalloc(mc_stock* s, const mc_stamp* stamps, int n, double* vol) {
@@ -1180,19 +1203,19 @@ int mc_stamp_batch_noalloc(mc_stock* s, const mc_stamp* stamps, int n, double* v
   if (n == 0) return MC_OK;
   mc_stamp* d_in;
   void *d_dev, *d_cull, *d_box;
-  int32_t* d_touched;
   double* d_vol;
-  if (mcs_scratch_regions(g_ctx.dev, &d_in, &d_dev, &d_cull, &d_box, &d_touched, &d_vol))
+  if (mcs_scratch_regions(g_ctx.dev, &d_in, &d_dev, &d_cull, &d_box, &d_vol))
     return fail(MC_ERR_NODEV, "the library is not initialised on a device");
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   hipStream_t q = g_ctx.stream;
   HIPCHK(hipMemcpyAsync(d_in, stamps, (size_t)n * sizeof(mc_stamp), hipMemcpyHostToDevice, q));
   // the volumes start at zero (a fill kernel: no memset node if a caller captures this);
-  // stamp_prep rewrites the records, boxes and touched flags of the n stamps
+  // stamp_prep rewrites the records and boxes of the n stamps
   if (vol && mcs_launch_fill((float*)d_vol, (size_t)n * 2, 0.0f, q)) return fail(MC_ERR_HIP, "fill failed");
-  if (mcs_launch_stamps(d_in, nullptr, n, nullptr, nullptr, nullptr, nullptr, d_dev, nullptr, d_cull, d_box, d_touched,
+  if (mcs_launch_stamps(d_in, nullptr, n, nullptr, nullptr, nullptr, nullptr, d_dev, nullptr, d_cull, d_box,
                         s->nx, s->ny, s->hx, s->hy, s->zTop, s->zB, s->band, s->H, s->err, s->wallD, s->lineCut,
-                        s->lineWall, s->markS, s->markF, s->rect, vol ? d_vol : nullptr, q, g_ctx.ev0, g_ctx.ev1))
+                        s->lineWall, s->markS, s->markF, s->rect, vol ? d_vol : nullptr, false, q, g_ctx.ev0,
+                        g_ctx.ev1))
     return fail(MC_ERR_HIP, "stamp kernels failed: %s", hipGetErrorString(hipGetLastError()));
   s->bmax_stale = true;
   if (vol) HIPCHK(hipMemcpyAsync(vol, d_vol, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, q));

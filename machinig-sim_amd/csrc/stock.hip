@@ -21,10 +21,16 @@
 //                and each wave skips the stamps at or above its own highest node: exact,
 //                since those stamps would change nothing.  The EX variant adds tip
 //                profiles and feed marks.
-//   dirty_rect   union of the boxes of the stamps that changed a node (Stock.dirty).
 //
-// Removed volume per stamp (the value stamp() returns) is summed with one FP64 atomic per
-// wavefront and stamp, so its last bits depend on the summation order.
+// Stock.dirty (src/stock.js:156-160), the union of the boxes of the stamps that changed a
+// node, is kept per wavefront while it walks the stamps and folded into the stock's rectangle
+// by four integer atomics per wavefront at the end (min / max: exact in any order).
+//
+// Removed volume: each wavefront's share of a stamp goes into an LDS partial of the tile
+// (one slot per compacted stamp of the chunk), and the tile adds its partials to the output
+// once per chunk: per stamp (the value stamp() returns), or per program line (the
+// simulator's per-segment removed volume; each wave folds its 64 slots of the chunk line by
+// line, so the stamps of one line cost one global atomic per wave and chunk).  The FP64 sums' last bits depend on the summation order.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -63,8 +69,7 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
                            const int32_t* __restrict__ lut_len, const float* __restrict__ lut_min,
                            int n, double hx, double hy, int nx, int ny, double zTop, double band,
                            StampDev* __restrict__ out, StampX* __restrict__ outx,
-                           int4* __restrict__ cull, float* __restrict__ cull_z,
-                           int32_t* __restrict__ touched, ChunkBox* __restrict__ box) {
+                           int4* __restrict__ cull, float* __restrict__ cull_z, ChunkBox* __restrict__ box) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ int bi0[256], bi1[256], bj0[256], bj1[256];
   __shared__ float bz[256];
@@ -109,7 +114,6 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
     cull_z[s] = __double2float_rd(d.zmin);
     zlow = cull_z[s];
     if (outx) outx[s] = x;
-    touched[s] = 0;
     if (d.i0 <= d.i1 && d.j0 <= d.j1) { i0 = d.i0; i1 = d.i1; j0 = d.j0; j1 = d.j1; }
   }
   bi0[threadIdx.x] = i0; bi1[threadIdx.x] = i1; bj0[threadIdx.x] = j0; bj1[threadIdx.x] = j1;
@@ -127,15 +131,17 @@ __global__ void stamp_prep(const mc_stamp* __restrict__ in, const mc_stamp_opt* 
   if (threadIdx.x == 0) box[blockIdx.x] = ChunkBox{bi0[0], bi1[0], bj0[0], bj1[0], bz[0], {0, 0, 0}};
 }
 
-// EX: tip profiles (luts) and feed marks (markS / markF) are live in this batch
-template <bool EX>
+// EX: tip profiles (luts) and feed marks (markS / markF) are live in this batch; LINES: vol
+// is indexed by the stamps' program line instead of their index
+template <bool EX, bool LINES>
 __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
     const StampDev* __restrict__ st, const StampX* __restrict__ stx, const int4* __restrict__ cull,
     const float* __restrict__ cull_z, const ChunkBox* __restrict__ box, int n, int nx, int ny, double hx, double hy, double zB, float* __restrict__ H,
     float* __restrict__ err, float* __restrict__ wallD, int32_t* __restrict__ lineCut,
     int32_t* __restrict__ lineWall, float* __restrict__ markS, float* __restrict__ markF,
-    const float* __restrict__ luts, int32_t* __restrict__ touched, double* __restrict__ vol) {
+    const float* __restrict__ luts, int32_t* __restrict__ rect, double* __restrict__ vol) {
   __shared__ StampDev list[CHUNK];
+  __shared__ double cvol[CHUNK];   // the tile's removed height per compacted stamp of the chunk
   __shared__ StampX listx[EX ? CHUNK : 1];
   const int t = threadIdx.x;
   const int ti0 = blockIdx.x * TILE, tj0 = blockIdx.y * TILE;
@@ -145,6 +151,7 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
   const int k = j * nx + i;
   float h = 0, e = 0, wd = 0, ms = 0, mfv = 0;
   int lc = 0, lw = 0;
+  int ri0 = 0x7fffffff, rj0 = 0x7fffffff, ri1 = -1, rj1 = -1;   // boxes of the stamps this wave changed
   if (own) {
     h = H[k]; e = err[k]; wd = wallD[k]; lc = lineCut[k]; lw = lineWall[k];
     if (EX) { ms = markS[k]; mfv = markF[k]; }
@@ -228,6 +235,7 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
         list[slot] = st[s];
         if (EX) listx[slot] = stx[s];
       }
+      cvol[t] = 0.0;
       __syncthreads();
       for (int q = 0; q < total; q++) {
         const StampDev& p = list[q];
@@ -270,11 +278,36 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
             }
           }
         }
-        if (__any(changed) && lane == 0) touched[p.index] = 1;
+        if (__any(changed)) {   // wave-uniform: the stamp's box joins the dirty rectangle
+          ri0 = min(ri0, p.i0); rj0 = min(rj0, p.j0); ri1 = max(ri1, p.i1); rj1 = max(rj1, p.j1);
+        }
         if (vol && __any(removed != 0.0)) {
           double r = removed;
           for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
-          if (lane == 0 && r != 0.0) atomicAdd(&vol[p.index], r);
+          if (lane == 0 && r != 0.0) atomicAdd(&cvol[q], r);
+        }
+      }
+      __syncthreads();
+      // the tile's partial volumes of this chunk, added to the output once
+      if (vol) {
+        if (!LINES) {
+          if (t < total && cvol[t] != 0.0) atomicAdd(&vol[list[t].index], cvol[t]);
+        } else {
+          // each wave folds its 64 slots line by line: one atomic per distinct line (the
+          // stamps of a chunk mostly share one or two)
+          bool live = t < total && cvol[t] != 0.0;
+          const int L = live ? list[t].line : -1;
+          const double v = live ? cvol[t] : 0.0;
+          for (unsigned long long m = __builtin_amdgcn_ballot_w64(live); m != 0;
+               m = __builtin_amdgcn_ballot_w64(live)) {   // wave-uniform
+            const int first = __builtin_ctzll(m);
+            const int Lf = __shfl(L, first, 64);
+            const bool mine = live && L == Lf;
+            double r = mine ? v : 0.0;
+            for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+            if (lane == first) atomicAdd(&vol[Lf], r);
+            live = live && !mine;
+          }
         }
       }
       __syncthreads();
@@ -285,22 +318,9 @@ __global__ __launch_bounds__(TILE * TILE) void stamp_tiles(
     H[k] = h; err[k] = e; wallD[k] = wd; lineCut[k] = lc; lineWall[k] = lw;
     if (EX) { markS[k] = ms; markF[k] = mfv; }
   }
-}
-
-// Stock.dirty (src/stock.js:156-160): union of the boxes of the stamps that changed a node,
-// folded into rect = {i0, j0, i1, j1}
-__global__ void dirty_rect(const StampDev* __restrict__ st, const int32_t* __restrict__ touched,
-                           int n, int32_t* __restrict__ rect) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  int i0 = 0x7fffffff, j0 = 0x7fffffff, i1 = -1, j1 = -1;
-  if (s < n && touched[s]) { i0 = st[s].i0; j0 = st[s].j0; i1 = st[s].i1; j1 = st[s].j1; }
-  for (int o = 32; o > 0; o >>= 1) {
-    i0 = min(i0, __shfl_xor(i0, o, 64)); j0 = min(j0, __shfl_xor(j0, o, 64));
-    i1 = max(i1, __shfl_xor(i1, o, 64)); j1 = max(j1, __shfl_xor(j1, o, 64));
-  }
-  if ((threadIdx.x & 63) == 0 && i1 >= 0) {
-    atomicMin(&rect[0], i0); atomicMin(&rect[1], j0);
-    atomicMax(&rect[2], i1); atomicMax(&rect[3], j1);
+  if (lane == 0 && ri1 >= 0) {
+    atomicMin(&rect[0], ri0); atomicMin(&rect[1], rj0);
+    atomicMax(&rect[2], ri1); atomicMax(&rect[3], rj1);
   }
 }
 
@@ -476,8 +496,8 @@ __global__ void fill_f32(float* p, size_t n, float v) {
 }  // namespace mcs
 
 // ---- scratch of the no-allocation stamp batch (mc_stamp_batch_noalloc)
-// The uploaded stamps, their expanded and culling records, the chunk boxes, the touched
-// flags and the per-stamp volumes of at most MC_NOALLOC_MAX_STAMPS stamps, in one file-scope
+// The uploaded stamps, their expanded and culling records, the chunk boxes and the per-stamp
+// volumes of at most MC_NOALLOC_MAX_STAMPS stamps, in one file-scope
 // device array.  Each device the library runs on holds its own copy until the process ends;
 // on the host the array's name is only a stand-in, so kernels get the address that
 // hipGetSymbolAddress returns for the current device (resolved once per device at set-up).
@@ -488,8 +508,7 @@ constexpr size_t NA_CHUNKS = (NA_MAX + CHUNK - 1) / CHUNK;
 constexpr size_t NA_OFF_DEV = align256(sizeof(mc_stamp) * NA_MAX);
 constexpr size_t NA_OFF_CULL = align256(NA_OFF_DEV + sizeof(StampDev) * NA_MAX);
 constexpr size_t NA_OFF_BOX = align256(NA_OFF_CULL + (sizeof(int4) + sizeof(float)) * NA_MAX);
-constexpr size_t NA_OFF_TOUCHED = align256(NA_OFF_BOX + sizeof(ChunkBox) * NA_CHUNKS);
-constexpr size_t NA_OFF_VOL = align256(NA_OFF_TOUCHED + sizeof(int32_t) * NA_MAX);
+constexpr size_t NA_OFF_VOL = align256(NA_OFF_BOX + sizeof(ChunkBox) * NA_CHUNKS);
 constexpr size_t NA_BYTES = align256(NA_OFF_VOL + sizeof(double) * NA_MAX);
 constexpr int NA_DEVICES = 64;
 static_assert(NA_BYTES <= ((size_t)MC_NOALLOC_SCRATCH_MB << 20), "scratch larger than the documented size");
@@ -514,8 +533,7 @@ int mcs_scratch_setup(int device) {
 }
 
 // the regions of the device's scratch (null when set-up has not resolved it)
-int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull, void** box, int32_t** touched,
-                        double** vol) {
+int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull, void** box, double** vol) {
   if (device < 0 || device >= mcs::NA_DEVICES) return -1;
   unsigned char* p;
   {
@@ -527,7 +545,6 @@ int mcs_scratch_regions(int device, mc_stamp** in, void** stampdev, void** cull,
   *stampdev = p + mcs::NA_OFF_DEV;
   *cull = p + mcs::NA_OFF_CULL;
   *box = p + mcs::NA_OFF_BOX;
-  *touched = (int32_t*)(p + mcs::NA_OFF_TOUCHED);
   *vol = (double*)(p + mcs::NA_OFF_VOL);
   return 0;
 }
@@ -543,10 +560,10 @@ int mcs_chunk() { return mcs::CHUNK; }
 
 int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, const float* luts,
                       const int32_t* lut_off, const int32_t* lut_len, const float* lut_min,
-                      void* d_stampdev, void* d_stampx, void* d_cull, void* d_box, int32_t* touched, int nx, int ny,
+                      void* d_stampdev, void* d_stampx, void* d_cull, void* d_box, int nx, int ny,
                       double hx, double hy, double zTop, double zB, double band, float* H,
                       float* err, float* wallD, int32_t* lineCut, int32_t* lineWall, float* markS,
-                      float* markF, int32_t* rect, double* vol, hipStream_t stream, hipEvent_t ev0,
+                      float* markF, int32_t* rect, double* vol, bool vol_lines, hipStream_t stream, hipEvent_t ev0,
                       hipEvent_t ev1) {
   if (n <= 0) return 0;
   const int chunks = (n + mcs::CHUNK - 1) / mcs::CHUNK;
@@ -554,24 +571,16 @@ int mcs_launch_stamps(const mc_stamp* d_in, const mc_stamp_opt* d_opt, int n, co
   hipLaunchKernelGGL(mcs::stamp_prep, dim3(chunks), dim3(mcs::CHUNK), 0, stream, d_in, d_opt, luts,
                      lut_off, lut_len, lut_min, n, hx, hy, nx, ny, zTop, band,
                      (mcs::StampDev*)d_stampdev, ex ? (mcs::StampX*)d_stampx : nullptr, (int4*)d_cull,
-                     (float*)((int4*)d_cull + n), touched, (mcs::ChunkBox*)d_box);
+                     (float*)((int4*)d_cull + n), (mcs::ChunkBox*)d_box);
   const dim3 grid((nx + mcs::TILE - 1) / mcs::TILE, (ny + mcs::TILE - 1) / mcs::TILE);
   if (ev0 && hipEventRecord(ev0, stream) != hipSuccess) return -2;
-  if (ex)
-    hipLaunchKernelGGL(mcs::stamp_tiles<true>, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
-                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)d_stampx, (const int4*)d_cull,
-                       (const float*)((int4*)d_cull + n), (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy,
-                       zB, H, err, wallD, lineCut,
-                       lineWall, markS, markF, luts, touched, vol);
-  else
-    hipLaunchKernelGGL(mcs::stamp_tiles<false>, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
-                       (const mcs::StampDev*)d_stampdev, (const mcs::StampX*)nullptr, (const int4*)d_cull,
-                       (const float*)((int4*)d_cull + n), (const mcs::ChunkBox*)d_box, n, nx, ny, hx, hy,
-                       zB, H, err, wallD, lineCut,
-                       lineWall, markS, markF, luts, touched, vol);
+  auto kern = ex ? (vol_lines ? mcs::stamp_tiles<true, true> : mcs::stamp_tiles<true, false>)
+                 : (vol_lines ? mcs::stamp_tiles<false, true> : mcs::stamp_tiles<false, false>);
+  hipLaunchKernelGGL(kern, grid, dim3(mcs::TILE * mcs::TILE), 0, stream,
+                     (const mcs::StampDev*)d_stampdev, ex ? (const mcs::StampX*)d_stampx : (const mcs::StampX*)nullptr,
+                     (const int4*)d_cull, (const float*)((int4*)d_cull + n), (const mcs::ChunkBox*)d_box, n, nx, ny,
+                     hx, hy, zB, H, err, wallD, lineCut, lineWall, markS, markF, luts, rect, vol);
   if (ev1 && hipEventRecord(ev1, stream) != hipSuccess) return -2;
-  hipLaunchKernelGGL(mcs::dirty_rect, dim3((n + 255) / 256), dim3(256), 0, stream,
-                     (const mcs::StampDev*)d_stampdev, (const int32_t*)touched, n, rect);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

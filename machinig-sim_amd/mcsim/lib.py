@@ -32,7 +32,7 @@ EXPORTS = [
     "mc_engine_set_maps", "mc_engine_get_state", "mc_engine_set_state", "mc_engine_run", "mc_engine_run_batch",
     "mc_engine_read_stats", "mc_engine_chip_events", "mc_engine_chatter_index", "mc_engine_read_rings",
     "mc_sim_run", "mc_js_math", "mc_stamp_batch_noalloc", "mc_path_discretize", "mc_sim_schedule",
-    "mc_last_search_trials",
+    "mc_last_search_trials", "mc_stamp_batch_lines",
 ]
 
 _LIB = None
@@ -71,6 +71,7 @@ def load():
     L.mc_stock_read_side.argtypes = [P, P, P, P, P]
     L.mc_stamp_batch.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
     L.mc_stamp_batch_noalloc.argtypes = [P, C.POINTER(Stamp), C.c_int, P]
+    L.mc_stamp_batch_lines.argtypes = [P, C.POINTER(Stamp), C.c_int, P, C.c_int]
     L.mc_stock_sample.argtypes = [P, P, C.c_int, P]
     L.mc_stamp_batch_ex.argtypes = [P, C.POINTER(Stamp), P, C.c_int, P, P, P, C.c_int, P]
     L.mc_stock_read_marks.argtypes = [P, P, P]
@@ -410,6 +411,14 @@ class Stock:
         n = len(stamps) if n is None else n
         vol = np.zeros(n) if want_vol else None
         _check(self.L.mc_stamp_batch(self.p, stamps, n, vol.ctypes.data if want_vol else None))
+        return vol
+
+    def stamps_lines(self, stamps, nlines, n=None):
+        """the stamps with their removed volume summed per program line (mc_stamp_batch_lines):
+        returns the nlines per-line volumes"""
+        n = len(stamps) if n is None else n
+        vol = np.zeros(int(nlines))
+        _check(self.L.mc_stamp_batch_lines(self.p, stamps, n, vol.ctypes.data, int(nlines)))
         return vol
 
     def sample(self, xy):

@@ -275,3 +275,32 @@ def test_full_zigzag_batch_matches_reference(mc):
     assert got == {k: g[k] for k in got}
     assert abs(float(np.sum(vol)) - g["removed"]) <= 1e-12 * g["removed"]
     s.close()
+
+
+def test_segment_volumes_and_maps_of_line_batch(mc):
+    """mc_stamp_batch_lines: the same maps and dirty rectangle as the per-stamp batch, and each
+    line's volume the sum of its stamps' volumes (within 1e-12 relative: the device sums in
+    another order); a line outside [0, nlines) is refused before any device work"""
+    import ctypes as C
+    import oracle
+    from mcsim import path, workloads as W
+    rec, _ = path.schedule(W.zigzag_segments(300), 0.1)
+    arr = path.as_stamp_array(rec)
+    nl = int(rec["line"].max()) + 1
+    a, b = mc.Stock(51.1, 51.1, 20, 0.1), mc.Stock(51.1, 51.1, 20, 0.1)
+    vs = a.stamps(arr)
+    lv = b.stamps_lines(arr, nl)
+    assert same_bits(a.H(), b.H())
+    for x, y in zip(a.side(), b.side()):
+        assert same_bits(x, y)
+    assert a.dirty() == b.dirty()
+    want = np.bincount(rec["line"], weights=vs, minlength=nl)
+    assert np.all(np.abs(lv - want) <= 1e-12 * np.maximum(1.0, np.abs(want)))
+    o = oracle.OracleStock(51.1, 51.1, 20, 0.1)
+    ov = o.stamps(arr, len(arr))
+    assert same_bits(b.H(), o.H())
+    ow = np.bincount(rec["line"], weights=ov, minlength=nl)
+    assert np.all(np.abs(lv - ow) <= 1e-11 * np.maximum(1.0, np.abs(ow)))
+    assert lv[0] == 0 and np.count_nonzero(lv) > 100
+    out = np.zeros(nl)
+    assert mc.load().mc_stamp_batch_lines(b.p, arr, len(arr), out.ctypes.data, nl - 1) != 0
