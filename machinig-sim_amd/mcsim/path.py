@@ -104,7 +104,11 @@ def path_points(x, y, z, rapid, line, F, S):
     n = len(x)
     pts = (PathPoint * max(1, n))()
     for k in range(n):
-        pts[k] = PathPoint(float(x[k]), float(y[k]), float(z[k]), float(F[k]), float(S[k]), 1 if rapid[k] else 0,
+        # a rapid runs at the machine's rapid rate: its feed may be absent (None)
+        if F[k] is None and not rapid[k]:
+            raise ValueError("path point %d: a feed move needs a feed rate" % k)
+        f = 0.0 if F[k] is None else float(F[k])
+        pts[k] = PathPoint(float(x[k]), float(y[k]), float(z[k]), f, float(S[k]), 1 if rapid[k] else 0,
                            int(line[k]), k, 0)
     return pts, n
 

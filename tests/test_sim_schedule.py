@@ -111,3 +111,22 @@ def test_schedule_abi_checks():
     arr = path.moves_array([mv])
     assert L.mc_path_discretize(arr, 1, 0.0, None, 0, C.byref(ns)) != 0
     assert L.mc_path_discretize(arr, 1, 0.5, None, 0, C.byref(ns)) == 0 and ns.value > 2
+
+
+def test_coupled_programs_schedule_with_feedless_rapids():
+    """the coupled workload's programs carry no feed on their rapids (None): the schedule
+    runs them at the rapid rate, and a feed move without a feed is an error"""
+    import pytest
+    from mcsim import path, workloads as W
+    width = 102.35
+    tool, m = W.coupled_plan(0, 4)
+    x, y, z, r, ln, F, S = W.coupled_program(1, width, m, speed=W.COUPLED_SPEED)
+    assert F[0] is None and r[0]
+    sub, n, frames = path.sim_frames(x, y, z, r, ln, F, S, tool["N"], width / 2047, 1e6)
+    assert n > 1000 and sum(c for _, c in frames) == n
+    sub2, n2 = path.sim_substeps(x, y, z, r, ln, F, S, tool["N"], width / 2047)
+    assert n2 == n
+    bad = list(F)
+    bad[-1] = None
+    with pytest.raises(ValueError):
+        path.sim_substeps(x, y, z, r, ln, bad, S, tool["N"], width / 2047)
