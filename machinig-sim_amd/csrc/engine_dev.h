@@ -36,7 +36,9 @@ enum { S_N, S_SUMF, S_MAXF, S_SUMFX, S_SUMFY, S_SUMFZ, S_MAXFZ, S_SUMT, S_MAXT, 
 enum { T_FX, T_FY, T_FZ, T_TQ, T_HB, T_B, T_RB, T_VOL, T_PHI, T_Z, T_COUNT };
 constexpr int T_SUMS = T_RB + 1;
 // per-step records of a chunk, for the statistics taken after it
-enum { R_FXY, R_FX, R_FY, R_FZ, R_TQ, R_X, R_Y, R_W, R_ENG, R_ZC, R_ZCOK, R_COUNT };
+// (R_NFZ holds -Fz and R_AW |w|, the operands of the Fz and wall maxima; R_ONE is 1.0 throughout,
+// the step count's operand and the condition of the unconditional fields)
+enum { R_FXY, R_FX, R_FY, R_FZ, R_TQ, R_X, R_Y, R_W, R_ENG, R_ZC, R_ZCOK, R_NFZ, R_AW, R_ONE, R_COUNT };
 
 // One engine.  The host keeps a mirror, uploads it before a run and reads it back after.
 struct EngDev {
@@ -91,6 +93,17 @@ __host__ __device__ __forceinline__ double jmin(double a, double b) {
   return a < b ? a : b;
 }
 __host__ __device__ __forceinline__ bool truthy(double v) { return v != 0 && v == v; }
+// the same as selects (no branches): for the statistics' per-lane operations
+__device__ __forceinline__ double jmax_sel(double a, double b) {
+  double r = a > b ? a : b;
+  r = (a == 0 && b == 0) ? (__builtin_signbit(a) ? b : a) : r;
+  return (a != a || b != b) ? __builtin_nan("") : r;
+}
+__device__ __forceinline__ double jmin_sel(double a, double b) {
+  double r = a < b ? a : b;
+  r = (a == 0 && b == 0) ? (__builtin_signbit(a) ? a : b) : r;
+  return (a != a || b != b) ? __builtin_nan("") : r;
+}
 
 // a chunk of steps is staged in LDS: the edge angle with its sine and cosine of every
 // (step, visit), and the delay-line cells of its slice visits; GCAP entries at most
@@ -287,6 +300,10 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
     default: break;
   }
   if (cond == 2 && !wallOn) op = OP_NONE;
+  // the record rows of this lane's operand and condition
+  const int srow = src >= 0 ? src : src == -1 ? R_ONE : src == -2 ? R_NFZ : R_AW;
+  const int crow = cond == 1 ? R_ZCOK : R_ONE;
+  rec[R_ONE][lane] = 1.0;
   double stv = op != OP_NONE ? E.st[lane] : 0.0;
   int64_t hist_i = E.hist_i, hist_count = E.hist_count, ts_i = E.ts_i, ts_count = E.ts_count;
   int64_t ws_i = E.ws_i, ws_count = E.ws_count, wsp_i = E.wsp_i, wsp_count = E.wsp_count;
@@ -485,6 +502,7 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
       if (lane == 0) {
         rec[R_FX][sc] = Fx; rec[R_FY][sc] = Fy; rec[R_FZ][sc] = res[T_FZ]; rec[R_TQ][sc] = res[T_TQ];
         rec[R_X][sc] = x; rec[R_Y][sc] = y; rec[R_W][sc] = w; rec[R_ENG][sc] = (double)__popcll(emask);
+        rec[R_NFZ][sc] = -res[T_FZ]; rec[R_AW][sc] = __builtin_fabs(w);
       }
     }
     wave_sync();
@@ -536,20 +554,22 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
     wave_sync();
     // the statistics (src/cutting.js:279-287), lane f over the chunk's steps in order, eight
     // records loaded at a time
+    // (selects only: the lanes' operations differ, and a branch per operation serialised them)
     for (int s8 = 0; s8 < ch; s8 += 8) {
-      double o[8], fxy[8];
-      bool ok[8];
+      double o[8], fxy[8], c[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int sc = s8 + u < ch ? s8 + u : s8;
         fxy[u] = rec[R_FXY][sc];
-        ok[u] = s8 + u < ch && op != OP_NONE && (cond != 1 || rec[R_ZCOK][sc] != 0);
-        o[u] = src >= 0 ? rec[src][sc] : src == -1 ? 1.0 : src == -2 ? -rec[R_FZ][sc] : __builtin_fabs(rec[R_W][sc]);
+        o[u] = rec[srow][sc];
+        c[u] = rec[crow][sc];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        if (s8 + u < ch) runMaxF = jmax(runMaxF, fxy[u]);
-        if (ok[u]) stv = op == OP_ADD ? stv + o[u] : op == OP_MAX ? jmax(stv, o[u]) : jmin(stv, o[u]);
+        const bool in = s8 + u < ch;
+        runMaxF = in ? jmax_sel(runMaxF, fxy[u]) : runMaxF;
+        const double nv = op == OP_ADD ? stv + o[u] : op == OP_MAX ? jmax_sel(stv, o[u]) : jmin_sel(stv, o[u]);
+        stv = (in && op != OP_NONE && c[u] != 0) ? nv : stv;
       }
     }
     // the rings and the history: step sc's pr, its phase in the tooth period and its count;
@@ -607,11 +627,11 @@ __device__ __forceinline__ void engine_steps(EngDev& E, int64_t nSteps, EngLds& 
 #pragma unroll
       for (int u = 0; u < 8; u++) wl[u] = rec[R_W][s8 + u < ch ? s8 + u : s8];
 #pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (s8 + u < ch) {
-          t += dt;
-          if (wallOn) wSum += wl[u];
-        }
+      for (int u = 0; u < 8; u++) {
+        const bool in = s8 + u < ch;
+        t = in ? t + dt : t;
+        wSum = in && wallOn ? wSum + wl[u] : wSum;
+      }
     }
     // ---- the chunk's delay-line cells back to the engine's record
     for (int e = lane; e < ch * NS; e += 64) {
