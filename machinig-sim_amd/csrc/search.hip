@@ -474,6 +474,11 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
       const int nmx = C.nmx, nmy = C.nmy;
       double x = 0, y = 0, vxs = 0.01, vys = 0;
       int pr = 0, ph = 0, dc = 0, ti = 0, tcount = 0, hpos = 0, hcount = 0;
+      // a history sample is stored one step late, beside the next step's delay-line stores:
+      // a store issued at the end of a step held up the wait for the next step's prefetch
+      bool hpend = false;
+      int hidx = 0;
+      float2 hval = make_float2(0.0f, 0.0f);
       long long my_eng = 0;
       float cur[MAXP];
       double2 csc[MAXP];
@@ -520,6 +525,7 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
         // 1) chip thickness and delay-line update of every visit; the engaged ones are
         //    recorded in (tooth, slice) order, the order in which the reference sums them
         int ne = 0;
+        float st_v[MAXP];
 #pragma unroll
         for (int p = 0; p < MAXP; p++) {
           // src/cutting.js:176-195, with its branches as selects.  chatterAt's feed frame is
@@ -536,7 +542,7 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
           const bool thin = h <= 0;  // the edge passes above the surface it left
           const float store = (cutting && thin) ? (float)(prev - feedH) : (float)rn;
           const bool eng = vok[p] && cutting && !thin;
-          if (vok[p]) *reinterpret_cast<float*>(ring_b + 4u * (rs[p] + rp[p] * (unsigned)nz)) = store;
+          st_v[p] = store;
           const unsigned long long m = __builtin_amdgcn_ballot_w64(eng) & smask;
           const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
           if (eng) {
@@ -545,6 +551,13 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
           }
           ne += __builtin_popcountll(m);
         }
+        // the delay-line stores after every pass's chip: issued together, behind the next
+        // step's prefetch, they keep the prefetch's loads from being waited for mid-step
+#pragma unroll
+        for (int p = 0; p < MAXP; p++)
+          if (vok[p]) *reinterpret_cast<float*>(ring_b + 4u * (rs[p] + rp[p] * (unsigned)nz)) = st_v[p];
+        if (hpend && g == 0 && in_slot) hist[hidx] = hval;
+        hpend = false;
         my_eng += (g == 0) ? ne : 0;
         wave_sync();
         // 2) cutting forces of the engaged visits, one visit per lane (src/cutting.js:196-206)
@@ -608,7 +621,9 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
         }
         dc = dc + 1 == decim ? 0 : dc + 1;
         if (dc == 0) {
-          if (g == 0 && in_slot) hist[hpos] = make_float2((float)(x * 1e6), (float)(y * 1e6));
+          hval = make_float2((float)(x * 1e6), (float)(y * 1e6));
+          hidx = hpos;
+          hpend = true;
           hpos = (hpos + 1) & (HIST - 1);
           hcount++;
         }
@@ -616,6 +631,7 @@ __device__ __forceinline__ void run_task(const LaunchDev& P, const TaskDev* __re
         for (int p = 0; p < MAXP; p++) { cur[p] = nxt[p]; csc[p] = nsc[p]; cbin[p] = nbin[p]; rp[p] = rpn[p]; }
         wave_sync();
       }
+      if (hpend && g == 0 && in_slot) hist[hidx] = hval;   // the last step's sample
 
       // ---- chatter index per slot (src/cutting.js:299-318)
       if (in_slot && g == 0) TL.eng[gslot] = my_eng;
