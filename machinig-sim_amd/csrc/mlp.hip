@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -572,25 +573,34 @@ int mc_mlp_train(mc_mlp* m, const float* X, const float* t, int n, const mc_trai
   hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, m->step.as<int>(), 1, 0);
   MCH_HIPCHK(hipStreamSynchronize(s));
 
-  // one training step, captured once and replayed
+  // one training step (the step counter lives on the device, so every launch of the sequence
+  // is the next step): captured once into a graph and replayed, or, with MC_MLP_EAGER set in
+  // the environment, launched kernel by kernel (the same kernels and results; for kernel
+  // tracers, which then see ordinary dispatches)
+  auto step_kernels = [&]() {
+    hipLaunchKernelGGL(gather, dim3(B), dim3(INP), 0, s, dx.as<float>(), dt.as<float>(), dp.as<int>(),
+                       m->step.as<int>(), spe, B, m->Xb.as<bf16>(), m->Tb.as<float>());
+    m->forward(B, s);
+    m->backward(B, s, m->step.as<int>());
+    hipLaunchKernelGGL(adam, dim3((NPARAM + 255) / 256), dim3(256), 0, s, m->P.as<float>(), m->G.as<float>(),
+                       m->M1.as<float>(), m->M2.as<float>(), NPARAM, m->step.as<int>(), total, o->lr, o->lr_final,
+                       o->beta1, o->beta2, o->eps);
+    hipLaunchKernelGGL(cast_weights, dim3(HID * HID / 256), dim3(256), 0, s, m->P.as<float>(), m->W1.as<bf16>(),
+                       m->W2.as<bf16>(), m->W3.as<bf16>(), m->W4.as<bf16>(), m->step.as<int>());
+  };
+  const bool eager = getenv("MC_MLP_EAGER") != nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  MCH_HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(gather, dim3(B), dim3(INP), 0, s, dx.as<float>(), dt.as<float>(), dp.as<int>(),
-                     m->step.as<int>(), spe, B, m->Xb.as<bf16>(), m->Tb.as<float>());
-  m->forward(B, s);
-  m->backward(B, s, m->step.as<int>());
-  hipLaunchKernelGGL(adam, dim3((NPARAM + 255) / 256), dim3(256), 0, s, m->P.as<float>(), m->G.as<float>(),
-                     m->M1.as<float>(), m->M2.as<float>(), NPARAM, m->step.as<int>(), total, o->lr, o->lr_final,
-                     o->beta1, o->beta2, o->eps);
-  hipLaunchKernelGGL(cast_weights, dim3(HID * HID / 256), dim3(256), 0, s, m->P.as<float>(), m->W1.as<bf16>(),
-                     m->W2.as<bf16>(), m->W3.as<bf16>(), m->W4.as<bf16>(), m->step.as<int>());
-  hipError_t ce = hipStreamEndCapture(s, &graph);
-  if (ce != hipSuccess) return mch::fail(MC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
-  ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (ce != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    return mch::fail(MC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ce));
+  if (!eager) {
+    MCH_HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    step_kernels();
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess) return mch::fail(MC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ce != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return mch::fail(MC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ce));
+    }
   }
   // epochs: a seeded Fisher-Yates permutation per epoch, then spe replays
   uint64_t rs = o->seed * 0x9E3779B97F4A7C15ull + 1;
@@ -607,8 +617,14 @@ int mc_mlp_train(mc_mlp* m, const float* X, const float* t, int n, const mc_trai
     if (hipMemcpyAsync(dp.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess) err = MC_ERR_HIP;
     // the copy must land before the replays read it; the host array is reused next epoch
     if (!err && hipStreamSynchronize(s) != hipSuccess) err = MC_ERR_HIP;
-    for (int k = 0; k < spe && !err; k++)
-      if (hipGraphLaunch(exec, s) != hipSuccess) err = MC_ERR_HIP;
+    for (int k = 0; k < spe && !err; k++) {
+      if (eager) {
+        step_kernels();
+        if (hipGetLastError() != hipSuccess) err = MC_ERR_HIP;
+      } else if (hipGraphLaunch(exec, s) != hipSuccess) {
+        err = MC_ERR_HIP;
+      }
+    }
   }
   (void)hipEventRecord(e1, s);
   std::vector<float> hl(total);
@@ -619,8 +635,8 @@ int mc_mlp_train(mc_mlp* m, const float* X, const float* t, int n, const mc_trai
   mch::set_last_ms(ms);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (err) return mch::fail(MC_ERR_HIP, "training step failed: %s", hipGetErrorString(hipGetLastError()));
   if (epoch_loss)
     for (int ep = 0; ep < o->epochs; ep++) {

@@ -78,6 +78,29 @@ def test_training_is_deterministic_and_fits(lib):
     assert runs[0][1][-1] < 0.05 * runs[0][1][0]
 
 
+def test_eager_training_equals_graph_replay(lib, monkeypatch):
+    """MC_MLP_EAGER launches the training step's kernels one by one instead of replaying the
+    captured graph (for kernel tracers): the same kernels in the same order, so the weights
+    and the losses are identical"""
+    rng = np.random.default_rng(4)
+    n = 4096
+    X = rng.normal(0, 1, (n, 24)).astype(np.float32)
+    t = (np.cos(X[:, 1]) - 0.2 * X[:, 0] * X[:, 5]).astype(np.float32)
+    runs = []
+    for eager in (False, True):
+        if eager:
+            monkeypatch.setenv("MC_MLP_EAGER", "1")
+        else:
+            monkeypatch.delenv("MC_MLP_EAGER", raising=False)
+        m = lib.Mlp(9)
+        loss = m.train(X, t, epochs=6, batch=512, lr=2e-3, lr_final=1e-4, seed=2)
+        runs.append((m.params(), loss, m.predict(X[:1000])))
+    monkeypatch.delenv("MC_MLP_EAGER", raising=False)
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2])
+
+
 def test_surrogate_bracketed_search(lib):
     """train on cells the GPU search resolves, then bracket held-out cells (+-15 %): a
     verified bracket is no wider than the tolerance, a failed one returns the plain
