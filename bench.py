@@ -228,6 +228,63 @@ def heightmap(steps, cpu=True, cpu_seconds=10.0):
     return out
 
 
+def zigzag_forces(n_sub, cpu):
+    """The zigzag workload's forces: its first n_sub sub-steps (the first layer with the flat
+    10 mm end mill and the start of the ball 6 mm layer, 512 x 512 stock at 0.1 mm) through
+    the simulator's sub-step loop (mc_sim_run: engagement, engine run, both stamps), one
+    program, the rigid mode of the reference's target test; per-segment maximum force
+    |(Fx, Fy)| from each sub-step's run statistics (src/cutting.js:280-281).  One program is one
+    serial chain (each sub-step reads the stock the previous one cut), so the rate is the
+    chain's, set beside the oracle's loop on one host thread, whose results the device's
+    equal (checked here)."""
+    from mcsim._abi import make_cell
+    layers = W.zigzag_layers()
+    cell_of = lambda t: make_cell(t, W.ZIGZAG_MODES, None, W.MAT_A5052, W.ZIGZAG_SPEED, (0.0, 0.0), 0.0, 1, True)
+    sub_of = lambda L: path.sim_substeps(L["x"], L["y"], L["z"], L["rapid"], L["line"], L["F"], L["S"],
+                                         L["tool"]["N"], 0.1)
+    typ = lambda tool: W.MC_TOOL_FLAT if tool["type"] == "flat" else W.MC_TOOL_BALL
+    kms = []
+
+    def run_dev(a, i, e, sub, n, tool, share):
+        r = lib.sim_run([dict(actual=a, ideal=i, engine=e, sub=sub, nsub=n, R=tool["R"], fluteLen=tool["fluteLen"],
+                              rw=0.0, share=share, type=typ(tool))])[0]
+        kms.append(lib.last_kernel_ms())
+        return r
+    a, i, e = lib.Stock(51.1, 51.1, 20, 0.1), lib.Stock(51.1, 51.1, 20, 0.1), lib.Engine()
+    i.set_wall_band(2.0)
+    t0 = time.perf_counter()
+    outs, lines = W.zigzag_coupled(layers, (a, i), e, run_dev, cell_of, sub_of, max_substeps=n_sub)
+    wall = time.perf_counter() - t0
+    seg = W.segment_max_force(outs, lines)
+    vals = np.array(list(seg.values()))
+    res = dict(workload="zigzag_512x512_coupled_first_%d_substeps" % len(outs), substeps=len(outs),
+               segments=len(seg), engine_steps=int(sum(o.nsteps for o in outs)), wall_s=wall,
+               kernel_ms=float(sum(kms)), substeps_per_s=len(outs) / wall,
+               segment_max_force_N=dict(max=float(vals.max()), median=float(np.median(vals)),
+                                        cutting_segments=int((vals > 0).sum())))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        def run_cpu(a, i, e, sub, n, tool, share):
+            return oracle.sim_run(a, i, e, sub, n, tool["R"], tool["fluteLen"], 0.0, share, typ(tool))
+        ao, io = oracle.OracleStock(51.1, 51.1, 20, 0.1), oracle.OracleStock(51.1, 51.1, 20, 0.1)
+        io.set_wall_band(2.0)
+        t0 = time.perf_counter()
+        o_outs, o_lines = W.zigzag_coupled(layers, (ao, io), oracle.OracleEngine(), run_cpu, cell_of, sub_of,
+                                           max_substeps=n_sub)
+        cw = time.perf_counter() - t0
+        same = (o_lines == lines and W.segment_max_force(o_outs, o_lines) == seg and
+                all((p.dx, p.dy, p.maxF, p.nsteps, p.ap) == (q.dx, q.dy, q.maxF, q.nsteps, q.ap)
+                    for p, q in zip(outs, o_outs)) and np.array_equal(a.H(), ao.H()))
+        res["cpu_baseline"] = dict(value=len(o_outs) / cw, unit="sub-steps/s", cores=1, kind="port",
+                                   sample="the same %d sub-steps, oracle/simloop.c on one host thread, %.1f s"
+                                          % (len(o_outs), cw))
+        res["identical_to_oracle"] = bool(same)
+    a.close(); i.close(); e.close()
+    return res
+
+
 def line_cells(progs, outs, frs, tools, revs=40):
     """The machining report's per-line stability question: each program's report is built
     from the pipeline's frames (mcsim.report: Simulator.collect and LineReport,
@@ -453,6 +510,8 @@ def main():
     ap.add_argument("--no-coupled", action="store_true")
     ap.add_argument("--coupled-programs", type=int, default=16)
     ap.add_argument("--coupled-lines", type=int, default=4)
+    ap.add_argument("--zigzag-substeps", type=int, default=25000,
+                    help="sub-steps of the zigzag program run through the coupled loop (its forces)")
     ap.add_argument("--train-cells", type=int, default=262144)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--heightmap-only", action="store_true",
@@ -551,6 +610,7 @@ def main():
     }
     if not a.no_heightmap:
         out["heightmap"] = heightmap(2, not a.no_cpu)
+        out["zigzag_forces"] = zigzag_forces(a.zigzag_substeps, not a.no_cpu)
     if not a.no_coupled:
         out["coupled"] = coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu, opts, model)
     if not a.no_cpu:

@@ -148,6 +148,66 @@ def zigzag_segments(n_segments=5000, width=51.1, seed=1):
     return segs[:n_segments]
 
 
+# the zigzag program run through the simulator's sub-step loop: a 10 mm flat and a 6 mm ball
+# end mill (three flutes, 30 deg helix), one rigid mode per axis as the reference's target test
+# (k 5e6 N/m, fn 6000 Hz, zeta 0.1, test/target.test.mjs:66), A5052, feed 1000 mm/min at
+# 10000 min^-1 (fz = F / (S N))
+ZIGZAG_TOOLS = {MC_TOOL_FLAT: dict(N=3, R=5.0, helixDeg=30.0, type="flat", fluteLen=20.0),
+                MC_TOOL_BALL: dict(N=3, R=3.0, helixDeg=30.0, type="ball", fluteLen=20.0)}
+ZIGZAG_MODES = [(5.0e3, 6000.0, 0.1)]
+ZIGZAG_FEED, ZIGZAG_SPEED = 1000.0, 10000.0
+
+
+def zigzag_layers(n_segments=5000, width=51.1, seed=1):
+    """The zigzag program of zigzag_segments as the simulator runs it, one entry per layer
+    (each layer has its own tool): dict(type, tool, x, y, z, rapid, line, F, S), the path's
+    points (point 0 the position the layer starts from, point k the end of the layer's k-th
+    segment, which carries that segment's program line), every segment a feed move."""
+    layers = []
+    for sg in zigzag_segments(n_segments, width, seed):
+        x0, y0, z0, x1, y1, z1, R, typ, line = sg
+        if not layers or layers[-1]["type"] != typ:
+            layers.append(dict(type=typ, tool=ZIGZAG_TOOLS[typ], x=[x0], y=[y0], z=[z0], rapid=[1], line=[line],
+                               F=[None], S=[ZIGZAG_SPEED]))
+        L = layers[-1]
+        L["x"].append(x1); L["y"].append(y1); L["z"].append(z1); L["rapid"].append(0); L["line"].append(line)
+        L["F"].append(ZIGZAG_FEED); L["S"].append(ZIGZAG_SPEED)
+    return layers
+
+
+def zigzag_coupled(layers, stocks, engine, run, cell_of, sub_of, max_substeps=None):
+    """Run the layers through the sub-step loop one after another on one pair of stocks
+    (actual, commanded) and one engine, configured for each layer's tool as the simulator's
+    ensureEngine does.  run(actual, ideal, engine, sub, n, tool, share) runs n sub-steps and
+    returns their SubstepOut records; cell_of(tool) gives the engine's configure argument and
+    sub_of(layer) the layer's sub-steps.  Stops after max_substeps.  Returns the records and
+    each record's program line."""
+    act, ide = stocks
+    outs, lines = [], []
+    left = max_substeps
+    for L in layers:
+        sub, n = sub_of(L)
+        if left is not None:
+            n = min(n, left)
+            left -= n
+        if n <= 0:
+            break
+        info = engine.configure(cell_of(L["tool"]))
+        engine.set_wear(MAT_A5052["Kte"], MAT_A5052["Kre"])
+        outs.extend(run(act, ide, engine, sub, n, L["tool"], info.stepsPerTooth / 4))
+        lines.extend(sub[k].line for k in range(n))
+    return outs, lines
+
+
+def segment_max_force(outs, lines):
+    """per program line (segment): the largest |(Fx, Fy)| over its cutting sub-steps' engine
+    runs (the run statistics' maxF, src/cutting.js:280-281); 0 for a segment that never cut"""
+    seg = {}
+    for o, ln in zip(outs, lines):
+        seg[ln] = max(seg.get(ln, 0.0), o.maxF)
+    return seg
+
+
 # ---------------------------------------------------------------- coupled sub-step programs
 
 
