@@ -69,6 +69,20 @@ def same_limits(res, ref):
     return int(np.count_nonzero((alim != ref["alim"]) | (lo != ref["lo"]) | (hi != ref["hi"])))
 
 
+def reference_differences(res, ref):
+    """the cells whose limits differ from the reference's: the cell, both limits and brackets,
+    and the distance between the limits in widths of the reference's final bracket (one
+    bisection step apart reads about 1)"""
+    out = []
+    for i, r in enumerate(res):
+        if (r.alim, r.lo, r.hi) != (ref["alim"][i], ref["lo"][i], ref["hi"][i]):
+            w = ref["hi"][i] - ref["lo"][i]
+            out.append(dict(cell=i, alim=r.alim, lo=r.lo, hi=r.hi, ref_alim=float(ref["alim"][i]),
+                            ref_lo=float(ref["lo"][i]), ref_hi=float(ref["hi"][i]),
+                            brackets_apart=float(abs(r.alim - ref["alim"][i]) / w) if w > 0 else None))
+    return out
+
+
 def cpu_baseline(cells, opts, target_s, gpu_res=None):
     """The oracle (FP64 CPU restatement) on a bounded sample of the same cells: a seeded
     shuffle of the grid, run in chunks of 8 cells per thread (one cell per thread at a
@@ -567,6 +581,7 @@ def main():
     ref_check = None
     if ref is not None:
         ref_check = dict(cells=n, plain_cells_differing=same_limits(res, ref),
+                         plain_differences=reference_differences(res, ref),
                          reference="timeDomainLimit of the reference (Node %s, %d processes, %.0f s for the grid: "
                                    "%.1f cells/s) from tests/golden/golden_tdl_grid.json" %
                                    (ref["node"], ref["procs"], ref["node_s"], n / ref["node_s"]),
