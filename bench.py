@@ -299,6 +299,48 @@ def zigzag_forces(n_sub, cpu):
     return res
 
 
+def sld_40rpm(cpu):
+    """Config 0, the reference's own CPU-runnable case: the 40-speed diagram of one tool
+    (timeDomainLimit at 40 speeds from 3000 to 15000 rpm, src/stabtd.js:24-32) as one batched
+    search, checked against the reference's limits (golden_tdl.json) and timed beside the
+    oracle on the host (one thread per cell, and one thread for all)."""
+    cells, opts = W.sld_40rpm()
+    arr = cell_array(cells)
+    res, kms, ws = None, [], []
+    for _ in range(4):                    # the first is a warm-up
+        t0 = time.perf_counter()
+        res = lib.bisect_batch(arr, **opts)
+        ws.append(time.perf_counter() - t0)
+        kms.append(lib.last_kernel_ms())
+    wall = float(np.mean(ws[1:]))
+    out = dict(workload="sld_40rpm_3000_15000", cells=len(cells), wall_ms=1e3 * wall,
+               kernel_ms=float(np.mean(kms[1:])), cells_per_s=len(cells) / wall)
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "golden_tdl.json")) as f:
+            g = json.load(f)
+        ref = [g["results"][i] for i in g["meta"]["sld40"]]
+        out["identical_to_reference"] = all((r.alim, r.lo) == (q["alim"], q["lo"]) and
+                                            (r.hi == q["hi"] or (q["hi"] == "inf" and r.open))
+                                            for r, q in zip(res, ref))
+    except OSError:
+        pass
+    # the reference's own timeDomainLimit on these 40 cells, one thread (Node 12, build container)
+    out["reference_node"] = dict(seconds=6.7, cells_per_s=40 / 6.7, cores=1,
+                                 sample="tests/golden/time_reference.py, in the build container")
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        for k in (1, threads):
+            t0 = time.perf_counter()
+            got = oracle.bisect_batch(arr, len(cells), threads=k, **opts)
+            dt = time.perf_counter() - t0
+            out["cpu_%d_threads" % k] = dict(value=len(cells) / dt, unit="SLD cells/s", cores=k, kind="port",
+                                             identical_to_gpu=all((a.alim, a.lo, a.hi) == (b.alim, b.lo, b.hi)
+                                                                  for a, b in zip(got, res)))
+    return out
+
+
 def line_cells(progs, outs, frs, tools, revs=40):
     """The machining report's per-line stability question: each program's report is built
     from the pipeline's frames (mcsim.report: Simulator.collect and LineReport,
@@ -623,6 +665,7 @@ def main():
                      "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated)"},
     }
+    out["sld_40rpm"] = sld_40rpm(not a.no_cpu)
     if not a.no_heightmap:
         out["heightmap"] = heightmap(2, not a.no_cpu)
         out["zigzag_forces"] = zigzag_forces(a.zigzag_substeps, not a.no_cpu)
