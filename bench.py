@@ -341,6 +341,59 @@ def sld_40rpm(cpu):
     return out
 
 
+def dropin_substeps(cpu):
+    """The drop-in path's cost per simulator sub-step: the JS modules called as src/sim.js
+    calls them (machinig-sim_amd/js/dropin_substeps.mjs: one device call each for the
+    engagement, the engine run and the two stamps, per sub-step) on the first recorded
+    simulator program of golden_simx.json, beside the oracle's fused loop on one host thread
+    on the same sub-steps, and the reference's own Simulator timed in the build container
+    (tests/golden/time_reference.py sim)."""
+    import subprocess
+    js = os.path.join(ROOT, "machinig-sim_amd", "js", "dropin_substeps.mjs")
+    golden = os.path.join(ROOT, "tests", "golden", "golden_simx.json")
+    out = dict(workload="golden_simx_program0_dropin", reference_node=dict(
+        us_per_substep=[196.1, 176.7, 126.7], cores=1,
+        sample="the reference's Simulator on the same program under Node 12 in the build container, "
+               "three warm runs (tests/golden/time_reference.py sim); the oracle's loop there: 45 us"))
+    try:
+        r = subprocess.run(["node", js, golden, "0", "3"], capture_output=True, text=True, timeout=120)
+        if r.returncode == 0:
+            out.update(json.loads(r.stdout.strip().splitlines()[-1]))
+        else:
+            out["error"] = r.stderr[-400:]
+    except (OSError, subprocess.TimeoutExpired) as ex:
+        out["error"] = str(ex)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        from mcsim._abi import make_cell, MC_TOOL_BALL, MC_TOOL_FLAT
+        with open(golden) as f:
+            g = json.load(f)
+        case, res = g["cases"][0], g["results"][0]
+        P = res["path"]
+        dec = lambda k: np.frombuffer(__import__("base64").b64decode(P[k]), dtype="<f8")
+        sub, n = path.sim_substeps(dec("x"), dec("y"), dec("z"), P["rapid"], P["line"], P["F"], P["S"], res["toolN"],
+                                   res["hx"])
+        c = res["cfgs"][0]
+        mat = dict(zip(("Ktc", "Krc", "Kte", "Kre", "Kac", "Kae", "mc", "Ci"), c[13]))
+        cell = make_cell(dict(N=c[1], R=c[2], helixDeg=c[3], type=c[4], fluteLen=c[5], runout=c[7], runoutPhase=c[8]),
+                         c[11], c[12], mat, c[9], (0.0, 0.0), 0.0, 1, c[6] != 0)
+        st = case["stock"]
+        ts = []
+        for _ in range(3):
+            a, i = oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"]), oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"])
+            i.set_wall_band(2.0)
+            e = oracle.OracleEngine()
+            info = e.configure(cell, c[10])
+            e.set_wear(res["wears"][0][1], res["wears"][0][2])
+            t0 = time.perf_counter()
+            oracle.sim_run(a, i, e, sub, n, res["toolR"], res["fluteLen"], 0.0, info.stepsPerTooth / 4,
+                           MC_TOOL_BALL if res["type"] == "ball" else MC_TOOL_FLAT)
+            ts.append(1e6 * (time.perf_counter() - t0) / n)
+        out["cpu_oracle_us_per_substep"] = ts
+    return out
+
+
 def line_cells(progs, outs, frs, tools, revs=40):
     """The machining report's per-line stability question: each program's report is built
     from the pipeline's frames (mcsim.report: Simulator.collect and LineReport,
@@ -666,6 +719,7 @@ def main():
                                        "copy-calibrated)"},
     }
     out["sld_40rpm"] = sld_40rpm(not a.no_cpu)
+    out["dropin_substeps"] = dropin_substeps(not a.no_cpu)
     if not a.no_heightmap:
         out["heightmap"] = heightmap(2, not a.no_cpu)
         out["zigzag_forces"] = zigzag_forces(a.zigzag_substeps, not a.no_cpu)

@@ -223,7 +223,8 @@ if (cases.sim) {
 if (cases.simx) {
   const { Simulator } = await import(REF + 'sim.js');
   const { modeSet } = await import(REF + 'frf.js');
-  out.simx = cases.simx.map(({ program, tool, stock, cell }) => {
+  out.simx = cases.simx.map(({ program, tool, stock, cell }, ci) => {
+    const tc = Date.now();
     const sim = new Simulator();
     sim.cfg.cell = cell;
     sim.cfg.workpieceFlex = false;
@@ -261,6 +262,7 @@ if (cases.simx) {
     e.run = n => { fr(n); runs.push([rec.actual.length, n]); };
     let frames = 0;
     while (!sim.done && frames < 100) { sim.step(1e6); frames++; }
+    console.error('simx case', ci, (Date.now() - tc) / 1e3, 's');
     const maps = s => ({ H: z32(Float32Array, s.H), err: z32(Float32Array, s.err), wallD: z32(Float32Array, s.wallD),
       lineCut: z32(Int32Array, s.lineCut), lineWall: z32(Int32Array, s.lineWall), markS: z32(Float32Array, s.markS),
       markF: z32(Float32Array, s.markF), removed: s.removed });
