@@ -423,7 +423,8 @@ int mc_tip_lut(int32_t shape, double D, double cornerR, const double* edge_r, in
  * commanded stamp at p and the actual stamp at p plus the deflection, with the feed mark.
  * One wavefront per program runs its whole sub-step list without returning to the host;
  * mc_sim_run puts many programs in one launch.  The workpiece is rigid (workpieceFlex off,
- * so setWork(null)), the flank wear fixed for the call (the engine's setWear and rw), and
+ * so setWork(null); the thin-wall form, flex, is switched off in this build), the flank wear
+ * fixed for the call (the engine's setWear and rw), and
  * the tool flat, ball or bull-nose (its profile cutterRadiusAt in the engine's slices, its
  * tip table in the stamps and the end face, as the simulator sets them up, src/sim.js:190-218). */
 
@@ -521,6 +522,25 @@ typedef struct {
   int32_t pad;
 } mc_substep_out;
 
+/* the thin wall the simulator's wall model found at a sub-step (wallAt, src/wall.js:11-112), what
+   it passed to setWork, and the wall-error mark it left after the stamp (src/sim.js:443) */
+typedef struct {
+  double k, m, fn, zeta;   /* setWork's argument: stiffness [N/mm], tip-equivalent mass [kg],
+                              natural frequency [Hz] (capped at MC_WALL_FN_CAP), damping ratio */
+  double nx, ny;           /* the wall normal (the thinnest crossing near the contact normal) */
+  double t, tRoot;         /* thickness at the load height and at the root [mm] */
+  double hWall, a;         /* free height above the higher floor, load height above it [mm] */
+  double bEff, base;       /* effective plate width [mm], the wall's base height [mm] */
+  double cx, cy;           /* the contact point [mm] */
+  double I;                /* second moment at the load height [mm^4] */
+  double markVal;          /* marked: markWall's value max(0, runMeanW 1e6) [um] */
+  double wnx, wny;         /* marked: the engine's wall normal markWall used */
+  int32_t found;           /* wallAt gave a wall (the fields above k..I are set); 0: null */
+  int32_t marked;          /* the actual stock's wall-error map was marked after the stamp:
+                              markWall(x + dx, y + dy, R, wnx, wny, markVal) (src/stock.js:100-113) */
+} mc_wall_out;
+enum { MC_WALL_FN_CAP = 5000 };   /* Hz (src/wall.js:9): the engine's fnMax with the wall model on */
+
 typedef struct {
   mc_stock* actual;        /* Simulator.stock: cut by the deflected tool */
   mc_stock* ideal;         /* Simulator.ideal: the commanded cut */
@@ -547,6 +567,19 @@ typedef struct {
   mc_frame_out* frames;    /* NULL, or room for the frames (MC_FRAME_LAST sub-steps) of the call:
                               at each frame's end the engine's stats are taken into it and its
                               chatter index computed */
+  int32_t flex;            /* must be 0 in this build: the device form of the thin-wall model is
+                              switched off (MC_ERR_LIMIT); the oracle restates it (oracle/wall.c).
+                              Meant as: 1: the thin-wall model (cfg.workpieceFlex, src/sim.js:407): each
+                              sub-step cutting deeper than 0.01 mm asks wallAt for the wall at the
+                              load height tipZ + ap / 2 and passes it to setWork; out of the cut a
+                              wall whose deflection has decayed below 1e-7 is dropped (:429).  The
+                              engine is then configured with fnMax MC_WALL_FN_CAP (:255).
+                              0: a rigid workpiece (setWork(null)) */
+  int32_t pad;
+  double wallE;            /* flex: the material's Young's modulus [N/mm^2] (mat.E) */
+  double wallRho;          /* flex: its density [kg/m^3] (mat.rho) */
+  double kTool;            /* flex: the tool's static stiffness [N/mm] (toolStaticK, src/sim.js:250) */
+  mc_wall_out* walls;      /* flex: NULL, or nsub records of the wall model per sub-step */
 } mc_sim_program;
 
 /* n programs, one workgroup each, one launch; the stocks and engines must be distinct */

@@ -243,9 +243,7 @@ int mc_engine_set_work(mc_engine* e, const mc_wall* w) {
   if (int st = check_engine(e)) return st;
   EngDev& h = e->h;
   if (!w) { eng_set_work_null(h); return MC_OK; }
-  if (truthy(h.wk) && w->nx * h.wnx + w->ny * h.wny < 0.8) { h.w = 0; h.wv = 0; }
-  h.wk = w->k * 1e3; h.wm = w->m; h.wc = 2 * w->zeta * std::sqrt(h.wk * w->m);
-  h.wnx = w->nx; h.wny = w->ny;
+  eng_set_work(h, w->k, w->m, w->zeta, w->nx, w->ny);
   return MC_OK;
 }
 

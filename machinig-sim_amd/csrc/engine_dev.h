@@ -675,6 +675,14 @@ __host__ __device__ __forceinline__ void eng_set_work_null(EngDev& h) {
   h.runMeanW = 0; h.wk = 0; h.wm = 1; h.wc = 0; h.w = 0; h.wv = 0; h.wnx = 1; h.wny = 0;
 }
 
+// setWork(wall), src/cutting.js:90-95: the wall state carries over while the normal stays
+// within about 37 degrees of the previous one
+__host__ __device__ __forceinline__ void eng_set_work(EngDev& h, double k, double m, double zeta, double nx, double ny) {
+  if (truthy(h.wk) && nx * h.wnx + ny * h.wny < 0.8) { h.w = 0; h.wv = 0; }
+  h.wk = k * 1e3; h.wm = m; h.wc = 2 * zeta * __builtin_sqrt(h.wk * m);
+  h.wnx = nx; h.wny = ny;
+}
+
 // the simulator's tool shapes as the engine and the stamps see them: flat, ball, or bull-nose
 // with corner radius rc and its tip table (lut of lutn heights every MC_LUT_DR)
 struct ToolShape {

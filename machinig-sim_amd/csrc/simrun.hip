@@ -46,6 +46,12 @@ struct ProgDev {
   mc_frame_out* frames;
   double R, fluteLen, rw, share, rc, dynBudget, fz0;
   int32_t nsub, ball, N, lutn, nframes, pad;
+  // the thin-wall model (flex): the half-cell walk table, its entries up to 20 and 60 mm, the
+  // step, the material and the tool's static stiffness, and the per-sub-step records
+  const double* sacc;
+  mc_wall_out* walls;
+  double wstep, wallE, wallRho, kTool;
+  int32_t flex, n20, n60, pad2;
 };
 
 // Stock.sample (src/stock.js:85-92)
@@ -346,16 +352,270 @@ __device__ __forceinline__ double stamp_part(const StockDev& S, const Box& bx, d
   return stamp_nodes<false>(S, bx, cx, cy, tipZ, R, ball, nullptr, 0, errVal, mux, muy, mfeed, line, t, nt, touched);
 }
 
+// ---- The thin-wall model (src/wall.js), read from the actual stock by the whole workgroup.
+// Every walk of the reference steps by half a cell with a running sum (s += step); the sums
+// from 0 are the table sacc (sacc[k] = s after k + 1 additions, up to 60 mm, made on the host
+// in the same order), so the walks from 0 are searched in parallel over the table, and the
+// walks from another start (the far floor from t, the thickness across from s0) are summed
+// lane by lane, 64 elements per pass, each lane keeping its own.
+constexpr int WALL_DIRS = 17;   // -40 .. 40 degrees in 5 degree steps (src/wall.js:26)
+constexpr int WALL_NZ = 10;     // heights of the thickness profile (src/wall.js:92)
+enum { W_SX, W_SY, W_N0, W_CX, W_CY, W_T, W_NX, W_NY, W_BASE, W_A, W_HW, W_MX, W_MY, W_COUNT };
+// block-uniform decisions, one slot per phase, so that no thread overwrites a flag that a
+// slower thread has still to read
+enum { F_CONTACT, F_DIRS, F_BEST, F_MEASURE, F_ALONG, F_FOUND, F_COUNT };
+
+struct WallLds {
+  double tx[MC_BINS / 2], ty[MC_BINS / 2];   // the contact terms of the engaged bins
+  double dcs[WALL_DIRS], dsn[WALL_DIRS];     // the crossing directions
+  double thick[WALL_NZ + 1];                 // thicknessAt at the load height, then the profile
+  double ts[WALL_NZ];                        // the profile scaled to the load height's thickness
+  double v[W_COUNT];
+  mc_wall_out rec;                           // the sub-step's record
+  int dirk[WALL_DIRS];                       // first crossing per direction (table index)
+  int lk[2];                                 // first break along the wall, each way
+  int go[F_COUNT];
+};
+
+// Stock.floor (src/stock.js:94-97)
+__device__ __forceinline__ double floor_at(const StockDev& S, double x, double y) {
+  const double v = sample_at(S, x, y);
+  return v == -__builtin_inf() ? S.zB : v;
+}
+
+// cellAt (src/wall.js:46-50): the nearest node's height
+__device__ __forceinline__ double cell_at(const StockDev& S, double x, double y) {
+  const double i = mcm::round_js(x / S.hx), j = mcm::round_js(y / S.hy);
+  if (!(i >= 0) || !(j >= 0) || i >= S.nx || j >= S.ny) return -__builtin_inf();
+  return S.H[(int64_t)j * S.nx + (int64_t)i];
+}
+
+__device__ __forceinline__ unsigned long long ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// thicknessAt (src/wall.js:54-60) by one wavefront: back out of the material from the
+// contact point (s0 = 0, -sacc[0], -sacc[1], ... while s0 > -6 and the node is at or above
+// z), then across from s0 + step while s <= s0 + 20.  Every lane returns the answer.
+__device__ double thickness_at_wave(const StockDev& S, const double* sacc, int nsacc, double step, double cx,
+                                    double cy, double nx, double ny, double z, int lane) {
+  int m = 0;
+  for (int base = 0;; base += 64) {   // wave-uniform
+    const int k = base + lane;
+    const double s0 = k == 0 ? 0.0 : k - 1 < nsacc ? -sacc[k - 1] : -__builtin_inf();
+    const bool stop = !(s0 > -6) || !(cell_at(S, cx + nx * s0, cy + ny * s0) >= z);
+    const unsigned long long b = ballot64(stop);
+    if (b) { m = base + __builtin_ctzll(b); break; }
+  }
+  const double s0 = m == 0 ? 0.0 : -sacc[m - 1 < nsacc ? m - 1 : nsacc - 1], lim = s0 + 20;
+  double cur = s0;   // the element before this pass's first
+  for (;;) {         // wave-uniform: the elements grow, so those within lim come first
+    double mine = 0;
+    for (int j = 0; j < 64; j++) { cur = cur + step; mine = j == lane ? cur : mine; }
+    const bool in = mine <= lim;
+    const unsigned long long h = ballot64(in && cell_at(S, cx + nx * mine, cy + ny * mine) < z);
+    if (h) return __shfl(mine, __builtin_ctzll(h), 64) - s0;
+    if (ballot64(!in)) return -1;
+  }
+}
+
+// wallAt (src/wall.js:11-34) with measure (:62-112), called by every thread of the
+// workgroup; Hb is the engine's map, (ux, uy) the feed direction.  Returns whether a wall was
+// found (the same on every thread); its fields are in W.rec.
+__device__ bool wall_at_block(const StockDev& S, const float* Hb, const double* sacc, int n20, int n60, double step,
+                              double px, double py, double pz, double R, double ux, double uy, double zLoad,
+                              double E, double rho, double kTool, WallLds& W) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double PI = 3.141592653589793;
+  // the contact point: the middle of the engaged arc at the load height (:12-23); a bin
+  // left out adds +0, which leaves a sum started at +0 unchanged
+  for (int b = tid; b < MC_BINS / 2; b += NT) {
+    double tx = 0, ty = 0;
+    if (!((double)Hb[b] <= zLoad)) {
+      const double phi = (b + 0.5) / MC_BINS * 2 * PI;
+      double sn, cs;
+      mcm::sincos_js(phi, sn, cs);
+      tx = sn * ux + cs * -uy;
+      ty = sn * uy + cs * ux;
+    }
+    W.tx[b] = tx; W.ty[b] = ty;
+  }
+  __syncthreads();
+  if (tid < 2) {   // the two sums in bin order, side by side on lanes 0 and 1
+    const double* a = tid == 0 ? W.tx : W.ty;
+    double s = 0;
+    for (int b = 0; b < MC_BINS / 2; b++) s += a[b];
+    W.v[W_SX + tid] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double sx = W.v[W_SX], sy = W.v[W_SY], L = mcm::hypot_js(sx, sy);
+    W.go[F_CONTACT] = !(L < 1e-9);
+    if (W.go[F_CONTACT]) {
+      W.v[W_N0] = mcm::atan2_js(sy, sx);
+      W.v[W_CX] = px + R * sx / L;
+      W.v[W_CY] = py + R * sy / L;
+    }
+  }
+  __syncthreads();
+  if (!W.go[F_CONTACT]) return false;
+  if (tid < WALL_DIRS) {
+    const double d = -40 + 5 * tid, a = W.v[W_N0] + d * PI / 180;
+    double sn, cs;
+    mcm::sincos_js(a, sn, cs);
+    W.dcs[tid] = cs; W.dsn[tid] = sn;
+    W.dirk[tid] = 0x7fffffff;
+  }
+  __syncthreads();
+  const double cx = W.v[W_CX], cy = W.v[W_CY];
+  // thickness (:36-42) along every direction: the first sacc[k] <= 20 whose bilinear sample
+  // lies below zLoad; 15 threads per direction, 8 steps each per round
+  for (int base = 0; base < n20; base += 120) {   // block-uniform
+    if (tid < 15 * WALL_DIRS) {
+      const int i = tid / 15, o = tid - 15 * i;
+      const double nx = W.dcs[i], ny = W.dsn[i];
+      int hit = 0x7fffffff;
+      for (int j = 7; j >= 0; j--) {
+        const int k = base + o + 15 * j;
+        if (k < n20) {
+          const double s = sacc[k];
+          if (sample_at(S, cx + nx * s, cy + ny * s) < zLoad) hit = k;
+        }
+      }
+      if (hit != 0x7fffffff) atomicMin(&W.dirk[i], hit);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int all = 1;
+      for (int i = 0; i < WALL_DIRS; i++) all &= W.dirk[i] != 0x7fffffff;
+      W.go[F_DIRS] = all;
+    }
+    __syncthreads();
+    if (W.go[F_DIRS]) break;
+  }
+  // the thinnest crossing, the first of equals in the loop's order (:27-31)
+  if (tid == 0) {
+    int bi = -1;
+    double bt = 0;
+    for (int i = 0; i < WALL_DIRS; i++) {
+      if (W.dirk[i] == 0x7fffffff) continue;
+      const double t = sacc[W.dirk[i]];
+      if (t > 0 && (bi < 0 || t < bt)) { bi = i; bt = t; }
+    }
+    W.go[F_BEST] = bi >= 0;
+    if (bi >= 0) { W.v[W_T] = bt; W.v[W_NX] = W.dcs[bi]; W.v[W_NY] = W.dsn[bi]; }
+  }
+  __syncthreads();
+  if (!W.go[F_BEST]) return false;
+  const double t = W.v[W_T], nx = W.v[W_NX], ny = W.v[W_NY];
+  // measure (:62-72): the far-side floor over [t, t + 3], the wall's base, free height and
+  // load height; slivers are left out
+  if (wv == 0) {
+    double cur = t, fm = __builtin_inf();
+    const double lim = t + 3;
+    for (;;) {   // wave-uniform
+      double mine = 0;
+      for (int j = 0; j < 64; j++) { mine = j == lane ? cur : mine; cur = cur + step; }
+      const bool in = mine <= lim;
+      if (in) fm = jmin(fm, floor_at(S, cx + nx * mine, cy + ny * mine));
+      if (ballot64(!in)) break;
+    }
+    for (int o = 32; o > 0; o >>= 1) fm = jmin(fm, __shfl_xor(fm, o, 64));
+    if (lane == 0) {
+      const double base = jmax(pz, fm), mx = cx + nx * t / 2, my = cy + ny * t / 2;
+      const double hWall = sample_at(S, mx, my) - base;
+      W.v[W_BASE] = base; W.v[W_MX] = mx; W.v[W_MY] = my; W.v[W_HW] = hWall;
+      W.v[W_A] = jmin(jmax(zLoad - base, 0.3), jmax(hWall, 0.3));
+      W.go[F_MEASURE] = !(hWall < 0.5 || t < 2 * step * 2);
+      W.lk[0] = 0x7fffffff; W.lk[1] = 0x7fffffff;
+    }
+  }
+  __syncthreads();
+  if (!W.go[F_MEASURE]) return false;
+  // the length along the wall at the load height (:74-83): the first break each way, the
+  // steps from 0 up to 60 mm read from the table
+  const double mx = W.v[W_MX], my = W.v[W_MY], tx = -ny, ty = nx;
+  for (int base = 0; base < n60; base += 4 * (NT / 2)) {   // block-uniform
+    const int side = tid / (NT / 2), o = tid - side * (NT / 2);
+    const double sg = side == 0 ? 1.0 : -1.0;
+    int hit = 0x7fffffff;
+    for (int j = 3; j >= 0; j--) {
+      const int k = base + o + (NT / 2) * j;
+      if (k < n60) {
+        const double s = sacc[k];
+        if (sample_at(S, mx + tx * s * sg, my + ty * s * sg) < zLoad) hit = k;
+      }
+    }
+    if (hit != 0x7fffffff) atomicMin(&W.lk[side], hit);
+    __syncthreads();
+    if (tid == 0) W.go[F_ALONG] = W.lk[0] != 0x7fffffff && W.lk[1] != 0x7fffffff;
+    __syncthreads();
+    if (W.go[F_ALONG]) break;
+  }
+  const double base = W.v[W_BASE], a = W.v[W_A], hWall = W.v[W_HW];
+  // the thickness profile (:88-96): thicknessAt at the load height and at ten heights from
+  // the root, three calls per wavefront
+  for (int c = wv; c < WALL_NZ + 1; c += NT / 64) {   // wave-uniform
+    const double z = c == 0 ? zLoad : base + (c - 1 + 0.5) / WALL_NZ * a;
+    const double tq = thickness_at_wave(S, sacc, n60, step, cx, cy, nx, ny, z, lane);
+    if (lane == 0) W.thick[c] = tq;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // the length: m1 steps one way (the table's sum), then m2 more steps the other way, summed
+    // on; once it reaches both 2 t and the plate-width bound nothing it decides can change
+    const int m1 = W.lk[0] == 0x7fffffff ? n60 : W.lk[0], m2 = W.lk[1] == 0x7fffffff ? n60 : W.lk[1];
+    const double X = 2 * a + t + R;
+    double L = m1 > 0 ? sacc[m1 - 1] : 0.0;
+    for (int j = 0; j < m2 && !(L >= X && L >= 2 * t); j++) L += step;
+    bool ok = !(L < 2 * t);
+    if (ok) {
+      // the cantilever strip's compliance at the load height (:97-104) and its mass (:105-110)
+      const double bEff = jmin(L, X), tRef = W.thick[0];
+      double* ts = W.ts;
+      double comp = 0, tMean = 0;
+      for (int q = 0; q < WALL_NZ; q++) {
+        const double tq = W.thick[1 + q];
+        ts[q] = tq > 0 && tRef > 0 ? jmax(t * tq / tRef, t) : t;
+      }
+      for (int q = 0; q < WALL_NZ; q++) {
+        const double z = (q + 0.5) / WALL_NZ * a, Iq = bEff * mcm::pow_js(ts[q], 3.0) / 12;
+        comp += (a - z) * (a - z) / (E * Iq) * (a / WALL_NZ);
+      }
+      const double k = 1 / comp, I = bEff * mcm::pow_js(t, 3.0) / 12;
+      ok = !(k > 20 * kTool);
+      if (ok) {
+        for (int q = 0; q < WALL_NZ; q++) tMean = tMean + ts[q];
+        tMean = tMean / WALL_NZ;
+        double m = 0.24 * rho * 1e-9 * tMean * bEff * hWall;
+        double fn = __builtin_sqrt(k * 1e3 / m) / (2 * PI);
+        if (fn > MC_WALL_FN_CAP) {
+          const double wc = 2 * PI * MC_WALL_FN_CAP;
+          fn = MC_WALL_FN_CAP;
+          m = k * 1e3 / (wc * wc);
+        }
+        mc_wall_out& r = W.rec;
+        r.k = k; r.m = m; r.fn = fn; r.zeta = 0.02; r.nx = nx; r.ny = ny; r.t = t; r.tRoot = ts[0];
+        r.hWall = hWall; r.a = a; r.bEff = bEff; r.base = base; r.cx = cx; r.cy = cy; r.I = I;
+        r.found = 1;
+      }
+    }
+    W.go[F_FOUND] = ok;
+  }
+  __syncthreads();
+  return W.go[F_FOUND] != 0;
+}
+
 // the dirty rectangle joins a changed stamp's box (src/stock.js:156-160)
 __device__ __forceinline__ void join_rect(const StockDev& S, const Box& b) {
   atomicMin(&S.rect[0], b.i0); atomicMin(&S.rect[1], b.j0);
   atomicMax(&S.rect[2], b.i1); atomicMax(&S.rect[3], b.j1);
 }
 
+template <bool FLEX>
 __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   __shared__ EngLds L;
   __shared__ double red[NT / 64];
   __shared__ int flags[2];   // flags[1]: the actual stamp changed a node
+  __shared__ WallLds WL[1];
   const ProgDev& P = progs[blockIdx.x];
   EngDev& E = *P.eng;
   const int tid = threadIdx.x;
@@ -367,6 +627,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   double fzLast = P.fz0;   // this.fz: the last cutting sub-step's feed per tooth
   double dynLeft = P.dynBudget;   // the frame's dynamics budget left (src/sim.js:360)
   int frame = 0;                  // frames ended so far
+  bool held = false;              // this.wall is not null (thread 0's)
   for (int k = 0; k < P.nsub; k++) {
     const mc_substep s = P.sub[k];
     if (s.frame & MC_FRAME_FIRST) dynLeft = P.dynBudget;
@@ -428,15 +689,31 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     int n = 0;
     // the commanded stamp does not depend on the dynamics: waves 1.. stamp it while wave 0
     // runs the engine
-    if (tid == 0) { flags[0] = 0; flags[1] = 0; }
+    if (tid == 0) {
+      flags[0] = 0; flags[1] = 0;
+      if (FLEX) WL[0].rec = mc_wall_out{};
+    }
     __threadfence_block();
     __syncthreads();
     if (cut) {
-      // setEngagement, setEndEngagement, setWork(null) (src/sim.js:403-408), on thread 0
+      // setEngagement, setEndEngagement (src/sim.js:403-404) on thread 0; the wall at the load
+      // height with the thin-wall model, else none (:407), and setWork (:408)
       if (tid == 0) {
         eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball, P.rc);
         eng_set_end_engagement(E, endCut ? s.fza : 0, haveEnd ? rOut : 0, nullptr, shape);
-        eng_set_work_null(E);
+      }
+      bool found = false;
+      if (FLEX && P.flex && ap > 0.01)   // block-uniform
+        found = wall_at_block(P.act, E.Hb, P.sacc, P.n20, P.n60, P.wstep, s.x, s.y, s.z, R, s.ux, s.uy,
+                              s.z + ap / 2, P.wallE, P.wallRho, P.kTool, WL[0]);
+      if (tid == 0) {
+        if (FLEX && found) {
+          const mc_wall_out& w = WL[0].rec;
+          eng_set_work(E, w.k, w.m, w.zeta, w.nx, w.ny);
+        } else {
+          eng_set_work_null(E);
+        }
+        held = found;
         E.hMax = jmax(0.1, 3 * E.fz);
       }
       __threadfence_block();
@@ -454,8 +731,12 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
       if (tid < 64) engine_steps(E, n, L);
       fzLast = s.fzEff;
     } else {
-      // out of the cut the vibration decays (:427-428)
-      if (tid == 0) { E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5; }
+      // out of the cut the vibration decays, and a wall whose deflection has died out is
+      // dropped (:427-429)
+      if (tid == 0) {
+        E.x *= 0.5; E.y *= 0.5; E.vx *= 0.5; E.vy *= 0.5; E.w *= 0.5; E.wv *= 0.5;
+        if (truthy(E.wk) && __builtin_fabs(E.w) < 1e-7) { held = false; eng_set_work_null(E); }
+      }
     }
     __threadfence_block();
     __syncthreads();
@@ -471,6 +752,15 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     if (tid == 0 && flags[1]) join_rect(P.act, ba);
     __threadfence_block();
     __syncthreads();
+    // the wall-error mark while a wall is held (:443): recorded for the caller's map
+    if (FLEX && P.walls && tid == 0) {
+      mc_wall_out& w = WL[0].rec;
+      if (held && truthy(E.wk)) {
+        const double v = E.runMeanW * 1e6;
+        w.marked = 1; w.markVal = v > 0 ? v : 0.0; w.wnx = E.wnx; w.wny = E.wny;
+      }
+      P.walls[k] = w;
+    }
     if (P.out && tid == 0) {
       mc_substep_out& o = P.out[k];
       o.dx = dx; o.dy = dy; o.ap = ap; o.errVal = errVal; o.vol = vol; o.maxF = maxF; o.nsteps = n; o.cut = cut ? 1 : 0;
@@ -527,6 +817,16 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
       return mch::fail(MC_ERR_ARG, "program %d: invalid radius, wear, flute length, feed, share or budget", p);
     if (g.nframes < 0 || (g.nframes > 0 && !g.frames))
       return mch::fail(MC_ERR_ARG, "program %d: null frames with a frame capacity", p);
+    if (g.flex != 0 && g.flex != 1) return mch::fail(MC_ERR_ARG, "program %d: flex must be 0 or 1", p);
+    // The device form of the thin-wall model (sim_run<true>, wall_at_block) is switched off:
+    // its first run on an MI355X ended in a memory access fault whose cause is not found (the
+    // same code run on the host, every buffer against a guard page, matches the oracle bit for
+    // bit).  Until the cause is shown a program with flex is refused here, before any launch.
+    if (g.flex)
+      return mch::fail(MC_ERR_LIMIT, "program %d: the thin-wall model (flex) is switched off in the device pipeline", p);
+    if (g.flex && !(std::isfinite(g.wallE) && g.wallE > 0 && std::isfinite(g.wallRho) && g.wallRho > 0 &&
+                    std::isfinite(g.kTool) && g.kTool > 0))
+      return mch::fail(MC_ERR_ARG, "program %d: the wall model needs a positive E, density and tool stiffness", p);
     for (const void* q : {(const void*)g.actual, (const void*)g.ideal, (const void*)g.engine})
       for (const void* o : seen)
         if (o == q) return mch::fail(MC_ERR_ARG, "program %d shares a stock or an engine with another", p);
@@ -560,6 +860,14 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     d.rc = g.type == MC_TOOL_BULL ? std::min(g.R, g.cornerR) : 0.0;
     d.dynBudget = g.dynBudget > 0 ? g.dynBudget : 0; d.fz0 = g.fz;
     d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0; d.pad = 0;
+    d.sacc = nullptr; d.walls = nullptr; d.flex = g.flex; d.n20 = 0; d.n60 = 0; d.pad2 = 0;
+    d.wallE = g.wallE; d.wallRho = g.wallRho; d.kTool = g.kTool;
+    // the wall model's walks step by half a cell (src/wall.js:37): at least 1e-4 mm, so that
+    // a walk to 60 mm stays under 600,000 steps, and at most 5 mm, so that every walk has
+    // steps before its end
+    d.wstep = std::min(a.hx, a.hy) * 0.5;
+    if (g.flex && !(d.wstep >= 1e-4 && d.wstep <= 5))
+      return mch::fail(MC_ERR_LIMIT, "program %d: the wall model needs cells of 0.0002 to 10 mm", p);
   }
   // The commanded stamps (Simulator.ideal, src/sim.js:436) depend only on the schedule and
   // nothing in the chain reads the commanded stock, so each program's are applied as one
@@ -593,36 +901,80 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (hipGetLastError() != hipSuccess) return mch::fail(MC_ERR_HIP, "strip maxima launch failed");
   }
   // sub-steps in, results out: one device buffer for every program
-  size_t nin = 0, nlut = 0, nfr = 0;
+  size_t nin = 0, nlut = 0, nfr = 0, nwall = 0;
+  bool flex = false;
   for (int p = 0; p < n; p++) {
     nin += (size_t)progs[p].nsub;
     if (progs[p].type == MC_TOOL_BULL) nlut += (size_t)progs[p].lutn;
     nfr += (size_t)pd[(size_t)p].nframes;
+    if (progs[p].flex && progs[p].walls) nwall += (size_t)progs[p].nsub;
+    flex = flex || progs[p].flex;
+  }
+  // the wall model's walk table, s = step, s + step, ... up to 60 mm, summed in the
+  // reference's order; one per distinct step
+  struct Table { double step; size_t at; int n20, n60; };
+  std::vector<double> sacc;
+  std::vector<Table> tables;
+  std::vector<size_t> soff((size_t)n, 0);
+  for (int p = 0; p < n; p++) {
+    ProgDev& d = pd[(size_t)p];
+    if (!d.flex) continue;
+    const Table* tb = nullptr;
+    for (const Table& t : tables)
+      if (t.step == d.wstep) { tb = &t; break; }
+    if (!tb) {
+      Table t{d.wstep, sacc.size(), 0, 0};
+      for (double v = d.wstep; v <= 60; v += d.wstep) {
+        sacc.push_back(v);
+        t.n60++;
+        if (v <= 20) t.n20++;
+      }
+      tables.push_back(t);
+      tb = &tables.back();
+    }
+    soff[(size_t)p] = tb->at;
+    d.n20 = tb->n20; d.n60 = tb->n60;
   }
   float* dlut = nullptr;
   mc_substep* dsub = nullptr;
   mc_substep_out* dout = nullptr;
   mc_frame_out* dfr = nullptr;
   ProgDev* dprog = nullptr;
+  double* dsacc = nullptr;
+  mc_wall_out* dwall = nullptr;
   auto release = [&]() {
     if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
     if (dout) (void)hipFree(dout);
     if (dfr) (void)hipFree(dfr);
     if (dprog) (void)hipFree(dprog);
+    if (dsacc) (void)hipFree(dsacc);
+    if (dwall) (void)hipFree(dwall);
   };
   if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dfr, sizeof(mc_frame_out) * std::max<size_t>(1, nfr)) != hipSuccess ||
       hipMalloc(&dprog, sizeof(ProgDev) * (size_t)n) != hipSuccess ||
-      hipMalloc(&dlut, sizeof(float) * std::max<size_t>(1, nlut)) != hipSuccess) {
+      hipMalloc(&dlut, sizeof(float) * std::max<size_t>(1, nlut)) != hipSuccess ||
+      hipMalloc(&dsacc, sizeof(double) * std::max<size_t>(1, sacc.size())) != hipSuccess ||
+      hipMalloc(&dwall, sizeof(mc_wall_out) * std::max<size_t>(1, nwall)) != hipSuccess) {
     release();
     return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
   }
-  size_t off = 0, loff = 0, foff = 0;
+  size_t off = 0, loff = 0, foff = 0, woff = 0;
   int st = MC_OK;
+  if (!sacc.empty() &&
+      hipMemcpyAsync(dsacc, sacc.data(), sizeof(double) * sacc.size(), hipMemcpyHostToDevice, q) != hipSuccess)
+    st = mch::fail(MC_ERR_HIP, "wall table upload");
   for (int p = 0; p < n && !st; p++) {
     const mc_sim_program& g = progs[p];
+    if (g.flex) {
+      pd[(size_t)p].sacc = dsacc + soff[(size_t)p];
+      if (g.walls) {
+        pd[(size_t)p].walls = dwall + woff;
+        woff += (size_t)g.nsub;
+      }
+    }
     if (g.type == MC_TOOL_BULL) {
       pd[(size_t)p].lut = dlut + loff;
       if (hipMemcpyAsync(dlut + loff, g.lut, sizeof(float) * (size_t)g.lutn, hipMemcpyHostToDevice, q) != hipSuccess)
@@ -649,7 +1001,10 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     st = mch::fail(MC_ERR_HIP, "timing events");
   if (!st) {
     (void)hipEventRecord(e0, q);
-    hipLaunchKernelGGL(sim_run, dim3(n), dim3(NT), 0, q, (const ProgDev*)dprog);
+    if (flex)
+      hipLaunchKernelGGL(sim_run<true>, dim3(n), dim3(NT), 0, q, (const ProgDev*)dprog);
+    else
+      hipLaunchKernelGGL(sim_run<false>, dim3(n), dim3(NT), 0, q, (const ProgDev*)dprog);
     if (hipGetLastError() != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline launch failed");
     (void)hipEventRecord(e1, q);
   }
@@ -658,6 +1013,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   const bool launched = !st;
   off = 0;
   foff = 0;
+  woff = 0;
   for (int p = 0; p < n && launched; p++) {
     const mc_sim_program& g = progs[p];
     if (!st && g.out && g.nsub > 0 &&
@@ -669,6 +1025,12 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
         hipMemcpyAsync(g.frames, dfr + foff, sizeof(mc_frame_out) * (size_t)nf, hipMemcpyDeviceToHost, q) != hipSuccess)
       st = mch::fail(MC_ERR_HIP, "frame download");
     foff += (size_t)nf;
+    if (g.flex && g.walls) {
+      if (!st && g.nsub > 0 &&
+          hipMemcpyAsync(g.walls, dwall + woff, sizeof(mc_wall_out) * (size_t)g.nsub, hipMemcpyDeviceToHost, q) != hipSuccess)
+        st = mch::fail(MC_ERR_HIP, "wall record download");
+      woff += (size_t)g.nsub;
+    }
     const int fst = mce_engine_finish(g.engine);
     if (!st) st = fst;
   }

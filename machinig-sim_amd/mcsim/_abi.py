@@ -151,13 +151,26 @@ class SubstepOut(C.Structure):
                [("nsteps", C.c_int32), ("cut", C.c_int32), ("arc", C.c_int32), ("pad", C.c_int32)]
 
 
+WALL_FIELDS = ("k", "m", "fn", "zeta", "nx", "ny", "t", "tRoot", "hWall", "a", "bEff", "base", "cx", "cy", "I")
+
+
+class WallOut(C.Structure):
+    """mc_wall_out: the wall wallAt found at a sub-step and the wall-error mark after it"""
+    _fields_ = [(n, C.c_double) for n in WALL_FIELDS + ("markVal", "wnx", "wny")] + \
+               [("found", C.c_int32), ("marked", C.c_int32)]
+
+
+MC_WALL_FN_CAP = 5000
+
+
 class SimProgram(C.Structure):
     _fields_ = [("actual", C.c_void_p), ("ideal", C.c_void_p), ("engine", C.c_void_p),
                 ("sub", C.POINTER(Substep)), ("out", C.POINTER(SubstepOut)),
                 ("R", C.c_double), ("fluteLen", C.c_double), ("rw", C.c_double), ("share", C.c_double),
                 ("nsub", C.c_int32), ("type", C.c_int32), ("lut", C.POINTER(C.c_float)), ("lutn", C.c_int32),
                 ("nframes", C.c_int32), ("cornerR", C.c_double), ("dynBudget", C.c_double), ("fz", C.c_double),
-                ("frames", C.c_void_p)]
+                ("frames", C.c_void_p), ("flex", C.c_int32), ("pad", C.c_int32), ("wallE", C.c_double),
+                ("wallRho", C.c_double), ("kTool", C.c_double), ("walls", C.POINTER(WallOut))]
 
 
 MC_STOCK_B = 8

@@ -11,7 +11,7 @@ import numpy as np
 from ._abi import (Cell, Trial, BisectOpts, Limit, Stamp, StampOpt, TrainOpts, LobeArgs, Mode,
                    SearchTuning, EngageQuery, EngageResult, EngageSpec, cell_array, MC_OK, MC_SURR_IN,
                    MC_STOCK_B, EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST,
-                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram, MC_MAX_MODES, FrameOut,
+                   MC_ENGINE_RING, Substep, SubstepOut, SimProgram, WallOut, MC_MAX_MODES, FrameOut,
                    MC_FRAME_LAST)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -323,17 +323,19 @@ def js_math(fn, x, y=None):
     return out
 
 
-def sim_run(programs, frames=False):
+def sim_run(programs, frames=False, walls=False):
     """the simulator's sub-step loop on the device for several programs in one launch.
     programs: dicts with actual, ideal (Stock), engine (Engine), sub (Substep array) and
     optionally nsub (its length), R, fluteLen, rw, share, type (MC_TOOL_*), for a bull-nose
     tool lut (its tipLut) and cornerR, dynBudget (> 0: the frame budget's share per
-    sub-step instead of the fixed share) and fz (this.fz at the start); returns a SubstepOut
-    list per program, and with frames=True also a FrameOut list per program (one per
-    sub-step flagged MC_FRAME_LAST)"""
+    sub-step instead of the fixed share), fz (this.fz at the start) and wall (dict(E, rho,
+    kTool): the thin-wall model, workpieceFlex; the engine configured with fnMax
+    MC_WALL_FN_CAP); returns a SubstepOut list per program, with frames=True also a FrameOut
+    list per program (one per sub-step flagged MC_FRAME_LAST), and with walls=True also a
+    WallOut list per program (None for a rigid workpiece), in that order"""
     n = len(programs)
     arr = (SimProgram * max(1, n))()
-    outs, keep, frs = [], [], []
+    outs, keep, frs, wls = [], [], [], []
     for i, g in enumerate(programs):
         sub = g["sub"]
         ns = int(g.get("nsub", len(sub)))
@@ -347,6 +349,9 @@ def sim_run(programs, frames=False):
         nf = sum(1 for k in range(ns) if sub[k].frame & MC_FRAME_LAST) if frames else 0
         fr = (FrameOut * max(1, nf))()
         frs.append((fr, nf))
+        w = g.get("wall")
+        wo = (WallOut * max(1, ns))() if w else None
+        wls.append((wo, ns))
         arr[i] = SimProgram(g["actual"].p.value, g["ideal"].p.value, g["engine"].p.value,
                             C.cast(sub, C.POINTER(Substep)), C.cast(out, C.POINTER(SubstepOut)),
                             float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g.get("share", 1.0)),
@@ -354,12 +359,16 @@ def sim_run(programs, frames=False):
                             lut_arr.ctypes.data_as(C.POINTER(C.c_float)) if lut_arr is not None else None,
                             len(lut_arr) if lut_arr is not None else 0, nf, float(g.get("cornerR", 0.0)),
                             float(g.get("dynBudget", 0.0)), float(g.get("fz", 0.0)),
-                            C.cast(fr, C.c_void_p) if nf else None)
+                            C.cast(fr, C.c_void_p) if nf else None, 1 if w else 0, 0,
+                            float(w["E"]) if w else 0.0, float(w["rho"]) if w else 0.0,
+                            float(w["kTool"]) if w else 0.0, C.cast(wo, C.POINTER(WallOut)) if w else None)
     _check(load().mc_sim_run(arr, n))
-    res = [o[:ns] for o, ns in outs]
+    res = [[o[:ns] for o, ns in outs]]
     if frames:
-        return res, [f[:nf] for f, nf in frs]
-    return res
+        res.append([f[:nf] for f, nf in frs])
+    if walls:
+        res.append([wo[:ns] if wo is not None else None for wo, ns in wls])
+    return res[0] if len(res) == 1 else tuple(res)
 
 
 class Stock:

@@ -473,6 +473,7 @@ void or_math_batch(int fn, const double *x, const double *y, double *out, int64_
       case 3: out[i] = or_pow(x[i], y[i]); break;
       case 4: out[i] = or_hypot(x[i], y[i]); break;
       case 5: out[i] = or_atan(x[i]); break;
+      case 6: out[i] = or_atan2(x[i], y[i]); break;
       default: out[i] = NAN;
     }
   }

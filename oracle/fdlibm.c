@@ -494,3 +494,42 @@ double or_atan(double x) {
   z = atanhi[id] - ((x * (s1 + s2) - atanlo[id]) - x);
   return (hx < 0) ? -z : z;
 }
+
+/* Math.atan2 (fdlibm e_atan2.c as FreeBSD's msun keeps it): NaN, x = 1, zeros and infinities
+   first, then atan(|y / x|) put in the quadrant with pi's low part; a ratio beyond 2^60 gives
+   +-pi/2 whatever the sign of x, and a negative x with a ratio under 2^-60 gives +-pi */
+double or_atan2(double y, double x) {
+  static const double tiny = 1.0e-300, pi_o_4 = 7.8539816339744827900e-01, pi_o_2 = 1.5707963267948965580e+00,
+                      pi = 3.1415926535897931160e+00, pi_lo = 1.2246467991473531772e-16;
+  const int32_t hx = hi_word(x), hy = hi_word(y), ix = hx & 0x7fffffff, iy = hy & 0x7fffffff;
+  int32_t k, m;
+  double z;
+  if (isnan(x) || isnan(y)) return x + y;
+  if (x == 1.0) return or_atan(y);
+  m = ((hy >> 31) & 1) | ((hx >> 30) & 2);   /* 2 sign(x) + sign(y) */
+  if (y == 0) {
+    if (m < 2) return y;
+    return m == 2 ? pi + tiny : -pi - tiny;
+  }
+  if (x == 0) return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+  if (isinf(x)) {
+    if (isinf(y)) {
+      if (m == 0) return pi_o_4 + tiny;
+      if (m == 1) return -pi_o_4 - tiny;
+      if (m == 2) return 3.0 * pi_o_4 + tiny;
+      return -3.0 * pi_o_4 - tiny;
+    }
+    if (m == 0) return 0.0;
+    if (m == 1) return -0.0;
+    return m == 2 ? pi + tiny : -pi - tiny;
+  }
+  if (isinf(y)) return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+  k = (iy - ix) >> 20;
+  if (k > 60) { z = pi_o_2 + 0.5 * pi_lo; m &= 1; }
+  else if (hx < 0 && k < -60) z = 0.0;
+  else z = or_atan(fabs(y / x));
+  if (m == 0) return z;
+  if (m == 1) return -z;
+  if (m == 2) return pi - (z - pi_lo);
+  return (z - pi_lo) - pi;
+}

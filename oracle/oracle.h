@@ -21,11 +21,12 @@ double or_pow(double x, double y);
 double or_hypot(double a, double b);
 double or_js_round(double x);
 double or_atan(double x);
+double or_atan2(double y, double x);
 int or_rem_pio2(double x, double *y);
 double or_kernel_sin(double x, double y, int iy);
 double or_kernel_cos(double x, double y);
 double or_kernel_tan(double x, double y, int iy);
-/* fn: 0 sin, 1 cos, 2 tan, 3 pow(x, y), 4 hypot(x, y), 5 atan */
+/* fn: 0 sin, 1 cos, 2 tan, 3 pow(x, y), 4 hypot(x, y), 5 atan, 6 atan2(x, y) = Math.atan2(x, y) */
 void or_math_batch(int fn, const double *x, const double *y, double *out, int64_t n);
 
 /* engine.c: CuttingEngine / chatterAt / timeDomainLimit restated */
@@ -124,6 +125,7 @@ void or_stock_clear_dirty(or_stock *s);
 float or_stock_block_max(or_stock *s, int32_t bi, int32_t bj);
 void or_stock_max_in_disk(or_stock *s, double cx, double cy, double r, double above, double *out);
 double or_stock_sample(const or_stock *s, double x, double y);
+double or_stock_zbottom(const or_stock *s);
 /* engage.c: Simulator.fillEngagement / immersion (src/sim.js:301-348) */
 double or_fill_engagement(const or_stock *s, double cx, double cy, double ux, double uy, double R,
                           float *Hb);
@@ -144,7 +146,11 @@ double or_fill_end_engagement(const or_stock *s, double cx, double cy, double ti
                               double R, int ball, float *Eb, double *rOut_out);
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
-               mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes);
+               mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes, int flex,
+               double E, double rho, double kTool, mc_wall_out *walls);
+/* wall.c: wallAt (src/wall.js:11-112); 1 with the wall fields of *w set, 0 for null */
+int or_wall_at(const or_stock *s, double px, double py, double pz, double R, const float *Hb, double ux, double uy,
+               double zLoad, double E, double rho, double kTool, mc_wall_out *w);
 int or_engine_spt(const or_engine *e);
 
 #ifdef __cplusplus

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "machinig-sim_amd"))
 
-from mcsim._abi import (Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec,  # noqa: E402
+from mcsim._abi import (Cell, Trial, BisectOpts, Limit, Mode, Stamp, StampOpt, EngageSpec, WallOut,  # noqa: E402
                         EngineInfo, Wall, EngineState, EngineStats, ChipEvent, MC_ENGINE_HIST, MC_ENGINE_RING,
                         Substep, SubstepOut, FrameOut)
 
@@ -62,7 +62,12 @@ def lib():
         L.or_engine_chatter_index.argtypes = [P, P]
         L.or_engine_read_rings.argtypes = [P, P, P, P, P]
         L.or_sim_run.argtypes = [P, P, P, C.POINTER(Substep), C.c_int] + [C.c_double] * 4 + \
-            [C.c_int, P, C.c_int, C.c_double, C.POINTER(SubstepOut), C.c_double, C.c_double, P, C.c_int]
+            [C.c_int, P, C.c_int, C.c_double, C.POINTER(SubstepOut), C.c_double, C.c_double, P, C.c_int,
+             C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(WallOut)]
+        L.or_wall_at.argtypes = [P] + [C.c_double] * 4 + [P] + [C.c_double] * 6 + [C.POINTER(WallOut)]
+        L.or_wall_at.restype = C.c_int
+        L.or_stock_zbottom.argtypes = [P]
+        L.or_stock_zbottom.restype = C.c_double
         L.or_directional_coeffs.argtypes = [C.c_double, C.c_double, C.c_double, P]
         L.or_stability_lobes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_int, C.c_double, C.c_double, C.POINTER(Mode), C.c_int,
@@ -110,7 +115,7 @@ def lib():
     return _LIB
 
 
-MATH_FN = dict(sin=0, cos=1, tan=2, pow=3, hypot=4, atan=5)
+MATH_FN = dict(sin=0, cos=1, tan=2, pow=3, hypot=4, atan=5, atan2=6)
 
 
 def math_batch(fn, x, y=None):
@@ -279,23 +284,42 @@ class OracleEngine:
 
 
 def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None, cornerR=0.0, dynBudget=0.0,
-            fz0=0.0, nframes=0):
+            fz0=0.0, nframes=0, wall=None):
     """the simulator's sub-step loop restated (oracle/simloop.c); returns SubstepOut records,
-    and with nframes > 0 also the FrameOut records of the frames that ended"""
+    and with nframes > 0 also the FrameOut records of the frames that ended.  wall: None for a
+    rigid workpiece, or dict(E, rho, kTool) for the thin-wall model (workpieceFlex); then the
+    WallOut records of every sub-step come last in the returned tuple"""
     import numpy as np
     out = (SubstepOut * max(1, n))()
     fr = (FrameOut * max(1, nframes))()
+    wo = (WallOut * max(1, n))()
     la = np.ascontiguousarray(lut, dtype=np.float32) if lut is not None else None
+    w = wall or {}
     st = lib().or_sim_run(actual.p, ideal.p, engine.h, C.cast(sub, C.POINTER(Substep)), int(n), float(R),
                           float(fluteLen), float(rw), float(share), int(typ),
                           la.ctypes.data if la is not None else None, len(la) if la is not None else 0,
-                          float(cornerR), out, float(dynBudget), float(fz0), fr if nframes else None, int(nframes))
+                          float(cornerR), out, float(dynBudget), float(fz0), fr if nframes else None, int(nframes),
+                          1 if wall else 0, float(w.get("E", 0.0)), float(w.get("rho", 0.0)),
+                          float(w.get("kTool", 0.0)), wo if wall else None)
     if st:
         raise RuntimeError("or_sim_run: %d" % st)
-    if not nframes:
-        return out[:n]
-    ended = min(nframes, sum(1 for k in range(n) if sub[k].frame & 2))
-    return out[:n], fr[:ended]
+    res = [out[:n]]
+    if nframes:
+        ended = min(nframes, sum(1 for k in range(n) if sub[k].frame & 2))
+        res.append(fr[:ended])
+    if wall:
+        res.append(wo[:n])
+    return res[0] if len(res) == 1 else tuple(res)
+
+
+def wall_at(stock, px, py, pz, R, Hb, ux, uy, zLoad, E, rho, kTool):
+    """wallAt restated (oracle/wall.c): a WallOut with found 0 (null) or the wall's fields"""
+    import numpy as np
+    hb = np.ascontiguousarray(Hb, dtype=np.float32)
+    w = WallOut()
+    w.found = lib().or_wall_at(stock.p, float(px), float(py), float(pz), float(R), hb.ctypes.data, float(ux),
+                               float(uy), float(zLoad), float(E), float(rho), float(kTool), C.byref(w))
+    return w
 
 
 def composite_lobes(nodes, step, hi):
@@ -347,6 +371,14 @@ class OracleStock:
 
     def H(self):
         return self._arr("or_stock_h", C.c_float)
+
+    def set_H(self, H):
+        """overwrite the heights (for reading a recorded state: sample, floor and the wall model
+        read H only; the block maxima are not refreshed)"""
+        import numpy as np
+        h = np.ascontiguousarray(H, dtype=np.float32)
+        assert h.size == self.nx * self.ny
+        C.memmove(self.L.or_stock_h(self.p), h.ctypes.data, h.nbytes)
 
     def err(self):
         return self._arr("or_stock_err", C.c_float)

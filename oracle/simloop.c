@@ -6,12 +6,14 @@
  *   fillEngagement     src/sim.js:301-319  (or_fill_engagement, engage.c)
  *   fillEndEngagement  src/sim.js:324-341  (below)
  *   axial depth, end cut, the cut decision                 src/sim.js:388-401
- *   setEngagement / setEndEngagement / setWork(null)       src/sim.js:403-408
+ *   setEngagement / setEndEngagement, the wall, setWork    src/sim.js:403-408
+ *     (workpieceFlex: wallAt at the load height tipZ + ap / 2, oracle/wall.c; else null)
  *   the run's step count (the frame budget's share)        src/sim.js:358-362,409-414
  *   the frame's end: takeStats() and chatterIndex()        src/sim.js:450,645-663
  *   the engagement arc it shows (immersion, :343-348)      src/sim.js:421-423
- *   the decay out of the cut                               src/sim.js:425-429
+ *   the decay out of the cut, a decayed wall dropped       src/sim.js:425-429
  *   deflection, the commanded and the actual stamp         src/sim.js:433-440
+ *   the wall-error mark (markWall) while a wall is held    src/sim.js:443
  *
  * The engine is oracle/cengine.c's object, the stocks oracle/stock.c's.  The tool is flat,
  * ball or bull-nose; the end-face shape zAt is the simulator's (src/sim.js:216-218: the tip
@@ -95,14 +97,15 @@ static void end_rings(double rOut, const Shape *t, double *g) {
 
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
-               mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes) {
+               mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes, int flex,
+               double E, double rho, double kTool, mc_wall_out *walls) {
   const int ball = type == MC_TOOL_BALL, bull = type == MC_TOOL_BULL;
   const Shape shape = {ball, R, bull ? lut : NULL, bull ? lutn : 0, bull ? (cornerR < R ? cornerR : R) : 0};
   const float *slut = bull ? lut : NULL;
   const int stype = ball ? MC_TOOL_BALL : MC_TOOL_FLAT;
   float Hb[MC_BINS], Eb[MC_END_BINS * MC_END_RINGS];
   double fzLast = fz0, dynLeft = dynBudget;
-  int k, frame = 0;
+  int k, frame = 0, held = 0;   /* held: this.wall is not null */
   memset(Hb, 0, sizeof(Hb));
   memset(Eb, 0, sizeof(Eb));
   for (k = 0; k < n; k++) {
@@ -111,6 +114,8 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     double hmax = -INFINITY, ap, dmax = 0, rOut = 0, dx, dy, errVal, vol, maxF = 0;
     int haveEnd = s->fza > 0, endCut, cut, nsteps = 0, arc;
     double phiSt, phiEx;
+    mc_wall_out wo;
+    memset(&wo, 0, sizeof(wo));
     if (s->frame & MC_FRAME_FIRST) dynLeft = dynBudget;
     if (!(s->dxy < 1e-6)) hmax = or_fill_engagement(act, s->x, s->y, s->ux, s->uy, R, Hb);
     ap = hmax - s->z > 0 ? hmax - s->z : 0;
@@ -145,7 +150,15 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
       }
       end_rings(haveEnd ? rOut : 0, &shape, g);
       or_engine_set_end_engagement(e, endCut ? s->fza : 0, haveEnd ? rOut : 0, g);
-      or_engine_set_work(e, NULL);
+      held = flex && ap > 0.01 &&
+             or_wall_at(act, s->x, s->y, s->z, R, Hb, s->ux, s->uy, s->z + ap / 2, E, rho, kTool, &wo);
+      wo.found = held;
+      if (held) {
+        const mc_wall wl = {wo.k, wo.m, wo.zeta, wo.nx, wo.ny};
+        or_engine_set_work(e, &wl);
+      } else {
+        or_engine_set_work(e, NULL);
+      }
       or_engine_set_maps(e, Hb, Eb);
       want = or_js_round(s->dtm / or_engine_dt(e));
       if (want < 1) want = 1;
@@ -173,6 +186,7 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     } else {
       st.x *= 0.5; st.y *= 0.5; st.vx *= 0.5; st.vy *= 0.5; st.w *= 0.5; st.wv *= 0.5;
       or_engine_set_state(e, &st);
+      if (st.wk != 0 && fabs(st.w) < 1e-7) { held = 0; or_engine_set_work(e, NULL); }
     }
     or_engine_get_state(e, &st);
     dx = (st.x - st.w * st.wnx) * 1e3;
@@ -184,6 +198,12 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
       vol = or_stock_stamp_ex(act, s->x + dx, s->y + dy, s->z + (ball ? rw : 0), R - rw, stype, errVal, s->line, mark,
                               slut, shape.lutn);
     }
+    if (held && st.wk != 0) {
+      const double v = st.runMeanW * 1e6;
+      wo.marked = 1; wo.markVal = v > 0 ? v : 0; wo.wnx = st.wnx; wo.wny = st.wny;
+      or_stock_mark_wall(act, s->x + dx, s->y + dy, R, st.wnx, st.wny, wo.markVal);
+    }
+    if (walls) walls[k] = wo;
     if (out) {
       out[k].dx = dx; out[k].dy = dy; out[k].ap = ap; out[k].errVal = errVal; out[k].vol = vol;
       out[k].maxF = maxF; out[k].nsteps = nsteps; out[k].cut = cut;

@@ -81,6 +81,7 @@ void or_stock_dims(const or_stock *s, int32_t *nx, int32_t *ny, double *hx, doub
   *nx = s->nx; *ny = s->ny; *hx = s->hx; *hy = s->hy;
 }
 void or_stock_set_wall_band(or_stock *s, double band) { s->wallBand = band; }
+double or_stock_zbottom(const or_stock *s) { return s->zBottom; }
 float *or_stock_h(or_stock *s) { return s->H; }
 float *or_stock_err(or_stock *s) { return s->err; }
 float *or_stock_walld(or_stock *s) { return s->wallD; }

@@ -23,6 +23,9 @@ below by run_reference.mjs.  The outputs are written as data to tests/golden/*.j
   golden_simx.json      Simulator runs with a rigid workpiece and the linear force law: the
                         path with feeds and speeds, the engine's configure / wear arguments,
                         every stamp, final maps
+  golden_simw.json      Simulator runs with the thin-wall model on (workpieceFlex): what simx
+                        records, every setWork argument (wallAt's answers), every markWall call,
+                        the wall-error map, and the whole input and answer of chosen wallAt calls
   golden_paths.json     the G-code parser's moves and the path discretizer's polylines
   golden_clobes.json    Simulator.lobes: the composite (speed-dependent) lobe curve and the
                         diagram at rest, with the speeds and modes its nodes were built from
@@ -227,6 +230,17 @@ def build_cases(quick):
     # rigid workpiece, linear force law, flat, ball and bull-nose tools: the device sub-step pipeline
     cases["simx"] = [dict(cases["sim"][0]), dict(cases["sim"][1]), dict(ribc),
                      dict(cases["sim"][0], tool=dict(type="ball", D=8.0)), dict(cases["sim"][2])]
+    # the thin-wall model on (workpieceFlex, the simulator's default): the rib program above with
+    # the linear force law and with the size effect, and a taller 1.5 mm wall between two slots,
+    # finished on its face, at 0.1 mm cells; `every` / `maxSnaps` choose the wallAt calls whose
+    # whole input is kept
+    wall = ["G90 G54 G17", "S8000 M3", "G0 X-6 Y4 Z5", "G1 Z-8 F200", "G1 X36 F600", "G0 Z5", "G0 X-6 Y13.5",
+            "G1 Z-8 F200", "G1 X36 F600", "G1 Y13.3", "G1 X-6 F500", "G0 Z5", "M30"]
+    cases["simw"] = [dict(ribc, nonlinear=False, every=9, maxSnaps=10),
+                     dict(ribc, nonlinear=True, every=13, maxSnaps=4),
+                     dict(program="\n".join(wall), tool=dict(type="flat", D=8.0),
+                          stock=dict(type="box", w=30.0, l=20.0, h=12.0), cell=0.1, nonlinear=False, every=25,
+                          maxSnaps=10)]
     # real-time frames: the same programs stepped at playback speeds 20 and 120 (the second
     # compresses the dynamics to the frame budget's share), and a longer raster at speed 60
     raster = ["G90 G54 G17", "S10000 M3", "G0 X-6 Y2 Z5", "G1 Z-1.2 F300"]
@@ -365,7 +379,7 @@ def generate(cases, quick, work):
                 m["k"], m["fn"] = tm["k"], tm["fn"]
             md["k"], md["fn"] = tm["k"], tm["fn"]
     keys = ("chatter", "tdl", "trace", "stock", "lobes", "toolModel", "tiplut", "engage", "sim", "spindle",
-            "enginex", "englog", "simx", "paths", "clobes", "report")
+            "enginex", "englog", "simx", "simw", "paths", "clobes", "report")
     if "--only" in sys.argv:
         keys = tuple(sys.argv[sys.argv.index("--only") + 1].split(","))
         # engage cases read the stock cases

@@ -281,6 +281,106 @@ if (cases.simx) {
   console.error('simx', (Date.now() - t0) / 1e3, 's');
 }
 
+// Simulator runs with the thin-wall model on (workpieceFlex, the simulator's default): what
+// simx records, plus every setWork argument the step loop passes (wallAt's answer, src/sim.js:407),
+// every markWall call on the actual stock (:443) and the final wall-error map; and, at chosen
+// wallAt calls, the call's whole input (the stock's heights, the tool position, Hb, the feed
+// direction, the load height, the material and the tool's static stiffness) with the reference
+// wallAt's own answer on it, so that wallAt is pinned call by call as well
+if (cases.simw) {
+  const { Simulator } = await import(REF + 'sim.js');
+  const { modeSet } = await import(REF + 'frf.js');
+  const { wallAt } = await import(REF + 'wall.js');
+  const WF = ['k', 'm', 'fn', 'zeta', 'nx', 'ny', 't', 'tRoot', 'hWall', 'a', 'bEff', 'base', 'cx', 'cy', 'I'];
+  out.simw = cases.simw.map(({ program, tool, stock, cell, nonlinear, every, maxSnaps }, ci) => {
+    const tc = Date.now();
+    const sim = new Simulator();
+    sim.cfg.cell = cell;
+    sim.cfg.workpieceFlex = true;
+    sim.cfg.cuttingNonlinear = nonlinear;
+    Object.assign(sim.cfg.tool, tool);
+    sim.load(program, { stock, origin: { x: 0, y: 0, z: 0 } });
+    const rec = { actual: [], ideal: [] };
+    const wrap = (st, o) => {
+      const f = st.stamp.bind(st);
+      st.stamp = (cx, cy, tipZ, R, type, errVal, mark, lut) => {
+        const v = f(cx, cy, tipZ, R, type, errVal, mark, lut);
+        o.push([cx, cy, tipZ, R, type === 'ball' ? 1 : 0, errVal, mark ? mark.f : 0, st.curLine, v]);
+        return v;
+      };
+    };
+    wrap(sim.stock, rec.actual); wrap(sim.ideal, rec.ideal);
+    const marks = [];
+    const fm = sim.stock.markWall.bind(sim.stock);
+    sim.stock.markWall = (cx, cy, R, nx, ny, val) => { marks.push([rec.actual.length, cx, cy, R, nx, ny, val]); fm(cx, cy, R, nx, ny, val); };
+    let px = 0, py = 0, pz = 0, pap = 0, found = 0, none = 0;
+    const fe = sim.fillEngagement.bind(sim);
+    sim.fillEngagement = (x, y, u) => { px = x; py = y; return fe(x, y, u); };
+    const e = sim.engine, cfgs = [], wears = [], runs = [], works = [], snaps = [];
+    const fc = e.configure.bind(e), fw = e.setWear.bind(e), fr = e.run.bind(e), fs = e.setEngagement.bind(e),
+      fk = e.setWork.bind(e);
+    let inCfg = false;
+    e.configure = (t, modal, mat, S, fnMax = 0) => {
+      inCfg = true;
+      fc(t, modal, mat, S, fnMax);
+      inCfg = false;
+      const ms = modeSet(modal), md = L => L.map(m => [m.k, m.fn, m.zeta]);
+      cfgs.push([rec.actual.length, t.N, t.R, t.helixDeg, t.type, t.fluteLen, t.nonlinear === false ? 0 : 1, t.runout || 0,
+        t.runoutPhase || 0, S, fnMax, md(ms.x), md(ms.y),
+        [mat.Ktc, mat.Krc, mat.Kte, mat.Kre, mat.Kac, mat.Kae, mat.mc != null ? mat.mc : 0, mat.Ci != null ? mat.Ci : 0]]);
+    };
+    e.setWear = edge => {
+      fw(edge);
+      const last = wears[wears.length - 1];
+      if (!last || last[1] !== edge.Kte || last[2] !== edge.Kre) wears.push([rec.actual.length, edge.Kte, edge.Kre]);
+    };
+    e.setEngagement = (u, fz, tipZ, ap) => { pz = tipZ; pap = ap; fs(u, fz, tipZ, ap); };
+    e.setWork = w => {
+      if (!inCfg) {
+        works.push(w ? [rec.actual.length, ...WF.map(k => w[k])] : [rec.actual.length]);
+        // the same call again on the same state, recorded with its input: every `every`-th call
+        // that finds a wall, and a few that find none
+        const pick = pap > 0.01 && (w ? found++ % every === 0 && snaps.length < maxSnaps : none++ % 97 === 5 && none < 400);
+        if (pick) {
+          const st = sim.stock, kTool = sim.toolStaticK(), zLoad = pz + pap / 2;
+          const again = wallAt(st, { x: px, y: py, z: pz }, sim.activeTool.R, e.Hb, sim.u, zLoad, sim.mat, kTool);
+          if (JSON.stringify(again) !== JSON.stringify(w)) throw new Error('wallAt replay differs');
+          snaps.push({ sub: rec.actual.length, x: px, y: py, z: pz, R: sim.activeTool.R, ux: sim.u[0], uy: sim.u[1], zLoad,
+            E: sim.mat.E, rho: sim.mat.rho, kTool, Hb: b64f32(e.Hb), H: z32(Float32Array, st.H),
+            wall: again ? WF.map(k => again[k]) : null });
+        }
+      }
+      fk(w);
+    };
+    e.run = n => { fr(n); runs.push([rec.actual.length, n]); };
+    let frames = 0;
+    while (!sim.done && frames < 100) { sim.step(1e6); frames++; }
+    console.error('simw case', ci, (Date.now() - tc) / 1e3, 's', works.filter(w => w.length > 1).length, 'walls');
+    const maps = s => ({ H: z32(Float32Array, s.H), err: z32(Float32Array, s.err), wallD: z32(Float32Array, s.wallD),
+      lineCut: z32(Int32Array, s.lineCut), lineWall: z32(Int32Array, s.lineWall), markS: z32(Float32Array, s.markS),
+      markF: z32(Float32Array, s.markF), wallErr: z32(Float32Array, s.wallErr), removed: s.removed });
+    const col = (rows, k) => rows.map(r => r[k]);
+    const pack = rows => ({ cx: b64f64(col(rows, 0)), cy: b64f64(col(rows, 1)), tipZ: b64f64(col(rows, 2)),
+      R: b64f64(col(rows, 3)), type: col(rows, 4), errVal: b64f64(col(rows, 5)), f: b64f64(col(rows, 6)),
+      line: col(rows, 7), vol: b64f64(col(rows, 8)) });
+    const P = sim.path, t = sim.activeTool;
+    const clr = t.clearanceDeg != null ? t.clearanceDeg : (sim.cfg.tool.clearanceDeg != null ? sim.cfg.tool.clearanceDeg : 10);
+    return { frames, nx: sim.stock.nx, ny: sim.stock.ny, hx: sim.stock.hx, hy: sim.stock.hy, zBottom: sim.stock.zBottom,
+      toolR: t.R, toolN: t.N, fluteLen: t.fluteLen, type: t.type, VB: sim.VB || 0, clearanceDeg: clr,
+      E: sim.mat.E, rho: sim.mat.rho, kTool: sim.toolStaticK(), wallFields: WF,
+      path: { x: b64f64(P.map(p => p.x)), y: b64f64(P.map(p => p.y)), z: b64f64(P.map(p => p.z)),
+        rapid: P.map(p => (p.rapid ? 1 : 0)), line: P.map(p => p.line), F: P.map(p => p.F), S: P.map(p => p.S) },
+      cfgs, wears, runs, snaps, actual: pack(rec.actual),
+      // setWork calls: the sub-step, 1 when the argument is a wall, and its fields (wallFields)
+      works: { sub: works.map(w => w[0]), wall: works.map(w => (w.length > 1 ? 1 : 0)),
+        fields: b64f64([].concat(...works.filter(w => w.length > 1).map(w => w.slice(1)))) },
+      // markWall calls: the sub-step and (cx, cy, R, nx, ny, val)
+      marks: { sub: marks.map(m => m[0]), args: b64f64([].concat(...marks.map(m => m.slice(1)))) }, ideal: pack(rec.ideal), stock: maps(sim.stock),
+      commanded: maps(sim.ideal) };
+  });
+  console.error('simw', (Date.now() - t0) / 1e3, 's');
+}
+
 // whole simulator runs with real-time frames (a rigid workpiece, the linear force law): the
 // sub-steps each frame of sim.step(dtReal) takes at playback speed `speed` under the dynamics
 // budget, every stamp, every engine run, and per frame (collect) the engine's taken stats,

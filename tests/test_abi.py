@@ -48,6 +48,8 @@ int main(void) {
          offsetof(mc_sim_program, frames));
   printf("%zu %zu %zu %zu %zu\n", sizeof(mc_move), offsetof(mc_move, axes), offsetof(mc_move, sweep),
          sizeof(mc_path_point), sizeof(mc_schedule_opts));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(mc_wall_out), offsetof(mc_wall_out, markVal), offsetof(mc_wall_out, found),
+         offsetof(mc_sim_program, flex), offsetof(mc_sim_program, kTool), offsetof(mc_sim_program, walls));
   return 0;
 }'''
     import tempfile
@@ -70,4 +72,6 @@ int main(void) {
     want += [C.sizeof(_abi.FrameOut), _abi.FrameOut.end.offset, _abi.Substep.guard.offset, _abi.SimProgram.frames.offset]
     want += [C.sizeof(_abi.Move), _abi.Move.axes.offset, _abi.Move.sweep.offset, C.sizeof(_abi.PathPoint),
              C.sizeof(_abi.ScheduleOpts)]
+    want += [C.sizeof(_abi.WallOut), _abi.WallOut.markVal.offset, _abi.WallOut.found.offset,
+             _abi.SimProgram.flex.offset, _abi.SimProgram.kTool.offset, _abi.SimProgram.walls.offset]
     assert [int(x) for x in out] == want
