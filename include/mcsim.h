@@ -43,7 +43,7 @@ typedef enum {
   MC_ERR_NOMEM = 5    /* device allocation failed */
 } mc_status;
 
-enum { MC_TOOL_FLAT = 0, MC_TOOL_BALL = 1, MC_TOOL_BULL = 2 };   /* BULL: mc_sim_run only */
+enum { MC_TOOL_FLAT = 0, MC_TOOL_BALL = 1, MC_TOOL_BULL = 2, MC_TOOL_PROFILE = 3 };   /* BULL, PROFILE: mc_sim_run only */
 
 /* ---- Path A data ---- */
 
@@ -425,8 +425,9 @@ int mc_tip_lut(int32_t shape, double D, double cornerR, const double* edge_r, in
  * mc_sim_run puts many programs in one launch.  The workpiece is rigid (workpieceFlex off,
  * so setWork(null); the thin-wall form, flex, is switched off in this build), the flank wear
  * fixed for the call (the engine's setWear and rw), and
- * the tool flat, ball or bull-nose (its profile cutterRadiusAt in the engine's slices, its
- * tip table in the stamps and the end face, as the simulator sets them up, src/sim.js:190-218). */
+ * the tool flat, ball, bull-nose or an imported edge profile (its cutterRadiusAt in the
+ * engine's slices, its tip table in the stamps and the end face, as the simulator sets them
+ * up, src/sim.js:190-218). */
 
 /* one sub-step, as the simulator's schedule produces it (src/sim.js:353-381,389-405) */
 typedef struct {
@@ -553,8 +554,9 @@ typedef struct {
   double share;            /* with dynBudget <= 0: the steps a sub-step may run besides its own
                               time, a fixed share of the frame budget (src/sim.js:410-412) */
   int32_t nsub;
-  int32_t type;            /* MC_TOOL_FLAT | MC_TOOL_BALL | MC_TOOL_BULL */
-  const float* lut;        /* MC_TOOL_BULL: the tool's tipLut (mc_tip_lut), lutn entries */
+  int32_t type;            /* MC_TOOL_FLAT | MC_TOOL_BALL | MC_TOOL_BULL | MC_TOOL_PROFILE (an
+                              imported edge, edge_r below) */
+  const float* lut;        /* MC_TOOL_BULL, MC_TOOL_PROFILE: the tool's tipLut (mc_tip_lut), lutn entries */
   int32_t lutn;
   int32_t nframes;         /* capacity of frames */
   double cornerR;          /* MC_TOOL_BULL: corner radius [mm] (cutterRadiusAt, src/assembly.js:23-30) */
@@ -580,6 +582,13 @@ typedef struct {
   double wallRho;          /* flex: its density [kg/m^3] (mat.rho) */
   double kTool;            /* flex: the tool's static stiffness [N/mm] (toolStaticK, src/sim.js:250) */
   mc_wall_out* walls;      /* flex: NULL, or nsub records of the wall model per sub-step */
+  const double* edge_r;    /* MC_TOOL_PROFILE: the imported cutting-edge envelope, edge_n >= 2 radii
+                              [mm] every edge_dz from the tip; the engine's slices take
+                              cutterRadiusAt = sampleEdge (src/assembly.js:22-35, src/cutting.js:
+                              115-119), the stamps and the end face the tip table lut */
+  double edge_dz;
+  int32_t edge_n;
+  int32_t pad3;
 } mc_sim_program;
 
 /* n programs, one workgroup each, one launch; the stocks and engines must be distinct */

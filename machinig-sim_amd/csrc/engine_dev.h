@@ -693,6 +693,23 @@ struct ToolShape {
   int lutn;
 };
 
+// an imported cutting-edge envelope: n >= 2 radii dz apart from the tip (null r: none)
+struct EdgeProfile {
+  const double* r;
+  int n;
+  double dz;
+};
+
+// sampleEdge (src/assembly.js:31-35): the envelope's radius at height z, linear between
+// the samples, the end samples held beyond the table
+__host__ __device__ __forceinline__ double edge_rat(const EdgeProfile& e, double z) {
+  const double f = z / e.dz;
+  const double i = jmin(e.n - 2, jmax(0, __builtin_floor(f)));
+  const double u = jmin(1, jmax(0, f - i));
+  const int k = (int)i;
+  return e.r[k] * (1 - u) + e.r[k + 1] * u;
+}
+
 // cutterRadiusAt (src/assembly.js:23-30) of a bull-nose tool
 __host__ __device__ __forceinline__ double bull_rat(double R, double rc, double z) {
   if (rc > 0 && z < rc) {
@@ -703,10 +720,11 @@ __host__ __device__ __forceinline__ double bull_rat(double R, double rc, double 
 }
 
 // setEngagement(u, fz, tipZ, ap), src/cutting.js:98-124: the flat / ball slices, the caller's
-// profile radii sR / sK (tool.rAt), or a bull-nose profile (rc > 0) evaluated here
+// profile radii sR / sK (tool.rAt), or an imported edge (edge) or a bull-nose profile (rc > 0)
+// evaluated here
 __host__ __device__ __forceinline__ int eng_set_engagement(EngDev& h, double ux, double uy, double fz, double tipZ,
                                                           double ap, const double* sR, const double* sK, bool ball,
-                                                          double rc = 0) {
+                                                          double rc = 0, const EdgeProfile* edge = nullptr) {
   h.ux = ux; h.uy = uy; h.ex = -uy; h.ey = ux;
   h.fz = fz; h.tipZ = tipZ;
   const int nz = ap <= 0 ? 0 : (int)jmin(h.maxSlices, __builtin_ceil(ap / h.dz));
@@ -718,7 +736,11 @@ __host__ __device__ __forceinline__ int eng_set_engagement(EngDev& h, double ux,
     h.sZ[k] = tipZ + zr;
     double r = h.R, kap = 1;
     if (sR) { r = sR[k]; kap = sK[k]; }
-    else if (rc > 0) {
+    else if (edge && edge->r) {
+      r = jmax(1e-3, edge_rat(*edge, zr));
+      const double dr = (edge_rat(*edge, zr + 0.05) - edge_rat(*edge, jmax(0, zr - 0.05))) / (zr < 0.05 ? 0.05 + zr : 0.1);
+      kap = 1 / __builtin_sqrt(1 + dr * dr);
+    } else if (rc > 0) {
       r = jmax(1e-3, bull_rat(h.R, rc, zr));
       const double dr = (bull_rat(h.R, rc, zr + 0.05) - bull_rat(h.R, rc, jmax(0, zr - 0.05))) / (zr < 0.05 ? 0.05 + zr : 0.1);
       kap = 1 / __builtin_sqrt(1 + dr * dr);

@@ -989,8 +989,9 @@ bool prop_val(napi_env env, napi_value o, const char* k, napi_value* v) {
 }
 
 // simRun(programs: [{actual, ideal (stock handles), engine (engine handle), sub (ArrayBuffer of
-//   mc_substep), R, fluteLen, rw, share, type, lut (Float32Array, bull-nose), cornerR,
-//   dynBudget, fz, frames (true: the frames' stats and chatter index)}])
+//   mc_substep), R, fluteLen, rw, share, type, lut (Float32Array, bull-nose or profile), cornerR,
+//   edgeR (Float64Array), edgeDz (an imported edge), dynBudget, fz, frames (true: the frames'
+//   stats and chatter index)}])
 //   -> [[ArrayBuffer of mc_substep_out, ArrayBuffer of mc_frame_out], ...]
 // the programs' sub-step loops in one device call (mc_sim_run)
 napi_value SimRun(napi_env env, napi_callback_info info) {
@@ -1037,6 +1038,14 @@ napi_value SimRun(napi_env env, napi_callback_info info) {
       if (!buffer_of(env, v, &lut, &lb)) return nullptr;
       g.lut = (const float*)lut;
       g.lutn = (int32_t)(lb / 4);
+    }
+    if (prop_val(env, o, "edgeR", &v)) {   // an imported edge's radii (Float64Array)
+      void* er;
+      size_t eb;
+      if (!buffer_of(env, v, &er, &eb)) return nullptr;
+      g.edge_r = (const double*)er;
+      g.edge_n = (int32_t)(eb / 8);
+      g.edge_dz = prop_num(env, o, "edgeDz", 0);
     }
     bool want = false;
     if (prop_val(env, o, "frames", &v)) napi_get_value_bool(env, v, &want);

@@ -52,6 +52,7 @@ struct ProgDev {
   mc_wall_out* walls;
   double wstep, wallE, wallRho, kTool;
   int32_t flex, n20, n60, pad2;
+  EdgeProfile edge;   // an imported edge profile (edge.r null otherwise)
 };
 
 // Stock.sample (src/stock.js:85-92)
@@ -699,7 +700,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
       // setEngagement, setEndEngagement (src/sim.js:403-404) on thread 0; the wall at the load
       // height with the thin-wall model, else none (:407), and setWork (:408)
       if (tid == 0) {
-        eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball, P.rc);
+        eng_set_engagement(E, s.ux, s.uy, s.fzEff, s.z, ap, nullptr, nullptr, ball, P.rc, &P.edge);
         eng_set_end_engagement(E, endCut ? s.fza : 0, haveEnd ? rOut : 0, nullptr, shape);
       }
       bool found = false;
@@ -804,11 +805,18 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (!g.actual || !g.ideal || !g.engine || g.actual == g.ideal)
       return mch::fail(MC_ERR_ARG, "program %d: needs an engine and two distinct stocks", p);
     if (g.nsub < 0 || (g.nsub > 0 && !g.sub)) return mch::fail(MC_ERR_ARG, "program %d: null sub-steps", p);
-    if (g.type != MC_TOOL_FLAT && g.type != MC_TOOL_BALL && g.type != MC_TOOL_BULL)
-      return mch::fail(MC_ERR_ARG, "program %d: flat, ball or bull-nose tools", p);
-    if (g.type == MC_TOOL_BULL) {
+    if (g.type != MC_TOOL_FLAT && g.type != MC_TOOL_BALL && g.type != MC_TOOL_BULL && g.type != MC_TOOL_PROFILE)
+      return mch::fail(MC_ERR_ARG, "program %d: flat, ball, bull-nose or profile tools", p);
+    const bool tabled = g.type == MC_TOOL_BULL || g.type == MC_TOOL_PROFILE;
+    if (g.type == MC_TOOL_PROFILE) {
+      if (!g.edge_r || g.edge_n < 2 || g.edge_n > (1 << 20) || !(g.edge_dz > 0) || !std::isfinite(g.edge_dz))
+        return mch::fail(MC_ERR_ARG, "program %d: a profile tool needs at least two edge radii and a positive step", p);
+      for (int i = 0; i < g.edge_n; i++)
+        if (!std::isfinite(g.edge_r[i])) return mch::fail(MC_ERR_ARG, "program %d: non-finite edge radius", p);
+    }
+    if (tabled) {
       if (!g.lut || g.lutn < 1 || g.lutn > (1 << 20) || !(g.cornerR >= 0))
-        return mch::fail(MC_ERR_ARG, "program %d: a bull-nose tool needs its tip table and corner radius", p);
+        return mch::fail(MC_ERR_ARG, "program %d: a bull-nose or profile tool needs its tip table (and corner radius)", p);
       for (int i = 0; i < g.lutn; i++)
         if (!std::isfinite(g.lut[i])) return mch::fail(MC_ERR_ARG, "program %d: non-finite tip table", p);
     }
@@ -856,8 +864,9 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     d.sub = nullptr; d.out = nullptr;
     d.R = g.R; d.fluteLen = g.fluteLen; d.rw = g.rw; d.share = g.share;
     d.nsub = g.nsub; d.ball = g.type == MC_TOOL_BALL; d.N = m->N;
-    d.lut = nullptr; d.lutn = g.type == MC_TOOL_BULL ? g.lutn : 0;
+    d.lut = nullptr; d.lutn = tabled ? g.lutn : 0;
     d.rc = g.type == MC_TOOL_BULL ? std::min(g.R, g.cornerR) : 0.0;
+    d.edge = EdgeProfile{nullptr, g.type == MC_TOOL_PROFILE ? g.edge_n : 0, g.edge_dz};
     d.dynBudget = g.dynBudget > 0 ? g.dynBudget : 0; d.fz0 = g.fz;
     d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0; d.pad = 0;
     d.sacc = nullptr; d.walls = nullptr; d.flex = g.flex; d.n20 = 0; d.n60 = 0; d.pad2 = 0;
@@ -878,7 +887,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     const mc_sim_program& g = progs[p];
     if (g.nsub == 0) continue;
     std::vector<mc_stamp> cmd((size_t)g.nsub);
-    std::vector<mc_stamp_opt> opt(g.type == MC_TOOL_BULL ? (size_t)g.nsub : 0);
+    std::vector<mc_stamp_opt> opt(g.type == MC_TOOL_BULL || g.type == MC_TOOL_PROFILE ? (size_t)g.nsub : 0);
     for (int k = 0; k < g.nsub; k++) {
       const mc_substep& s = g.sub[k];
       cmd[(size_t)k] = mc_stamp{s.x, s.y, s.z, g.R, g.type == MC_TOOL_BALL ? MC_TOOL_BALL : MC_TOOL_FLAT, s.line, 0.0};
@@ -901,11 +910,12 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (hipGetLastError() != hipSuccess) return mch::fail(MC_ERR_HIP, "strip maxima launch failed");
   }
   // sub-steps in, results out: one device buffer for every program
-  size_t nin = 0, nlut = 0, nfr = 0, nwall = 0;
+  size_t nin = 0, nlut = 0, nfr = 0, nwall = 0, nedge = 0;
   bool flex = false;
   for (int p = 0; p < n; p++) {
     nin += (size_t)progs[p].nsub;
-    if (progs[p].type == MC_TOOL_BULL) nlut += (size_t)progs[p].lutn;
+    if (progs[p].type == MC_TOOL_BULL || progs[p].type == MC_TOOL_PROFILE) nlut += (size_t)progs[p].lutn;
+    if (progs[p].type == MC_TOOL_PROFILE) nedge += (size_t)progs[p].edge_n;
     nfr += (size_t)pd[(size_t)p].nframes;
     if (progs[p].flex && progs[p].walls) nwall += (size_t)progs[p].nsub;
     flex = flex || progs[p].flex;
@@ -942,6 +952,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   ProgDev* dprog = nullptr;
   double* dsacc = nullptr;
   mc_wall_out* dwall = nullptr;
+  double* dedge = nullptr;
   auto release = [&]() {
     if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
@@ -950,6 +961,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (dprog) (void)hipFree(dprog);
     if (dsacc) (void)hipFree(dsacc);
     if (dwall) (void)hipFree(dwall);
+    if (dedge) (void)hipFree(dedge);
   };
   if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
@@ -957,11 +969,12 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
       hipMalloc(&dprog, sizeof(ProgDev) * (size_t)n) != hipSuccess ||
       hipMalloc(&dlut, sizeof(float) * std::max<size_t>(1, nlut)) != hipSuccess ||
       hipMalloc(&dsacc, sizeof(double) * std::max<size_t>(1, sacc.size())) != hipSuccess ||
-      hipMalloc(&dwall, sizeof(mc_wall_out) * std::max<size_t>(1, nwall)) != hipSuccess) {
+      hipMalloc(&dwall, sizeof(mc_wall_out) * std::max<size_t>(1, nwall)) != hipSuccess ||
+      hipMalloc(&dedge, sizeof(double) * std::max<size_t>(1, nedge)) != hipSuccess) {
     release();
     return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
   }
-  size_t off = 0, loff = 0, foff = 0, woff = 0;
+  size_t off = 0, loff = 0, foff = 0, woff = 0, eoff = 0;
   int st = MC_OK;
   if (!sacc.empty() &&
       hipMemcpyAsync(dsacc, sacc.data(), sizeof(double) * sacc.size(), hipMemcpyHostToDevice, q) != hipSuccess)
@@ -975,7 +988,13 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
         woff += (size_t)g.nsub;
       }
     }
-    if (g.type == MC_TOOL_BULL) {
+    if (g.type == MC_TOOL_PROFILE) {
+      pd[(size_t)p].edge.r = dedge + eoff;
+      if (hipMemcpyAsync(dedge + eoff, g.edge_r, sizeof(double) * (size_t)g.edge_n, hipMemcpyHostToDevice, q) != hipSuccess)
+        st = mch::fail(MC_ERR_HIP, "edge profile upload");
+      eoff += (size_t)g.edge_n;
+    }
+    if (g.type == MC_TOOL_BULL || g.type == MC_TOOL_PROFILE) {
       pd[(size_t)p].lut = dlut + loff;
       if (hipMemcpyAsync(dlut + loff, g.lut, sizeof(float) * (size_t)g.lutn, hipMemcpyHostToDevice, q) != hipSuccess)
         st = mch::fail(MC_ERR_HIP, "tip table upload");

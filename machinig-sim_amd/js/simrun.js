@@ -8,7 +8,7 @@
 import { native } from './native.js';
 import { MC_FRAME_LAST } from './path.js';
 
-const OUT_BYTES = 80, FRAME_BYTES = 216, TYPE = { flat: 0, ball: 1, bull: 2 };
+const OUT_BYTES = 80, FRAME_BYTES = 216, TYPE = { flat: 0, ball: 1, bull: 2, profile: 3 };
 const STATS = ['n', 'sumF', 'maxF', 'sumFx', 'sumFy', 'sumFz', 'maxFz', 'sumT', 'maxT', 'sumX', 'sumY', 'minX',
   'maxX', 'minY', 'maxY', 'sumZc', 'wZc', 'sumW', 'maxW', 'sumHb', 'sumB', 'sumRb', 'engTeeth'];
 const OUT_F = ['dx', 'dy', 'ap', 'errVal', 'vol', 'maxF', 'phiSt', 'phiEx'];
@@ -16,7 +16,8 @@ const OUT_F = ['dx', 'dy', 'ap', 'errVal', 'vol', 'maxF', 'phiSt', 'phiEx'];
 // programs: [{stock (Simulator.stock), ideal (Simulator.ideal), engine (a CuttingEngine
 // configured for the program's tool and speed, its setWear applied; the loop starts from the
 // state its device calls left), schedule ({sub, n} from simSchedule), tool {R, fluteLen, type: 'flat' | 'ball' |
-// 'bull', cornerR, lut}, rw, share, dynBudget, fz, frames (true: frame results)}].
+// 'bull' | 'profile', cornerR, lut, edge (an imported edge {dz, r}, src/assembly.js:31-35)}, rw,
+// share, dynBudget, fz, frames (true: frame results)}].
 // Returns per program {out: [{dx, dy, ap, errVal, vol, maxF, phiSt, phiEx, nsteps, cut, arc}],
 // frames: [{stats, index, amp, once, end}]}.  The actual stock's removed volume and curLine
 // advance as its stamps do; the commanded stock's removed volume is not advanced (the
@@ -26,11 +27,13 @@ export function simRun(programs) {
     const t = p.tool, e = p.engine, mat = e.mat;
     // the engine's setWear reaches the device with its next run: here, this call
     native().engineSetWear(e._h, e.Kte != null ? e.Kte : mat.Kte, e.Kre != null ? e.Kre : mat.Kre);
-    if (!(t.type in TYPE)) throw new RangeError('tool type must be flat, ball or bull');
+    if (!(t.type in TYPE)) throw new RangeError('tool type must be flat, ball, bull or profile');
+    if (t.type === 'profile' && !t.edge) throw new RangeError('a profile tool needs its edge {dz, r}');
     return { actual: p.stock.handle, ideal: p.ideal.handle, engine: p.engine._h, sub: p.schedule.sub,
       nsub: p.schedule.n, R: t.R, fluteLen: t.fluteLen, rw: p.rw || 0, share: p.share === undefined ? 1 : p.share,
-      type: TYPE[t.type], lut: t.type === 'bull' ? Float32Array.from(t.lut) : null, cornerR: t.cornerR || 0,
-      dynBudget: p.dynBudget || 0, fz: p.fz || 0, frames: !!p.frames };
+      type: TYPE[t.type], lut: t.type === 'bull' || t.type === 'profile' ? Float32Array.from(t.lut) : null,
+      cornerR: t.cornerR || 0, edgeR: t.type === 'profile' ? Float64Array.from(t.edge.r) : null,
+      edgeDz: t.type === 'profile' ? t.edge.dz : 0, dynBudget: p.dynBudget || 0, fz: p.fz || 0, frames: !!p.frames };
   });
   const res = native().simRun(args);
   return programs.map((p, i) => {

@@ -11,7 +11,7 @@ MC_MAX_SLICES = 64
 MC_CHATTER_IDX = 0.45
 
 MC_OK, MC_ERR_ARG, MC_ERR_LIMIT, MC_ERR_NODEV, MC_ERR_HIP, MC_ERR_NOMEM = range(6)
-MC_TOOL_FLAT, MC_TOOL_BALL, MC_TOOL_BULL = 0, 1, 2
+MC_TOOL_FLAT, MC_TOOL_BALL, MC_TOOL_BULL, MC_TOOL_PROFILE = 0, 1, 2, 3
 
 
 class Tool(C.Structure):
@@ -170,7 +170,9 @@ class SimProgram(C.Structure):
                 ("nsub", C.c_int32), ("type", C.c_int32), ("lut", C.POINTER(C.c_float)), ("lutn", C.c_int32),
                 ("nframes", C.c_int32), ("cornerR", C.c_double), ("dynBudget", C.c_double), ("fz", C.c_double),
                 ("frames", C.c_void_p), ("flex", C.c_int32), ("pad", C.c_int32), ("wallE", C.c_double),
-                ("wallRho", C.c_double), ("kTool", C.c_double), ("walls", C.POINTER(WallOut))]
+                ("wallRho", C.c_double), ("kTool", C.c_double), ("walls", C.POINTER(WallOut)),
+                ("edge_r", C.POINTER(C.c_double)), ("edge_dz", C.c_double), ("edge_n", C.c_int32),
+                ("pad3", C.c_int32)]
 
 
 MC_STOCK_B = 8

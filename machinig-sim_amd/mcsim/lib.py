@@ -328,7 +328,8 @@ def sim_run(programs, frames=False, walls=False):
     programs: dicts with actual, ideal (Stock), engine (Engine), sub (Substep array) and
     optionally nsub (its length), R, fluteLen, rw, share, type (MC_TOOL_*), for a bull-nose
     tool lut (its tipLut) and cornerR, dynBudget (> 0: the frame budget's share per
-    sub-step instead of the fixed share), fz (this.fz at the start) and wall (dict(E, rho,
+    sub-step instead of the fixed share), fz (this.fz at the start), edge (dict(r, dz): the
+    imported edge of an MC_TOOL_PROFILE tool, whose lut is its mc_tip_lut table) and wall (dict(E, rho,
     kTool): the thin-wall model, workpieceFlex; the engine configured with fnMax
     MC_WALL_FN_CAP); returns a SubstepOut list per program, with frames=True also a FrameOut
     list per program (one per sub-step flagged MC_FRAME_LAST), and with walls=True also a
@@ -352,6 +353,9 @@ def sim_run(programs, frames=False, walls=False):
         w = g.get("wall")
         wo = (WallOut * max(1, ns))() if w else None
         wls.append((wo, ns))
+        e = g.get("edge")
+        er = np.ascontiguousarray(e["r"], dtype=np.float64) if e is not None else None
+        keep.append(er)
         arr[i] = SimProgram(g["actual"].p.value, g["ideal"].p.value, g["engine"].p.value,
                             C.cast(sub, C.POINTER(Substep)), C.cast(out, C.POINTER(SubstepOut)),
                             float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g.get("share", 1.0)),
@@ -361,7 +365,9 @@ def sim_run(programs, frames=False, walls=False):
                             float(g.get("dynBudget", 0.0)), float(g.get("fz", 0.0)),
                             C.cast(fr, C.c_void_p) if nf else None, 1 if w else 0, 0,
                             float(w["E"]) if w else 0.0, float(w["rho"]) if w else 0.0,
-                            float(w["kTool"]) if w else 0.0, C.cast(wo, C.POINTER(WallOut)) if w else None)
+                            float(w["kTool"]) if w else 0.0, C.cast(wo, C.POINTER(WallOut)) if w else None,
+                            er.ctypes.data_as(C.POINTER(C.c_double)) if er is not None else None,
+                            float(e["dz"]) if e is not None else 0.0, len(er) if er is not None else 0, 0)
     _check(load().mc_sim_run(arr, n))
     res = [[o[:ns] for o, ns in outs]]
     if frames:

@@ -28,7 +28,7 @@
 
 #define PI 3.141592653589793
 
-typedef struct { int ball; double R; const float *lut; int lutn; double rc; } Shape;
+typedef struct { int ball; double R; const float *lut; int lutn; double rc; const double *er; int en; double edz; } Shape;
 
 static double zat(const Shape *t, double r) {
   if (t->lut) {
@@ -47,6 +47,21 @@ static double bull_rat(double R, double rc, double z) {
   }
   return R;
 }
+
+/* cutterRadiusAt of an imported edge: sampleEdge (src/assembly.js:31-35), the radii en
+   samples edz apart, linear between them, the end samples held beyond the table */
+static double edge_rat(const Shape *t, double z) {
+  const double f = z / t->edz;
+  double i = floor(f), u;
+  if (i < 0) i = 0;
+  if (i > t->en - 2) i = t->en - 2;
+  u = f - i;
+  if (u < 0) u = 0;
+  if (u > 1) u = 1;
+  return t->er[(int)i] * (1 - u) + t->er[(int)i + 1] * u;
+}
+
+static double profile_rat(const Shape *t, double z) { return t->er ? edge_rat(t, z) : bull_rat(t->R, t->rc, z); }
 
 /* Simulator.fillEndEngagement (src/sim.js:324-341): Eb[b * 6 + j]; returns dmax, *rOut */
 static double fill_end(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
@@ -75,7 +90,7 @@ static double fill_end(const or_stock *s, double cx, double cy, double tipZ, dou
 
 double or_fill_end_engagement(const or_stock *s, double cx, double cy, double tipZ, double ux, double uy,
                               double R, int ball, float *Eb, double *rOut_out) {
-  const Shape t = {ball, R, NULL, 0, 0};
+  const Shape t = {ball, R, NULL, 0, 0, NULL, 0, 0};
   return fill_end(s, cx, cy, tipZ, ux, uy, &t, Eb, rOut_out);
 }
 
@@ -98,9 +113,12 @@ static void end_rings(double rOut, const Shape *t, double *g) {
 int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub, int n, double R,
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
                mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes, int flex,
-               double E, double rho, double kTool, mc_wall_out *walls) {
-  const int ball = type == MC_TOOL_BALL, bull = type == MC_TOOL_BULL;
-  const Shape shape = {ball, R, bull ? lut : NULL, bull ? lutn : 0, bull ? (cornerR < R ? cornerR : R) : 0};
+               double E, double rho, double kTool, mc_wall_out *walls, const double *edge_r, int edge_n,
+               double edge_dz) {
+  const int ball = type == MC_TOOL_BALL, prof = type == MC_TOOL_PROFILE, bull = type == MC_TOOL_BULL || prof;
+  const Shape shape = {ball, R, bull ? lut : NULL, bull ? lutn : 0,
+                       type == MC_TOOL_BULL ? (cornerR < R ? cornerR : R) : 0, prof ? edge_r : NULL,
+                       prof ? edge_n : 0, edge_dz};
   const float *slut = bull ? lut : NULL;
   const int stype = ball ? MC_TOOL_BALL : MC_TOOL_FLAT;
   float Hb[MC_BINS], Eb[MC_END_BINS * MC_END_RINGS];
@@ -131,16 +149,17 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
     if (cut) {
       double g[4 * MC_END_RINGS], want;
       if (bull) {
-        /* setEngagement's rAt slices (src/cutting.js:112-119) */
+        /* setEngagement's rAt slices (src/cutting.js:112-119): a bull-nose corner or an
+           imported edge */
         double sr[MC_MAX_SLICES], sk[MC_MAX_SLICES];
         const double dz = or_engine_dz(e);
         const int ms = or_engine_max_slices(e);
         int k, nz = ap <= 0 ? 0 : (int)fmin(ms, ceil(ap / dz));
         for (k = 0; k < nz; k++) {
           const double b = dz < ap - k * dz ? dz : ap - k * dz, zr = k * dz + 0.5 * b;
-          const double r0 = bull_rat(R, shape.rc, zr);
+          const double r0 = profile_rat(&shape, zr);
           const double lo = zr - 0.05 > 0 ? zr - 0.05 : 0;
-          const double dr = (bull_rat(R, shape.rc, zr + 0.05) - bull_rat(R, shape.rc, lo)) / (zr < 0.05 ? 0.05 + zr : 0.1);
+          const double dr = (profile_rat(&shape, zr + 0.05) - profile_rat(&shape, lo)) / (zr < 0.05 ? 0.05 + zr : 0.1);
           sr[k] = r0 > 1e-3 ? r0 : 1e-3;
           sk[k] = 1 / sqrt(1 + dr * dr);
         }

@@ -228,8 +228,12 @@ def build_cases(quick):
     cases["paths"] = [dict(program=c["program"], maxSegs=[0.5, 0.2]) for c in cases["sim"]] + \
         [dict(program="\n".join(arcs), maxSegs=[0.5, 0.2, 2.0])]
     # rigid workpiece, linear force law, flat, ball and bull-nose tools: the device sub-step pipeline
+    # an imported edge profile (type "profile", src/assembly.js:31-35): a rounded 6 mm tip
+    prof = dict(type="profile", D=6.0, edge=dict(dz=0.2, r=[0.6, 1.3, 1.85, 2.25, 2.55, 2.75, 2.88, 2.96, 3.0, 3.0,
+                                                          3.0, 3.0, 3.0, 3.0, 3.0, 3.0]))
     cases["simx"] = [dict(cases["sim"][0]), dict(cases["sim"][1]), dict(ribc),
-                     dict(cases["sim"][0], tool=dict(type="ball", D=8.0)), dict(cases["sim"][2])]
+                     dict(cases["sim"][0], tool=dict(type="ball", D=8.0)), dict(cases["sim"][2]),
+                     dict(cases["sim"][1], tool=prof)]
     # the thin-wall model on (workpieceFlex, the simulator's default): the rib program above with
     # the linear force law and with the size effect, and a taller 1.5 mm wall between two slots,
     # finished on its face, at 0.1 mm cells; `every` / `maxSnaps` choose the wallAt calls whose

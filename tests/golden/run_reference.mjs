@@ -229,6 +229,8 @@ if (cases.simx) {
     sim.cfg.cell = cell;
     sim.cfg.workpieceFlex = false;
     sim.cfg.cuttingNonlinear = false;
+    // an imported edge profile keeps its radii as float32, as an imported mesh gives them
+    if (tool.edge) tool = { ...tool, edge: { dz: tool.edge.dz, r: Float32Array.from(tool.edge.r) } };
     Object.assign(sim.cfg.tool, tool);
     sim.load(program, { stock, origin: { x: 0, y: 0, z: 0 } });
     const rec = { actual: [], ideal: [] };

@@ -337,14 +337,26 @@ def simx_cell(res):
                      mxs, mys, dict(zip(MAT_ORDER, mat)), S, (0.0, 0.0), 0.0, 1, nl), fnMax
 
 
+def simx_edge(case):
+    """the imported edge (dict(r, dz), radii as the float32 the reference holds) of a simx
+    run's profile tool, else None"""
+    e = case["tool"].get("edge")
+    if e is None:
+        return None
+    return dict(r=np.asarray(e["r"], dtype=np.float32).astype(np.float64), dz=float(e["dz"]))
+
+
 def simx_tool(case, res, tip_lut):
     """(type, lut, cornerR) of a simx run's tool for mc_sim_run / the oracle's loop; tip_lut is
     the library's or the oracle's tipLut"""
-    from mcsim._abi import MC_TOOL_BALL, MC_TOOL_BULL, MC_TOOL_FLAT
+    from mcsim._abi import MC_TOOL_BALL, MC_TOOL_BULL, MC_TOOL_FLAT, MC_TOOL_PROFILE
     t = res["type"]
     if t == "bull":
         rc = case["tool"]["cornerR"]
         return MC_TOOL_BULL, tip_lut(MC_TIP_BULL, 2 * res["toolR"], rc), rc
+    if t == "profile":
+        e = simx_edge(case)
+        return MC_TOOL_PROFILE, tip_lut(MC_TIP_EDGE, 2 * res["toolR"], 0.0, e["r"], e["dz"]), 0.0
     return (MC_TOOL_BALL if t == "ball" else MC_TOOL_FLAT), None, 0.0
 
 

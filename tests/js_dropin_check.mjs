@@ -290,7 +290,7 @@ async function main() {
       check(sg.predict([byOpts[0]], { revs: c0.revs, nonlinear: c0.nonlinear }).every(Number.isFinite), 'surrogate predict');
     }
     // the simulator's sub-step loop through simRun: the recorded rigid-workpiece runs (flat,
-    // ball, bull-nose) in one call, their deflected stamps, vibration values, step counts and
+    // ball, bull-nose, an imported edge) in one call, their deflected stamps, vibration values, step counts and
     // both stocks' maps as the reference's
     const sx = load('simx'), progs = [];
     sx.cases.forEach((c, k) => {
@@ -305,6 +305,10 @@ async function main() {
       e.setWear({ Kte: res.wears[0][1], Kre: res.wears[0][2] });
       const tool = { R: res.toolR, fluteLen: res.fluteLen, type: res.type };
       if (res.type === 'bull') Object.assign(tool, { cornerR: c.tool.cornerR, lut: m.tipLut({ type: 'bull', D: 2 * res.toolR, cornerR: c.tool.cornerR }) });
+      if (res.type === 'profile') {
+        const edge = { dz: c.tool.edge.dz, r: Float32Array.from(c.tool.edge.r) };
+        Object.assign(tool, { edge, lut: m.tipLut({ type: 'profile', D: 2 * res.toolR, edge }) });
+      }
       progs.push({ stock: act, ideal: ide, engine: e, schedule: schedule(m, res), tool, share: e.stepsPerTooth / 4 });
     });
     const ran = m.simRun(progs);

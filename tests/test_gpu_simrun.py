@@ -1,9 +1,10 @@
 """The simulator's sub-step loop on the device (mc_sim_run: engagement, engine run and both
 stamps per sub-step, src/sim.js:351-449) against whole runs of the reference Simulator with a
-rigid workpiece and the linear force law (golden_simx.json): the deflected stamp positions,
+rigid workpiece and the linear force law (golden_simx.json; flat, ball, bull-nose and
+imported-edge tools): the deflected stamp positions,
 their vibration values, the engine's step counts, and every map of the actual and the
 commanded stock bit for bit; the removed volume within 1e-12 (its sum order differs).  The
-five programs (flat, ball and bull-nose tools) run in one launch, one workgroup each."""
+six programs run in one launch, one workgroup each."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,7 @@ def test_sim_pipeline_matches_reference_runs():
         sub, n = G.simx_program(res)
         typ, lut, rc = G.simx_tool(case, res, lib.tip_lut)
         progs.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=n, R=res["toolR"], fluteLen=res["fluteLen"],
-                          rw=0.0, share=info.stepsPerTooth / 4, type=typ, lut=lut, cornerR=rc))
+                          rw=0.0, share=info.stepsPerTooth / 4, type=typ, lut=lut, cornerR=rc, edge=G.simx_edge(case)))
         keep.append((case, res, n))
     print("sim pipeline: %d programs, %s sub-steps" % (len(progs), [k[2] for k in keep]), flush=True)
     outs = lib.sim_run(progs)
@@ -86,7 +87,7 @@ def test_sim_pipeline_matches_oracle_per_substep():
         eng.set_wear(kte, kre)
         typ, lut, rc = G.simx_tool(case, res, lib.tip_lut)
         progs.append(dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=n, R=res["toolR"], fluteLen=res["fluteLen"],
-                          rw=0.0, share=info.stepsPerTooth / 4, type=typ, lut=lut, cornerR=rc))
+                          rw=0.0, share=info.stepsPerTooth / 4, type=typ, lut=lut, cornerR=rc, edge=G.simx_edge(case)))
         oa, oi = oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"]), oracle.OracleStock(st["w"], st["l"], st["h"], case["cell"])
         oi.set_wall_band(2.0)
         oe = oracle.OracleEngine()
@@ -94,7 +95,7 @@ def test_sim_pipeline_matches_oracle_per_substep():
         oe.set_wear(kte, kre)
         typ_o, lut_o, _ = G.simx_tool(case, res, oracle.tip_lut)
         orc.append(oracle.sim_run(oa, oi, oe, sub, n, res["toolR"], res["fluteLen"], 0.0, info.stepsPerTooth / 4,
-                                  typ_o, lut_o, rc))
+                                  typ_o, lut_o, rc, edge=G.simx_edge(case)))
     outs = lib.sim_run(progs)
     cuts = 0
     for (case, res), out, want in zip(zip(g["cases"], g["results"]), outs, orc):

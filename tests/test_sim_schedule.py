@@ -36,8 +36,8 @@ def test_sim_substeps_match_reference_runs():
 
 def test_oracle_sim_loop_matches_reference_runs():
     """the oracle's restatement of the sub-step loop (oracle/simloop.c) against the same
-    runs (flat, ball and bull-nose tools): deflected stamp positions, vibration values, step
-    counts, per-stamp volumes and every map, bit for bit"""
+    runs (flat, ball, bull-nose and imported-edge tools): deflected stamp positions, vibration
+    values, step counts, per-stamp volumes and every map, bit for bit"""
     import oracle
     g = G.load("simx")
     for case, res in zip(g["cases"], g["results"]):
@@ -53,7 +53,7 @@ def test_oracle_sim_loop_matches_reference_runs():
         sub, n = G.simx_program(res)
         typ, lut, rc = G.simx_tool(case, res, oracle.tip_lut)
         out = oracle.sim_run(act, ide, eng, sub, n, res["toolR"], res["fluteLen"], 0.0, info.stepsPerTooth / 4,
-                             typ, lut, rc)
+                             typ, lut, rc, edge=G.simx_edge(case))
         a = res["actual"]
         assert np.array_equal([sub[k].x + out[k].dx for k in range(n)], G.f64(a["cx"])), case["tool"]
         assert np.array_equal([sub[k].y + out[k].dy for k in range(n)], G.f64(a["cy"])), case["tool"]
