@@ -319,7 +319,7 @@ __host__ __device__ inline double pow_js(double x, double y) {
   t1 = z - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
   r = (z * t1) / (t1 - 2.0) - (w + z * w);
   z = 1.0 - (r - z);
-  j = hiw(z) + (n << 20);
+  j = hiw(z) + (int32_t)((uint32_t)n << 20);   // n may be negative: shift its bits
   if ((j >> 20) <= 0) return __builtin_ldexp(z, n);  // subnormal result
   return words((uint32_t)j, low(z));
 }
@@ -404,7 +404,7 @@ MC_HD double pow_cell(double x, double y, double y1, double yr) {
   const double e1 = z - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
   r = (z * e1) / (e1 - 2.0) - (w + z * w);
   z = 1.0 - (r - z);
-  const double res = words((uint32_t)(hiw(z) + (n << 20)), low(z));
+  const double res = words((uint32_t)hiw(z) + ((uint32_t)n << 20), low(z));
   return x == 1.0 ? 1.0 : res;
 }
 
@@ -488,7 +488,7 @@ MC_HD double atan2_js(double y, double x) {
   const uint32_t lx = low(x), ly = low(y);
   if ((uint32_t)(ix | ((lx | (0u - lx)) >> 31)) > 0x7ff00000u || (uint32_t)(iy | ((ly | (0u - ly)) >> 31)) > 0x7ff00000u)
     return x + y;   // a NaN
-  if (((hx - 0x3ff00000) | (int32_t)lx) == 0) return atan_js(y);   // x = 1
+  if ((((uint32_t)hx - 0x3ff00000u) | lx) == 0) return atan_js(y);   // x = 1
   int m = ((hy >> 31) & 1) | ((hx >> 30) & 2);                      // 2 sign(x) + sign(y)
   if ((iy | (int32_t)ly) == 0) {                                     // y = 0
     if (m < 2) return y;
