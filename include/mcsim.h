@@ -589,6 +589,13 @@ typedef struct {
   double edge_dz;
   int32_t edge_n;
   int32_t pad3;
+  double* budgetLeft;      /* dynBudget > 0: NULL, or in and out.  In: the frame's budget left when the
+                              call starts, taken unless the call's first sub-step starts a frame (a
+                              call that begins inside a frame: the simulator re-configures the engine
+                              on a spindle-speed change, ensureEngine, src/sim.js:252-257,382, so a
+                              caller ends the call before that sub-step, configures, and carries the
+                              budget into the next call).  Out: the budget left after the call's last
+                              sub-step.  NULL: the full budget when the call starts */
 } mc_sim_program;
 
 /* n programs, one workgroup each, one launch; the stocks and engines must be distinct */

@@ -991,8 +991,10 @@ bool prop_val(napi_env env, napi_value o, const char* k, napi_value* v) {
 // simRun(programs: [{actual, ideal (stock handles), engine (engine handle), sub (ArrayBuffer of
 //   mc_substep), R, fluteLen, rw, share, type, lut (Float32Array, bull-nose or profile), cornerR,
 //   edgeR (Float64Array), edgeDz (an imported edge), dynBudget, fz, frames (true: the frames'
-//   stats and chatter index)}])
-//   -> [[ArrayBuffer of mc_substep_out, ArrayBuffer of mc_frame_out], ...]
+//   stats and chatter index), budgetLeft (the frame budget left when the call starts inside a
+//   frame)}])
+//   -> [[ArrayBuffer of mc_substep_out, ArrayBuffer of mc_frame_out, budget left after the call
+//        (a number, or undefined without budgetLeft)], ...]
 // the programs' sub-step loops in one device call (mc_sim_run)
 napi_value SimRun(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -1004,11 +1006,16 @@ napi_value SimRun(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_array_length(env, argv[0], &n));
   std::vector<mc_sim_program> progs(n);
   std::vector<napi_value> outs(n), frs(n);
+  std::vector<double> left(n, 0.0);
   for (uint32_t i = 0; i < n; i++) {
     napi_value o, v;
     NAPI_OK(napi_get_element(env, argv[0], i, &o));
     mc_sim_program& g = progs[i];
     std::memset(&g, 0, sizeof g);
+    if (prop_val(env, o, "budgetLeft", &v)) {
+      left[i] = prop_num(env, o, "budgetLeft", 0);
+      g.budgetLeft = &left[i];
+    }
     if (!prop_val(env, o, "actual", &v) || !(g.actual = stock_of(env, v))) return nullptr;
     if (!prop_val(env, o, "ideal", &v) || !(g.ideal = stock_of(env, v))) return nullptr;
     if (!prop_val(env, o, "engine", &v) || !(g.engine = engine_of(env, v))) return nullptr;
@@ -1065,10 +1072,13 @@ napi_value SimRun(napi_env env, napi_callback_info info) {
   napi_value res;
   NAPI_OK(napi_create_array_with_length(env, n, &res));
   for (uint32_t i = 0; i < n; i++) {
-    napi_value pair;
-    NAPI_OK(napi_create_array_with_length(env, 2, &pair));
+    napi_value pair, bl;
+    NAPI_OK(napi_create_array_with_length(env, 3, &pair));
     NAPI_OK(napi_set_element(env, pair, 0, outs[i]));
     NAPI_OK(napi_set_element(env, pair, 1, frs[i]));
+    if (progs[i].budgetLeft) NAPI_OK(napi_create_double(env, left[i], &bl));
+    else NAPI_OK(napi_get_undefined(env, &bl));
+    NAPI_OK(napi_set_element(env, pair, 2, bl));
     NAPI_OK(napi_set_element(env, res, i, pair));
   }
   return res;
@@ -1083,6 +1093,7 @@ napi_value Sizes(napi_env env, napi_callback_info) {
       {"mc_engage_query", sizeof(mc_engage_query)}, {"mc_engage_result", sizeof(mc_engage_result)}, {"mc_lobe_args", sizeof(mc_lobe_args)},
       {"mc_move", sizeof(mc_move)}, {"mc_path_point", sizeof(mc_path_point)}, {"mc_substep", sizeof(mc_substep)},
       {"mc_substep_out", sizeof(mc_substep_out)}, {"mc_frame_out", sizeof(mc_frame_out)},
+      {"mc_sim_program", sizeof(mc_sim_program)},
       {"MC_SURR_IN", MC_SURR_IN}, {"MC_MAX_MODES", MC_MAX_MODES}, {"MC_MAX_FLUTES", MC_MAX_FLUTES}};
   for (auto& e : t) {
     napi_create_uint32(env, (uint32_t)e.s, &v);

@@ -53,6 +53,8 @@ struct ProgDev {
   double wstep, wallE, wallRho, kTool;
   int32_t flex, n20, n60, pad2;
   EdgeProfile edge;   // an imported edge profile (edge.r null otherwise)
+  double dynLeft0;    // the frame budget left when the call starts (a call may begin inside a frame)
+  double* dynOut;     // the budget left after the last sub-step, or null
 };
 
 // Stock.sample (src/stock.js:85-92)
@@ -626,7 +628,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
   const ToolShape shape{P.ball, R, P.rc, P.lut, P.lutn};
   const double cell = jmax(P.act.hx, P.act.hy);
   double fzLast = P.fz0;   // this.fz: the last cutting sub-step's feed per tooth
-  double dynLeft = P.dynBudget;   // the frame's dynamics budget left (src/sim.js:360)
+  double dynLeft = P.dynLeft0;    // the frame's dynamics budget left (src/sim.js:360)
   int frame = 0;                  // frames ended so far
   bool held = false;              // this.wall is not null (thread 0's)
   for (int k = 0; k < P.nsub; k++) {
@@ -785,6 +787,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
       __syncthreads();
     }
   }
+  if (P.dynOut && tid == 0) *P.dynOut = dynLeft;
 }
 
 }  // namespace
@@ -868,6 +871,10 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     d.rc = g.type == MC_TOOL_BULL ? std::min(g.R, g.cornerR) : 0.0;
     d.edge = EdgeProfile{nullptr, g.type == MC_TOOL_PROFILE ? g.edge_n : 0, g.edge_dz};
     d.dynBudget = g.dynBudget > 0 ? g.dynBudget : 0; d.fz0 = g.fz;
+    d.dynLeft0 = g.budgetLeft ? *g.budgetLeft : d.dynBudget;
+    d.dynOut = nullptr;
+    if (g.budgetLeft && !std::isfinite(d.dynLeft0))
+      return mch::fail(MC_ERR_ARG, "program %d: non-finite frame budget left", p);
     d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0; d.pad = 0;
     d.sacc = nullptr; d.walls = nullptr; d.flex = g.flex; d.n20 = 0; d.n60 = 0; d.pad2 = 0;
     d.wallE = g.wallE; d.wallRho = g.wallRho; d.kTool = g.kTool;
@@ -953,6 +960,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   double* dsacc = nullptr;
   mc_wall_out* dwall = nullptr;
   double* dedge = nullptr;
+  double* ddyn = nullptr;
   auto release = [&]() {
     if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
@@ -962,6 +970,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (dsacc) (void)hipFree(dsacc);
     if (dwall) (void)hipFree(dwall);
     if (dedge) (void)hipFree(dedge);
+    if (ddyn) (void)hipFree(ddyn);
   };
   if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
@@ -970,7 +979,8 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
       hipMalloc(&dlut, sizeof(float) * std::max<size_t>(1, nlut)) != hipSuccess ||
       hipMalloc(&dsacc, sizeof(double) * std::max<size_t>(1, sacc.size())) != hipSuccess ||
       hipMalloc(&dwall, sizeof(mc_wall_out) * std::max<size_t>(1, nwall)) != hipSuccess ||
-      hipMalloc(&dedge, sizeof(double) * std::max<size_t>(1, nedge)) != hipSuccess) {
+      hipMalloc(&dedge, sizeof(double) * std::max<size_t>(1, nedge)) != hipSuccess ||
+      hipMalloc(&ddyn, sizeof(double) * (size_t)n) != hipSuccess) {
     release();
     return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
   }
@@ -1002,6 +1012,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     }
     pd[(size_t)p].sub = dsub + off;
     pd[(size_t)p].out = dout + off;
+    if (g.budgetLeft) pd[(size_t)p].dynOut = ddyn + p;
     if (pd[(size_t)p].nframes > 0) {
       pd[(size_t)p].frames = dfr + foff;
       foff += (size_t)pd[(size_t)p].nframes;
@@ -1050,6 +1061,9 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
         st = mch::fail(MC_ERR_HIP, "wall record download");
       woff += (size_t)g.nsub;
     }
+    if (!st && g.budgetLeft &&
+        hipMemcpyAsync(g.budgetLeft, ddyn + p, sizeof(double), hipMemcpyDeviceToHost, q) != hipSuccess)
+      st = mch::fail(MC_ERR_HIP, "budget download");
     const int fst = mce_engine_finish(g.engine);
     if (!st) st = fst;
   }

@@ -17,9 +17,12 @@ const OUT_F = ['dx', 'dy', 'ap', 'errVal', 'vol', 'maxF', 'phiSt', 'phiEx'];
 // configured for the program's tool and speed, its setWear applied; the loop starts from the
 // state its device calls left), schedule ({sub, n} from simSchedule), tool {R, fluteLen, type: 'flat' | 'ball' |
 // 'bull' | 'profile', cornerR, lut, edge (an imported edge {dz, r}, src/assembly.js:31-35)}, rw,
-// share, dynBudget, fz, frames (true: frame results)}].
+// share, dynBudget, fz, frames (true: frame results), budgetLeft (the frame budget left when the
+// call starts inside a frame: after a spindle-speed change re-configured the engine,
+// ensureEngine, src/sim.js:252-257)}].
 // Returns per program {out: [{dx, dy, ap, errVal, vol, maxF, phiSt, phiEx, nsteps, cut, arc}],
-// frames: [{stats, index, amp, once, end}]}.  The actual stock's removed volume and curLine
+// frames: [{stats, index, amp, once, end}], budgetLeft (the budget left after the last sub-step,
+// when budgetLeft was given)}.  The actual stock's removed volume and curLine
 // advance as its stamps do; the commanded stock's removed volume is not advanced (the
 // simulator reads the actual one).
 export function simRun(programs) {
@@ -33,11 +36,12 @@ export function simRun(programs) {
       nsub: p.schedule.n, R: t.R, fluteLen: t.fluteLen, rw: p.rw || 0, share: p.share === undefined ? 1 : p.share,
       type: TYPE[t.type], lut: t.type === 'bull' || t.type === 'profile' ? Float32Array.from(t.lut) : null,
       cornerR: t.cornerR || 0, edgeR: t.type === 'profile' ? Float64Array.from(t.edge.r) : null,
-      edgeDz: t.type === 'profile' ? t.edge.dz : 0, dynBudget: p.dynBudget || 0, fz: p.fz || 0, frames: !!p.frames };
+      edgeDz: t.type === 'profile' ? t.edge.dz : 0, dynBudget: p.dynBudget || 0, fz: p.fz || 0, frames: !!p.frames,
+      budgetLeft: p.budgetLeft === undefined ? null : p.budgetLeft };
   });
   const res = native().simRun(args);
   return programs.map((p, i) => {
-    const [ob, fb] = res[i], od = new DataView(ob), fd = new DataView(fb);
+    const [ob, fb, left] = res[i], od = new DataView(ob), fd = new DataView(fb);
     const n = ob.byteLength / OUT_BYTES, out = new Array(n);
     let vol = 0;
     for (let k = 0; k < n; k++) {
@@ -63,7 +67,7 @@ export function simRun(programs) {
     p.stock._changed();
     p.ideal._changed();
     p.engine._sync(native().engineState(p.engine._h), false);
-    return { out, frames };
+    return left === undefined ? { out, frames } : { out, frames, budgetLeft: left };
   });
 }
 

@@ -172,7 +172,7 @@ class SimProgram(C.Structure):
                 ("frames", C.c_void_p), ("flex", C.c_int32), ("pad", C.c_int32), ("wallE", C.c_double),
                 ("wallRho", C.c_double), ("kTool", C.c_double), ("walls", C.POINTER(WallOut)),
                 ("edge_r", C.POINTER(C.c_double)), ("edge_dz", C.c_double), ("edge_n", C.c_int32),
-                ("pad3", C.c_int32)]
+                ("pad3", C.c_int32), ("budgetLeft", C.POINTER(C.c_double))]
 
 
 MC_STOCK_B = 8

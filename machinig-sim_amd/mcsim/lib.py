@@ -331,12 +331,14 @@ def sim_run(programs, frames=False, walls=False):
     sub-step instead of the fixed share), fz (this.fz at the start), edge (dict(r, dz): the
     imported edge of an MC_TOOL_PROFILE tool, whose lut is its mc_tip_lut table) and wall (dict(E, rho,
     kTool): the thin-wall model, workpieceFlex; the engine configured with fnMax
-    MC_WALL_FN_CAP); returns a SubstepOut list per program, with frames=True also a FrameOut
+    MC_WALL_FN_CAP) and budget_left (a one-element list: the frame budget left when the call
+    starts inside a frame, replaced by the budget left after its last sub-step); returns a
+    SubstepOut list per program, with frames=True also a FrameOut
     list per program (one per sub-step flagged MC_FRAME_LAST), and with walls=True also a
     WallOut list per program (None for a rigid workpiece), in that order"""
     n = len(programs)
     arr = (SimProgram * max(1, n))()
-    outs, keep, frs, wls = [], [], [], []
+    outs, keep, frs, wls, bls = [], [], [], [], []
     for i, g in enumerate(programs):
         sub = g["sub"]
         ns = int(g.get("nsub", len(sub)))
@@ -356,6 +358,9 @@ def sim_run(programs, frames=False, walls=False):
         e = g.get("edge")
         er = np.ascontiguousarray(e["r"], dtype=np.float64) if e is not None else None
         keep.append(er)
+        bl = g.get("budget_left")
+        bcell = C.c_double(float(bl[0])) if bl is not None else None
+        bls.append((bl, bcell))
         arr[i] = SimProgram(g["actual"].p.value, g["ideal"].p.value, g["engine"].p.value,
                             C.cast(sub, C.POINTER(Substep)), C.cast(out, C.POINTER(SubstepOut)),
                             float(g["R"]), float(g["fluteLen"]), float(g.get("rw", 0.0)), float(g.get("share", 1.0)),
@@ -367,8 +372,12 @@ def sim_run(programs, frames=False, walls=False):
                             float(w["E"]) if w else 0.0, float(w["rho"]) if w else 0.0,
                             float(w["kTool"]) if w else 0.0, C.cast(wo, C.POINTER(WallOut)) if w else None,
                             er.ctypes.data_as(C.POINTER(C.c_double)) if er is not None else None,
-                            float(e["dz"]) if e is not None else 0.0, len(er) if er is not None else 0, 0)
+                            float(e["dz"]) if e is not None else 0.0, len(er) if er is not None else 0, 0,
+                            C.pointer(bcell) if bcell is not None else None)
     _check(load().mc_sim_run(arr, n))
+    for bl, bcell in bls:
+        if bl is not None:
+            bl[0] = bcell.value
     res = [[o[:ns] for o, ns in outs]]
     if frames:
         res.append([f[:nf] for f, nf in frs])

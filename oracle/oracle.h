@@ -148,7 +148,7 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
                mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes, int flex,
                double E, double rho, double kTool, mc_wall_out *walls, const double *edge_r, int edge_n,
-               double edge_dz);
+               double edge_dz, double *budgetLeft);
 /* wall.c: wallAt (src/wall.js:11-112); 1 with the wall fields of *w set, 0 for null */
 int or_wall_at(const or_stock *s, double px, double py, double pz, double R, const float *Hb, double ux, double uy,
                double zLoad, double E, double rho, double kTool, mc_wall_out *w);

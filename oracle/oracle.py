@@ -63,7 +63,7 @@ def lib():
         L.or_engine_read_rings.argtypes = [P, P, P, P, P]
         L.or_sim_run.argtypes = [P, P, P, C.POINTER(Substep), C.c_int] + [C.c_double] * 4 + \
             [C.c_int, P, C.c_int, C.c_double, C.POINTER(SubstepOut), C.c_double, C.c_double, P, C.c_int,
-             C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(WallOut), P, C.c_int, C.c_double]
+             C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(WallOut), P, C.c_int, C.c_double, P]
         L.or_wall_at.argtypes = [P] + [C.c_double] * 4 + [P] + [C.c_double] * 6 + [C.POINTER(WallOut)]
         L.or_wall_at.restype = C.c_int
         L.or_stock_zbottom.argtypes = [P]
@@ -284,12 +284,14 @@ class OracleEngine:
 
 
 def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None, cornerR=0.0, dynBudget=0.0,
-            fz0=0.0, nframes=0, wall=None, edge=None):
+            fz0=0.0, nframes=0, wall=None, edge=None, budget_left=None):
     """the simulator's sub-step loop restated (oracle/simloop.c); returns SubstepOut records,
     and with nframes > 0 also the FrameOut records of the frames that ended.  wall: None for a
     rigid workpiece, or dict(E, rho, kTool) for the thin-wall model (workpieceFlex); then the
     WallOut records of every sub-step come last in the returned tuple.  edge: dict(r, dz) of
-    an MC_TOOL_PROFILE tool (its imported edge)"""
+    an MC_TOOL_PROFILE tool (its imported edge).  budget_left: None, or a one-element list
+    holding the frame budget left when the call starts (a call that begins inside a frame),
+    replaced by the budget left after its last sub-step"""
     import numpy as np
     out = (SubstepOut * max(1, n))()
     fr = (FrameOut * max(1, nframes))()
@@ -297,6 +299,7 @@ def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None
     la = np.ascontiguousarray(lut, dtype=np.float32) if lut is not None else None
     w = wall or {}
     er = np.ascontiguousarray(edge["r"], dtype=np.float64) if edge is not None else None
+    bl = C.c_double(float(budget_left[0]) if budget_left is not None else 0.0)
     st = lib().or_sim_run(actual.p, ideal.p, engine.h, C.cast(sub, C.POINTER(Substep)), int(n), float(R),
                           float(fluteLen), float(rw), float(share), int(typ),
                           la.ctypes.data if la is not None else None, len(la) if la is not None else 0,
@@ -304,9 +307,12 @@ def sim_run(actual, ideal, engine, sub, n, R, fluteLen, rw, share, typ, lut=None
                           1 if wall else 0, float(w.get("E", 0.0)), float(w.get("rho", 0.0)),
                           float(w.get("kTool", 0.0)), wo if wall else None,
                           er.ctypes.data if er is not None else None, len(er) if er is not None else 0,
-                          float(edge["dz"]) if edge is not None else 0.0)
+                          float(edge["dz"]) if edge is not None else 0.0,
+                          C.byref(bl) if budget_left is not None else None)
     if st:
         raise RuntimeError("or_sim_run: %d" % st)
+    if budget_left is not None:
+        budget_left[0] = bl.value
     res = [out[:n]]
     if nframes:
         ended = min(nframes, sum(1 for k in range(n) if sub[k].frame & 2))

@@ -114,7 +114,7 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
                double fluteLen, double rw, double share, int type, const float *lut, int lutn, double cornerR,
                mc_substep_out *out, double dynBudget, double fz0, mc_frame_out *frames, int nframes, int flex,
                double E, double rho, double kTool, mc_wall_out *walls, const double *edge_r, int edge_n,
-               double edge_dz) {
+               double edge_dz, double *budgetLeft) {
   const int ball = type == MC_TOOL_BALL, prof = type == MC_TOOL_PROFILE, bull = type == MC_TOOL_BULL || prof;
   const Shape shape = {ball, R, bull ? lut : NULL, bull ? lutn : 0,
                        type == MC_TOOL_BULL ? (cornerR < R ? cornerR : R) : 0, prof ? edge_r : NULL,
@@ -122,7 +122,7 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
   const float *slut = bull ? lut : NULL;
   const int stype = ball ? MC_TOOL_BALL : MC_TOOL_FLAT;
   float Hb[MC_BINS], Eb[MC_END_BINS * MC_END_RINGS];
-  double fzLast = fz0, dynLeft = dynBudget;
+  double fzLast = fz0, dynLeft = budgetLeft ? *budgetLeft : dynBudget;   /* a call may begin inside a frame */
   int k, frame = 0, held = 0;   /* held: this.wall is not null */
   memset(Hb, 0, sizeof(Hb));
   memset(Eb, 0, sizeof(Eb));
@@ -238,5 +238,6 @@ int or_sim_run(or_stock *act, or_stock *ide, or_engine *e, const mc_substep *sub
       frame++;
     }
   }
+  if (budgetLeft) *budgetLeft = dynLeft;
   return MC_OK;
 }

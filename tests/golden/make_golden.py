@@ -252,11 +252,20 @@ def build_cases(quick):
         y0 = 2 + 3 * k
         raster += ["G1 X34 Y%g F1200" % y0, "G1 Y%g" % (y0 + 1.5), "G1 X-6", "G1 Y%g" % (y0 + 3)]
     raster += ["G0 Z5", "M30"]
+    # the same raster with spindle-speed changes inside passes: the simulator re-configures its
+    # engine at the sub-step that first sees the new speed (ensureEngine), inside a frame
+    sraster = list(raster)
+    for cut, at, S in (("G1 X34 Y8 F1200", "G1 X14 Y8 F1200", 12500), ("G1 X34 Y17 F1200", "G1 X20 Y17 F1200", 8000)):
+        i = sraster.index(cut)
+        sraster[i:i + 1] = [at, "G1 X34 S%d" % S]
     cases["report"] = [dict(cases["sim"][0], speed=20, dtReal=1 / 60), dict(cases["sim"][1], speed=120, dtReal=1 / 60),
                        dict(cases["sim"][2], speed=60, dtReal=1 / 60),
                        dict(program="\n".join(raster), tool=dict(type="flat", D=8.0),
                             stock=dict(type="box", w=30.0, l=24.0, h=6.0), cell=0.2, speed=60, dtReal=1 / 60,
-                            dynBudget=60000)]
+                            dynBudget=60000),
+                       dict(program="\n".join(sraster), tool=dict(type="flat", D=8.0),
+                            stock=dict(type="box", w=30.0, l=24.0, h=6.0), cell=0.2, speed=40, dtReal=1 / 60,
+                            dynBudget=20000)]
     cases["tiplut"] = tools + [dict(D=0.03, type="ball"), dict(D=12.0, type="bull", cornerR=0.0),
                                dict(D=10.0, type="bull", cornerR=2.5), dict(D=7.3, type="flat"),
                                dict(D=3.0, type="flat", edge=dict(dz=0.1, r=[0.2, 0.8, 1.5, 1.5, 1.5]))]

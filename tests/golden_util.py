@@ -329,9 +329,9 @@ def simx_program(res):
     return path.sim_substeps(x, y, z, P["rapid"], P["line"], P["F"], P["S"], res["toolN"], res["hx"])
 
 
-def simx_cell(res):
-    """the engine's configure arguments recorded in a simx run"""
-    op = res["cfgs"][0]
+def simx_cell(res, j=0):
+    """the engine's configure arguments recorded in a simx run (its j-th configure call)"""
+    op = res["cfgs"][j]
     _, N, R, hd, typ, fl, nl, ro, rph, S, fnMax, mxs, mys, mat = op
     return make_cell(dict(N=N, R=R, helixDeg=hd, type=typ, fluteLen=fl, runout=ro, runoutPhase=rph),
                      mxs, mys, dict(zip(MAT_ORDER, mat)), S, (0.0, 0.0), 0.0, 1, nl), fnMax
@@ -405,8 +405,12 @@ def run_report(case, res, mk_stock, mk_engine, run_frame, tip_lut, tan):
     """replay a recorded real-time run frame by frame: each frame is one call of the sub-step
     loop with the frame's flank wear (the setWear the simulator issued, and the radial wear
     rw = min(R/2, VB tan(clearance)) of src/sim.js:439 from the wear VB it had) and the feed
-    mark carried over; the report is built from the outputs.  run_frame(act, ide, eng, sub,
-    count, R, fluteLen, rw, typ, lut, rc, dynBudget, fz0) returns (outs, [frame]).
+    mark carried over; the report is built from the outputs.  Where the simulator re-configured
+    its engine inside a frame (a spindle-speed change, ensureEngine, src/sim.js:252-257,382),
+    the frame is two calls: the first ends before that sub-step, the engine is configured with
+    the recorded arguments, and the second call starts with the frame budget the first left.
+    run_frame(act, ide, eng, sub, count, R, fluteLen, rw, typ, lut, rc, dynBudget, fz0,
+    budget_left) returns (outs, [frame] or []).
     Returns (outs, frames, reporter, stocks, sub, n, frame list)."""
     import ctypes as C
     from mcsim import report as RP
@@ -424,20 +428,35 @@ def run_report(case, res, mk_stock, mk_engine, run_frame, tip_lut, tan):
     rep = RP.FrameReporter()
     outs, fouts, fz0 = [], [], 0.0
     wears = res["wears"]
+    recfg = [op[0] for op in res["cfgs"]]   # sub-step index of each configure call
     for f, (first, count) in enumerate(frames):
         w = [x for x in wears if x[0] <= first][-1] if count else wears[0]
         eng.set_wear(w[1], w[2])
         VB = res["frames"][f]["VB0"]
         rw = min(0.5 * R, VB * clr)
         if count:
-            part = (Substep * count).from_address(C.addressof(sub[first]))
-            o, fo = run_frame(act, ide, eng, part, count, R, res["fluteLen"], rw, typ, lut, rc, res["dynBudget"], fz0)
+            # the frame's calls: a new call wherever the engine was re-configured inside it
+            cuts = sorted(set([first, first + count] + [c for c in recfg[1:] if first < c < first + count]))
+            left = [float(res["dynBudget"])] if len(cuts) > 2 else None
+            o, fo = [], []
+            for a, b in zip(cuts[:-1], cuts[1:]):
+                if a > 0 and a in recfg:   # the last configure call the simulator made at sub-step a
+                    j = max(q for q, c in enumerate(recfg) if c == a)
+                    cell_j, fn_j = simx_cell(res, j)
+                    eng.configure(cell_j, fn_j)
+                    eng.set_wear(w[1], w[2])
+                part = (Substep * (b - a)).from_address(C.addressof(sub[a]))
+                o2, fo2 = run_frame(act, ide, eng, part, b - a, R, res["fluteLen"], rw, typ, lut, rc, res["dynBudget"],
+                                    fz0, left)
+                for k in range(b - a):
+                    if o2[k].cut:
+                        fz0 = part[k].fzEff
+                o += list(o2)
+                fo += list(fo2)
             assert len(fo) == 1
-            outs += list(o)
+            outs += o
             fouts.append(fo[0])
-            for k in range(count):
-                if o[k].cut:
-                    fz0 = part[k].fzEff
+            part = (Substep * count).from_address(C.addressof(sub[first]))
             cut_time = rep.substeps(part, o, 0, count)
             rep.collect(fo[0].stats, (fo[0].index, fo[0].amp, fo[0].once), cut_time)
         else:

@@ -13,10 +13,11 @@ import golden_util as G
 pytestmark = pytest.mark.gpu
 
 
-def _frame_gpu(act, ide, eng, sub, count, R, fl, rw, typ, lut, rc, dyn, fz0):
+def _frame_gpu(act, ide, eng, sub, count, R, fl, rw, typ, lut, rc, dyn, fz0, left):
     from mcsim import lib
     outs, frs = lib.sim_run([dict(actual=act, ideal=ide, engine=eng, sub=sub, nsub=count, R=R, fluteLen=fl, rw=rw,
-                                  type=typ, lut=lut, cornerR=rc, dynBudget=dyn, fz=fz0)], frames=True)
+                                  type=typ, lut=lut, cornerR=rc, dynBudget=dyn, fz=fz0, budget_left=left)],
+                            frames=True)
     return outs[0], frs[0]
 
 

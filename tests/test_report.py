@@ -11,9 +11,10 @@ import numpy as np
 import golden_util as G
 
 
-def _frame_oracle(act, ide, eng, sub, count, R, fl, rw, typ, lut, rc, dyn, fz0):
+def _frame_oracle(act, ide, eng, sub, count, R, fl, rw, typ, lut, rc, dyn, fz0, left):
     import oracle
-    return oracle.sim_run(act, ide, eng, sub, count, R, fl, rw, 1.0, typ, lut, rc, dynBudget=dyn, fz0=fz0, nframes=1)
+    return oracle.sim_run(act, ide, eng, sub, count, R, fl, rw, 1.0, typ, lut, rc, dynBudget=dyn, fz0=fz0, nframes=1,
+                          budget_left=left)
 
 
 def _maps(s):
