@@ -28,7 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "machinig-sim_amd"))
 import numpy as np  # noqa: E402
 
 from mcsim import lib, workloads as W, path  # noqa: E402
-from mcsim._abi import cell_array, MC_TOOL_BALL  # noqa: E402
+from mcsim._abi import cell_array, MC_TOOL_BALL, MC_FRAME_LAST  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, vendor figure
 DYN_BUDGET = 220000.0     # the simulator's default dynamics budget per frame (src/sim.js:50)
@@ -394,26 +394,31 @@ def dropin_substeps(cpu):
     return out
 
 
-def line_cells(progs, outs, frs, tools, revs=40):
+def line_cells(progs, outs, frs, tools, revs=40, ckeys=None):
     """The machining report's per-line stability question: each program's report is built
     from the pipeline's frames (mcsim.report: Simulator.collect and LineReport,
     src/sim.js:644-710, src/report.js:9-33), and every line with an engagement arc asks for
     its limit at its heaviest cut's arc and the line's speed (src/main.js:1033-1038), here
     by the time-domain search.  Returns the cells, (program, line, ap) keys and the per-frame
-    chatter states of every program."""
+    chatter states of every program; with a list ckeys, each cell's (tool, arc, feed) identity
+    is appended to it (cells of equal identity are the same search)."""
     from mcsim._abi import make_cell
     from mcsim import report as RP
     cells, keys, states = [], [], []
     for p, ((sub, n, frames), out, fr, tool) in enumerate(zip(progs, outs, frs, tools)):
         rep = RP.report_program(sub, out, frames, fr)
         states.append(rep.states)
+        if ckeys is not None:
+            print("coupled: report of program %d built" % p, file=sys.stderr, flush=True)
         for ln, ap, st, ex, fz in RP.line_engagements(rep.report, sub, out):
             cells.append(make_cell(tool, W.COUPLED_MODES, None, W.MAT_A5052, W.COUPLED_SPEED, (st, ex), fz, revs, True))
             keys.append((p, ln, ap))
+            if ckeys is not None:
+                ckeys.append((tool["type"], st, ex, fz))
     return cells, keys, states
 
 
-def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
+def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None, full_segments=None, cpu_prefix=None):
     """The simulator's coupled sub-step loop (engagement -> engine run -> commanded and
     deflected stamps, src/sim.js:351-449) for n_prog seeded raster programs, each on its own
     2048 x 2048 stock at 0.05 mm (102.35 mm square), flat 12 mm and ball 8 mm tools taking
@@ -425,14 +430,21 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     device's deflections, step counts, depths, forces, frames and final heights.  Then the
     report (mcsim.report) is built from the frames and every line's stability check
     (line_cells) runs as one batched time-domain search, guided by the surrogate when one was
-    trained (checked identical to the plain search)."""
+    trained (checked identical to the plain search).  full_segments: config 4 at full size
+    instead, programs of that many moves (workloads.coupled_full_program: a zigzag pocket, then
+    chorded contour passes); the lines' checks then run once per distinct (tool, arc, feed),
+    and the oracle's CPU baseline runs the first cpu_prefix sub-steps of every program."""
     from mcsim._abi import make_cell, MC_TOOL_FLAT, MC_TOOL_BALL
     width, cell = 102.35, 0.05
     hx = width / 2047
-    progs, tools, passes = [], [], []
+    progs, tools, passes, nseg = [], [], [], []
     for p in range(n_prog):
         tool, m = W.coupled_plan(p, n_lines)
-        x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, m, speed=W.COUPLED_SPEED)
+        if full_segments:
+            x, y, z, r, ln, F, S = W.coupled_full_program(p + 1, width, full_segments, speed=W.COUPLED_SPEED)
+        else:
+            x, y, z, r, ln, F, S = W.coupled_program(p + 1, width, m, speed=W.COUPLED_SPEED)
+        nseg.append(len(x) - 1)
         # frames of 4000 loop passes with an unbounded machine-time budget (sim.step(1e6)): the
         # frame budget's share is then stepsPerTooth / 4 for every cutting sub-step
         progs.append(path.sim_frames(x, y, z, r, ln, F, S, tool["N"], hx, 1e6))
@@ -462,6 +474,8 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     lib.sim_run(W0)          # warm-up launch (code objects, allocations) on the programs' first sub-steps
     release(W0)
     P = setup()
+    print("coupled: %d programs, %d sub-steps, one launch" % (n_prog, sum(n for _, n, _ in progs)), file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
     outs, frs = lib.sim_run(P, frames=True)
     wall = time.perf_counter() - t0
@@ -473,30 +487,46 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
     # node of their bounding boxes (the commanded stock's wall band is 2 mm)
     nodes = 0
     for (sub, n, _), t in zip(progs, tools):
-        cxs = np.array([sub[k].x for k in range(n)]); cys = np.array([sub[k].y for k in range(n)])
+        sa = np.ctypeslib.as_array(sub)[:n]
+        cxs, cys = sa["x"].astype(np.float64), sa["y"].astype(np.float64)
         for Rw in (t["R"] + 2 * hx, t["R"] + 2.0):
             i0 = np.maximum(0, np.floor((cxs - Rw) / hx)); i1 = np.minimum(2047, np.ceil((cxs + Rw) / hx))
             j0 = np.maximum(0, np.floor((cys - Rw) / hx)); j1 = np.minimum(2047, np.ceil((cys + Rw) / hx))
             nodes += np.where((i1 >= i0) & (j1 >= j0), (i1 - i0 + 1) * (j1 - j0 + 1), 0).sum()
-    res = dict(workload="coupled_%dprog_2048x2048_%dto%dpasses_flat12_ball8" % (n_prog, min(passes), max(passes)),
-               programs=n_prog, passes=passes,
+    wl = ("coupled_%dprog_2048x2048_%dseg_zigzag_contour_flat12_ball8" % (n_prog, full_segments) if full_segments
+          else "coupled_%dprog_2048x2048_%dto%dpasses_flat12_ball8" % (n_prog, min(passes), max(passes)))
+    res = dict(workload=wl, programs=n_prog, passes=None if full_segments else passes,
+               segments=nseg if full_segments else None,
                substeps=nsub, cutting_substeps=cuts, engine_steps=steps, wall_s=wall, kernel_ms=kms,
                substeps_per_s=nsub / wall, engine_steps_per_s=steps / wall,
                roofline={"bound": "hbm", "achieved": 8.0 * nodes / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": 8.0 * nodes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                         "kernel": "sim_run", "traffic": profiled_traffic("sim_run")},
+                         "kernel": "sim_run", "traffic": None if full_segments else profiled_traffic("sim_run")},
                note="algorithmic bytes = 8 B x bbox nodes of both stamps per sub-step; one workgroup per "
                     "program runs its sub-steps in order (each reads the stock the previous one cut), so "
                     "the kernel is bound by the latency of that chain, far below the HBM roof")
-    lc, keys, states = line_cells(progs, outs, frs, tools)
+    print("coupled: kernel %.1f s; building the reports" % (kms / 1e3), file=sys.stderr, flush=True)
+    ck = [] if full_segments else None
+    lc, keys, states = line_cells(progs, outs, frs, tools, ckeys=ck)
+    which = list(range(len(lc)))
+    if full_segments:   # one search per distinct cell; every line reads its cell's result
+        index, uniq = {}, []
+        for i, key in enumerate(ck):
+            if key not in index:
+                index[key] = len(uniq)
+                uniq.append(lc[i])
+            which[i] = index[key]
+        lines_total, lc = len(keys), uniq
     res["frames"] = sum(len(f) for f in frs)
     res["frame_states"] = {k: sum(st.count(k) for st in states) for k in ("stable", "marginal", "chatter", "idle")}
     t0 = time.perf_counter()
     plain = lib.bisect_batch(lc, **opts)
     t_plain = time.perf_counter() - t0
-    checks = dict(lines=len(lc), plain_s=t_plain, plain_kernel_ms=lib.last_kernel_ms(),
-                  lines_over_limit=sum(1 for (p, ln, ap), r in zip(keys, plain) if ap > r.alim),
+    checks = dict(lines=len(keys), plain_s=t_plain, plain_kernel_ms=lib.last_kernel_ms(),
+                  lines_over_limit=sum(1 for (p, ln, ap), w in zip(keys, which) if ap > plain[w].alim),
                   alim_mm_min=min(r.alim for r in plain), alim_mm_max=max(r.alim for r in plain))
+    if full_segments:
+        checks["distinct_cells"] = len(lc)
     if surrogate is not None:
         from mcsim import surrogate as S
         model, sigma = surrogate
@@ -512,6 +542,7 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         k = min(cpu_threads, n_prog)
+        lim = [progs[i][1] if not cpu_prefix else min(progs[i][1], cpu_prefix) for i in range(k)]
         outs_c = [None] * k
         frs_c = [None] * k
         H_c = [None] * k
@@ -523,9 +554,10 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
             eng = oracle.OracleEngine()
             eng.configure(ecells[i])
             eng.set_wear(kte, kre)
-            outs_c[i], frs_c[i] = oracle.sim_run(act, ide, eng, sub, n, tools[i]["R"], tools[i]["fluteLen"], 0.0, 1.0,
-                                                 ttype[i], dynBudget=DYN_BUDGET,
-                                                 nframes=sum(1 for _, c in frames if c))
+            nf = sum(1 for q in range(lim[i]) if sub[q].frame & MC_FRAME_LAST)
+            r = oracle.sim_run(act, ide, eng, sub, lim[i], tools[i]["R"], tools[i]["fluteLen"], 0.0, 1.0, ttype[i],
+                               dynBudget=DYN_BUDGET, nframes=nf)
+            outs_c[i], frs_c[i] = r if nf else (r, [])
             H_c[i] = act.H()
         th = [threading.Thread(target=work, args=(i,)) for i in range(k)]
         t0 = time.perf_counter()
@@ -537,13 +569,16 @@ def coupled(n_prog, n_lines, cpu_threads, cpu, opts, surrogate=None):
         same = all(all(a.dx == b.dx and a.dy == b.dy and a.nsteps == b.nsteps and a.arc == b.arc and
                        a.phiSt == b.phiSt and a.phiEx == b.phiEx and a.ap == b.ap and a.errVal == b.errVal and
                        a.maxF == b.maxF and a.cut == b.cut and abs(a.vol - b.vol) <= 1e-12 * max(1.0, abs(b.vol))
-                       for a, b in zip(outs[i], outs_c[i]))
-                   and all(bytes(a) == bytes(b) for a, b in zip(frs[i], frs_c[i])) and len(frs[i]) == len(frs_c[i])
-                   and np.array_equal(P[i]["actual"].H(), H_c[i]) for i in range(k))
-        csub = sum(progs[i][1] for i in range(k))
+                       for a, b in zip(outs[i][:lim[i]], outs_c[i]))
+                   and len(outs_c[i]) == lim[i]
+                   and all(bytes(a) == bytes(b) for a, b in zip(frs[i], frs_c[i])) and len(frs_c[i]) <= len(frs[i])
+                   and (lim[i] < progs[i][1] or len(frs[i]) == len(frs_c[i]))
+                   and (lim[i] < progs[i][1] or np.array_equal(P[i]["actual"].H(), H_c[i])) for i in range(k))
+        csub = sum(lim)
         res["cpu_baseline"] = dict(value=csub / cw, unit="sub-steps/s", cores=k, kind="port",
-                                   sample="%d of the %d programs on %d host threads (oracle/simloop.c), %.1f s" %
-                                   (k, n_prog, k, cw))
+                                   sample="%d of the %d programs on %d host threads (oracle/simloop.c), %s%.1f s" %
+                                   (k, n_prog, k, "the first %d sub-steps of each, " % cpu_prefix if cpu_prefix else "",
+                                    cw))
         res["identical_to_oracle"] = bool(same)
     release(P)
     return res
@@ -623,6 +658,10 @@ def main():
                     help="sub-steps of the zigzag program run through the coupled loop (its forces)")
     ap.add_argument("--train-cells", type=int, default=262144)
     ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--coupled-full", type=int, default=0, metavar="SEGMENTS",
+                    help="only the coupled leg at config 4's full size: programs of SEGMENTS moves (1e5), the "
+                         "oracle on the first --cpu-prefix sub-steps of each: one JSON object")
+    ap.add_argument("--cpu-prefix", type=int, default=20000)
     ap.add_argument("--heightmap-only", action="store_true",
                     help="only the heightmap leg (for its profiles): one JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -637,6 +676,11 @@ def main():
         sys.exit(2)
     if a.heightmap_only:
         print(json.dumps(heightmap(a.steps, not a.no_cpu, a.cpu_seconds)))
+        return
+    if a.coupled_full:
+        _, opts = W.sld_grid(n_speeds=2, n_tools=1)
+        print(json.dumps(coupled(a.coupled_programs, a.coupled_lines, 16, not a.no_cpu, opts,
+                                 full_segments=a.coupled_full, cpu_prefix=a.cpu_prefix)))
         return
     cells, opts = W.sld_grid(n_speeds=a.speeds, n_tools=a.tools)
     arr = cell_array(cells)

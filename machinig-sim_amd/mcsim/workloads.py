@@ -241,6 +241,79 @@ def coupled_program(seed, width, n_lines=4, feed=1000.0, speed=10000.0):
     return xs, ys, zs, rapid, line, F, S
 
 
+def coupled_full_program(seed, width, n_segments=100000, feed=1000.0, speed=10000.0, chord=0.09):
+    """One seeded mixed roughing / finishing program of about n_segments linear moves for the
+    coupled sub-step loop on a width x width stock: a plunge beside the stock's corner, a
+    zigzag pocket at depth d ~ U(0.5, 3) with stepover ~ U(2, 5) along y over the whole
+    stock, then contour passes: rounded-rectangle loops (corner radius 8 mm) inset by
+    ~ U(3, 6) mm from the stock's edge, each loop 0.2 mm deeper than the last below the
+    pocket floor and 0.3 mm further in, chorded into moves of about `chord` mm (as the path
+    discretizer chords arcs), until the program holds n_segments moves.  Returns the path as
+    lists (x, y, z, rapid, line, F, S), point 0 being the start of the first move."""
+    import math
+    rng = ParkMiller(seed)
+    d = rng.uniform(0.5, 3.0)
+    step = rng.uniform(2.0, 5.0)
+    inset = rng.uniform(3.0, 6.0)
+    y = 2.0
+    pts = [(-6.0, y, 5.0, 1), (-6.0, y, 2.0, 1), (2.0, y, 2.0, 1), (2.0, y, -d, 0)]
+    F = [None, None, None, feed / 3]
+    direction = 1
+    while True:
+        x = width - 2.0 if direction > 0 else 2.0
+        pts.append((x, y, -d, 0)); F.append(feed)
+        if y + step > width - 2.0:
+            break
+        y += step
+        pts.append((x, y, -d, 0)); F.append(feed)
+        direction = -direction
+    # contour loops: a rounded rectangle traced counter-clockwise from the middle of its
+    # lower edge, chorded; each loop starts with a short ramp to its depth
+    rc, z, o = 8.0, -d, inset
+    while len(pts) < n_segments + 1:
+        z -= 0.2
+        lo, hi = o, width - o
+        if hi - lo < 2 * rc + 1.0:
+            break
+        path = []
+        cx0, cy0 = (lo + hi) / 2, lo
+        path.append((cx0, cy0))
+        corners = [(hi - rc, lo + rc, -math.pi / 2), (hi - rc, hi - rc, 0.0), (lo + rc, hi - rc, math.pi / 2),
+                   (lo + rc, lo + rc, math.pi)]
+        edges = [((cx0, lo), (hi - rc, lo)), ((hi, lo + rc), (hi, hi - rc)), ((hi - rc, hi), (lo + rc, hi)),
+                 ((lo, hi - rc), (lo, lo + rc))]
+        for (a, b), (kx, ky, a0) in zip(edges, corners):
+            L = math.hypot(b[0] - a[0], b[1] - a[1])
+            m = max(1, int(math.ceil(L / chord)))
+            for i in range(1, m + 1):
+                path.append((a[0] + (b[0] - a[0]) * i / m, a[1] + (b[1] - a[1]) * i / m))
+            m = max(1, int(math.ceil(rc * math.pi / 2 / chord)))
+            for i in range(1, m + 1):
+                t = a0 + math.pi / 2 * i / m
+                path.append((kx + rc * math.cos(t), ky + rc * math.sin(t)))
+        L = math.hypot(cx0 - lo - rc, 0.0)
+        m = max(1, int(math.ceil(L / chord)))
+        for i in range(1, m + 1):
+            path.append((lo + rc + (cx0 - lo - rc) * i / m, lo))
+        # rapid up, over to the loop's start, down, then ramp along the first chords
+        pts.append((pts[-1][0], pts[-1][1], 2.0, 1)); F.append(None)
+        pts.append((path[0][0], path[0][1], 2.0, 1)); F.append(None)
+        pts.append((path[0][0], path[0][1], z + 0.2, 1)); F.append(None)
+        for i, (px, py) in enumerate(path[1:]):
+            if len(pts) >= n_segments + 1:
+                break
+            pz = z + 0.2 * max(0.0, 1.0 - (i + 1) / 20.0)
+            pts.append((px, py, pz, 0)); F.append(feed)
+        o += 0.3
+    xs = [p[0] for p in pts]
+    ys = [p[1] for p in pts]
+    zs = [p[2] for p in pts]
+    rapid = [p[3] for p in pts]
+    line = list(range(1, len(pts) + 1))
+    S = [speed] * len(pts)
+    return xs, ys, zs, rapid, line, F, S
+
+
 # the coupled programs' tools, modes and material: a 12 mm flat end mill and an 8 mm ball
 # end mill taking turns, two modes per axis, A5052 with the size effect and process damping;
 # program p runs n_lines - 1 + p % 4 passes, so the programs differ in length
