@@ -45,7 +45,8 @@ struct ProgDev {
   const float* lut;   // bull-nose: the tip table (null otherwise)
   mc_frame_out* frames;
   double R, fluteLen, rw, share, rc, dynBudget, fz0;
-  int32_t nsub, ball, N, lutn, nframes, pad;
+  int32_t nsub, ball, N, lutn, nframes;
+  int32_t hOnly;      // flat end: the actual stamps write the heights alone (the rest after the launch)
   // the thin-wall model (flex): the half-cell walk table, its entries up to 20 and 60 mm, the
   // step, the material and the tool's static stiffness, and the per-sub-step records
   const double* sacc;
@@ -338,11 +339,98 @@ __device__ __forceinline__ double stamp_blocks(const StockDev& S, const Box& bx,
   return vol;
 }
 
+// A flat end's stamp on the heights alone, for the chain: each sub-step reads only the
+// heights the previous one left, so the deflected stamps' errors, wall distances, lines, feed
+// marks, volumes and dirty box are written after the launch by one batch with Stock.stamp's
+// sequential semantics on the heights saved before it.  A node inside the cutting radius
+// with H > tipZ takes nh = f32(max(tipZ, zB)) (src/stock.js:141-147); stock heights are never
+// below f32(zB) and no f32 value lies strictly between tipZ and its rounding, so a node with H
+// at or below max(tipZ, nh) keeps its height, and a strip whose maximum is at or below that
+// is skipped exactly (the floor the previous sub-step left at this depth).  A strip with a
+// lowered node gets its maximum again from its nodes.
+__device__ __forceinline__ void stamp_heights_flat(const StockDev& S, const Box& bx, double cx, double cy,
+                                                   double tipZ, double R, int wave, int nw) {
+  const int lane = threadIdx.x & 63;
+  const double tie = R * R - 0.003 * R;
+  const float nh = (float)jmax(tipZ, S.zB);
+  const double zskip = jmax(tipZ, (double)nh);
+  const int bi0 = bx.i0 / SW, bi1 = bx.i1 / SW, bj0 = bx.j0 / SH, bj1 = bx.j1 / SH;
+  const int bw = bi1 - bi0 + 1, nb = bw * (bj1 - bj0 + 1);
+  for (int b0 = wave * 64; b0 < nb; b0 += nw * 64) {
+    bool live = false;
+    {
+      const int q = b0 + lane;
+      if (q < nb) {
+        const int bi = bi0 + q % bw, bj = bj0 + q / bw;
+        const int ia = bi * SW > bx.i0 ? bi * SW : bx.i0, ib = bi * SW + SW - 1 < bx.i1 ? bi * SW + SW - 1 : bx.i1;
+        const int ja = bj * SH > bx.j0 ? bj * SH : bx.j0, jb = bj * SH + SH - 1 < bx.j1 ? bj * SH + SH - 1 : bx.j1;
+        const double ex = jmax(0, jmax(ia * S.hx - cx, cx - ib * S.hx));
+        const double ey = jmax(0, jmax(ja * S.hy - cy, cy - jb * S.hy));
+        const double rmin2 = (ex * ex + ey * ey) * (1 - 1e-12);
+        live = rmin2 < tie && (double)S.smax[bj * S.snx + bi] > zskip;
+      }
+    }
+    unsigned long long m = __ballot(live);
+    while (m) {
+      constexpr int U = 8;
+      int qb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        qb[u] = m ? b0 + __builtin_ctzll(m) : -1;
+        m &= m ? m - 1 : 0ull;
+      }
+      int64_t k[U];
+      int ni[U], nj[U];
+      bool inB[U];
+      float h[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int q = qb[u] >= 0 ? qb[u] : qb[0];
+        const int bi = bi0 + q % bw, bj = bj0 + q / bw;
+        ni[u] = bi * SW + (lane & (SW - 1)); nj[u] = bj * SH + lane / SW;
+        const bool inS = qb[u] >= 0 && ni[u] < S.nx && nj[u] < S.ny;
+        inB[u] = inS && ni[u] >= bx.i0 && ni[u] <= bx.i1 && nj[u] >= bx.j0 && nj[u] <= bx.j1;
+        k[u] = (int64_t)(nj[u] < S.ny ? nj[u] : S.ny - 1) * S.nx + (ni[u] < S.nx ? ni[u] : S.nx - 1);
+        const float hv = S.H[k[u]];
+        h[u] = inS ? hv : -__builtin_inff();
+      }
+      bool cutb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        cutb[u] = false;
+        if (qb[u] < 0) continue;   // wave-uniform
+        bool cut = false;
+        if (inB[u]) {
+          const double dy = nj[u] * S.hy - cy, dx = ni[u] * S.hx - cx;
+          if (dx * dx + dy * dy < tie && (double)h[u] > tipZ) {
+            S.H[k[u]] = nh;
+            cut = true;
+            h[u] = nh;
+          }
+        }
+        cutb[u] = __any(cut);
+      }
+      float hm[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) hm[u] = h[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int u = 0; u < U; u++) hm[u] = fmaxf(hm[u], __shfl_xor(hm[u], o, 64));
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (cutb[u] && lane == 0) S.smax[(nj[u] / SH) * S.snx + ni[u] / SW] = hm[u];
+    }
+  }
+}
+
 // The tip-table form runs in a code path of its own: mixed into the flat / ball loop it
 // raised the kernel's register pressure into scratch spills.  A ball's stamps skip strips by
 // their maxima (the floor a ball leaves lies above most of the next stamp's sphere); a flat
-// end rewrites every floor node under it (the feed marks, and the cut wherever the f32
-// floor height rounded above tipZ), so its stamps take every node of the box.
+// end with all its maps rewrites every floor node under it (the feed marks, and the cut
+// wherever the f32 floor height rounded above tipZ), so this form takes every node of the
+// box (the pipeline's flat programs take stamp_heights_flat instead).
 __device__ __forceinline__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
                                              double R, bool ball, const float* lut, int lutn, double errVal,
                                              double mux, double muy, double mfeed, int32_t line, int t, int nt,
@@ -749,10 +837,15 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     const double maxF = cut ? E.runMaxF : 0.0;
     const double ax = s.x + dx, ay = s.y + dy, az = s.z + (ball ? P.rw : 0), aR = R - P.rw;
     const Box ba = stamp_box(P.act, ax, ay, az, aR);
-    const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, P.lut, P.lutn, errVal, s.ux, s.uy,
-                                   fzLast * P.N, s.line, tid, NT, &flags[1]);
-    const double vol = block_sum(part, red) * P.act.hx * P.act.hy;
-    if (tid == 0 && flags[1]) join_rect(P.act, ba);
+    double vol = 0;   // with hOnly, the batch after the launch gives the volume
+    if (P.hOnly) {   // block-uniform
+      if (ba.ok) stamp_heights_flat(P.act, ba, ax, ay, az, aR, tid >> 6, NT / 64);
+    } else {
+      const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, P.lut, P.lutn, errVal, s.ux, s.uy,
+                                     fzLast * P.N, s.line, tid, NT, &flags[1]);
+      vol = block_sum(part, red) * P.act.hx * P.act.hy;
+      if (tid == 0 && flags[1]) join_rect(P.act, ba);
+    }
     __threadfence_block();
     __syncthreads();
     // the wall-error mark while a wall is held (:443): recorded for the caller's map
@@ -860,7 +953,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (a.nx < 2 || a.ny < 2 || i.nx < 2 || i.ny < 2) return mch::fail(MC_ERR_ARG, "program %d: stocks need 2 x 2 nodes", p);
     float* sa = nullptr;
     int snxa = 0, snya = 0;
-    if (g.type == MC_TOOL_BALL)
+    if (g.type == MC_TOOL_BALL || g.type == MC_TOOL_FLAT)
       if (int st = mch::stock_strips(g.actual, &sa, &snxa, &snya)) return st;
     d.act = StockDev{a.H, a.err, a.wallD, a.markS, a.markF, a.lineCut, a.lineWall, a.rect, sa, a.nx, a.ny, snxa, snya,
                      a.hx, a.hy, a.zTop, a.zBottom, a.band};
@@ -875,7 +968,8 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     d.dynOut = nullptr;
     if (g.budgetLeft && !std::isfinite(d.dynLeft0))
       return mch::fail(MC_ERR_ARG, "program %d: non-finite frame budget left", p);
-    d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0; d.pad = 0;
+    d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0;
+    d.hOnly = g.type == MC_TOOL_FLAT ? 1 : 0;
     d.sacc = nullptr; d.walls = nullptr; d.flex = g.flex; d.n20 = 0; d.n60 = 0; d.pad2 = 0;
     d.wallE = g.wallE; d.wallRho = g.wallRho; d.kTool = g.kTool;
     // the wall model's walks step by half a cell (src/wall.js:37): at least 1e-4 mm, so that
@@ -905,7 +999,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
                                : mc_stamp_batch_ex(g.ideal, cmd.data(), opt.data(), g.nsub, g.lut, &lo, &ln, 1, nullptr);
     if (st) return st;
   }
-  std::lock_guard<std::mutex> lk(mch::lock());
+  std::unique_lock<std::mutex> lk(mch::lock());
   hipStream_t q = mch::stream();
   // a ball's stamps skip node strips by their maxima and keep them exact: take them from
   // the heights first
@@ -917,7 +1011,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (hipGetLastError() != hipSuccess) return mch::fail(MC_ERR_HIP, "strip maxima launch failed");
   }
   // sub-steps in, results out: one device buffer for every program
-  size_t nin = 0, nlut = 0, nfr = 0, nwall = 0, nedge = 0;
+  size_t nin = 0, nlut = 0, nfr = 0, nwall = 0, nedge = 0, nsave = 0;
   bool flex = false;
   for (int p = 0; p < n; p++) {
     nin += (size_t)progs[p].nsub;
@@ -926,6 +1020,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     nfr += (size_t)pd[(size_t)p].nframes;
     if (progs[p].flex && progs[p].walls) nwall += (size_t)progs[p].nsub;
     flex = flex || progs[p].flex;
+    if (pd[(size_t)p].hOnly && progs[p].nsub > 0) nsave += (size_t)pd[(size_t)p].act.nx * (size_t)pd[(size_t)p].act.ny;
   }
   // the wall model's walk table, s = step, s + step, ... up to 60 mm, summed in the
   // reference's order; one per distinct step
@@ -961,6 +1056,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   mc_wall_out* dwall = nullptr;
   double* dedge = nullptr;
   double* ddyn = nullptr;
+  float* dsave = nullptr;   // flat programs: the actual stock's heights before the launch
   auto release = [&]() {
     if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
@@ -971,6 +1067,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (dwall) (void)hipFree(dwall);
     if (dedge) (void)hipFree(dedge);
     if (ddyn) (void)hipFree(ddyn);
+    if (dsave) (void)hipFree(dsave);
   };
   if (hipMalloc(&dsub, sizeof(mc_substep) * std::max<size_t>(1, nin)) != hipSuccess ||
       hipMalloc(&dout, sizeof(mc_substep_out) * std::max<size_t>(1, nin)) != hipSuccess ||
@@ -980,7 +1077,8 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
       hipMalloc(&dsacc, sizeof(double) * std::max<size_t>(1, sacc.size())) != hipSuccess ||
       hipMalloc(&dwall, sizeof(mc_wall_out) * std::max<size_t>(1, nwall)) != hipSuccess ||
       hipMalloc(&dedge, sizeof(double) * std::max<size_t>(1, nedge)) != hipSuccess ||
-      hipMalloc(&ddyn, sizeof(double) * (size_t)n) != hipSuccess) {
+      hipMalloc(&ddyn, sizeof(double) * (size_t)n) != hipSuccess ||
+      hipMalloc(&dsave, sizeof(float) * std::max<size_t>(1, nsave)) != hipSuccess) {
     release();
     return mch::fail(MC_ERR_NOMEM, "sub-step buffers");
   }
@@ -1026,6 +1124,15 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   }
   if (!st && hipMemcpyAsync(dprog, pd.data(), sizeof(ProgDev) * (size_t)n, hipMemcpyHostToDevice, q) != hipSuccess)
     st = mch::fail(MC_ERR_HIP, "program upload");
+  std::vector<size_t> saveAt((size_t)n, 0);
+  for (size_t p = 0, at = 0; p < (size_t)n && !st; p++) {
+    const StockDev& A = pd[p].act;
+    if (!pd[p].hOnly || progs[p].nsub == 0) continue;
+    saveAt[p] = at;
+    if (hipMemcpyAsync(dsave + at, A.H, sizeof(float) * (size_t)A.nx * A.ny, hipMemcpyDeviceToDevice, q) != hipSuccess)
+      st = mch::fail(MC_ERR_HIP, "height save");
+    at += (size_t)A.nx * A.ny;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     st = mch::fail(MC_ERR_HIP, "timing events");
@@ -1044,10 +1151,12 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   off = 0;
   foff = 0;
   woff = 0;
+  std::vector<mc_substep_out> hout(std::max<size_t>(1, nin));
   for (int p = 0; p < n && launched; p++) {
     const mc_sim_program& g = progs[p];
-    if (!st && g.out && g.nsub > 0 &&
-        hipMemcpyAsync(g.out, dout + off, sizeof(mc_substep_out) * (size_t)g.nsub, hipMemcpyDeviceToHost, q) != hipSuccess)
+    if (!st && (g.out || pd[(size_t)p].hOnly) && g.nsub > 0 &&
+        hipMemcpyAsync(hout.data() + off, dout + off, sizeof(mc_substep_out) * (size_t)g.nsub, hipMemcpyDeviceToHost,
+                       q) != hipSuccess)
       st = mch::fail(MC_ERR_HIP, "result download");
     off += (size_t)g.nsub;
     const int nf = pd[(size_t)p].nframes;
@@ -1068,12 +1177,51 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (!st) st = fst;
   }
   if (!st && hipStreamSynchronize(q) != hipSuccess) st = mch::fail(MC_ERR_HIP, "sub-step pipeline failed");
-  if (!st) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) mch::set_last_ms(ms);
-  }
+  float ms = 0;
+  if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 0;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  release();
+  // Flat programs: the chain wrote the actual stock's heights alone.  Its deflected stamps
+  // now run as one batch with Stock.stamp's sequential semantics (src/stock.js:118-164) from
+  // the heights saved before the launch, writing the heights again (the same values) with the
+  // errors, wall distances, lines, feed marks and dirty box, and giving each stamp's volume.
+  // The stamp is the chain's: at p plus the deflection, R less the radial wear, the sub-step's
+  // error value and line, the feed mark of the last cutting sub-step.
+  lk.unlock();
+  off = 0;
+  for (int p = 0; p < n && !st; p++) {
+    const mc_sim_program& g = progs[p];
+    if (pd[(size_t)p].hOnly && g.nsub > 0) {
+      const StockDev& A = pd[(size_t)p].act;
+      std::vector<mc_stamp> cmd((size_t)g.nsub);
+      std::vector<mc_stamp_opt> opt((size_t)g.nsub);
+      std::vector<double> vol((size_t)g.nsub);
+      double fzl = g.fz;
+      for (int k = 0; k < g.nsub; k++) {
+        const mc_substep& s = g.sub[k];
+        const mc_substep_out& o = hout[off + (size_t)k];
+        if (o.cut) fzl = s.fzEff;
+        cmd[(size_t)k] = mc_stamp{s.x + o.dx, s.y + o.dy, s.z, g.R - g.rw, MC_TOOL_FLAT, s.line, o.errVal};
+        opt[(size_t)k] = mc_stamp_opt{s.ux, s.uy, fzl * pd[(size_t)p].N, -1, 0};
+      }
+      {
+        std::lock_guard<std::mutex> lk2(mch::lock());
+        if (hipMemcpyAsync(A.H, dsave + saveAt[(size_t)p], sizeof(float) * (size_t)A.nx * A.ny, hipMemcpyDeviceToDevice,
+                           q) != hipSuccess ||
+            hipStreamSynchronize(q) != hipSuccess)
+          st = mch::fail(MC_ERR_HIP, "height restore");
+      }
+      if (!st) st = mc_stamp_batch_ex(g.actual, cmd.data(), opt.data(), g.nsub, nullptr, nullptr, nullptr, 0, vol.data());
+      for (int k = 0; k < g.nsub && !st; k++) hout[off + (size_t)k].vol = vol[(size_t)k];
+    }
+    if (!st && g.out && g.nsub > 0)
+      std::memcpy(g.out, hout.data() + off, sizeof(mc_substep_out) * (size_t)g.nsub);
+    off += (size_t)g.nsub;
+  }
+  if (!st) mch::set_last_ms(ms);
+  {
+    std::lock_guard<std::mutex> lk2(mch::lock());
+    release();
+  }
   return st;
 }
