@@ -46,7 +46,7 @@ struct ProgDev {
   mc_frame_out* frames;
   double R, fluteLen, rw, share, rc, dynBudget, fz0;
   int32_t nsub, ball, N, lutn, nframes;
-  int32_t hOnly;      // flat end: the actual stamps write the heights alone (the rest after the launch)
+  int32_t hOnly;      // flat or ball end: the actual stamps write the heights alone (the rest after the launch)
   // the thin-wall model (flex): the half-cell walk table, its entries up to 20 and 60 mm, the
   // step, the material and the tool's static stiffness, and the per-sub-step records
   const double* sacc;
@@ -339,19 +339,22 @@ __device__ __forceinline__ double stamp_blocks(const StockDev& S, const Box& bx,
   return vol;
 }
 
-// A flat end's stamp on the heights alone, for the chain: each sub-step reads only the
-// heights the previous one left, so the deflected stamps' errors, wall distances, lines, feed
-// marks, volumes and dirty box are written after the launch by one batch with Stock.stamp's
-// sequential semantics on the heights saved before it.  A node inside the cutting radius
-// with H > tipZ takes nh = f32(max(tipZ, zB)) (src/stock.js:141-147); stock heights are never
-// below f32(zB) and no f32 value lies strictly between tipZ and its rounding, so a node with H
-// at or below max(tipZ, nh) keeps its height, and a strip whose maximum is at or below that
-// is skipped exactly (the floor the previous sub-step left at this depth).  A strip with a
-// lowered node gets its maximum again from its nodes.
-__device__ __forceinline__ void stamp_heights_flat(const StockDev& S, const Box& bx, double cx, double cy,
-                                                   double tipZ, double R, int wave, int nw) {
+// A flat or ball end's stamp on the heights alone, for the chain: each sub-step reads only
+// the heights the previous one left, so the deflected stamps' errors, wall distances, lines,
+// feed marks, volumes and dirty box are written after the launch by one batch with
+// Stock.stamp's sequential semantics on the heights saved before it.  A node inside the
+// cutting radius with H > z (tipZ, or the sphere's height over the node) takes
+// f32(max(z, zB)) (src/stock.js:141-147).  Flat: stock heights are never below f32(zB) and no
+// f32 value lies strictly between tipZ and its rounding nh, so a node with H at or below
+// max(tipZ, nh) keeps its height, and a strip whose maximum is at or below that is skipped
+// exactly (the floor the previous sub-step left at this depth).  Ball: a strip whose maximum
+// lies at or below the sphere's height at its point nearest the axis (less a margin above the
+// rounding) has no node above the sphere.  A strip with a lowered node gets its maximum again
+// from its nodes.
+__device__ __forceinline__ void stamp_heights(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
+                                              double R, bool ball, int wave, int nw) {
   const int lane = threadIdx.x & 63;
-  const double tie = R * R - 0.003 * R;
+  const double R2 = R * R, tie = R2 - 0.003 * R;
   const float nh = (float)jmax(tipZ, S.zB);
   const double zskip = jmax(tipZ, (double)nh);
   const int bi0 = bx.i0 / SW, bi1 = bx.i1 / SW, bj0 = bx.j0 / SH, bj1 = bx.j1 / SH;
@@ -367,7 +370,8 @@ __device__ __forceinline__ void stamp_heights_flat(const StockDev& S, const Box&
         const double ex = jmax(0, jmax(ia * S.hx - cx, cx - ib * S.hx));
         const double ey = jmax(0, jmax(ja * S.hy - cy, cy - jb * S.hy));
         const double rmin2 = (ex * ex + ey * ey) * (1 - 1e-12);
-        live = rmin2 < tie && (double)S.smax[bj * S.snx + bi] > zskip;
+        const double zlow = ball ? tipZ + R - __builtin_sqrt(R2 - rmin2) - 1e-9 : zskip;
+        live = rmin2 < tie && (double)S.smax[bj * S.snx + bi] > zlow;
       }
     }
     unsigned long long m = __ballot(live);
@@ -402,10 +406,15 @@ __device__ __forceinline__ void stamp_heights_flat(const StockDev& S, const Box&
         bool cut = false;
         if (inB[u]) {
           const double dy = nj[u] * S.hy - cy, dx = ni[u] * S.hx - cx;
-          if (dx * dx + dy * dy < tie && (double)h[u] > tipZ) {
-            S.H[k[u]] = nh;
-            cut = true;
-            h[u] = nh;
+          const double r2 = dx * dx + dy * dy;
+          if (r2 < tie) {
+            const double z = ball ? tipZ + R - __builtin_sqrt(R2 - r2) : tipZ;
+            if ((double)h[u] > z) {
+              const float v = ball ? (float)jmax(z, S.zB) : nh;
+              S.H[k[u]] = v;
+              cut = true;
+              h[u] = v;
+            }
           }
         }
         cutb[u] = __any(cut);
@@ -430,7 +439,7 @@ __device__ __forceinline__ void stamp_heights_flat(const StockDev& S, const Box&
 // their maxima (the floor a ball leaves lies above most of the next stamp's sphere); a flat
 // end with all its maps rewrites every floor node under it (the feed marks, and the cut
 // wherever the f32 floor height rounded above tipZ), so this form takes every node of the
-// box (the pipeline's flat programs take stamp_heights_flat instead).
+// box (the pipeline's flat and ball programs take stamp_heights instead).
 __device__ __forceinline__ double stamp_part(const StockDev& S, const Box& bx, double cx, double cy, double tipZ,
                                              double R, bool ball, const float* lut, int lutn, double errVal,
                                              double mux, double muy, double mfeed, int32_t line, int t, int nt,
@@ -839,7 +848,7 @@ __global__ __launch_bounds__(NT) void sim_run(const ProgDev* progs) {
     const Box ba = stamp_box(P.act, ax, ay, az, aR);
     double vol = 0;   // with hOnly, the batch after the launch gives the volume
     if (P.hOnly) {   // block-uniform
-      if (ba.ok) stamp_heights_flat(P.act, ba, ax, ay, az, aR, tid >> 6, NT / 64);
+      if (ba.ok) stamp_heights(P.act, ba, ax, ay, az, aR, ball, tid >> 6, NT / 64);
     } else {
       const double part = stamp_part(P.act, ba, ax, ay, az, aR, ball, P.lut, P.lutn, errVal, s.ux, s.uy,
                                      fzLast * P.N, s.line, tid, NT, &flags[1]);
@@ -969,7 +978,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
     if (g.budgetLeft && !std::isfinite(d.dynLeft0))
       return mch::fail(MC_ERR_ARG, "program %d: non-finite frame budget left", p);
     d.frames = nullptr; d.nframes = g.frames ? g.nframes : 0;
-    d.hOnly = g.type == MC_TOOL_FLAT ? 1 : 0;
+    d.hOnly = g.type == MC_TOOL_FLAT || g.type == MC_TOOL_BALL ? 1 : 0;
     d.sacc = nullptr; d.walls = nullptr; d.flex = g.flex; d.n20 = 0; d.n60 = 0; d.pad2 = 0;
     d.wallE = g.wallE; d.wallRho = g.wallRho; d.kTool = g.kTool;
     // the wall model's walks step by half a cell (src/wall.js:37): at least 1e-4 mm, so that
@@ -1056,7 +1065,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   mc_wall_out* dwall = nullptr;
   double* dedge = nullptr;
   double* ddyn = nullptr;
-  float* dsave = nullptr;   // flat programs: the actual stock's heights before the launch
+  float* dsave = nullptr;   // flat and ball programs: the actual stock's heights before the launch
   auto release = [&]() {
     if (dlut) (void)hipFree(dlut);
     if (dsub) (void)hipFree(dsub);
@@ -1181,7 +1190,7 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
   if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 0;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  // Flat programs: the chain wrote the actual stock's heights alone.  Its deflected stamps
+  // Flat and ball programs: the chain wrote the actual stock's heights alone.  Its deflected stamps
   // now run as one batch with Stock.stamp's sequential semantics (src/stock.js:118-164) from
   // the heights saved before the launch, writing the heights again (the same values) with the
   // errors, wall distances, lines, feed marks and dirty box, and giving each stamp's volume.
@@ -1201,7 +1210,9 @@ extern "C" int mc_sim_run(const mc_sim_program* progs, int n) {
         const mc_substep& s = g.sub[k];
         const mc_substep_out& o = hout[off + (size_t)k];
         if (o.cut) fzl = s.fzEff;
-        cmd[(size_t)k] = mc_stamp{s.x + o.dx, s.y + o.dy, s.z, g.R - g.rw, MC_TOOL_FLAT, s.line, o.errVal};
+        const bool bl = g.type == MC_TOOL_BALL;
+        cmd[(size_t)k] = mc_stamp{s.x + o.dx, s.y + o.dy, s.z + (bl ? g.rw : 0.0), g.R - g.rw,
+                                  bl ? MC_TOOL_BALL : MC_TOOL_FLAT, s.line, o.errVal};
         opt[(size_t)k] = mc_stamp_opt{s.ux, s.uy, fzl * pd[(size_t)p].N, -1, 0};
       }
       {
